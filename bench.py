@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the enblend/enfuse pyramid path on one MI355X.
+
+Workload (one step): one enblend blend step of two 4096x3072 RGBA uint16
+images sharing one synthetic scene, 25% horizontal overlap, on their
+7168x3072 union canvas with 10 pyramid levels: mask Gaussian pyramid,
+two Laplacian pyramids under the images' alphas, blend, collapse and the
+dithered copy back into the 16-bit image (src/enblend.h:378-510).  Inputs are
+resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is input Mpix (2 x 4096 x 3072 per step)
+per second of wall time over the timed steps.  `roofline` is the dominant
+kernel's algorithmic bytes per launch over its HIP-event launch duration,
+measured on the library's stream in a profiled pass of the same workload.
+`cpu_baseline` times the oracle (the C++ restatement of the reference's
+algorithm, OpenMP pointwise stages, single-threaded SKIPSM passes) on the
+host cores for the same workload.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mpix/s fused per GPU; % HBM peak; batched vs plain Mpix/s; adjoint rel err"
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(white, wa, black, ba, mask, levels, repeats):
+    """Time the oracle's enblend step on the host (checker code, bench only)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_binding import U16, load_oracle
+    o = load_oracle()
+    times = []
+    out = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = o.enblend_pair(white, wa, black, ba, mask, levels, False, U16, 1, 1)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    return times[len(times) // 2], o.lib.orc_num_threads(), out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
+    scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
+    print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
+    white, wa, black, ba, mask = scenes.enblend_pair(seed=1)
+    h, w, C = white.shape
+    levels = enbl.auto_levels(w, h)
+    mpix = 2 * 4096 * 3072 / 1e6
+
+    dev = torch.device("cuda:0")
+    d_white = torch.from_numpy(white).to(dev)
+    d_wa = torch.from_numpy(wa).to(dev)
+    d_black0 = torch.from_numpy(black).to(dev)
+    d_ba0 = torch.from_numpy(ba).to(dev)
+    d_mask = torch.from_numpy(mask).to(dev)
+    d_black = d_black0.clone()
+    d_ba = d_ba0.clone()
+    torch.cuda.synchronize()
+
+    ctx = enbl.Context(0)
+    L = enbl.lib()
+
+    def vp(t):
+        import ctypes
+        return ctypes.c_void_p(t.data_ptr())
+
+    def step(bk, bka):
+        enbl.check(L.enbl_enblend_pair(ctx.h, enbl.U16, C, w, h, levels, 0,
+                                       vp(d_white), w * C, vp(d_wa), w, vp(bk), w * C, vp(bka), w,
+                                       vp(d_mask), w, enbl.DITHER_DET, 1))
+
+    # Parity of this exact workload against the oracle (bit-exact).
+    print("bench: GPU step for parity", file=sys.stderr, flush=True)
+    step(d_black, d_ba)
+    ctx.sync()
+    gpu_out = d_black.cpu().numpy()
+    gpu_alpha = d_ba.cpu().numpy()
+
+    for _ in range(args.warmup):
+        step(d_black, d_ba)
+    ctx.sync()
+    torch.cuda.synchronize()
+    print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(d_black, d_ba)
+    ctx.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+
+    # Profiled pass: HIP events around every launch on the library's stream.
+    ctx.profile(True)
+    nprof = 3
+    for _ in range(nprof):
+        step(d_black, d_ba)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    total_ms = sum(v["ms"] for v in rep.values())
+    dom_name, dom = max(rep.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = dom["ms"] / dom["launches"]
+    avg_bytes = dom["bytes"] / dom["launches"]
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    all_gbs = sum(v["bytes"] for v in rep.values()) / (total_ms * 1e-3) / 1e9
+    kernels = {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
+                   "GB_per_s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in sorted(rep.items())}
+
+    result = {
+        "metric": METRIC,
+        "value": round(mpix / (ms * 1e-3), 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "enblend step, 2 x 4096x3072 RGBA uint16, 25% overlap, 7168x3072 ROI, "
+                               f"{levels} levels, Int32 fixed-point pyramids",
+                   "input_mpix_per_step": mpix, "levels": levels},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "all_kernels_GB_per_s": round(all_gbs, 1), "kernel_ms_per_step": round(total_ms / nprof, 4)},
+        "kernels": kernels,
+    }
+
+    if not args.no_cpu:
+        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu_s, threads, (cpu_out, cpu_alpha) = cpu_baseline(white, wa, black, ba, mask, levels, args.cpu_repeats)
+        result["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                                  "sample": f"full workload, median of {args.cpu_repeats} runs "
+                                            f"({cpu_s:.2f} s each)"}
+        result["parity"] = bool(np.array_equal(cpu_out, gpu_out) and np.array_equal(cpu_alpha, gpu_alpha))
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

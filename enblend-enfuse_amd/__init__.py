@@ -1,0 +1,193 @@
+"""Python harness over the C ABI of the enblend/enfuse pyramid path on MI355X.
+
+The product is the HIP shared library libenblend_amd.so (C ABI declared in
+include/enblend_amd.h).  This module binds it with ctypes for the test and
+benchmark harness; device buffers come from torch tensors (plumbing only).
+There is no CPU fallback: loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libenblend_amd.so")
+HEADER = os.path.join(HERE, "..", "include", "enblend_amd.h")
+
+ENBL_OK = 0
+ENBL_E_PRECONDITION = -1
+ENBL_E_SIZE = -2
+ENBL_E_ARG = -3
+ENBL_E_UNSUPPORTED = -4
+ENBL_E_OOM = -5
+ENBL_E_HIP = -6
+
+I16, I32, F64 = 0, 1, 2
+U8, U16, F32 = 10, 11, 12
+DITHER_NONE, DITHER_DET = 0, 1
+
+_c_int, _c_long, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
+
+_SIGS = {
+    "enbl_context_create": (_c_int, [_c_int, ctypes.POINTER(_vp)]),
+    "enbl_context_destroy": (_c_int, [_vp]),
+    "enbl_context_set_stream": (_c_int, [_vp, _vp]),
+    "enbl_context_get_stream": (_c_int, [_vp, ctypes.POINTER(_vp)]),
+    "enbl_synchronize": (_c_int, [_vp]),
+    "enbl_last_error": (ctypes.c_char_p, []),
+    "enbl_profile_enable": (_c_int, [_vp, _c_int]),
+    "enbl_profile_report": (_c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "enbl_reduce": (_c_int, [_vp, _c_int, _c_int, _c_int,
+                             _vp, _c_int, _c_int, _c_long, _c_long, _vp, _c_long,
+                             _vp, _c_int, _c_int, _c_long, _c_long, _vp, _c_long]),
+    "enbl_expand": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int,
+                             _vp, _c_int, _c_int, _c_long, _c_long,
+                             _vp, _c_int, _c_int, _c_long, _c_long]),
+    "enbl_pyramid_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "enbl_pyramid_destroy": (_c_int, [_vp]),
+    "enbl_pyramid_levels": (_c_int, [_vp]),
+    "enbl_pyramid_level": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int),
+                                    ctypes.POINTER(_c_int), ctypes.POINTER(_c_long), ctypes.POINTER(_c_long)]),
+    "enbl_pyramid_copy_level": (_c_int, [_vp, _c_int, _vp, _c_int]),
+    "enbl_auto_levels": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
+    "enbl_gaussian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
+    "enbl_laplacian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
+    "enbl_collapse_pyramid": (_c_int, [_vp, _c_int, _vp]),
+    "enbl_blend": (_c_int, [_vp, _vp, _vp, _vp, ctypes.c_double]),
+    "enbl_copy_from_pyramid_if": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _vp, _c_int, _c_long,
+                                           _c_int, ctypes.c_uint64]),
+    "enbl_enblend_pair": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                   _vp, _c_long, _vp, _c_long, _vp, _c_long, _vp, _c_long,
+                                   _vp, _c_long, _c_int, ctypes.c_uint64]),
+}
+
+_lib = None
+
+
+class EnblError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"enblend_amd error {code}: {msg}")
+        self.code = code
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Load libenblend_amd.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with enblend-enfuse_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != ENBL_OK:
+        raise EnblError(rc, lib().enbl_last_error().decode())
+    return rc
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor or None."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """Owns an enbl_context on one HIP device."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().enbl_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().enbl_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().enbl_synchronize(self.h))
+
+    def stream(self):
+        s = _vp()
+        check(lib().enbl_context_get_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def profile(self, on=True):
+        check(lib().enbl_profile_enable(self.h, 1 if on else 0))
+
+    def profile_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().enbl_profile_report(self.h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, n, nbytes = line.split()
+            out[name] = {"ms": float(ms), "launches": int(n), "bytes": float(nbytes)}
+        return out
+
+
+class Pyramid:
+    """Device pyramid (enbl_pyramid); levels are read and written as numpy arrays."""
+
+    def __init__(self, ctx, ptype, channels, w, h, levels):
+        hnd = _vp()
+        check(lib().enbl_pyramid_create(ctx.h, ptype, channels, w, h, levels, ctypes.byref(hnd)))
+        self.h = hnd
+        self.ctx = ctx
+        self.ptype, self.channels, self.levels = ptype, channels, levels
+
+    def close(self):
+        if self.h:
+            lib().enbl_pyramid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def level_info(self, l):
+        d, w, h, p, pl = _vp(), _c_int(), _c_int(), _c_long(), _c_long()
+        check(lib().enbl_pyramid_level(self.h, l, ctypes.byref(d), ctypes.byref(w), ctypes.byref(h),
+                                       ctypes.byref(p), ctypes.byref(pl)))
+        return d.value, w.value, h.value, p.value, pl.value
+
+    def shape(self, l):
+        _, w, h, _, _ = self.level_info(l)
+        return (self.channels, h, w)
+
+    def read(self, l):
+        """Level l as a host numpy array of shape (channels, h, w)."""
+        import numpy as np
+        out = np.empty(self.shape(l), dtype=_NP[self.ptype])
+        check(lib().enbl_pyramid_copy_level(self.h, l, out.ctypes.data_as(_vp), 0))
+        return out
+
+    def write(self, l, arr):
+        """Upload a host array of shape (channels, h, w) into level l."""
+        import numpy as np
+        a = np.ascontiguousarray(arr, dtype=_NP[self.ptype])
+        assert a.shape == self.shape(l), (a.shape, self.shape(l))
+        check(lib().enbl_pyramid_copy_level(self.h, l, a.ctypes.data_as(_vp), 1))
+
+
+_NP = {I16: "int16", I32: "int32", F64: "float64"}
+
+
+def auto_levels(w, h, minimum_levels=1, exact_levels=0):
+    return lib().enbl_auto_levels(w, h, minimum_levels, exact_levels)

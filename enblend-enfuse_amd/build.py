@@ -1,0 +1,50 @@
+"""Build the gfx950 shared library of the enblend/enfuse pyramid path.
+
+Compiles every HIP source under csrc/ with hipcc into
+enblend-enfuse_amd/libenblend_amd.so (in-tree, so it travels with the source
+tree).  Flags:
+  * --offload-arch=gfx950 only (MI355X);
+  * -ffp-contract=off so that double arithmetic keeps the reference's
+    operation order (no FMA contraction);
+  * -Xclang -target-feature -Xclang -packed-fp32-ops keeps the compiler from
+    forming packed FP32 instructions.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libenblend_amd.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def build(verbose=False):
+    """Compile the library; raises CalledProcessError on failure."""
+    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", LIB + ".tmp"] + sources()
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,534 @@
+// C ABI of the enblend/enfuse pyramid path (declared in include/enblend_amd.h).
+//
+// The reference's pyramid functions are header templates that allocate their
+// levels with new and throw vigra::PreconditionViolation; here the levels live
+// in one device arena per pyramid, every call is ordered on the context's
+// stream and errors come back as status codes with a thread-local message.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+
+#include "enbl_internal.h"
+
+using namespace enbl;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return ENBL_OK;
+    if (e == hipErrorOutOfMemory) return fail(ENBL_E_OOM, "%s: out of memory", what);
+    return fail(ENBL_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int launch_status(const char* what)
+{
+    return hip_check(hipGetLastError(), what);
+}
+
+// Brackets one launch with HIP events when the context profiles.
+struct Timed {
+    enbl_context* ctx;
+    size_t idx;
+    Timed(enbl_context* c, const char* name, double bytes) : ctx(c), idx((size_t)-1) {
+        if (!ctx->profile) return;
+        enbl_context::Timed t;
+        t.name = name;
+        t.bytes = bytes;
+        if (hipEventCreate(&t.start) != hipSuccess) return;
+        if (hipEventCreate(&t.stop) != hipSuccess) { (void)hipEventDestroy(t.start); return; }
+        (void)hipEventRecord(t.start, ctx->stream);
+        ctx->timed.push_back(t);
+        idx = ctx->timed.size() - 1;
+    }
+    ~Timed() {
+        if (idx != (size_t)-1) (void)hipEventRecord(ctx->timed[idx].stop, ctx->stream);
+    }
+};
+
+const char* pname(int t)
+{
+    switch (t) {
+    case ENBL_I16: return "i16";
+    case ENBL_I32: return "i32";
+    case ENBL_F64: return "f64";
+    }
+    return "?";
+}
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+bool aligned2(const void* p, int ptype) { return ((uintptr_t)p % (2 * (size_t)ptype_size(ptype))) == 0; }
+
+int mask_ptype_for(int itype) { return itype == ENBL_U8 ? ENBL_I16 : ENBL_I32; }
+int image_ptype_for(int itype) { return itype == ENBL_U8 ? ENBL_I16 : itype == ENBL_U16 ? ENBL_I32 : ENBL_F64; }
+
+int alpha_scratch(enbl_context* ctx, size_t bytes, uint8_t** a, uint8_t** b)
+{
+    if (ctx->scratch_alpha_bytes < 2 * bytes) {
+        if (ctx->scratch_alpha) {
+            int rc = hip_check(hipStreamSynchronize(ctx->stream), "alpha scratch");
+            if (rc) return rc;
+            (void)hipFree(ctx->scratch_alpha);
+        }
+        ctx->scratch_alpha = nullptr;
+        ctx->scratch_alpha_bytes = 0;
+        int rc = hip_check(hipMalloc(&ctx->scratch_alpha, 2 * bytes), "alpha scratch");
+        if (rc) return rc;
+        ctx->scratch_alpha_bytes = 2 * bytes;
+    }
+    *a = (uint8_t*)ctx->scratch_alpha;
+    *b = (uint8_t*)ctx->scratch_alpha + bytes;
+    return ENBL_OK;
+}
+
+void* level_ptr(const enbl_pyramid* p, int l) { return (char*)p->base + p->lv[l].offset; }
+
+int do_reduce(enbl_context* ctx, int ptype, int C, int wrap,
+              const void* src, int sw, int sh, long sp, long spl, const uint8_t* sa, long sap,
+              void* dst, int dw, int dh, long dp, long dpl, uint8_t* da, long dap)
+{
+    if (sw < 2 || sh < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
+    if (dw != (sw + 1) / 2 || dh != (sh + 1) / 2)
+        return fail(ENBL_E_SIZE, "reduce: destination %dx%d does not match source %dx%d", dw, dh, sw, sh);
+    ReduceArgs a;
+    a.src = src; a.spitch = sp; a.splane = spl; a.sw = sw; a.sh = sh;
+    a.salpha = sa; a.sapitch = sap;
+    a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
+    a.dalpha = da; a.dapitch = dap;
+    a.channels = C; a.wrap = wrap; a.rows = 16;
+    char name[64];
+    snprintf(name, sizeof name, "reduce_%s%s_c%d", sa ? "alpha_" : "", pname(ptype), C);
+    const double es = (double)ptype_size(ptype) * C + (sa ? 1.0 : 0.0);
+    Timed t(ctx, name, es * ((double)sw * sh + (double)dw * dh));
+    int rc = launch_reduce(ptype, a, ctx->stream);
+    if (rc) return fail(rc, "reduce: unsupported type/channels");
+    return launch_status(name);
+}
+
+int do_expand(enbl_context* ctx, int ptype, int C, int add, int wrap,
+              const void* src, int sw, int sh, long sp, long spl,
+              void* dst, int dw, int dh, long dp, long dpl)
+{
+    if (sw < 1 || sh < 1) return fail(ENBL_E_SIZE, "expand: empty source");
+    if ((dw != 2 * sw && dw != 2 * sw - 1) || (dh != 2 * sh && dh != 2 * sh - 1) || (dw < 2 && dh < 2))
+        return fail(ENBL_E_SIZE, "expand: destination %dx%d does not match source %dx%d", dw, dh, sw, sh);
+    ExpandArgs a;
+    a.src = src; a.spitch = sp; a.splane = spl; a.sw = sw; a.sh = sh;
+    a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
+    a.channels = C; a.wrap = wrap; a.add = add; a.rows = 16;
+    char name[64];
+    snprintf(name, sizeof name, "expand_%s_%s_c%d", add ? "add" : "sub", pname(ptype), C);
+    const double es = (double)ptype_size(ptype) * C;
+    Timed t(ctx, name, es * ((double)sw * sh + 2.0 * dw * dh));
+    int rc = launch_expand(ptype, a, ctx->stream);
+    if (rc) return fail(rc, "expand: unsupported type/channels");
+    return launch_status(name);
+}
+
+int gaussian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch)
+{
+    uint8_t* abuf[2] = {nullptr, nullptr};
+    long apl = 0;
+    if (alpha && p->levels > 1) {
+        apl = (long)round_up((size_t)p->lv[1].w, 256);
+        int rc = alpha_scratch(ctx, (size_t)apl * p->lv[1].h, &abuf[0], &abuf[1]);
+        if (rc) return rc;
+    }
+    const uint8_t* sa = alpha;
+    long sap = apitch;
+    for (int l = 1; l < p->levels; ++l) {
+        const Level& s = p->lv[l - 1];
+        const Level& d = p->lv[l];
+        uint8_t* da = alpha ? abuf[(l - 1) & 1] : nullptr;
+        int rc = do_reduce(ctx, p->ptype, p->channels, wrap, level_ptr(p, l - 1), s.w, s.h, s.pitch, s.plane,
+                           sa, sap, level_ptr(p, l), d.w, d.h, d.pitch, d.plane, da, apl);
+        if (rc) return rc;
+        sa = da;
+        sap = apl;
+    }
+    return ENBL_OK;
+}
+
+int laplacian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap)
+{
+    for (int l = 0; l + 1 < p->levels; ++l) {
+        const Level& f = p->lv[l];
+        const Level& c = p->lv[l + 1];
+        int rc = do_expand(ctx, p->ptype, p->channels, 0, wrap, level_ptr(p, l + 1), c.w, c.h, c.pitch, c.plane,
+                           level_ptr(p, l), f.w, f.h, f.pitch, f.plane);
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const void* img, long ipitch)
+{
+    const Level& l0 = p->lv[0];
+    char name[64];
+    snprintf(name, sizeof name, "to_pyramid_%s_c%d", pname(p->ptype), p->channels);
+    Timed t(ctx, name, (double)l0.w * l0.h * p->channels * (ptype_size(itype) + ptype_size(p->ptype)));
+    int rc = launch_to_pyramid(itype, p->ptype, frac, p->channels, img, ipitch, l0.w, l0.h,
+                               level_ptr(p, 0), l0.pitch, l0.plane, ctx->stream);
+    if (rc) return fail(rc, "copyToPyramidImage: unsupported image/pyramid type pair");
+    return launch_status(name);
+}
+
+int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels, int slot, enbl_pyramid** out)
+{
+    int seen = 0;
+    for (auto& c : ctx->cache) {
+        if (c.ptype == ptype && c.channels == C && c.w == w && c.h == h && c.levels == levels) {
+            if (seen == slot) { *out = c.p; return ENBL_OK; }
+            ++seen;
+        }
+    }
+    enbl_pyramid* p = nullptr;
+    int rc = enbl_pyramid_create(ctx, ptype, C, w, h, levels, &p);
+    if (rc) return rc;
+    ctx->cache.push_back({ptype, C, w, h, levels, p});
+    *out = p;
+    return ENBL_OK;
+}
+
+}  // namespace
+
+namespace enbl {
+
+std::vector<Level> plan_levels(int ptype, int C, int w, int h, int levels, size_t* bytes)
+{
+    std::vector<Level> lv;
+    size_t off = 0;
+    const size_t es = (size_t)ptype_size(ptype);
+    for (int l = 0; l < levels; ++l) {
+        Level L;
+        L.w = w;
+        L.h = h;
+        L.pitch = (long)round_up((size_t)w, 64);
+        L.plane = (long)round_up((size_t)L.pitch * h, 64);
+        L.offset = off;
+        off += round_up((size_t)L.plane * C * es, 256);
+        lv.push_back(L);
+        w = (w + 1) >> 1;
+        h = (h + 1) >> 1;
+    }
+    if (bytes) *bytes = off;
+    return lv;
+}
+
+}  // namespace enbl
+
+extern "C" {
+
+const char* enbl_last_error(void) { return g_error.c_str(); }
+
+int enbl_context_create(int device, enbl_context** out)
+{
+    if (!out) return fail(ENBL_E_ARG, "enbl_context_create: null out");
+    int n = 0;
+    int rc = hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (rc) return rc;
+    if (device < 0 || device >= n) return fail(ENBL_E_ARG, "enbl_context_create: no HIP device %d", device);
+    rc = hip_check(hipSetDevice(device), "hipSetDevice");
+    if (rc) return rc;
+    enbl_context* c = new enbl_context();
+    c->device = device;
+    c->profile = false;
+    c->scratch_alpha = nullptr;
+    c->scratch_alpha_bytes = 0;
+    rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc) { delete c; return rc; }
+    c->stream = c->own_stream;
+    *out = c;
+    return ENBL_OK;
+}
+
+int enbl_context_destroy(enbl_context* ctx)
+{
+    if (!ctx) return ENBL_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& c : ctx->cache) enbl_pyramid_destroy(c.p);
+    for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
+    if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
+    (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return ENBL_OK;
+}
+
+int enbl_context_set_stream(enbl_context* ctx, void* s)
+{
+    if (!ctx) return fail(ENBL_E_ARG, "null context");
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return ENBL_OK;
+}
+
+int enbl_context_get_stream(const enbl_context* ctx, void** s)
+{
+    if (!ctx || !s) return fail(ENBL_E_ARG, "null argument");
+    *s = (void*)ctx->stream;
+    return ENBL_OK;
+}
+
+int enbl_synchronize(enbl_context* ctx)
+{
+    if (!ctx) return fail(ENBL_E_ARG, "null context");
+    return hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int enbl_profile_enable(enbl_context* ctx, int on)
+{
+    if (!ctx) return fail(ENBL_E_ARG, "null context");
+    ctx->profile = on != 0;
+    return ENBL_OK;
+}
+
+int enbl_profile_report(enbl_context* ctx, char* buf, size_t n)
+{
+    if (!ctx || !buf || n == 0) return fail(ENBL_E_ARG, "null argument");
+    int rc = hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (rc) return rc;
+    struct Acc { double ms = 0.0, bytes = 0.0; int n = 0; };
+    std::map<std::string, Acc> acc;
+    for (auto& t : ctx->timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            auto& e = acc[t.name];
+            e.ms += ms;
+            e.bytes += t.bytes;
+            e.n += 1;
+        }
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    ctx->timed.clear();
+    std::string out;
+    for (auto& kv : acc) {
+        char line[200];
+        snprintf(line, sizeof line, "%s %.6f %d %.0f\n", kv.first.c_str(), kv.second.ms, kv.second.n, kv.second.bytes);
+        out += line;
+    }
+    snprintf(buf, n, "%s", out.c_str());
+    return ENBL_OK;
+}
+
+int enbl_reduce(enbl_context* ctx, int ptype, int C, int wrap,
+                const void* src, int sw, int sh, long sp, long spl, const uint8_t* sa, long sap,
+                void* dst, int dw, int dh, long dp, long dpl, uint8_t* da, long dap)
+{
+    if (!ctx || !src || !dst) return fail(ENBL_E_ARG, "enbl_reduce: null argument");
+    if (sa && !da) return fail(ENBL_E_ARG, "enbl_reduce: alpha source needs a destination alpha");
+    if (ptype_size(ptype) == 0 || ptype > ENBL_F64) return fail(ENBL_E_UNSUPPORTED, "enbl_reduce: pyramid type %d", ptype);
+    if ((sp & 1) || (spl & 1) || !aligned2(src, ptype))
+        return fail(ENBL_E_ARG, "enbl_reduce: source rows must start on even elements");
+    return do_reduce(ctx, ptype, C, wrap, src, sw, sh, sp, spl, sa, sap, dst, dw, dh, dp, dpl, da, dap);
+}
+
+int enbl_expand(enbl_context* ctx, int ptype, int C, int add, int wrap,
+                const void* src, int sw, int sh, long sp, long spl,
+                void* dst, int dw, int dh, long dp, long dpl)
+{
+    if (!ctx || !src || !dst) return fail(ENBL_E_ARG, "enbl_expand: null argument");
+    if (ptype_size(ptype) == 0 || ptype > ENBL_F64) return fail(ENBL_E_UNSUPPORTED, "enbl_expand: pyramid type %d", ptype);
+    if ((dp & 1) || (dpl & 1) || !aligned2(dst, ptype))
+        return fail(ENBL_E_ARG, "enbl_expand: destination rows must start on even elements");
+    return do_expand(ctx, ptype, C, add, wrap, src, sw, sh, sp, spl, dst, dw, dh, dp, dpl);
+}
+
+int enbl_pyramid_create(enbl_context* ctx, int ptype, int C, int w, int h, int levels, enbl_pyramid** out)
+{
+    if (!ctx || !out) return fail(ENBL_E_ARG, "enbl_pyramid_create: null argument");
+    if (ptype_size(ptype) == 0 || ptype > ENBL_F64) return fail(ENBL_E_UNSUPPORTED, "pyramid type %d", ptype);
+    if (C != 1 && C != 3) return fail(ENBL_E_UNSUPPORTED, "pyramid channels %d", C);
+    if (w < 1 || h < 1) return fail(ENBL_E_SIZE, "empty pyramid");
+    if (levels < 1 || levels > 29) return fail(ENBL_E_PRECONDITION, "filterHalfWidth: levels outside of permissible range");
+    enbl_pyramid* p = new enbl_pyramid();
+    p->ctx = ctx;
+    p->ptype = ptype; p->channels = C; p->w = w; p->h = h; p->levels = levels;
+    p->lv = plan_levels(ptype, C, w, h, levels, &p->bytes);
+    p->base = nullptr;
+    int rc = hip_check(hipMalloc(&p->base, p->bytes), "pyramid allocation");
+    if (rc) { delete p; return rc; }
+    *out = p;
+    return ENBL_OK;
+}
+
+int enbl_pyramid_destroy(enbl_pyramid* p)
+{
+    if (!p) return ENBL_OK;
+    if (p->base) (void)hipFree(p->base);
+    delete p;
+    return ENBL_OK;
+}
+
+int enbl_pyramid_levels(const enbl_pyramid* p) { return p ? p->levels : 0; }
+
+int enbl_pyramid_level(const enbl_pyramid* p, int l, void** data, int* w, int* h, long* pitch, long* plane)
+{
+    if (!p || l < 0 || l >= p->levels) return fail(ENBL_E_ARG, "enbl_pyramid_level: bad level");
+    if (data) *data = level_ptr(p, l);
+    if (w) *w = p->lv[l].w;
+    if (h) *h = p->lv[l].h;
+    if (pitch) *pitch = p->lv[l].pitch;
+    if (plane) *plane = p->lv[l].plane;
+    return ENBL_OK;
+}
+
+int enbl_pyramid_copy_level(enbl_pyramid* p, int l, void* buf, int to_pyramid)
+{
+    if (!p || !buf || l < 0 || l >= p->levels) return fail(ENBL_E_ARG, "enbl_pyramid_copy_level: bad argument");
+    const Level& L = p->lv[l];
+    const size_t es = (size_t)ptype_size(p->ptype);
+    hipStream_t s = p->ctx->stream;
+    for (int c = 0; c < p->channels; ++c) {
+        char* lvl = (char*)level_ptr(p, l) + (size_t)c * L.plane * es;
+        char* pk = (char*)buf + (size_t)c * L.w * L.h * es;
+        hipError_t e = to_pyramid
+            ? hipMemcpy2DAsync(lvl, L.pitch * es, pk, L.w * es, L.w * es, L.h, hipMemcpyDefault, s)
+            : hipMemcpy2DAsync(pk, L.w * es, lvl, L.pitch * es, L.w * es, L.h, hipMemcpyDefault, s);
+        int rc = hip_check(e, "enbl_pyramid_copy_level");
+        if (rc) return rc;
+    }
+    return hip_check(hipStreamSynchronize(s), "enbl_pyramid_copy_level");
+}
+
+int enbl_auto_levels(int w, int h, int minimum_levels, int exact_levels)
+{
+    unsigned s = (unsigned)std::max(0, std::min(w, h));
+    const unsigned lo = minimum_levels < 1 ? 1u : (unsigned)minimum_levels;
+    unsigned levels = lo;
+    while (levels <= 29) {
+        if (s <= 8u) break;
+        s = (s + 1u) >> 1;
+        ++levels;
+    }
+    if (levels > lo) {
+        if (exact_levels >= 1) levels = std::min(levels, (unsigned)exact_levels);
+        else if (exact_levels < 0) levels = ((int)levels + exact_levels >= (int)lo) ? levels - (unsigned)(-exact_levels) : lo;
+    }
+    return (int)std::min(levels, 29u);
+}
+
+int enbl_gaussian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
+                          const uint8_t* alpha, long apitch, enbl_pyramid* out)
+{
+    if (!ctx || !img || !out) return fail(ENBL_E_ARG, "enbl_gaussian_pyramid: null argument");
+    int rc = build_level0(ctx, out, itype, frac, img, ipitch);
+    if (rc) return rc;
+    return gaussian_levels(ctx, out, wrap, alpha, apitch);
+}
+
+int enbl_laplacian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
+                           const uint8_t* alpha, long apitch, enbl_pyramid* out)
+{
+    int rc = enbl_gaussian_pyramid(ctx, wrap, itype, frac, img, ipitch, alpha, apitch, out);
+    if (rc) return rc;
+    return laplacian_levels(ctx, out, wrap);
+}
+
+int enbl_collapse_pyramid(enbl_context* ctx, int wrap, enbl_pyramid* p)
+{
+    if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_collapse_pyramid: null argument");
+    for (int l = p->levels - 2; l >= 0; --l) {
+        const Level& f = p->lv[l];
+        const Level& c = p->lv[l + 1];
+        int rc = do_expand(ctx, p->ptype, p->channels, 1, wrap, level_ptr(p, l + 1), c.w, c.h, c.pitch, c.plane,
+                           level_ptr(p, l), f.w, f.h, f.pitch, f.plane);
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+int enbl_blend(enbl_context* ctx, const enbl_pyramid* m, const enbl_pyramid* wlp, enbl_pyramid* blp, double wv)
+{
+    if (!ctx || !m || !wlp || !blp) return fail(ENBL_E_ARG, "enbl_blend: null argument");
+    if (m->channels != 1 || wlp->channels != blp->channels || wlp->ptype != blp->ptype ||
+        m->levels < blp->levels || wlp->levels < blp->levels)
+        return fail(ENBL_E_SIZE, "enbl_blend: pyramid shapes disagree");
+    for (int l = 0; l < blp->levels; ++l) {
+        const Level& L = blp->lv[l];
+        if (m->lv[l].w != L.w || m->lv[l].h != L.h || wlp->lv[l].w != L.w || wlp->lv[l].h != L.h)
+            return fail(ENBL_E_SIZE, "enbl_blend: level %d sizes disagree", l);
+        Timed t(ctx, "blend", (double)L.w * L.h * (ptype_size(m->ptype) + 3.0 * blp->channels * ptype_size(blp->ptype)));
+        int rc = launch_blend(m->ptype, blp->ptype, blp->channels, L.w, L.h, level_ptr(m, l), m->lv[l].pitch,
+                              level_ptr(wlp, l), wlp->lv[l].pitch, wlp->lv[l].plane,
+                              level_ptr(blp, l), L.pitch, L.plane, wv, ctx->stream);
+        if (rc) return fail(rc, "enbl_blend: unsupported mask/pyramid types");
+        rc = launch_status("blend");
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+int enbl_copy_from_pyramid_if(enbl_context* ctx, const enbl_pyramid* p, int level, int frac,
+                              const uint8_t* mask, long mpitch, void* img, int itype, long ipitch,
+                              int dither, uint64_t seed)
+{
+    if (!ctx || !p || !mask || !img) return fail(ENBL_E_ARG, "enbl_copy_from_pyramid_if: null argument");
+    if (level < 0 || level >= p->levels) return fail(ENBL_E_ARG, "bad level");
+    const Level& L = p->lv[level];
+    Timed t(ctx, "from_pyramid", (double)L.w * L.h * (1.0 + p->channels * (ptype_size(p->ptype) + ptype_size(itype))));
+    int rc = launch_from_pyramid_if(p->ptype, itype, frac, p->channels, level_ptr(p, level), L.pitch, L.plane,
+                                    L.w, L.h, mask, mpitch, img, ipitch, dither, seed, ctx->stream);
+    if (rc) return fail(rc, "copyFromPyramidImageIf: unsupported pyramid/image type pair");
+    return launch_status("from_pyramid");
+}
+
+int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int levels, int wrap,
+                      const void* white, long wpitch, const uint8_t* walpha, long wapitch,
+                      void* black, long bpitch, uint8_t* balpha, long bapitch,
+                      const uint8_t* mask, long mpitch, int dither, uint64_t seed)
+{
+    if (!ctx || !white || !walpha || !black || !balpha || !mask) return fail(ENBL_E_ARG, "enbl_enblend_pair: null argument");
+    if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "image type %d", itype);
+    const int pt = image_ptype_for(itype);
+    const int mt = mask_ptype_for(itype);
+    const int ifrac = itype == ENBL_F32 ? 0 : 7;
+    const int mfrac = itype == ENBL_U8 ? 7 : 15;
+    enbl_pyramid *mgp = nullptr, *wlp = nullptr, *blp = nullptr;
+    int rc = cached_pyramid(ctx, mt, 1, w, h, levels, 0, &mgp);
+    if (!rc) rc = cached_pyramid(ctx, pt, C, w, h, levels, 0, &wlp);
+    if (!rc) rc = cached_pyramid(ctx, pt, C, w, h, levels, 1, &blp);
+    if (rc) return rc;
+
+    // Mask Gaussian pyramid without alpha (src/enblend.h:384-388).
+    rc = enbl_gaussian_pyramid(ctx, wrap, ENBL_U8, mfrac, mask, mpitch, nullptr, 0, mgp);
+    if (rc) return rc;
+    // Laplacian pyramids of white and black (src/enblend.h:419-446).
+    rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, white, wpitch, walpha, wapitch, wlp);
+    if (rc) return rc;
+    rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, black, bpitch, balpha, bapitch, blp);
+    if (rc) return rc;
+    // Union alpha (src/enblend.h:461-463).
+    {
+        Timed t(ctx, "alpha_union", 3.0 * w * h);
+        launch_alpha_union(walpha, wapitch, balpha, bapitch, w, h, ctx->stream);
+        rc = launch_status("alpha_union");
+        if (rc) return rc;
+    }
+    // blend with white = 255 << mask fraction bits (src/enblend.h:472-474).
+    rc = enbl_blend(ctx, mgp, wlp, blp, (double)(255 << mfrac));
+    if (rc) return rc;
+    rc = enbl_collapse_pyramid(ctx, wrap, blp);
+    if (rc) return rc;
+    return enbl_copy_from_pyramid_if(ctx, blp, 0, ifrac, balpha, bapitch, black, itype, bpitch, dither, seed);
+}
+
+}  // extern "C"

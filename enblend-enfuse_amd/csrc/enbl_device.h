@@ -1,0 +1,103 @@
+// Device-side arithmetic contract shared by all enblend/enfuse kernels.
+//
+// The pyramid types follow src/numerictraits.h:168-189 of the reference:
+//   Int16 pyramid / Int32 SKIPSM accumulator (8-bit images),
+//   Int32 pyramid / Int32 SKIPSM accumulator (16-bit images, masks),
+//   double pyramid / double accumulator (float images).
+// Integer accumulators wrap (two's complement) exactly like the reference's
+// Int32 arithmetic on x86-64; double operations keep the reference's
+// evaluation order and every file is compiled with -ffp-contract=off, so the
+// double path reproduces the reference's rounding step by step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace enbl {
+
+constexpr int kWave = 64;
+// Lanes 1..62 of a wave own output columns; lanes 0 and 63 only supply the
+// horizontal halo through cross-lane shuffles.
+constexpr int kOwned = kWave - 2;
+
+struct AccI {
+    typedef int32_t T;
+    __device__ static __forceinline__ T add(T a, T b) { return (T)((uint32_t)a + (uint32_t)b); }
+    __device__ static __forceinline__ T sub(T a, T b) { return (T)((uint32_t)a - (uint32_t)b); }
+    __device__ static __forceinline__ T mul(T a, int k) { return (T)((uint32_t)a * (uint32_t)k); }
+    __device__ static __forceinline__ T div(T a, int d) { return a / d; }
+    __device__ static __forceinline__ T zero() { return 0; }
+};
+
+struct AccD {
+    typedef double T;
+    __device__ static __forceinline__ T add(T a, T b) { return a + b; }
+    __device__ static __forceinline__ T sub(T a, T b) { return a - b; }
+    __device__ static __forceinline__ T mul(T a, int k) { return a * (double)k; }
+    __device__ static __forceinline__ T div(T a, int d) { return a / (double)d; }
+    __device__ static __forceinline__ T zero() { return 0.0; }
+};
+
+template <class P> struct Pix;
+template <> struct Pix<int16_t> {
+    typedef AccI A;
+    typedef int32_t S;
+    __device__ static __forceinline__ S load(int16_t v) { return v; }
+    // Store of an int into Int16 through VIGRA's accessor: static_cast (wrap).
+    __device__ static __forceinline__ int16_t wrap(S v) { return (int16_t)(uint16_t)(uint32_t)v; }
+    // NumericTraits<Int16>::fromPromote(int): clamp.
+    __device__ static __forceinline__ int16_t sat(S v) { return (int16_t)min(32767, max(-32768, v)); }
+};
+template <> struct Pix<int32_t> {
+    typedef AccI A;
+    typedef int32_t S;
+    __device__ static __forceinline__ S load(int32_t v) { return v; }
+    __device__ static __forceinline__ int32_t wrap(S v) { return v; }
+    __device__ static __forceinline__ int32_t sat(S v) { return v; }
+};
+template <> struct Pix<double> {
+    typedef AccD A;
+    typedef double S;
+    __device__ static __forceinline__ S load(double v) { return v; }
+    __device__ static __forceinline__ double wrap(S v) { return v; }
+    __device__ static __forceinline__ double sat(S v) { return v; }
+};
+
+// NumericTraits<T>::fromRealPromote: v -/+ 0.5 then truncate, clamped to T.
+// NaN follows the reference as built for x86-64 (cvttsd2si): INT_MIN for a
+// 32-bit target, 0 for a 16-bit target (low half of INT_MIN).
+template <class T> __device__ T from_real(double v);
+template <> __device__ __forceinline__ int32_t from_real<int32_t>(double v) {
+    if (v != v) return INT32_MIN;
+    if (v < 0.0) return v < -2147483648.0 ? INT32_MIN : (int32_t)(v - 0.5);
+    return v > 2147483647.0 ? INT32_MAX : (int32_t)(v + 0.5);
+}
+template <> __device__ __forceinline__ int16_t from_real<int16_t>(double v) {
+    if (v != v) return 0;
+    if (v < 0.0) return v < -32768.0 ? (int16_t)-32768 : (int16_t)(int32_t)(v - 0.5);
+    return v > 32767.0 ? (int16_t)32767 : (int16_t)(int32_t)(v + 0.5);
+}
+template <> __device__ __forceinline__ double from_real<double>(double v) { return v; }
+
+// Cross-lane moves within a wave (ds_bpermute); 64-bit values move as two
+// 32-bit halves.
+__device__ __forceinline__ int32_t lane_up(int32_t v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ int32_t lane_down(int32_t v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, kWave); }
+__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, kWave); }
+
+// Counter-based dither draw keyed by (seed, x, y, channel), mapped onto the
+// reference's integer grid (r & half_m1) + quarter (src/fixmath.h:246-264).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ int dither_rand(uint64_t seed, int x, int y, int c, int frac) {
+    const uint64_t key = seed ^ ((uint64_t)(uint32_t)y << 32) ^ ((uint64_t)(uint32_t)x << 2) ^ (uint64_t)(uint32_t)c;
+    const uint64_t r = splitmix64(key);
+    return (int)(r & (uint64_t)((1 << (frac - 1)) - 1)) + (1 << (frac - 2));
+}
+
+}  // namespace enbl

@@ -1,0 +1,89 @@
+// Host-side internals shared by the launchers and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/enblend_amd.h"
+
+namespace enbl {
+
+inline int ptype_size(int t) {
+    switch (t) {
+    case ENBL_I16: return 2;
+    case ENBL_I32: return 4;
+    case ENBL_F64: return 8;
+    case ENBL_U8: return 1;
+    case ENBL_U16: return 2;
+    case ENBL_F32: return 4;
+    }
+    return 0;
+}
+
+struct ReduceArgs {
+    const void* src; long spitch; long splane; int sw; int sh;
+    const uint8_t* salpha; long sapitch;
+    void* dst; long dpitch; long dplane; int dw; int dh;
+    uint8_t* dalpha; long dapitch;
+    int channels; int wrap; int rows;
+};
+
+struct ExpandArgs {
+    const void* src; long spitch; long splane; int sw; int sh;
+    void* dst; long dpitch; long dplane; int dw; int dh;
+    int channels; int wrap; int add; int rows;
+};
+
+int launch_reduce(int ptype, ReduceArgs a, hipStream_t s);
+int launch_expand(int ptype, ExpandArgs a, hipStream_t s);
+
+// Pointwise kernels (pointwise.hip).
+int launch_to_pyramid(int itype, int ptype, int frac, int channels,
+                      const void* img, long ipitch, int w, int h,
+                      void* planes, long ppitch, long pplane, hipStream_t s);
+int launch_from_pyramid_if(int ptype, int itype, int frac, int channels,
+                           const void* planes, long ppitch, long pplane, int w, int h,
+                           const uint8_t* mask, long mpitch, void* img, long ipitch,
+                           int dither, uint64_t seed, hipStream_t s);
+int launch_blend(int mtype, int ptype, int channels, int w, int h,
+                 const void* mask, long mpitch,
+                 const void* white, long wpitch, long wplane,
+                 void* black, long bpitch, long bplane,
+                 double white_value, hipStream_t s);
+int launch_alpha_union(const uint8_t* src, long spitch, uint8_t* dst, long dpitch, int w, int h,
+                       hipStream_t s);
+
+struct Level {
+    int w, h;
+    long pitch;   // elements
+    long plane;   // elements between channel planes
+    size_t offset;  // bytes from the arena base
+};
+
+std::vector<Level> plan_levels(int ptype, int channels, int w, int h, int levels, size_t* bytes);
+
+}  // namespace enbl
+
+struct enbl_context {
+    int device;
+    hipStream_t own_stream;
+    hipStream_t stream;
+    bool profile;
+    struct Timed { std::string name; double bytes; hipEvent_t start, stop; };
+    std::vector<Timed> timed;
+    struct Cached { int ptype, channels, w, h, levels; enbl_pyramid* p; };
+    std::vector<Cached> cache;   // pipeline scratch pyramids
+    void* scratch_alpha;         // per-level alpha scratch for Gaussian builds
+    size_t scratch_alpha_bytes;
+};
+
+struct enbl_pyramid {
+    enbl_context* ctx;
+    int ptype, channels, w, h, levels;
+    void* base;
+    size_t bytes;
+    std::vector<enbl::Level> lv;
+};

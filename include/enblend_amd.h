@@ -1,0 +1,172 @@
+/*
+ * enblend_amd -- C ABI of the MI355X (gfx950) implementation of the
+ * enblend/enfuse multiresolution path.
+ *
+ * Every entry point takes plain device pointers, sizes and pitches, returns
+ * an int status (ENBL_OK or a negative ENBL_E_* code, never throws) and is
+ * ordered on the context's HIP stream.  Images are interleaved like the
+ * reference's vigra::RGBValue images; pyramids are planar per channel.  All
+ * pitches are in elements of the buffer's own type.
+ *
+ * The reference exposes this path as C++ templates, not a C ABI; each
+ * declaration names the template it replaces (jackmitch/enblend-enfuse,
+ * VERSION 4.3-af6cddaed31b, paths relative to the repository root).
+ *
+ * Type codes follow the reference's numeric traits (src/numerictraits.h:168-189):
+ *   8-bit images   -> ENBL_I16 pyramid, 7 fraction bits; mask pyramid ENBL_I16, 7
+ *   16-bit images  -> ENBL_I32 pyramid, 7 fraction bits; mask pyramid ENBL_I32, 15
+ *   float images   -> ENBL_F64 pyramid (log transform); mask pyramid ENBL_I32, 15
+ */
+#ifndef ENBLEND_AMD_H
+#define ENBLEND_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ENBL_OK 0
+#define ENBL_E_PRECONDITION (-1) /* vigra_precondition violated in the reference */
+#define ENBL_E_SIZE (-2)         /* inconsistent level sizes */
+#define ENBL_E_ARG (-3)          /* null pointer, bad pitch or alignment */
+#define ENBL_E_UNSUPPORTED (-4)  /* type / channel combination not provided */
+#define ENBL_E_OOM (-5)          /* device allocation failed (std::bad_alloc) */
+#define ENBL_E_HIP (-6)          /* HIP runtime error */
+
+/* pixel type codes */
+#define ENBL_I16 0
+#define ENBL_I32 1
+#define ENBL_F64 2
+#define ENBL_U8 10
+#define ENBL_U16 11
+#define ENBL_F32 12
+
+/* dither modes of the pyramid-to-image conversion */
+#define ENBL_DITHER_NONE 0 /* round half up inside the dither band */
+#define ENBL_DITHER_DET 1  /* counter-based draw keyed by (seed, x, y, channel) */
+
+typedef struct enbl_context enbl_context;
+typedef struct enbl_pyramid enbl_pyramid;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own default-priority
+ * stream. */
+int enbl_context_create(int device, enbl_context** out);
+int enbl_context_destroy(enbl_context* ctx);
+/* Order all later work on an external HIP stream (NULL restores the
+ * context's own stream). */
+int enbl_context_set_stream(enbl_context* ctx, void* hip_stream);
+int enbl_context_get_stream(const enbl_context* ctx, void** hip_stream);
+int enbl_synchronize(enbl_context* ctx);
+/* Message of the last failing call on this thread. */
+const char* enbl_last_error(void);
+/* Kernel timing: when enabled every launch is bracketed by HIP events on the
+ * context's stream; the report synchronises and writes one line per kernel
+ * name: "name total_ms launches algorithmic_bytes" (bytes = operands at the
+ * reference's storage types, each read once and each result written once). */
+int enbl_profile_enable(enbl_context* ctx, int on);
+int enbl_profile_report(enbl_context* ctx, char* buf, size_t buflen);
+
+/* ---- single-level operations ---------------------------------------- */
+
+/* Replaces reduce<SKIPSMImagePixelType, SKIPSMAlphaPixelType>(wraparound,
+ * src, mask, dest, destMask) -- src/pyramid.h:166-573 -- when src_alpha is
+ * non-NULL, and reduce<SKIPSMImagePixelType>(wraparound, src, dest) --
+ * src/pyramid.h:601-848 -- when it is NULL.  `channels` planes of `ptype`
+ * (plane strides splane/dplane); dst is ceil(sw/2) x ceil(sh/2).
+ * ENBL_E_PRECONDITION when sw < 2 or sh < 2 ("src image too small in reduce"). */
+int enbl_reduce(enbl_context* ctx, int ptype, int channels, int wraparound,
+                const void* src, int sw, int sh, long spitch, long splane,
+                const uint8_t* src_alpha, long sapitch,
+                void* dst, int dw, int dh, long dpitch, long dplane,
+                uint8_t* dst_alpha, long dapitch);
+
+/* Replaces expand<SKIPSMImagePixelType>(add, wraparound, src, dest) --
+ * src/pyramid.h:1102-1419 with the combine functors of :1425-1457:
+ * add != 0: dest = fromPromote(dest + expand(src)) (collapse),
+ * add == 0: dest = dest - expand(src) (Laplacian).  dw in {2sw-1, 2sw}. */
+int enbl_expand(enbl_context* ctx, int ptype, int channels, int add, int wraparound,
+                const void* src, int sw, int sh, long spitch, long splane,
+                void* dst, int dw, int dh, long dpitch, long dplane);
+
+/* ---- pyramids --------------------------------------------------------- */
+
+/* Device pyramid of `levels` levels; level l is ceil(w/2^l) x ceil(h/2^l)
+ * (src/pyramid.h:1509-1510), `channels` planes per level.  Replaces the
+ * std::vector<PyramidImageType*> the reference allocates. */
+int enbl_pyramid_create(enbl_context* ctx, int ptype, int channels, int w, int h, int levels,
+                        enbl_pyramid** out);
+int enbl_pyramid_destroy(enbl_pyramid* p);
+int enbl_pyramid_levels(const enbl_pyramid* p);
+int enbl_pyramid_level(const enbl_pyramid* p, int level, void** data, int* w, int* h,
+                       long* pitch, long* plane);
+
+/* Copy level `level` between the pyramid and a packed buffer of
+ * channels x h x w elements (host or device memory).  to_pyramid != 0
+ * writes the buffer into the level.  Synchronous with respect to the host.
+ * Replaces direct element access to the reference's BasicImage levels. */
+int enbl_pyramid_copy_level(enbl_pyramid* p, int level, void* buf, int to_pyramid);
+
+/* Level count of roiBounds (src/bounds.h:121-133): halve the short side of
+ * the ROI until it is <= 8, starting from minimum_levels, capped at 29;
+ * exact_levels > 0 caps it, exact_levels < 0 subtracts (--levels). */
+int enbl_auto_levels(int w, int h, int minimum_levels, int exact_levels);
+
+/* Replaces gaussianPyramid<...>(numLevels, wraparound, src, alpha) --
+ * src/pyramid.h:1467-1559 (alpha != NULL) and :1562-1636 (alpha == NULL) --
+ * with copyToPyramidImage (src/fixmath.h:2018-2067) for level 0.  `img` is
+ * an interleaved device image of `itype` (ENBL_U8, ENBL_U16, ENBL_F32) with
+ * the pyramid's channel count; frac_bits is PyramidFractionBits. */
+int enbl_gaussian_pyramid(enbl_context* ctx, int wraparound, int itype, int frac_bits,
+                          const void* img, long ipitch, const uint8_t* alpha, long apitch,
+                          enbl_pyramid* out);
+
+/* Replaces laplacianPyramid<...>(exportName, numLevels, wraparound, src,
+ * alpha) -- src/pyramid.h:1646-1747. */
+int enbl_laplacian_pyramid(enbl_context* ctx, int wraparound, int itype, int frac_bits,
+                           const void* img, long ipitch, const uint8_t* alpha, long apitch,
+                           enbl_pyramid* out);
+
+/* Replaces collapsePyramid<SKIPSMImagePixelType>(wraparound, p) --
+ * src/pyramid.h:1757-1784.  In place; level 0 holds the result. */
+int enbl_collapse_pyramid(enbl_context* ctx, int wraparound, enbl_pyramid* p);
+
+/* Replaces blend(maskGP, whiteLP, blackLP, maskPyramidWhiteValue) --
+ * src/blend.h:81-121 with CartesianBlendFunctor (:41-76).  Writes black_lp. */
+int enbl_blend(enbl_context* ctx, const enbl_pyramid* mask_gp, const enbl_pyramid* white_lp,
+               enbl_pyramid* black_lp, double white_value);
+
+/* Replaces copyFromPyramidImageIf<...>(src, mask, dest) --
+ * src/fixmath.h:2157-2294 with SpecializedConvertPyramidToScalarFunctor
+ * (:209-358) for the identity colour space: writes level `level` of `p` into
+ * the interleaved device image `img` of `itype` where mask != 0.  Integer
+ * pyramids dither inside the band [1/4, 3/4) of a unit per `dither_mode`. */
+int enbl_copy_from_pyramid_if(enbl_context* ctx, const enbl_pyramid* p, int level, int frac_bits,
+                              const uint8_t* mask, long mpitch, void* img, int itype, long ipitch,
+                              int dither_mode, uint64_t seed);
+
+/* ---- pipelines ---------------------------------------------------------- */
+
+/* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510
+ * of enblendMain: mask Gaussian pyramid (no alpha) of the 0/255 seam mask,
+ * Laplacian pyramids of the white and black images under their own alpha,
+ * black alpha |= white alpha, blend, collapse and the masked copy into the
+ * black image.  black/black_alpha are updated in place.  Device buffers;
+ * intermediate pyramids are cached in the context and reused by later calls
+ * of the same geometry. */
+int enbl_enblend_pair(enbl_context* ctx, int itype, int channels, int w, int h, int levels,
+                      int wraparound,
+                      const void* white, long wpitch, const uint8_t* white_alpha, long wapitch,
+                      void* black, long bpitch, uint8_t* black_alpha, long bapitch,
+                      const uint8_t* mask, long mpitch,
+                      int dither_mode, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENBLEND_AMD_H */

@@ -1,0 +1,102 @@
+/*
+ * CPU restatement of the enblend/enfuse multiresolution path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in enblend-enfuse_amd/ and the CPU baseline timed by
+ * bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it; the product path never does.
+ *
+ * Every entry point restates the reference algorithm it names (file:line of
+ * jackmitch/enblend-enfuse, VERSION 4.3-af6cddaed31b).  Images are planar
+ * per channel unless stated otherwise; pitches are in elements.
+ *
+ * Pixel-type codes (pyramid types, src/numerictraits.h:168-189):
+ *   ORC_I16 = Int16 pyramid, Int32 SKIPSM accumulator
+ *   ORC_I32 = Int32 pyramid, Int32 SKIPSM accumulator
+ *   ORC_F64 = double pyramid, double SKIPSM accumulator
+ * Image-type codes: ORC_U8, ORC_U16, ORC_F32 (interleaved channels).
+ *
+ * All functions return 0 on success and a negative code on a violated
+ * precondition (the reference's vigra_precondition).
+ */
+#ifndef ENBLEND_ORACLE_H
+#define ENBLEND_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_I16 = 0, ORC_I32 = 1, ORC_F64 = 2 };
+enum { ORC_U8 = 10, ORC_U16 = 11, ORC_F32 = 12 };
+enum { ORC_DITHER_NONE = 0, ORC_DITHER_DET = 1 };
+
+/* reduce with alpha (src/pyramid.h:166-573) when alpha != NULL, otherwise
+ * reduce without alpha (src/pyramid.h:601-848).  One channel. */
+int orc_reduce(int ptype, int wrap,
+               const void* src, int sw, int sh, long spitch,
+               const uint8_t* alpha, long apitch,
+               void* dst, int dw, int dh, long dpitch,
+               uint8_t* dalpha, long dapitch);
+
+/* expand (src/pyramid.h:1102-1419) with the add (collapse) or subtract
+ * (Laplacian) combine functor (src/pyramid.h:1425-1457).  One channel. */
+int orc_expand(int ptype, int add, int wrap,
+               const void* src, int sw, int sh, long spitch,
+               void* dst, int dw, int dh, long dpitch);
+
+/* Level count rule of roiBounds (src/bounds.h:121-133) for a ROI of w x h,
+ * minimum-pyramid-levels = 1, no --levels override. */
+int orc_auto_levels(int w, int h);
+
+/* Pointwise converters (src/fixmath.h:116-206, 209-358). */
+int orc_to_pyramid(int itype, int ptype, int frac_bits, int channels,
+                   const void* img, int w, int h, long ipitch,
+                   void* planes, long ppitch, long pplane);
+int orc_from_pyramid_if(int ptype, int itype, int frac_bits, int channels,
+                        const void* planes, int w, int h, long ppitch, long pplane,
+                        const uint8_t* mask, long mpitch,
+                        void* img, long ipitch,
+                        int dither_mode, uint64_t seed);
+
+/* Dither random integer in [quarter, quarter + half_m1] for pixel (x, y, c);
+ * shared deterministic dither contract. */
+int orc_dither_rand(uint64_t seed, int x, int y, int c, int frac_bits);
+
+/* CartesianBlendFunctor over one level (src/blend.h:41-76, 81-121). */
+int orc_blend_level(int mtype, int ptype, int channels, int w, int h,
+                    const void* mask, long mpitch,
+                    const void* white, long wpitch, long wplane,
+                    void* black, long bpitch, long bplane,
+                    double white_value);
+
+/* Whole enblend step for one white/black pair on the ROI canvas
+ * (src/enblend.h:378-510): mask Gaussian pyramid (no alpha), two Laplacian
+ * pyramids (alpha), union alpha, blend, collapse, masked copy out.
+ * black/balpha are updated in place. itype = ORC_U8, ORC_U16 or ORC_F32. */
+int orc_enblend_pair(int itype, int channels, int w, int h, int levels, int wrap,
+                     const void* white, const uint8_t* walpha,
+                     void* black, uint8_t* balpha,
+                     const uint8_t* mask,
+                     int dither_mode, uint64_t seed);
+
+/* Gaussian / Laplacian pyramid of an interleaved image with alpha into a
+ * caller-provided packed planar buffer (level l: channels planes of
+ * w_l*h_l, pitch w_l, levels concatenated). Returns total elements written
+ * in *n_out. */
+int orc_pyramid(int laplacian, int itype, int channels, int w, int h, int levels, int wrap,
+                const void* img, const uint8_t* alpha, void* out, long* n_out);
+
+/* Collapse a packed planar pyramid in place (src/pyramid.h:1757-1784). */
+int orc_collapse(int ptype, int channels, int w, int h, int levels, int wrap, void* pyr);
+
+/* Number of OpenMP threads the pointwise stages use. */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

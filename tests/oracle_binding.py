@@ -1,0 +1,170 @@
+"""ctypes binding of the oracle (oracle/liboracle.so): the CPU restatement
+used as the checker.  Builds it with oracle/Makefile when missing."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+I16, I32, F64 = 0, 1, 2
+U8, U16, F32 = 10, 11, 12
+NP_P = {I16: np.int16, I32: np.int32, F64: np.float64}
+NP_I = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+
+_vp, _i, _l = ctypes.c_void_p, ctypes.c_int, ctypes.c_long
+_SIGS = {
+    "orc_reduce": [_i, _i, _vp, _i, _i, _l, _vp, _l, _vp, _i, _i, _l, _vp, _l],
+    "orc_expand": [_i, _i, _i, _vp, _i, _i, _l, _vp, _i, _i, _l],
+    "orc_auto_levels": [_i, _i],
+    "orc_to_pyramid": [_i, _i, _i, _i, _vp, _i, _i, _l, _vp, _l, _l],
+    "orc_from_pyramid_if": [_i, _i, _i, _i, _vp, _i, _i, _l, _l, _vp, _l, _vp, _l, _i, ctypes.c_uint64],
+    "orc_dither_rand": [ctypes.c_uint64, _i, _i, _i, _i],
+    "orc_blend_level": [_i, _i, _i, _i, _i, _vp, _l, _vp, _l, _l, _vp, _l, _l, ctypes.c_double],
+    "orc_enblend_pair": [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, ctypes.c_uint64],
+    "orc_pyramid": [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, ctypes.POINTER(_l)],
+    "orc_collapse": [_i, _i, _i, _i, _i, _i, _vp],
+    "orc_num_threads": [],
+}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+class Oracle:
+    def __init__(self, lib):
+        self.lib = lib
+
+    def reduce(self, src, alpha, wrap, ptype):
+        """src: (h, w) array of the pyramid type; alpha: (h, w) uint8 or None."""
+        sh, sw = src.shape
+        dh, dw = (sh + 1) // 2, (sw + 1) // 2
+        src = np.ascontiguousarray(src, NP_P[ptype])
+        dst = np.zeros((dh, dw), NP_P[ptype])
+        da = np.zeros((dh, dw), np.uint8) if alpha is not None else None
+        if alpha is not None:
+            alpha = np.ascontiguousarray(alpha, np.uint8)
+        rc = self.lib.orc_reduce(ptype, int(wrap), _p(src), sw, sh, sw, _p(alpha), sw,
+                                 _p(dst), dw, dh, dw, _p(da), dw)
+        if rc:
+            raise ValueError(f"orc_reduce rc={rc}")
+        return (dst, da) if alpha is not None else dst
+
+    def expand(self, src, dst, add, wrap, ptype):
+        """Returns a new array: dst combined with expand(src)."""
+        sh, sw = src.shape
+        dh, dw = dst.shape
+        src = np.ascontiguousarray(src, NP_P[ptype])
+        out = np.array(dst, dtype=NP_P[ptype], copy=True)
+        rc = self.lib.orc_expand(ptype, int(add), int(wrap), _p(src), sw, sh, sw, _p(out), dw, dh, dw)
+        if rc:
+            raise ValueError(f"orc_expand rc={rc}")
+        return out
+
+    def auto_levels(self, w, h):
+        return self.lib.orc_auto_levels(w, h)
+
+    def pyramid(self, img, alpha, levels, wrap, laplacian, itype):
+        """img: (h, w, C) interleaved; returns a list of (C, h_l, w_l) levels."""
+        h, w, C = img.shape
+        ptype = {U8: I16, U16: I32, F32: F64}[itype]
+        dims = []
+        tw, th = w, h
+        total = 0
+        for _ in range(levels):
+            dims.append((th, tw))
+            total += C * tw * th
+            tw, th = (tw + 1) // 2, (th + 1) // 2
+        out = np.zeros(total, NP_P[ptype])
+        n = _l()
+        img = np.ascontiguousarray(img, NP_I[itype])
+        a = None if alpha is None else np.ascontiguousarray(alpha, np.uint8)
+        rc = self.lib.orc_pyramid(int(laplacian), itype, C, w, h, levels, int(wrap), _p(img), _p(a), _p(out),
+                                  ctypes.byref(n))
+        if rc:
+            raise ValueError(f"orc_pyramid rc={rc}")
+        res, off = [], 0
+        for (lh, lw) in dims:
+            res.append(out[off: off + C * lh * lw].reshape(C, lh, lw).copy())
+            off += C * lh * lw
+        return res
+
+    def collapse(self, levels_list, wrap, ptype):
+        C = levels_list[0].shape[0]
+        h, w = levels_list[0].shape[1:]
+        flat = np.concatenate([np.ascontiguousarray(l, NP_P[ptype]).ravel() for l in levels_list])
+        rc = self.lib.orc_collapse(ptype, C, w, h, len(levels_list), int(wrap), _p(flat))
+        if rc:
+            raise ValueError(f"orc_collapse rc={rc}")
+        res, off = [], 0
+        for l in levels_list:
+            n = l.size
+            res.append(flat[off: off + n].reshape(l.shape).copy())
+            off += n
+        return res
+
+    def to_pyramid(self, img, itype, ptype, frac):
+        h, w, C = img.shape
+        out = np.zeros((C, h, w), NP_P[ptype])
+        img = np.ascontiguousarray(img, NP_I[itype])
+        rc = self.lib.orc_to_pyramid(itype, ptype, frac, C, _p(img), w, h, w * C, _p(out), w, w * h)
+        if rc:
+            raise ValueError(f"orc_to_pyramid rc={rc}")
+        return out
+
+    def from_pyramid_if(self, planes, mask, img, ptype, itype, frac, dither, seed):
+        C, h, w = planes.shape
+        out = np.array(img, dtype=NP_I[itype], copy=True)
+        planes = np.ascontiguousarray(planes, NP_P[ptype])
+        mask = np.ascontiguousarray(mask, np.uint8)
+        rc = self.lib.orc_from_pyramid_if(ptype, itype, frac, C, _p(planes), w, h, w, w * h, _p(mask), w,
+                                          _p(out), w * C, dither, seed)
+        if rc:
+            raise ValueError(f"orc_from_pyramid_if rc={rc}")
+        return out
+
+    def dither_rand(self, seed, x, y, c, frac):
+        return self.lib.orc_dither_rand(seed, x, y, c, frac)
+
+    def blend_level(self, mask, white, black, mtype, ptype, white_value):
+        C, h, w = white.shape
+        out = np.array(black, dtype=NP_P[ptype], copy=True)
+        mask = np.ascontiguousarray(mask, NP_P[mtype])
+        white = np.ascontiguousarray(white, NP_P[ptype])
+        rc = self.lib.orc_blend_level(mtype, ptype, C, w, h, _p(mask), w, _p(white), w, w * h, _p(out), w, w * h,
+                                      float(white_value))
+        if rc:
+            raise ValueError(f"orc_blend_level rc={rc}")
+        return out
+
+    def enblend_pair(self, white, walpha, black, balpha, mask, levels, wrap, itype, dither, seed):
+        h, w, C = white.shape
+        b = np.array(black, dtype=NP_I[itype], copy=True)
+        ba = np.array(balpha, dtype=np.uint8, copy=True)
+        white = np.ascontiguousarray(white, NP_I[itype])
+        walpha = np.ascontiguousarray(walpha, np.uint8)
+        mask = np.ascontiguousarray(mask, np.uint8)
+        rc = self.lib.orc_enblend_pair(itype, C, w, h, levels, int(wrap), _p(white), _p(walpha), _p(b), _p(ba),
+                                       _p(mask), dither, seed)
+        if rc:
+            raise ValueError(f"orc_enblend_pair rc={rc}")
+        return b, ba
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def load_oracle():
+    if not os.path.exists(ORACLE_LIB):
+        build_oracle()
+    lib = ctypes.CDLL(ORACLE_LIB)
+    for name, args in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    return Oracle(lib)
