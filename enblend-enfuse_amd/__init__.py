@@ -54,12 +54,53 @@ _SIGS = {
     "enbl_blend": (_c_int, [_vp, _vp, _vp, _vp, ctypes.c_double]),
     "enbl_copy_from_pyramid_if": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _vp, _c_int, _c_long,
                                            _c_int, ctypes.c_uint64]),
+    "enbl_enfuse_mask": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp, _vp, _c_long]),
+    "enbl_enfuse": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int,
+                             _vp, _c_long, _vp, _c_long, _c_int, ctypes.c_uint64]),
     "enbl_enblend_pair": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                    _vp, _c_long, _vp, _c_long, _vp, _c_long, _vp, _c_long,
                                    _vp, _c_long, _c_int, ctypes.c_uint64]),
 }
 
 _lib = None
+
+EXPOSURE_GAUSSIAN, EXPOSURE_LORENTZIAN, EXPOSURE_HALFSINE, EXPOSURE_FULLSINE, EXPOSURE_BISQUARE = range(5)
+
+
+class EnfuseParams(ctypes.Structure):
+    """enbl_enfuse_params (include/enblend_amd.h)."""
+    _fields_ = [("w_exposure", ctypes.c_double), ("w_saturation", ctypes.c_double),
+                ("w_contrast", ctypes.c_double), ("w_entropy", ctypes.c_double),
+                ("exposure_optimum", ctypes.c_double), ("exposure_width", ctypes.c_double),
+                ("exposure_function", ctypes.c_int), ("contrast_window", ctypes.c_int),
+                ("entropy_window", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("exposure_table", ctypes.c_void_p)]
+
+
+def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
+                  exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXPOSURE_GAUSSIAN,
+                  contrast_window=5, entropy_window=3, exposure_table=None):
+    """Parameters with the reference's defaults (src/enfuse.cc:118-146)."""
+    p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
+                     exposure_function, contrast_window, entropy_window, 0, None)
+    if exposure_table is not None:
+        import numpy as np
+        tab = np.ascontiguousarray(exposure_table, np.float64)
+        p.exposure_table = tab.ctypes.data
+        p._keep = tab
+    return p
+
+
+def enfuse(ctx, itype, imgs, alphas, prm, levels, out, out_alpha, wrap=False, dither=1, seed=0):
+    """Fuse K device images (torch tensors (h, w, 3)) into out/out_alpha."""
+    K = len(imgs)
+    h, w = imgs[0].shape[0], imgs[0].shape[1]
+    ip = (_vp * K)(*[t.data_ptr() for t in imgs])
+    ap = (_vp * K)(*[t.data_ptr() for t in alphas])
+    ipp = (_c_long * K)(*[w * 3] * K)
+    app = (_c_long * K)(*[w] * K)
+    check(lib().enbl_enfuse(ctx.h, itype, w, h, K, ip, ipp, ap, app, ctypes.byref(prm), levels, int(wrap),
+                            _vp(out.data_ptr()), w * 3, _vp(out_alpha.data_ptr()), w, dither, seed))
 
 
 class EnblError(RuntimeError):

@@ -15,8 +15,10 @@
 #include "enbl_internal.h"
 
 using namespace enbl;
+using namespace enbl::host;
 
-namespace {
+namespace enbl {
+namespace host {
 
 thread_local std::string g_error;
 
@@ -43,11 +45,7 @@ int launch_status(const char* what)
     return hip_check(hipGetLastError(), what);
 }
 
-// Brackets one launch with HIP events when the context profiles.
-struct Timed {
-    enbl_context* ctx;
-    size_t idx;
-    Timed(enbl_context* c, const char* name, double bytes) : ctx(c), idx((size_t)-1) {
+Timed::Timed(enbl_context* c, const char* name, double bytes) : ctx(c), idx((size_t)-1) {
         if (!ctx->profile) return;
         enbl_context::Timed t;
         t.name = name;
@@ -57,11 +55,11 @@ struct Timed {
         (void)hipEventRecord(t.start, ctx->stream);
         ctx->timed.push_back(t);
         idx = ctx->timed.size() - 1;
-    }
-    ~Timed() {
-        if (idx != (size_t)-1) (void)hipEventRecord(ctx->timed[idx].stop, ctx->stream);
-    }
-};
+}
+
+Timed::~Timed() {
+    if (idx != (size_t)-1) (void)hipEventRecord(ctx->timed[idx].stop, ctx->stream);
+}
 
 const char* pname(int t)
 {
@@ -208,9 +206,26 @@ int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels
     return ENBL_OK;
 }
 
-}  // namespace
+int scratch(enbl_context* ctx, int slot, size_t bytes, void** out)
+{
+    if (slot < 0 || slot >= 8) return fail(ENBL_E_ARG, "scratch slot %d", slot);
+    if (ctx->scratch_bytes[slot] < bytes) {
+        if (ctx->scratch_buf[slot]) {
+            int rc = hip_check(hipStreamSynchronize(ctx->stream), "scratch");
+            if (rc) return rc;
+            (void)hipFree(ctx->scratch_buf[slot]);
+            ctx->scratch_buf[slot] = nullptr;
+            ctx->scratch_bytes[slot] = 0;
+        }
+        int rc = hip_check(hipMalloc(&ctx->scratch_buf[slot], bytes), "scratch allocation");
+        if (rc) return rc;
+        ctx->scratch_bytes[slot] = bytes;
+    }
+    *out = ctx->scratch_buf[slot];
+    return ENBL_OK;
+}
 
-namespace enbl {
+}  // namespace host
 
 std::vector<Level> plan_levels(int ptype, int C, int w, int h, int levels, size_t* bytes)
 {
@@ -253,6 +268,7 @@ int enbl_context_create(int device, enbl_context** out)
     c->profile = false;
     c->scratch_alpha = nullptr;
     c->scratch_alpha_bytes = 0;
+    for (int i = 0; i < 8; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) { delete c; return rc; }
     c->stream = c->own_stream;
@@ -268,6 +284,7 @@ int enbl_context_destroy(enbl_context* ctx)
     for (auto& c : ctx->cache) enbl_pyramid_destroy(c.p);
     for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
     if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
+    for (int i = 0; i < 8; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return ENBL_OK;

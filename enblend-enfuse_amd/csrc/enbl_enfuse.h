@@ -1,0 +1,31 @@
+// Launchers of the enfuse weighting and merge kernels (enfuse.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/enblend_amd.h"
+
+namespace enbl {
+
+struct WeightArgs {
+    double w_exposure, w_saturation, w_contrast, w_entropy;
+    double exposure_optimum, exposure_width;   // width already rescaled to the function's FWHM
+    int exposure_function;
+};
+
+int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s);
+int launch_stddev(int itype, const void* gray, long gp, const uint8_t* alpha, long ap, int w, int h, int n,
+                  void* out, long op, hipStream_t s);
+int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h, int n,
+                   const double* tab, void* out, long op, hipStream_t s);
+int launch_weights(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h,
+                   const WeightArgs& a, const float* lut, const void* grad, long gp, const void* ent, long ep,
+                   float* mask, long mp, float* norm, long np_, uint8_t* ualpha, long up, hipStream_t s);
+int launch_normalize(int mtype, const float* mask, long mp, const float* norm, long np_, int w, int h,
+                     float fallback, double scale, void* out, long op, hipStream_t s);
+int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
+                 int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
+int launch_fill(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace enbl

@@ -67,6 +67,41 @@ std::vector<Level> plan_levels(int ptype, int channels, int w, int h, int levels
 
 }  // namespace enbl
 
+struct enbl_context;
+struct enbl_pyramid;
+
+namespace enbl {
+namespace host {
+
+// Error reporting: stores the message for enbl_last_error and returns code.
+int fail(int code, const char* fmt, ...);
+int hip_check(hipError_t e, const char* what);
+int launch_status(const char* what);
+
+// Brackets one launch with HIP events when the context profiles; `bytes` is
+// the launch's algorithmic byte count.
+struct Timed {
+    enbl_context* ctx;
+    size_t idx;
+    Timed(enbl_context* c, const char* name, double bytes);
+    ~Timed();
+};
+
+const char* pname(int t);
+size_t round_up(size_t v, size_t m);
+int mask_ptype_for(int itype);
+int image_ptype_for(int itype);
+void* level_ptr(const enbl_pyramid* p, int l);
+int gaussian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch);
+int laplacian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap);
+int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const void* img, long ipitch);
+int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels, int slot, enbl_pyramid** out);
+// Device scratch buffer `slot` of the context, grown to at least `bytes`.
+int scratch(enbl_context* ctx, int slot, size_t bytes, void** out);
+
+}  // namespace host
+}  // namespace enbl
+
 struct enbl_context {
     int device;
     hipStream_t own_stream;
@@ -78,6 +113,8 @@ struct enbl_context {
     std::vector<Cached> cache;   // pipeline scratch pyramids
     void* scratch_alpha;         // per-level alpha scratch for Gaussian builds
     size_t scratch_alpha_bytes;
+    void* scratch_buf[8];        // general device scratch (enfuse weights)
+    size_t scratch_bytes[8];
 };
 
 struct enbl_pyramid {

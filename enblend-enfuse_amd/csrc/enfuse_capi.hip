@@ -1,0 +1,296 @@
+// C ABI of enfuse's weighting and fusion (declared in include/enblend_amd.h).
+//
+// enbl_enfuse follows enfuseMain (src/enfuse.h:1217-1609) stage by stage but
+// keeps every intermediate on the device: the K weight images and the norm
+// image are built first (the reference's first pass over the inputs), then
+// each input's Laplacian pyramid and normalised mask Gaussian pyramid are
+// merged level by level into the result pyramid, which is collapsed and
+// converted once.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "enbl_enfuse.h"
+#include "enbl_internal.h"
+
+using namespace enbl;
+using namespace enbl::host;
+
+namespace {
+
+const double kFwhmGaussian = 2.3548200450309493820231386529193992755;
+const double kFwhm[] = {kFwhmGaussian, 2.8284271247461900976033774484193961571,
+                        2.0943951023931954923084289221863352561, M_PI,
+                        1.6817928305074290860622509524664297901};
+
+// Built-in ExposureWeight::weight (src/exposure_weight.h:44-120), evaluated
+// on the host for the integer-image tables.
+double exposure_host(int fn, double opt, double width, double y)
+{
+    const double z = (y - opt) / width;
+    switch (fn) {
+    case ENBL_EXPOSURE_GAUSSIAN: return exp(-0.5 * z * z);
+    case ENBL_EXPOSURE_LORENTZIAN: return 1.0 / (1.0 + 0.5 * z * z);
+    case ENBL_EXPOSURE_HALFSINE: return fabs(z) <= M_PI_2 ? cos(z) : 0.0;
+    case ENBL_EXPOSURE_FULLSINE: return fabs(z) <= M_PI ? 0.5 + cos(z) / 2.0 : 0.0;
+    default:
+        if (fabs(z) <= 1.0) { const double z2 = z * z; return (1.0 - z2) * (1.0 + z2); }
+        return 0.0;
+    }
+}
+
+int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
+{
+    if (!p) return fail(ENBL_E_ARG, "enfuse: null parameters");
+    if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "enfuse: image type %d", itype);
+    if (p->exposure_function < 0 || p->exposure_function > ENBL_EXPOSURE_BISQUARE)
+        return fail(ENBL_E_ARG, "enfuse: exposure function %d", p->exposure_function);
+    if (p->w_exposure > 0.0 && (p->exposure_optimum < 0.0 || p->exposure_optimum > 1.0))
+        return fail(ENBL_E_ARG, "y_optimum");       // ExposureWeight::check_invariant
+    if (p->w_exposure > 0.0 && !(p->exposure_width > 0.0))
+        return fail(ENBL_E_ARG, "width_parameter");
+    if (p->w_contrast > 0.0) {
+        const int n = p->contrast_window;
+        if (n < 3 || !(n & 1)) return fail(ENBL_E_ARG, "enfuse: contrast window must be odd and >= 3");
+        if (w < n || h < n) return fail(ENBL_E_PRECONDITION, "localStdDevIf(): window larger than image");
+    }
+    if (p->w_entropy > 0.0) {
+        const int n = p->entropy_window;
+        if (n < 3 || !(n & 1) || n * n > 121) return fail(ENBL_E_UNSUPPORTED, "enfuse: entropy window %d", n);
+        if (w < n || h < n) return fail(ENBL_E_PRECONDITION, "localEntropyIf(): window larger than image");
+    }
+    return ENBL_OK;
+}
+
+WeightArgs weight_args(const enbl_enfuse_params* p)
+{
+    WeightArgs a;
+    a.w_exposure = p->w_exposure;
+    a.w_saturation = p->w_saturation;
+    a.w_contrast = p->w_contrast;
+    a.w_entropy = p->w_entropy;
+    a.exposure_optimum = p->exposure_optimum;
+    a.exposure_function = p->exposure_function;
+    a.exposure_width = p->exposure_function == 0 ? p->exposure_width
+                                                 : p->exposure_width * kFwhmGaussian / kFwhm[p->exposure_function];
+    return a;
+}
+
+struct WeightScratch {
+    float* lut = nullptr;
+    void* gray = nullptr;
+    void* grad = nullptr;
+    double* tab = nullptr;
+    void* ent = nullptr;
+    long pitch = 0;   // elements, shared by gray/grad/ent
+};
+
+// Host tables and device scratch for one image size (ExposureFunctor table,
+// gray/gradient images, entropy tables).
+int prepare_weights(enbl_context* ctx, int itype, int w, int h, const enbl_enfuse_params* p, const WeightArgs& a,
+                    WeightScratch* ws)
+{
+    const long pitch = (long)round_up((size_t)w, 64);
+    ws->pitch = pitch;
+    const size_t npx = (size_t)pitch * h;
+    int rc;
+    if (p->w_exposure > 0.0 && itype != ENBL_F32) {
+        const int n = itype == ENBL_U8 ? 256 : 65536;
+        const double maxv = itype == ENBL_U8 ? 255.0 : 65535.0;
+        std::vector<float> lut((size_t)n);
+        for (int g = 0; g < n; ++g) {
+            const double wt = p->exposure_table ? p->exposure_table[g]
+                                                : exposure_host(a.exposure_function, a.exposure_optimum, a.exposure_width, (double)g / maxv);
+            lut[(size_t)g] = (float)(p->w_exposure * wt);
+        }
+        void* d;
+        rc = scratch(ctx, 0, (size_t)n * sizeof(float), &d);
+        if (rc) return rc;
+        rc = hip_check(hipMemcpyAsync(d, lut.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice, ctx->stream), "exposure table");
+        if (rc) return rc;
+        rc = hip_check(hipStreamSynchronize(ctx->stream), "exposure table");
+        if (rc) return rc;
+        ws->lut = (float*)d;
+    }
+    if (p->w_contrast > 0.0) {
+        rc = scratch(ctx, 1, npx * 4, &ws->gray);
+        if (!rc) rc = scratch(ctx, 2, npx * 4, &ws->grad);
+        if (rc) return rc;
+    }
+    if (p->w_entropy > 0.0) {
+        const int full = p->entropy_window * p->entropy_window;
+        std::vector<double> tab((size_t)(2 * (full + 1) + (full + 1) * (full + 1)), 0.0);
+        double* pent = tab.data();
+        double* plog = pent + (full + 1);
+        double* pgen = plog + (full + 1);
+        for (int i = 1; i <= full; ++i) {
+            const double pr = (double)i / (double)full;
+            plog[i] = log((double)i);
+            pent[i] = pr * log(pr);
+        }
+        for (int t = 1; t <= full; ++t)
+            for (int k = 1; k <= t; ++k) {
+                const double pr = k / (double)t;
+                pgen[t * (full + 1) + k] = pr * log(pr);
+            }
+        void* d;
+        rc = scratch(ctx, 3, tab.size() * sizeof(double), &d);
+        if (rc) return rc;
+        rc = hip_check(hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "entropy table");
+        if (!rc) rc = hip_check(hipStreamSynchronize(ctx->stream), "entropy table");
+        if (rc) return rc;
+        ws->tab = (double*)d;
+        rc = scratch(ctx, 4, npx * 4, &ws->ent);
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+// Weights of one image into mask; norm += mask; union alpha updated.
+int image_weights(enbl_context* ctx, int itype, int w, int h, const void* img, long ip, const uint8_t* alpha, long ap,
+                  const enbl_enfuse_params* p, const WeightArgs& a, const WeightScratch& ws,
+                  float* mask, long mp, float* norm, long np_, uint8_t* ua, long up)
+{
+    const hipStream_t s = ctx->stream;
+    int rc;
+    if (p->w_contrast > 0.0) {
+        {
+            Timed t(ctx, "gray", (double)w * h * (3.0 * ptype_size(itype) + 4.0));
+            rc = launch_gray(itype, img, ip, w, h, ws.gray, ws.pitch, s);
+            if (!rc) rc = launch_status("gray");
+            if (rc) return rc;
+        }
+        if (itype == ENBL_F32) {
+            rc = launch_fill(ws.grad, (size_t)ws.pitch * h * 4, s);
+            if (!rc) rc = launch_status("fill");
+            if (rc) return rc;
+        }
+        Timed t(ctx, "stddev", (double)w * h * 9.0);
+        rc = launch_stddev(itype, ws.gray, ws.pitch, alpha, ap, w, h, p->contrast_window, ws.grad, ws.pitch, s);
+        if (rc) return fail(rc, "localStdDevIf: unsupported window");
+        rc = launch_status("stddev");
+        if (rc) return rc;
+    }
+    if (p->w_entropy > 0.0) {
+        Timed t(ctx, "entropy", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + ptype_size(itype)));
+        rc = launch_entropy(itype, img, ip, alpha, ap, w, h, p->entropy_window, ws.tab, ws.ent, ws.pitch, s);
+        if (rc) return fail(rc, "localEntropyIf: unsupported window");
+        rc = launch_status("entropy");
+        if (rc) return rc;
+    }
+    Timed t(ctx, "weights", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + 4.0 + 8.0 + 1.0));
+    rc = launch_weights(itype, img, ip, alpha, ap, w, h, a, ws.lut, ws.grad, ws.pitch, ws.ent, ws.pitch,
+                        mask, mp, norm, np_, ua, up, s);
+    if (rc) return fail(rc, "enfuse weights: unsupported image type");
+    return launch_status("weights");
+}
+
+}  // namespace
+
+extern "C" {
+
+int enbl_enfuse_mask(enbl_context* ctx, int itype, int w, int h, const void* img, long ip,
+                     const uint8_t* alpha, long ap, const enbl_enfuse_params* p, float* mask, long mp)
+{
+    if (!ctx || !img || !alpha || !mask) return fail(ENBL_E_ARG, "enbl_enfuse_mask: null argument");
+    int rc = check_params(itype, w, h, p);
+    if (rc) return rc;
+    const WeightArgs a = weight_args(p);
+    WeightScratch ws;
+    rc = prepare_weights(ctx, itype, w, h, p, a, &ws);
+    if (rc) return rc;
+    // norm and union alpha are by-products here; park them in scratch.
+    void *norm, *ua;
+    const long pitch = ws.pitch;
+    rc = scratch(ctx, 5, (size_t)pitch * h * 4, &norm);
+    if (!rc) rc = scratch(ctx, 6, (size_t)pitch * h, &ua);
+    if (rc) return rc;
+    rc = launch_fill(norm, (size_t)pitch * h * 4, ctx->stream);
+    if (rc) return rc;
+    return image_weights(ctx, itype, w, h, img, ip, alpha, ap, p, a, ws, mask, mp, (float*)norm, pitch,
+                         (uint8_t*)ua, pitch);
+}
+
+int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, int levels, int wrap,
+                void* out, long op, uint8_t* oa, long oap, int dither, uint64_t seed)
+{
+    if (!ctx || !imgs || !ips || !alphas || !aps || !out || !oa) return fail(ENBL_E_ARG, "enbl_enfuse: null argument");
+    if (K < 1) return fail(ENBL_E_ARG, "enbl_enfuse: no images");
+    for (int m = 0; m < K; ++m)
+        if (!imgs[m] || !alphas[m]) return fail(ENBL_E_ARG, "enbl_enfuse: null image %d", m);
+    int rc = check_params(itype, w, h, p);
+    if (rc) return rc;
+    if (levels < 1 || levels > 29) return fail(ENBL_E_PRECONDITION, "filterHalfWidth: levels outside of permissible range");
+    const WeightArgs a = weight_args(p);
+    WeightScratch ws;
+    rc = prepare_weights(ctx, itype, w, h, p, a, &ws);
+    if (rc) return rc;
+    const hipStream_t s = ctx->stream;
+    const long pitch = ws.pitch;
+    const size_t plane = (size_t)pitch * h;
+    void *norm, *ua, *masks;
+    rc = scratch(ctx, 5, plane * 4, &norm);
+    if (!rc) rc = scratch(ctx, 6, plane, &ua);
+    if (!rc) rc = scratch(ctx, 7, plane * 4 * (size_t)K, &masks);
+    if (rc) return rc;
+    rc = launch_fill(norm, plane * 4, s);
+    if (!rc) rc = launch_fill(ua, plane, s);
+    if (rc) return rc;
+
+    // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388).
+    for (int m = 0; m < K; ++m) {
+        rc = image_weights(ctx, itype, w, h, imgs[m], ips[m], alphas[m], aps[m], p, a, ws,
+                           (float*)masks + plane * m, pitch, (float*)norm, pitch, (uint8_t*)ua, pitch);
+        if (rc) return rc;
+    }
+
+    // Pass 2: per input, Laplacian pyramid, normalised mask Gaussian pyramid
+    // over the union alpha, multiply and accumulate (src/enfuse.h:1497-1590).
+    const int pt = image_ptype_for(itype);
+    const int mt = mask_ptype_for(itype);
+    const int ifrac = itype == ENBL_F32 ? 0 : 7;
+    const int mfrac = itype == ENBL_U8 ? 7 : 15;
+    enbl_pyramid *lp = nullptr, *res = nullptr, *mgp = nullptr;
+    rc = cached_pyramid(ctx, pt, 3, w, h, levels, 0, &lp);
+    if (!rc) rc = cached_pyramid(ctx, pt, 3, w, h, levels, 1, &res);
+    if (!rc) rc = cached_pyramid(ctx, mt, 1, w, h, levels, 0, &mgp);
+    if (rc) return rc;
+    const float fallback = (float)(255 / K);
+    const double scale = (double)(1u << mfrac);
+    const double maxpyr = (double)(255 << mfrac);
+    for (int m = 0; m < K; ++m) {
+        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, imgs[m], ips[m], alphas[m], aps[m], lp);
+        if (rc) return rc;
+        {
+            const Level& L0 = mgp->lv[0];
+            Timed t(ctx, "normalize", (double)w * h * (8.0 + ptype_size(mt)));
+            rc = launch_normalize(mt, (const float*)masks + plane * m, pitch, (const float*)norm, pitch, w, h,
+                                  fallback, scale, level_ptr(mgp, 0), L0.pitch, s);
+            if (!rc) rc = launch_status("normalize");
+            if (rc) return rc;
+        }
+        rc = gaussian_levels(ctx, mgp, wrap, (const uint8_t*)ua, pitch);
+        if (rc) return rc;
+        for (int l = 0; l < levels; ++l) {
+            const Level& L = lp->lv[l];
+            const Level& G = mgp->lv[l];
+            const Level& R = res->lv[l];
+            Timed t(ctx, "merge", (double)L.w * L.h * ((m ? 3.0 : 2.0) * 3 * ptype_size(pt) + ptype_size(mt)));
+            rc = launch_merge(pt, mt, m == 0, level_ptr(lp, l), L.pitch, L.plane, level_ptr(mgp, l), G.pitch,
+                              L.w, L.h, maxpyr, level_ptr(res, l), R.pitch, R.plane, s);
+            if (!rc) rc = launch_status("merge");
+            if (rc) return rc;
+        }
+    }
+    rc = enbl_collapse_pyramid(ctx, wrap, res);
+    if (rc) return rc;
+    rc = enbl_copy_from_pyramid_if(ctx, res, 0, ifrac, (const uint8_t*)ua, pitch, out, itype, op, dither, seed);
+    if (rc) return rc;
+    return hip_check(hipMemcpy2DAsync(oa, (size_t)oap, ua, (size_t)pitch, (size_t)w, (size_t)h, hipMemcpyDeviceToDevice, s),
+                     "union alpha");
+}
+
+}  // extern "C"

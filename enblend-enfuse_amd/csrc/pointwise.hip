@@ -5,7 +5,10 @@
 #include <math.h>
 
 #include "enbl_device.h"
+#include "enbl_enfuse.h"
 #include "enbl_internal.h"
+
+#include <algorithm>
 
 namespace enbl {
 namespace {
@@ -137,6 +140,15 @@ __global__ __launch_bounds__(kTile) void blend_kernel(const M* __restrict__ mask
     }
 }
 
+// Zero fill (used instead of hipMemsetAsync so that captured graphs replay
+// correctly).
+__global__ __launch_bounds__(kTile) void fill_kernel(uint32_t* __restrict__ p, size_t words, uint8_t* tail, int ntail)
+{
+    const size_t i = (size_t)blockIdx.x * kTile + threadIdx.x;
+    if (i < words) p[i] = 0u;
+    if (i < (size_t)ntail) tail[i] = 0;
+}
+
 __global__ __launch_bounds__(kTile) void alpha_union_kernel(const uint8_t* __restrict__ s, long sp,
                                                             uint8_t* __restrict__ d, long dp, int w)
 {
@@ -222,6 +234,18 @@ int launch_blend(int mtype, int ptype, int C, int w, int h, const void* m, long 
     if (mtype == ENBL_I32 && ptype == ENBL_F64) { blend_t<int32_t, double>(C, w, h, m, mp, wh, wp, wl, bk, bp, bl, wv, s); return 0; }
     if (mtype == ENBL_I16 && ptype == ENBL_I32) { blend_t<int16_t, int32_t>(C, w, h, m, mp, wh, wp, wl, bk, bp, bl, wv, s); return 0; }
     return ENBL_E_UNSUPPORTED;
+}
+
+int launch_fill(void* p, size_t bytes, hipStream_t s)
+{
+    // p is at least 4-byte aligned (device allocations and plane offsets).
+    const size_t words = bytes / 4;
+    const int ntail = (int)(bytes % 4);
+    const size_t n = std::max(words, (size_t)ntail);
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kTile), 0, s, (uint32_t*)p, words,
+                       (uint8_t*)p + words * 4, ntail);
+    return 0;
 }
 
 int launch_alpha_union(const uint8_t* src, long sp, uint8_t* dst, long dp, int w, int h, hipStream_t s)

@@ -149,6 +149,52 @@ int enbl_copy_from_pyramid_if(enbl_context* ctx, const enbl_pyramid* p, int leve
                               const uint8_t* mask, long mpitch, void* img, int itype, long ipitch,
                               int dither_mode, uint64_t seed);
 
+/* ---- enfuse weights and fusion -------------------------------------- */
+
+/* built-in exposure weight functions (src/exposure_weight.h:44-120) */
+#define ENBL_EXPOSURE_GAUSSIAN 0
+#define ENBL_EXPOSURE_LORENTZIAN 1
+#define ENBL_EXPOSURE_HALFSINE 2
+#define ENBL_EXPOSURE_FULLSINE 3
+#define ENBL_EXPOSURE_BISQUARE 4
+
+/* enfuse weighting options (src/enfuse.cc:118-146; defaults exposure 1.0,
+ * optimum 0.5, width 0.2, gaussian, contrast window 5, entropy window 3). */
+typedef struct enbl_enfuse_params {
+    double w_exposure, w_saturation, w_contrast, w_entropy;
+    double exposure_optimum, exposure_width;
+    int exposure_function;        /* ENBL_EXPOSURE_* */
+    int contrast_window;          /* odd, >= 3 */
+    int entropy_window;           /* odd, >= 3, window area <= 121 */
+    int reserved;                 /* 0 */
+    /* Host table of ExposureWeight::weight(code / max) for every gray code
+     * of an integer image (256 or 65536 doubles), e.g. from a user plugin
+     * (src/exposure_weight_base.h:29-88).  NULL: the built-in function. */
+    const double* exposure_table;
+} enbl_enfuse_params;
+
+/* Replaces enfuseMask<ImageType, AlphaType, MaskType>(src, mask, result) --
+ * src/enfuse.h:962-1212 -- for an interleaved RGB device image: the float
+ * weight image (exposure, contrast, saturation, entropy, accumulated in that
+ * order where alpha != 0, zero elsewhere). */
+int enbl_enfuse_mask(enbl_context* ctx, int itype, int w, int h, const void* img, long ipitch,
+                     const uint8_t* alpha, long apitch, const enbl_enfuse_params* prm,
+                     float* mask, long mpitch);
+
+/* Replaces the fusion of enfuseMain<ImagePixelType> -- src/enfuse.h:1217-1609,
+ * soft masks, identity colour space -- for K interleaved RGB device images of
+ * one size: weights, union alpha, normalisation, Laplacian pyramids, mask
+ * Gaussian pyramids over the union alpha, multiply, accumulate, collapse and
+ * the dithered masked copy.  imgs/alphas/pitches are host arrays of K
+ * entries.  `out` receives the fused image where the union alpha is set
+ * (other pixels untouched), `out_alpha` the union alpha. */
+int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K,
+                const void* const* imgs, const long* ipitches,
+                const uint8_t* const* alphas, const long* apitches,
+                const enbl_enfuse_params* prm, int levels, int wraparound,
+                void* out, long opitch, uint8_t* out_alpha, long oapitch,
+                int dither_mode, uint64_t seed);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510

@@ -92,6 +92,36 @@ int orc_pyramid(int laplacian, int itype, int channels, int w, int h, int levels
 /* Collapse a packed planar pyramid in place (src/pyramid.h:1757-1784). */
 int orc_collapse(int ptype, int channels, int w, int h, int levels, int wrap, void* pyr);
 
+/* enfuse weighting parameters (src/enfuse.cc:118-146 defaults). */
+enum { ORC_EXP_GAUSSIAN = 0, ORC_EXP_LORENTZIAN = 1, ORC_EXP_HALFSINE = 2, ORC_EXP_FULLSINE = 3,
+       ORC_EXP_BISQUARE = 4 };
+typedef struct orc_enfuse_params {
+    double w_exposure, w_saturation, w_contrast, w_entropy;
+    double exposure_optimum, exposure_width;
+    int exposure_function;          /* ORC_EXP_* built-in */
+    int contrast_window;            /* --contrast-window-size, default 5 */
+    int entropy_window;             /* --entropy-window-size, default 3 */
+    int pad_;
+    const double* exposure_table;   /* optional: weight(code / max) for every
+                                       gray code of an integer image (user
+                                       ExposureWeight); NULL = built-in */
+} orc_enfuse_params;
+
+/* enfuseMask (src/enfuse.h:962-1212) for one RGB image with alpha: the float
+ * weight image, written where alpha != 0 (zero elsewhere). */
+int orc_enfuse_mask(int itype, const void* img, const uint8_t* alpha, int w, int h,
+                    const orc_enfuse_params* prm, float* mask);
+
+/* enfuseMain's fusion of K RGB images of one size (src/enfuse.h:1217-1609),
+ * soft masks, identity colour space: weights, normalisation, Laplacian
+ * pyramids, mask Gaussian pyramids over the union alpha, multiply,
+ * accumulate, collapse, masked conversion.  `out` (w*h*3) and `out_alpha`
+ * (w*h) are written; pixels outside the union alpha are left unchanged. */
+int orc_enfuse(int itype, int w, int h, int K, const void* const* imgs,
+               const uint8_t* const* alphas, const orc_enfuse_params* prm,
+               int levels, int wrap, void* out, uint8_t* out_alpha,
+               int dither_mode, uint64_t seed);
+
 /* Number of OpenMP threads the pointwise stages use. */
 int orc_num_threads(void);
 

@@ -1,0 +1,388 @@
+// CPU restatement of enfuse's weighting and fusion (TEST INFRASTRUCTURE
+// ONLY: checker of the HIP path and bench.py's CPU baseline).
+//
+// Follows src/enfuse.h:64-1212 (weights) and :1217-1609 (enfuseMain) of the
+// reference, with the default "average" grayscale projector (src/mga.h:188-193),
+// soft masks and the identity colour space.  Parallel structure as in the
+// reference: pointwise stages and the local standard deviation run over rows
+// with OpenMP, the local entropy is one sequential column sweep.
+#include <float.h>
+
+#include <map>
+
+#include "oracle_core.h"
+
+using namespace orc;
+
+namespace {
+
+// ---- grayscale "average" projector, src/mga.h:188-193 ---------------------
+// Integer channels promote to double; float channels promote to float
+// (RealPromote<float> = float), so their sum is a float sum.
+template <class T, class R> struct Gray;
+template <class R> struct Gray<uint8_t, R> {
+    static R get(const uint8_t* p) { return fromRealPromote<R>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
+};
+template <class R> struct Gray<uint16_t, R> {
+    static R get(const uint16_t* p) { return fromRealPromote<R>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
+};
+template <class R> struct Gray<float, R> {
+    static R get(const float* p) { return (R)((double)(p[0] + p[1] + p[2]) / 3.0); }
+};
+
+template <class T> struct Max;
+template <> struct Max<uint8_t> { static double v() { return 255.0; } };
+template <> struct Max<uint16_t> { static double v() { return 65535.0; } };
+template <> struct Max<float> { static double v() { return (double)FLT_MAX; } };
+
+// ---- exposure weight functions, src/exposure_weight.h:44-120 -------------
+const double kFwhmGaussian = 2.3548200450309493820231386529193992755;
+
+struct Exposure {
+    int fn;
+    double opt, width;
+    Exposure(int f, double o, double w) : fn(f), opt(o), width(w) {
+        static const double fwhm[] = {kFwhmGaussian, 2.8284271247461900976033774484193961571,
+                                      2.0943951023931954923084289221863352561, M_PI,
+                                      1.6817928305074290860622509524664297901};
+        if (fn > 0) width = w * kFwhmGaussian / fwhm[fn];
+    }
+    double weight(double y) const {
+        const double z = (y - opt) / width;
+        switch (fn) {
+        case ORC_EXP_GAUSSIAN: return exp(-0.5 * z * z);
+        case ORC_EXP_LORENTZIAN: return 1.0 / (1.0 + 0.5 * z * z);
+        case ORC_EXP_HALFSINE: return fabs(z) <= M_PI_2 ? cos(z) : 0.0;
+        case ORC_EXP_FULLSINE: return fabs(z) <= M_PI ? 0.5 + cos(z) / 2.0 : 0.0;
+        default:
+            if (fabs(z) <= 1.0) { const double z2 = z * z; return (1.0 - z2) * (1.0 + z2); }
+            return 0.0;
+        }
+    }
+};
+
+// y = gray / max: double for integer images, float for float images
+// (src/enfuse.h:637).
+template <class T> double exposure_y(T gray);
+template <> double exposure_y<uint8_t>(uint8_t g) { return (double)g / 255.0; }
+template <> double exposure_y<uint16_t>(uint16_t g) { return (double)g / 65535.0; }
+template <> double exposure_y<float>(float g) { return (double)(g / FLT_MAX); }
+
+template <class T> struct GrayIndex { static int of(T g) { return (int)g; } };
+
+// ---- local standard deviation, src/enfuse.h:75-219 -----------------------
+// S = RealPromote of the gray type (double for int, float for float).
+template <class G, class S>
+void local_stddev(const G* gray, const uint8_t* mask, int w, int h, int n, G* out)
+{
+    const int b = n / 2;
+#pragma omp parallel for schedule(static)
+    for (int row = 0; row < h - 2 * b; ++row) {
+        const int yc = row + b;
+        std::vector<S> cs(w + 1), cq(w + 1);
+        std::vector<size_t> cn(w + 1);
+        S sum = S(0), sumSqr = S(0);
+        size_t cnt = 0;
+        auto column = [&](int x, S& s, S& q, size_t& k) {
+            s = S(0); q = S(0); k = 0;
+            for (int y = yc - b; y <= yc + b; ++y) {
+                if (mask[(long)y * w + x]) {
+                    const S v = (S)gray[(long)y * w + x];
+                    s += v;
+                    q += v * v;
+                    ++k;
+                }
+            }
+        };
+        for (int x = 0; x <= 2 * b; ++x) {
+            column(x, cs[x], cq[x], cn[x]);
+            sum += cs[x];
+            sumSqr += cq[x];
+            cnt += cn[x];
+        }
+        for (int xc = b;; ++xc) {
+            if (mask[(long)yc * w + xc]) {
+                const S r = cnt <= 1 ? S(0) : (S)sqrt((sumSqr - sum * sum / (S)cnt) / (S)(cnt - 1));
+                out[(long)yc * w + xc] = fromRealPromote<G>((double)r);
+            }
+            if (xc == w - b - 1) break;
+            const int xn = xc + b + 1;
+            S s, q;
+            size_t k;
+            column(xn, s, q, k);
+            cs[xn] = s; cq[xn] = q; cn[xn] = k;
+            sum += s - cs[xn - n];
+            sumSqr += q - cq[xn - n];
+            cnt += k - cn[xn - n];
+        }
+    }
+}
+
+// ---- local entropy, src/enfuse.h:222-566 ---------------------------------
+// Exact-value histograms per channel; the entropy sums p*log(p) over the bins
+// in ascending key order, using precomputed tables when the window is full.
+template <class T>
+struct Hist {
+    std::map<T, unsigned> h[3];
+    unsigned total[3] = {0, 0, 0};
+    void clear() { for (int c = 0; c < 3; ++c) { h[c].clear(); total[c] = 0; } }
+    void insert(int c, T k, unsigned n) {
+        auto it = h[c].lower_bound(k);
+        if (it != h[c].end() && !(k < it->first)) it->second += n;
+        else h[c].insert(it, std::make_pair(k, n));
+        total[c] += n;
+    }
+    void erase(int c, T k, unsigned n) {
+        auto it = h[c].find(k);
+        if (it->second > n) it->second -= n;
+        else h[c].erase(it);
+        total[c] -= n;
+    }
+    void insert(const Hist& o) { for (int c = 0; c < 3; ++c) for (auto& kv : o.h[c]) insert(c, kv.first, kv.second); }
+    void erase(const Hist& o) { for (int c = 0; c < 3; ++c) for (auto& kv : o.h[c]) erase(c, kv.first, kv.second); }
+};
+
+template <class T>
+void local_entropy(const T* img, const uint8_t* mask, int w, int h, int n, T* out)
+{
+    const size_t full = (size_t)n * n;
+    std::vector<double> plog(full + 1), pent(full + 1);
+    plog[0] = 0.0; pent[0] = 0.0;
+    for (size_t i = 1; i <= full; ++i) {
+        const double p = (double)i / (double)full;
+        plog[i] = log((double)i);
+        pent[i] = p * log(p);
+    }
+    const double maxv = Max<T>::v();
+    auto entropy_of = [&](const Hist<T>& H, int c) -> double {
+        const unsigned total = H.total[c];
+        const size_t bins = H.h[c].size();
+        if (total == 0 || bins <= 1) return 0.0;
+        double e = 0.0;
+        if (total == full) {
+            for (auto& kv : H.h[c]) e += pent[kv.second];
+            return -e / plog[bins];
+        }
+        for (auto& kv : H.h[c]) {
+            const double p = kv.second / (double)total;
+            e += p * log(p);
+        }
+        return -e / log((double)bins);
+    };
+    const int b = n / 2;
+    std::vector<Hist<T>> pad(h + 1);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < n; ++x)
+            if (mask[(long)y * w + x])
+                for (int c = 0; c < 3; ++c) pad[y].insert(c, img[((long)y * w + x) * 3 + c], 1);
+    Hist<T> hist;
+    for (int xc = b; xc < w - b; ++xc) {
+        hist.clear();
+        for (int y = 0; y <= 2 * b; ++y) hist.insert(pad[y]);
+        for (int yc = b; yc < h - b; ++yc) {
+            if (mask[(long)yc * w + xc])
+                for (int c = 0; c < 3; ++c) out[((long)yc * w + xc) * 3 + c] = fromRealPromote<T>(entropy_of(hist, c) * maxv);
+            hist.erase(pad[yc - b]);
+            hist.insert(pad[yc + b + 1]);
+        }
+        for (int y = 0; y < h; ++y) {
+            if (mask[(long)y * w + xc - b])
+                for (int c = 0; c < 3; ++c) pad[y].erase(c, img[((long)y * w + xc - b) * 3 + c], 1);
+            if (xc + b + 1 < w && mask[(long)y * w + xc + b + 1])
+                for (int c = 0; c < 3; ++c) pad[y].insert(c, img[((long)y * w + xc + b + 1) * 3 + c], 1);
+        }
+    }
+}
+
+template <class T> struct Contrast;   // gray type used by the contrast weight
+template <> struct Contrast<uint8_t> { typedef int32_t G; typedef double S; };
+template <> struct Contrast<uint16_t> { typedef int32_t G; typedef double S; };
+template <> struct Contrast<float> { typedef float G; typedef float S; };
+
+template <class T> float contrast_weight(double wc, typename Contrast<T>::G g);
+template <> float contrast_weight<uint8_t>(double wc, int32_t g) { return (float)(wc * (double)g / 255.0); }
+template <> float contrast_weight<uint16_t>(double wc, int32_t g) { return (float)(wc * (double)g / 65535.0); }
+template <> float contrast_weight<float>(double wc, float g) { return (float)(wc * (double)g); }
+
+template <class T> bool is_integral() { return true; }
+template <> bool is_integral<float>() { return false; }
+
+// enfuseMask, src/enfuse.h:962-1212.
+template <class T>
+int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_enfuse_params* p, float* mask)
+{
+    const long N = (long)w * h;
+    for (long i = 0; i < N; ++i) mask[i] = 0.0f;
+
+    if (p->w_exposure > 0.0) {
+        const Exposure ex(p->exposure_function, p->exposure_optimum, p->exposure_width);
+#pragma omp parallel for schedule(guided)
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const long i = (long)y * w + x;
+                if (!alpha[i]) continue;
+                const T g = Gray<T, T>::get(img + 3 * i);
+                double wt;
+                if (p->exposure_table && is_integral<T>()) wt = p->exposure_table[GrayIndex<T>::of(g)];
+                else wt = ex.weight(exposure_y<T>(g));
+                mask[i] = (float)(p->w_exposure * wt);
+            }
+    }
+
+    if (p->w_contrast > 0.0) {
+        typedef typename Contrast<T>::G G;
+        typedef typename Contrast<T>::S S;
+        const int n = p->contrast_window;
+        if (n < 2 || w < n || h < n) return -1;
+        std::vector<G> gray((size_t)N), grad((size_t)N, G(0));
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) gray[i] = Gray<T, G>::get(img + 3 * i);
+        local_stddev<G, S>(gray.data(), alpha, w, h, n, grad.data());
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i)
+            if (alpha[i]) mask[i] = contrast_weight<T>(p->w_contrast, grad[i]) + mask[i];
+    }
+
+    if (p->w_saturation > 0.0) {
+        const double max_value = is_integral<T>() ? Max<T>::v() : 1.0;
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) {
+            if (!alpha[i]) continue;
+            const T* q = img + 3 * i;
+            const T mx = std::max(q[0], std::max(q[1], q[2]));
+            const T mn = std::min(q[0], std::min(q[1], q[2]));
+            float s = 0.0f;
+            if (!(mx == mn)) {
+                const double sum = (double)mx + (double)mn;
+                const double diff = (double)mx - (double)mn;
+                const double sat = sum <= max_value ? diff / sum : diff / (2.0 * max_value - sum);
+                s = (float)(p->w_saturation * sat);
+            }
+            mask[i] = s + mask[i];
+        }
+    }
+
+    if (p->w_entropy > 0.0) {
+        const int n = p->entropy_window;
+        if (w < n || h < n) return -1;
+        std::vector<T> ent((size_t)N * 3, T(0));
+        local_entropy<T>(img, alpha, w, h, n, ent.data());
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) {
+            if (!alpha[i]) continue;
+            const T mn = std::min(std::min(ent[3 * i], ent[3 * i + 1]), ent[3 * i + 2]);
+            mask[i] = (float)(p->w_entropy * (double)(float)mn) + mask[i];
+        }
+    }
+    return 0;
+}
+
+template <class P> int ptype_of();
+template <> int ptype_of<int16_t>() { return ORC_I16; }
+template <> int ptype_of<int32_t>() { return ORC_I32; }
+template <> int ptype_of<double>() { return ORC_F64; }
+
+// enfuseMain, src/enfuse.h:1217-1609 (soft masks).
+template <class T, class P, class M>
+int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* alphas,
+             const orc_enfuse_params* prm, int levels, bool wrap, int img_frac, int mask_frac,
+             T* out, uint8_t* out_alpha, int dither, uint64_t seed)
+{
+    const long N = (long)w * h;
+    std::vector<std::vector<float>> masks(K, std::vector<float>((size_t)N));
+    std::vector<float> norm((size_t)N, 0.0f);
+    std::vector<uint8_t> ua((size_t)N, 0);
+    for (int m = 0; m < K; ++m) {
+        int rc = enfuse_mask_t<T>(imgs[m], alphas[m], w, h, prm, masks[m].data());
+        if (rc) return rc;
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) {
+            if (alphas[m][i]) ua[i] = alphas[m][i];
+            norm[i] = masks[m][i] + norm[i];
+        }
+    }
+    long nimg = 0, nmask = 0;
+    std::vector<Dims> di = pyramid_dims(w, h, levels, 3, &nimg);
+    std::vector<Dims> dm = pyramid_dims(w, h, levels, 1, &nmask);
+    std::vector<P> result;
+    const float fallback = (float)(255 / K);
+    const double max_mask_pyr = (double)((M)(255 << mask_frac));
+    const double scale = (double)(1u << mask_frac);
+    for (int m = 0; m < K; ++m) {
+        std::vector<P> lp((size_t)nimg);
+        image_to_level0(img_frac, 3, imgs[m], w, h, lp.data());
+        int rc = gaussian_levels<P>(lp.data(), di, 3, wrap, alphas[m]);
+        if (rc) return rc;
+        laplacian_levels<P>(lp.data(), di, 3, wrap);
+
+        // Normalise and convert to fixed point (src/enfuse.h:1529-1546,
+        // src/fixmath.h:188-194).
+        std::vector<M> mgp((size_t)nmask);
+        const std::vector<float>& mk = masks[m];
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) {
+            const float nm = norm[i] > 0.0 ? (255.0f * mk[i]) / norm[i] : fallback;
+            mgp[i] = fromRealPromote<M>((double)nm * scale);
+        }
+        rc = gaussian_levels<M>(mgp.data(), dm, 1, wrap, ua.data());
+        if (rc) return rc;
+
+        // ImageMaskMultiplyFunctor (src/enfuse.h:601-620) and accumulation.
+        for (int l = 0; l < levels; ++l) {
+            const long n = (long)di[l].w * di[l].h;
+            P* L = lp.data() + di[l].off;
+            const M* G = mgp.data() + dm[l].off;
+#pragma omp parallel for schedule(guided)
+            for (long i = 0; i < n; ++i) {
+                const double coeff = (double)G[i] / max_mask_pyr;
+                for (int c = 0; c < 3; ++c) L[c * n + i] = fromRealPromote<P>((double)L[c * n + i] * coeff);
+            }
+        }
+        if (m == 0) {
+            result.swap(lp);
+        } else {
+#pragma omp parallel for schedule(guided)
+            for (long i = 0; i < nimg; ++i) result[i] = Store<P>::wrap(Store<P>::A::add((typename Store<P>::A::T)lp[i], (typename Store<P>::A::T)result[i]));
+        }
+    }
+    collapse_levels<P>(result.data(), di, 3, wrap);
+    level0_to_image(img_frac, 3, result.data(), w, h, ua.data(), out, dither, seed);
+    memcpy(out_alpha, ua.data(), (size_t)N);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_enfuse_mask(int itype, const void* img, const uint8_t* alpha, int w, int h,
+                    const orc_enfuse_params* prm, float* mask)
+{
+    switch (itype) {
+    case ORC_U8: return enfuse_mask_t<uint8_t>((const uint8_t*)img, alpha, w, h, prm, mask);
+    case ORC_U16: return enfuse_mask_t<uint16_t>((const uint16_t*)img, alpha, w, h, prm, mask);
+    case ORC_F32: return enfuse_mask_t<float>((const float*)img, alpha, w, h, prm, mask);
+    }
+    return -4;
+}
+
+int orc_enfuse(int itype, int w, int h, int K, const void* const* imgs, const uint8_t* const* alphas,
+               const orc_enfuse_params* prm, int levels, int wrap, void* out, uint8_t* out_alpha,
+               int dither, uint64_t seed)
+{
+    if (K < 1 || levels < 1 || levels > 29) return -1;
+    switch (itype) {
+    case ORC_U8:
+        return enfuse_t<uint8_t, int16_t, int16_t>(w, h, K, (const uint8_t* const*)imgs, alphas, prm, levels, wrap, 7, 7,
+                                                   (uint8_t*)out, out_alpha, dither, seed);
+    case ORC_U16:
+        return enfuse_t<uint16_t, int32_t, int32_t>(w, h, K, (const uint16_t* const*)imgs, alphas, prm, levels, wrap, 7, 15,
+                                                    (uint16_t*)out, out_alpha, dither, seed);
+    case ORC_F32:
+        return enfuse_t<float, double, int32_t>(w, h, K, (const float* const*)imgs, alphas, prm, levels, wrap, 0, 15,
+                                                (float*)out, out_alpha, dither, seed);
+    }
+    return -4;
+}
+
+}  // extern "C"

@@ -28,7 +28,33 @@ _SIGS = {
     "orc_pyramid": [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, ctypes.POINTER(_l)],
     "orc_collapse": [_i, _i, _i, _i, _i, _i, _vp],
     "orc_num_threads": [],
+    "orc_enfuse_mask": [_i, _vp, _vp, _i, _i, _vp, _vp],
+    "orc_enfuse": [_i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _i, ctypes.c_uint64],
 }
+
+EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE = range(5)
+
+
+class EnfuseParams(ctypes.Structure):
+    """orc_enfuse_params / enbl_enfuse_params (same layout)."""
+    _fields_ = [("w_exposure", ctypes.c_double), ("w_saturation", ctypes.c_double),
+                ("w_contrast", ctypes.c_double), ("w_entropy", ctypes.c_double),
+                ("exposure_optimum", ctypes.c_double), ("exposure_width", ctypes.c_double),
+                ("exposure_function", ctypes.c_int), ("contrast_window", ctypes.c_int),
+                ("entropy_window", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("exposure_table", ctypes.c_void_p)]
+
+
+def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
+                  exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXP_GAUSSIAN,
+                  contrast_window=5, entropy_window=3, exposure_table=None):
+    p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
+                     exposure_function, contrast_window, entropy_window, 0, None)
+    if exposure_table is not None:
+        tab = np.ascontiguousarray(exposure_table, np.float64)
+        p.exposure_table = tab.ctypes.data
+        p._keep = tab
+    return p
 
 
 def _p(a):
@@ -153,6 +179,37 @@ class Oracle:
         if rc:
             raise ValueError(f"orc_enblend_pair rc={rc}")
         return b, ba
+
+
+def _enfuse_mask(self, img, alpha, itype, prm):
+    h, w, _ = img.shape
+    out = np.zeros((h, w), np.float32)
+    img = np.ascontiguousarray(img, NP_I[itype])
+    alpha = np.ascontiguousarray(alpha, np.uint8)
+    rc = self.lib.orc_enfuse_mask(itype, _p(img), _p(alpha), w, h, ctypes.byref(prm), _p(out))
+    if rc:
+        raise ValueError(f"orc_enfuse_mask rc={rc}")
+    return out
+
+
+def _enfuse(self, imgs, alphas, itype, prm, levels, wrap=False, dither=1, seed=0):
+    h, w, _ = imgs[0].shape
+    K = len(imgs)
+    imgs = [np.ascontiguousarray(i, NP_I[itype]) for i in imgs]
+    alphas = [np.ascontiguousarray(a, np.uint8) for a in alphas]
+    ip = (ctypes.c_void_p * K)(*[i.ctypes.data for i in imgs])
+    ap = (ctypes.c_void_p * K)(*[a.ctypes.data for a in alphas])
+    out = np.zeros((h, w, 3), NP_I[itype])
+    oa = np.zeros((h, w), np.uint8)
+    rc = self.lib.orc_enfuse(itype, w, h, K, ip, ap, ctypes.byref(prm), levels, int(wrap), _p(out), _p(oa),
+                             dither, seed)
+    if rc:
+        raise ValueError(f"orc_enfuse rc={rc}")
+    return out, oa
+
+
+Oracle.enfuse_mask = _enfuse_mask
+Oracle.enfuse = _enfuse
 
 
 def build_oracle():

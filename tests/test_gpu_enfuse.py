@@ -1,0 +1,90 @@
+"""GPU parity of enfuse weights and fusion against the oracle (C ABI)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle_binding import F32, U8, U16, enfuse_params as orc_params
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stack(rng, itype, K, h, w, scene=True):
+    """K exposures of one smooth scene (so weights vary smoothly)."""
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.5 + 0.4 * np.sin(xx / 7.0) * np.cos(yy / 5.0)
+    tint = rng.uniform(0.7, 1.3, size=3)
+    imgs = []
+    for k in range(K):
+        ev = (k - (K - 1) / 2) * 1.0
+        rad = base[..., None] * tint[None, None, :] * 2.0 ** ev
+        if itype == F32:
+            imgs.append((rad + rng.normal(0, 1e-3, rad.shape)).astype(np.float32))
+        else:
+            m = 255 if itype == U8 else 65535
+            v = np.clip(rad, 0, 1) ** (1 / 2.2) * m + rng.normal(0, 1, rad.shape)
+            imgs.append(np.clip(np.rint(v), 0, m).astype(np.uint8 if itype == U8 else np.uint16))
+    return imgs
+
+
+CASES = [
+    dict(w_exposure=1.0, w_saturation=0.2),
+    dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3, contrast_window=5, entropy_window=3),
+    dict(w_exposure=0.5, w_contrast=1.0, w_entropy=0.3, contrast_window=3, entropy_window=7, exposure_function=2),
+]
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_enfuse_mask_matches_oracle(enbl, gpu_ctx, oracle, itype, case):
+    rng = np.random.default_rng(700 + itype + case)
+    h, w = 61, 83
+    img = _stack(rng, itype, 1, h, w)[0]
+    a = np.where(rng.random((h, w)) < 0.9, 255, 0).astype(np.uint8)
+    kw = CASES[case]
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(img).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"enfuse_mask itype={itype} case={case}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    got = dm.cpu().numpy()
+    if itype == F32:
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+    else:
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, case):
+    rng = np.random.default_rng(800 + itype + case)
+    for (h, w, K) in [(64, 96, 3), (133, 101, 4)]:
+        imgs = _stack(rng, itype, K, h, w)
+        alphas = [np.where(rng.random((h, w)) < 0.95, 255, 0).astype(np.uint8) for _ in range(K)]
+        L = oracle.auto_levels(w, h)
+        kw = CASES[case]
+        print(f"enfuse itype={itype} case={case} {w}x{h} K={K} L={L}", flush=True)
+        want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), L, dither=1, seed=3)
+        dimgs = [torch.from_numpy(i).cuda() for i in imgs]
+        das = [torch.from_numpy(a).cuda() for a in alphas]
+        out = torch.zeros_like(dimgs[0])
+        oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        enbl.enfuse(gpu_ctx, itype, dimgs, das, enbl.enfuse_params(**kw), L, out, oa, dither=1, seed=3)
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
+        got = out.cpu().numpy()
+        if itype == F32:
+            sel = want_a != 0
+            np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
+        else:
+            np.testing.assert_array_equal(got, want)

@@ -1,0 +1,208 @@
+"""Pins the oracle's enfuse weights and fusion (src/enfuse.h) by direct
+per-pixel recomputation from the formulas, exact-integer arithmetic where
+the reference's arithmetic is exact, and known answers."""
+import math
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from oracle_binding import (EXP_BISQUARE, EXP_FULLSINE, EXP_GAUSSIAN, EXP_HALFSINE, EXP_LORENTZIAN, F32, U8, U16,
+                            enfuse_params)
+
+FWHM = {EXP_GAUSSIAN: 2.3548200450309493820231386529193992755, EXP_LORENTZIAN: 2.8284271247461900976033774484193961571,
+        EXP_HALFSINE: 2.0943951023931954923084289221863352561, EXP_FULLSINE: math.pi,
+        EXP_BISQUARE: 1.6817928305074290860622509524664297901}
+MAXV = {U8: 255.0, U16: 65535.0}
+
+
+def round_half_away(v):
+    return math.floor(v + 0.5) if v >= 0 else math.ceil(v - 0.5)
+
+
+def gray_int(px, itype):
+    v = (float(px[0]) + float(px[1]) + float(px[2])) / 3.0
+    return int(min(MAXV[itype], max(0.0, round_half_away(v))))
+
+
+def weight_fn(fn, opt, width, y):
+    width = width * FWHM[EXP_GAUSSIAN] / FWHM[fn] if fn else width
+    z = (y - opt) / width
+    if fn == EXP_GAUSSIAN:
+        return math.exp(-0.5 * z * z)
+    if fn == EXP_LORENTZIAN:
+        return 1.0 / (1.0 + 0.5 * z * z)
+    if fn == EXP_HALFSINE:
+        return math.cos(z) if abs(z) <= math.pi / 2 else 0.0
+    if fn == EXP_FULLSINE:
+        return 0.5 + math.cos(z) / 2.0 if abs(z) <= math.pi else 0.0
+    if abs(z) <= 1.0:
+        return (1.0 - z * z) * (1.0 + z * z)
+    return 0.0
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+def direct_mask(img, alpha, itype, w_exp=0.0, w_sat=0.0, w_con=0.0, w_ent=0.0, fn=EXP_GAUSSIAN, opt=0.5, width=0.2,
+                cwin=5, ewin=3):
+    """Per-pixel restatement of enfuseMask for integer RGB images."""
+    h, w, _ = img.shape
+    maxv = MAXV[itype]
+    out = np.zeros((h, w), np.float32)
+    pl = [math.log(i) if i else 0.0 for i in range(ewin * ewin + 1)]
+    pe = [(i / (ewin * ewin)) * math.log(i / (ewin * ewin)) if i else 0.0 for i in range(ewin * ewin + 1)]
+    for y in range(h):
+        for x in range(w):
+            if not alpha[y, x]:
+                continue
+            m = np.float32(0.0)
+            px = img[y, x]
+            if w_exp > 0:
+                g = gray_int(px, itype)
+                m = np.float32(w_exp * weight_fn(fn, opt, width, g / maxv))
+            if w_con > 0:
+                b = cwin // 2
+                sd = 0
+                if b <= y < h - b and b <= x < w - b:
+                    vals = [gray_int(img[yy, xx], itype) for yy in range(y - b, y + b + 1)
+                            for xx in range(x - b, x + b + 1) if alpha[yy, xx]]
+                    n = len(vals)
+                    if n > 1:
+                        s, q = float(sum(vals)), float(sum(v * v for v in vals))
+                        sd = round_half_away(math.sqrt((q - s * s / n) / (n - 1)))
+                m = np.float32(f32(w_con * sd / maxv) + m)
+            if w_sat > 0:
+                mx, mn = max(int(c) for c in px), min(int(c) for c in px)
+                sat = 0.0
+                if mx != mn:
+                    sm, df = float(mx + mn), float(mx - mn)
+                    sat = f32(w_sat * (df / sm if sm <= maxv else df / (2.0 * maxv - sm)))
+                m = np.float32(np.float32(sat) + m)
+            if w_ent > 0:
+                b = ewin // 2
+                ents = [0, 0, 0]
+                if b <= y < h - b and b <= x < w - b:
+                    for c in range(3):
+                        vals = [int(img[yy, xx, c]) for yy in range(y - b, y + b + 1)
+                                for xx in range(x - b, x + b + 1) if alpha[yy, xx]]
+                        cnt = Counter(vals)
+                        total, bins = len(vals), len(cnt)
+                        e = 0.0
+                        if total and bins > 1:
+                            if total == ewin * ewin:
+                                for k in sorted(cnt):
+                                    e += pe[cnt[k]]
+                                e = -e / pl[bins]
+                            else:
+                                for k in sorted(cnt):
+                                    p = cnt[k] / float(total)
+                                    e += p * math.log(p)
+                                e = -e / math.log(bins)
+                        ents[c] = int(min(maxv, max(0, round_half_away(e * maxv))))
+                m = np.float32(f32(w_ent * float(np.float32(min(ents)))) + m)
+            out[y, x] = m
+    return out
+
+
+def _img(rng, itype, h, w, levels=None):
+    hi = 255 if itype == U8 else 65535
+    dt = np.uint8 if itype == U8 else np.uint16
+    if levels:   # few distinct values so histograms have repeated keys
+        return rng.choice(np.linspace(0, hi, levels).astype(dt), size=(h, w, 3))
+    return rng.integers(0, hi + 1, size=(h, w, 3)).astype(dt)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("fn", [EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE])
+def test_exposure_weight(oracle, itype, fn):
+    rng = np.random.default_rng(fn + itype)
+    img = _img(rng, itype, 11, 13)
+    a = np.where(rng.random((11, 13)) < 0.8, 255, 0).astype(np.uint8)
+    prm = enfuse_params(w_exposure=0.7, exposure_function=fn, exposure_optimum=0.4, exposure_width=0.25)
+    got = oracle.enfuse_mask(img, a, itype, prm)
+    want = direct_mask(img, a, itype, w_exp=0.7, fn=fn, opt=0.4, width=0.25)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_exposure_table_matches_builtin(oracle, itype):
+    """A tabulated ExposureWeight (the user-plugin path) gives the built-in's
+    weights bit for bit when the table holds weight(code / max)."""
+    rng = np.random.default_rng(3)
+    img = _img(rng, itype, 9, 10)
+    a = np.full((9, 10), 255, np.uint8)
+    maxv = MAXV[itype]
+    tab = np.array([weight_fn(EXP_GAUSSIAN, 0.5, 0.2, g / maxv) for g in range(int(maxv) + 1)])
+    got = oracle.enfuse_mask(img, a, itype, enfuse_params(exposure_table=tab))
+    want = oracle.enfuse_mask(img, a, itype, enfuse_params())
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_all_weights(oracle, itype):
+    rng = np.random.default_rng(11 + itype)
+    for (h, w, cwin, ewin, lv) in [(12, 14, 5, 3, None), (13, 11, 3, 5, 4), (10, 16, 5, 7, 6)]:
+        img = _img(rng, itype, h, w, lv)
+        a = np.where(rng.random((h, w)) < 0.85, 255, 0).astype(np.uint8)
+        kw = dict(w_exp=1.0, w_sat=0.2, w_con=0.5, w_ent=0.3, cwin=cwin, ewin=ewin)
+        prm = enfuse_params(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3,
+                            contrast_window=cwin, entropy_window=ewin)
+        got = oracle.enfuse_mask(img, a, itype, prm)
+        want = direct_mask(img, a, itype, **kw)
+        np.testing.assert_array_equal(got, want, err_msg=f"{w}x{h} c{cwin} e{ewin}")
+
+
+def test_float_weights_structure(oracle):
+    """Float images: exposure y = gray / FLT_MAX collapses to weight(0)
+    (src/enfuse.h:637) and saturation uses max_value 1.0; contrast and
+    entropy follow the integer structure (float sums, FLT_MAX scale)."""
+    rng = np.random.default_rng(5)
+    h, w = 12, 13
+    img = (rng.random((h, w, 3)) * 0.4 + 0.05).astype(np.float32)
+    a = np.full((h, w), 255, np.uint8)
+    got = oracle.enfuse_mask(img, a, F32, enfuse_params(w_exposure=1.0))
+    assert np.all(got == np.float32(math.exp(-0.5 * (0.5 / 0.2) ** 2)))
+    got = oracle.enfuse_mask(img, a, F32, enfuse_params(w_exposure=0.0, w_saturation=1.0))
+    mx, mn = img.max(2).astype(np.float64), img.min(2).astype(np.float64)
+    np.testing.assert_array_equal(got, ((mx - mn) / (mx + mn)).astype(np.float32))
+    # entropy of a window of distinct values is 1, scaled by FLT_MAX
+    got = oracle.enfuse_mask(img, a, F32, enfuse_params(w_exposure=0.0, w_entropy=0.5, entropy_window=3))
+    inner = got[1:-1, 1:-1]
+    np.testing.assert_allclose(inner, 0.5 * np.finfo(np.float32).max, rtol=1e-6)
+    assert np.all(got[0] == 0) and np.all(got[:, 0] == 0)
+
+
+def test_fusion_identity_single_image(oracle):
+    """K = 1 with zero weights: the normalised mask is the fallback 255/K =
+    255, the mask coefficient is exactly 1 and collapse inverts the
+    Laplacian, so the output is the input wherever alpha is set."""
+    rng = np.random.default_rng(9)
+    for itype in (U8, U16):
+        img = _img(rng, itype, 37, 45)
+        a = np.where(rng.random((37, 45)) < 0.8, 255, 0).astype(np.uint8)
+        out, oa = oracle.enfuse([img], [a], itype, enfuse_params(w_exposure=0.0), oracle.auto_levels(45, 37))
+        sel = a != 0
+        np.testing.assert_array_equal(oa, a)
+        np.testing.assert_array_equal(out[sel], img[sel])
+        assert np.all(out[~sel] == 0)
+
+
+def test_fusion_known_answer_constant_images(oracle):
+    """Two constant u8 images with saturation weights 1/2 and 1/4 normalise
+    to masks 170 and 85 (exact in float), i.e. coefficients 2/3 and 1/3 of
+    the coarsest Gaussian level; every Laplacian level is 0."""
+    h, w = 24, 40
+    A = np.zeros((h, w, 3), np.uint8); A[...] = (90, 30, 30)       # saturation 1/2
+    B = np.zeros((h, w, 3), np.uint8); B[...] = (150, 90, 90)      # saturation 1/4
+    full = np.full((h, w), 255, np.uint8)
+    prm = enfuse_params(w_exposure=0.0, w_saturation=1.0)
+    out, _ = oracle.enfuse([A, B], [full, full], U8, prm, oracle.auto_levels(w, h), dither=0)
+    cA = (170 << 7) / float(255 << 7)
+    cB = (85 << 7) / float(255 << 7)
+    for c in range(3):
+        v = round_half_away(int(A[0, 0, c]) * 128 * cA) + round_half_away(int(B[0, 0, c]) * 128 * cB)
+        q, f = v >> 7, v & 127
+        want = q + 1 if f >= 64 else q
+        assert np.all(out[..., c] == want), (c, out[0, 0], want)
