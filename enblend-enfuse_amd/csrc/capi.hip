@@ -111,7 +111,7 @@ int do_reduce(enbl_context* ctx, int ptype, int C, int wrap,
     a.salpha = sa; a.sapitch = sap;
     a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
     a.dalpha = da; a.dapitch = dap;
-    a.channels = C; a.wrap = wrap; a.rows = 16;
+    a.channels = C; a.wrap = wrap; a.rows = 0;
     char name[64];
     snprintf(name, sizeof name, "reduce_%s%s_c%d", sa ? "alpha_" : "", pname(ptype), C);
     const double es = (double)ptype_size(ptype) * C + (sa ? 1.0 : 0.0);
@@ -131,7 +131,7 @@ int do_expand(enbl_context* ctx, int ptype, int C, int add, int wrap,
     ExpandArgs a;
     a.src = src; a.spitch = sp; a.splane = spl; a.sw = sw; a.sh = sh;
     a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
-    a.channels = C; a.wrap = wrap; a.add = add; a.rows = 16;
+    a.channels = C; a.wrap = wrap; a.add = add; a.rows = 0;
     char name[64];
     snprintf(name, sizeof name, "expand_%s_%s_c%d", add ? "add" : "sub", pname(ptype), C);
     const double es = (double)ptype_size(ptype) * C;

@@ -89,12 +89,130 @@ __device__ __forceinline__ int wrap_index(int k, int n) {
 // ---------------------------------------------------------------------------
 // reduce
 // ---------------------------------------------------------------------------
-template <class P, int C, bool ALPHA>
-__global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
-{
+
+// Raw samples of one source row held by one lane: its own pair (2j, 2j+1),
+// alpha-masked, and, for wraparound edge lanes, the four taps that wrap to
+// the far side of the row.
+template <class S, int C, bool WRAP>
+struct ReduceRow {
+    S x0[C], x1[C];
+    int m0, m1;
+    S wa[WRAP ? C : 1], wb[WRAP ? C : 1], wd[WRAP ? C : 1], we[WRAP ? C : 1];
+    int qa, qb, qd, qe;
+};
+
+template <class P, int C, bool ALPHA, bool WRAP>
+struct ReduceLane {
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
+    typedef ReduceRow<S, C, WRAP> Row;
+
+    const ReduceArgs& a;
+    int c;          // source centre column 2j
+    bool valid;     // lane's destination column exists
+    bool both;      // c + 1 is inside the row
+    bool edge;      // a tap of this lane leaves the row
+
+    // Issue the loads of source row r (no waiting, no cross-lane traffic).
+    __device__ __forceinline__ void load(int r, Row& R) const {
+        const P* src = static_cast<const P*>(a.src);
+        R.m0 = 0;
+        R.m1 = 0;
+        if (ALPHA && valid) {
+            const uint8_t* ar = a.salpha + (long)r * a.sapitch;
+            R.m0 = ar[c] ? 1 : 0;
+            R.m1 = (both && ar[c + 1]) ? 1 : 0;
+        }
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            S x0 = A::zero(), x1 = A::zero();
+            if (valid) Vec2<P>::load(src + ch * a.splane + (long)r * a.spitch + c, both, x0, x1);
+            if (ALPHA) {
+                if (!R.m0) x0 = A::zero();
+                if (!R.m1) x1 = A::zero();
+            }
+            R.x0[ch] = x0;
+            R.x1[ch] = x1;
+        }
+        if (WRAP) {
+            R.qa = R.qb = R.qd = R.qe = 0;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) { R.wa[ch] = R.wb[ch] = R.wd[ch] = R.we[ch] = A::zero(); }
+            if (edge) {
+                const int sw = a.sw;
+                const int ia = wrap_index(c - 2, sw), ib = wrap_index(c - 1, sw);
+                const int id = wrap_index(c + 1, sw), ie = wrap_index(c + 2, sw);
+                int ma = 1, mb = 1, md = 1, me = 1;
+                if (ALPHA) {
+                    const uint8_t* ar = a.salpha + (long)r * a.sapitch;
+                    ma = ar[ia] ? 1 : 0; mb = ar[ib] ? 1 : 0; md = ar[id] ? 1 : 0; me = ar[ie] ? 1 : 0;
+                }
+                R.qa = ma; R.qb = mb; R.qd = md; R.qe = me;
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) {
+                    const P* row = src + ch * a.splane + (long)r * a.spitch;
+                    R.wa[ch] = ma ? Px::load(row[ia]) : A::zero();
+                    R.wb[ch] = mb ? Px::load(row[ib]) : A::zero();
+                    R.wd[ch] = md ? Px::load(row[id]) : A::zero();
+                    R.we[ch] = me ? Px::load(row[ie]) : A::zero();
+                }
+            }
+        }
+    }
+
+    // Horizontal 5-tap sums of a loaded row (every lane of the wave executes
+    // this: the neighbour taps arrive through shuffles).
+    __device__ __forceinline__ void sums(const Row& R, S (&h)[C], int& hm) const {
+        const int sw = a.sw;
+        // boundary case of the reference's row tail (src/pyramid.h:264-299,
+        // :410-455, :688-707): 1 = centre is the last sample, 2 = centre is
+        // the second to last sample (no wraparound only)
+        const int rcase = WRAP ? 0 : (c == sw - 1 ? 1 : (c == sw - 2 ? 2 : 0));
+        if (ALPHA) {
+            const int ml0 = lane_up(R.m0), ml1 = lane_up(R.m1), mr0 = lane_down(R.m0);
+            int qa, qb, qd, qe;
+            if (WRAP && edge) { qa = R.qa; qb = R.qb; qd = R.qd; qe = R.qe; }
+            else { qa = c >= 2 ? ml0 : 0; qb = c >= 1 ? ml1 : 0; qd = R.m1; qe = mr0; }
+            hm = qa + 4 * qb + 6 * R.m0;
+            if (rcase != 1) hm += 4 * qd;
+            if (rcase == 0) hm += qe;
+        }
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const S x0 = R.x0[ch];
+            const S l0 = lane_up(x0), l1 = lane_up(R.x1[ch]), r0 = lane_down(x0);
+            S va, vb, vd, ve;
+            if (WRAP && edge) {
+                va = R.wa[ch]; vb = R.wb[ch]; vd = R.wd[ch]; ve = R.we[ch];
+            } else {
+                va = c >= 2 ? l0 : (ALPHA ? A::zero() : x0);
+                vb = c >= 1 ? l1 : (ALPHA ? A::zero() : x0);
+                vd = R.x1[ch];
+                ve = r0;
+            }
+            const S a0 = A::add(va, A::mul(vb, 4));
+            if (rcase == 1) {
+                h[ch] = ALPHA ? A::add(a0, A::mul(x0, 6)) : A::add(a0, A::mul(x0, 11));
+            } else if (rcase == 2) {
+                const S d4 = A::mul(vd, 4);
+                h[ch] = ALPHA ? A::add(A::add(a0, A::mul(x0, 6)), d4)
+                              : A::add(A::add(A::add(a0, A::mul(x0, 6)), d4), A::div(d4, 4));
+            } else {
+                h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), A::mul(vd, 4)), ve);
+            }
+        }
+    }
+};
+
+template <class P, int C, bool ALPHA, bool WRAP>
+__global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
+{
+    typedef ReduceLane<P, C, ALPHA, WRAP> Lane;
+    typedef typename Lane::A A;
+    typedef typename Lane::S S;
+    typedef typename Lane::Row Row;
+    typedef Pix<P> Px;
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
@@ -102,119 +220,42 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
     const int i0 = blockIdx.y * a.rows;
     if (i0 >= a.dh) return;
     const int i1 = min(a.dh, i0 + a.rows);
-
-    const int sw = a.sw, sh = a.sh;
+    const int sh = a.sh;
     const int j = wx * kOwned + lane - 1;
     const int c = 2 * j;
     const bool valid = j >= 0 && j < a.dw;
     const bool owned = lane >= 1 && lane <= kOwned && valid;
-    const bool both = c + 1 < sw;
-    const bool edge = valid && (j == 0 || c + 2 >= sw);
-    const bool wrap = a.wrap != 0;
-
-    const P* src = static_cast<const P*>(a.src);
+    const Lane L{a, c, valid, c + 1 < a.sw, valid && (j == 0 || c + 2 >= a.sw)};
     P* dst = static_cast<P*>(a.dst);
-
-    // Horizontal row sum of source row r at centre column c, all channels;
-    // alpha weight sum in *hm.  Executed by every lane of the wave (shuffles).
-    auto hrow = [&](int r, S (&h)[C], int& hm) {
-        const uint8_t* arow = ALPHA ? a.salpha + (long)r * a.sapitch : nullptr;
-        int m0 = 0, m1 = 0;
-        if (ALPHA && valid) {
-            m0 = arow[c] ? 1 : 0;
-            m1 = both ? (arow[c + 1] ? 1 : 0) : 0;
-        }
-        int ml0 = 0, ml1 = 0, mr0 = 0;
-        if (ALPHA) {
-            ml0 = lane_up(m0);
-            ml1 = lane_up(m1);
-            mr0 = lane_down(m0);
-        }
-        if (ALPHA) {
-            if (!edge) {
-                hm = ml0 + 4 * ml1 + 6 * m0 + 4 * m1 + mr0;
-            } else {
-                auto M = [&](int k) -> int { return arow[k] ? 1 : 0; };
-                int ma, mb, md, me;
-                if (wrap) {
-                    ma = M(wrap_index(c - 2, sw)); mb = M(wrap_index(c - 1, sw));
-                    md = M(wrap_index(c + 1, sw)); me = M(wrap_index(c + 2, sw));
-                    hm = ma + 4 * mb + 6 * m0 + 4 * md + me;
-                } else {
-                    ma = c >= 2 ? M(c - 2) : 0; mb = c >= 1 ? M(c - 1) : 0;
-                    md = c + 1 < sw ? M(c + 1) : 0; me = c + 2 < sw ? M(c + 2) : 0;
-                    hm = ma + 4 * mb + 6 * m0 + 4 * md + me;
-                }
-            }
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const P* row = src + ch * a.splane + (long)r * a.spitch;
-            S x0 = A::zero(), x1 = A::zero();
-            if (valid) {
-                Vec2<P>::load(row + c, both, x0, x1);
-                if (ALPHA) {
-                    if (!m0) x0 = A::zero();
-                    if (!m1) x1 = A::zero();
-                }
-            }
-            const S l0 = lane_up(x0);
-            const S l1 = lane_up(x1);
-            const S r0 = lane_down(x0);
-            if (!edge) {
-                h[ch] = A::add(A::add(A::add(A::add(l0, A::mul(l1, 4)), A::mul(x0, 6)), A::mul(x1, 4)), r0);
-                continue;
-            }
-            // Boundary lanes: literal boundary expressions of the reference.
-            auto F = [&](int k) -> S {
-                if (ALPHA) return arow[k] ? Px::load(row[k]) : A::zero();
-                return Px::load(row[k]);
-            };
-            if (wrap) {
-                const S va = F(wrap_index(c - 2, sw)), vb = F(wrap_index(c - 1, sw));
-                const S vd = F(wrap_index(c + 1, sw)), ve = F(wrap_index(c + 2, sw));
-                h[ch] = A::add(A::add(A::add(A::add(va, A::mul(vb, 4)), A::mul(x0, 6)), A::mul(vd, 4)), ve);
-            } else if (ALPHA) {
-                const S va = c >= 2 ? F(c - 2) : A::zero();
-                const S vb = c >= 1 ? F(c - 1) : A::zero();
-                const S a0 = A::add(va, A::mul(vb, 4));
-                if (c == sw - 1) {
-                    h[ch] = A::add(a0, A::mul(x0, 6));
-                } else if (c == sw - 2) {
-                    h[ch] = A::add(A::add(a0, A::mul(x0, 6)), A::mul(x1, 4));
-                } else {
-                    h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), A::mul(x1, 4)), F(c + 2));
-                }
-            } else {
-                const S va = c >= 2 ? F(c - 2) : F(0);
-                const S vb = c >= 1 ? F(c - 1) : F(0);
-                const S a0 = A::add(va, A::mul(vb, 4));
-                if (c == sw - 1) {
-                    h[ch] = A::add(a0, A::mul(x0, 11));
-                } else if (c == sw - 2) {
-                    const S d4 = A::mul(x1, 4);
-                    h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), d4), A::div(d4, 4));
-                } else {
-                    h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), A::mul(x1, 4)), F(c + 2));
-                }
-            }
-        }
-    };
 
     S hm2[C], hm1[C], hc[C], hp1[C], hp2[C];
     int qm2 = 0, qm1 = 0, qc = 0, qp1 = 0, qp2 = 0;
 #pragma unroll
-    for (int ch = 0; ch < C; ++ch) { hm2[ch] = A::zero(); hm1[ch] = A::zero(); }
-    if (i0 > 0) {
-        hrow(2 * i0 - 2, hm2, qm2);
-        hrow(2 * i0 - 1, hm1, qm1);
+    for (int ch = 0; ch < C; ++ch) { hm2[ch] = hm1[ch] = hc[ch] = hp1[ch] = hp2[ch] = A::zero(); }
+    Row ra, rb, na, nb;
+    {
+        Row t;
+        if (i0 > 0) {
+            L.load(2 * i0 - 2, t);
+            L.sums(t, hm2, qm2);
+            L.load(2 * i0 - 1, t);
+            L.sums(t, hm1, qm1);
+        }
+        L.load(2 * i0, t);
+        L.sums(t, hc, qc);
     }
-    hrow(2 * i0, hc, qc);
+    if (2 * i0 + 1 < sh) L.load(2 * i0 + 1, ra);
+    if (2 * i0 + 2 < sh) L.load(2 * i0 + 2, rb);
 
     for (int i = i0; i < i1; ++i) {
         const int r = 2 * i;
-        if (r + 1 < sh) hrow(r + 1, hp1, qp1);
-        if (r + 2 < sh) hrow(r + 2, hp2, qp2);
+        // prefetch the next output row's two source rows
+        if (i + 1 < i1) {
+            if (r + 3 < sh) L.load(r + 3, na);
+            if (r + 4 < sh) L.load(r + 4, nb);
+        }
+        if (r + 1 < sh) L.sums(ra, hp1, qp1);
+        if (r + 2 < sh) L.sums(rb, hp2, qp2);
 
         int ap = 0;
         if (ALPHA) {
@@ -227,11 +268,13 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
         for (int ch = 0; ch < C; ++ch) {
             S v;
             if (ALPHA) {
+                // src/pyramid.h:350-384, :459-504 (column recursion)
                 const S a0 = r == 0 ? A::zero() : A::add(hm2[ch], A::mul(hm1[ch], 4));
                 if (r == sh - 1) v = A::add(a0, A::mul(hc[ch], 6));
                 else if (r == sh - 2) v = A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4));
                 else v = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4)), hp2[ch]);
             } else {
+                // src/pyramid.h:669-672, :741-760, :809-846 (clamped columns)
                 const S a0 = r == 0 ? A::mul(hc[ch], 5) : A::add(hm2[ch], A::mul(hm1[ch], 4));
                 if (r == sh - 1) {
                     v = A::add(a0, A::mul(hc[ch], 11));
@@ -254,6 +297,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
         }
         if (ALPHA && owned) a.dalpha[(long)i * a.dapitch + j] = ap ? 255 : 0;
         qm2 = qc; qm1 = qp1; qc = qp2;
+        ra = na;
+        rb = nb;
     }
 }
 
@@ -282,45 +327,71 @@ __device__ __forceinline__ int expand_vsum(int Y, int sh) {
     return 1 + (i < sh - 1 ? 1 : 0);
 }
 
-template <class P, int C, bool ADD>
-__global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
-{
+template <class S, int C, bool WRAP>
+struct ExpandRow {
+    S x[C];
+    S xw[WRAP ? C : 1];   // far-side sample of a wraparound edge lane
+};
+
+template <class S, int C>
+struct DestPair {
+    S d0[C], d1[C], e0[C], e1[C];   // rows Y and Y+1, columns X and X+1
+};
+
+template <class P, int C, bool ADD, bool WRAP>
+struct ExpandLane {
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
+    typedef ExpandRow<S, C, WRAP> Row;
+    typedef DestPair<S, C> Dest;
 
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
-    if (wx * kOwned - 1 >= a.sw) return;
-    const int i0 = blockIdx.y * a.rows;
-    if (i0 >= a.sh) return;
-    const int i1 = min(a.sh, i0 + a.rows);
+    const ExpandArgs& a;
+    int j;
+    bool valid, owned, dst_even, xb;
 
-    const int sw = a.sw, sh = a.sh, dw = a.dw, dh = a.dh;
-    const int j = wx * kOwned + lane - 1;
-    const bool valid = j >= 0 && j < sw;
-    const bool owned = lane >= 1 && lane <= kOwned && valid;
-    const bool wrap = a.wrap != 0;
-    const bool dst_even = (dw & 1) == 0;
-
-    const P* src = static_cast<const P*>(a.src);
-    P* dst = static_cast<P*>(a.dst);
-
-    // Horizontal polyphase sums of source row i: hA for destination column 2j
-    // (taps 1 6 1), hB for 2j+1 (taps 4 4).
-    auto hrow = [&](int i, S (&hA)[C], S (&hB)[C]) {
+    __device__ __forceinline__ void load(int i, Row& R) const {
+        const P* src = static_cast<const P*>(a.src);
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             const P* row = src + ch * a.splane + (long)i * a.spitch;
-            const S x = valid ? Px::load(row[j]) : A::zero();
+            R.x[ch] = valid ? Px::load(row[j]) : A::zero();
+            if (WRAP) {
+                R.xw[ch] = A::zero();
+                if (valid && j == 0) R.xw[ch] = Px::load(row[a.sw - 1]);
+                else if (valid && j == a.sw - 1) R.xw[ch] = Px::load(row[0]);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void load_dest(int Y, Dest& D) const {
+        const P* dst = static_cast<const P*>(a.dst);
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            D.d0[ch] = D.d1[ch] = D.e0[ch] = D.e1[ch] = A::zero();
+            if (owned) {
+                const P* r0 = dst + ch * a.dplane + (long)Y * a.dpitch + 2 * j;
+                Vec2<P>::load(r0, xb, D.d0[ch], D.d1[ch]);
+                if (Y + 1 < a.dh) Vec2<P>::load(r0 + a.dpitch, xb, D.e0[ch], D.e1[ch]);
+            }
+        }
+    }
+
+    // Polyphase row sums of source row i: hA for destination column 2j
+    // (taps 1 6 1), hB for 2j+1 (taps 4 4); literal boundary forms of
+    // src/pyramid.h:1148-1194 and the SKIPSM_EXPAND*_END macros.
+    __device__ __forceinline__ void sums(const Row& R, int i, S (&hA)[C], S (&hB)[C]) const {
+        const int sw = a.sw;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const S x = R.x[ch];
             const S l = lane_up(x);
             const S r = lane_down(x);
             if (sw == 1) {
                 S r1 = A::zero();
-                if (wrap) r1 = dst_even ? x : A::mul(x, 4);
-                if (wrap && i == 0) {
-                    hA[ch] = dst_even ? A::add(A::add(r1, A::mul(x, 6)), x)
-                                      : A::add(A::add(r1, A::mul(x, 6)), A::mul(x, 4));
+                if (WRAP) r1 = dst_even ? x : A::mul(x, 4);
+                if (WRAP && i == 0) {
+                    hA[ch] = dst_even ? A::add(A::add(r1, A::mul(x, 6)), x) : A::add(A::add(r1, A::mul(x, 6)), A::mul(x, 4));
                     hB[ch] = A::mul(A::add(x, x), 4);
                 } else {
                     hA[ch] = A::add(r1, A::mul(x, 6));
@@ -328,17 +399,13 @@ __global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
                 }
             } else if (j == 0) {
                 S r1 = A::zero();
-                if (wrap) {
-                    const S xl = Px::load(row[sw - 1]);
-                    r1 = dst_even ? xl : A::mul(xl, 4);
-                }
+                if (WRAP) r1 = dst_even ? R.xw[ch] : A::mul(R.xw[ch], 4);
                 hA[ch] = A::add(A::add(r1, A::mul(x, 6)), r);
                 hB[ch] = A::mul(A::add(x, r), 4);
             } else if (j == sw - 1) {
-                if (wrap) {
-                    const S xf = Px::load(row[0]);
-                    hA[ch] = dst_even ? A::add(A::add(l, A::mul(x, 6)), xf)
-                                      : A::add(A::add(l, A::mul(x, 6)), A::mul(xf, 4));
+                if (WRAP) {
+                    const S xf = R.xw[ch];
+                    hA[ch] = dst_even ? A::add(A::add(l, A::mul(x, 6)), xf) : A::add(A::add(l, A::mul(x, 6)), A::mul(xf, 4));
                     hB[ch] = A::mul(A::add(x, xf), 4);
                 } else {
                     hA[ch] = A::add(l, A::mul(x, 6));
@@ -349,22 +416,62 @@ __global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
                 hB[ch] = A::mul(A::add(x, r), 4);
             }
         }
-    };
+    }
+};
 
+template <class P, int C, bool ADD, bool WRAP>
+__global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
+{
+    typedef ExpandLane<P, C, ADD, WRAP> Lane;
+    typedef typename Lane::A A;
+    typedef typename Lane::S S;
+    typedef typename Lane::Row Row;
+    typedef typename Lane::Dest Dest;
+    typedef Pix<P> Px;
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    if (wx * kOwned - 1 >= a.sw) return;
+    const int i0 = blockIdx.y * a.rows;
+    if (i0 >= a.sh) return;
+    const int i1 = min(a.sh, i0 + a.rows);
+    const int sw = a.sw, sh = a.sh, dw = a.dw, dh = a.dh;
+    const int j = wx * kOwned + lane - 1;
+    const bool valid = j >= 0 && j < sw;
+    const bool owned = lane >= 1 && lane <= kOwned && valid;
+    const bool dst_even = (dw & 1) == 0;
     const int X = 2 * j;
     const bool xb = X + 1 < dw;
-    const int hdA = expand_hsum(X, sw, wrap, dst_even);
-    const int hdB = expand_hsum(X + 1, sw, wrap, dst_even);
+    const Lane L{a, j, valid, owned, dst_even, xb};
+    P* dst = static_cast<P*>(a.dst);
+
+    const int hdA = expand_hsum(X, sw, WRAP, dst_even);
+    const int hdB = expand_hsum(X + 1, sw, WRAP, dst_even);
 
     S pA[C], pB[C], cA[C], cB[C], nA[C], nB[C];
 #pragma unroll
-    for (int ch = 0; ch < C; ++ch) { pA[ch] = A::zero(); pB[ch] = A::zero(); }
-    if (i0 > 0) hrow(i0 - 1, pA, pB);
-    hrow(i0, cA, cB);
+    for (int ch = 0; ch < C; ++ch) { pA[ch] = pB[ch] = cA[ch] = cB[ch] = nA[ch] = nB[ch] = A::zero(); }
+    Row rn, rnn;
+    Dest dc, dn;
+    {
+        Row t;
+        if (i0 > 0) {
+            L.load(i0 - 1, t);
+            L.sums(t, i0 - 1, pA, pB);
+        }
+        L.load(i0, t);
+        L.sums(t, i0, cA, cB);
+    }
+    if (i0 + 1 < sh) L.load(i0 + 1, rn);
+    L.load_dest(2 * i0, dc);
 
     for (int i = i0; i < i1; ++i) {
         const bool more = i + 1 < sh;
-        if (more) hrow(i + 1, nA, nB);
+        if (i + 1 < i1) {
+            if (i + 2 < sh) L.load(i + 2, rnn);
+            L.load_dest(2 * (i + 1), dn);
+        }
+        if (more) L.sums(rn, i + 1, nA, nB);
         const int Y = 2 * i;
         const bool yb = Y + 1 < dh;
         const int v0 = expand_vsum(Y, sh);
@@ -385,46 +492,59 @@ __global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
             }
             if (owned) {
                 P* row0 = dst + ch * a.dplane + (long)Y * a.dpitch + X;
-                S d0, d1;
-                Vec2<P>::load(row0, xb, d0, d1);
                 const S e0 = A::div(o00, hdA * v0);
                 const S e1 = A::div(o10, hdB * v0);
                 P w0, w1;
-                if (ADD) { w0 = Px::sat(A::add(d0, e0)); w1 = Px::sat(A::add(d1, e1)); }
-                else { w0 = Px::wrap(A::sub(d0, e0)); w1 = Px::wrap(A::sub(d1, e1)); }
+                if (ADD) { w0 = Px::sat(A::add(dc.d0[ch], e0)); w1 = Px::sat(A::add(dc.d1[ch], e1)); }
+                else { w0 = Px::wrap(A::sub(dc.d0[ch], e0)); w1 = Px::wrap(A::sub(dc.d1[ch], e1)); }
                 Vec2<P>::store(row0, xb, w0, w1);
                 if (yb) {
-                    P* row1 = row0 + a.dpitch;
-                    Vec2<P>::load(row1, xb, d0, d1);
                     const S f0 = A::div(o01, hdA * v1);
                     const S f1 = A::div(o11, hdB * v1);
-                    if (ADD) { w0 = Px::sat(A::add(d0, f0)); w1 = Px::sat(A::add(d1, f1)); }
-                    else { w0 = Px::wrap(A::sub(d0, f0)); w1 = Px::wrap(A::sub(d1, f1)); }
-                    Vec2<P>::store(row1, xb, w0, w1);
+                    if (ADD) { w0 = Px::sat(A::add(dc.e0[ch], f0)); w1 = Px::sat(A::add(dc.e1[ch], f1)); }
+                    else { w0 = Px::wrap(A::sub(dc.e0[ch], f0)); w1 = Px::wrap(A::sub(dc.e1[ch], f1)); }
+                    Vec2<P>::store(row0 + a.dpitch, xb, w0, w1);
                 }
             }
             pA[ch] = cA[ch]; pB[ch] = cB[ch];
             cA[ch] = nA[ch]; cB[ch] = nB[ch];
         }
+        rn = rnn;
+        dc = dn;
     }
 }
 
+// Rows of output per strip: enough strips to keep every SIMD busy, but long
+// enough that the 3-row (reduce) / 2-row (expand) strip halo stays small.
+inline int strip_rows(int waves_x, int out_rows)
+{
+    const long target = 8192;
+    long r = ((long)waves_x * out_rows + target - 1) / target;
+    if (r < 4) r = 4;
+    if (r > 32) r = 32;
+    return (int)r;
+}
+
 template <class P, int C, bool ALPHA>
-void launch_reduce_t(const ReduceArgs& a, hipStream_t s)
+void launch_reduce_t(ReduceArgs a, hipStream_t s)
 {
     const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
+    if (a.rows <= 0) a.rows = strip_rows(waves_x, a.dh);
     dim3 block(256);
     dim3 grid((waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows);
-    hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA>), grid, block, 0, s, a);
+    if (a.wrap) hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, false>), grid, block, 0, s, a);
 }
 
 template <class P, int C, bool ADD>
-void launch_expand_t(const ExpandArgs& a, hipStream_t s)
+void launch_expand_t(ExpandArgs a, hipStream_t s)
 {
     const int waves_x = (a.sw + kOwned - 1) / kOwned + 1;
+    if (a.rows <= 0) a.rows = strip_rows(waves_x, a.sh);
     dim3 block(256);
     dim3 grid((waves_x + 3) / 4, (a.sh + a.rows - 1) / a.rows);
-    hipLaunchKernelGGL((expand_kernel<P, C, ADD>), grid, block, 0, s, a);
+    if (a.wrap) hipLaunchKernelGGL((expand_kernel<P, C, ADD, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((expand_kernel<P, C, ADD, false>), grid, block, 0, s, a);
 }
 
 template <class P>
@@ -448,7 +568,6 @@ int expand_by_c(const ExpandArgs& a, hipStream_t s)
 
 int launch_reduce(int ptype, ReduceArgs a, hipStream_t s)
 {
-    if (a.rows <= 0) a.rows = 16;
     switch (ptype) {
     case ENBL_I16: return reduce_by_c<int16_t>(a, s);
     case ENBL_I32: return reduce_by_c<int32_t>(a, s);
@@ -459,7 +578,6 @@ int launch_reduce(int ptype, ReduceArgs a, hipStream_t s)
 
 int launch_expand(int ptype, ExpandArgs a, hipStream_t s)
 {
-    if (a.rows <= 0) a.rows = 16;
     switch (ptype) {
     case ENBL_I16: return expand_by_c<int16_t>(a, s);
     case ENBL_I32: return expand_by_c<int32_t>(a, s);
