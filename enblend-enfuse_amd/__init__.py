@@ -23,6 +23,7 @@ ENBL_E_HIP = -6
 I16, I32, F64 = 0, 1, 2
 U8, U16, F32 = 10, 11, 12
 DITHER_NONE, DITHER_DET = 0, 1
+KERNELS_AUTO, KERNELS_DIRECT, KERNELS_STREAMING = 0, 1, 2
 
 _c_int, _c_long, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
 
@@ -35,6 +36,7 @@ _SIGS = {
     "enbl_last_error": (ctypes.c_char_p, []),
     "enbl_profile_enable": (_c_int, [_vp, _c_int]),
     "enbl_profile_report": (_c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "enbl_set_kernel_policy": (_c_int, [_c_int]),
     "enbl_reduce": (_c_int, [_vp, _c_int, _c_int, _c_int,
                              _vp, _c_int, _c_int, _c_long, _c_long, _vp, _c_long,
                              _vp, _c_int, _c_int, _c_long, _c_long, _vp, _c_long]),

@@ -21,6 +21,8 @@ namespace enbl {
 namespace host {
 
 thread_local std::string g_error;
+// Kernel selection (enbl_set_kernel_policy).
+int g_policy = ENBL_KERNELS_AUTO;
 
 int fail(int code, const char* fmt, ...)
 {
@@ -111,7 +113,7 @@ int do_reduce(enbl_context* ctx, int ptype, int C, int wrap,
     a.salpha = sa; a.sapitch = sap;
     a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
     a.dalpha = da; a.dapitch = dap;
-    a.channels = C; a.wrap = wrap; a.rows = 0;
+    a.channels = C; a.wrap = wrap; a.rows = 0; a.force_small = g_policy;
     char name[64];
     snprintf(name, sizeof name, "reduce_%s%s_c%d", sa ? "alpha_" : "", pname(ptype), C);
     const double es = (double)ptype_size(ptype) * C + (sa ? 1.0 : 0.0);
@@ -131,7 +133,7 @@ int do_expand(enbl_context* ctx, int ptype, int C, int add, int wrap,
     ExpandArgs a;
     a.src = src; a.spitch = sp; a.splane = spl; a.sw = sw; a.sh = sh;
     a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
-    a.channels = C; a.wrap = wrap; a.add = add; a.rows = 0;
+    a.channels = C; a.wrap = wrap; a.add = add; a.rows = 0; a.force_small = g_policy;
     char name[64];
     snprintf(name, sizeof name, "expand_%s_%s_c%d", add ? "add" : "sub", pname(ptype), C);
     const double es = (double)ptype_size(ptype) * C;
@@ -301,6 +303,13 @@ int enbl_context_get_stream(const enbl_context* ctx, void** s)
 {
     if (!ctx || !s) return fail(ENBL_E_ARG, "null argument");
     *s = (void*)ctx->stream;
+    return ENBL_OK;
+}
+
+int enbl_set_kernel_policy(int policy)
+{
+    if (policy < ENBL_KERNELS_AUTO || policy > ENBL_KERNELS_STREAMING) return fail(ENBL_E_ARG, "kernel policy %d", policy);
+    g_policy = policy;
     return ENBL_OK;
 }
 

@@ -29,12 +29,14 @@ struct ReduceArgs {
     void* dst; long dpitch; long dplane; int dw; int dh;
     uint8_t* dalpha; long dapitch;
     int channels; int wrap; int rows;
+    int force_small;   // ENBL_KERNELS_* policy
 };
 
 struct ExpandArgs {
     const void* src; long spitch; long splane; int sw; int sh;
     void* dst; long dpitch; long dplane; int dw; int dh;
     int channels; int wrap; int add; int rows;
+    int force_small;
 };
 
 int launch_reduce(int ptype, ReduceArgs a, hipStream_t s);

@@ -62,6 +62,14 @@ int enbl_context_destroy(enbl_context* ctx);
 int enbl_context_set_stream(enbl_context* ctx, void* hip_stream);
 int enbl_context_get_stream(const enbl_context* ctx, void** hip_stream);
 int enbl_synchronize(enbl_context* ctx);
+/* Kernel selection of reduce/expand for every context of the process:
+ * AUTO picks the one-thread-per-output kernels for small levels and the
+ * streaming kernels for large ones; DIRECT and STREAMING force one family
+ * (used by the parity tests and the tile autotuner).  Not thread-safe. */
+#define ENBL_KERNELS_AUTO 0
+#define ENBL_KERNELS_DIRECT 1
+#define ENBL_KERNELS_STREAMING 2
+int enbl_set_kernel_policy(int policy);
 /* Message of the last failing call on this thread. */
 const char* enbl_last_error(void);
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the

@@ -81,11 +81,18 @@ REDUCE_SHAPES = [(2, 2), (2, 3), (3, 2), (3, 3), (5, 4), (4, 5), (7, 9), (16, 16
                  (64, 130), (125, 127), (126, 250), (131, 124), (61, 300), (300, 61), (513, 257)]
 
 
+@pytest.fixture(params=[1, 2], ids=["direct", "streaming"])
+def policy(request, enbl):
+    enbl.check(enbl.lib().enbl_set_kernel_policy(request.param))
+    yield request.param
+    enbl.check(enbl.lib().enbl_set_kernel_policy(0))
+
+
 @pytest.mark.parametrize("ptype", [I16, I32, F64])
 @pytest.mark.parametrize("C", [1, 3])
 @pytest.mark.parametrize("wrap", [False, True])
 @pytest.mark.parametrize("use_alpha", [True, False])
-def test_reduce_matches_oracle(enbl, gpu_ctx, oracle, ptype, C, wrap, use_alpha):
+def test_reduce_matches_oracle(enbl, gpu_ctx, oracle, ptype, C, wrap, use_alpha, policy):
     rng = np.random.default_rng(100 + ptype * 8 + C * 2 + wrap + 16 * use_alpha)
     for (h, w) in REDUCE_SHAPES:
         print(f"reduce ptype={ptype} C={C} wrap={wrap} alpha={use_alpha} {w}x{h}", flush=True)
@@ -110,7 +117,7 @@ EXPAND_SHAPES = [(1, 1, 1, 2), (1, 1, 2, 1), (1, 1, 2, 2), (1, 3, 2, 5), (1, 3, 
 @pytest.mark.parametrize("C", [1, 3])
 @pytest.mark.parametrize("wrap", [False, True])
 @pytest.mark.parametrize("add", [False, True])
-def test_expand_matches_oracle(enbl, gpu_ctx, oracle, ptype, C, wrap, add):
+def test_expand_matches_oracle(enbl, gpu_ctx, oracle, ptype, C, wrap, add, policy):
     rng = np.random.default_rng(200 + ptype * 8 + C * 2 + wrap + 16 * add)
     for (sh, sw, dh, dw) in EXPAND_SHAPES:
         print(f"expand ptype={ptype} C={C} wrap={wrap} add={add} src {sw}x{sh} dst {dw}x{dh}", flush=True)
@@ -141,7 +148,7 @@ def _img(rng, itype, h, w, C=3):
 
 @pytest.mark.parametrize("itype", [U8, U16, F32])
 @pytest.mark.parametrize("wrap", [False, True])
-def test_pyramids_match_oracle(enbl, gpu_ctx, oracle, itype, wrap):
+def test_pyramids_match_oracle(enbl, gpu_ctx, oracle, itype, wrap, policy):
     rng = np.random.default_rng(300 + itype + wrap)
     ptype = {U8: I16, U16: I32, F32: F64}[itype]
     frac = 0 if itype == F32 else 7
@@ -176,7 +183,7 @@ def test_pyramids_match_oracle(enbl, gpu_ctx, oracle, itype, wrap):
 
 
 @pytest.mark.parametrize("wrap", [False, True])
-def test_double_pyramid_ops_bit_exact(enbl, gpu_ctx, oracle, wrap):
+def test_double_pyramid_ops_bit_exact(enbl, gpu_ctx, oracle, wrap, policy):
     """Reduce / expand chains on a double pyramid with an identical level 0
     are bit-identical to the oracle (same operation order, no contraction)."""
     rng = np.random.default_rng(400 + wrap)
@@ -246,7 +253,7 @@ def _enblend_inputs(rng, itype, h, w):
 
 @pytest.mark.parametrize("itype", [U8, U16, F32])
 @pytest.mark.parametrize("wrap", [False, True])
-def test_enblend_pair_matches_oracle(enbl, gpu_ctx, oracle, itype, wrap):
+def test_enblend_pair_matches_oracle(enbl, gpu_ctx, oracle, itype, wrap, policy):
     rng = np.random.default_rng(600 + itype + wrap)
     for (h, w) in [(96, 160), (301, 457)]:
         L = oracle.auto_levels(w, h)
