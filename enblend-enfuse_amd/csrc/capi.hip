@@ -12,6 +12,7 @@
 #include <map>
 #include <string>
 
+#include "enbl_fused.h"
 #include "enbl_internal.h"
 
 using namespace enbl;
@@ -208,9 +209,102 @@ int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels
     return ENBL_OK;
 }
 
+bool streaming_level(int cw, int ch)
+{
+    if (cw < 2 || ch < 2) return false;
+    if (g_policy == ENBL_KERNELS_STREAMING) return true;
+    if (g_policy == ENBL_KERNELS_DIRECT) return false;
+    return (long)cw * ch > (1L << 20);
+}
+
+int alpha_pingpong(enbl_context* ctx, const enbl_pyramid* p, uint8_t** a, uint8_t** b, long* pitch)
+{
+    *pitch = (long)round_up((size_t)p->lv[1].w, 256);
+    return alpha_scratch(ctx, (size_t)*pitch * p->lv[1].h, a, b);
+}
+
+// laplacianPyramid (src/pyramid.h:1646-1747) with the fused
+// reduce + expand-subtract pass on every level whose coarse level is large,
+// and the separate reduce / expand kernels on the small tail.
+int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
+                          const uint8_t* alpha, long apitch, enbl_pyramid* p)
+{
+    const int L = p->levels;
+    int s = 0;
+    if (!wrap && alpha)
+        while (s + 1 < L && streaming_level(p->lv[s + 1].w, p->lv[s + 1].h)) ++s;
+    if (s == 0) {
+        int rc = build_level0(ctx, p, itype, frac, img, ipitch);
+        if (!rc) rc = gaussian_levels(ctx, p, wrap, alpha, apitch);
+        if (!rc) rc = laplacian_levels(ctx, p, wrap);
+        return rc;
+    }
+    const size_t es = (size_t)ptype_size(p->ptype);
+    void* gbuf[2] = {nullptr, nullptr};
+    if (s >= 2) {
+        const size_t gbytes = round_up((size_t)p->lv[1].plane * p->channels * es, 256);
+        void* g;
+        int rc = scratch(ctx, 8, 2 * gbytes, &g);
+        if (rc) return rc;
+        gbuf[0] = g;
+        gbuf[1] = (char*)g + gbytes;
+    }
+    uint8_t* abuf[2];
+    long apl;
+    int rc = alpha_pingpong(ctx, p, &abuf[0], &abuf[1], &apl);
+    if (rc) return rc;
+    for (int l = 0; l < s; ++l) {
+        const Level& F = p->lv[l];
+        const Level& Cl = p->lv[l + 1];
+        FusedLapArgs a;
+        if (l == 0) { a.src = img; a.spitch = ipitch; a.splane = 0; a.salpha = alpha; a.sapitch = apitch; }
+        else { a.src = gbuf[(l - 1) & 1]; a.spitch = F.pitch; a.splane = F.plane; a.salpha = abuf[(l - 1) & 1]; a.sapitch = apl; }
+        a.frac = frac;
+        a.fw = F.w; a.fh = F.h;
+        a.lap = level_ptr(p, l); a.lpitch = F.pitch; a.lplane = F.plane;
+        if (l + 1 < s) { a.coarse = gbuf[l & 1]; }
+        else { a.coarse = level_ptr(p, l + 1); }
+        a.cpitch = Cl.pitch; a.cplane = Cl.plane;
+        a.calpha = abuf[l & 1]; a.capitch = apl;
+        a.cw = Cl.w; a.ch = Cl.h;
+        a.rows = 0;
+        const double esC = (double)es * p->channels;
+        char name[64];
+        snprintf(name, sizeof name, "laplacian_level_%s_c%d", pname(p->ptype), p->channels);
+        // credited bytes: the reduce and the expand-subtract it replaces
+        Timed t(ctx, name, (esC + 1.0) * ((double)F.w * F.h + (double)Cl.w * Cl.h) +
+                               esC * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h) +
+                               (l == 0 ? (double)F.w * F.h * p->channels * (ptype_size(itype) + (double)es) : 0.0));
+        rc = launch_laplacian_level(p->ptype, l == 0 ? itype : 0, a, p->channels, ctx->stream);
+        if (rc) return fail(rc, "laplacian level: unsupported type");
+        rc = launch_status(name);
+        if (rc) return rc;
+    }
+    // Small tail: Gaussian levels s+1.. from level s in place, then
+    // expand-subtract levels s..L-2.
+    const uint8_t* sa = abuf[(s - 1) & 1];
+    for (int l = s + 1; l < L; ++l) {
+        const Level& S_ = p->lv[l - 1];
+        const Level& D = p->lv[l];
+        uint8_t* da = abuf[(l - 1) & 1];
+        rc = do_reduce(ctx, p->ptype, p->channels, wrap, level_ptr(p, l - 1), S_.w, S_.h, S_.pitch, S_.plane,
+                       sa, apl, level_ptr(p, l), D.w, D.h, D.pitch, D.plane, da, apl);
+        if (rc) return rc;
+        sa = da;
+    }
+    for (int l = s; l + 1 < L; ++l) {
+        const Level& F = p->lv[l];
+        const Level& Cl = p->lv[l + 1];
+        rc = do_expand(ctx, p->ptype, p->channels, 0, wrap, level_ptr(p, l + 1), Cl.w, Cl.h, Cl.pitch, Cl.plane,
+                       level_ptr(p, l), F.w, F.h, F.pitch, F.plane);
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
 int scratch(enbl_context* ctx, int slot, size_t bytes, void** out)
 {
-    if (slot < 0 || slot >= 8) return fail(ENBL_E_ARG, "scratch slot %d", slot);
+    if (slot < 0 || slot >= 16) return fail(ENBL_E_ARG, "scratch slot %d", slot);
     if (ctx->scratch_bytes[slot] < bytes) {
         if (ctx->scratch_buf[slot]) {
             int rc = hip_check(hipStreamSynchronize(ctx->stream), "scratch");
@@ -270,7 +364,7 @@ int enbl_context_create(int device, enbl_context** out)
     c->profile = false;
     c->scratch_alpha = nullptr;
     c->scratch_alpha_bytes = 0;
-    for (int i = 0; i < 8; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
+    for (int i = 0; i < 16; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) { delete c; return rc; }
     c->stream = c->own_stream;
@@ -286,7 +380,7 @@ int enbl_context_destroy(enbl_context* ctx)
     for (auto& c : ctx->cache) enbl_pyramid_destroy(c.p);
     for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
     if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
-    for (int i = 0; i < 8; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
+    for (int i = 0; i < 16; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return ENBL_OK;
@@ -464,9 +558,8 @@ int enbl_gaussian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, cons
 int enbl_laplacian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
                            const uint8_t* alpha, long apitch, enbl_pyramid* out)
 {
-    int rc = enbl_gaussian_pyramid(ctx, wrap, itype, frac, img, ipitch, alpha, apitch, out);
-    if (rc) return rc;
-    return laplacian_levels(ctx, out, wrap);
+    if (!ctx || !img || !out) return fail(ENBL_E_ARG, "enbl_laplacian_pyramid: null argument");
+    return laplacian_pyramid_any(ctx, wrap, itype, frac, img, ipitch, alpha, apitch, out);
 }
 
 int enbl_collapse_pyramid(enbl_context* ctx, int wrap, enbl_pyramid* p)
@@ -542,19 +635,70 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     if (rc) return rc;
     rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, black, bpitch, balpha, bapitch, blp);
     if (rc) return rc;
-    // Union alpha (src/enblend.h:461-463).
-    {
-        Timed t(ctx, "alpha_union", 3.0 * w * h);
-        launch_alpha_union(walpha, wapitch, balpha, bapitch, w, h, ctx->stream);
-        rc = launch_status("alpha_union");
+    const bool fused0 = !wrap && streaming_level(wlp->lv[1].w, wlp->lv[1].h) && levels > 1;
+    if (!fused0) {
+        // Union alpha (src/enblend.h:461-463).
+        {
+            Timed t(ctx, "alpha_union", 3.0 * w * h);
+            launch_alpha_union(walpha, wapitch, balpha, bapitch, w, h, ctx->stream);
+            rc = launch_status("alpha_union");
+            if (rc) return rc;
+        }
+        // blend with white = 255 << mask fraction bits (src/enblend.h:472-474).
+        rc = enbl_blend(ctx, mgp, wlp, blp, (double)(255 << mfrac));
+        if (rc) return rc;
+        rc = enbl_collapse_pyramid(ctx, wrap, blp);
+        if (rc) return rc;
+        return enbl_copy_from_pyramid_if(ctx, blp, 0, ifrac, balpha, bapitch, black, itype, bpitch, dither, seed);
+    }
+    // Blend each level right before it is collapsed; the large levels blend,
+    // add the expanded coarser level and (level 0) convert in one pass.
+    const double wv = (double)(255 << mfrac);
+    const int esP = ptype_size(pt), esM = ptype_size(mt);
+    for (int l = levels - 1; l >= 0; --l) {
+        const Level& L = blp->lv[l];
+        const bool fused = l + 1 < levels && streaming_level(blp->lv[l + 1].w, blp->lv[l + 1].h);
+        if (!fused) {
+            {
+                Timed t(ctx, "blend", (double)L.w * L.h * (esM + 3.0 * C * esP));
+                rc = launch_blend(mt, pt, C, L.w, L.h, level_ptr(mgp, l), mgp->lv[l].pitch, level_ptr(wlp, l), wlp->lv[l].pitch,
+                                  wlp->lv[l].plane, level_ptr(blp, l), L.pitch, L.plane, wv, ctx->stream);
+                if (rc) return fail(rc, "blend: unsupported types");
+                rc = launch_status("blend");
+                if (rc) return rc;
+            }
+            if (l + 1 < levels) {
+                const Level& Cl = blp->lv[l + 1];
+                rc = do_expand(ctx, pt, C, 1, wrap, level_ptr(blp, l + 1), Cl.w, Cl.h, Cl.pitch, Cl.plane,
+                               level_ptr(blp, l), L.w, L.h, L.pitch, L.plane);
+                if (rc) return rc;
+            }
+            continue;
+        }
+        const Level& Cl = blp->lv[l + 1];
+        FusedCollapseArgs a;
+        a.coarse = level_ptr(blp, l + 1); a.cpitch = Cl.pitch; a.cplane = Cl.plane; a.cw = Cl.w; a.ch = Cl.h;
+        a.black = level_ptr(blp, l); a.fpitch = L.pitch; a.fplane = L.plane;
+        a.white = level_ptr(wlp, l); a.wpitch = wlp->lv[l].pitch; a.wplane = wlp->lv[l].plane;
+        a.fw = L.w; a.fh = L.h;
+        a.white_value = wv;
+        a.img = black; a.ipitch = bpitch; a.ifrac = ifrac;
+        a.walpha = walpha; a.wapitch = wapitch; a.balpha = balpha; a.bapitch = bapitch;
+        a.dither = dither; a.seed = seed; a.rows = 0;
+        if (l == 0) { a.mask = mask; a.mpitch = mpitch; a.mfrac = mfrac; }
+        else { a.mask = level_ptr(mgp, l); a.mpitch = mgp->lv[l].pitch; a.mfrac = 0; }
+        char name[64];
+        snprintf(name, sizeof name, "collapse_level%s_%s_c%d", l == 0 ? "0" : "", pname(pt), C);
+        // credited bytes: blend + expand-add (+ masked conversion and alpha union at level 0)
+        double bytes = (double)L.w * L.h * (esM + 3.0 * C * esP) + C * esP * ((double)Cl.w * Cl.h + 2.0 * L.w * L.h);
+        if (l == 0) bytes += (double)L.w * L.h * (1.0 + C * (esP + ptype_size(itype))) + 3.0 * w * h;
+        Timed t(ctx, name, bytes);
+        rc = launch_collapse_level(pt, mt, l == 0 ? itype : 0, l == 0, a, C, ctx->stream);
+        if (rc) return fail(rc, "collapse level: unsupported types");
+        rc = launch_status(name);
         if (rc) return rc;
     }
-    // blend with white = 255 << mask fraction bits (src/enblend.h:472-474).
-    rc = enbl_blend(ctx, mgp, wlp, blp, (double)(255 << mfrac));
-    if (rc) return rc;
-    rc = enbl_collapse_pyramid(ctx, wrap, blp);
-    if (rc) return rc;
-    return enbl_copy_from_pyramid_if(ctx, blp, 0, ifrac, balpha, bapitch, black, itype, bpitch, dither, seed);
+    return ENBL_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,41 @@
+// Launch arguments of the fused pyramid passes (fused.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace enbl {
+
+struct FusedLapArgs {
+    const void* src; long spitch; long splane; int frac;   // fine level: planes, or interleaved image
+    const uint8_t* salpha; long sapitch;
+    int fw, fh;
+    void* lap; long lpitch; long lplane;                    // Laplacian level out
+    void* coarse; long cpitch; long cplane;                 // next Gaussian level out
+    uint8_t* calpha; long capitch;
+    int cw, ch;
+    int rows;
+};
+
+struct FusedCollapseArgs {
+    const void* coarse; long cpitch; long cplane; int cw, ch;   // collapsed level l+1
+    void* black; long fpitch; long fplane;                      // black Laplacian level l (in/out)
+    const void* white; long wpitch; long wplane;                // white Laplacian level l
+    int fw, fh;
+    const void* mask; long mpitch; int mfrac;                   // mask level l (planes or 0/255 bytes)
+    double white_value;
+    void* img; long ipitch; int ifrac;                          // output image (level 0)
+    const uint8_t* walpha; long wapitch;
+    uint8_t* balpha; long bapitch;
+    int dither; uint64_t seed;
+    int rows;
+};
+
+// src_kind 0: planes of ptype; ENBL_U8/ENBL_U16/ENBL_F32: interleaved image.
+int launch_laplacian_level(int ptype, int src_kind, const FusedLapArgs& a, int C, hipStream_t s);
+// out_itype 0: write the collapsed level into `black`; otherwise convert into
+// the interleaved image of that type (mask given as 0/255 bytes).
+int launch_collapse_level(int ptype, int mtype, int out_itype, bool mask8, const FusedCollapseArgs& a, int C,
+                          hipStream_t s);
+
+}  // namespace enbl

@@ -100,6 +100,12 @@ int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const 
 int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels, int slot, enbl_pyramid** out);
 // Device scratch buffer `slot` of the context, grown to at least `bytes`.
 int scratch(enbl_context* ctx, int slot, size_t bytes, void** out);
+// Whether the pass producing a coarse level of cw x ch takes the streaming
+// (and fused) kernels under the current kernel policy.
+bool streaming_level(int cw, int ch);
+int alpha_pingpong(enbl_context* ctx, const enbl_pyramid* p, uint8_t** a, uint8_t** b, long* pitch);
+int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
+                          const uint8_t* alpha, long apitch, enbl_pyramid* out);
 
 }  // namespace host
 }  // namespace enbl
@@ -115,8 +121,8 @@ struct enbl_context {
     std::vector<Cached> cache;   // pipeline scratch pyramids
     void* scratch_alpha;         // per-level alpha scratch for Gaussian builds
     size_t scratch_alpha_bytes;
-    void* scratch_buf[8];        // general device scratch (enfuse weights)
-    size_t scratch_bytes[8];
+    void* scratch_buf[16];       // general device scratch (weights, fused passes)
+    size_t scratch_bytes[16];
 };
 
 struct enbl_pyramid {

@@ -1,0 +1,96 @@
+// Shared device helpers of the pyramid kernels (pyramid.hip, fused.hip).
+#pragma once
+
+#include "enbl_device.h"
+
+namespace enbl {
+
+// Two adjacent samples (2j, 2j+1) of a row as one vector access; `both`
+// false loads/stores the first only.  Callers keep rows 2-element aligned.
+template <class P> struct Vec2;
+template <> struct Vec2<int16_t> {
+    __device__ static __forceinline__ void load(const int16_t* p, bool both, int32_t& a, int32_t& b) {
+        if (both) {
+            const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+            a = (int16_t)(v & 0xffffu);
+            b = (int16_t)(v >> 16);
+        } else {
+            a = p[0];
+            b = 0;
+        }
+    }
+    __device__ static __forceinline__ void store(int16_t* p, bool both, int16_t a, int16_t b) {
+        if (both) {
+            *reinterpret_cast<uint32_t*>(p) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16);
+        } else {
+            p[0] = a;
+        }
+    }
+};
+template <> struct Vec2<int32_t> {
+    __device__ static __forceinline__ void load(const int32_t* p, bool both, int32_t& a, int32_t& b) {
+        if (both) {
+            const int2 v = *reinterpret_cast<const int2*>(p);
+            a = v.x;
+            b = v.y;
+        } else {
+            a = p[0];
+            b = 0;
+        }
+    }
+    __device__ static __forceinline__ void store(int32_t* p, bool both, int32_t a, int32_t b) {
+        if (both) {
+            *reinterpret_cast<int2*>(p) = make_int2(a, b);
+        } else {
+            p[0] = a;
+        }
+    }
+};
+template <> struct Vec2<double> {
+    __device__ static __forceinline__ void load(const double* p, bool both, double& a, double& b) {
+        if (both) {
+            const double2 v = *reinterpret_cast<const double2*>(p);
+            a = v.x;
+            b = v.y;
+        } else {
+            a = p[0];
+            b = 0.0;
+        }
+    }
+    __device__ static __forceinline__ void store(double* p, bool both, double a, double b) {
+        if (both) {
+            *reinterpret_cast<double2*>(p) = make_double2(a, b);
+        } else {
+            p[0] = a;
+        }
+    }
+};
+
+__device__ __forceinline__ int wrap_index(int k, int n) {
+    k %= n;
+    return k < 0 ? k + n : k;
+}
+
+// Horizontal weight sum behind the reference's SKIPSM_* scale constants for
+// destination column X (src/pyramid.h:1199-1411).  A 1-pixel-wide source
+// always takes the no-wraparound constants.
+__device__ __forceinline__ int expand_hsum(int X, int sw, bool wrap, bool dst_even) {
+    const int j = X >> 1;
+    if (sw == 1) return (X & 1) ? 4 : 6;
+    if ((X & 1) == 0) {
+        if (!wrap) return 6 + (j > 0 ? 1 : 0) + (j < sw - 1 ? 1 : 0);
+        if (dst_even) return 8;
+        return 6 + (j == 0 ? 4 : 1) + (j == sw - 1 ? 4 : 1);
+    }
+    if (!wrap) return 4 + (j < sw - 1 ? 4 : 0);
+    return 8;
+}
+
+__device__ __forceinline__ int expand_vsum(int Y, int sh) {
+    const int i = Y >> 1;
+    if ((Y & 1) == 0) return 6 + (i > 0 ? 1 : 0) + (i < sh - 1 ? 1 : 0);
+    return 1 + (i < sh - 1 ? 1 : 0);
+}
+
+
+}  // namespace enbl

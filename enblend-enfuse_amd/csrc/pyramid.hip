@@ -18,73 +18,10 @@
 // path exact.
 #include "enbl_device.h"
 #include "enbl_internal.h"
+#include "pyr_common.h"
 
 namespace enbl {
 namespace {
-
-template <class P> struct Vec2;
-template <> struct Vec2<int16_t> {
-    __device__ static __forceinline__ void load(const int16_t* p, bool both, int32_t& a, int32_t& b) {
-        if (both) {
-            const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
-            a = (int16_t)(v & 0xffffu);
-            b = (int16_t)(v >> 16);
-        } else {
-            a = p[0];
-            b = 0;
-        }
-    }
-    __device__ static __forceinline__ void store(int16_t* p, bool both, int16_t a, int16_t b) {
-        if (both) {
-            *reinterpret_cast<uint32_t*>(p) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16);
-        } else {
-            p[0] = a;
-        }
-    }
-};
-template <> struct Vec2<int32_t> {
-    __device__ static __forceinline__ void load(const int32_t* p, bool both, int32_t& a, int32_t& b) {
-        if (both) {
-            const int2 v = *reinterpret_cast<const int2*>(p);
-            a = v.x;
-            b = v.y;
-        } else {
-            a = p[0];
-            b = 0;
-        }
-    }
-    __device__ static __forceinline__ void store(int32_t* p, bool both, int32_t a, int32_t b) {
-        if (both) {
-            *reinterpret_cast<int2*>(p) = make_int2(a, b);
-        } else {
-            p[0] = a;
-        }
-    }
-};
-template <> struct Vec2<double> {
-    __device__ static __forceinline__ void load(const double* p, bool both, double& a, double& b) {
-        if (both) {
-            const double2 v = *reinterpret_cast<const double2*>(p);
-            a = v.x;
-            b = v.y;
-        } else {
-            a = p[0];
-            b = 0.0;
-        }
-    }
-    __device__ static __forceinline__ void store(double* p, bool both, double a, double b) {
-        if (both) {
-            *reinterpret_cast<double2*>(p) = make_double2(a, b);
-        } else {
-            p[0] = a;
-        }
-    }
-};
-
-__device__ __forceinline__ int wrap_index(int k, int n) {
-    k %= n;
-    return k < 0 ? k + n : k;
-}
 
 // ---------------------------------------------------------------------------
 // reduce
@@ -306,27 +243,6 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
 // expand
 // ---------------------------------------------------------------------------
 
-// Horizontal weight sum behind the reference's SKIPSM_* scale constants for
-// destination column X (src/pyramid.h:1199-1411).  A 1-pixel-wide source
-// always takes the no-wraparound constants.
-__device__ __forceinline__ int expand_hsum(int X, int sw, bool wrap, bool dst_even) {
-    const int j = X >> 1;
-    if (sw == 1) return (X & 1) ? 4 : 6;
-    if ((X & 1) == 0) {
-        if (!wrap) return 6 + (j > 0 ? 1 : 0) + (j < sw - 1 ? 1 : 0);
-        if (dst_even) return 8;
-        return 6 + (j == 0 ? 4 : 1) + (j == sw - 1 ? 4 : 1);
-    }
-    if (!wrap) return 4 + (j < sw - 1 ? 4 : 0);
-    return 8;
-}
-
-__device__ __forceinline__ int expand_vsum(int Y, int sh) {
-    const int i = Y >> 1;
-    if ((Y & 1) == 0) return 6 + (i > 0 ? 1 : 0) + (i < sh - 1 ? 1 : 0);
-    return 1 + (i < sh - 1 ? 1 : 0);
-}
-
 template <class S, int C, bool WRAP>
 struct ExpandRow {
     S x[C];
@@ -542,9 +458,28 @@ __global__ __launch_bounds__(256) void reduce_small_kernel(ReduceArgs a)
         in[k] = WRAP || (x >= 0 && x < sw);
         col[k] = WRAP ? wrap_index(x, sw) : (x < 0 ? 0 : (x >= sw ? sw - 1 : x));
     }
+    // Issue every load of the 5x5 window first (clamped indices are always
+    // inside the level), then evaluate the literal boundary forms.
+    const int r = 2 * i;
+    int rowi[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int y = r - 2 + k;
+        rowi[k] = y < 0 ? 0 : (y >= sh ? sh - 1 : y);
+    }
+    uint8_t al[5][5];
+    S raw[5][5][C];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            al[k][t] = ALPHA ? a.salpha[(long)rowi[k] * a.sapitch + col[t]] : (uint8_t)1;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) raw[k][t][ch] = Px::load(src[ch * a.splane + (long)rowi[k] * a.spitch + col[t]]);
+        }
+    }
     S hrow[5][C];
     int mrow[5];
-    const int r = 2 * i;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int y = r - 2 + k;
@@ -553,9 +488,8 @@ __global__ __launch_bounds__(256) void reduce_small_kernel(ReduceArgs a)
         for (int ch = 0; ch < C; ++ch) hrow[k][ch] = A::zero();
         if (y < 0 || y >= sh) continue;
         int m[5];
-        const uint8_t* ar = ALPHA ? a.salpha + (long)y * a.sapitch : nullptr;
 #pragma unroll
-        for (int t = 0; t < 5; ++t) m[t] = ALPHA ? ((in[t] && ar[col[t]]) ? 1 : 0) : 1;
+        for (int t = 0; t < 5; ++t) m[t] = ALPHA ? ((in[t] && al[k][t]) ? 1 : 0) : 1;
         if (ALPHA) {
             int hm = m[0] + 4 * m[1] + 6 * m[2];
             if (rcase != 1) hm += 4 * m[3];
@@ -564,12 +498,11 @@ __global__ __launch_bounds__(256) void reduce_small_kernel(ReduceArgs a)
         }
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            const P* row = src + ch * a.splane + (long)y * a.spitch;
             S v[5];
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
-                if (ALPHA) v[t] = m[t] ? Px::load(row[col[t]]) : A::zero();
-                else v[t] = (WRAP || in[t] || t < 2) ? Px::load(row[col[t]]) : A::zero();
+                if (ALPHA) v[t] = m[t] ? raw[k][t][ch] : A::zero();
+                else v[t] = (WRAP || in[t] || t < 2) ? raw[k][t][ch] : A::zero();
             }
             const S a0 = A::add(v[0], A::mul(v[1], 4));
             S hv;
@@ -709,7 +642,7 @@ __global__ __launch_bounds__(256) void expand_small_kernel(ExpandArgs a)
 
 // Levels of at most this many (reduce: destination, expand: source) pixels
 // take the direct kernels.
-constexpr long kSmallLevel = 1L << 16;
+constexpr long kSmallLevel = 1L << 20;
 
 // Rows of output per strip: enough strips to keep every SIMD busy, but long
 // enough that the 3-row (reduce) / 2-row (expand) strip halo stays small.

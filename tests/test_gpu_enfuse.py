@@ -33,6 +33,13 @@ def _stack(rng, itype, K, h, w, scene=True):
     return imgs
 
 
+@pytest.fixture(params=[0, 2], ids=["auto", "streaming"])
+def policy(request, enbl):
+    enbl.check(enbl.lib().enbl_set_kernel_policy(request.param))
+    yield request.param
+    enbl.check(enbl.lib().enbl_set_kernel_policy(0))
+
+
 CASES = [
     dict(w_exposure=1.0, w_saturation=0.2),
     dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3, contrast_window=5, entropy_window=3),
@@ -65,7 +72,7 @@ def test_enfuse_mask_matches_oracle(enbl, gpu_ctx, oracle, itype, case):
 
 @pytest.mark.parametrize("itype", [U8, U16, F32])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, case):
+def test_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, case, policy):
     rng = np.random.default_rng(800 + itype + case)
     for (h, w, K) in [(64, 96, 3), (133, 101, 4)]:
         imgs = _stack(rng, itype, K, h, w)
