@@ -51,10 +51,26 @@ template <class I, class P> struct IntImageSrc {
     template <int C>
     __device__ static __forceinline__ void load(const FusedLapArgs& a, int r, int c, bool both, S (&x0)[C], S (&x1)[C]) {
         const I* p = static_cast<const I*>(a.src) + (long)r * a.spitch + (long)c * C;
+        I v[2 * C];
+        if constexpr (sizeof(I) == 2 && C == 3) {
+            // two RGB16 pixels = 12 bytes: one dwordx3 access when aligned
+            if (both && ((uintptr_t)p & 3) == 0) {
+                const uint3 d = *reinterpret_cast<const uint3*>(p);
+                v[0] = (I)(d.x & 0xffffu); v[1] = (I)(d.x >> 16);
+                v[2] = (I)(d.y & 0xffffu); v[3] = (I)(d.y >> 16);
+                v[4] = (I)(d.z & 0xffffu); v[5] = (I)(d.z >> 16);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+        }
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            x0[ch] = (S)(P)((P)p[ch] << a.frac);
-            x1[ch] = both ? (S)(P)((P)p[C + ch] << a.frac) : (S)0;
+            x0[ch] = (S)(P)((P)v[ch] << a.frac);
+            x1[ch] = both ? (S)(P)((P)v[C + ch] << a.frac) : (S)0;
         }
     }
 };
@@ -423,6 +439,22 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
             IT* img = static_cast<IT*>(a.img) + (long)Y * a.ipitch + (long)X * C;
             const bool u0 = F.wa0 || F.ba0;
             const bool u1 = F.wa1 || F.ba1;
+            if constexpr (sizeof(IT) == 2 && C == 3) {
+                if (u0 && xb && u1 && ((uintptr_t)img & 3) == 0) {
+                    IT v[6];
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        v[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, a.dither, a.seed, X, Y, ch);
+                        v[3 + ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, a.dither, a.seed, X + 1, Y, ch);
+                    }
+                    *reinterpret_cast<uint3*>(img) = make_uint3((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                                                                (uint32_t)v[2] | ((uint32_t)v[3] << 16),
+                                                                (uint32_t)v[4] | ((uint32_t)v[5] << 16));
+                    if (F.wa0) a.balpha[(long)Y * a.bapitch + X] = 255;
+                    if (F.wa1) a.balpha[(long)Y * a.bapitch + X + 1] = 255;
+                    return;
+                }
+            }
             if (u0) {
 #pragma unroll
                 for (int ch = 0; ch < C; ++ch) img[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, a.dither, a.seed, X, Y, ch);
