@@ -400,6 +400,34 @@ int enbl_context_get_stream(const enbl_context* ctx, void** s)
     return ENBL_OK;
 }
 
+int enbl_device_alloc(enbl_context* ctx, size_t bytes, void** out)
+{
+    if (!ctx || !out) return fail(ENBL_E_ARG, "enbl_device_alloc: null argument");
+    *out = nullptr;
+    if (bytes == 0) return ENBL_OK;
+    int rc = hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (rc) return rc;
+    return hip_check(hipMalloc(out, bytes), "enbl_device_alloc");
+}
+
+int enbl_device_free(enbl_context* ctx, void* p)
+{
+    if (!ctx) return fail(ENBL_E_ARG, "enbl_device_free: null context");
+    if (!p) return ENBL_OK;
+    int rc = hip_check(hipStreamSynchronize(ctx->stream), "enbl_device_free");
+    if (rc) return rc;
+    return hip_check(hipFree(p), "enbl_device_free");
+}
+
+int enbl_copy(enbl_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ENBL_E_ARG, "enbl_copy: null argument");
+    if (bytes == 0) return ENBL_OK;
+    int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "enbl_copy");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(ctx->stream), "enbl_copy");
+}
+
 int enbl_set_kernel_policy(int policy)
 {
     if (policy < ENBL_KERNELS_AUTO || policy > ENBL_KERNELS_STREAMING) return fail(ENBL_E_ARG, "kernel policy %d", policy);

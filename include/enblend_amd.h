@@ -79,6 +79,13 @@ const char* enbl_last_error(void);
 int enbl_profile_enable(enbl_context* ctx, int on);
 int enbl_profile_report(enbl_context* ctx, char* buf, size_t buflen);
 
+/* Device memory of the context's device, for hosts that do not link HIP
+ * themselves (e.g. the C++ drop-in headers).  enbl_copy is ordered on the
+ * context's stream and returns after the copy completed. */
+int enbl_device_alloc(enbl_context* ctx, size_t bytes, void** out);
+int enbl_device_free(enbl_context* ctx, void* p);
+int enbl_copy(enbl_context* ctx, void* dst, const void* src, size_t bytes);
+
 /* ---- single-level operations ---------------------------------------- */
 
 /* Replaces reduce<SKIPSMImagePixelType, SKIPSMAlphaPixelType>(wraparound,

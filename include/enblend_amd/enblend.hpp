@@ -1,0 +1,459 @@
+// C++ drop-in for the multiresolution path of enblend/enfuse.
+//
+// The reference implements this path as header templates in namespace
+// enblend over VIGRA image traversers and accessors.  This header keeps the
+// names, template parameters and argument lists of those templates, so the
+// reference's callers (enblendMain, enfuseMain) compile against it
+// unchanged, and runs every operation through the C ABI of libenblend_amd
+// (include/enblend_amd.h) on the GPU.  Host images stay caller-owned: each
+// call stages the traversed region to device memory and writes the results
+// back through the destination accessor.
+//
+// Requirements on the image types (all met by vigra::BasicImage):
+//   traverser: members x and y that support +=, and the accessor call
+//   acc(traverser); accessor: value_type, operator()(traverser) and
+//   set(value, traverser); image types: constructor (width, height),
+//   width(), height(), upperLeft(), lowerRight(), accessor().
+// Pixel values are either arithmetic scalars or 3-component vectors with
+// operator[] and a 3-argument constructor (vigra::RGBValue).
+//
+// Errors: a failing status raises enblend_amd::error, which derives from
+// std::runtime_error (the reference raises vigra::PreconditionViolation or
+// std::bad_alloc, and both end in its main's exception handlers).
+#pragma once
+
+#include <stdint.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../enblend_amd.h"
+
+namespace enblend_amd {
+
+struct error : public std::runtime_error {
+    int code;
+    error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc)
+{
+    if (rc != ENBL_OK) throw error(rc, enbl_last_error());
+}
+
+// One context per process, on HIP device 0 unless set_device() came first.
+inline int& device_ordinal()
+{
+    static int d = 0;
+    return d;
+}
+
+inline void set_device(int d) { device_ordinal() = d; }
+
+inline enbl_context* context()
+{
+    struct Holder {
+        enbl_context* c = nullptr;
+        ~Holder() { if (c) enbl_context_destroy(c); }
+    };
+    static Holder h;
+    if (!h.c) check(enbl_context_create(device_ordinal(), &h.c));
+    return h.c;
+}
+
+// Device buffer owned by the context's device.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes) { if (bytes) check(enbl_device_alloc(context(), bytes, &p_)); }
+    ~DeviceBuffer() { if (p_) enbl_device_free(context(), p_); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    void* get() const { return p_; }
+    void upload(const void* src, size_t bytes) { check(enbl_copy(context(), p_, src, bytes)); }
+    void download(void* dst, size_t bytes) const { check(enbl_copy(context(), dst, p_, bytes)); }
+
+private:
+    void* p_ = nullptr;
+};
+
+// ---- pixel type mapping ---------------------------------------------------
+template <class T, class = void> struct pixel {
+    // 3-component vector pixel (vigra::RGBValue)
+    typedef typename std::decay<decltype(std::declval<T>()[0])>::type component;
+    static constexpr int channels = 3;
+    static component get(const T& v, int c) { return v[c]; }
+    static T make(const component* c) { return T(c[0], c[1], c[2]); }
+};
+template <class T> struct pixel<T, typename std::enable_if<std::is_arithmetic<T>::value>::type> {
+    typedef T component;
+    static constexpr int channels = 1;
+    static component get(const T& v, int) { return v; }
+    static T make(const component* c) { return c[0]; }
+};
+
+template <class T> constexpr int pyramid_code()
+{
+    return std::is_same<T, int16_t>::value ? ENBL_I16
+         : std::is_same<T, int32_t>::value ? ENBL_I32
+         : std::is_same<T, double>::value ? ENBL_F64 : -1;
+}
+
+template <class T> constexpr int image_code()
+{
+    return std::is_same<T, uint8_t>::value ? ENBL_U8
+         : std::is_same<T, uint16_t>::value ? ENBL_U16
+         : std::is_same<T, float>::value ? ENBL_F32 : -1;
+}
+
+// Read a traversed region into an interleaved (or, for `planar`, per-channel)
+// host buffer of component type Out.
+template <class Out, class It, class Acc>
+std::vector<Out> gather(It ul, int w, int h, Acc acc, bool planar)
+{
+    typedef typename Acc::value_type V;
+    const int C = pixel<V>::channels;
+    std::vector<Out> buf((size_t)w * h * C);
+    It row = ul;
+    for (int y = 0; y < h; ++y, ++row.y) {
+        It p = row;
+        for (int x = 0; x < w; ++x, ++p.x) {
+            const V v = acc(p);
+            for (int c = 0; c < C; ++c) {
+                const size_t i = planar ? ((size_t)c * h + y) * w + x : ((size_t)y * w + x) * C + c;
+                buf[i] = (Out)pixel<V>::get(v, c);
+            }
+        }
+    }
+    return buf;
+}
+
+template <class In, class It, class Acc>
+void scatter(const std::vector<In>& buf, It ul, int w, int h, Acc acc, bool planar, const uint8_t* mask = nullptr)
+{
+    typedef typename Acc::value_type V;
+    typedef typename pixel<V>::component Comp;
+    const int C = pixel<V>::channels;
+    It row = ul;
+    for (int y = 0; y < h; ++y, ++row.y) {
+        It p = row;
+        for (int x = 0; x < w; ++x, ++p.x) {
+            if (mask && !mask[(size_t)y * w + x]) continue;
+            Comp comp[3];
+            for (int c = 0; c < C; ++c) {
+                const size_t i = planar ? ((size_t)c * h + y) * w + x : ((size_t)y * w + x) * C + c;
+                comp[c] = (Comp)buf[i];
+            }
+            acc.set(pixel<V>::make(comp), p);
+        }
+    }
+}
+
+template <class It, class Acc>
+std::vector<uint8_t> gather_mask(It ul, int w, int h, Acc acc)
+{
+    std::vector<uint8_t> m((size_t)w * h);
+    It row = ul;
+    for (int y = 0; y < h; ++y, ++row.y) {
+        It p = row;
+        for (int x = 0; x < w; ++x, ++p.x) m[(size_t)y * w + x] = acc(p) ? 255 : 0;
+    }
+    return m;
+}
+
+// Device pyramid wrapper.
+class Pyramid {
+public:
+    Pyramid(int ptype, int C, int w, int h, int levels) { check(enbl_pyramid_create(context(), ptype, C, w, h, levels, &p_)); }
+    ~Pyramid() { if (p_) enbl_pyramid_destroy(p_); }
+    Pyramid(const Pyramid&) = delete;
+    Pyramid& operator=(const Pyramid&) = delete;
+    enbl_pyramid* get() const { return p_; }
+    void dims(int l, int* w, int* h) const { check(enbl_pyramid_level(p_, l, nullptr, w, h, nullptr, nullptr)); }
+    template <class T> std::vector<T> read(int l, int C) const {
+        int w, h;
+        dims(l, &w, &h);
+        std::vector<T> v((size_t)w * h * C);
+        check(enbl_pyramid_copy_level(p_, l, v.data(), 0));
+        return v;
+    }
+    template <class T> void write(int l, const std::vector<T>& v) { check(enbl_pyramid_copy_level(p_, l, const_cast<T*>(v.data()), 1)); }
+
+private:
+    enbl_pyramid* p_ = nullptr;
+};
+
+// Host pyramid levels <-> device pyramid.
+template <class PyramidImageType>
+Pyramid* to_device(const std::vector<PyramidImageType*>& levels)
+{
+    typedef typename PyramidImageType::value_type V;
+    typedef typename pixel<V>::component Comp;
+    const int C = pixel<V>::channels;
+    Pyramid* p = new Pyramid(pyramid_code<Comp>(), C, levels[0]->width(), levels[0]->height(), (int)levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        PyramidImageType& im = *levels[l];
+        p->write(l, gather<Comp>(im.upperLeft(), im.width(), im.height(), im.accessor(), true));
+    }
+    return p;
+}
+
+template <class PyramidImageType>
+void from_device(const Pyramid& p, std::vector<PyramidImageType*>& levels)
+{
+    typedef typename PyramidImageType::value_type V;
+    typedef typename pixel<V>::component Comp;
+    const int C = pixel<V>::channels;
+    for (size_t l = 0; l < levels.size(); ++l) {
+        PyramidImageType& im = *levels[l];
+        scatter(p.read<Comp>((int)l, C), im.upperLeft(), im.width(), im.height(), im.accessor(), true);
+    }
+}
+
+template <class PyramidImageType>
+std::vector<PyramidImageType*>* new_levels(const Pyramid& p, int levels)
+{
+    auto* v = new std::vector<PyramidImageType*>();
+    for (int l = 0; l < levels; ++l) {
+        int w, h;
+        p.dims(l, &w, &h);
+        v->push_back(new PyramidImageType(w, h));
+    }
+    return v;
+}
+
+}  // namespace enblend_amd
+
+namespace enblend {
+
+/** Replaces filterHalfWidth (src/pyramid.h:56-64). */
+inline unsigned int filterHalfWidth(const unsigned int levels)
+{
+    if (levels < 1 || levels > 29) throw enblend_amd::error(ENBL_E_PRECONDITION, "filterHalfWidth: levels outside of permissible range");
+    return levels == 1 ? 0 : (1u << (levels + 1)) - 4;
+}
+
+/** Replaces gaussianPyramid with alpha (src/pyramid.h:1467-1559). */
+template <typename SrcImageType, typename AlphaImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType>
+std::vector<PyramidImageType*>*
+gaussianPyramid(unsigned int numLevels, bool wraparound,
+                typename SrcImageType::const_traverser src_upperleft,
+                typename SrcImageType::const_traverser src_lowerright,
+                typename SrcImageType::ConstAccessor sa,
+                typename AlphaImageType::const_traverser alpha_upperleft,
+                typename AlphaImageType::ConstAccessor aa)
+{
+    using namespace enblend_amd;
+    typedef typename SrcImageType::value_type SV;
+    typedef typename pixel<SV>::component SComp;
+    typedef typename pixel<typename PyramidImageType::value_type>::component PComp;
+    const int w = src_lowerright.x - src_upperleft.x, h = src_lowerright.y - src_upperleft.y;
+    const int C = pixel<SV>::channels;
+    std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
+    std::vector<uint8_t> al = gather_mask(alpha_upperleft, w, h, aa);
+    DeviceBuffer dimg(img.size() * sizeof(SComp)), dal(al.size());
+    dimg.upload(img.data(), img.size() * sizeof(SComp));
+    dal.upload(al.data(), al.size());
+    Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
+    check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
+                                (const uint8_t*)dal.get(), w, p.get()));
+    auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
+    from_device(p, *out);
+    return out;
+}
+
+/** Replaces gaussianPyramid without alpha (src/pyramid.h:1562-1636). */
+template <typename SrcImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType>
+std::vector<PyramidImageType*>*
+gaussianPyramid(unsigned int numLevels, bool wraparound,
+                typename SrcImageType::const_traverser src_upperleft,
+                typename SrcImageType::const_traverser src_lowerright,
+                typename SrcImageType::ConstAccessor sa)
+{
+    using namespace enblend_amd;
+    typedef typename SrcImageType::value_type SV;
+    typedef typename pixel<SV>::component SComp;
+    typedef typename pixel<typename PyramidImageType::value_type>::component PComp;
+    const int w = src_lowerright.x - src_upperleft.x, h = src_lowerright.y - src_upperleft.y;
+    const int C = pixel<SV>::channels;
+    std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
+    DeviceBuffer dimg(img.size() * sizeof(SComp));
+    dimg.upload(img.data(), img.size() * sizeof(SComp));
+    Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
+    check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
+                                nullptr, 0, p.get()));
+    auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
+    from_device(p, *out);
+    return out;
+}
+
+/** Replaces laplacianPyramid (src/pyramid.h:1646-1747). */
+template <typename SrcImageType, typename AlphaImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType>
+std::vector<PyramidImageType*>*
+laplacianPyramid(const char* /*exportName*/, unsigned int numLevels, bool wraparound,
+                 typename SrcImageType::const_traverser src_upperleft,
+                 typename SrcImageType::const_traverser src_lowerright,
+                 typename SrcImageType::ConstAccessor sa,
+                 typename AlphaImageType::const_traverser alpha_upperleft,
+                 typename AlphaImageType::ConstAccessor aa)
+{
+    using namespace enblend_amd;
+    typedef typename SrcImageType::value_type SV;
+    typedef typename pixel<SV>::component SComp;
+    typedef typename pixel<typename PyramidImageType::value_type>::component PComp;
+    const int w = src_lowerright.x - src_upperleft.x, h = src_lowerright.y - src_upperleft.y;
+    const int C = pixel<SV>::channels;
+    std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
+    std::vector<uint8_t> al = gather_mask(alpha_upperleft, w, h, aa);
+    DeviceBuffer dimg(img.size() * sizeof(SComp)), dal(al.size());
+    dimg.upload(img.data(), img.size() * sizeof(SComp));
+    dal.upload(al.data(), al.size());
+    Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
+    check(enbl_laplacian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
+                                 (const uint8_t*)dal.get(), w, p.get()));
+    auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
+    from_device(p, *out);
+    return out;
+}
+
+/** Replaces collapsePyramid (src/pyramid.h:1757-1784); in place. */
+template <typename SKIPSMImagePixelType, typename PyramidImageType>
+void collapsePyramid(bool wraparound, std::vector<PyramidImageType*>* p)
+{
+    using namespace enblend_amd;
+    std::unique_ptr<Pyramid> d(to_device(*p));
+    check(enbl_collapse_pyramid(context(), wraparound, d->get()));
+    from_device(*d, *p);
+}
+
+/** Replaces blend (src/blend.h:81-121); writes blackLP. */
+template <typename MaskPyramidType, typename ImagePyramidType>
+void blend(std::vector<MaskPyramidType*>* maskGP, std::vector<ImagePyramidType*>* whiteLP,
+           std::vector<ImagePyramidType*>* blackLP, typename MaskPyramidType::value_type maskPyramidWhiteValue)
+{
+    using namespace enblend_amd;
+    std::unique_ptr<Pyramid> m(to_device(*maskGP)), wl(to_device(*whiteLP)), bl(to_device(*blackLP));
+    check(enbl_blend(context(), m->get(), wl->get(), bl->get(), (double)maskPyramidWhiteValue));
+    from_device(*bl, *blackLP);
+}
+
+/** Replaces reduce with alpha (src/pyramid.h:166-573). */
+template <typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType,
+          typename SrcImageIterator, typename SrcAccessor,
+          typename AlphaIterator, typename AlphaAccessor,
+          typename DestImageIterator, typename DestAccessor,
+          typename DestAlphaIterator, typename DestAlphaAccessor>
+void reduce(bool wraparound,
+            SrcImageIterator src_upperleft, SrcImageIterator src_lowerright, SrcAccessor sa,
+            AlphaIterator alpha_upperleft, AlphaAccessor aa,
+            DestImageIterator dest_upperleft, DestImageIterator dest_lowerright, DestAccessor da,
+            DestAlphaIterator dest_alpha_upperleft, DestAlphaIterator /*dest_alpha_lowerright*/, DestAlphaAccessor daa)
+{
+    using namespace enblend_amd;
+    typedef typename SrcAccessor::value_type SV;
+    typedef typename pixel<SV>::component Comp;
+    const int C = pixel<SV>::channels;
+    const int sw = src_lowerright.x - src_upperleft.x, sh = src_lowerright.y - src_upperleft.y;
+    const int dw = dest_lowerright.x - dest_upperleft.x, dh = dest_lowerright.y - dest_upperleft.y;
+    const long sp = (sw + 1) & ~1, dp = (dw + 1) & ~1;
+    std::vector<Comp> src((size_t)C * sp * sh, Comp(0));
+    std::vector<Comp> g = gather<Comp>(src_upperleft, sw, sh, sa, true);
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < sh; ++y)
+            for (int x = 0; x < sw; ++x) src[((size_t)c * sh + y) * sp + x] = g[((size_t)c * sh + y) * sw + x];
+    std::vector<uint8_t> al = gather_mask(alpha_upperleft, sw, sh, aa);
+    DeviceBuffer ds(src.size() * sizeof(Comp)), dal(al.size()), dd((size_t)C * dp * dh * sizeof(Comp)), dda((size_t)dw * dh);
+    ds.upload(src.data(), src.size() * sizeof(Comp));
+    dal.upload(al.data(), al.size());
+    check(enbl_reduce(context(), pyramid_code<Comp>(), C, wraparound, ds.get(), sw, sh, sp, sp * sh,
+                      (const uint8_t*)dal.get(), sw, dd.get(), dw, dh, dp, dp * dh, (uint8_t*)dda.get(), dw));
+    std::vector<Comp> out((size_t)C * dp * dh), packed((size_t)C * dw * dh);
+    dd.download(out.data(), out.size() * sizeof(Comp));
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < dh; ++y)
+            for (int x = 0; x < dw; ++x) packed[((size_t)c * dh + y) * dw + x] = out[((size_t)c * dh + y) * dp + x];
+    scatter(packed, dest_upperleft, dw, dh, da, true);
+    std::vector<uint8_t> oa((size_t)dw * dh);
+    dda.download(oa.data(), oa.size());
+    DestAlphaIterator row = dest_alpha_upperleft;
+    for (int y = 0; y < dh; ++y, ++row.y) {
+        DestAlphaIterator p = row;
+        for (int x = 0; x < dw; ++x, ++p.x) daa.set(oa[(size_t)y * dw + x], p);
+    }
+}
+
+/** Replaces expand (src/pyramid.h:1102-1419); the combine functor is the one
+ * the reference's factory picks for `add` (src/pyramid.h:1439-1457). */
+template <typename SKIPSMImagePixelType,
+          typename SrcImageIterator, typename SrcAccessor,
+          typename DestImageIterator, typename DestAccessor,
+          typename CombineFunctor>
+void expand(bool add, bool wraparound,
+            SrcImageIterator src_upperleft, SrcImageIterator src_lowerright, SrcAccessor sa,
+            DestImageIterator dest_upperleft, DestImageIterator dest_lowerright, DestAccessor da,
+            CombineFunctor)
+{
+    using namespace enblend_amd;
+    typedef typename DestAccessor::value_type DV;
+    typedef typename pixel<DV>::component Comp;
+    const int C = pixel<DV>::channels;
+    const int sw = src_lowerright.x - src_upperleft.x, sh = src_lowerright.y - src_upperleft.y;
+    const int dw = dest_lowerright.x - dest_upperleft.x, dh = dest_lowerright.y - dest_upperleft.y;
+    const long dp = (dw + 1) & ~1;
+    std::vector<Comp> s = gather<Comp>(src_upperleft, sw, sh, sa, true);
+    std::vector<Comp> d0 = gather<Comp>(dest_upperleft, dw, dh, da, true), d((size_t)C * dp * dh, Comp(0));
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < dh; ++y)
+            for (int x = 0; x < dw; ++x) d[((size_t)c * dh + y) * dp + x] = d0[((size_t)c * dh + y) * dw + x];
+    DeviceBuffer ds(s.size() * sizeof(Comp)), dd(d.size() * sizeof(Comp));
+    ds.upload(s.data(), s.size() * sizeof(Comp));
+    dd.upload(d.data(), d.size() * sizeof(Comp));
+    check(enbl_expand(context(), pyramid_code<Comp>(), C, add, wraparound, ds.get(), sw, sh, sw, (long)sw * sh,
+                      dd.get(), dw, dh, dp, dp * dh));
+    dd.download(d.data(), d.size() * sizeof(Comp));
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < dh; ++y)
+            for (int x = 0; x < dw; ++x) d0[((size_t)c * dh + y) * dw + x] = d[((size_t)c * dh + y) * dp + x];
+    scatter(d0, dest_upperleft, dw, dh, da, true);
+}
+
+/** Replaces copyFromPyramidImageIf (src/fixmath.h:2157-2294), identity
+ * colour space; integer pyramids dither deterministically (seed 0). */
+template <typename PyramidImageType, typename MaskImageType, typename DestImageType,
+          int PyramidIntegerBits, int PyramidFractionBits>
+void copyFromPyramidImageIf(typename PyramidImageType::const_traverser src_upperleft,
+                            typename PyramidImageType::const_traverser src_lowerright,
+                            typename PyramidImageType::ConstAccessor sa,
+                            typename MaskImageType::const_traverser mask_upperleft,
+                            typename MaskImageType::ConstAccessor ma,
+                            typename DestImageType::traverser dest_upperleft,
+                            typename DestImageType::Accessor da)
+{
+    using namespace enblend_amd;
+    typedef typename PyramidImageType::value_type PV;
+    typedef typename pixel<PV>::component PComp;
+    typedef typename pixel<typename DestImageType::value_type>::component DComp;
+    const int C = pixel<PV>::channels;
+    const int w = src_lowerright.x - src_upperleft.x, h = src_lowerright.y - src_upperleft.y;
+    Pyramid p(pyramid_code<PComp>(), C, w, h, 1);
+    p.write(0, gather<PComp>(src_upperleft, w, h, sa, true));
+    std::vector<uint8_t> m = gather_mask(mask_upperleft, w, h, ma);
+    DeviceBuffer dm(m.size()), dimg((size_t)w * h * C * sizeof(DComp));
+    dm.upload(m.data(), m.size());
+    check(enbl_copy_from_pyramid_if(context(), p.get(), 0, PyramidFractionBits, (const uint8_t*)dm.get(), w, dimg.get(),
+                                    image_code<DComp>(), (long)w * C, ENBL_DITHER_DET, 0));
+    std::vector<DComp> out((size_t)w * h * C);
+    dimg.download(out.data(), out.size() * sizeof(DComp));
+    scatter(out, dest_upperleft, w, h, da, false, m.data());
+}
+
+}  // namespace enblend
