@@ -51,6 +51,53 @@ def cpu_baseline(white, wa, black, ba, mask, levels, repeats):
     return times[len(times) // 2], o.lib.orc_num_threads(), out
 
 
+def enfuse_workload(args, enbl, scenes, np, torch):
+    """enfuse timings: the 3 x 1024x768 u8 stack, or the 64-stack uint16
+    batch through the plain per-stack path."""
+    ctx = enbl.Context(0)
+    dev = torch.device("cuda:0")
+    prm = enbl.enfuse_params(w_exposure=1.0, w_saturation=0.2)
+    if args.workload == "enfuse-c0":
+        imgs = [torch.from_numpy(i).to(dev) for i in scenes.exposure_stack(1024, 768, 3, 0, bits=8)]
+        stacks = [(imgs, enbl.U8)]
+    else:
+        stacks = []
+        for s, (w, h, K, ev) in enumerate(scenes.batch_plan()):
+            stacks.append((scenes.device_stack(torch, w, h, K, 1000 + s, ev, bits=16), enbl.U16))
+    prepared = []
+    mpix = 0.0
+    for imgs, itype in stacks:
+        h, w = imgs[0].shape[0], imgs[0].shape[1]
+        alphas = [torch.full((h, w), 255, dtype=torch.uint8, device=dev) for _ in imgs]
+        out = torch.zeros_like(imgs[0])
+        oa = torch.zeros((h, w), dtype=torch.uint8, device=dev)
+        prepared.append((imgs, alphas, out, oa, itype, enbl.auto_levels(w, h)))
+        mpix += len(imgs) * w * h / 1e6
+    torch.cuda.synchronize()
+
+    def run():
+        for imgs, alphas, out, oa, itype, L in prepared:
+            enbl.enfuse(ctx, itype, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
+
+    for _ in range(args.warmup):
+        run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ctx.profile(True)
+    run()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    ctx.close()
+    return {"workload": args.workload, "stacks": len(prepared), "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3),
+            "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+            "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
+                            "GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +105,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="enblend", choices=["enblend", "enfuse-c0", "batch-c3"],
+                    help="enblend: the headline line (default); enfuse-c0 / batch-c3: enfuse timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -68,6 +117,9 @@ def main():
 
     enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
     scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
+    if args.workload != "enblend":
+        print(json.dumps(enfuse_workload(args, enbl, scenes, np, torch)), flush=True)
+        return
     print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
     white, wa, black, ba, mask = scenes.enblend_pair(seed=1)
     h, w, C = white.shape

@@ -97,3 +97,59 @@ def enblend_pair(seed=1, w_img=4096, h=3072, overlap=0.25, bits=16):
     seam = mid + 0.1 * (w_img - off) * np.sin(2 * np.pi * yy / h * rng.uniform(1, 3) + rng.uniform(0, 6.3))
     mask = np.where(np.arange(W)[None, :] >= seam[:, None], 255, 0).astype(np.uint8)
     return Wt, wa, B, ba, mask
+
+
+def exposure_stack(w, h, K, seed, bits=8, ev_step=2.0):
+    """K integer exposures (2^EV times one radiance, EV centred on 0),
+    e.g. the 3-exposure 1024x768 uint8 stack: exposure_stack(1024, 768, 3, 0)."""
+    rad = radiance(w, h, seed)
+    evs = [(k - (K - 1) / 2) * ev_step for k in range(K)]
+    return [expose_int(rad, ev, bits, seed, 100 + k) for k, ev in enumerate(evs)]
+
+
+BATCH_SIZES = [(640, 480), (1280, 720), (1920, 1080), (2448, 2048), (3840, 2160), (4096, 3072)]
+
+
+def batch_plan(n_stacks=64, seed=3):
+    """Geometry of the batched workload: per stack (w, h, K, EV spacing)."""
+    rng = _rng(seed, 9)
+    plan = []
+    for s in range(n_stacks):
+        w, h = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
+        K = int(rng.integers(3, 10))
+        plan.append((w, h, K, float(rng.uniform(1.0, 2.0))))
+    return plan
+
+
+def device_stack(torch, w, h, K, seed, ev_step, bits=16, device="cuda"):
+    """The same scene model generated on the device (torch), for workloads
+    too large to synthesise on the host in bounded time.  Returns K (h, w, 3)
+    tensors of uint16 (bits=16) or uint8."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) * 7919 + 13)
+
+    def U(a, b, n=1):
+        return torch.rand(n, generator=g, device=device, dtype=torch.float32) * (b - a) + a
+
+    ys = torch.arange(h, device=device, dtype=torch.float32)[:, None]
+    xs = torch.arange(w, device=device, dtype=torch.float32)[None, :]
+    r = 0.2 + U(0.2, 0.8) * xs / max(w - 1, 1) + 0.0 * ys
+    for _ in range(40):
+        cx, cy = U(0, w), U(0, h)
+        s = U(5, 80)
+        r = r + U(0, 2) * torch.exp(-((xs - cx) ** 2 + (ys - cy) ** 2) / (2 * s * s))
+    for _ in range(20):
+        x0, x1 = sorted(int(v) for v in (U(0, w, 2).tolist()))
+        y0, y1 = sorted(int(v) for v in (U(0, h, 2).tolist()))
+        r[y0:y1 + 1, x0:x1 + 1] = U(0, 1.5)
+    tint = U(0.7, 1.3, 3)
+    rad = r[:, :, None] * tint[None, None, :]
+    m = (1 << bits) - 1
+    out = []
+    for k in range(K):
+        ev = (k - (K - 1) / 2) * ev_step
+        v = torch.clamp(rad * (2.0 ** ev), 0, 1) ** (1 / 2.2) * m
+        v = v + torch.randn(v.shape, generator=g, device=device)
+        v = torch.clamp(torch.round(v), 0, m)
+        out.append(v.to(torch.int32).to(torch.uint16 if bits == 16 else torch.uint8).contiguous())
+    return out
