@@ -51,33 +51,106 @@ def cpu_baseline(white, wa, black, ba, mask, levels, repeats):
     return times[len(times) // 2], o.lib.orc_num_threads(), out
 
 
+def batch_stacks(enbl, scenes, torch, dev):
+    """The 64-stack uint16 workload: per stack K exposures (3-9) of one scene,
+    sizes from 640x480 to 4096x3072, all-255 alphas, generated on the device."""
+    stacks = []
+    mpix = 0.0
+    for s, (w, h, K, ev) in enumerate(scenes.batch_plan()):
+        imgs = scenes.device_stack(torch, w, h, K, 1000 + s, ev, bits=16)
+        alphas = [torch.full((h, w), 255, dtype=torch.uint8, device=dev) for _ in imgs]
+        stacks.append((imgs, alphas, enbl.auto_levels(w, h)))
+        mpix += K * w * h / 1e6
+    torch.cuda.synchronize()
+    return stacks, mpix
+
+
+def _kernels(rep, runs=1):
+    return {k: {"ms": round(v["ms"] / runs, 3), "launches": v["launches"] // runs,
+                "GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}
+
+
+def batch_compare(args, enbl, scenes, torch, steps, warmup):
+    """Batched (enbl_enfuse_batch) vs plain per-stack (enbl_enfuse) GPU
+    throughput on the 64-stack uint16 workload, with bit-exact parity of the
+    two outputs on every stack."""
+    dev = torch.device("cuda:0")
+    print("bench: generating the 64-stack uint16 batch", file=sys.stderr, flush=True)
+    stacks, mpix = batch_stacks(enbl, scenes, torch, dev)
+    ctx = enbl.Context(0)
+    prm = enbl.enfuse_params(w_exposure=1.0, w_saturation=0.2)
+    tuning_src = "built-in defaults"
+    if args.tuning and os.path.exists(args.tuning):
+        with open(args.tuning) as f:
+            enbl.batch_tuning(ctx, json.load(f)["tuning"])
+        tuning_src = os.path.relpath(args.tuning, ROOT)
+    plain_out = [(torch.zeros_like(i[0]), torch.zeros(i[0].shape[:2], dtype=torch.uint8, device=dev)) for i, _, _ in stacks]
+    batch_out = [(torch.zeros_like(i[0]), torch.zeros(i[0].shape[:2], dtype=torch.uint8, device=dev)) for i, _, _ in stacks]
+    batch = [(imgs, alphas, o, oa, L, 1) for (imgs, alphas, L), (o, oa) in zip(stacks, batch_out)]
+
+    def run_plain():
+        for (imgs, alphas, L), (o, oa) in zip(stacks, plain_out):
+            enbl.enfuse(ctx, enbl.U16, imgs, alphas, prm, L, o, oa, dither=enbl.DITHER_DET, seed=1)
+
+    def run_batch():
+        enbl.enfuse_batch(ctx, enbl.U16, batch, prm, dither=enbl.DITHER_DET)
+
+    if args.autotune:
+        print("bench: autotuning the batched passes", file=sys.stderr, flush=True)
+        best = enbl.batch_autotune(ctx, enbl.U16, batch, prm, dither=enbl.DITHER_DET, repeats=2)
+        with open(args.autotune, "w") as f:
+            json.dump({"workload": "64-stack uint16 batch (scenes.batch_plan)", "tuning": best}, f, indent=1)
+        tuning_src = "autotuned: " + os.path.relpath(args.autotune, ROOT)
+
+    def timed(fn, n, label):
+        print(f"bench: {label}: {warmup} warmup + {n} timed runs", file=sys.stderr, flush=True)
+        for _ in range(warmup):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    ms_plain = timed(run_plain, steps, "plain per-stack enfuse")
+    ms_batch = timed(run_batch, steps, "batched enfuse")
+    parity = all(torch.equal(a, b) and torch.equal(aa, ba) for (a, aa), (b, ba) in zip(plain_out, batch_out))
+    ctx.profile(True)
+    run_batch()
+    rep_b = ctx.profile_report()
+    ctx.profile(False)
+    ctx.profile(True)
+    run_plain()
+    rep_p = ctx.profile_report()
+    ctx.profile(False)
+    tuning = enbl.batch_tuning(ctx)
+    ctx.close()
+    return {"workload": "64 exposure stacks, uint16 RGB, K=3-9, 640x480..4096x3072, exposure 1.0 + saturation 0.2",
+            "input_mpix": round(mpix, 1), "plain_ms": round(ms_plain, 3), "batched_ms": round(ms_batch, 3),
+            "plain_mpix_per_s": round(mpix / (ms_plain * 1e-3), 1),
+            "batched_mpix_per_s": round(mpix / (ms_batch * 1e-3), 1),
+            "batched_vs_plain": round(ms_plain / ms_batch, 3), "parity_batched_eq_plain": bool(parity),
+            "tuning": tuning, "tuning_source": tuning_src,
+            "batched_kernels": _kernels(rep_b), "plain_kernels": _kernels(rep_p)}
+
+
 def enfuse_workload(args, enbl, scenes, np, torch):
-    """enfuse timings: the 3 x 1024x768 u8 stack, or the 64-stack uint16
-    batch through the plain per-stack path."""
+    """enfuse timings of the 3 x 1024x768 u8 stack (plain path)."""
     ctx = enbl.Context(0)
     dev = torch.device("cuda:0")
     prm = enbl.enfuse_params(w_exposure=1.0, w_saturation=0.2)
-    if args.workload == "enfuse-c0":
-        imgs = [torch.from_numpy(i).to(dev) for i in scenes.exposure_stack(1024, 768, 3, 0, bits=8)]
-        stacks = [(imgs, enbl.U8)]
-    else:
-        stacks = []
-        for s, (w, h, K, ev) in enumerate(scenes.batch_plan()):
-            stacks.append((scenes.device_stack(torch, w, h, K, 1000 + s, ev, bits=16), enbl.U16))
-    prepared = []
-    mpix = 0.0
-    for imgs, itype in stacks:
-        h, w = imgs[0].shape[0], imgs[0].shape[1]
-        alphas = [torch.full((h, w), 255, dtype=torch.uint8, device=dev) for _ in imgs]
-        out = torch.zeros_like(imgs[0])
-        oa = torch.zeros((h, w), dtype=torch.uint8, device=dev)
-        prepared.append((imgs, alphas, out, oa, itype, enbl.auto_levels(w, h)))
-        mpix += len(imgs) * w * h / 1e6
+    imgs = [torch.from_numpy(i).to(dev) for i in scenes.exposure_stack(1024, 768, 3, 0, bits=8)]
+    h, w = imgs[0].shape[0], imgs[0].shape[1]
+    alphas = [torch.full((h, w), 255, dtype=torch.uint8, device=dev) for _ in imgs]
+    out = torch.zeros_like(imgs[0])
+    oa = torch.zeros((h, w), dtype=torch.uint8, device=dev)
+    L = enbl.auto_levels(w, h)
+    mpix = len(imgs) * w * h / 1e6
     torch.cuda.synchronize()
 
     def run():
-        for imgs, alphas, out, oa, itype, L in prepared:
-            enbl.enfuse(ctx, itype, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
+        enbl.enfuse(ctx, enbl.U8, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
 
     for _ in range(args.warmup):
         run()
@@ -92,10 +165,8 @@ def enfuse_workload(args, enbl, scenes, np, torch):
     rep = ctx.profile_report()
     ctx.profile(False)
     ctx.close()
-    return {"workload": args.workload, "stacks": len(prepared), "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3),
-            "mpix_per_s": round(mpix / (ms * 1e-3), 1),
-            "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
-                            "GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}}
+    return {"workload": args.workload, "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3),
+            "mpix_per_s": round(mpix / (ms * 1e-3), 1), "kernels": _kernels(rep)}
 
 
 def main():
@@ -107,6 +178,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="enblend", choices=["enblend", "enfuse-c0", "batch-c3"],
                     help="enblend: the headline line (default); enfuse-c0 / batch-c3: enfuse timings")
+    ap.add_argument("--no-batch", action="store_true", help="leave the batched-vs-plain comparison out of the line")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "enblend-enfuse_amd", "batch_tuning.json"),
+                    help="launch shapes of the batched passes (JSON written by --autotune)")
+    ap.add_argument("--autotune", default=None, help="run the autotuner on the batch first and write its JSON here")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -117,8 +193,11 @@ def main():
 
     enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
     scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
-    if args.workload != "enblend":
+    if args.workload == "enfuse-c0":
         print(json.dumps(enfuse_workload(args, enbl, scenes, np, torch)), flush=True)
+        return
+    if args.workload == "batch-c3":
+        print(json.dumps(batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)), flush=True)
         return
     print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
     white, wa, black, ba, mask = scenes.enblend_pair(seed=1)
@@ -214,6 +293,9 @@ def main():
                                             f"({cpu_s:.2f} s each)"}
         result["parity"] = bool(np.array_equal(cpu_out, gpu_out) and np.array_equal(cpu_alpha, gpu_alpha))
     ctx.close()
+    del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
+    if not args.no_batch:
+        result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
     print(json.dumps(result), flush=True)
 
 

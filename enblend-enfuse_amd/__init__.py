@@ -59,6 +59,10 @@ _SIGS = {
     "enbl_enfuse_mask": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp, _vp, _c_long]),
     "enbl_enfuse": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int,
                              _vp, _c_long, _vp, _c_long, _c_int, ctypes.c_uint64]),
+    "enbl_enfuse_batch": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int]),
+    "enbl_batch_get_tuning": (_c_int, [_vp, _vp]),
+    "enbl_batch_set_tuning": (_c_int, [_vp, _vp]),
+    "enbl_batch_autotune": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
     "enbl_enblend_pair": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                    _vp, _c_long, _vp, _c_long, _vp, _c_long, _vp, _c_long,
                                    _vp, _c_long, _c_int, ctypes.c_uint64]),
@@ -103,6 +107,67 @@ def enfuse(ctx, itype, imgs, alphas, prm, levels, out, out_alpha, wrap=False, di
     app = (_c_long * K)(*[w] * K)
     check(lib().enbl_enfuse(ctx.h, itype, w, h, K, ip, ipp, ap, app, ctypes.byref(prm), levels, int(wrap),
                             _vp(out.data_ptr()), w * 3, _vp(out_alpha.data_ptr()), w, dither, seed))
+
+
+class Stack(ctypes.Structure):
+    """enbl_stack (include/enblend_amd.h)."""
+    _fields_ = [("w", _c_int), ("h", _c_int), ("K", _c_int), ("levels", _c_int),
+                ("imgs", _vp), ("ipitches", _vp), ("alphas", _vp), ("apitches", _vp),
+                ("out", _vp), ("opitch", _c_long), ("out_alpha", _vp), ("oapitch", _c_long),
+                ("seed", ctypes.c_uint64)]
+
+
+class BatchTuning(ctypes.Structure):
+    """enbl_batch_tuning (include/enblend_amd.h)."""
+    _fields_ = [("rows", _c_int * 3), ("blocks_per_cu", _c_int * 3)]
+
+    def as_dict(self):
+        return {"rows": list(self.rows), "blocks_per_cu": list(self.blocks_per_cu)}
+
+
+def _stack_array(stacks):
+    """enbl_stack array for [(imgs, alphas, out, out_alpha, levels, seed), ...]
+    of contiguous (h, w, 3) / (h, w) device tensors.  Returns (array, keep)."""
+    arr = (Stack * len(stacks))()
+    keep = []
+    for i, (imgs, alphas, out, oa, levels, seed) in enumerate(stacks):
+        K = len(imgs)
+        h, w = imgs[0].shape[0], imgs[0].shape[1]
+        ip = (_vp * K)(*[t.data_ptr() for t in imgs])
+        ap = (_vp * K)(*[t.data_ptr() for t in alphas])
+        ipp = (_c_long * K)(*[w * 3] * K)
+        app = (_c_long * K)(*[w] * K)
+        keep += [ip, ap, ipp, app]
+        arr[i] = Stack(w, h, K, levels, ctypes.cast(ip, _vp), ctypes.cast(ipp, _vp), ctypes.cast(ap, _vp),
+                       ctypes.cast(app, _vp), out.data_ptr(), w * 3, oa.data_ptr(), w, seed)
+    return arr, keep
+
+
+def enfuse_batch(ctx, itype, stacks, prm, wrap=False, dither=1):
+    """Batched enfuse over [(imgs, alphas, out, out_alpha, levels, seed), ...]."""
+    arr, keep = _stack_array(stacks)
+    check(lib().enbl_enfuse_batch(ctx.h, itype, len(stacks), arr, ctypes.byref(prm), int(wrap), dither))
+    del keep
+
+
+def batch_tuning(ctx, tuning=None):
+    """Get (tuning None) or set the batched passes' launch shape; returns it as a dict."""
+    t = BatchTuning()
+    if tuning is not None:
+        t.rows[:] = list(tuning["rows"])
+        t.blocks_per_cu[:] = list(tuning["blocks_per_cu"])
+        check(lib().enbl_batch_set_tuning(ctx.h, ctypes.byref(t)))
+    check(lib().enbl_batch_get_tuning(ctx.h, ctypes.byref(t)))
+    return t.as_dict()
+
+
+def batch_autotune(ctx, itype, stacks, prm, dither=1, repeats=2):
+    """Run the autotuner on this batch; installs and returns the best shape."""
+    arr, keep = _stack_array(stacks)
+    t = BatchTuning()
+    check(lib().enbl_batch_autotune(ctx.h, itype, len(stacks), arr, ctypes.byref(prm), dither, repeats, ctypes.byref(t)))
+    del keep
+    return t.as_dict()
 
 
 class EnblError(RuntimeError):

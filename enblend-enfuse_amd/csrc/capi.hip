@@ -365,6 +365,8 @@ int enbl_context_create(int device, enbl_context** out)
     c->scratch_alpha = nullptr;
     c->scratch_alpha_bytes = 0;
     for (int i = 0; i < 16; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
+    c->batch = nullptr;
+    for (int i = 0; i < 3; ++i) { c->batch_rows[i] = 0; c->batch_bpc[i] = 0; }
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) { delete c; return rc; }
     c->stream = c->own_stream;
@@ -378,6 +380,7 @@ int enbl_context_destroy(enbl_context* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& c : ctx->cache) enbl_pyramid_destroy(c.p);
+    batch_release(ctx);
     for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
     if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
     for (int i = 0; i < 16; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);

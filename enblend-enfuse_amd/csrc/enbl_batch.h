@@ -1,0 +1,80 @@
+// Job descriptors of the batched enfuse passes (batch.hip).
+//
+// Every pass is one persistent launch over a packed tile list: job j owns
+// tiles [tile0[j], tile0[j+1]) of the list, a tile covering 64 columns x
+// `rows` rows of the job's work grid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace enbl {
+
+struct BPlane {           // planar level (pyramid type), or an interleaved RGB image
+    const void* p;
+    long pitch;           // elements
+    long plane;           // elements between channel planes (unused for an image)
+};
+
+// Weights of all K images of a stack, their norm, the union alpha and the
+// normalised fixed-point mask level 0 of every image.
+struct BWeightJob {
+    int w, h, K;
+    int tiles_x;
+    const void* const* imgs;        // device array of K interleaved images
+    const long* ipitch;             // device array (elements)
+    const uint8_t* const* alphas;   // device array
+    const long* apitch;             // device array (bytes)
+    void* const* mask0;             // device array of K mask level-0 planes
+    long mpitch;
+    uint8_t* ualpha; long upitch;   // union alpha level 0
+    float fallback;                 // (float)(255 / K)
+    int pad_;                       // 0 (no uninitialised bytes in a job table)
+};
+
+// Reduce with alpha of one pyramid level (image or mask pyramid).
+struct BReduceJob {
+    int sw, sh, dw, dh;
+    int tiles_x;
+    int channels;                   // 3: image pyramid, 1: mask pyramid
+    int src_image;                  // 1: the source is the interleaved input image
+    int pad_;                       // 0
+    BPlane src;
+    const uint8_t* salpha; long sapitch;
+    void* dst; long dpitch; long dplane;
+    uint8_t* dalpha; long dapitch;  // null: the level's alpha is not needed
+};
+
+// Merge of level l fused with the collapse step of level l:
+//   acc = sum over images m (in order) of
+//         fromRealPromote((G_m[l] - expand(G_m[l+1])) * mask_m[l] / maxpyr)
+//   res[l] = sat(acc + expand(res[l+1]))
+// At the coarsest level of a stack (cw == 0) the Laplacian level is G_m[l]
+// and nothing is added.  At level 0 the value goes straight into the output
+// image under the union alpha.
+struct BFuseJob {
+    int fw, fh, cw, ch;
+    int K;
+    int tiles_x;
+    const BPlane* fine;             // device array of K entries (level l; level 0: the images)
+    const BPlane* coarse;           // device array of K entries (level l+1)
+    const BPlane* mask;             // device array of K entries (mask level l)
+    const void* rcoarse; long rcpitch; long rcplane;   // result level l+1
+    void* res; long rpitch; long rplane;               // result level l (l > 0)
+    void* out; long opitch;                            // level 0: output image (elements)
+    const uint8_t* ualpha; long upitch;                // level 0: union alpha
+    uint8_t* oalpha; long oapitch;                     // level 0: output alpha
+    uint64_t seed;                                     // level 0: dither seed
+};
+
+enum { kBatchWeights = 0, kBatchReduce = 1, kBatchFuse = 2, kBatchKinds = 3 };
+
+int launch_batch_weights(int itype, const BWeightJob* jobs, const int* tile0, int njobs, int total_tiles,
+                         const float* lut, double w_exposure, double w_saturation, double scale, int rows,
+                         int grid, hipStream_t s);
+int launch_batch_reduce(int itype, int frac, const BReduceJob* jobs, const int* tile0, int njobs, int total_tiles,
+                        int rows, int grid, hipStream_t s);
+int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, const int* tile0, int njobs,
+                      int total_tiles, double maxpyr, int dither, int rows, int grid, hipStream_t s);
+
+}  // namespace enbl

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/enblend_amd.h"
 
 namespace enbl {
@@ -27,5 +29,13 @@ int launch_normalize(int mtype, const float* mask, long mp, const float* norm, l
 int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
                  int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
 int launch_fill(void* p, size_t bytes, hipStream_t s);
+
+namespace host {
+// Parameter checks of enfuse (the reference's preconditions and option checks).
+int check_params(int itype, int w, int h, const enbl_enfuse_params* p);
+WeightArgs weight_args(const enbl_enfuse_params* p);
+// (float)(W_exposure * weight(code / max)) for every gray code of an integer image type.
+void exposure_lut(int itype, const enbl_enfuse_params* p, const WeightArgs& a, std::vector<float>& lut);
+}  // namespace host
 
 }  // namespace enbl

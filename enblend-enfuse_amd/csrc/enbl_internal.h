@@ -73,6 +73,13 @@ struct enbl_context;
 struct enbl_pyramid;
 
 namespace enbl {
+struct BatchState;   // batched enfuse: arena, job tables, captured graph (batch_capi.hip)
+namespace host {
+void batch_release(enbl_context* ctx);
+}
+}  // namespace enbl
+
+namespace enbl {
 namespace host {
 
 // Error reporting: stores the message for enbl_last_error and returns code.
@@ -123,6 +130,9 @@ struct enbl_context {
     size_t scratch_alpha_bytes;
     void* scratch_buf[16];       // general device scratch (weights, fused passes)
     size_t scratch_bytes[16];
+    enbl::BatchState* batch;     // batched enfuse state, created on first use
+    int batch_rows[3];           // launch shape of the batched passes (0: default)
+    int batch_bpc[3];
 };
 
 struct enbl_pyramid {

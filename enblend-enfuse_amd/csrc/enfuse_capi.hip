@@ -18,7 +18,8 @@
 using namespace enbl;
 using namespace enbl::host;
 
-namespace {
+namespace enbl {
+namespace host {
 
 const double kFwhmGaussian = 2.3548200450309493820231386529193992755;
 const double kFwhm[] = {kFwhmGaussian, 2.8284271247461900976033774484193961571,
@@ -78,6 +79,23 @@ WeightArgs weight_args(const enbl_enfuse_params* p)
     return a;
 }
 
+void exposure_lut(int itype, const enbl_enfuse_params* p, const WeightArgs& a, std::vector<float>& lut)
+{
+    const int n = itype == ENBL_U8 ? 256 : 65536;
+    const double maxv = itype == ENBL_U8 ? 255.0 : 65535.0;
+    lut.resize((size_t)n);
+    for (int g = 0; g < n; ++g) {
+        const double wt = p->exposure_table ? p->exposure_table[g]
+                                            : exposure_host(a.exposure_function, a.exposure_optimum, a.exposure_width, (double)g / maxv);
+        lut[(size_t)g] = (float)(p->w_exposure * wt);
+    }
+}
+
+}  // namespace host
+}  // namespace enbl
+
+namespace {
+
 struct WeightScratch {
     float* lut = nullptr;
     void* gray = nullptr;
@@ -97,14 +115,9 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, const enbl_enfus
     const size_t npx = (size_t)pitch * h;
     int rc;
     if (p->w_exposure > 0.0 && itype != ENBL_F32) {
-        const int n = itype == ENBL_U8 ? 256 : 65536;
-        const double maxv = itype == ENBL_U8 ? 255.0 : 65535.0;
-        std::vector<float> lut((size_t)n);
-        for (int g = 0; g < n; ++g) {
-            const double wt = p->exposure_table ? p->exposure_table[g]
-                                                : exposure_host(a.exposure_function, a.exposure_optimum, a.exposure_width, (double)g / maxv);
-            lut[(size_t)g] = (float)(p->w_exposure * wt);
-        }
+        std::vector<float> lut;
+        exposure_lut(itype, p, a, lut);
+        const int n = (int)lut.size();
         void* d;
         rc = scratch(ctx, 0, (size_t)n * sizeof(float), &d);
         if (rc) return rc;

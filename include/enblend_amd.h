@@ -210,6 +210,54 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K,
                 void* out, long opitch, uint8_t* out_alpha, long oapitch,
                 int dither_mode, uint64_t seed);
 
+/* ---- batched enfuse ------------------------------------------------------ */
+
+/* One exposure stack of a batched enfuse call: the arguments of one
+ * enbl_enfuse call (host arrays of K device pointers / pitches). */
+typedef struct enbl_stack {
+    int w, h;                       /* image size */
+    int K;                          /* images in the stack, >= 1 */
+    int levels;                     /* pyramid levels, 1..29 */
+    const void* const* imgs;        /* K interleaved RGB device images of itype */
+    const long* ipitches;           /* K row pitches (elements) */
+    const uint8_t* const* alphas;   /* K 0/255 device alpha planes */
+    const long* apitches;           /* K alpha row pitches (bytes) */
+    void* out; long opitch;         /* fused image, written where the union alpha is set */
+    uint8_t* out_alpha; long oapitch;  /* union alpha */
+    uint64_t seed;                  /* dither seed */
+} enbl_stack;
+
+/* Batched form of enbl_enfuse -- no counterpart in the reference, which runs
+ * enfuseMain (src/enfuse.h:1217-1609) once per stack: `nstacks` stacks of
+ * any sizes, depths and level counts fused with the same parameters, with
+ * results identical to one enbl_enfuse call per stack.  Integer images with
+ * exposure and saturation weights and no wraparound take the batched
+ * schedule (one persistent launch per pyramid level over all stacks, replayed
+ * from a HIP graph when the same batch comes again); any other combination
+ * runs enbl_enfuse per stack.  The context keeps the intermediate pyramids
+ * and the captured graph until it is destroyed. */
+int enbl_enfuse_batch(enbl_context* ctx, int itype, int nstacks, const enbl_stack* stacks,
+                      const enbl_enfuse_params* prm, int wraparound, int dither_mode);
+
+/* Launch shape of the batched passes: rows of a 64-column tile and resident
+ * blocks per compute unit of the persistent grid, per pass kind (0 weights,
+ * 1 reduce, 2 merge + collapse).  Zeroes select the built-in defaults. */
+typedef struct enbl_batch_tuning {
+    int rows[3];
+    int blocks_per_cu[3];
+} enbl_batch_tuning;
+
+int enbl_batch_get_tuning(const enbl_context* ctx, enbl_batch_tuning* t);
+int enbl_batch_set_tuning(enbl_context* ctx, const enbl_batch_tuning* t);
+
+/* Autotuner: times every candidate launch shape of each pass kind on this
+ * batch (HIP events, `repeats` runs each, best-of), installs the fastest in
+ * the context and returns it.  The batch's outputs are written as by
+ * enbl_enfuse_batch. */
+int enbl_batch_autotune(enbl_context* ctx, int itype, int nstacks, const enbl_stack* stacks,
+                        const enbl_enfuse_params* prm, int dither_mode, int repeats,
+                        enbl_batch_tuning* best);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510
