@@ -25,6 +25,7 @@
 #include "enbl_device.h"
 #include "enbl_internal.h"
 #include "pyr_common.h"
+#include "reduce_stream.h"
 
 namespace enbl {
 namespace {
@@ -149,113 +150,34 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
 }
 
 // ---------------------------------------------------------------------------
-// Reduce with alpha, no wraparound (src/pyramid.h:161-560 with a mask):
-// zero-extended 1-4-6-4-1 sums normalised by the in-mask weight, the
-// reference's boundary forms and operation order; one output per thread.
+// Reduce with alpha, no wraparound: the streaming strips of reduce_stream.h
+// (src/pyramid.h:166-573), one tile = 4 adjacent strips x a.rows rows.
 // ---------------------------------------------------------------------------
-template <class P, int C, class LOAD>
-__device__ __forceinline__ void reduce_rows(const BReduceJob& J, int j, int ybase, int rows, int ly, int frac)
-{
-    typedef Pix<P> Px;
-    typedef typename Px::A A;
-    typedef typename Px::S S;
-    const int sw = J.sw, sh = J.sh;
-    const int c = 2 * j;
-    const int rcase = c == sw - 1 ? 1 : (c == sw - 2 ? 2 : 0);
-    int col[5];
-    bool in[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int x = c - 2 + k;
-        in[k] = x >= 0 && x < sw;
-        col[k] = x < 0 ? 0 : (x >= sw ? sw - 1 : x);
-    }
-    P* dst = static_cast<P*>(J.dst);
-    const int iend = min(J.dh, ybase + rows);
-    for (int i = ybase + ly; i < iend; i += 4) {
-        const int r = 2 * i;
-        int rowi[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const int y = r - 2 + k;
-            rowi[k] = y < 0 ? 0 : (y >= sh ? sh - 1 : y);
-        }
-        // All window loads first (clamped indices stay inside the level).
-        uint8_t al[5][5];
-        S raw[5][5][C];
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                al[k][q] = J.salpha[(long)rowi[k] * J.sapitch + col[q]];
-#pragma unroll
-                for (int ch = 0; ch < C; ++ch) raw[k][q][ch] = LOAD::get(J.src, rowi[k], col[q], ch, frac);
-            }
-        S hrow[5][C];
-        int mrow[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const int y = r - 2 + k;
-            mrow[k] = 0;
-#pragma unroll
-            for (int ch = 0; ch < C; ++ch) hrow[k][ch] = A::zero();
-            if (y < 0 || y >= sh) continue;
-            int m[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) m[q] = (in[q] && al[k][q]) ? 1 : 0;
-            int hm = m[0] + 4 * m[1] + 6 * m[2];
-            if (rcase != 1) hm += 4 * m[3];
-            if (rcase == 0) hm += m[4];
-            mrow[k] = hm;
-#pragma unroll
-            for (int ch = 0; ch < C; ++ch) {
-                S v[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) v[q] = m[q] ? raw[k][q][ch] : A::zero();
-                const S a0 = A::add(v[0], A::mul(v[1], 4));
-                if (rcase == 1) hrow[k][ch] = A::add(a0, A::mul(v[2], 6));
-                else if (rcase == 2) hrow[k][ch] = A::add(A::add(a0, A::mul(v[2], 6)), A::mul(v[3], 4));
-                else hrow[k][ch] = A::add(A::add(A::add(a0, A::mul(v[2], 6)), A::mul(v[3], 4)), v[4]);
-            }
-        }
-        const int q0 = r == 0 ? 0 : mrow[0] + 4 * mrow[1];
-        int ap;
-        if (r == sh - 1) ap = q0 + 6 * mrow[2];
-        else if (r == sh - 2) ap = q0 + 6 * mrow[2] + 4 * mrow[3];
-        else ap = q0 + 6 * mrow[2] + 4 * mrow[3] + mrow[4];
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const S a0 = r == 0 ? A::zero() : A::add(hrow[0][ch], A::mul(hrow[1][ch], 4));
-            S v;
-            if (r == sh - 1) v = A::add(a0, A::mul(hrow[2][ch], 6));
-            else if (r == sh - 2) v = A::add(A::add(a0, A::mul(hrow[2][ch], 6)), A::mul(hrow[3][ch], 4));
-            else v = A::add(A::add(A::add(a0, A::mul(hrow[2][ch], 6)), A::mul(hrow[3][ch], 4)), hrow[4][ch]);
-            dst[ch * J.dplane + (long)i * J.dpitch + j] = ap ? Px::wrap(A::div(v, ap)) : (P)0;
-        }
-        if (J.dalpha) J.dalpha[(long)i * J.dapitch + j] = ap ? 255 : 0;
-    }
-}
-
 template <class P, class I>
 __global__ __launch_bounds__(256) void batch_reduce_kernel(const BReduceJob* __restrict__ jobs, const int* __restrict__ tile0,
-                                                           int njobs, int total, int rows, int frac)
+                                                           int njobs, int total)
 {
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int wv = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BReduceJob& J = jobs[jn];
         const int local = t - tile0[jn];
-        const int j = (local % J.tiles_x) * 64 + lx;
-        const int ybase = (local / J.tiles_x) * rows;
-        if (j >= J.dw) continue;
-        if (J.channels == 1) reduce_rows<P, 1, PlaneLoad<P>>(J, j, ybase, rows, ly, frac);
-        else if (J.src_image) reduce_rows<P, 3, ImageLoad<I, P>>(J, j, ybase, rows, ly, frac);
-        else reduce_rows<P, 3, PlaneLoad<P>>(J, j, ybase, rows, ly, frac);
+        const ReduceArgs a = J.a;
+        const int wx = (local % J.groups_x) * 4 + wv;
+        const int i0 = (local / J.groups_x) * a.rows;
+        if (J.kind == kReduceImage3) reduce_strip<P, 3, true, false, ImageRows<I, P>>(a, wx, i0);
+        else if (J.kind == kReducePlane3) reduce_strip<P, 3, true, false, PlaneRows<P>>(a, wx, i0);
+        else reduce_strip<P, 1, true, false, PlaneRows<P>>(a, wx, i0);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Merge + collapse step of one level.
+//
+// Wave strips: lane L of wave wx owns coarse column j = 62 wx + L - 1 for L
+// in [1, 62] and its 2 x 2 fine block; lanes 0 and 63 load the halo columns
+// that the owned lanes take through cross-lane shuffles.  Each coarse row
+// of the expand is read from the caches once per output row.
 // ---------------------------------------------------------------------------
 
 // Polyphase row sums of the expand at coarse column j, no wraparound
@@ -269,35 +191,42 @@ __device__ __forceinline__ void poly_row(S xl, S x, S xr, int j, int cw, S& hA, 
     else { hA = A::add(A::add(xl, A::mul(x, 6)), xr); hB = A::mul(A::add(x, xr), 4); }
 }
 
-// expand() of one coarse sample's 2x2 destination block before the division:
-// o[q], q = dx + 2 dy, from the 3x3 coarse neighbourhood.
-template <class P, class GET>
-__device__ __forceinline__ void expand_block(GET&& get, int i, int j, int cw, int chh, typename Pix<P>::S o[4])
+// expand() sums of the 2 x 2 destination block of coarse sample (i, j)
+// before the division: o[q][ch], q = dx + 2 dy.  `row(y, ch)` returns this
+// lane's sample of coarse row y; rows outside the level contribute nothing.
+template <class P, int C, class ROW>
+__device__ __forceinline__ void expand_block(ROW&& row, int i, int j, int cw, int chh, typename Pix<P>::S (&o)[4][C])
 {
     typedef typename Pix<P>::A A;
     typedef typename Pix<P>::S S;
-    S hA[3], hB[3];
+    S hA[3][C], hB[3][C];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        hA[k] = A::zero();
-        hB[k] = A::zero();
         const int y = i - 1 + k;
-        if (y < 0 || y >= chh) continue;
-        const S x = get(y, j);
-        const S xl = j > 0 ? get(y, j - 1) : A::zero();
-        const S xr = j + 1 < cw ? get(y, j + 1) : A::zero();
-        poly_row<A, S>(xl, x, xr, j, cw, hA[k], hB[k]);
+        const bool in = y >= 0 && y < chh;   // uniform over the wave
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            hA[k][ch] = A::zero();
+            hB[k][ch] = A::zero();
+            if (!in) continue;
+            const S x = row(y, ch);
+            const S xl = lane_up(x), xr = lane_down(x);
+            poly_row<A, S>(xl, x, xr, j, cw, hA[k][ch], hB[k][ch]);
+        }
     }
-    if (i + 1 < chh) {
-        o[0] = A::add(A::add(hA[0], A::mul(hA[1], 6)), hA[2]);
-        o[1] = A::add(A::add(hB[0], A::mul(hB[1], 6)), hB[2]);
-        o[2] = A::add(hA[1], hA[2]);
-        o[3] = A::add(hB[1], hB[2]);
-    } else {
-        o[0] = A::add(hA[0], A::mul(hA[1], 6));
-        o[1] = A::add(hB[0], A::mul(hB[1], 6));
-        o[2] = hA[1];
-        o[3] = hB[1];
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) {
+        if (i + 1 < chh) {
+            o[0][ch] = A::add(A::add(hA[0][ch], A::mul(hA[1][ch], 6)), hA[2][ch]);
+            o[1][ch] = A::add(A::add(hB[0][ch], A::mul(hB[1][ch], 6)), hB[2][ch]);
+            o[2][ch] = A::add(hA[1][ch], hA[2][ch]);
+            o[3][ch] = A::add(hB[1][ch], hB[2][ch]);
+        } else {
+            o[0][ch] = A::add(hA[0][ch], A::mul(hA[1][ch], 6));
+            o[1][ch] = A::add(hB[0][ch], A::mul(hB[1][ch], 6));
+            o[2][ch] = hA[1][ch];
+            o[3][ch] = hB[1][ch];
+        }
     }
 }
 
@@ -319,6 +248,59 @@ __device__ __forceinline__ I to_image(int32_t v, int frac, int dither, uint64_t 
     return (I)min(ImgMax<I>::i, max(0, r));
 }
 
+// Fine-level row source of a job entry, in the form the row loaders take.
+struct RowSrc {
+    const void* src;
+    long spitch, splane;
+    int frac;
+};
+
+// Output of one fine row pair (X, X+1) at level 0: copy under the union alpha
+// (src/fixmath.h:2157-2294 with the dithered conversion) and the alpha itself.
+template <class P, class I>
+__device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool xb, const P (&v0)[3], const P (&v1)[3],
+                                          int frac, int dither)
+{
+    const uint8_t* ur = J.ualpha + (long)Y * J.upitch + X;
+    uint8_t u0, u1;
+    if (xb) {
+        const uint16_t u = *reinterpret_cast<const uint16_t*>(ur);   // X even, rows 64-byte aligned
+        u0 = (uint8_t)(u & 0xffu);
+        u1 = (uint8_t)(u >> 8);
+    } else {
+        u0 = ur[0];
+        u1 = 0;
+    }
+    uint8_t* oa = J.oalpha + (long)Y * J.oapitch + X;
+    oa[0] = u0;
+    if (xb) oa[1] = u1;
+    I* o = static_cast<I*>(J.out) + (long)Y * J.opitch + 3L * X;
+    I r0[3], r1[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        r0[ch] = to_image<I>(v0[ch], frac, dither, J.seed, X, Y, ch);
+        r1[ch] = to_image<I>(v1[ch], frac, dither, J.seed, X + 1, Y, ch);
+    }
+    if constexpr (sizeof(I) == 2) {
+        if (xb && u0 && u1 && ((uintptr_t)o & 3) == 0) {
+            uint3 d;
+            d.x = (uint32_t)r0[0] | ((uint32_t)r0[1] << 16);
+            d.y = (uint32_t)r0[2] | ((uint32_t)r1[0] << 16);
+            d.z = (uint32_t)r1[1] | ((uint32_t)r1[2] << 16);
+            *reinterpret_cast<uint3*>(o) = d;
+            return;
+        }
+    }
+    if (u0) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) o[ch] = r0[ch];
+    }
+    if (xb && u1) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) o[3 + ch] = r1[ch];
+    }
+}
+
 template <class P, class M, class I, bool LEVEL0>
 __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restrict__ jobs, const int* __restrict__ tile0,
                                                          int njobs, int total, double maxpyr, int frac, int dither,
@@ -327,28 +309,29 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
-    typedef typename std::conditional<LEVEL0, ImageLoad<I, P>, PlaneLoad<P>>::type FINE;
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    typedef typename std::conditional<LEVEL0, ImageRows<I, P>, PlaneRows<P>>::type FINE;
+    typedef typename std::conditional<LEVEL0, ImageLoad<I, P>, PlaneLoad<P>>::type FINE1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BFuseJob& J = jobs[jn];
         const int local = t - tile0[jn];
-        const int j = (local % J.tiles_x) * 64 + lx;
-        const int ybase = (local / J.tiles_x) * rows;
         const int fw = J.fw, fh = J.fh, cw = J.cw, chh = J.ch;
         if (cw == 0) {
             // Coarsest level of the stack: the Laplacian level is the Gaussian
             // level and nothing is added.  One fine sample per thread.
+            const int j = (local % J.tiles_x) * 64 + lane;
+            const int ybase = (local / J.tiles_x) * rows;
             if (j >= fw) continue;
             const int yend = min(fh, ybase + rows);
-            for (int y = ybase + ly; y < yend; y += 4) {
+            for (int y = ybase + wv; y < yend; y += 4) {
                 S acc[3];
                 for (int m = 0; m < J.K; ++m) {
                     const BPlane mk = J.mask[m];
                     const double coeff = (double)static_cast<const M*>(mk.p)[(long)y * mk.pitch + j] / maxpyr;
 #pragma unroll
                     for (int ch = 0; ch < 3; ++ch) {
-                        const S v = Px::load(from_real<P>((double)FINE::get(J.fine[m], y, j, ch, frac) * coeff));
+                        const S v = Px::load(from_real<P>((double)FINE1::get(J.fine[m], y, j, ch, frac) * coeff));
                         acc[ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[ch])));
                     }
                 }
@@ -367,66 +350,78 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
             }
             continue;
         }
-        if (j >= cw) continue;
+        const int wx = (local % J.tiles_x) * 4 + wv;
+        if (wx * kOwned >= cw) continue;   // the whole wave is beyond the level
+        const int i0 = (local / J.tiles_x) * rows;
+        const int i1 = min(chh, i0 + rows);
+        const int j = wx * kOwned + lane - 1;
+        const bool valid = j >= 0 && j < cw;
+        const bool owned = valid && lane >= 1 && lane <= kOwned;
         const int X = 2 * j;
+        const bool xb = X + 1 < fw;
         const int hdA = expand_hsum(X, cw, false, true), hdB = expand_hsum(X + 1, cw, false, true);
-        const int iend = min(chh, ybase + rows);
-        for (int i = ybase + ly; i < iend; i += 4) {
+        for (int i = i0; i < i1; ++i) {
             const int Y = 2 * i;
+            const bool yb = Y + 1 < fh;
             const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
             const int dv[4] = {hdA * v0, hdB * v0, hdA * v1, hdB * v1};
-            bool live[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) live[q] = X + (q & 1) < fw && Y + (q >> 1) < fh;
             S acc[4][3];
             for (int m = 0; m < J.K; ++m) {
                 const BPlane cs = J.coarse[m];
                 const BPlane fs = J.fine[m];
                 const BPlane mk = J.mask[m];
-                double coeff[4];
+                const P* cp = static_cast<const P*>(cs.p);
+                S o[4][3];
+                expand_block<P, 3>([&](int y, int ch) { return valid ? Px::load(cp[ch * cs.plane + (long)y * cs.pitch + j]) : A::zero(); },
+                                   i, j, cw, chh, o);
+                // Fine samples and mask of rows Y, Y+1 (vector loads of the column pair).
+                const RowSrc rs{fs.p, fs.pitch, fs.plane, frac};
+                S f[4][3];
+                int32_t mv[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    coeff[q] = live[q] ? (double)static_cast<const M*>(mk.p)[(long)(Y + (q >> 1)) * mk.pitch + X + (q & 1)] / maxpyr : 0.0;
+                for (int ch = 0; ch < 3; ++ch) f[0][ch] = f[1][ch] = f[2][ch] = f[3][ch] = A::zero();
+                if (valid) {
+                    const M* mrow = static_cast<const M*>(mk.p) + (long)Y * mk.pitch + X;
+                    FINE::template load<3>(rs, Y, X, xb, f[0], f[1]);
+                    Vec2<M>::load(mrow, xb, mv[0], mv[1]);
+                    if (yb) {
+                        FINE::template load<3>(rs, Y + 1, X, xb, f[2], f[3]);
+                        Vec2<M>::load(mrow + mk.pitch, xb, mv[2], mv[3]);
+                    }
+                }
 #pragma unroll
-                for (int ch = 0; ch < 3; ++ch) {
-                    S o[4];
-                    expand_block<P>([&](int y, int x) { return PlaneLoad<P>::get(cs, y, x, ch, 0); }, i, j, cw, chh, o);
+                for (int q = 0; q < 4; ++q) {
+                    // Laplacian value (src/pyramid.h:1705-1707), mask multiply
+                    // (src/enfuse.h:601-620) and accumulation (:1576-1588).
+                    const double coeff = (double)mv[q] / maxpyr;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (!live[q]) continue;
-                        // Laplacian value (src/pyramid.h:1705-1707), mask multiply
-                        // (src/enfuse.h:601-620) and accumulation (:1576-1588).
-                        const P lapv = Px::wrap(A::sub(FINE::get(fs, Y + (q >> 1), X + (q & 1), ch, frac), A::div(o[q], dv[q])));
-                        const S v = Px::load(from_real<P>((double)lapv * coeff[q]));
+                    for (int ch = 0; ch < 3; ++ch) {
+                        const P lapv = Px::wrap(A::sub(f[q][ch], A::div(o[q][ch], dv[q])));
+                        const S v = Px::load(from_real<P>((double)lapv * coeff));
                         acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
                     }
                 }
             }
             // Collapse step: + expand(result[l+1]), saturating (src/pyramid.h:1837-1886).
             const P* rc = static_cast<const P*>(J.rcoarse);
+            S o[4][3];
+            expand_block<P, 3>([&](int y, int ch) { return valid ? Px::load(rc[ch * J.rcplane + (long)y * J.rcpitch + j]) : A::zero(); },
+                               i, j, cw, chh, o);
             P outv[4][3];
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                S o[4];
-                expand_block<P>([&](int y, int x) { return Px::load(rc[ch * J.rcplane + (long)y * J.rcpitch + x]); }, i, j, cw, chh, o);
+            for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) outv[q][ch] = live[q] ? Px::sat(A::add(acc[q][ch], A::div(o[q], dv[q]))) : (P)0;
-            }
+                for (int ch = 0; ch < 3; ++ch) outv[q][ch] = Px::sat(A::add(acc[q][ch], A::div(o[q][ch], dv[q])));
+            if (!owned) continue;
+            if (LEVEL0) {
+                emit_pair<P, I>(J, X, Y, xb, outv[0], outv[1], frac, dither);
+                if (yb) emit_pair<P, I>(J, X, Y + 1, xb, outv[2], outv[3], frac, dither);
+            } else {
+                P* r0 = static_cast<P*>(J.res) + (long)Y * J.rpitch + X;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (!live[q]) continue;
-                const int xx = X + (q & 1), yy = Y + (q >> 1);
-                if (LEVEL0) {
-                    const uint8_t ua = J.ualpha[(long)yy * J.upitch + xx];
-                    J.oalpha[(long)yy * J.oapitch + xx] = ua;
-                    if (ua) {
-                        I* o = static_cast<I*>(J.out) + (long)yy * J.opitch + 3L * xx;
-#pragma unroll
-                        for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(outv[q][ch], frac, dither, J.seed, xx, yy, ch);
-                    }
-                } else {
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch) static_cast<P*>(J.res)[ch * J.rplane + (long)yy * J.rpitch + xx] = outv[q][ch];
+                for (int ch = 0; ch < 3; ++ch) {
+                    Vec2<P>::store(r0 + ch * J.rplane, xb, outv[0][ch], outv[1][ch]);
+                    if (yb) Vec2<P>::store(r0 + ch * J.rplane + J.rpitch, xb, outv[2][ch], outv[3][ch]);
                 }
             }
         }
@@ -449,14 +444,13 @@ int launch_batch_weights(int itype, const BWeightJob* jobs, const int* tile0, in
     return 0;
 }
 
-int launch_batch_reduce(int itype, int frac, const BReduceJob* jobs, const int* tile0, int njobs, int total, int rows,
-                        int grid, hipStream_t s)
+int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int njobs, int total, int grid, hipStream_t s)
 {
     if (total <= 0) return 0;
     if (itype == ENBL_U8)
-        hipLaunchKernelGGL((batch_reduce_kernel<int16_t, uint8_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, rows, frac);
+        hipLaunchKernelGGL((batch_reduce_kernel<int16_t, uint8_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
     else if (itype == ENBL_U16)
-        hipLaunchKernelGGL((batch_reduce_kernel<int32_t, uint16_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, rows, frac);
+        hipLaunchKernelGGL((batch_reduce_kernel<int32_t, uint16_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
     else return ENBL_E_UNSUPPORTED;
     return 0;
 }

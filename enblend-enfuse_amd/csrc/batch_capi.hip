@@ -61,6 +61,7 @@ namespace {
 const int kDefaultRows[kBatchKinds] = {16, 16, 8};
 const int kDefaultBpc[kBatchKinds] = {8, 8, 8};
 const char* const kKindName[kBatchKinds] = {"batch_weights", "batch_reduce", "batch_fuse"};
+const int kStrip = 62;   // owned columns of a wave strip (reduce_stream.h, batch.hip)
 
 struct Tuning {
     int rows[kBatchKinds];
@@ -308,58 +309,48 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             const Level& DA = Y.ial[(size_t)l];
             const bool keep_alpha = l + 2 < S.levels;   // level l + 1 is reduced again
             const int sw = l == 0 ? S.w : Y.img[(size_t)l - 1].w, sh = l == 0 ? S.h : Y.img[(size_t)l - 1].h;
+            auto add_job = [&](int kind, const void* src, long spitch, long splane, int sw_, int sh_, const uint8_t* sa,
+                               long sap, void* dst, const Level& DL, uint8_t* da, long dap) {
+                jobs.emplace_back();
+                BReduceJob& J = jobs.back();
+                ReduceArgs& a = J.a;
+                a.src = src; a.spitch = spitch; a.splane = splane; a.sw = sw_; a.sh = sh_;
+                a.salpha = sa; a.sapitch = sap;
+                a.dst = dst; a.dpitch = DL.pitch; a.dplane = DL.plane; a.dw = DL.w; a.dh = DL.h;
+                a.dalpha = da; a.dapitch = dap;
+                a.channels = kind == kReducePlane1 ? 1 : 3;
+                a.wrap = 0;
+                a.rows = tu.rows[kBatchReduce];
+                a.force_small = ENBL_KERNELS_STREAMING;
+                a.frac = 7;
+                J.kind = kind;
+                J.groups_x = ((DL.w + kStrip - 1) / kStrip + 3) / 4;
+                t0.push_back(total);
+                total += J.groups_x * ((DL.h + a.rows - 1) / a.rows);
+            };
             for (int m = 0; m < S.K; ++m) {
-                BReduceJob J{};
-                J.sw = sw;
-                J.sh = sh;
-                J.dw = D.w;
-                J.dh = D.h;
-                J.channels = 3;
-                J.src_image = l == 0;
+                uint8_t* da = keep_alpha ? (uint8_t*)at(Y.ial_at(S.K, m) + DA.offset) : nullptr;
                 if (l == 0) {
-                    J.src = {S.imgs[m], S.ipitches[m], 0};
-                    J.salpha = S.alphas[m];
-                    J.sapitch = S.apitches[m];
+                    add_job(kReduceImage3, S.imgs[m], S.ipitches[m], 0, sw, sh, S.alphas[m], S.apitches[m],
+                            at(Y.img_at(m) + D.offset), D, da, DA.pitch);
                 } else {
                     const Level& SL = Y.img[(size_t)l - 1];
-                    J.src = {at(Y.img_at(m) + SL.offset), SL.pitch, SL.plane};
-                    J.salpha = (const uint8_t*)at(Y.ial_at(S.K, m) + Y.ial[(size_t)l - 1].offset);
-                    J.sapitch = Y.ial[(size_t)l - 1].pitch;
+                    const Level& SA = Y.ial[(size_t)l - 1];
+                    add_job(kReducePlane3, at(Y.img_at(m) + SL.offset), SL.pitch, SL.plane, sw, sh,
+                            (const uint8_t*)at(Y.ial_at(S.K, m) + SA.offset), SA.pitch, at(Y.img_at(m) + D.offset), D, da,
+                            DA.pitch);
                 }
-                J.dst = at(Y.img_at(m) + D.offset);
-                J.dpitch = D.pitch;
-                J.dplane = D.plane;
-                J.dalpha = keep_alpha ? (uint8_t*)at(Y.ial_at(S.K, m) + DA.offset) : nullptr;
-                J.dapitch = DA.pitch;
-                const int nt = tiles(D.w, D.h, tu.rows[kBatchReduce], &J.tiles_x);
-                jobs.push_back(J);
-                t0.push_back(total);
-                total += nt;
                 bytes += (double)sw * sh * (3.0 * (l == 0 ? isz : es) + 1.0) + (double)D.w * D.h * (3.0 * es + (keep_alpha ? 1.0 : 0.0));
             }
             const Level& MS = Y.msk[(size_t)l];
             const Level& MD = Y.msk[(size_t)l + 1];
+            const Level& UA = Y.ual[(size_t)l];
+            const Level& UD = Y.ual[(size_t)l + 1];
             for (int m = 0; m < S.K; ++m) {
-                BReduceJob J{};
-                J.sw = MS.w;
-                J.sh = MS.h;
-                J.dw = MD.w;
-                J.dh = MD.h;
-                J.channels = 1;
-                J.src_image = 0;
-                J.src = {at(Y.msk_at(S.K, m) + MS.offset), MS.pitch, MS.plane};
-                J.salpha = (const uint8_t*)at(Y.ual_at(S.K) + Y.ual[(size_t)l].offset);
-                J.sapitch = Y.ual[(size_t)l].pitch;
-                J.dst = at(Y.msk_at(S.K, m) + MD.offset);
-                J.dpitch = MD.pitch;
-                J.dplane = MD.plane;
                 const bool write_ua = m == 0 && keep_alpha;
-                J.dalpha = write_ua ? (uint8_t*)at(Y.ual_at(S.K) + Y.ual[(size_t)l + 1].offset) : nullptr;
-                J.dapitch = Y.ual[(size_t)l + 1].pitch;
-                const int nt = tiles(MD.w, MD.h, tu.rows[kBatchReduce], &J.tiles_x);
-                jobs.push_back(J);
-                t0.push_back(total);
-                total += nt;
+                add_job(kReducePlane1, at(Y.msk_at(S.K, m) + MS.offset), MS.pitch, MS.plane, MS.w, MS.h,
+                        (const uint8_t*)at(Y.ual_at(S.K) + UA.offset), UA.pitch, at(Y.msk_at(S.K, m) + MD.offset), MD,
+                        write_ua ? (uint8_t*)at(Y.ual_at(S.K) + UD.offset) : nullptr, UD.pitch);
                 bytes += (double)MS.w * MS.h * (ms + (m == 0 ? 1.0 : 0.0)) + (double)MD.w * MD.h * (ms + (write_ua ? 1.0 : 0.0));
             }
         }
@@ -425,8 +416,13 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             J.oalpha = S.out_alpha;
             J.oapitch = S.oapitch;
             J.seed = S.seed;
-            const int nt = coarsest ? tiles(J.fw, J.fh, tu.rows[kBatchFuse], &J.tiles_x)
-                                    : tiles(J.cw, J.ch, tu.rows[kBatchFuse], &J.tiles_x);
+            int nt;
+            if (coarsest) {
+                nt = tiles(J.fw, J.fh, tu.rows[kBatchFuse], &J.tiles_x);
+            } else {
+                J.tiles_x = ((J.cw + kStrip - 1) / kStrip + 3) / 4;
+                nt = J.tiles_x * ((J.ch + tu.rows[kBatchFuse] - 1) / tu.rows[kBatchFuse]);
+            }
             jobs.push_back(J);
             t0.push_back(total);
             total += nt;
@@ -463,8 +459,8 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
                                           reinterpret_cast<const float*>(dtab + P.lut_off), P.w_exp, P.w_sat, P.scale,
                                           L.rows, L.grid, s);
             else if (L.kind == kBatchReduce)
-                rc = launch_batch_reduce(itype, P.frac, reinterpret_cast<const BReduceJob*>(dtab + L.jobs), t0, L.njobs,
-                                         L.total, L.rows, L.grid, s);
+                rc = launch_batch_reduce(itype, reinterpret_cast<const BReduceJob*>(dtab + L.jobs), t0, L.njobs, L.total,
+                                         L.grid, s);
             else
                 rc = launch_batch_fuse(itype, P.frac, L.level0, reinterpret_cast<const BFuseJob*>(dtab + L.jobs), t0,
                                        L.njobs, L.total, P.maxpyr, P.dither, L.rows, L.grid, s);
@@ -617,7 +613,9 @@ int enbl_batch_autotune(enbl_context* ctx, int itype, int nstacks, const enbl_st
     BatchState* b;
     rc = state(ctx, &b);
     if (rc) return rc;
-    static const int kRows[] = {4, 8, 16, 32};
+    // Candidate tile rows per pass kind (the streaming reduce amortises its
+    // 2-row halo over longer tiles) and resident blocks per CU.
+    static const int kRows[kBatchKinds][4] = {{4, 8, 16, 32}, {8, 16, 32, 64}, {4, 8, 16, 32}};
     static const int kBpc[] = {2, 4, 8, 16};
     Tuning cur = context_tuning(ctx);
     const bool prof = ctx->profile;
@@ -625,7 +623,7 @@ int enbl_batch_autotune(enbl_context* ctx, int itype, int nstacks, const enbl_st
     for (int kind = 0; kind < kBatchKinds && !rc; ++kind) {
         float best_ms = 1e30f;
         Tuning win = cur;
-        for (int r : kRows) {
+        for (int r : kRows[kind]) {
             for (int q : kBpc) {
                 Tuning t = cur;
                 t.rows[kind] = r;

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "enbl_internal.h"
+
 namespace enbl {
 
 struct BPlane {           // planar level (pyramid type), or an interleaved RGB image
@@ -32,20 +34,19 @@ struct BWeightJob {
     int pad_;                       // 0 (no uninitialised bytes in a job table)
 };
 
-// Reduce with alpha of one pyramid level (image or mask pyramid).
+// Reduce with alpha of one pyramid level (image or mask pyramid), as the
+// streaming strips of reduce_stream.h: a tile is 4 adjacent 62-column wave
+// strips x a.rows output rows.
+enum { kReducePlane3 = 0, kReduceImage3 = 1, kReducePlane1 = 2 };
 struct BReduceJob {
-    int sw, sh, dw, dh;
-    int tiles_x;
-    int channels;                   // 3: image pyramid, 1: mask pyramid
-    int src_image;                  // 1: the source is the interleaved input image
-    int pad_;                       // 0
-    BPlane src;
-    const uint8_t* salpha; long sapitch;
-    void* dst; long dpitch; long dplane;
-    uint8_t* dalpha; long dapitch;  // null: the level's alpha is not needed
+    ReduceArgs a;                   // no wraparound; a.rows = tile rows; a.dalpha may be null
+    int groups_x;                   // 4-strip groups across the destination level
+    int kind;                       // kReduce*
 };
 
-// Merge of level l fused with the collapse step of level l:
+// Merge of level l fused with the collapse step of level l, one tile = 4
+// adjacent wave strips of 62 coarse columns (2 x 2 fine samples each) x
+// `rows` coarse rows:
 //   acc = sum over images m (in order) of
 //         fromRealPromote((G_m[l] - expand(G_m[l+1])) * mask_m[l] / maxpyr)
 //   res[l] = sat(acc + expand(res[l+1]))
@@ -55,7 +56,7 @@ struct BReduceJob {
 struct BFuseJob {
     int fw, fh, cw, ch;
     int K;
-    int tiles_x;
+    int tiles_x;                    // tiles across: 4-strip groups, or 64-column tiles at the coarsest level
     const BPlane* fine;             // device array of K entries (level l; level 0: the images)
     const BPlane* coarse;           // device array of K entries (level l+1)
     const BPlane* mask;             // device array of K entries (mask level l)
@@ -72,8 +73,8 @@ enum { kBatchWeights = 0, kBatchReduce = 1, kBatchFuse = 2, kBatchKinds = 3 };
 int launch_batch_weights(int itype, const BWeightJob* jobs, const int* tile0, int njobs, int total_tiles,
                          const float* lut, double w_exposure, double w_saturation, double scale, int rows,
                          int grid, hipStream_t s);
-int launch_batch_reduce(int itype, int frac, const BReduceJob* jobs, const int* tile0, int njobs, int total_tiles,
-                        int rows, int grid, hipStream_t s);
+int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int njobs, int total_tiles, int grid,
+                        hipStream_t s);
 int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, const int* tile0, int njobs,
                       int total_tiles, double maxpyr, int dither, int rows, int grid, hipStream_t s);
 

@@ -30,6 +30,8 @@ struct ReduceArgs {
     uint8_t* dalpha; long dapitch;
     int channels; int wrap; int rows;
     int force_small;   // ENBL_KERNELS_* policy
+    int frac;          // interleaved integer image source: fixed-point shift
+    int pad_;          // 0 (no uninitialised bytes: batched job tables are compared bytewise)
 };
 
 struct ExpandArgs {

@@ -19,224 +19,19 @@
 #include "enbl_device.h"
 #include "enbl_internal.h"
 #include "pyr_common.h"
+#include "reduce_stream.h"
 
 namespace enbl {
 namespace {
 
 // ---------------------------------------------------------------------------
-// reduce
+// reduce (streaming strips: reduce_stream.h)
 // ---------------------------------------------------------------------------
-
-// Raw samples of one source row held by one lane: its own pair (2j, 2j+1),
-// alpha-masked, and, for wraparound edge lanes, the four taps that wrap to
-// the far side of the row.
-template <class S, int C, bool WRAP>
-struct ReduceRow {
-    S x0[C], x1[C];
-    int m0, m1;
-    S wa[WRAP ? C : 1], wb[WRAP ? C : 1], wd[WRAP ? C : 1], we[WRAP ? C : 1];
-    int qa, qb, qd, qe;
-};
-
-template <class P, int C, bool ALPHA, bool WRAP>
-struct ReduceLane {
-    typedef Pix<P> Px;
-    typedef typename Px::A A;
-    typedef typename Px::S S;
-    typedef ReduceRow<S, C, WRAP> Row;
-
-    const ReduceArgs& a;
-    int c;          // source centre column 2j
-    bool valid;     // lane's destination column exists
-    bool both;      // c + 1 is inside the row
-    bool edge;      // a tap of this lane leaves the row
-
-    // Issue the loads of source row r (no waiting, no cross-lane traffic).
-    __device__ __forceinline__ void load(int r, Row& R) const {
-        const P* src = static_cast<const P*>(a.src);
-        R.m0 = 0;
-        R.m1 = 0;
-        if (ALPHA && valid) {
-            const uint8_t* ar = a.salpha + (long)r * a.sapitch;
-            R.m0 = ar[c] ? 1 : 0;
-            R.m1 = (both && ar[c + 1]) ? 1 : 0;
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            S x0 = A::zero(), x1 = A::zero();
-            if (valid) Vec2<P>::load(src + ch * a.splane + (long)r * a.spitch + c, both, x0, x1);
-            if (ALPHA) {
-                if (!R.m0) x0 = A::zero();
-                if (!R.m1) x1 = A::zero();
-            }
-            R.x0[ch] = x0;
-            R.x1[ch] = x1;
-        }
-        if (WRAP) {
-            R.qa = R.qb = R.qd = R.qe = 0;
-#pragma unroll
-            for (int ch = 0; ch < C; ++ch) { R.wa[ch] = R.wb[ch] = R.wd[ch] = R.we[ch] = A::zero(); }
-            if (edge) {
-                const int sw = a.sw;
-                const int ia = wrap_index(c - 2, sw), ib = wrap_index(c - 1, sw);
-                const int id = wrap_index(c + 1, sw), ie = wrap_index(c + 2, sw);
-                int ma = 1, mb = 1, md = 1, me = 1;
-                if (ALPHA) {
-                    const uint8_t* ar = a.salpha + (long)r * a.sapitch;
-                    ma = ar[ia] ? 1 : 0; mb = ar[ib] ? 1 : 0; md = ar[id] ? 1 : 0; me = ar[ie] ? 1 : 0;
-                }
-                R.qa = ma; R.qb = mb; R.qd = md; R.qe = me;
-#pragma unroll
-                for (int ch = 0; ch < C; ++ch) {
-                    const P* row = src + ch * a.splane + (long)r * a.spitch;
-                    R.wa[ch] = ma ? Px::load(row[ia]) : A::zero();
-                    R.wb[ch] = mb ? Px::load(row[ib]) : A::zero();
-                    R.wd[ch] = md ? Px::load(row[id]) : A::zero();
-                    R.we[ch] = me ? Px::load(row[ie]) : A::zero();
-                }
-            }
-        }
-    }
-
-    // Horizontal 5-tap sums of a loaded row (every lane of the wave executes
-    // this: the neighbour taps arrive through shuffles).
-    __device__ __forceinline__ void sums(const Row& R, S (&h)[C], int& hm) const {
-        const int sw = a.sw;
-        // boundary case of the reference's row tail (src/pyramid.h:264-299,
-        // :410-455, :688-707): 1 = centre is the last sample, 2 = centre is
-        // the second to last sample (no wraparound only)
-        const int rcase = WRAP ? 0 : (c == sw - 1 ? 1 : (c == sw - 2 ? 2 : 0));
-        if (ALPHA) {
-            const int ml0 = lane_up(R.m0), ml1 = lane_up(R.m1), mr0 = lane_down(R.m0);
-            int qa, qb, qd, qe;
-            if (WRAP && edge) { qa = R.qa; qb = R.qb; qd = R.qd; qe = R.qe; }
-            else { qa = c >= 2 ? ml0 : 0; qb = c >= 1 ? ml1 : 0; qd = R.m1; qe = mr0; }
-            hm = qa + 4 * qb + 6 * R.m0;
-            if (rcase != 1) hm += 4 * qd;
-            if (rcase == 0) hm += qe;
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const S x0 = R.x0[ch];
-            const S l0 = lane_up(x0), l1 = lane_up(R.x1[ch]), r0 = lane_down(x0);
-            S va, vb, vd, ve;
-            if (WRAP && edge) {
-                va = R.wa[ch]; vb = R.wb[ch]; vd = R.wd[ch]; ve = R.we[ch];
-            } else {
-                va = c >= 2 ? l0 : (ALPHA ? A::zero() : x0);
-                vb = c >= 1 ? l1 : (ALPHA ? A::zero() : x0);
-                vd = R.x1[ch];
-                ve = r0;
-            }
-            const S a0 = A::add(va, A::mul(vb, 4));
-            if (rcase == 1) {
-                h[ch] = ALPHA ? A::add(a0, A::mul(x0, 6)) : A::add(a0, A::mul(x0, 11));
-            } else if (rcase == 2) {
-                const S d4 = A::mul(vd, 4);
-                h[ch] = ALPHA ? A::add(A::add(a0, A::mul(x0, 6)), d4)
-                              : A::add(A::add(A::add(a0, A::mul(x0, 6)), d4), A::div(d4, 4));
-            } else {
-                h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), A::mul(vd, 4)), ve);
-            }
-        }
-    }
-};
-
 template <class P, int C, bool ALPHA, bool WRAP>
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
 {
-    typedef ReduceLane<P, C, ALPHA, WRAP> Lane;
-    typedef typename Lane::A A;
-    typedef typename Lane::S S;
-    typedef typename Lane::Row Row;
-    typedef Pix<P> Px;
-
-    const int lane = threadIdx.x & (kWave - 1);
     const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
-    if (wx * kOwned - 1 >= a.dw) return;  // whole wave beyond the level
-    const int i0 = blockIdx.y * a.rows;
-    if (i0 >= a.dh) return;
-    const int i1 = min(a.dh, i0 + a.rows);
-    const int sh = a.sh;
-    const int j = wx * kOwned + lane - 1;
-    const int c = 2 * j;
-    const bool valid = j >= 0 && j < a.dw;
-    const bool owned = lane >= 1 && lane <= kOwned && valid;
-    const Lane L{a, c, valid, c + 1 < a.sw, valid && (j == 0 || c + 2 >= a.sw)};
-    P* dst = static_cast<P*>(a.dst);
-
-    S hm2[C], hm1[C], hc[C], hp1[C], hp2[C];
-    int qm2 = 0, qm1 = 0, qc = 0, qp1 = 0, qp2 = 0;
-#pragma unroll
-    for (int ch = 0; ch < C; ++ch) { hm2[ch] = hm1[ch] = hc[ch] = hp1[ch] = hp2[ch] = A::zero(); }
-    Row ra, rb, na, nb;
-    {
-        Row t;
-        if (i0 > 0) {
-            L.load(2 * i0 - 2, t);
-            L.sums(t, hm2, qm2);
-            L.load(2 * i0 - 1, t);
-            L.sums(t, hm1, qm1);
-        }
-        L.load(2 * i0, t);
-        L.sums(t, hc, qc);
-    }
-    if (2 * i0 + 1 < sh) L.load(2 * i0 + 1, ra);
-    if (2 * i0 + 2 < sh) L.load(2 * i0 + 2, rb);
-
-    for (int i = i0; i < i1; ++i) {
-        const int r = 2 * i;
-        // prefetch the next output row's two source rows
-        if (i + 1 < i1) {
-            if (r + 3 < sh) L.load(r + 3, na);
-            if (r + 4 < sh) L.load(r + 4, nb);
-        }
-        if (r + 1 < sh) L.sums(ra, hp1, qp1);
-        if (r + 2 < sh) L.sums(rb, hp2, qp2);
-
-        int ap = 0;
-        if (ALPHA) {
-            const int q0 = r == 0 ? 0 : qm2 + 4 * qm1;
-            if (r == sh - 1) ap = q0 + 6 * qc;
-            else if (r == sh - 2) ap = q0 + 6 * qc + 4 * qp1;
-            else ap = q0 + 6 * qc + 4 * qp1 + qp2;
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            S v;
-            if (ALPHA) {
-                // src/pyramid.h:350-384, :459-504 (column recursion)
-                const S a0 = r == 0 ? A::zero() : A::add(hm2[ch], A::mul(hm1[ch], 4));
-                if (r == sh - 1) v = A::add(a0, A::mul(hc[ch], 6));
-                else if (r == sh - 2) v = A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4));
-                else v = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4)), hp2[ch]);
-            } else {
-                // src/pyramid.h:669-672, :741-760, :809-846 (clamped columns)
-                const S a0 = r == 0 ? A::mul(hc[ch], 5) : A::add(hm2[ch], A::mul(hm1[ch], 4));
-                if (r == sh - 1) {
-                    v = A::add(a0, A::mul(hc[ch], 11));
-                } else if (r == sh - 2) {
-                    const S d4 = A::mul(hp1[ch], 4);
-                    v = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), d4), A::div(d4, 4));
-                } else {
-                    v = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4)), hp2[ch]);
-                }
-            }
-            if (owned) {
-                P out;
-                if (ALPHA) out = ap ? Px::wrap(A::div(v, ap)) : (P)0;
-                else out = Px::wrap(A::div(v, 256));
-                dst[ch * a.dplane + (long)i * a.dpitch + j] = out;
-            }
-            hm2[ch] = hc[ch];
-            hm1[ch] = hp1[ch];
-            hc[ch] = hp2[ch];
-        }
-        if (ALPHA && owned) a.dalpha[(long)i * a.dapitch + j] = ap ? 255 : 0;
-        qm2 = qc; qm1 = qp1; qc = qp2;
-        ra = na;
-        rb = nb;
-    }
+    reduce_strip<P, C, ALPHA, WRAP, PlaneRows<P>>(a, wx, blockIdx.y * a.rows);
 }
 
 // ---------------------------------------------------------------------------
