@@ -75,17 +75,44 @@ template <class T> __device__ __forceinline__ int gray_code(T p0, T p1, T p2)
 // image order (:1377-1385); normalisation and fixed-point mask level 0
 // (:1529-1546, src/fixmath.h:188-194).
 // ---------------------------------------------------------------------------
-constexpr int kRegK = 16;   // weights of up to this many images stay in registers
-
+// The weights of a pixel pair (x, x+1) for every image of the stack, all
+// loads issued before any arithmetic; NK bounds the stack depth this
+// instance keeps in registers (deeper stacks take the looping instance).
 template <class T>
-__device__ __forceinline__ float pixel_weight(const BWeightJob& J, int m, int x, int y, const float* __restrict__ lut,
-                                              double w_exp, double w_sat, uint8_t& al)
+__device__ __forceinline__ void load_pair(const T* p, bool two, T (&v)[6])
 {
-    al = J.alphas[m][(long)y * J.apitch[m] + x];
+    if constexpr (sizeof(T) == 2) {
+        if (two && ((uintptr_t)p & 3) == 0) {   // two RGB16 pixels: one dwordx3
+            const uint3 d = *reinterpret_cast<const uint3*>(p);
+            v[0] = (T)(d.x & 0xffffu); v[1] = (T)(d.x >> 16);
+            v[2] = (T)(d.y & 0xffffu); v[3] = (T)(d.y >> 16);
+            v[4] = (T)(d.z & 0xffffu); v[5] = (T)(d.z >> 16);
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? p[k] : (T)0;
+}
+
+__device__ __forceinline__ void load_alpha_pair(const uint8_t* p, bool two, uint8_t& a0, uint8_t& a1)
+{
+    if (two && ((uintptr_t)p & 1) == 0) {
+        const uint16_t u = *reinterpret_cast<const uint16_t*>(p);
+        a0 = (uint8_t)(u & 0xffu);
+        a1 = (uint8_t)(u >> 8);
+    } else {
+        a0 = p[0];
+        a1 = two ? p[1] : (uint8_t)0;
+    }
+}
+
+// enfuseMask's exposure + saturation sum of one pixel (zero outside alpha).
+template <class T>
+__device__ __forceinline__ float weight_of(T p0, T p1, T p2, uint8_t al, const float* __restrict__ lut, double w_exp,
+                                          double w_sat)
+{
     float wm = 0.f;
     if (!al) return wm;
-    const T* p = static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x;
-    const T p0 = p[0], p1 = p[1], p2 = p[2];
     if (w_exp > 0.0) wm = lut[gray_code<T>(p0, p1, p2)];
     if (w_sat > 0.0) {
         const T mx = max(p0, max(p1, p2));
@@ -100,50 +127,82 @@ __device__ __forceinline__ float pixel_weight(const BWeightJob& J, int m, int x,
     return wm;
 }
 
-template <class T, class M>
+template <class T, class M, int NK>
 __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __restrict__ jobs, const int* __restrict__ tile0,
                                                             int njobs, int total, const float* __restrict__ lut,
                                                             double w_exp, double w_sat, double scale, int rows)
 {
-    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BWeightJob& J = jobs[jn];
         const int local = t - tile0[jn];
-        const int x = (local % J.tiles_x) * 64 + lx;
+        const int x = (local % J.tiles_x) * 128 + 2 * lane;
         const int ybase = (local / J.tiles_x) * rows;
         if (x >= J.w) continue;
+        const bool two = x + 1 < J.w;
+        const int K = J.K;
         const int yend = min(J.h, ybase + rows);
-        for (int y = ybase + ly; y < yend; y += 4) {
-            float wreg[kRegK];
-            float norm = 0.f;
-            uint8_t ua = 0;
+        for (int y = ybase + wv; y < yend; y += 4) {
+            float w0[NK > 0 ? NK : 1], w1[NK > 0 ? NK : 1];
+            float n0 = 0.f, n1 = 0.f;
+            uint8_t u0 = 0, u1 = 0;
+            if constexpr (NK > 0) {
+                T px[NK][6];
+                uint8_t a0[NK], a1[NK];
 #pragma unroll
-            for (int m = 0; m < kRegK; ++m) {
-                if (m >= J.K) break;
-                uint8_t al;
-                wreg[m] = pixel_weight<T>(J, m, x, y, lut, w_exp, w_sat, al);
-                if (al) ua = al;
-                norm = wreg[m] + norm;
-            }
-            for (int m = kRegK; m < J.K; ++m) {
-                uint8_t al;
-                const float wm = pixel_weight<T>(J, m, x, y, lut, w_exp, w_sat, al);
-                if (al) ua = al;
-                norm = wm + norm;
-            }
-            J.ualpha[(long)y * J.upitch + x] = ua;
+                for (int m = 0; m < NK; ++m) {
+                    if (m < K) {
+                        load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px[m]);
+                        load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0[m], a1[m]);
+                    }
+                }
 #pragma unroll
-            for (int m = 0; m < kRegK; ++m) {
-                if (m >= J.K) break;
-                const float nm = norm > 0.0 ? (255.0f * wreg[m]) / norm : J.fallback;
-                static_cast<M*>(J.mask0[m])[(long)y * J.mpitch + x] = from_real<M>((double)nm * scale);
+                for (int m = 0; m < NK; ++m) {
+                    if (m < K) {
+                        w0[m] = weight_of<T>(px[m][0], px[m][1], px[m][2], a0[m], lut, w_exp, w_sat);
+                        w1[m] = weight_of<T>(px[m][3], px[m][4], px[m][5], a1[m], lut, w_exp, w_sat);
+                        if (a0[m]) u0 = a0[m];
+                        if (a1[m]) u1 = a1[m];
+                        n0 = w0[m] + n0;
+                        n1 = w1[m] + n1;
+                    }
+                }
+            } else {
+                for (int m = 0; m < K; ++m) {
+                    T px[6];
+                    uint8_t a0, a1;
+                    load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px);
+                    load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0, a1);
+                    n0 = weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat) + n0;
+                    n1 = weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat) + n1;
+                    if (a0) u0 = a0;
+                    if (a1) u1 = a1;
+                }
             }
-            for (int m = kRegK; m < J.K; ++m) {
-                uint8_t al;
-                const float wm = pixel_weight<T>(J, m, x, y, lut, w_exp, w_sat, al);
-                const float nm = norm > 0.0 ? (255.0f * wm) / norm : J.fallback;
-                static_cast<M*>(J.mask0[m])[(long)y * J.mpitch + x] = from_real<M>((double)nm * scale);
+            uint8_t* ur = J.ualpha + (long)y * J.upitch + x;   // x even, rows 64-byte aligned
+            if (two) *reinterpret_cast<uint16_t*>(ur) = (uint16_t)(u0 | (u1 << 8));
+            else ur[0] = u0;
+            // Normalisation and fixed-point mask level 0 (src/enfuse.h:1529-1546).
+            auto store_mask = [&](int m, float wm0, float wm1) {
+                const float nm0 = n0 > 0.0 ? (255.0f * wm0) / n0 : J.fallback;
+                const float nm1 = n1 > 0.0 ? (255.0f * wm1) / n1 : J.fallback;
+                Vec2<M>::store(static_cast<M*>(J.mask0[m]) + (long)y * J.mpitch + x, two,
+                               from_real<M>((double)nm0 * scale), from_real<M>((double)nm1 * scale));
+            };
+            if constexpr (NK > 0) {
+#pragma unroll
+                for (int m = 0; m < NK; ++m)
+                    if (m < K) store_mask(m, w0[m], w1[m]);
+            } else {
+                for (int m = 0; m < K; ++m) {
+                    T px[6];
+                    uint8_t a0, a1;
+                    load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px);
+                    load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0, a1);
+                    store_mask(m, weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat),
+                               weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat));
+                }
             }
         }
     }
@@ -192,10 +251,12 @@ __device__ __forceinline__ void poly_row(S xl, S x, S xr, int j, int cw, S& hA, 
 }
 
 // expand() sums of the 2 x 2 destination block of coarse sample (i, j)
-// before the division: o[q][ch], q = dx + 2 dy.  `row(y, ch)` returns this
-// lane's sample of coarse row y; rows outside the level contribute nothing.
-template <class P, int C, class ROW>
-__device__ __forceinline__ void expand_block(ROW&& row, int i, int j, int cw, int chh, typename Pix<P>::S (&o)[4][C])
+// before the division: o[q][ch], q = dx + 2 dy, from this lane's samples
+// c[k][ch] of coarse rows i-1+k (zero outside the level); the horizontal
+// neighbours come from the adjacent lanes.
+template <class P, int C>
+__device__ __forceinline__ void expand_sums(const typename Pix<P>::S (&c)[3][C], int i, int j, int cw, int chh,
+                                            typename Pix<P>::S (&o)[4][C])
 {
     typedef typename Pix<P>::A A;
     typedef typename Pix<P>::S S;
@@ -209,7 +270,7 @@ __device__ __forceinline__ void expand_block(ROW&& row, int i, int j, int cw, in
             hA[k][ch] = A::zero();
             hB[k][ch] = A::zero();
             if (!in) continue;
-            const S x = row(y, ch);
+            const S x = c[k][ch];
             const S xl = lane_up(x), xr = lane_down(x);
             poly_row<A, S>(xl, x, xr, j, cw, hA[k][ch], hB[k][ch]);
         }
@@ -228,6 +289,39 @@ __device__ __forceinline__ void expand_block(ROW&& row, int i, int j, int cw, in
             o[3][ch] = hB[1][ch];
         }
     }
+}
+
+// This lane's samples of coarse rows i-1, i, i+1 of a planar level.
+template <class P, int C>
+__device__ __forceinline__ void load_coarse(const P* p, long pitch, long plane, int i, int j, int chh, bool valid,
+                                            typename Pix<P>::S (&c)[3][C])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int y = i - 1 + k;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch)
+            c[k][ch] = (valid && y >= 0 && y < chh) ? Pix<P>::load(p[ch * plane + (long)y * pitch + j]) : Pix<P>::A::zero();
+    }
+}
+
+// (double)v / maxpyr for a mask sample v of a normalised fixed-point mask
+// pyramid, maxpyr = 255 * 2^mfrac (the ImageMaskMultiplyFunctor coefficient,
+// src/enfuse.h:601-620).  For 0 <= v < 2^24 -- every value such a pyramid
+// holds -- the quotient by 255 comes from one reciprocal product and one
+// fused residual correction, which equals the correctly rounded division on
+// that whole range (checked exhaustively by the test suite); the power-of-two
+// factor is exact.  Other values take the division.
+__device__ __forceinline__ double mask_coeff(int32_t v, double maxpyr, double inv_scale)
+{
+    if ((uint32_t)v < (1u << 24)) {
+        const double x = (double)v;
+        const double r = 1.0 / 255.0;
+        const double q0 = x * r;
+        const double e = __fma_rn(-q0, 255.0, x);
+        return __fma_rn(e, r, q0) * inv_scale;
+    }
+    return (double)v / maxpyr;
 }
 
 // SpecializedConvertPyramidToScalarFunctor (src/fixmath.h:236-290).
@@ -303,8 +397,8 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
 
 template <class P, class M, class I, bool LEVEL0>
 __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restrict__ jobs, const int* __restrict__ tile0,
-                                                         int njobs, int total, double maxpyr, int frac, int dither,
-                                                         int rows)
+                                                         int njobs, int total, double maxpyr, double inv_scale, int frac,
+                                                         int dither, int rows)
 {
     typedef Pix<P> Px;
     typedef typename Px::A A;
@@ -328,7 +422,7 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                 S acc[3];
                 for (int m = 0; m < J.K; ++m) {
                     const BPlane mk = J.mask[m];
-                    const double coeff = (double)static_cast<const M*>(mk.p)[(long)y * mk.pitch + j] / maxpyr;
+                    const double coeff = mask_coeff(static_cast<const M*>(mk.p)[(long)y * mk.pitch + j], maxpyr, inv_scale);
 #pragma unroll
                     for (int ch = 0; ch < 3; ++ch) {
                         const S v = Px::load(from_real<P>((double)FINE1::get(J.fine[m], y, j, ch, frac) * coeff));
@@ -365,48 +459,59 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
             const bool yb = Y + 1 < fh;
             const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
             const int dv[4] = {hdA * v0, hdB * v0, hdA * v1, hdB * v1};
-            S acc[4][3];
-            for (int m = 0; m < J.K; ++m) {
+            // The inputs of image m + 1 are loaded while image m is merged.
+            struct In {
+                S c[3][3];     // coarse rows i-1..i+1, own column
+                S f[4][3];     // fine samples of the 2 x 2 block
+                int32_t mv[4]; // mask samples of the 2 x 2 block
+            };
+            auto load_in = [&](int m, In& D) {
                 const BPlane cs = J.coarse[m];
                 const BPlane fs = J.fine[m];
                 const BPlane mk = J.mask[m];
-                const P* cp = static_cast<const P*>(cs.p);
-                S o[4][3];
-                expand_block<P, 3>([&](int y, int ch) { return valid ? Px::load(cp[ch * cs.plane + (long)y * cs.pitch + j]) : A::zero(); },
-                                   i, j, cw, chh, o);
-                // Fine samples and mask of rows Y, Y+1 (vector loads of the column pair).
+                load_coarse<P, 3>(static_cast<const P*>(cs.p), cs.pitch, cs.plane, i, j, chh, valid, D.c);
                 const RowSrc rs{fs.p, fs.pitch, fs.plane, frac};
-                S f[4][3];
-                int32_t mv[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int ch = 0; ch < 3; ++ch) f[0][ch] = f[1][ch] = f[2][ch] = f[3][ch] = A::zero();
+                for (int q = 0; q < 4; ++q) {
+                    D.mv[q] = 0;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) D.f[q][ch] = A::zero();
+                }
                 if (valid) {
                     const M* mrow = static_cast<const M*>(mk.p) + (long)Y * mk.pitch + X;
-                    FINE::template load<3>(rs, Y, X, xb, f[0], f[1]);
-                    Vec2<M>::load(mrow, xb, mv[0], mv[1]);
+                    FINE::template load<3>(rs, Y, X, xb, D.f[0], D.f[1]);
+                    Vec2<M>::load(mrow, xb, D.mv[0], D.mv[1]);
                     if (yb) {
-                        FINE::template load<3>(rs, Y + 1, X, xb, f[2], f[3]);
-                        Vec2<M>::load(mrow + mk.pitch, xb, mv[2], mv[3]);
+                        FINE::template load<3>(rs, Y + 1, X, xb, D.f[2], D.f[3]);
+                        Vec2<M>::load(mrow + mk.pitch, xb, D.mv[2], D.mv[3]);
                     }
                 }
+            };
+            S acc[4][3];
+            In cur, nxt;
+            load_in(0, cur);
+            for (int m = 0; m < J.K; ++m) {
+                if (m + 1 < J.K) load_in(m + 1, nxt);
+                S o[4][3];
+                expand_sums<P, 3>(cur.c, i, j, cw, chh, o);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     // Laplacian value (src/pyramid.h:1705-1707), mask multiply
                     // (src/enfuse.h:601-620) and accumulation (:1576-1588).
-                    const double coeff = (double)mv[q] / maxpyr;
+                    const double coeff = mask_coeff(cur.mv[q], maxpyr, inv_scale);
 #pragma unroll
                     for (int ch = 0; ch < 3; ++ch) {
-                        const P lapv = Px::wrap(A::sub(f[q][ch], A::div(o[q][ch], dv[q])));
+                        const P lapv = Px::wrap(A::sub(cur.f[q][ch], A::div(o[q][ch], dv[q])));
                         const S v = Px::load(from_real<P>((double)lapv * coeff));
                         acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
                     }
                 }
+                cur = nxt;
             }
             // Collapse step: + expand(result[l+1]), saturating (src/pyramid.h:1837-1886).
-            const P* rc = static_cast<const P*>(J.rcoarse);
-            S o[4][3];
-            expand_block<P, 3>([&](int y, int ch) { return valid ? Px::load(rc[ch * J.rcplane + (long)y * J.rcpitch + j]) : A::zero(); },
-                               i, j, cw, chh, o);
+            S rcv[3][3], o[4][3];
+            load_coarse<P, 3>(static_cast<const P*>(J.rcoarse), J.rcpitch, J.rcplane, i, j, chh, valid, rcv);
+            expand_sums<P, 3>(rcv, i, j, cw, chh, o);
             P outv[4][3];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -430,17 +535,26 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
 
 }  // namespace
 
-int launch_batch_weights(int itype, const BWeightJob* jobs, const int* tile0, int njobs, int total, const float* lut,
-                         double w_exposure, double w_saturation, double scale, int rows, int grid, hipStream_t s)
+int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int* tile0, int njobs, int total,
+                         const float* lut, double w_exposure, double w_saturation, double scale, int rows, int grid,
+                         hipStream_t s)
 {
     if (total <= 0) return 0;
-    if (itype == ENBL_U8)
-        hipLaunchKernelGGL((batch_weights_kernel<uint8_t, int16_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, lut,
-                           w_exposure, w_saturation, scale, rows);
-    else if (itype == ENBL_U16)
-        hipLaunchKernelGGL((batch_weights_kernel<uint16_t, int32_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, lut,
-                           w_exposure, w_saturation, scale, rows);
+#define ENBL_BW(T, M, NK) hipLaunchKernelGGL((batch_weights_kernel<T, M, NK>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, \
+                                             total, lut, w_exposure, w_saturation, scale, rows)
+#define ENBL_BW_K(T, M)                                  \
+    do {                                                 \
+        if (max_k <= 4) ENBL_BW(T, M, 4);                \
+        else if (max_k <= 8) ENBL_BW(T, M, 8);           \
+        else if (max_k <= 12) ENBL_BW(T, M, 12);         \
+        else if (max_k <= 16) ENBL_BW(T, M, 16);         \
+        else ENBL_BW(T, M, 0);                           \
+    } while (0)
+    if (itype == ENBL_U8) ENBL_BW_K(uint8_t, int16_t);
+    else if (itype == ENBL_U16) ENBL_BW_K(uint16_t, int32_t);
     else return ENBL_E_UNSUPPORTED;
+#undef ENBL_BW_K
+#undef ENBL_BW
     return 0;
 }
 
@@ -459,12 +573,13 @@ int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, con
                       double maxpyr, int dither, int rows, int grid, hipStream_t s)
 {
     if (total <= 0) return 0;
+    const double inv_scale = 255.0 / maxpyr;   // 2^-mfrac, exact
 #define ENBL_BFUSE(P, M, I)                                                                                          \
     do {                                                                                                             \
         if (level0) hipLaunchKernelGGL((batch_fuse_kernel<P, M, I, true>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, \
-                                       maxpyr, frac, dither, rows);                                                  \
+                                       maxpyr, inv_scale, frac, dither, rows);                                       \
         else hipLaunchKernelGGL((batch_fuse_kernel<P, M, I, false>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total,   \
-                                maxpyr, frac, dither, rows);                                                         \
+                                maxpyr, inv_scale, frac, dither, rows);                                              \
     } while (0)
     if (itype == ENBL_U8) ENBL_BFUSE(int16_t, int16_t, uint8_t);
     else if (itype == ENBL_U16) ENBL_BFUSE(int32_t, int32_t, uint16_t);

@@ -94,6 +94,7 @@ struct Plan {
     size_t lut_off = 0;
     double w_exp = 0, w_sat = 0, scale = 0, maxpyr = 0;
     int frac = 0, dither = 0;
+    int max_k = 0;   // deepest stack
 };
 
 // Device layout of one stack inside the arena.
@@ -260,7 +261,11 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
     };
     auto at = [&](size_t off) { return arena + off; };
     int Lmax = 0;
-    for (int s = 0; s < n; ++s) Lmax = std::max(Lmax, st[s].levels);
+    P.max_k = 0;
+    for (int s = 0; s < n; ++s) {
+        Lmax = std::max(Lmax, st[s].levels);
+        P.max_k = std::max(P.max_k, st[s].K);
+    }
 
     // Weights.
     {
@@ -277,7 +282,8 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             J.w = S.w;
             J.h = S.h;
             J.K = S.K;
-            const int nt = tiles(S.w, S.h, tu.rows[kBatchWeights], &J.tiles_x);
+            J.tiles_x = (S.w + 127) / 128;
+            const int nt = J.tiles_x * ((S.h + tu.rows[kBatchWeights] - 1) / tu.rows[kBatchWeights]);
             J.imgs = T.dev<const void*>(T.put(S.imgs, (size_t)S.K));
             J.ipitch = T.dev<long>(T.put(S.ipitches, (size_t)S.K));
             J.alphas = T.dev<const uint8_t*>(T.put(S.alphas, (size_t)S.K));
@@ -455,7 +461,7 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
             Timed t(ctx, kKindName[L.kind], L.bytes);
             const int* t0 = reinterpret_cast<const int*>(dtab + L.tile0);
             if (L.kind == kBatchWeights)
-                rc = launch_batch_weights(itype, reinterpret_cast<const BWeightJob*>(dtab + L.jobs), t0, L.njobs, L.total,
+                rc = launch_batch_weights(itype, P.max_k, reinterpret_cast<const BWeightJob*>(dtab + L.jobs), t0, L.njobs, L.total,
                                           reinterpret_cast<const float*>(dtab + P.lut_off), P.w_exp, P.w_sat, P.scale,
                                           L.rows, L.grid, s);
             else if (L.kind == kBatchReduce)
@@ -510,7 +516,7 @@ int prepare(enbl_context* ctx, BatchState* b, int itype, int n, const enbl_stack
     for (const Launch& L : P.launches) append(&L, sizeof L);
     const double sc[4] = {P.w_exp, P.w_sat, P.scale, P.maxpyr};
     append(sc, sizeof sc);
-    const int iv[3] = {P.frac, P.dither, itype};
+    const int iv[4] = {P.frac, P.dither, itype, P.max_k};
     append(iv, sizeof iv);
     *same = b->exec && b->exec_stream == ctx->stream && key == b->key;
     if (*same) return ENBL_OK;

@@ -19,7 +19,8 @@ struct BPlane {           // planar level (pyramid type), or an interleaved RGB 
 };
 
 // Weights of all K images of a stack, their norm, the union alpha and the
-// normalised fixed-point mask level 0 of every image.
+// normalised fixed-point mask level 0 of every image; a tile is 128 columns
+// (a pixel pair per lane) x `rows` rows.
 struct BWeightJob {
     int w, h, K;
     int tiles_x;
@@ -70,7 +71,8 @@ struct BFuseJob {
 
 enum { kBatchWeights = 0, kBatchReduce = 1, kBatchFuse = 2, kBatchKinds = 3 };
 
-int launch_batch_weights(int itype, const BWeightJob* jobs, const int* tile0, int njobs, int total_tiles,
+// max_k: the deepest stack of the launch (selects the register-resident instance).
+int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int* tile0, int njobs, int total_tiles,
                          const float* lut, double w_exposure, double w_saturation, double scale, int rows,
                          int grid, hipStream_t s);
 int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int njobs, int total_tiles, int grid,

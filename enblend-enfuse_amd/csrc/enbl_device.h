@@ -25,7 +25,17 @@ struct AccI {
     __device__ static __forceinline__ T add(T a, T b) { return (T)((uint32_t)a + (uint32_t)b); }
     __device__ static __forceinline__ T sub(T a, T b) { return (T)((uint32_t)a - (uint32_t)b); }
     __device__ static __forceinline__ T mul(T a, int k) { return (T)((uint32_t)a * (uint32_t)k); }
-    __device__ static __forceinline__ T div(T a, int d) { return a / d; }
+    // C++ truncating division by a positive divisor.  The filter weight sums
+    // are powers of two away from borders and alpha edges (256, 64, 16, ...):
+    // those take a shift with the round-toward-zero correction instead of the
+    // general division sequence.
+    __device__ static __forceinline__ T div(T a, int d) {
+        if ((d & (d - 1)) == 0) {
+            const int s = __builtin_ctz((unsigned)d);
+            return (a + ((a >> 31) & (d - 1))) >> s;
+        }
+        return a / d;
+    }
     __device__ static __forceinline__ T zero() { return 0; }
 };
 
