@@ -50,13 +50,13 @@ template <> struct ImgMax<uint16_t> { static constexpr int i = 65535; static con
 template <class P> struct PlaneLoad {
     typedef typename Pix<P>::S S;
     __device__ static __forceinline__ S get(const BPlane& b, int y, int x, int ch, int) {
-        return Pix<P>::load(static_cast<const P*>(b.p)[ch * b.plane + (long)y * b.pitch + x]);
+        return Pix<P>::load(*gmem(static_cast<const P*>(b.p) + ch * b.plane + (long)y * b.pitch + x));
     }
 };
 template <class I, class P> struct ImageLoad {
     typedef typename Pix<P>::S S;
     __device__ static __forceinline__ S get(const BPlane& b, int y, int x, int ch, int frac) {
-        return (S)(P)((P)static_cast<const I*>(b.p)[(long)y * b.pitch + 3L * x + ch] << frac);
+        return (S)(P)((P)*gmem(static_cast<const I*>(b.p) + (long)y * b.pitch + 3L * x + ch) << frac);
     }
 };
 
@@ -83,26 +83,29 @@ __device__ __forceinline__ void load_pair(const T* p, bool two, T (&v)[6])
 {
     if constexpr (sizeof(T) == 2) {
         if (two && ((uintptr_t)p & 3) == 0) {   // two RGB16 pixels: one dwordx3
-            const uint3 d = *reinterpret_cast<const uint3*>(p);
-            v[0] = (T)(d.x & 0xffffu); v[1] = (T)(d.x >> 16);
-            v[2] = (T)(d.y & 0xffffu); v[3] = (T)(d.y >> 16);
-            v[4] = (T)(d.z & 0xffffu); v[5] = (T)(d.z >> 16);
+            const auto* d = gmem(reinterpret_cast<const uint32_t*>(p));
+            const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+            v[0] = (T)(d0 & 0xffffu); v[1] = (T)(d0 >> 16);
+            v[2] = (T)(d1 & 0xffffu); v[3] = (T)(d1 >> 16);
+            v[4] = (T)(d2 & 0xffffu); v[5] = (T)(d2 >> 16);
             return;
         }
     }
+    const auto* g = gmem(p);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? p[k] : (T)0;
+    for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? g[k] : (T)0;
 }
 
 __device__ __forceinline__ void load_alpha_pair(const uint8_t* p, bool two, uint8_t& a0, uint8_t& a1)
 {
     if (two && ((uintptr_t)p & 1) == 0) {
-        const uint16_t u = *reinterpret_cast<const uint16_t*>(p);
+        const uint16_t u = *gmem(reinterpret_cast<const uint16_t*>(p));
         a0 = (uint8_t)(u & 0xffu);
         a1 = (uint8_t)(u >> 8);
     } else {
-        a0 = p[0];
-        a1 = two ? p[1] : (uint8_t)0;
+        const auto* g = gmem(p);
+        a0 = g[0];
+        a1 = two ? g[1] : (uint8_t)0;
     }
 }
 
@@ -129,10 +132,12 @@ __device__ __forceinline__ float weight_of(T p0, T p1, T p2, uint8_t al, const f
 
 template <class T, class M, int NK>
 __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __restrict__ jobs, const int* __restrict__ tile0,
-                                                            int njobs, int total, const float* __restrict__ lut,
-                                                            double w_exp, double w_sat, double scale, int rows)
+                                                            int njobs, int total, const BImage* __restrict__ images,
+                                                            const float* __restrict__ lut, double w_exp, double w_sat,
+                                                            double scale, int rows)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BWeightJob& J = jobs[jn];
@@ -142,6 +147,7 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
         if (x >= J.w) continue;
         const bool two = x + 1 < J.w;
         const int K = J.K;
+        const BImage* im = images + J.img0;
         const int yend = min(J.h, ybase + rows);
         for (int y = ybase + wv; y < yend; y += 4) {
             float w0[NK > 0 ? NK : 1], w1[NK > 0 ? NK : 1];
@@ -153,8 +159,8 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
 #pragma unroll
                 for (int m = 0; m < NK; ++m) {
                     if (m < K) {
-                        load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px[m]);
-                        load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0[m], a1[m]);
+                        load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px[m]);
+                        load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0[m], a1[m]);
                     }
                 }
 #pragma unroll
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
                 for (int m = 0; m < K; ++m) {
                     T px[6];
                     uint8_t a0, a1;
-                    load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px);
-                    load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0, a1);
+                    load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px);
+                    load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0, a1);
                     n0 = weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat) + n0;
                     n1 = weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat) + n1;
                     if (a0) u0 = a0;
@@ -181,13 +187,13 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
                 }
             }
             uint8_t* ur = J.ualpha + (long)y * J.upitch + x;   // x even, rows 64-byte aligned
-            if (two) *reinterpret_cast<uint16_t*>(ur) = (uint16_t)(u0 | (u1 << 8));
-            else ur[0] = u0;
+            if (two) *gmem(reinterpret_cast<uint16_t*>(ur)) = (uint16_t)(u0 | (u1 << 8));
+            else *gmem(ur) = u0;
             // Normalisation and fixed-point mask level 0 (src/enfuse.h:1529-1546).
             auto store_mask = [&](int m, float wm0, float wm1) {
                 const float nm0 = n0 > 0.0 ? (255.0f * wm0) / n0 : J.fallback;
                 const float nm1 = n1 > 0.0 ? (255.0f * wm1) / n1 : J.fallback;
-                Vec2<M>::store(static_cast<M*>(J.mask0[m]) + (long)y * J.mpitch + x, two,
+                Vec2<M>::store(static_cast<M*>(im[m].mask0) + (long)y * J.mpitch + x, two,
                                from_real<M>((double)nm0 * scale), from_real<M>((double)nm1 * scale));
             };
             if constexpr (NK > 0) {
@@ -198,8 +204,8 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
                 for (int m = 0; m < K; ++m) {
                     T px[6];
                     uint8_t a0, a1;
-                    load_pair<T>(static_cast<const T*>(J.imgs[m]) + (long)y * J.ipitch[m] + 3L * x, two, px);
-                    load_alpha_pair(J.alphas[m] + (long)y * J.apitch[m] + x, two, a0, a1);
+                    load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px);
+                    load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0, a1);
                     store_mask(m, weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat),
                                weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat));
                 }
@@ -216,7 +222,7 @@ template <class P, class I>
 __global__ __launch_bounds__(256) void batch_reduce_kernel(const BReduceJob* __restrict__ jobs, const int* __restrict__ tile0,
                                                            int njobs, int total)
 {
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BReduceJob& J = jobs[jn];
@@ -301,7 +307,7 @@ __device__ __forceinline__ void load_coarse(const P* p, long pitch, long plane, 
         const int y = i - 1 + k;
 #pragma unroll
         for (int ch = 0; ch < C; ++ch)
-            c[k][ch] = (valid && y >= 0 && y < chh) ? Pix<P>::load(p[ch * plane + (long)y * pitch + j]) : Pix<P>::A::zero();
+            c[k][ch] = (valid && y >= 0 && y < chh) ? Pix<P>::load(*gmem(p + ch * plane + (long)y * pitch + j)) : Pix<P>::A::zero();
     }
 }
 
@@ -358,17 +364,18 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
     const uint8_t* ur = J.ualpha + (long)Y * J.upitch + X;
     uint8_t u0, u1;
     if (xb) {
-        const uint16_t u = *reinterpret_cast<const uint16_t*>(ur);   // X even, rows 64-byte aligned
+        const uint16_t u = *gmem(reinterpret_cast<const uint16_t*>(ur));   // X even, rows 64-byte aligned
         u0 = (uint8_t)(u & 0xffu);
         u1 = (uint8_t)(u >> 8);
     } else {
-        u0 = ur[0];
+        u0 = *gmem(ur);
         u1 = 0;
     }
-    uint8_t* oa = J.oalpha + (long)Y * J.oapitch + X;
+    auto* oa = gmem(J.oalpha + (long)Y * J.oapitch + X);
     oa[0] = u0;
     if (xb) oa[1] = u1;
-    I* o = static_cast<I*>(J.out) + (long)Y * J.opitch + 3L * X;
+    I* op = static_cast<I*>(J.out) + (long)Y * J.opitch + 3L * X;
+    auto* o = gmem(op);
     I r0[3], r1[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -376,12 +383,11 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
         r1[ch] = to_image<I>(v1[ch], frac, dither, J.seed, X + 1, Y, ch);
     }
     if constexpr (sizeof(I) == 2) {
-        if (xb && u0 && u1 && ((uintptr_t)o & 3) == 0) {
-            uint3 d;
-            d.x = (uint32_t)r0[0] | ((uint32_t)r0[1] << 16);
-            d.y = (uint32_t)r0[2] | ((uint32_t)r1[0] << 16);
-            d.z = (uint32_t)r1[1] | ((uint32_t)r1[2] << 16);
-            *reinterpret_cast<uint3*>(o) = d;
+        if (xb && u0 && u1 && ((uintptr_t)op & 3) == 0) {
+            auto* d = gmem(reinterpret_cast<uint32_t*>(op));
+            d[0] = (uint32_t)r0[0] | ((uint32_t)r0[1] << 16);
+            d[1] = (uint32_t)r0[2] | ((uint32_t)r1[0] << 16);
+            d[2] = (uint32_t)r1[1] | ((uint32_t)r1[2] << 16);
             return;
         }
     }
@@ -397,20 +403,22 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
 
 template <class P, class M, class I, bool LEVEL0>
 __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restrict__ jobs, const int* __restrict__ tile0,
-                                                         int njobs, int total, double maxpyr, double inv_scale, int frac,
-                                                         int dither, int rows)
+                                                         int njobs, int total, const BPlane* __restrict__ planes,
+                                                         double maxpyr, double inv_scale, int frac, int dither, int rows)
 {
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
     typedef typename std::conditional<LEVEL0, ImageRows<I, P>, PlaneRows<P>>::type FINE;
     typedef typename std::conditional<LEVEL0, ImageLoad<I, P>, PlaneLoad<P>>::type FINE1;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int jn = find_job(tile0, njobs, t);
         const BFuseJob& J = jobs[jn];
         const int local = t - tile0[jn];
         const int fw = J.fw, fh = J.fh, cw = J.cw, chh = J.ch;
+        const BPlane* pl = planes + J.plane0;   // per image: fine, coarse, mask
         if (cw == 0) {
             // Coarsest level of the stack: the Laplacian level is the Gaussian
             // level and nothing is added.  One fine sample per thread.
@@ -421,25 +429,25 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
             for (int y = ybase + wv; y < yend; y += 4) {
                 S acc[3];
                 for (int m = 0; m < J.K; ++m) {
-                    const BPlane mk = J.mask[m];
-                    const double coeff = mask_coeff(static_cast<const M*>(mk.p)[(long)y * mk.pitch + j], maxpyr, inv_scale);
+                    const BPlane mk = pl[3 * m + 2];
+                    const double coeff = mask_coeff(*gmem(static_cast<const M*>(mk.p) + (long)y * mk.pitch + j), maxpyr, inv_scale);
 #pragma unroll
                     for (int ch = 0; ch < 3; ++ch) {
-                        const S v = Px::load(from_real<P>((double)FINE1::get(J.fine[m], y, j, ch, frac) * coeff));
+                        const S v = Px::load(from_real<P>((double)FINE1::get(pl[3 * m], y, j, ch, frac) * coeff));
                         acc[ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[ch])));
                     }
                 }
                 if (LEVEL0) {
-                    const uint8_t ua = J.ualpha[(long)y * J.upitch + j];
-                    J.oalpha[(long)y * J.oapitch + j] = ua;
+                    const uint8_t ua = *gmem(J.ualpha + (long)y * J.upitch + j);
+                    *gmem(J.oalpha + (long)y * J.oapitch + j) = ua;
                     if (ua) {
-                        I* o = static_cast<I*>(J.out) + (long)y * J.opitch + 3L * j;
+                        auto* o = gmem(static_cast<I*>(J.out) + (long)y * J.opitch + 3L * j);
 #pragma unroll
                         for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(acc[ch], frac, dither, J.seed, j, y, ch);
                     }
                 } else {
 #pragma unroll
-                    for (int ch = 0; ch < 3; ++ch) static_cast<P*>(J.res)[ch * J.rplane + (long)y * J.rpitch + j] = (P)acc[ch];
+                    for (int ch = 0; ch < 3; ++ch) *gmem(static_cast<P*>(J.res) + ch * J.rplane + (long)y * J.rpitch + j) = (P)acc[ch];
                 }
             }
             continue;
@@ -466,9 +474,9 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                 int32_t mv[4]; // mask samples of the 2 x 2 block
             };
             auto load_in = [&](int m, In& D) {
-                const BPlane cs = J.coarse[m];
-                const BPlane fs = J.fine[m];
-                const BPlane mk = J.mask[m];
+                const BPlane fs = pl[3 * m];
+                const BPlane cs = pl[3 * m + 1];
+                const BPlane mk = pl[3 * m + 2];
                 load_coarse<P, 3>(static_cast<const P*>(cs.p), cs.pitch, cs.plane, i, j, chh, valid, D.c);
                 const RowSrc rs{fs.p, fs.pitch, fs.plane, frac};
 #pragma unroll
@@ -536,12 +544,12 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
 }  // namespace
 
 int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int* tile0, int njobs, int total,
-                         const float* lut, double w_exposure, double w_saturation, double scale, int rows, int grid,
+                         const BImage* images, const float* lut, double w_exposure, double w_saturation, double scale, int rows, int grid,
                          hipStream_t s)
 {
     if (total <= 0) return 0;
 #define ENBL_BW(T, M, NK) hipLaunchKernelGGL((batch_weights_kernel<T, M, NK>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, \
-                                             total, lut, w_exposure, w_saturation, scale, rows)
+                                             total, images, lut, w_exposure, w_saturation, scale, rows)
 #define ENBL_BW_K(T, M)                                  \
     do {                                                 \
         if (max_k <= 4) ENBL_BW(T, M, 4);                \
@@ -570,16 +578,16 @@ int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int
 }
 
 int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, const int* tile0, int njobs, int total,
-                      double maxpyr, int dither, int rows, int grid, hipStream_t s)
+                      const BPlane* planes, double maxpyr, int dither, int rows, int grid, hipStream_t s)
 {
     if (total <= 0) return 0;
     const double inv_scale = 255.0 / maxpyr;   // 2^-mfrac, exact
 #define ENBL_BFUSE(P, M, I)                                                                                          \
     do {                                                                                                             \
         if (level0) hipLaunchKernelGGL((batch_fuse_kernel<P, M, I, true>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total, \
-                                       maxpyr, inv_scale, frac, dither, rows);                                       \
+                                       planes, maxpyr, inv_scale, frac, dither, rows);                                       \
         else hipLaunchKernelGGL((batch_fuse_kernel<P, M, I, false>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total,   \
-                                maxpyr, inv_scale, frac, dither, rows);                                              \
+                                planes, maxpyr, inv_scale, frac, dither, rows);                                              \
     } while (0)
     if (itype == ENBL_U8) ENBL_BFUSE(int16_t, int16_t, uint8_t);
     else if (itype == ENBL_U16) ENBL_BFUSE(int32_t, int32_t, uint16_t);

@@ -95,6 +95,7 @@ struct Plan {
     double w_exp = 0, w_sat = 0, scale = 0, maxpyr = 0;
     int frac = 0, dither = 0;
     int max_k = 0;   // deepest stack
+    size_t images_off = 0, planes_off = 0;   // BImage / BPlane tables
 };
 
 // Device layout of one stack inside the arena.
@@ -260,6 +261,8 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
         P.launches.push_back(L);
     };
     auto at = [&](size_t off) { return arena + off; };
+    std::vector<BImage> images;   // weights pass: per-image inputs
+    std::vector<BPlane> planes;   // merge + collapse passes: per image fine, coarse, mask
     int Lmax = 0;
     P.max_k = 0;
     for (int s = 0; s < n; ++s) {
@@ -276,19 +279,22 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
         for (int s = 0; s < n; ++s) {
             const enbl_stack& S = st[s];
             const StackLayout& Y = lay[(size_t)s];
-            std::vector<void*> m0((size_t)S.K);
-            for (int m = 0; m < S.K; ++m) m0[(size_t)m] = at(Y.msk_at(S.K, m) + Y.msk[0].offset);
             BWeightJob J{};
             J.w = S.w;
             J.h = S.h;
             J.K = S.K;
             J.tiles_x = (S.w + 127) / 128;
             const int nt = J.tiles_x * ((S.h + tu.rows[kBatchWeights] - 1) / tu.rows[kBatchWeights]);
-            J.imgs = T.dev<const void*>(T.put(S.imgs, (size_t)S.K));
-            J.ipitch = T.dev<long>(T.put(S.ipitches, (size_t)S.K));
-            J.alphas = T.dev<const uint8_t*>(T.put(S.alphas, (size_t)S.K));
-            J.apitch = T.dev<long>(T.put(S.apitches, (size_t)S.K));
-            J.mask0 = T.dev<void*>(T.put(m0.data(), m0.size()));
+            J.img0 = (int)images.size();
+            for (int m = 0; m < S.K; ++m) {
+                images.emplace_back();
+                BImage& B = images.back();
+                B.img = S.imgs[m];
+                B.ipitch = S.ipitches[m];
+                B.alpha = S.alphas[m];
+                B.apitch = S.apitches[m];
+                B.mask0 = at(Y.msk_at(S.K, m) + Y.msk[0].offset);
+            }
             J.mpitch = Y.msk[0].pitch;
             J.ualpha = (uint8_t*)at(Y.ual_at(S.K) + Y.ual[0].offset);
             J.upitch = Y.ual[0].pitch;
@@ -374,31 +380,32 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             const StackLayout& Y = lay[(size_t)s];
             if (l >= S.levels) continue;
             const bool coarsest = l == S.levels - 1;
-            std::vector<BPlane> fine((size_t)S.K), coarse((size_t)S.K), mask((size_t)S.K);
+            BFuseJob J{};
+            J.plane0 = (int)planes.size();
             for (int m = 0; m < S.K; ++m) {
-                if (l == 0) fine[(size_t)m] = {S.imgs[m], S.ipitches[m], 0};
+                BPlane f, c, k;
+                if (l == 0) f = {S.imgs[m], S.ipitches[m], 0};
                 else {
                     const Level& F = Y.img[(size_t)l - 1];
-                    fine[(size_t)m] = {at(Y.img_at(m) + F.offset), F.pitch, F.plane};
+                    f = {at(Y.img_at(m) + F.offset), F.pitch, F.plane};
                 }
                 if (!coarsest) {
                     const Level& C = Y.img[(size_t)l];
-                    coarse[(size_t)m] = {at(Y.img_at(m) + C.offset), C.pitch, C.plane};
+                    c = {at(Y.img_at(m) + C.offset), C.pitch, C.plane};
                 } else {
-                    coarse[(size_t)m] = {nullptr, 0, 0};
+                    c = {nullptr, 0, 0};
                 }
                 const Level& M = Y.msk[(size_t)l];
-                mask[(size_t)m] = {at(Y.msk_at(S.K, m) + M.offset), M.pitch, M.plane};
+                k = {at(Y.msk_at(S.K, m) + M.offset), M.pitch, M.plane};
+                planes.push_back(f);
+                planes.push_back(c);
+                planes.push_back(k);
             }
-            BFuseJob J{};
             J.fw = Y.msk[(size_t)l].w;
             J.fh = Y.msk[(size_t)l].h;
             J.cw = coarsest ? 0 : Y.msk[(size_t)l + 1].w;
             J.ch = coarsest ? 0 : Y.msk[(size_t)l + 1].h;
             J.K = S.K;
-            J.fine = T.dev<BPlane>(T.put(fine.data(), fine.size()));
-            J.coarse = T.dev<BPlane>(T.put(coarse.data(), coarse.size()));
-            J.mask = T.dev<BPlane>(T.put(mask.data(), mask.size()));
             J.rcoarse = nullptr;
             J.rcpitch = J.rcplane = 0;
             if (!coarsest) {
@@ -438,6 +445,8 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
         }
         add_launch(kBatchFuse, l == 0, T.put(jobs.data(), jobs.size()), t0, total, bytes);
     }
+    P.images_off = T.put(images.data(), images.size());
+    P.planes_off = T.put(planes.data(), planes.size());
 }
 
 int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int time_kind, float* kind_ms)
@@ -462,6 +471,7 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
             const int* t0 = reinterpret_cast<const int*>(dtab + L.tile0);
             if (L.kind == kBatchWeights)
                 rc = launch_batch_weights(itype, P.max_k, reinterpret_cast<const BWeightJob*>(dtab + L.jobs), t0, L.njobs, L.total,
+                                          reinterpret_cast<const BImage*>(dtab + P.images_off),
                                           reinterpret_cast<const float*>(dtab + P.lut_off), P.w_exp, P.w_sat, P.scale,
                                           L.rows, L.grid, s);
             else if (L.kind == kBatchReduce)
@@ -469,7 +479,7 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
                                          L.grid, s);
             else
                 rc = launch_batch_fuse(itype, P.frac, L.level0, reinterpret_cast<const BFuseJob*>(dtab + L.jobs), t0,
-                                       L.njobs, L.total, P.maxpyr, P.dither, L.rows, L.grid, s);
+                                       L.njobs, L.total, reinterpret_cast<const BPlane*>(dtab + P.planes_off), P.maxpyr, P.dither, L.rows, L.grid, s);
             if (!rc) rc = launch_status(kKindName[L.kind]);
         }
         if (timed) (void)hipEventRecord(ev.back(), s);

@@ -18,21 +18,27 @@ struct BPlane {           // planar level (pyramid type), or an interleaved RGB 
     long plane;           // elements between channel planes (unused for an image)
 };
 
+// Per-image inputs of the weights pass (one table for the whole launch; a
+// job's images are entries img0 .. img0 + K - 1).
+struct BImage {
+    const void* img; long ipitch;       // interleaved image, pitch in elements
+    const uint8_t* alpha; long apitch;  // pitch in bytes
+    void* mask0;                        // mask level 0 (pitch: the job's mpitch)
+    long pad_;                          // 0
+};
+
 // Weights of all K images of a stack, their norm, the union alpha and the
 // normalised fixed-point mask level 0 of every image; a tile is 128 columns
 // (a pixel pair per lane) x `rows` rows.
 struct BWeightJob {
     int w, h, K;
     int tiles_x;
-    const void* const* imgs;        // device array of K interleaved images
-    const long* ipitch;             // device array (elements)
-    const uint8_t* const* alphas;   // device array
-    const long* apitch;             // device array (bytes)
-    void* const* mask0;             // device array of K mask level-0 planes
+    int img0;                       // first BImage entry of the stack
+    int pad0_;                      // 0 (no uninitialised bytes in a job table)
     long mpitch;
     uint8_t* ualpha; long upitch;   // union alpha level 0
     float fallback;                 // (float)(255 / K)
-    int pad_;                       // 0 (no uninitialised bytes in a job table)
+    int pad_;                       // 0
 };
 
 // Reduce with alpha of one pyramid level (image or mask pyramid), as the
@@ -58,9 +64,8 @@ struct BFuseJob {
     int fw, fh, cw, ch;
     int K;
     int tiles_x;                    // tiles across: 4-strip groups, or 64-column tiles at the coarsest level
-    const BPlane* fine;             // device array of K entries (level l; level 0: the images)
-    const BPlane* coarse;           // device array of K entries (level l+1)
-    const BPlane* mask;             // device array of K entries (mask level l)
+    int plane0;                     // BPlane entries plane0 + 3 m + {0: fine level l (level 0: the
+    int pad_;                       //   image), 1: coarse level l + 1, 2: mask level l} of image m; 0
     const void* rcoarse; long rcpitch; long rcplane;   // result level l+1
     void* res; long rpitch; long rplane;               // result level l (l > 0)
     void* out; long opitch;                            // level 0: output image (elements)
@@ -73,11 +78,11 @@ enum { kBatchWeights = 0, kBatchReduce = 1, kBatchFuse = 2, kBatchKinds = 3 };
 
 // max_k: the deepest stack of the launch (selects the register-resident instance).
 int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int* tile0, int njobs, int total_tiles,
-                         const float* lut, double w_exposure, double w_saturation, double scale, int rows,
+                         const BImage* images, const float* lut, double w_exposure, double w_saturation, double scale, int rows,
                          int grid, hipStream_t s);
 int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int njobs, int total_tiles, int grid,
                         hipStream_t s);
 int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, const int* tile0, int njobs,
-                      int total_tiles, double maxpyr, int dither, int rows, int grid, hipStream_t s);
+                      int total_tiles, const BPlane* planes, double maxpyr, int dither, int rows, int grid, hipStream_t s);
 
 }  // namespace enbl

@@ -15,6 +15,22 @@
 
 namespace enbl {
 
+// Every buffer the kernels touch is device (global) memory.  Loads and stores
+// through these views compile to global instructions (SGPR base + lane
+// offset) even where the pointer itself was read from a job table, instead of
+// flat instructions, whose completion the scalar-load waits also cover.
+template <class T> __device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// Native vectors for the wide accesses (HIP's int2/uint3 classes cannot be
+// read through an address-space-qualified pointer).
+typedef int32_t v2i32 __attribute__((ext_vector_type(2)));
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+
 constexpr int kWave = 64;
 // Lanes 1..62 of a wave own output columns; lanes 0 and 63 only supply the
 // horizontal halo through cross-lane shuffles.
@@ -77,15 +93,18 @@ template <> struct Pix<double> {
 // NaN follows the reference as built for x86-64 (cvttsd2si): INT_MIN for a
 // 32-bit target, 0 for a 16-bit target (low half of INT_MIN).
 template <class T> __device__ T from_real(double v);
+// Evaluated without branches: t = v -/+ 0.5 as in the reference, then
+// clamped to the target range (a t beyond the range belongs to a v beyond it,
+// and truncation inside the range is unchanged by the clamp), then truncated.
 template <> __device__ __forceinline__ int32_t from_real<int32_t>(double v) {
-    if (v != v) return INT32_MIN;
-    if (v < 0.0) return v < -2147483648.0 ? INT32_MIN : (int32_t)(v - 0.5);
-    return v > 2147483647.0 ? INT32_MAX : (int32_t)(v + 0.5);
+    const double t = v < 0.0 ? v - 0.5 : v + 0.5;
+    const int32_t r = (int32_t)fmin(fmax(t, -2147483648.0), 2147483647.0);
+    return v != v ? INT32_MIN : r;
 }
 template <> __device__ __forceinline__ int16_t from_real<int16_t>(double v) {
-    if (v != v) return 0;
-    if (v < 0.0) return v < -32768.0 ? (int16_t)-32768 : (int16_t)(int32_t)(v - 0.5);
-    return v > 32767.0 ? (int16_t)32767 : (int16_t)(int32_t)(v + 0.5);
+    const double t = v < 0.0 ? v - 0.5 : v + 0.5;
+    const int32_t r = (int32_t)fmin(fmax(t, -32768.0), 32767.0);
+    return v != v ? (int16_t)0 : (int16_t)r;
 }
 template <> __device__ __forceinline__ double from_real<double>(double v) { return v; }
 

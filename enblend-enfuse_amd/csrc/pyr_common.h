@@ -11,57 +11,57 @@ template <class P> struct Vec2;
 template <> struct Vec2<int16_t> {
     __device__ static __forceinline__ void load(const int16_t* p, bool both, int32_t& a, int32_t& b) {
         if (both) {
-            const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+            const uint32_t v = *gmem(reinterpret_cast<const uint32_t*>(p));
             a = (int16_t)(v & 0xffffu);
             b = (int16_t)(v >> 16);
         } else {
-            a = p[0];
+            a = *gmem(p);
             b = 0;
         }
     }
     __device__ static __forceinline__ void store(int16_t* p, bool both, int16_t a, int16_t b) {
         if (both) {
-            *reinterpret_cast<uint32_t*>(p) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16);
+            *gmem(reinterpret_cast<uint32_t*>(p)) = (uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16);
         } else {
-            p[0] = a;
+            *gmem(p) = a;
         }
     }
 };
 template <> struct Vec2<int32_t> {
     __device__ static __forceinline__ void load(const int32_t* p, bool both, int32_t& a, int32_t& b) {
         if (both) {
-            const int2 v = *reinterpret_cast<const int2*>(p);
+            const v2i32 v = *gmem(reinterpret_cast<const v2i32*>(p));
             a = v.x;
             b = v.y;
         } else {
-            a = p[0];
+            a = *gmem(p);
             b = 0;
         }
     }
     __device__ static __forceinline__ void store(int32_t* p, bool both, int32_t a, int32_t b) {
         if (both) {
-            *reinterpret_cast<int2*>(p) = make_int2(a, b);
+            *gmem(reinterpret_cast<v2i32*>(p)) = v2i32{a, b};
         } else {
-            p[0] = a;
+            *gmem(p) = a;
         }
     }
 };
 template <> struct Vec2<double> {
     __device__ static __forceinline__ void load(const double* p, bool both, double& a, double& b) {
         if (both) {
-            const double2 v = *reinterpret_cast<const double2*>(p);
+            const v2f64 v = *gmem(reinterpret_cast<const v2f64*>(p));
             a = v.x;
             b = v.y;
         } else {
-            a = p[0];
+            a = *gmem(p);
             b = 0.0;
         }
     }
     __device__ static __forceinline__ void store(double* p, bool both, double a, double b) {
         if (both) {
-            *reinterpret_cast<double2*>(p) = make_double2(a, b);
+            *gmem(reinterpret_cast<v2f64*>(p)) = v2f64{a, b};
         } else {
-            p[0] = a;
+            *gmem(p) = a;
         }
     }
 };

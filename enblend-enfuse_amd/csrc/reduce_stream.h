@@ -45,21 +45,23 @@ template <class I, class P> struct ImageRows {
     template <int C, class Args>
     __device__ static __forceinline__ void load(const Args& a, int r, int c, bool both, S (&x0)[C], S (&x1)[C]) {
         const I* p = static_cast<const I*>(a.src) + (long)r * a.spitch + (long)c * C;
+        const auto* g = gmem(p);
         I v[2 * C];
         if constexpr (sizeof(I) == 2 && C == 3) {
             // two RGB16 pixels = 12 bytes: one dwordx3 access when aligned
             if (both && ((uintptr_t)p & 3) == 0) {
-                const uint3 d = *reinterpret_cast<const uint3*>(p);
-                v[0] = (I)(d.x & 0xffffu); v[1] = (I)(d.x >> 16);
-                v[2] = (I)(d.y & 0xffffu); v[3] = (I)(d.y >> 16);
-                v[4] = (I)(d.z & 0xffffu); v[5] = (I)(d.z >> 16);
+                const auto* d = gmem(reinterpret_cast<const uint32_t*>(p));
+                const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+                v[0] = (I)(d0 & 0xffffu); v[1] = (I)(d0 >> 16);
+                v[2] = (I)(d1 & 0xffffu); v[3] = (I)(d1 >> 16);
+                v[4] = (I)(d2 & 0xffffu); v[5] = (I)(d2 >> 16);
             } else {
 #pragma unroll
-                for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+                for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? g[k] : (I)0;
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+            for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? g[k] : (I)0;
         }
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
@@ -100,7 +102,7 @@ struct ReduceLane {
         R.m0 = 0;
         R.m1 = 0;
         if (ALPHA && valid) {
-            const uint8_t* ar = a.salpha + (long)r * a.sapitch;
+            const auto* ar = gmem(a.salpha + (long)r * a.sapitch);
             R.m0 = ar[c] ? 1 : 0;
             R.m1 = (both && ar[c + 1]) ? 1 : 0;
         }
@@ -268,13 +270,13 @@ __device__ __forceinline__ void reduce_strip(const ReduceArgs& a, int wx, int i0
                 P out;
                 if (ALPHA) out = ap ? Px::wrap(A::div(v, ap)) : (P)0;
                 else out = Px::wrap(A::div(v, 256));
-                dst[ch * a.dplane + (long)i * a.dpitch + j] = out;
+                *gmem(dst + ch * a.dplane + (long)i * a.dpitch + j) = out;
             }
             hm2[ch] = hc[ch];
             hm1[ch] = hp1[ch];
             hc[ch] = hp2[ch];
         }
-        if (ALPHA && owned && a.dalpha) a.dalpha[(long)i * a.dapitch + j] = ap ? 255 : 0;
+        if (ALPHA && owned && a.dalpha) *gmem(a.dalpha + (long)i * a.dapitch + j) = (uint8_t)(ap ? 255 : 0);
         qm2 = qc; qm1 = qp1; qc = qp2;
         ra = na;
         rb = nb;
