@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
             const bool yb = Y + 1 < fh;
             const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
             const int dv[4] = {hdA * v0, hdB * v0, hdA * v1, hdB * v1};
-            // The inputs of image m + 1 are loaded while image m is merged.
+            // The inputs of image m at this coarse row.
             struct In {
                 S c[3][3];     // coarse rows i-1..i+1, own column
                 S f[4][3];     // fine samples of the 2 x 2 block
@@ -496,10 +496,9 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                 }
             };
             S acc[4][3];
-            In cur, nxt;
-            load_in(0, cur);
             for (int m = 0; m < J.K; ++m) {
-                if (m + 1 < J.K) load_in(m + 1, nxt);
+                In cur;
+                load_in(m, cur);
                 S o[4][3];
                 expand_sums<P, 3>(cur.c, i, j, cw, chh, o);
 #pragma unroll
@@ -514,7 +513,6 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                         acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
                     }
                 }
-                cur = nxt;
             }
             // Collapse step: + expand(result[l+1]), saturating (src/pyramid.h:1837-1886).
             S rcv[3][3], o[4][3];
