@@ -108,12 +108,23 @@ template <> __device__ __forceinline__ int16_t from_real<int16_t>(double v) {
 }
 template <> __device__ __forceinline__ double from_real<double>(double v) { return v; }
 
-// Cross-lane moves within a wave (ds_bpermute); 64-bit values move as two
-// 32-bit halves.
-__device__ __forceinline__ int32_t lane_up(int32_t v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ int32_t lane_down(int32_t v) { return __shfl_down(v, 1, kWave); }
-__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, kWave); }
+// Cross-lane moves within a wave: lane_up gives lane L the value of lane
+// L-1, lane_down the value of lane L+1, as DPP wave shifts (wave_shr:1,
+// wave_shl:1) folded into a VALU move instead of an LDS permute.  The lane
+// without a source (0 for lane_up, 63 for lane_down) receives 0; the kernels
+// only read these values in halo lanes.  64-bit values move as two halves.
+__device__ __forceinline__ int32_t lane_up(int32_t v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int32_t lane_down(int32_t v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ double lane_up(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)lane_up((int32_t)(uint32_t)b), hi = (uint32_t)lane_up((int32_t)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_down(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)lane_down((int32_t)(uint32_t)b), hi = (uint32_t)lane_down((int32_t)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 // Counter-based dither draw keyed by (seed, x, y, channel), mapped onto the
 // reference's integer grid (r & half_m1) + quarter (src/fixmath.h:246-264).
