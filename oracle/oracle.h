@@ -122,6 +122,19 @@ int orc_enfuse(int itype, int w, int h, int K, const void* const* imgs,
                int levels, int wrap, void* out, uint8_t* out_alpha,
                int dither_mode, uint64_t seed);
 
+/* Relaxed differentiable enfuse (FP64 throughout; not reference parity):
+ * K interleaved RGB double images (K*h*w*3), parameters prm[5] =
+ * {w_exposure, w_saturation, w_contrast, exposure_optimum, exposure_width},
+ * contrast window n.  forward writes the fused image out (h*w*3);
+ * backward writes d loss / d images (K*h*w*3) and d loss / d prm (5) for the
+ * output gradient gout (h*w*3). */
+int orc_diff_forward(int w, int h, int K, const double* imgs, const double* prm, int levels, int n, double* out);
+int orc_diff_backward(int w, int h, int K, const double* imgs, const double* prm, int levels, int n,
+                      const double* gout, double* gimgs, double* gprm);
+/* Transposed pyramid operators of the adjoint (accumulating). */
+void orc_diff_expand_T(const double* g, int fw, int fh, double* gc, int cw, int ch);
+void orc_diff_reduce_T(const double* g, int dw, int dh, double* gf, int sw, int sh);
+
 /* Number of OpenMP threads the pointwise stages use. */
 int orc_num_threads(void);
 

@@ -30,6 +30,10 @@ _SIGS = {
     "orc_num_threads": [],
     "orc_enfuse_mask": [_i, _vp, _vp, _i, _i, _vp, _vp],
     "orc_enfuse": [_i, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _i, ctypes.c_uint64],
+    "orc_diff_forward": [_i, _i, _i, _vp, _vp, _i, _i, _vp],
+    "orc_diff_backward": [_i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp],
+    "orc_diff_expand_T": [_vp, _i, _i, _vp, _i, _i],
+    "orc_diff_reduce_T": [_vp, _i, _i, _vp, _i, _i],
 }
 
 EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE = range(5)
@@ -210,6 +214,53 @@ def _enfuse(self, imgs, alphas, itype, prm, levels, wrap=False, dither=1, seed=0
 
 Oracle.enfuse_mask = _enfuse_mask
 Oracle.enfuse = _enfuse
+
+
+def _diff_forward(self, imgs, prm, levels, n=5):
+    """Relaxed differentiable enfuse: imgs (K, h, w, 3) float64, prm 5 values."""
+    imgs = np.ascontiguousarray(imgs, np.float64)
+    K, h, w, _ = imgs.shape
+    prm = np.ascontiguousarray(prm, np.float64)
+    out = np.zeros((h, w, 3), np.float64)
+    rc = self.lib.orc_diff_forward(w, h, K, _p(imgs), _p(prm), levels, n, _p(out))
+    if rc:
+        raise ValueError(f"orc_diff_forward rc={rc}")
+    return out
+
+
+def _diff_backward(self, imgs, prm, levels, gout, n=5):
+    imgs = np.ascontiguousarray(imgs, np.float64)
+    K, h, w, _ = imgs.shape
+    prm = np.ascontiguousarray(prm, np.float64)
+    gout = np.ascontiguousarray(gout, np.float64)
+    gimgs = np.zeros_like(imgs)
+    gprm = np.zeros(5, np.float64)
+    rc = self.lib.orc_diff_backward(w, h, K, _p(imgs), _p(prm), levels, n, _p(gout), _p(gimgs), _p(gprm))
+    if rc:
+        raise ValueError(f"orc_diff_backward rc={rc}")
+    return gimgs, gprm
+
+
+def _expand_T(self, g, cw, ch):
+    g = np.ascontiguousarray(g, np.float64)
+    fh, fw = g.shape
+    gc = np.zeros((ch, cw), np.float64)
+    self.lib.orc_diff_expand_T(_p(g), fw, fh, _p(gc), cw, ch)
+    return gc
+
+
+def _reduce_T(self, g, sw, sh):
+    g = np.ascontiguousarray(g, np.float64)
+    dh, dw = g.shape
+    gf = np.zeros((sh, sw), np.float64)
+    self.lib.orc_diff_reduce_T(_p(g), dw, dh, _p(gf), sw, sh)
+    return gf
+
+
+Oracle.diff_forward = _diff_forward
+Oracle.diff_backward = _diff_backward
+Oracle.expand_T = _expand_T
+Oracle.reduce_T = _reduce_T
 
 
 def build_oracle():
