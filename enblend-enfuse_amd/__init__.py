@@ -63,6 +63,10 @@ _SIGS = {
     "enbl_batch_get_tuning": (_c_int, [_vp, _vp]),
     "enbl_batch_set_tuning": (_c_int, [_vp, _vp]),
     "enbl_batch_autotune": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
+    "enbl_diff_enfuse_forward": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
+    "enbl_diff_enfuse_backward": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
+    "enbl_expand_adjoint": (_c_int, [_vp, _vp, _c_long, _c_int, _c_int, _vp, _c_long, _c_int, _c_int]),
+    "enbl_reduce_adjoint": (_c_int, [_vp, _vp, _c_long, _c_int, _c_int, _vp, _c_long, _c_int, _c_int]),
     "enbl_enblend_pair": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                    _vp, _c_long, _vp, _c_long, _vp, _c_long, _vp, _c_long,
                                    _vp, _c_long, _c_int, ctypes.c_uint64]),
@@ -234,6 +238,10 @@ class Context:
         s = _vp()
         check(lib().enbl_context_get_stream(self.h, ctypes.byref(s)))
         return s.value
+
+    def set_stream(self, hip_stream):
+        """Order the library's work on an external HIP stream (None: its own)."""
+        check(lib().enbl_context_set_stream(self.h, _vp(hip_stream) if hip_stream else None))
 
     def profile(self, on=True):
         check(lib().enbl_profile_enable(self.h, 1 if on else 0))

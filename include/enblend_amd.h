@@ -258,6 +258,42 @@ int enbl_batch_autotune(enbl_context* ctx, int itype, int nstacks, const enbl_st
                         const enbl_enfuse_params* prm, int dither_mode, int repeats,
                         enbl_batch_tuning* best);
 
+/* ---- differentiable enfuse (relaxed variant; not reference parity) ------- */
+
+/* The reference's arithmetic cannot be differentiated: its mask pyramid is
+ * fixed point and its float exposure weight is constant (y = gray / FLT_MAX,
+ * src/enfuse.h:637).  The relaxed variant keeps enfuseMain's dataflow
+ * (src/enfuse.h:1217-1609) in double throughout: y = gray, smooth saturation
+ * (max-min)/(max+min+1e-6), contrast sqrt(var + 1e-12) on the n x n window,
+ * weight w_e E + w_c sd + w_s s + 1e-12, normalisation, Gaussian pyramids of
+ * the log-transformed images and of the normalised weights, merge of the
+ * Laplacian levels, collapse, inverse transform.  Parameters: */
+typedef struct enbl_diff_params {
+    double w_exposure, w_saturation, w_contrast;
+    double exposure_optimum, exposure_width;   /* Gaussian exposure weight */
+} enbl_diff_params;
+
+/* Forward: K interleaved RGB double device images (K*h*w*3) -> out (h*w*3). */
+int enbl_diff_enfuse_forward(enbl_context* ctx, int w, int h, int K, const double* imgs,
+                             const enbl_diff_params* prm, int levels, int contrast_window, double* out);
+
+/* Adjoint: for the output gradient gout (h*w*3) writes d/d imgs (K*h*w*3) and
+ * d/d {w_exposure, w_saturation, w_contrast, exposure_optimum,
+ * exposure_width} (5 doubles, device).  Recomputes the forward pyramids; the
+ * Laplacian levels are evaluated on the fly, never stored. */
+int enbl_diff_enfuse_backward(enbl_context* ctx, int w, int h, int K, const double* imgs,
+                              const enbl_diff_params* prm, int levels, int contrast_window,
+                              const double* gout, double* gimgs, double* gparams);
+
+/* Transposed pyramid operators on one double plane (accumulate into the
+ * output): adjoint of expand (src/pyramid.h:873-1419, no wraparound) maps a
+ * fine gradient to the coarse level, adjoint of reduce with an all-set alpha
+ * (src/pyramid.h:166-573) maps a coarse gradient to the fine level. */
+int enbl_expand_adjoint(enbl_context* ctx, const double* g, long gpitch, int fw, int fh,
+                        double* gc, long cpitch, int cw, int ch);
+int enbl_reduce_adjoint(enbl_context* ctx, const double* g, long gpitch, int dw, int dh,
+                        double* gf, long fpitch, int sw, int sh);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510
