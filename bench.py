@@ -135,6 +135,122 @@ def batch_compare(args, enbl, scenes, torch, steps, warmup):
             "batched_kernels": _kernels(rep_b), "plain_kernels": _kernels(rep_p)}
 
 
+def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
+    """Differentiable enfuse, 5 x 2048x1536 float32 exposures (EV -4..+4),
+    loss = sum of squared differences from the Reinhard tone map of the scene
+    radiance: hand-written adjoint vs FP64 central differences of the GPU's
+    FP64 forward on 256 random input samples and all 5 parameters, then Adam on
+    the 5 parameters.  Steps are 1e-4 for the parameters and 1e-4 relative for
+    the image samples: the darkest exposure holds values near 6e-5, where an
+    absolute 1e-4 step crosses zero.  Each difference of two losses is summed
+    elementwise as (O1 - O2)(O1 + O2 - 2T)."""
+    import numpy as np
+    diff = _load("enblend_diff", os.path.join(ROOT, "enblend-enfuse_amd", "diff.py"))
+    dev = torch.device("cuda:0")
+    print("bench: generating the 5 x 2048x1536 float scene", file=sys.stderr, flush=True)
+    f32, target = scenes.float_scene(torch, 2048, 1536, [-4, -2, 0, 2, 4], seed=4)
+    imgs = torch.stack([t.to(torch.float64) for t in f32]).contiguous()
+    K, h, w, _ = imgs.shape
+    L = enbl.auto_levels(w, h)
+    ctx = enbl.Context(0)
+    prm0 = torch.tensor([1.0, 0.2, 0.5, 0.5, 0.2], dtype=torch.float64)
+
+    def loss_diff(i1, p1, i2, p2):
+        """loss(i1, p1) - loss(i2, p2) = sum (O1 - O2)(O1 + O2 - 2 T), elementwise,
+        so the difference of two nearby losses keeps its precision."""
+        o1 = diff.forward(enbl, ctx, i1, p1, L)
+        o2 = diff.forward(enbl, ctx, i2, p2, L)
+        return float(torch.sum((o1 - o2) * (o1 + o2 - 2.0 * target)))
+
+    out = diff.forward(enbl, ctx, imgs, prm0, L)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        out = diff.forward(enbl, ctx, imgs, prm0, L)
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / 3
+    gout = 2.0 * (out - target)
+    gi, gp = diff.backward(enbl, ctx, imgs, prm0, L, gout)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gi, gp = diff.backward(enbl, ctx, imgs, prm0, L, gout)
+    bwd_ms = (time.perf_counter() - t0) * 1e3 / 3
+    gi_h, gp_h = gi.cpu().numpy(), gp.cpu().numpy()
+    print(f"bench: finite differences on {fd_points} samples + 5 parameters", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(4)
+    step = 1e-4
+    # Every estimate is taken at two refinements; one that moves by more than
+    # 1e-5 relative between them (FP64 rounding noise of the whole forward, or a
+    # kink of max/min within the step) cannot test a 1e-4 claim and is reported
+    # apart.
+    pscale = max(float(np.abs(gp_h).max()), 1e-300)
+    iscale = float(np.abs(gi_h).max())
+    checked, unresolved = [], []
+
+    def judge(g, fd1, fd2, floor, what):
+        den = max(abs(fd2), floor)
+        rec = {"what": what, "adjoint": float(g), "fd": float(fd2), "rel_err": abs(g - fd2) / den}
+        (checked if abs(fd1 - fd2) <= 1e-5 * den else unresolved).append(rec)
+
+    for q in range(5):
+        fds = []
+        for hs in (step, step / 2):
+            p1, p2 = prm0.clone(), prm0.clone()
+            p1[q] += hs
+            p2[q] -= hs
+            fds.append(loss_diff(imgs, p1, imgs, p2) / (2 * hs))
+        judge(gp_h[q], fds[0], fds[1], 1e-6 * pscale, f"param {q}")
+    i1, i2 = imgs.clone(), imgs.clone()
+    for _ in range(fd_points):
+        k, y, x, c = (int(rng.integers(0, K)), int(rng.integers(0, h)), int(rng.integers(0, w)), int(rng.integers(0, 3)))
+        old = float(imgs[k, y, x, c])
+        fds = []
+        for rel in (1e-2, 5e-3, 2.5e-3):
+            hs = rel * max(abs(old), 1e-6)
+            i1[k, y, x, c] = old + hs
+            i2[k, y, x, c] = old - hs
+            fds.append(loss_diff(i1, prm0, i2, prm0) / (2 * hs))
+        i1[k, y, x, c] = old
+        i2[k, y, x, c] = old
+        # Richardson extrapolation of the central differences (cancels the h^2 term)
+        r1 = (4.0 * fds[1] - fds[0]) / 3.0
+        r2 = (4.0 * fds[2] - fds[1]) / 3.0
+        judge(gi_h[k, y, x, c], r1, r2, 1e-6 * iscale, f"image {k} ({y},{x}) ch {c} value {old:.3e}")
+    print(f"bench: {adam_steps} Adam steps on the 5 parameters", file=sys.stderr, flush=True)
+    prm = prm0.clone().to(dev).requires_grad_(True)
+    opt = torch.optim.Adam([prm], lr=0.01)
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(adam_steps):
+        opt.zero_grad()
+        o = diff.DiffEnfuse.apply(imgs, prm, enbl, ctx, L, 5)
+        loss = torch.sum((o - target) ** 2)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            prm[:3].clamp_(min=0.0)
+            prm[3].clamp_(0.0, 1.0)
+            prm[4].clamp_(min=0.02)
+        losses.append(float(loss.detach()))
+    adam_s = time.perf_counter() - t0
+    ctx.close()
+    mpix = K * w * h / 1e6
+    return {"workload": "differentiable enfuse (relaxed float variant), 5 x 2048x1536 RGB, EV -4..+4, FP64 pyramids, "
+                        f"{L} levels, L2 loss to the Reinhard tone map",
+            "adjoint_rel_err_max": max(r["rel_err"] for r in checked),
+            "adjoint_rel_err_params": [r["rel_err"] for r in checked + unresolved if r["what"].startswith("param")],
+            "fd_checked": len(checked), "fd_unresolved": len(unresolved),
+            "fd_unresolved_worst": sorted(unresolved, key=lambda r: -r["rel_err"])[:3],
+            "fd": "central differences of the GPU's FP64 forward. Parameters: h = 1e-4, error against the h/2 "
+                  "estimate. Image samples (values down to 6e-5): steps 1e-2, 5e-3, 2.5e-3 x |value| with Richardson "
+                  "extrapolation, error against the finer extrapolate. An estimate that moves by > 1e-5 relative "
+                  "between its two refinements (FP64 rounding noise of the whole forward, or a max/min kink within "
+                  "the step) is reported as unresolved. Denominator floor 1e-6 x the largest gradient component",
+            "fd_points": fd_points + 5,
+            "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
+            "forward_mpix_per_s": round(mpix / (fwd_ms * 1e-3), 1),
+            "adam_steps": adam_steps, "adam_s": round(adam_s, 2), "loss_first": losses[0], "loss_last": losses[-1],
+            "params_initial": prm0.tolist(), "params_final": [float(v) for v in prm.detach().cpu()]}
+
+
 def enfuse_workload(args, enbl, scenes, np, torch):
     """enfuse timings of the 3 x 1024x768 u8 stack (plain path)."""
     ctx = enbl.Context(0)
@@ -176,9 +292,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="enblend", choices=["enblend", "enfuse-c0", "batch-c3"],
+    ap.add_argument("--workload", default="enblend", choices=["enblend", "enfuse-c0", "batch-c3", "adjoint-c4"],
                     help="enblend: the headline line (default); enfuse-c0 / batch-c3: enfuse timings")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-vs-plain comparison out of the line")
+    ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint check and tuning run out of the line")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--tuning", default=os.path.join(ROOT, "enblend-enfuse_amd", "batch_tuning.json"),
                     help="launch shapes of the batched passes (JSON written by --autotune)")
@@ -198,6 +315,9 @@ def main():
         return
     if args.workload == "batch-c3":
         print(json.dumps(batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)), flush=True)
+        return
+    if args.workload == "adjoint-c4":
+        print(json.dumps(adjoint_c4(args, enbl, scenes, torch)), flush=True)
         return
     print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
     white, wa, black, ba, mask = scenes.enblend_pair(seed=1)
@@ -296,6 +416,8 @@ def main():
     del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
     if not args.no_batch:
         result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_c4(args, enbl, scenes, torch)
     print(json.dumps(result), flush=True)
 
 

@@ -153,3 +153,32 @@ def device_stack(torch, w, h, K, seed, ev_step, bits=16, device="cuda"):
         v = torch.clamp(torch.round(v), 0, m)
         out.append(v.to(torch.int32).to(torch.uint16 if bits == 16 else torch.uint8).contiguous())
     return out
+
+
+def float_scene(torch, w, h, evs, seed, device="cuda"):
+    """Float exposures of one device-generated scene: radiance remapped
+    log-uniformly to [1e-3, 1e2], exposures R * 2^EV as linear float32 with no
+    clipping, and the Reinhard global tone map R / (1 + R) of the radiance.
+    Returns (list of (h, w, 3) float32 tensors, target (h, w, 3) float64)."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) * 7919 + 17)
+
+    def U(a, b, n=1):
+        return torch.rand(n, generator=g, device=device, dtype=torch.float64) * (b - a) + a
+
+    ys = torch.arange(h, device=device, dtype=torch.float64)[:, None]
+    xs = torch.arange(w, device=device, dtype=torch.float64)[None, :]
+    r = 0.2 + U(0.2, 0.8) * xs / max(w - 1, 1) + 0.0 * ys
+    for _ in range(40):
+        cx, cy = U(0, w), U(0, h)
+        s = U(5, 80)
+        r = r + U(0, 2) * torch.exp(-((xs - cx) ** 2 + (ys - cy) ** 2) / (2 * s * s))
+    for _ in range(20):
+        x0, x1 = sorted(int(v) for v in (U(0, w, 2).tolist()))
+        y0, y1 = sorted(int(v) for v in (U(0, h, 2).tolist()))
+        r[y0:y1 + 1, x0:x1 + 1] = U(0, 1.5)
+    rad = r[:, :, None] * U(0.7, 1.3, 3)[None, None, :]
+    lo, hi = float(rad.min()), float(rad.max())
+    rad = 1e-3 * torch.pow(torch.tensor(1e5, dtype=torch.float64, device=device), (rad - lo) / max(hi - lo, 1e-12))
+    imgs = [(rad * (2.0 ** ev)).to(torch.float32).contiguous() for ev in evs]
+    return imgs, (rad / (1.0 + rad)).contiguous()
