@@ -26,9 +26,21 @@ template <class T> __device__ __forceinline__ __attribute__((address_space(1))) 
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// Raw buffer accesses through a per-row resource: a lane whose byte offset
+// is kBufSkip (or any offset at or past the resource's size) reads zeros and
+// its store is dropped, with no branch and no memory traffic.  Kernels use
+// them where every iteration of a loop must issue the same memory
+// instructions (vmcnt waits then count exactly and never wait for stores).
+constexpr uint32_t kBufSkip = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+
 // Native vectors for the wide accesses (HIP's int2/uint3 classes cannot be
 // read through an address-space-qualified pointer).
 typedef int32_t v2i32 __attribute__((ext_vector_type(2)));
+typedef int32_t v3i32 __attribute__((ext_vector_type(3)));
+typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
 typedef double v2f64 __attribute__((ext_vector_type(2)));
 
 constexpr int kWave = 64;

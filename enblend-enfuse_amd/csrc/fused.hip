@@ -34,69 +34,219 @@ namespace {
 
 constexpr int kOwnedF = kWave - 4;
 
-// ---- sources of the fine level -------------------------------------------
-template <class P> struct PlaneSrc {
-    typedef typename Pix<P>::S S;
-    template <int C>
-    __device__ static __forceinline__ void load(const FusedLapArgs& a, int r, int c, bool both, S (&x0)[C], S (&x1)[C]) {
-        const P* src = static_cast<const P*>(a.src);
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) Vec2<P>::load(src + ch * a.splane + (long)r * a.spitch + c, both, x0[ch], x1[ch]);
+// ---- buffer accesses ------------------------------------------------------
+template <class T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ uint32_t bload<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+template <> __device__ __forceinline__ v2i32 bload<v2i32>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(v2i32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ v3i32 bload<v3i32>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(v3i32, __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ v2f64 bload<v2f64>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(v2f64, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t bload_u16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+__device__ __forceinline__ uint32_t bload_u8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
+template <class P> struct BPair;   // pair (2j, 2j+1) and single-sample buffer stores
+template <> struct BPair<int16_t> {
+    typedef uint32_t Pair;
+    __device__ static __forceinline__ Pair fetch(__amdgpu_buffer_rsrc_t r, uint32_t off) { return bload<uint32_t>(r, off); }
+    __device__ static __forceinline__ void store2(__amdgpu_buffer_rsrc_t r, uint32_t off, int16_t x, int16_t y) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(uint16_t)x | ((uint32_t)(uint16_t)y << 16), r, off, 0, 0);
+    }
+    __device__ static __forceinline__ void store1(__amdgpu_buffer_rsrc_t r, uint32_t off, int16_t x) {
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)x, r, off, 0, 0);
+    }
+};
+template <> struct BPair<int32_t> {
+    typedef v2i32 Pair;
+    __device__ static __forceinline__ Pair fetch(__amdgpu_buffer_rsrc_t r, uint32_t off) { return bload<v2i32>(r, off); }
+    __device__ static __forceinline__ void store2(__amdgpu_buffer_rsrc_t r, uint32_t off, int32_t x, int32_t y) {
+        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(u2{(uint32_t)x, (uint32_t)y}, r, off, 0, 0);
+    }
+    __device__ static __forceinline__ void store1(__amdgpu_buffer_rsrc_t r, uint32_t off, int32_t x) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x, r, off, 0, 0);
+    }
+};
+template <> struct BPair<double> {
+    typedef v2f64 Pair;
+    __device__ static __forceinline__ Pair fetch(__amdgpu_buffer_rsrc_t r, uint32_t off) { return bload<v2f64>(r, off); }
+    __device__ static __forceinline__ void store2(__amdgpu_buffer_rsrc_t r, uint32_t off, double x, double y) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v2f64{x, y}), r, off, 0, 0);
+    }
+    __device__ static __forceinline__ void store1(__amdgpu_buffer_rsrc_t r, uint32_t off, double x) {
+        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, off, 0, 0);
     }
 };
 
-// Interleaved integer image, v << frac (src/fixmath.h:197-202).
-template <class I, class P> struct IntImageSrc {
+__device__ __forceinline__ uint32_t round_up2(uint32_t v) { return (v + 1u) & ~1u; }
+__device__ __forceinline__ uint32_t round_up4(uint32_t v) { return (v + 3u) & ~3u; }
+
+// Column facts of the pixel pair (2j, 2j+1) a lane handles.
+struct LaneCols {
+    bool valid;   // j inside the coarse level (else the lane reads zeros)
+    bool both;    // 2j+1 inside the fine level
+    bool tailw;   // wave-uniform: some lane has valid && !both
+    int c;        // 2j
+};
+
+// ---- sources of the fine level -------------------------------------------
+// fetch() issues the loads of fine row r for this lane's pixel pair through
+// a buffer resource over that row (row_ok false: a zero-size resource, so the
+// same instructions issue and read nothing) and keeps the words as loaded;
+// unpack() converts them where the row is used, an iteration later.  A pair
+// whose second pixel is past the row end reads that pixel as padding of the
+// row's last aligned word (never past the word holding the row's last byte)
+// or not at all.
+
+// Pyramid planes (pitch padded to an even element count, so the pair at the
+// last odd column reads padding for its second element).
+template <class P, int C> struct PlaneSrc {
     typedef typename Pix<P>::S S;
-    template <int C>
-    __device__ static __forceinline__ void load(const FusedLapArgs& a, int r, int c, bool both, S (&x0)[C], S (&x1)[C]) {
-        const I* p = static_cast<const I*>(a.src) + (long)r * a.spitch + (long)c * C;
-        I v[2 * C];
-        if constexpr (sizeof(I) == 2 && C == 3) {
-            // two RGB16 pixels = 12 bytes: one dwordx3 access when aligned
-            if (both && ((uintptr_t)p & 3) == 0) {
-                const uint3 d = *reinterpret_cast<const uint3*>(p);
-                v[0] = (I)(d.x & 0xffffu); v[1] = (I)(d.x >> 16);
-                v[2] = (I)(d.y & 0xffffu); v[3] = (I)(d.y >> 16);
-                v[4] = (I)(d.z & 0xffffu); v[5] = (I)(d.z >> 16);
-            } else {
+    typedef BPair<P> B;
+    struct Raw { typename B::Pair v[C]; };
+    __device__ static __forceinline__ void fetch(const FusedLapArgs& a, int r, bool row_ok, const LaneCols& L, Raw& R) {
+        const P* row = static_cast<const P*>(a.src) + (long)r * a.spitch;
+        const uint32_t bytes = row_ok ? round_up2((uint32_t)a.fw) * (uint32_t)sizeof(P) : 0u;
+        const uint32_t off = L.valid ? (uint32_t)L.c * (uint32_t)sizeof(P) : kBufSkip;
 #pragma unroll
-                for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+        for (int ch = 0; ch < C; ++ch) R.v[ch] = B::fetch(buf_rsrc(row + ch * a.splane, bytes), off);
+    }
+    __device__ static __forceinline__ void unpack(const FusedLapArgs&, const Raw& R, S (&x0)[C], S (&x1)[C]) {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) Vec2<P>::split(R.v[ch], x0[ch], x1[ch]);
+    }
+};
+
+// Interleaved integer image, v << frac (src/fixmath.h:197-202).  WIDE: the
+// rows are aligned for whole-word pair loads (4 bytes for 16-bit samples, 2
+// bytes for one 8-bit channel); otherwise one load per sample.
+template <class I, class P, int C, bool WIDE> struct IntImageSrc {
+    typedef typename Pix<P>::S S;
+    static constexpr bool RGB16 = WIDE && sizeof(I) == 2 && C == 3;   // 12 bytes = 3 words per pair
+    static constexpr bool ONE = WIDE && C == 1;                       // 2 or 4 bytes per pair
+    static constexpr int NW = RGB16 ? 3 : (ONE ? 1 : 2 * C);
+    struct Raw { uint32_t w[NW]; };
+    __device__ static __forceinline__ void fetch(const FusedLapArgs& a, int r, bool row_ok, const LaneCols& L, Raw& R) {
+        const I* row = static_cast<const I*>(a.src) + (long)r * a.spitch;
+        const uint32_t rb = (uint32_t)a.fw * C * (uint32_t)sizeof(I);
+        const uint32_t off = L.valid ? (uint32_t)L.c * C * (uint32_t)sizeof(I) : kBufSkip;
+        if constexpr (RGB16) {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row, row_ok ? round_up4(rb) : 0u);
+            if (L.tailw) {
+                // a last-column pair has its 6 bytes in words 0-1; word 2 is past the row
+                const v2i32 lo = bload<v2i32>(rs, off);
+                R.w[0] = (uint32_t)lo.x;
+                R.w[1] = (uint32_t)lo.y;
+                R.w[2] = bload<uint32_t>(rs, L.both ? off + 8u : kBufSkip);
+            } else {
+                const v3i32 v = bload<v3i32>(rs, off);
+                R.w[0] = (uint32_t)v.x;
+                R.w[1] = (uint32_t)v.y;
+                R.w[2] = (uint32_t)v.z;
             }
+        } else if constexpr (ONE && sizeof(I) == 2) {
+            R.w[0] = bload<uint32_t>(buf_rsrc(row, row_ok ? round_up4(rb) : 0u), off);
+        } else if constexpr (ONE) {
+            R.w[0] = bload_u16(buf_rsrc(row, row_ok ? round_up2(rb) : 0u), off);
+        } else {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row, row_ok ? rb : 0u);
+            const uint32_t off1 = L.valid && L.both ? off + C * (uint32_t)sizeof(I) : kBufSkip;
+#pragma unroll
+            for (int k = 0; k < C; ++k) {
+                if constexpr (sizeof(I) == 2) {
+                    R.w[k] = bload_u16(rs, off + 2u * k);
+                    R.w[C + k] = bload_u16(rs, off1 + 2u * k);
+                } else {
+                    R.w[k] = bload_u8(rs, off + k);
+                    R.w[C + k] = bload_u8(rs, off1 + k);
+                }
+            }
+        }
+    }
+    __device__ static __forceinline__ void unpack(const FusedLapArgs& a, const Raw& R, S (&x0)[C], S (&x1)[C]) {
+        uint32_t v[2 * C];
+        if constexpr (RGB16) {
+            v[0] = R.w[0] & 0xffffu; v[1] = R.w[0] >> 16;
+            v[2] = R.w[1] & 0xffffu; v[3] = R.w[1] >> 16;
+            v[4] = R.w[2] & 0xffffu; v[5] = R.w[2] >> 16;
+        } else if constexpr (ONE) {
+            constexpr int B = 8 * sizeof(I);
+            v[0] = R.w[0] & ((1u << B) - 1u);
+            v[1] = (R.w[0] >> B) & ((1u << B) - 1u);
         } else {
 #pragma unroll
-            for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? p[k] : (I)0;
+            for (int k = 0; k < 2 * C; ++k) v[k] = R.w[k];
         }
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             x0[ch] = (S)(P)((P)v[ch] << a.frac);
-            x1[ch] = both ? (S)(P)((P)v[C + ch] << a.frac) : (S)0;
+            x1[ch] = (S)(P)((P)v[C + ch] << a.frac);
         }
     }
 };
 
 // Interleaved float image, 1 + log1p(v) or 1 / (1 - v) (src/fixmath.h:165-186).
-struct FloatImageSrc {
+template <int C> struct FloatImageSrc {
     typedef double S;
+    struct Raw { float f[2 * C]; };
     __device__ static __forceinline__ double conv(float f) {
         const double v = (double)f;
         return v >= 0.0 ? 1.0 + log1p(v) : 1.0 / (1.0 - v);
     }
-    template <int C>
-    __device__ static __forceinline__ void load(const FusedLapArgs& a, int r, int c, bool both, S (&x0)[C], S (&x1)[C]) {
-        const float* p = static_cast<const float*>(a.src) + (long)r * a.spitch + (long)c * C;
-        float f0[C], f1[C];
+    __device__ static __forceinline__ void fetch(const FusedLapArgs& a, int r, bool row_ok, const LaneCols& L, Raw& R) {
+        const float* row = static_cast<const float*>(a.src) + (long)r * a.spitch;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row, row_ok ? (uint32_t)a.fw * C * 4u : 0u);
+        const uint32_t off = L.valid ? (uint32_t)L.c * C * 4u : kBufSkip;
+        const uint32_t off1 = L.valid && L.both ? off + C * 4u : kBufSkip;
+        if constexpr (C == 3) {
+            const v3i32 p0 = bload<v3i32>(rs, off), p1 = bload<v3i32>(rs, off1);
+            R.f[0] = __int_as_float(p0.x); R.f[1] = __int_as_float(p0.y); R.f[2] = __int_as_float(p0.z);
+            R.f[3] = __int_as_float(p1.x); R.f[4] = __int_as_float(p1.y); R.f[5] = __int_as_float(p1.z);
+        } else {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            f0[ch] = p[ch];
-            f1[ch] = both ? p[C + ch] : 0.f;
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            x0[ch] = conv(f0[ch]);
-            x1[ch] = both ? conv(f1[ch]) : 0.0;
+            for (int k = 0; k < C; ++k) {
+                R.f[k] = __uint_as_float(bload<uint32_t>(rs, off + 4u * k));
+                R.f[C + k] = __uint_as_float(bload<uint32_t>(rs, off1 + 4u * k));
+            }
         }
     }
+    __device__ static __forceinline__ void unpack(const FusedLapArgs&, const Raw& R, S (&x0)[C], S (&x1)[C]) {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            x0[ch] = conv(R.f[ch]);
+            x1[ch] = conv(R.f[C + ch]);
+        }
+    }
+};
+
+// The alpha bytes of a pair: one 16-bit load when the rows are 2-byte
+// aligned (the byte after a last-column pair shares its 2-byte word),
+// otherwise one load per byte.
+template <bool WIDE> struct AlphaPair {
+    uint32_t w[WIDE ? 1 : 2];
+    __device__ __forceinline__ void fetch(const uint8_t* row, bool row_ok, int fw, const LaneCols& L) {
+        if constexpr (WIDE) {
+            w[0] = bload_u16(buf_rsrc(row, row_ok ? round_up2((uint32_t)fw) : 0u), L.valid ? (uint32_t)L.c : kBufSkip);
+        } else {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row, row_ok ? (uint32_t)fw : 0u);
+            w[0] = bload_u8(rs, L.valid ? (uint32_t)L.c : kBufSkip);
+            w[1] = bload_u8(rs, L.valid && L.both ? (uint32_t)L.c + 1u : kBufSkip);
+        }
+    }
+    __device__ __forceinline__ bool first() const { return WIDE ? (w[0] & 0xffu) != 0 : w[0] != 0; }
+    __device__ __forceinline__ bool second() const { return WIDE ? (w[0] >> 8) != 0 : w[1] != 0; }
 };
 
 template <class S, int C>
@@ -110,7 +260,27 @@ struct CoarseRow {
     S hA[C], hB[C];   // polyphase row sums of one collapsed/Gaussian coarse row
 };
 
-template <class P, int C, class SRC>
+// C++ truncating division of an integer accumulator by 2^s.
+__device__ __forceinline__ int32_t div_pow2(int32_t a, int s) { return (a + ((a >> 31) & ((1 << s) - 1))) >> s; }
+
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+
+template <int N> using Phase = std::integral_constant<int, N>;
+
+// Streaming structure: coarse row k of the strip needs fine rows 2k-2..2k+2
+// (horizontal sums of rows 2k+1, 2k+2 are formed at step k) and the
+// Laplacian of fine rows 2k-2, 2k-1 is written at step k, once coarse row k
+// exists.  Raw fine rows live in a ring of 8 register slots (row r in slot
+// (r - 2k0 + 2) & 7); the loop is unrolled by its period of 4 steps so every
+// slot index is a constant and no in-flight register is ever copied.  Step k
+// loads rows 2k+3, 2k+4 for step k+1 and issues the same memory instructions
+// whatever it writes (rows and lanes that must not be touched get a zero-size
+// resource or an out-of-range offset), so the wait for the rows of step k-1
+// counts exactly and never waits for a store.
+template <class SRC> struct ConvertsOnUnpack : std::false_type {};
+template <int C> struct ConvertsOnUnpack<FloatImageSrc<C>> : std::true_type {};
+
+template <class P, int C, class SRC, bool AW>
 __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
 {
     typedef Pix<P> Px;
@@ -118,6 +288,13 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
     typedef typename Px::S S;
     typedef FRow<S, C> Row;
     typedef CoarseRow<S, C> CRow;
+    typedef BPair<P> B;
+    constexpr bool INT = std::is_same<A, AccI>::value;
+    constexpr uint32_t ES = sizeof(P);
+    struct RawRow {
+        typename SRC::Raw px;
+        AlphaPair<AW> al;
+    };
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
@@ -132,26 +309,30 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
     const bool owned = lane >= 2 && lane < kWave - 2 && valid;
     const bool both = c + 1 < fw;
     const int rcase = c == fw - 1 ? 1 : (c == fw - 2 ? 2 : 0);
+    LaneCols L;
+    L.valid = valid;
+    L.both = both;
+    L.tailw = !wave_all(!valid || both);
+    L.c = c;
 
-    auto load = [&](int r, Row& R) {
-        R.m0 = 0;
-        R.m1 = 0;
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) { R.x0[ch] = A::zero(); R.x1[ch] = A::zero(); }
-        if (!valid) return;
-        const uint8_t* ar = a.salpha + (long)r * a.sapitch;
-        R.m0 = ar[c] ? 1 : 0;
-        R.m1 = (both && ar[c + 1]) ? 1 : 0;
-        SRC::template load<C>(a, r, c, both, R.x0, R.x1);
+    auto fetch = [&](int r, bool want, RawRow& R) {
+        const bool ok = want && r >= 0 && r < fh;
+        const int rr = ok ? r : 0;
+        SRC::fetch(a, rr, ok, L, R.px);
+        R.al.fetch(a.salpha + (long)rr * a.sapitch, ok, fw, L);
+    };
+    auto unpack = [&](const RawRow& R, Row& X) {
+        SRC::unpack(a, R.px, X.x0, X.x1);
+        X.m0 = valid && R.al.first() ? 1 : 0;
+        X.m1 = valid && both && R.al.second() ? 1 : 0;
     };
 
-    // Horizontal 5-tap sums with zero-extension (reduce with alpha).
+    // Horizontal 5-tap sums with zero-extension (reduce with alpha); the
+    // right-border cases are selected, not branched, per lane.
     auto sums = [&](const Row& R, S (&h)[C], int& hm) {
         const int ml0 = lane_up(R.m0), ml1 = lane_up(R.m1), mr0 = lane_down(R.m0);
         const int qa = c >= 2 ? ml0 : 0, qb = c >= 1 ? ml1 : 0;
-        hm = qa + 4 * qb + 6 * R.m0;
-        if (rcase != 1) hm += 4 * R.m1;
-        if (rcase == 0) hm += mr0;
+        hm = qa + 4 * qb + 6 * R.m0 + (rcase != 1 ? 4 * R.m1 : 0) + (rcase == 0 ? mr0 : 0);
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             const S x0 = R.m0 ? R.x0[ch] : A::zero();
@@ -160,28 +341,23 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
             const S va = c >= 2 ? l0 : A::zero();
             const S vb = c >= 1 ? l1 : A::zero();
             const S a0 = A::add(va, A::mul(vb, 4));
-            if (rcase == 1) h[ch] = A::add(a0, A::mul(x0, 6));
-            else if (rcase == 2) h[ch] = A::add(A::add(a0, A::mul(x0, 6)), A::mul(x1, 4));
-            else h[ch] = A::add(A::add(A::add(a0, A::mul(x0, 6)), A::mul(x1, 4)), r0);
+            const S t1 = A::add(a0, A::mul(x0, 6));
+            const S t2 = A::add(t1, A::mul(x1, 4));
+            const S t0 = A::add(t2, r0);
+            h[ch] = rcase == 1 ? t1 : (rcase == 2 ? t2 : t0);
         }
     };
 
     // Polyphase sums of a coarse row held one value per lane (no wraparound).
+    const bool cfirst = j == 0, clast = j == cw - 1 && j != 0;
     auto coarse_sums = [&](const S (&g)[C], CRow& out) {
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             const S x = g[ch];
             const S l = lane_up(x), r = lane_down(x);
-            if (j == 0) {
-                out.hA[ch] = A::add(A::add(A::zero(), A::mul(x, 6)), r);
-                out.hB[ch] = A::mul(A::add(x, r), 4);
-            } else if (j == cw - 1) {
-                out.hA[ch] = A::add(l, A::mul(x, 6));
-                out.hB[ch] = A::mul(x, 4);
-            } else {
-                out.hA[ch] = A::add(A::add(l, A::mul(x, 6)), r);
-                out.hB[ch] = A::mul(A::add(x, r), 4);
-            }
+            const S t = A::add(cfirst ? A::zero() : l, A::mul(x, 6));
+            out.hA[ch] = clast ? t : A::add(t, r);
+            out.hB[ch] = clast ? A::mul(x, 4) : A::mul(A::add(x, r), 4);
         }
     };
 
@@ -190,16 +366,34 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
     const bool xb = X + 1 < fw;
     const int hdA = expand_hsum(X, cw, false, fw_even);
     const int hdB = expand_hsum(X + 1, cw, false, fw_even);
+    // Power-of-two weight sums (every column but the first and last) divide
+    // by a shift; the decision is per wave and row, so it is a uniform branch.
+    const bool colp2 = wave_all(!owned || (((hdA & (hdA - 1)) == 0) && ((hdB & (hdB - 1)) == 0)));
+    const int sA = __builtin_ctz((unsigned)hdA), sB = __builtin_ctz((unsigned)hdB);
+    const uint32_t off2 = owned && xb ? (uint32_t)X * ES : kBufSkip;    // pair stores
+    const uint32_t off1 = owned && !xb ? (uint32_t)X * ES : kBufSkip;   // last odd column
+    const uint32_t offc = owned ? (uint32_t)j * ES : kBufSkip;          // coarse level
+    const uint32_t offa = owned ? (uint32_t)j : kBufSkip;               // coarse alpha
+    const uint32_t lbytes = (uint32_t)fw * ES;
     P* lap = static_cast<P*>(a.lap);
     P* coarse = static_cast<P*>(a.coarse);
 
     // Laplacian rows 2i and 2i+1 from the raw fine rows and the coarse rows
-    // i-1 (p), i (q) and i+1 (n, when `more`).
-    auto emit = [&](int i, const CRow& p, const CRow& q, const CRow& n, bool more, const Row& RA, const Row& RB) {
-        if (!owned) return;
+    // i-1 (p), i (q) and i+1 (n, when `more`); rows outside the strip are
+    // computed and dropped (`ok` false).
+    auto emit = [&](int i, const CRow& p, const CRow& q, const CRow& n, bool more, const RawRow& RA, const RawRow& RB,
+                    bool ok) {
         const int Y = 2 * i;
         const bool yb = Y + 1 < fh;
         const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
+        S xa0[C], xa1[C], xb0[C], xb1[C];
+        SRC::unpack(a, RA.px, xa0, xa1);
+        SRC::unpack(a, RB.px, xb0, xb1);
+        const bool fast = INT && colp2 && (v0 & (v0 - 1)) == 0 && (v1 & (v1 - 1)) == 0;
+        const int s0 = __builtin_ctz((unsigned)v0), s1 = __builtin_ctz((unsigned)v1);
+        const P* lrow = lap + (long)Y * a.lpitch;
+        const uint32_t b0 = ok ? lbytes : 0u, b1 = ok && yb ? lbytes : 0u;
+        // one channel at a time: its four expansions, differences and stores
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             S o00, o10, o01, o11;
@@ -214,32 +408,60 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
                 o01 = q.hA[ch];
                 o11 = q.hB[ch];
             }
-            P* row0 = lap + ch * a.lplane + (long)Y * a.lpitch + X;
-            Vec2<P>::store(row0, xb, Px::wrap(A::sub(RA.x0[ch], A::div(o00, hdA * v0))),
-                           Px::wrap(A::sub(RA.x1[ch], A::div(o10, hdB * v0))));
-            if (yb)
-                Vec2<P>::store(row0 + a.lpitch, xb, Px::wrap(A::sub(RB.x0[ch], A::div(o01, hdA * v1))),
-                               Px::wrap(A::sub(RB.x1[ch], A::div(o11, hdB * v1))));
+            S e00, e10, e01, e11;
+            if constexpr (INT) {
+                if (fast) {
+                    e00 = div_pow2(o00, sA + s0);
+                    e10 = div_pow2(o10, sB + s0);
+                    e01 = div_pow2(o01, sA + s1);
+                    e11 = div_pow2(o11, sB + s1);
+                } else {
+                    e00 = o00 / (hdA * v0);
+                    e10 = o10 / (hdB * v0);
+                    e01 = o01 / (hdA * v1);
+                    e11 = o11 / (hdB * v1);
+                }
+            } else {
+                e00 = A::div(o00, hdA * v0);
+                e10 = A::div(o10, hdB * v0);
+                e01 = A::div(o01, hdA * v1);
+                e11 = A::div(o11, hdB * v1);
+            }
+            const __amdgpu_buffer_rsrc_t r0 = buf_rsrc(lrow + ch * a.lplane, b0);
+            const __amdgpu_buffer_rsrc_t r1 = buf_rsrc(lrow + ch * a.lplane + a.lpitch, b1);
+            const P y00 = Px::wrap(A::sub(xa0[ch], e00)), y10 = Px::wrap(A::sub(xa1[ch], e10));
+            const P y01 = Px::wrap(A::sub(xb0[ch], e01)), y11 = Px::wrap(A::sub(xb1[ch], e11));
+            B::store2(r0, off2, y00, y10);
+            B::store2(r1, off2, y01, y11);
+            if (L.tailw) {
+                B::store1(r0, off1, y00);
+                B::store1(r1, off1, y01);
+            }
         }
     };
 
     const int k0 = max(i0 - 1, 0);
     const int k1 = min(i1, chh - 1);
-    Row w[5], n0, n1;
-    S h[5][C];
-    int q[5];
+    const int base = 2 * k0 - 2;
+    RawRow ring[8];
+    // Horizontal sums of the window rows 2k-2..2k+2: the two oldest only
+    // enter as h(2k-2) + 4 h(2k-1), kept combined (hc, qc).
+    S hc[C], h2[C], h3[C], h4[C];
+    int qc = 0, q2 = 0, q3 = 0, q4 = 0;
 #pragma unroll
-    for (int t = 0; t < 5; ++t) {
-        q[t] = 0;
+    for (int t = 0; t < 5; ++t) fetch(base + t, true, ring[t]);
+    {
+        S h0[C], h1[C];
+        int q0 = 0, q1 = 0;
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) h[t][ch] = A::zero();
-        const int r = 2 * k0 - 2 + t;
-        if (r >= 0 && r < fh) load(r, w[t]);
-    }
+        for (int ch = 0; ch < C; ++ch) h0[ch] = h1[ch] = h2[ch] = h3[ch] = h4[ch] = A::zero();
+        Row R;
+        if (base >= 0 && base < fh) { unpack(ring[0], R); sums(R, h0, q0); }
+        if (base + 1 >= 0 && base + 1 < fh) { unpack(ring[1], R); sums(R, h1, q1); }
+        if (base + 2 >= 0 && base + 2 < fh) { unpack(ring[2], R); sums(R, h2, q2); }
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        const int r = 2 * k0 - 2 + t;
-        if (r >= 0 && r < fh) sums(w[t], h[t], q[t]);
+        for (int ch = 0; ch < C; ++ch) hc[ch] = A::add(h0[ch], A::mul(h1[ch], 4));
+        qc = q0 + 4 * q1;
     }
     CRow g0, g1, g2;   // coarse rows k-2, k-1, k
 #pragma unroll
@@ -247,46 +469,82 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
         g0.hA[ch] = g0.hB[ch] = g1.hA[ch] = g1.hB[ch] = g2.hA[ch] = g2.hB[ch] = A::zero();
     }
 
-    for (int k = k0; k <= k1; ++k) {
+    auto step = [&](auto ph, int k) {
+        constexpr int PH = decltype(ph)::value;
+        constexpr int S0 = (2 * PH) & 7, S1 = (2 * PH + 1) & 7, S2 = (2 * PH + 2) & 7, S3 = (2 * PH + 3) & 7;
+        constexpr int S4 = (2 * PH + 4) & 7, S5 = (2 * PH + 5) & 7, S6 = (2 * PH + 6) & 7;
         const int r = 2 * k;
-        if (k < k1) {
-            if (r + 3 < fh) load(r + 3, n0);
-            if (r + 4 < fh) load(r + 4, n1);
+        fetch(r + 3, k < k1, ring[S5]);
+        fetch(r + 4, k < k1, ring[S6]);
+        if (r + 1 < fh) {
+            Row R;
+            unpack(ring[S3], R);
+            sums(R, h3, q3);
         }
-        if (r + 1 < fh) sums(w[3], h[3], q[3]);
-        if (r + 2 < fh) sums(w[4], h[4], q[4]);
+        if (r + 2 < fh) {
+            Row R;
+            unpack(ring[S4], R);
+            sums(R, h4, q4);
+        }
 
         // Vertical reduce: coarse row k (src/pyramid.h:350-384, :459-504).
-        const int q0 = r == 0 ? 0 : q[0] + 4 * q[1];
+        const int q0 = r == 0 ? 0 : qc;
         int ap;
-        if (r == fh - 1) ap = q0 + 6 * q[2];
-        else if (r == fh - 2) ap = q0 + 6 * q[2] + 4 * q[3];
-        else ap = q0 + 6 * q[2] + 4 * q[3] + q[4];
-        S gk[C];
+        if (r == fh - 1) ap = q0 + 6 * q2;
+        else if (r == fh - 2) ap = q0 + 6 * q2 + 4 * q3;
+        else ap = q0 + 6 * q2 + 4 * q3 + q4;
+        S v[C];
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            const S a0 = r == 0 ? A::zero() : A::add(h[0][ch], A::mul(h[1][ch], 4));
-            S v;
-            if (r == fh - 1) v = A::add(a0, A::mul(h[2][ch], 6));
-            else if (r == fh - 2) v = A::add(A::add(a0, A::mul(h[2][ch], 6)), A::mul(h[3][ch], 4));
-            else v = A::add(A::add(A::add(a0, A::mul(h[2][ch], 6)), A::mul(h[3][ch], 4)), h[4][ch]);
-            const P out = ap ? Px::wrap(A::div(v, ap)) : (P)0;
-            gk[ch] = Px::load(out);
-            if (owned && k >= i0 && k < i1) coarse[ch * a.cplane + (long)k * a.cpitch + j] = out;
+            const S a0 = r == 0 ? A::zero() : hc[ch];
+            if (r == fh - 1) v[ch] = A::add(a0, A::mul(h2[ch], 6));
+            else if (r == fh - 2) v[ch] = A::add(A::add(a0, A::mul(h2[ch], 6)), A::mul(h3[ch], 4));
+            else v[ch] = A::add(A::add(A::add(a0, A::mul(h2[ch], 6)), A::mul(h3[ch], 4)), h4[ch]);
         }
-        if (owned && k >= i0 && k < i1) a.calpha[(long)k * a.capitch + j] = ap ? 255 : 0;
+        P out[C];
+        if constexpr (INT) {
+            const int d = ap ? ap : 1;
+            if (wave_all((d & (d - 1)) == 0)) {
+                const int sd = __builtin_ctz((unsigned)d);
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) out[ch] = ap ? Px::wrap(div_pow2(v[ch], sd)) : (P)0;
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) out[ch] = ap ? Px::wrap(v[ch] / d) : (P)0;
+            }
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) out[ch] = ap ? Px::wrap(A::div(v[ch], ap)) : (P)0;
+        }
+        S gk[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) gk[ch] = Px::load(out[ch]);
+        {
+            const bool mine = k >= i0 && k < i1;
+            const uint32_t cb = mine ? (uint32_t)cw * ES : 0u;
+            const P* crow = coarse + (long)k * a.cpitch;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) {
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(crow + ch * a.cplane, cb);
+                if constexpr (sizeof(P) == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)out[ch], rs, offc, 0, 0);
+                else if constexpr (sizeof(P) == 4) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)out[ch], rs, offc, 0, 0);
+                else B::store1(rs, offc, out[ch]);
+            }
+            __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(ap ? 255 : 0),
+                                                 buf_rsrc(a.calpha + (long)k * a.capitch, mine ? (uint32_t)cw : 0u), offa, 0, 0);
+        }
 
         g0 = g1;
         g1 = g2;
         coarse_sums(gk, g2);
 
-        if (k >= 1 && k - 1 >= i0 && k - 1 < i1) {
+        {
             CRow p = g0;
             if (k - 1 == 0) {
 #pragma unroll
                 for (int ch = 0; ch < C; ++ch) { p.hA[ch] = A::zero(); p.hB[ch] = A::zero(); }
             }
-            emit(k - 1, p, g1, g2, true, w[0], w[1]);
+            emit(k - 1, p, g1, g2, true, ring[S0], ring[S1], k >= 1 && k - 1 >= i0 && k - 1 < i1);
         }
         if (k == chh - 1 && k >= i0 && k < i1) {
             CRow p = g1;
@@ -294,14 +552,40 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
 #pragma unroll
                 for (int ch = 0; ch < C; ++ch) { p.hA[ch] = A::zero(); p.hB[ch] = A::zero(); }
             }
-            emit(k, p, g2, g2, false, w[2], w[3]);
+            emit(k, p, g2, g2, false, ring[S2], ring[S3], true);
         }
 
-        if (k < k1) {
-            w[0] = w[2]; w[1] = w[3]; w[2] = w[4]; w[3] = n0; w[4] = n1;
 #pragma unroll
-            for (int ch = 0; ch < C; ++ch) { h[0][ch] = h[2][ch]; h[1][ch] = h[3][ch]; h[2][ch] = h[4][ch]; }
-            q[0] = q[2]; q[1] = q[3]; q[2] = q[4];
+        for (int ch = 0; ch < C; ++ch) {
+            hc[ch] = A::add(h2[ch], A::mul(h3[ch], 4));
+            h2[ch] = h4[ch];
+        }
+        qc = q2 + 4 * q3;
+        q2 = q4;
+    };
+
+    if constexpr (ConvertsOnUnpack<SRC>::value) {
+        // Float sources convert (log1p) on every unpack, so this pass is
+        // arithmetic-bound: one step body, with the ring shifted by copies.
+        for (int k = k0; k <= k1; ++k) {
+            step(Phase<0>(), k);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) ring[t] = ring[t + 2];
+        }
+    } else {
+        int k = k0;
+        for (; k + 3 <= k1; k += 4) {
+            step(Phase<0>(), k);
+            step(Phase<1>(), k + 1);
+            step(Phase<2>(), k + 2);
+            step(Phase<3>(), k + 3);
+        }
+        if (k <= k1) {
+            step(Phase<0>(), k);
+            if (k + 1 <= k1) {
+                step(Phase<1>(), k + 1);
+                if (k + 2 <= k1) step(Phase<2>(), k + 2);
+            }
         }
     }
 }
@@ -537,14 +821,24 @@ inline int strip_rows_f(int waves_x, int out_rows)
     return (int)r;
 }
 
-template <class P, class SRC>
-void launch_lap_t(FusedLapArgs a, int C, hipStream_t s)
+template <class P, int C, class SRC, bool AW>
+void launch_lap_t(FusedLapArgs a, hipStream_t s)
 {
     const int waves_x = (a.cw + kOwnedF - 1) / kOwnedF + 1;
     if (a.rows <= 0) a.rows = strip_rows_f(waves_x, a.ch);
     dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
-    if (C == 1) hipLaunchKernelGGL((laplacian_level_kernel<P, 1, SRC>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((laplacian_level_kernel<P, 3, SRC>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((laplacian_level_kernel<P, C, SRC, AW>), grid, dim3(256), 0, s, a);
+}
+
+// Alpha rows 2-byte aligned: one 16-bit load per pixel pair.
+inline bool alpha_wide(const FusedLapArgs& a) { return ((uintptr_t)a.salpha & 1) == 0 && (a.sapitch & 1) == 0; }
+
+template <class P, class SRC1, class SRC3>
+void launch_lap_c(const FusedLapArgs& a, int C, hipStream_t s)
+{
+    const bool aw = alpha_wide(a);
+    if (C == 1) { if (aw) launch_lap_t<P, 1, SRC1, true>(a, s); else launch_lap_t<P, 1, SRC1, false>(a, s); }
+    else { if (aw) launch_lap_t<P, 3, SRC3, true>(a, s); else launch_lap_t<P, 3, SRC3, false>(a, s); }
 }
 
 template <class P, class M, class I, bool MASK8>
@@ -563,15 +857,23 @@ int launch_laplacian_level(int ptype, int src_kind, const FusedLapArgs& a, int C
 {
     if (C != 1 && C != 3) return ENBL_E_UNSUPPORTED;
     if (src_kind == 0) {
-        if (ptype == ENBL_I16) { launch_lap_t<int16_t, PlaneSrc<int16_t>>(a, C, s); return 0; }
-        if (ptype == ENBL_I32) { launch_lap_t<int32_t, PlaneSrc<int32_t>>(a, C, s); return 0; }
-        if (ptype == ENBL_F64) { launch_lap_t<double, PlaneSrc<double>>(a, C, s); return 0; }
+        if (ptype == ENBL_I16) { launch_lap_c<int16_t, PlaneSrc<int16_t, 1>, PlaneSrc<int16_t, 3>>(a, C, s); return 0; }
+        if (ptype == ENBL_I32) { launch_lap_c<int32_t, PlaneSrc<int32_t, 1>, PlaneSrc<int32_t, 3>>(a, C, s); return 0; }
+        if (ptype == ENBL_F64) { launch_lap_c<double, PlaneSrc<double, 1>, PlaneSrc<double, 3>>(a, C, s); return 0; }
     } else if (src_kind == ENBL_U8 && ptype == ENBL_I16) {
-        launch_lap_t<int16_t, IntImageSrc<uint8_t, int16_t>>(a, C, s); return 0;
+        // one 8-bit channel: pairs are 2-byte words when the rows are 2-byte aligned
+        const bool iw = ((uintptr_t)a.src & 1) == 0 && (a.spitch & 1) == 0;
+        if (C == 1 && iw) launch_lap_c<int16_t, IntImageSrc<uint8_t, int16_t, 1, true>, IntImageSrc<uint8_t, int16_t, 3, false>>(a, C, s);
+        else launch_lap_c<int16_t, IntImageSrc<uint8_t, int16_t, 1, false>, IntImageSrc<uint8_t, int16_t, 3, false>>(a, C, s);
+        return 0;
     } else if (src_kind == ENBL_U16 && ptype == ENBL_I32) {
-        launch_lap_t<int32_t, IntImageSrc<uint16_t, int32_t>>(a, C, s); return 0;
+        // 16-bit samples: pairs are whole 32-bit words when the rows are 4-byte aligned
+        const bool iw = ((uintptr_t)a.src & 3) == 0 && (a.spitch & 1) == 0;
+        if (iw) launch_lap_c<int32_t, IntImageSrc<uint16_t, int32_t, 1, true>, IntImageSrc<uint16_t, int32_t, 3, true>>(a, C, s);
+        else launch_lap_c<int32_t, IntImageSrc<uint16_t, int32_t, 1, false>, IntImageSrc<uint16_t, int32_t, 3, false>>(a, C, s);
+        return 0;
     } else if (src_kind == ENBL_F32 && ptype == ENBL_F64) {
-        launch_lap_t<double, FloatImageSrc>(a, C, s); return 0;
+        launch_lap_c<double, FloatImageSrc<1>, FloatImageSrc<3>>(a, C, s); return 0;
     }
     return ENBL_E_UNSUPPORTED;
 }

@@ -7,8 +7,18 @@ namespace enbl {
 
 // Two adjacent samples (2j, 2j+1) of a row as one vector access; `both`
 // false loads/stores the first only.  Callers keep rows 2-element aligned.
+// fetch() always reads the whole pair and leaves it packed (Pair); split()
+// unpacks it where it is used, so a fetch issued ahead of its use stays in
+// flight.  fetch() at the last odd column reads one element past the row,
+// which pyramid planes hold as pitch padding.
 template <class P> struct Vec2;
 template <> struct Vec2<int16_t> {
+    typedef uint32_t Pair;
+    __device__ static __forceinline__ Pair fetch(const int16_t* p) { return *gmem(reinterpret_cast<const uint32_t*>(p)); }
+    __device__ static __forceinline__ void split(Pair v, int32_t& a, int32_t& b) {
+        a = (int16_t)(v & 0xffffu);
+        b = (int16_t)(v >> 16);
+    }
     __device__ static __forceinline__ void load(const int16_t* p, bool both, int32_t& a, int32_t& b) {
         if (both) {
             const uint32_t v = *gmem(reinterpret_cast<const uint32_t*>(p));
@@ -28,6 +38,12 @@ template <> struct Vec2<int16_t> {
     }
 };
 template <> struct Vec2<int32_t> {
+    typedef v2i32 Pair;
+    __device__ static __forceinline__ Pair fetch(const int32_t* p) { return *gmem(reinterpret_cast<const v2i32*>(p)); }
+    __device__ static __forceinline__ void split(Pair v, int32_t& a, int32_t& b) {
+        a = v.x;
+        b = v.y;
+    }
     __device__ static __forceinline__ void load(const int32_t* p, bool both, int32_t& a, int32_t& b) {
         if (both) {
             const v2i32 v = *gmem(reinterpret_cast<const v2i32*>(p));
@@ -47,6 +63,12 @@ template <> struct Vec2<int32_t> {
     }
 };
 template <> struct Vec2<double> {
+    typedef v2f64 Pair;
+    __device__ static __forceinline__ Pair fetch(const double* p) { return *gmem(reinterpret_cast<const v2f64*>(p)); }
+    __device__ static __forceinline__ void split(Pair v, double& a, double& b) {
+        a = v.x;
+        b = v.y;
+    }
     __device__ static __forceinline__ void load(const double* p, bool both, double& a, double& b) {
         if (both) {
             const v2f64 v = *gmem(reinterpret_cast<const v2f64*>(p));
