@@ -320,13 +320,7 @@ __device__ __forceinline__ void load_coarse(const P* p, long pitch, long plane, 
 // factor is exact.  Other values take the division.
 __device__ __forceinline__ double mask_coeff(int32_t v, double maxpyr, double inv_scale)
 {
-    if ((uint32_t)v < (1u << 24)) {
-        const double x = (double)v;
-        const double r = 1.0 / 255.0;
-        const double q0 = x * r;
-        const double e = __fma_rn(-q0, 255.0, x);
-        return __fma_rn(e, r, q0) * inv_scale;
-    }
+    if ((uint32_t)v < (1u << 24)) return quot255(v) * inv_scale;
     return (double)v / maxpyr;
 }
 

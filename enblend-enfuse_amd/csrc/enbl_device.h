@@ -138,6 +138,18 @@ __device__ __forceinline__ double lane_down(double v) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// The correctly rounded quotient v / 255.0 for 0 <= v < 2^24 (every value a
+// normalised fixed-point mask pyramid holds): one reciprocal product and one
+// fused residual correction, equal to the division on that whole range
+// (checked exhaustively by the test suite).
+__device__ __forceinline__ double quot255(int32_t v) {
+    const double x = (double)v;
+    const double r = 1.0 / 255.0;
+    const double q0 = x * r;
+    const double e = __fma_rn(-q0, 255.0, x);
+    return __fma_rn(e, r, q0);
+}
+
 // Counter-based dither draw keyed by (seed, x, y, channel), mapped onto the
 // reference's integer grid (r & half_m1) + quarter (src/fixmath.h:246-264).
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {

@@ -24,6 +24,7 @@ struct FusedCollapseArgs {
     int fw, fh;
     const void* mask; long mpitch; int mfrac;                   // mask level l (planes or 0/255 bytes)
     double white_value;
+    double wc_scale;                                            // set by the launcher
     void* img; long ipitch; int ifrac;                          // output image (level 0)
     const uint8_t* walpha; long wapitch;
     uint8_t* balpha; long bapitch;

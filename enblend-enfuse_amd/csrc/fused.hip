@@ -591,57 +591,134 @@ __global__ __launch_bounds__(256) void laplacian_level_kernel(FusedLapArgs a)
 }
 
 // ---- fused blend + collapse (+ output) -----------------------------------
-template <class S, int C, class M>
-struct FinePair {
-    S b0[C], b1[C], w0[C], w1[C];   // black / white samples X, X+1 of one fine row
-    M m0, m1;                       // mask samples
-    uint8_t wa0, wa1, ba0, ba1;     // alphas (output level only)
-};
-
 template <class P> __device__ __forceinline__ P blend_px(double wc, P w, P b) {
-    // CartesianBlendFunctor (src/blend.h:47-70)
-    if (wc >= 1.0) return w;
-    if (wc <= 0.0) return b;
-    return from_real<P>((wc * (double)w) + ((1.0 - wc) * (double)b));
+    // CartesianBlendFunctor (src/blend.h:47-70), selected rather than branched
+    const P mix = from_real<P>((wc * (double)w) + ((1.0 - wc) * (double)b));
+    return wc >= 1.0 ? w : (wc <= 0.0 ? b : mix);
 }
 
-template <class P, class I> struct OutConv;
-template <class P> struct OutConv<P, uint8_t> {
-    __device__ static __forceinline__ uint8_t conv(P v, int frac, int dither, uint64_t seed, int x, int y, int c) {
+// SpecializedConvertPyramidToScalarFunctor with the deterministic dither
+// draw (src/fixmath.h:2157-2294), without per-lane branches.
+template <class P, class I> struct OutConv {
+    __device__ static __forceinline__ I conv(P v, int frac, bool det, uint64_t seed, int x, int y, int c) {
+        constexpr int32_t hi = sizeof(I) == 1 ? 255 : 65535;
         const int32_t q = (int32_t)v >> frac, f = (int32_t)v & ((1 << frac) - 1);
-        int32_t r;
-        if (f >= (1 << (frac - 2)) && f < (3 << (frac - 2)))
-            r = dither == ENBL_DITHER_DET ? (dither_rand(seed, x, y, c, frac) <= f ? q + 1 : q) : (f >= (1 << (frac - 1)) ? q + 1 : q);
-        else r = f >= (1 << (frac - 2)) ? q + 1 : q;
-        return (uint8_t)min(255, max(0, r));
-    }
-};
-template <class P> struct OutConv<P, uint16_t> {
-    __device__ static __forceinline__ uint16_t conv(P v, int frac, int dither, uint64_t seed, int x, int y, int c) {
-        const int32_t q = (int32_t)v >> frac, f = (int32_t)v & ((1 << frac) - 1);
-        int32_t r;
-        if (f >= (1 << (frac - 2)) && f < (3 << (frac - 2)))
-            r = dither == ENBL_DITHER_DET ? (dither_rand(seed, x, y, c, frac) <= f ? q + 1 : q) : (f >= (1 << (frac - 1)) ? q + 1 : q);
-        else r = f >= (1 << (frac - 2)) ? q + 1 : q;
-        return (uint16_t)min(65535, max(0, r));
+        const int32_t q1 = 1 << (frac - 2), q3 = 3 << (frac - 2);
+        const bool up_rand = det ? dither_rand(seed, x, y, c, frac) <= f : f >= (1 << (frac - 1));
+        const bool band = f >= q1 && f < q3;
+        const int32_t r = (band ? up_rand : f >= q1) ? q + 1 : q;
+        return (I)min(hi, max(0, r));
     }
 };
 template <> struct OutConv<double, float> {
-    __device__ static __forceinline__ float conv(double v, int, int, uint64_t, int, int, int) {
+    __device__ static __forceinline__ float conv(double v, int, bool, uint64_t, int, int, int) {
         return (float)(v >= 1.0 ? expm1(v - 1.0) : 1.0 - 1.0 / v);
     }
 };
 
-// I = void: write the collapsed level back into black (levels >= 1);
-// otherwise convert into the interleaved output image of type I.
-template <class P, class M, int C, class I, bool MASK8>
+// Output pixel pair stores into an interleaved row.  u0 / u1: write pixel X /
+// X+1 (ownership, row end and the union alpha folded in).  A pair whose
+// pixels are both written is one access where the row alignment allows;
+// waves holding a pair with one pixel written (`split`, wave-uniform) add the
+// per-pixel stores.
+template <class I, int C, bool WIDE> struct OutPair {
+    static constexpr bool RGB16 = WIDE && sizeof(I) == 2 && C == 3;
+    static constexpr bool ONE = WIDE && C == 1 && sizeof(I) <= 2;
+    __device__ static __forceinline__ void store(__amdgpu_buffer_rsrc_t rs, int X, const I (&v0)[C], const I (&v1)[C],
+                                                 bool u0, bool u1, bool split) {
+        if constexpr (RGB16) {
+            const uint32_t o = 6u * (uint32_t)X;
+            typedef uint32_t u3 __attribute__((ext_vector_type(3)));
+            const uint32_t w0 = (uint32_t)v0[0] | ((uint32_t)v0[1] << 16);
+            const uint32_t w1 = (uint32_t)v0[2] | ((uint32_t)v1[0] << 16);
+            const uint32_t w2 = (uint32_t)v1[1] | ((uint32_t)v1[2] << 16);
+            __builtin_amdgcn_raw_buffer_store_b96(u3{w0, w1, w2}, rs, u0 && u1 ? o : kBufSkip, 0, 0);
+            if (split) {
+                const uint32_t a0 = u0 && !u1 ? o : kBufSkip, a1 = u1 && !u0 ? o : kBufSkip;
+                __builtin_amdgcn_raw_buffer_store_b32(w0, rs, a0, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v0[2], rs, a0 + 4u, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v1[0], rs, a1 + 6u, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(w2, rs, a1 + 8u, 0, 0);
+            }
+        } else if constexpr (ONE && sizeof(I) == 2) {
+            const uint32_t o = 2u * (uint32_t)X;
+            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v0[0] | ((uint32_t)v1[0] << 16), rs, u0 && u1 ? o : kBufSkip, 0, 0);
+            if (split) {
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v0[0], rs, u0 && !u1 ? o : kBufSkip, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v1[0], rs, u1 && !u0 ? o + 2u : kBufSkip, 0, 0);
+            }
+        } else if constexpr (ONE) {
+            const uint32_t o = (uint32_t)X;
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)((uint32_t)v0[0] | ((uint32_t)v1[0] << 8)), rs,
+                                                  u0 && u1 ? o : kBufSkip, 0, 0);
+            if (split) {
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v0[0], rs, u0 && !u1 ? o : kBufSkip, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v1[0], rs, u1 && !u0 ? o + 1u : kBufSkip, 0, 0);
+            }
+        } else if constexpr (sizeof(I) == 4 && C == 3) {
+            typedef uint32_t u3 __attribute__((ext_vector_type(3)));
+            const uint32_t o = 12u * (uint32_t)X;
+            __builtin_amdgcn_raw_buffer_store_b96(u3{__float_as_uint(v0[0]), __float_as_uint(v0[1]), __float_as_uint(v0[2])}, rs,
+                                                  u0 ? o : kBufSkip, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b96(u3{__float_as_uint(v1[0]), __float_as_uint(v1[1]), __float_as_uint(v1[2])}, rs,
+                                                  u1 ? o + 12u : kBufSkip, 0, 0);
+        } else {
+            constexpr uint32_t sz = sizeof(I);
+            const uint32_t o0 = u0 ? (uint32_t)X * C * sz : kBufSkip, o1 = u1 ? ((uint32_t)X + 1u) * C * sz : kBufSkip;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) {
+                if constexpr (sz == 1) {
+                    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v0[ch], rs, o0 + ch, 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v1[ch], rs, o1 + ch, 0, 0);
+                } else if constexpr (sz == 2) {
+                    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v0[ch], rs, o0 + 2u * ch, 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v1[ch], rs, o1 + 2u * ch, 0, 0);
+                } else {
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v0[ch]), rs, o0 + 4u * ch, 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v1[ch]), rs, o1 + 4u * ch, 0, 0);
+                }
+            }
+        }
+    }
+};
+
+// One pyramid element of a plane row at column j (coarse level reads).
+template <class P> __device__ __forceinline__ P bload_elem(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(P) == 2) return (P)(int16_t)(uint16_t)bload_u16(r, off);
+    else if constexpr (sizeof(P) == 4) return (P)bload<uint32_t>(r, off);
+    else return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, bload<v2i32>(r, off)));
+}
+
+// Streaming structure: step i blends fine rows 2i, 2i+1 of the white and
+// black Laplacian levels, adds the expansion of collapsed coarse rows i-1, i,
+// i+1 and writes the rows (at level 0, converted into the output image with
+// the union alpha).  Raw fine rows live in a 4-slot ring and raw coarse rows
+// in a 2-slot ring; the loop is unrolled by their period of 2 steps, so no
+// in-flight register is copied.  Step i loads fine rows 2i+2, 2i+3 and coarse
+// row i+2 for step i+1 and issues the same memory instructions whatever it
+// writes (zero-size resources and out-of-range offsets mask rows and lanes),
+// so the waits for the previous step's loads count exactly.
+//
+// I = void: the collapsed level goes back into `black` (levels >= 1);
+// otherwise it is converted into the interleaved output image of type I.
+// WIDE: image rows 4-byte aligned and alpha / mask rows 2-byte aligned.
+template <class P, class M, int C, class I, bool MASK8, bool WIDE>
 __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a)
 {
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
-    typedef FinePair<S, C, M> Fine;
+    typedef BPair<P> B;
     constexpr bool OUT = !std::is_same<I, void>::value;
+    typedef typename std::conditional<OUT, I, uint8_t>::type IT;
+    constexpr uint32_t ES = sizeof(P);
+    // raw words of one fine row for this lane's pair
+    struct FineRaw {
+        typename B::Pair b[C], w[C];
+        typename std::conditional<MASK8, AlphaPair<WIDE>, typename BPair<M>::Pair>::type m;
+        AlphaPair<WIDE> wa, ba;   // level 0 only
+    };
+    struct CoarseRaw { P v[C]; };
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
@@ -655,131 +732,143 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
     const bool owned = lane >= 1 && lane <= kOwned && valid;
     const int X = 2 * j;
     const bool xb = X + 1 < fw;
+    LaneCols L;
+    L.valid = owned;            // fine data is only read for owned columns
+    L.both = xb;
+    L.tailw = !wave_all(!owned || xb);
+    L.c = X;
     const bool fw_even = (fw & 1) == 0;
     const int hdA = expand_hsum(X, cw, false, fw_even);
     const int hdB = expand_hsum(X + 1, cw, false, fw_even);
-    const P* cs = static_cast<const P*>(a.coarse);
-    P* blk = static_cast<P*>(a.black);
-    const P* wht = static_cast<const P*>(a.white);
+    const bool colp2 = wave_all(!owned || (((hdA & (hdA - 1)) == 0) && ((hdB & (hdB - 1)) == 0)));
+    const int sA = __builtin_ctz((unsigned)hdA), sB = __builtin_ctz((unsigned)hdB);
+    const bool cfirst = j == 0, clast = j == cw - 1 && j != 0;
+    const uint32_t offc = valid ? (uint32_t)j * ES : kBufSkip;
+    const uint32_t off2 = owned && xb ? (uint32_t)X * ES : kBufSkip;
+    const uint32_t off1 = owned && !xb ? (uint32_t)X * ES : kBufSkip;
+    const uint32_t fbytes = round_up2((uint32_t)fw) * ES;   // pyramid rows: pitch-padded pairs
 
-    auto load_coarse = [&](int i, S (&x)[C]) {
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) x[ch] = valid ? Px::load(cs[ch * a.cplane + (long)i * a.cpitch + j]) : A::zero();
-    };
-    auto coarse_sums = [&](const S (&x)[C], S (&hA)[C], S (&hB)[C]) {
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const S v = x[ch];
-            const S l = lane_up(v), r = lane_down(v);
-            if (j == 0) {
-                hA[ch] = A::add(A::add(A::zero(), A::mul(v, 6)), r);
-                hB[ch] = A::mul(A::add(v, r), 4);
-            } else if (j == cw - 1) {
-                hA[ch] = A::add(l, A::mul(v, 6));
-                hB[ch] = A::mul(v, 4);
-            } else {
-                hA[ch] = A::add(A::add(l, A::mul(v, 6)), r);
-                hB[ch] = A::mul(A::add(v, r), 4);
-            }
-        }
-    };
-    auto load_fine = [&](int Y, Fine& F) {
-        if (!owned || Y >= fh) return;
+    auto fetch_fine = [&](int Y, bool want, FineRaw& F) {
+        const bool ok = want && Y < fh;
+        const long y = ok ? Y : 0;
+        const uint32_t nb = ok ? fbytes : 0u;
+        const uint32_t off = owned ? (uint32_t)X * ES : kBufSkip;
+        const P* brow = static_cast<const P*>(a.black) + y * a.fpitch;
+        const P* wrow = static_cast<const P*>(a.white) + y * a.wpitch;
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            Vec2<P>::load(blk + ch * a.fplane + (long)Y * a.fpitch + X, xb, F.b0[ch], F.b1[ch]);
-            Vec2<P>::load(wht + ch * a.wplane + (long)Y * a.wpitch + X, xb, F.w0[ch], F.w1[ch]);
+            F.b[ch] = B::fetch(buf_rsrc(brow + ch * a.fplane, nb), off);
+            F.w[ch] = B::fetch(buf_rsrc(wrow + ch * a.wplane, nb), off);
         }
-        if (MASK8) {
-            const uint8_t* m = static_cast<const uint8_t*>(a.mask) + (long)Y * a.mpitch + X;
-            F.m0 = (M)((M)m[0] << a.mfrac);
-            F.m1 = xb ? (M)((M)m[1] << a.mfrac) : (M)0;
+        if constexpr (MASK8) {
+            F.m.fetch(static_cast<const uint8_t*>(a.mask) + y * a.mpitch, ok, fw, L);
         } else {
-            const M* m = static_cast<const M*>(a.mask) + (long)Y * a.mpitch + X;
-            F.m0 = m[0];
-            F.m1 = xb ? m[1] : (M)0;
+            const M* mrow = static_cast<const M*>(a.mask) + y * a.mpitch;
+            F.m = BPair<M>::fetch(buf_rsrc(mrow, ok ? round_up2((uint32_t)fw) * (uint32_t)sizeof(M) : 0u),
+                                  owned ? (uint32_t)X * (uint32_t)sizeof(M) : kBufSkip);
         }
-        if (OUT) {
-            F.wa0 = a.walpha[(long)Y * a.wapitch + X];
-            F.ba0 = a.balpha[(long)Y * a.bapitch + X];
-            F.wa1 = xb ? a.walpha[(long)Y * a.wapitch + X + 1] : 0;
-            F.ba1 = xb ? a.balpha[(long)Y * a.bapitch + X + 1] : 0;
+        if constexpr (OUT) {
+            F.wa.fetch(a.walpha + y * a.wapitch, ok, fw, L);
+            F.ba.fetch(a.balpha + y * a.bapitch, ok, fw, L);
         }
     };
-    auto finish = [&](int Y, const Fine& F, const S (&e0)[C], const S (&e1)[C]) {
-        const double wc0 = (double)F.m0 / a.white_value;
-        const double wc1 = (double)F.m1 / a.white_value;
+    auto fetch_coarse = [&](int i, bool want, CoarseRaw& R) {
+        const bool ok = want && i >= 0 && i < chh;
+        const P* row = static_cast<const P*>(a.coarse) + (long)(ok ? i : 0) * a.cpitch;
+        const uint32_t nb = ok ? (uint32_t)cw * ES : 0u;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) R.v[ch] = bload_elem<P>(buf_rsrc(row + ch * a.cplane, nb), offc);
+    };
+    // polyphase sums of a coarse row held one value per lane (no wraparound)
+    auto coarse_sums = [&](const CoarseRaw& R, S (&hA)[C], S (&hB)[C]) {
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const S x = Px::load(R.v[ch]);
+            const S l = lane_up(x), r = lane_down(x);
+            const S t = A::add(cfirst ? A::zero() : l, A::mul(x, 6));
+            hA[ch] = clast ? t : A::add(t, r);
+            hB[ch] = clast ? A::mul(x, 4) : A::mul(A::add(x, r), 4);
+        }
+    };
+    // blend weight (double)m / white_value; white_value = 255 * 2^f takes
+    // the exact quotient by 255 times 2^-f for 0 <= m < 2^24
+    auto weight = [&](M m) -> double {
+        if (a.wc_scale != 0.0 && wave_all((uint32_t)(int32_t)m < (1u << 24))) return quot255((int32_t)m) * a.wc_scale;
+        return (double)m / a.white_value;
+    };
+    const bool det = a.dither == ENBL_DITHER_DET;
+
+    // blend + add the expansion e (per channel) + store of fine row Y
+    auto finish = [&](int Y, const FineRaw& F, const S (&e0)[C], const S (&e1)[C]) {
+        const bool ok = Y < fh;
+        M m0, m1;
+        if constexpr (MASK8) {
+            m0 = (M)((M)(F.m.w[0] & 0xffu) << a.mfrac);
+            m1 = (M)((M)(WIDE ? (F.m.w[0] >> 8) & 0xffu : F.m.w[WIDE ? 0 : 1]) << a.mfrac);
+        } else {
+            typename Pix<M>::S t0, t1;
+            Vec2<M>::split(F.m, t0, t1);
+            m0 = (M)t0;
+            m1 = (M)t1;
+        }
+        const double wc0 = weight(m0), wc1 = weight(m1);
         P o0[C], o1[C];
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            o0[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc0, (P)F.w0[ch], (P)F.b0[ch])), e0[ch]));
-            o1[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc1, (P)F.w1[ch], (P)F.b1[ch])), e1[ch]));
+            S b0, b1, w0, w1;
+            Vec2<P>::split(F.b[ch], b0, b1);
+            Vec2<P>::split(F.w[ch], w0, w1);
+            o0[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc0, (P)w0, (P)b0)), e0[ch]));
+            o1[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc1, (P)w1, (P)b1)), e1[ch]));
         }
         if constexpr (!OUT) {
+            const P* row = static_cast<const P*>(a.black) + (long)(ok ? Y : 0) * a.fpitch;
+            const uint32_t nb = ok ? (uint32_t)fw * ES : 0u;
 #pragma unroll
-            for (int ch = 0; ch < C; ++ch) Vec2<P>::store(blk + ch * a.fplane + (long)Y * a.fpitch + X, xb, o0[ch], o1[ch]);
+            for (int ch = 0; ch < C; ++ch) {
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row + ch * a.fplane, nb);
+                B::store2(rs, off2, o0[ch], o1[ch]);
+                if (L.tailw) B::store1(rs, off1, o0[ch]);
+            }
         } else {
-            typedef typename std::conditional<OUT, I, uint8_t>::type IT;
-            IT* img = static_cast<IT*>(a.img) + (long)Y * a.ipitch + (long)X * C;
-            const bool u0 = F.wa0 || F.ba0;
-            const bool u1 = F.wa1 || F.ba1;
-            if constexpr (sizeof(IT) == 2 && C == 3) {
-                if (u0 && xb && u1 && ((uintptr_t)img & 3) == 0) {
-                    IT v[6];
+            const bool wa0 = F.wa.first(), wa1 = F.wa.second();
+            const bool ba0 = F.ba.first(), ba1 = F.ba.second();
+            const bool u0 = owned && (wa0 || ba0);
+            const bool u1 = owned && xb && (wa1 || ba1);
+            IT v0[C], v1[C];
 #pragma unroll
-                    for (int ch = 0; ch < 3; ++ch) {
-                        v[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, a.dither, a.seed, X, Y, ch);
-                        v[3 + ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, a.dither, a.seed, X + 1, Y, ch);
-                    }
-                    *reinterpret_cast<uint3*>(img) = make_uint3((uint32_t)v[0] | ((uint32_t)v[1] << 16),
-                                                                (uint32_t)v[2] | ((uint32_t)v[3] << 16),
-                                                                (uint32_t)v[4] | ((uint32_t)v[5] << 16));
-                    if (F.wa0) a.balpha[(long)Y * a.bapitch + X] = 255;
-                    if (F.wa1) a.balpha[(long)Y * a.bapitch + X + 1] = 255;
-                    return;
-                }
+            for (int ch = 0; ch < C; ++ch) {
+                v0[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, det, a.seed, X, Y, ch);
+                v1[ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, det, a.seed, X + 1, Y, ch);
             }
-            if (u0) {
-#pragma unroll
-                for (int ch = 0; ch < C; ++ch) img[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, a.dither, a.seed, X, Y, ch);
+            const long y = ok ? Y : 0;
+            const IT* irow = static_cast<const IT*>(a.img) + y * a.ipitch;
+            const uint32_t ib = ok ? (uint32_t)fw * C * (uint32_t)sizeof(IT) : 0u;
+            const bool split = !wave_all(u0 == u1);
+            OutPair<IT, C, WIDE>::store(buf_rsrc(irow, ib), X, v0, v1, u0, u1, split);
+            // union alpha: black alpha = 255 where white alpha is set (src/enblend.h:461-463)
+            const uint8_t* arow = a.balpha + y * a.bapitch;
+            const uint32_t n0 = wa0 ? 255u : (WIDE ? F.ba.w[0] & 0xffu : F.ba.w[0]);
+            const uint32_t n1 = wa1 ? 255u : (WIDE ? F.ba.w[0] >> 8 : F.ba.w[WIDE ? 0 : 1]);
+            if constexpr (WIDE) {
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(arow, ok ? (uint32_t)fw : 0u);
+                __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(n0 | (n1 << 8)), rs, owned && xb ? (uint32_t)X : kBufSkip, 0, 0);
+                if (L.tailw) __builtin_amdgcn_raw_buffer_store_b8((unsigned char)n0, rs, owned && !xb ? (uint32_t)X : kBufSkip, 0, 0);
+            } else {
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(arow, ok ? (uint32_t)fw : 0u);
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)n0, rs, owned ? (uint32_t)X : kBufSkip, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)n1, rs, owned && xb ? (uint32_t)X + 1u : kBufSkip, 0, 0);
             }
-            if (xb && u1) {
-#pragma unroll
-                for (int ch = 0; ch < C; ++ch) img[C + ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, a.dither, a.seed, X + 1, Y, ch);
-            }
-            if (F.wa0) a.balpha[(long)Y * a.bapitch + X] = 255;
-            if (xb && F.wa1) a.balpha[(long)Y * a.bapitch + X + 1] = 255;
         }
     };
 
-    S pA[C], pB[C], cA[C], cB[C], nA[C], nB[C], xr[C], xn[C];
-#pragma unroll
-    for (int ch = 0; ch < C; ++ch) { pA[ch] = pB[ch] = cA[ch] = cB[ch] = nA[ch] = nB[ch] = xr[ch] = xn[ch] = A::zero(); }
-    {
-        S t[C];
-        if (i0 > 0) {
-            load_coarse(i0 - 1, t);
-            coarse_sums(t, pA, pB);
-        }
-        load_coarse(i0, t);
-        coarse_sums(t, cA, cB);
-    }
-    if (i0 + 1 < chh) load_coarse(i0 + 1, xr);
-    Fine f0, f1, g0, g1;
-    load_fine(2 * i0, f0);
-    load_fine(2 * i0 + 1, f1);
-
-    for (int i = i0; i < i1; ++i) {
-        const bool more = i + 1 < chh;
-        if (i + 1 < i1) {
-            if (i + 2 < chh) load_coarse(i + 2, xn);
-            load_fine(2 * i + 2, g0);
-            load_fine(2 * i + 3, g1);
-        }
-        if (more) coarse_sums(xr, nA, nB);
+    // expansion of the coarse rows into fine rows 2i (e00/e10) and 2i+1 (e01/e11)
+    auto expand2 = [&](int i, bool more, const S (&pA)[C], const S (&pB)[C], const S (&cA)[C], const S (&cB)[C],
+                       const S (&nA)[C], const S (&nB)[C], S (&e00)[C], S (&e10)[C], S (&e01)[C], S (&e11)[C]) {
         const int Y = 2 * i;
         const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
-        S e00[C], e10[C], e01[C], e11[C];
+        const bool fast = std::is_same<A, AccI>::value && colp2 && (v0 & (v0 - 1)) == 0 && (v1 & (v1 - 1)) == 0;
+        const int s0 = __builtin_ctz((unsigned)v0), s1 = __builtin_ctz((unsigned)v1);
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
             S o00, o10, o01, o11;
@@ -794,22 +883,69 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
                 o01 = cA[ch];
                 o11 = cB[ch];
             }
-            e00[ch] = A::div(o00, hdA * v0);
-            e10[ch] = A::div(o10, hdB * v0);
-            e01[ch] = A::div(o01, hdA * v1);
-            e11[ch] = A::div(o11, hdB * v1);
+            if constexpr (std::is_same<A, AccI>::value) {
+                if (fast) {
+                    e00[ch] = div_pow2(o00, sA + s0);
+                    e10[ch] = div_pow2(o10, sB + s0);
+                    e01[ch] = div_pow2(o01, sA + s1);
+                    e11[ch] = div_pow2(o11, sB + s1);
+                } else {
+                    e00[ch] = o00 / (hdA * v0);
+                    e10[ch] = o10 / (hdB * v0);
+                    e01[ch] = o01 / (hdA * v1);
+                    e11[ch] = o11 / (hdB * v1);
+                }
+            } else {
+                e00[ch] = A::div(o00, hdA * v0);
+                e10[ch] = A::div(o10, hdB * v0);
+                e01[ch] = A::div(o01, hdA * v1);
+                e11[ch] = A::div(o11, hdB * v1);
+            }
         }
-        if (owned) {
-            finish(Y, f0, e00, e10);
-            if (Y + 1 < fh) finish(Y + 1, f1, e01, e11);
-        }
+    };
+
+    S pA[C], pB[C], cA[C], cB[C];
+    FineRaw fr[4];
+    CoarseRaw cr[2];
+    {
+        CoarseRaw tp, tc;
+        fetch_coarse(i0 - 1, i0 > 0, tp);
+        fetch_coarse(i0, true, tc);
+        fetch_coarse(i0 + 1, true, cr[1]);
+        fetch_fine(2 * i0, true, fr[0]);
+        fetch_fine(2 * i0 + 1, true, fr[1]);
+        coarse_sums(tp, pA, pB);
+        if (i0 == 0) {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            pA[ch] = cA[ch]; pB[ch] = cB[ch]; cA[ch] = nA[ch]; cB[ch] = nB[ch]; xr[ch] = xn[ch];
+            for (int ch = 0; ch < C; ++ch) pA[ch] = pB[ch] = A::zero();
         }
-        f0 = g0;
-        f1 = g1;
+        coarse_sums(tc, cA, cB);
     }
+
+    auto step = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        constexpr int F0 = 2 * PH, F1 = 2 * PH + 1, N0 = 2 * (1 - PH), N1 = 2 * (1 - PH) + 1;
+        const bool next = i + 1 < i1;
+        fetch_coarse(i + 2, next, cr[PH]);
+        fetch_fine(2 * i + 2, next, fr[N0]);
+        fetch_fine(2 * i + 3, next, fr[N1]);
+        const bool more = i + 1 < chh;
+        S nA[C], nB[C];
+        coarse_sums(cr[1 - PH], nA, nB);
+        S e00[C], e10[C], e01[C], e11[C];
+        expand2(i, more, pA, pB, cA, cB, nA, nB, e00, e10, e01, e11);
+        finish(2 * i, fr[F0], e00, e10);
+        finish(2 * i + 1, fr[F1], e01, e11);
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) { pA[ch] = cA[ch]; pB[ch] = cB[ch]; cA[ch] = nA[ch]; cB[ch] = nB[ch]; }
+    };
+
+    int i = i0;
+    for (; i + 1 < i1; i += 2) {
+        step(Phase<0>(), i);
+        step(Phase<1>(), i + 1);
+    }
+    if (i < i1) step(Phase<0>(), i);
 }
 
 inline int strip_rows_f(int waves_x, int out_rows)
@@ -841,14 +977,39 @@ void launch_lap_c(const FusedLapArgs& a, int C, hipStream_t s)
     else { if (aw) launch_lap_t<P, 3, SRC3, true>(a, s); else launch_lap_t<P, 3, SRC3, false>(a, s); }
 }
 
-template <class P, class M, class I, bool MASK8>
+template <class P, class M, class I, bool MASK8, bool WIDE>
 void launch_col_t(FusedCollapseArgs a, int C, hipStream_t s)
 {
     const int waves_x = (a.cw + kOwned - 1) / kOwned + 1;
     if (a.rows <= 0) a.rows = strip_rows_f(waves_x, a.ch);
     dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
-    if (C == 1) hipLaunchKernelGGL((collapse_level_kernel<P, M, 1, I, MASK8>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((collapse_level_kernel<P, M, 3, I, MASK8>), grid, dim3(256), 0, s, a);
+    if (C == 1) hipLaunchKernelGGL((collapse_level_kernel<P, M, 1, I, MASK8, WIDE>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((collapse_level_kernel<P, M, 3, I, MASK8, WIDE>), grid, dim3(256), 0, s, a);
+}
+
+// Output rows aligned for whole-pair stores and 8-bit planes (mask, alphas)
+// 2-byte aligned.
+template <class I> bool collapse_wide(const FusedCollapseArgs& a, bool mask8)
+{
+    const auto even = [](const void* p, long pitch) { return ((uintptr_t)p & 1) == 0 && (pitch & 1) == 0; };
+    if (mask8 && !even(a.mask, a.mpitch)) return false;
+    if constexpr (!std::is_same<I, void>::value) {
+        if (!even(a.walpha, a.wapitch) || !even(a.balpha, a.bapitch)) return false;
+        if (sizeof(I) == 2 && (((uintptr_t)a.img & 3) != 0 || (a.ipitch & 1) != 0)) return false;
+        if (sizeof(I) == 1 && !even(a.img, a.ipitch)) return false;
+    }
+    return true;
+}
+
+template <class P, class M, class I, bool MASK8>
+void launch_col_w(const FusedCollapseArgs& a0, int C, hipStream_t s)
+{
+    FusedCollapseArgs a = a0;
+    // white_value = 255 * 2^f: blend weights by the exact quotient by 255
+    const double f = a.white_value / 255.0;
+    a.wc_scale = (f >= 1.0 && f == ldexp(1.0, ilogb(f))) ? 1.0 / f : 0.0;
+    if (collapse_wide<I>(a, MASK8)) launch_col_t<P, M, I, MASK8, true>(a, C, s);
+    else launch_col_t<P, M, I, MASK8, false>(a, C, s);
 }
 
 }  // namespace
@@ -883,15 +1044,15 @@ int launch_collapse_level(int ptype, int mtype, int out_itype, bool mask8, const
     if (C != 1 && C != 3) return ENBL_E_UNSUPPORTED;
 #define ENBL_COL(P, M)                                                                                      \
     do {                                                                                                    \
-        if (out_itype == 0) { if (mask8) launch_col_t<P, M, void, true>(a, C, s); else launch_col_t<P, M, void, false>(a, C, s); return 0; } \
-        if (out_itype == ENBL_U8 && std::is_integral<P>::value) { launch_col_t<P, M, uint8_t, true>(a, C, s); return 0; }       \
-        if (out_itype == ENBL_U16 && std::is_integral<P>::value) { launch_col_t<P, M, uint16_t, true>(a, C, s); return 0; }     \
+        if (out_itype == 0) { if (mask8) launch_col_w<P, M, void, true>(a, C, s); else launch_col_w<P, M, void, false>(a, C, s); return 0; } \
+        if (out_itype == ENBL_U8 && std::is_integral<P>::value) { launch_col_w<P, M, uint8_t, true>(a, C, s); return 0; }       \
+        if (out_itype == ENBL_U16 && std::is_integral<P>::value) { launch_col_w<P, M, uint16_t, true>(a, C, s); return 0; }     \
     } while (0)
     if (ptype == ENBL_I16 && mtype == ENBL_I16) ENBL_COL(int16_t, int16_t);
     if (ptype == ENBL_I32 && mtype == ENBL_I32) ENBL_COL(int32_t, int32_t);
     if (ptype == ENBL_F64 && mtype == ENBL_I32) {
-        if (out_itype == 0) { if (mask8) launch_col_t<double, int32_t, void, true>(a, C, s); else launch_col_t<double, int32_t, void, false>(a, C, s); return 0; }
-        if (out_itype == ENBL_F32) { launch_col_t<double, int32_t, float, true>(a, C, s); return 0; }
+        if (out_itype == 0) { if (mask8) launch_col_w<double, int32_t, void, true>(a, C, s); else launch_col_w<double, int32_t, void, false>(a, C, s); return 0; }
+        if (out_itype == ENBL_F32) { launch_col_w<double, int32_t, float, true>(a, C, s); return 0; }
     }
 #undef ENBL_COL
     return ENBL_E_UNSUPPORTED;
