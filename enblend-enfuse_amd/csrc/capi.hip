@@ -13,6 +13,7 @@
 #include <string>
 
 #include "enbl_fused.h"
+#include "enbl_tail.h"
 #include "enbl_internal.h"
 
 using namespace enbl;
@@ -223,22 +224,23 @@ int alpha_pingpong(enbl_context* ctx, const enbl_pyramid* p, uint8_t** a, uint8_
     return alpha_scratch(ctx, (size_t)*pitch * p->lv[1].h, a, b);
 }
 
-// laplacianPyramid (src/pyramid.h:1646-1747) with the fused
-// reduce + expand-subtract pass on every level whose coarse level is large,
-// and the separate reduce / expand kernels on the small tail.
-int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
-                          const uint8_t* alpha, long apitch, enbl_pyramid* p)
+// Number of leading levels built by the fused reduce + expand-subtract
+// pass: those whose coarse level is a streaming level (0 with wraparound or
+// without alpha).
+int fused_level_count(const enbl_pyramid* p, int wrap, const uint8_t* alpha)
 {
-    const int L = p->levels;
     int s = 0;
     if (!wrap && alpha)
-        while (s + 1 < L && streaming_level(p->lv[s + 1].w, p->lv[s + 1].h)) ++s;
-    if (s == 0) {
-        int rc = build_level0(ctx, p, itype, frac, img, ipitch);
-        if (!rc) rc = gaussian_levels(ctx, p, wrap, alpha, apitch);
-        if (!rc) rc = laplacian_levels(ctx, p, wrap);
-        return rc;
-    }
+        while (s + 1 < p->levels && streaming_level(p->lv[s + 1].w, p->lv[s + 1].h)) ++s;
+    return s;
+}
+
+// Fused passes over levels 0..s-1 (s >= 1): Laplacian levels 0..s-1 and
+// Gaussian level s land in the pyramid; the alpha of level l+1 goes to
+// abuf[l & 1] (pitch apl), so level s's alpha ends in abuf[(s - 1) & 1].
+int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* img, long ipitch,
+                           const uint8_t* alpha, long apitch, enbl_pyramid* p, int s, uint8_t* const abuf[2], long apl)
+{
     const size_t es = (size_t)ptype_size(p->ptype);
     void* gbuf[2] = {nullptr, nullptr};
     if (s >= 2) {
@@ -249,10 +251,6 @@ int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, cons
         gbuf[0] = g;
         gbuf[1] = (char*)g + gbytes;
     }
-    uint8_t* abuf[2];
-    long apl;
-    int rc = alpha_pingpong(ctx, p, &abuf[0], &abuf[1], &apl);
-    if (rc) return rc;
     for (int l = 0; l < s; ++l) {
         const Level& F = p->lv[l];
         const Level& Cl = p->lv[l + 1];
@@ -275,11 +273,33 @@ int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, cons
         Timed t(ctx, name, (esC + 1.0) * ((double)F.w * F.h + (double)Cl.w * Cl.h) +
                                esC * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h) +
                                (l == 0 ? (double)F.w * F.h * p->channels * (ptype_size(itype) + (double)es) : 0.0));
-        rc = launch_laplacian_level(p->ptype, l == 0 ? itype : 0, a, p->channels, ctx->stream);
+        int rc = launch_laplacian_level(p->ptype, l == 0 ? itype : 0, a, p->channels, ctx->stream);
         if (rc) return fail(rc, "laplacian level: unsupported type");
         rc = launch_status(name);
         if (rc) return rc;
     }
+    return ENBL_OK;
+}
+
+// laplacianPyramid (src/pyramid.h:1646-1747) with the fused
+// reduce + expand-subtract pass on every level whose coarse level is large,
+// and the separate reduce / expand kernels on the small tail.
+int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
+                          const uint8_t* alpha, long apitch, enbl_pyramid* p)
+{
+    const int L = p->levels;
+    const int s = fused_level_count(p, wrap, alpha);
+    if (s == 0) {
+        int rc = build_level0(ctx, p, itype, frac, img, ipitch);
+        if (!rc) rc = gaussian_levels(ctx, p, wrap, alpha, apitch);
+        if (!rc) rc = laplacian_levels(ctx, p, wrap);
+        return rc;
+    }
+    uint8_t* abuf[2];
+    long apl;
+    int rc = alpha_pingpong(ctx, p, &abuf[0], &abuf[1], &apl);
+    if (!rc) rc = fused_laplacian_levels(ctx, itype, frac, img, ipitch, alpha, apitch, p, s, abuf, apl);
+    if (rc) return rc;
     // Small tail: Gaussian levels s+1.. from level s in place, then
     // expand-subtract levels s..L-2.
     const uint8_t* sa = abuf[(s - 1) & 1];
@@ -297,6 +317,146 @@ int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, cons
         const Level& Cl = p->lv[l + 1];
         rc = do_expand(ctx, p->ptype, p->channels, 0, wrap, level_ptr(p, l + 1), Cl.w, Cl.h, Cl.pitch, Cl.plane,
                        level_ptr(p, l), F.w, F.h, F.pitch, F.plane);
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+// One multi-job launch of small-level work (tail.hip).
+struct TailBuilder {
+    TailLaunch t;
+    double bytes = 0.0;
+    TailBuilder() { memset(&t, 0, sizeof t); }
+    TailJob& add(int kind, int w, int h) {
+        TailJob& J = t.job[t.njobs++];
+        J.kind = kind;
+        J.tiles_x = (w + 63) / 64;
+        J.block0 = t.blocks;
+        t.blocks += J.tiles_x * ((h + 3) / 4);
+        return J;
+    }
+};
+
+ReduceArgs tail_reduce(const enbl_pyramid* p, int l, const uint8_t* sa, long sap, uint8_t* da, long dap)
+{
+    ReduceArgs a;
+    memset(&a, 0, sizeof a);
+    const Level& S_ = p->lv[l];
+    const Level& D = p->lv[l + 1];
+    a.src = level_ptr(p, l); a.spitch = S_.pitch; a.splane = S_.plane; a.sw = S_.w; a.sh = S_.h;
+    a.salpha = sa; a.sapitch = sap;
+    a.dst = level_ptr(p, l + 1); a.dpitch = D.pitch; a.dplane = D.plane; a.dw = D.w; a.dh = D.h;
+    a.dalpha = da; a.dapitch = dap;
+    a.channels = p->channels;
+    return a;
+}
+
+BlendArgs tail_blend(const enbl_pyramid* m, const enbl_pyramid* wlp, enbl_pyramid* blp, int l, double wv)
+{
+    BlendArgs b;
+    memset(&b, 0, sizeof b);
+    b.mask = level_ptr(m, l); b.mpitch = m->lv[l].pitch;
+    b.white = level_ptr(wlp, l); b.wpitch = wlp->lv[l].pitch; b.wplane = wlp->lv[l].plane;
+    b.black = level_ptr(blp, l); b.bpitch = blp->lv[l].pitch; b.bplane = blp->lv[l].plane;
+    b.w = blp->lv[l].w; b.h = blp->lv[l].h;
+    b.white_value = wv;
+    return b;
+}
+
+int run_tail(enbl_context* ctx, int pt, int mt, int C, TailBuilder& b)
+{
+    char name[64];
+    snprintf(name, sizeof name, "tail_%s_c%d", pname(pt), C);
+    Timed t(ctx, name, b.bytes);
+    int rc = launch_tail(pt, mt, C, b.t, ctx->stream);
+    if (rc) return fail(rc, "small-level launch: unsupported types");
+    return launch_status(name);
+}
+
+// The enblend step below the fused levels (no wraparound): the white and
+// black Laplacian tails and the mask Gaussian tail from level s (Gaussian
+// levels s in place, alphas of level s given), then the blend of every
+// level >= s and the collapse down to level s.  Launch t reduces level s+t
+// of all three pyramids and expand-subtracts level s+t-1 of the two image
+// pyramids: no job of a launch writes what another reads.
+int enblend_tail(enbl_context* ctx, const enbl_pyramid* mgp, enbl_pyramid* wlp, enbl_pyramid* blp, int s,
+                 const uint8_t* wa_s, const uint8_t* ba_s, long apl, double wv)
+{
+    const int L = blp->levels;
+    const int pt = blp->ptype, mt = mgp->ptype, C = blp->channels;
+    const double esP = (double)ptype_size(pt) * C, esM = ptype_size(mt);
+    // alpha of the tail levels s+1..L-1, white then black
+    std::vector<size_t> aoff(L + 1, 0);
+    size_t abytes = 0;
+    for (int l = s + 1; l < L; ++l) {
+        aoff[l] = abytes;
+        abytes += round_up((size_t)round_up((size_t)blp->lv[l].w, 64) * blp->lv[l].h, 256);
+    }
+    uint8_t* tail_alpha = nullptr;
+    if (abytes) {
+        void* p;
+        int rc = scratch(ctx, 15, 2 * abytes, &p);
+        if (rc) return rc;
+        tail_alpha = (uint8_t*)p;
+    }
+    auto apitch = [&](int l) { return l == s ? apl : (long)round_up((size_t)blp->lv[l].w, 64); };
+    auto walpha = [&](int l) -> uint8_t* { return l == s ? const_cast<uint8_t*>(wa_s) : tail_alpha + aoff[l]; };
+    auto balpha = [&](int l) -> uint8_t* { return l == s ? const_cast<uint8_t*>(ba_s) : tail_alpha + abytes + aoff[l]; };
+    auto expand_sub = [&](TailBuilder& b, enbl_pyramid* p, int l) {
+        const Level& F = p->lv[l];
+        const Level& Cl = p->lv[l + 1];
+        ExpandArgs e;
+        memset(&e, 0, sizeof e);
+        e.src = level_ptr(p, l + 1); e.spitch = Cl.pitch; e.splane = Cl.plane; e.sw = Cl.w; e.sh = Cl.h;
+        e.dst = level_ptr(p, l); e.dpitch = F.pitch; e.dplane = F.plane; e.dw = F.w; e.dh = F.h;
+        e.channels = C;
+        b.add(kTailExpandSub, Cl.w, Cl.h).u.e = e;
+        b.bytes += esP * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h);
+    };
+    for (int l = s; l + 1 < L; ++l) {
+        const Level& S_ = blp->lv[l];
+        const Level& D = blp->lv[l + 1];
+        if (S_.w < 2 || S_.h < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
+        TailBuilder b;
+        b.add(kTailReduceImage, D.w, D.h).u.r = tail_reduce(wlp, l, walpha(l), apitch(l), walpha(l + 1), apitch(l + 1));
+        b.add(kTailReduceImage, D.w, D.h).u.r = tail_reduce(blp, l, balpha(l), apitch(l), balpha(l + 1), apitch(l + 1));
+        b.add(kTailReduceMask, D.w, D.h).u.r = tail_reduce(mgp, l, nullptr, 0, nullptr, 0);
+        b.bytes += 2.0 * (esP + 1.0) * ((double)S_.w * S_.h + (double)D.w * D.h) + esM * ((double)S_.w * S_.h + (double)D.w * D.h);
+        if (l > s) {
+            expand_sub(b, wlp, l - 1);
+            expand_sub(b, blp, l - 1);
+        }
+        int rc = run_tail(ctx, pt, mt, C, b);
+        if (rc) return rc;
+    }
+    if (L - 2 >= s) {
+        TailBuilder b;
+        expand_sub(b, wlp, L - 2);
+        expand_sub(b, blp, L - 2);
+        int rc = run_tail(ctx, pt, mt, C, b);
+        if (rc) return rc;
+    }
+    // Blend of the coarsest level (src/enblend.h:472-474), then blend +
+    // collapse (src/pyramid.h:1757-1784) of levels L-2..s.
+    {
+        TailBuilder b;
+        const Level& T = blp->lv[L - 1];
+        b.add(kTailBlend, T.w, T.h).u.b = tail_blend(mgp, wlp, blp, L - 1, wv);
+        b.bytes += (double)T.w * T.h * (esM + 3.0 * esP);
+        int rc = run_tail(ctx, pt, mt, C, b);
+        if (rc) return rc;
+    }
+    for (int l = L - 2; l >= s; --l) {
+        TailBuilder b;
+        const Level& F = blp->lv[l];
+        const Level& Cl = blp->lv[l + 1];
+        CollapseSmallArgs c;
+        memset(&c, 0, sizeof c);
+        c.coarse = level_ptr(blp, l + 1); c.cpitch = Cl.pitch; c.cplane = Cl.plane; c.cw = Cl.w; c.ch = Cl.h;
+        c.fine = tail_blend(mgp, wlp, blp, l, wv);
+        b.add(kTailCollapse, Cl.w, Cl.h).u.c = c;
+        b.bytes += (double)F.w * F.h * (esM + 3.0 * esP) + esP * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h);
+        int rc = run_tail(ctx, pt, mt, C, b);
         if (rc) return rc;
     }
     return ENBL_OK;
@@ -658,16 +818,17 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     if (!rc) rc = cached_pyramid(ctx, pt, C, w, h, levels, 1, &blp);
     if (rc) return rc;
 
-    // Mask Gaussian pyramid without alpha (src/enblend.h:384-388).
-    rc = enbl_gaussian_pyramid(ctx, wrap, ENBL_U8, mfrac, mask, mpitch, nullptr, 0, mgp);
-    if (rc) return rc;
-    // Laplacian pyramids of white and black (src/enblend.h:419-446).
-    rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, white, wpitch, walpha, wapitch, wlp);
-    if (rc) return rc;
-    rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, black, bpitch, balpha, bapitch, blp);
-    if (rc) return rc;
-    const bool fused0 = !wrap && streaming_level(wlp->lv[1].w, wlp->lv[1].h) && levels > 1;
-    if (!fused0) {
+    const double wv = (double)(255 << mfrac);
+    const int s = fused_level_count(wlp, wrap, walpha);
+    if (s == 0) {
+        // Mask Gaussian pyramid without alpha (src/enblend.h:384-388).
+        rc = enbl_gaussian_pyramid(ctx, wrap, ENBL_U8, mfrac, mask, mpitch, nullptr, 0, mgp);
+        if (rc) return rc;
+        // Laplacian pyramids of white and black (src/enblend.h:419-446).
+        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, white, wpitch, walpha, wapitch, wlp);
+        if (rc) return rc;
+        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, black, bpitch, balpha, bapitch, blp);
+        if (rc) return rc;
         // Union alpha (src/enblend.h:461-463).
         {
             Timed t(ctx, "alpha_union", 3.0 * w * h);
@@ -676,36 +837,45 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
             if (rc) return rc;
         }
         // blend with white = 255 << mask fraction bits (src/enblend.h:472-474).
-        rc = enbl_blend(ctx, mgp, wlp, blp, (double)(255 << mfrac));
+        rc = enbl_blend(ctx, mgp, wlp, blp, wv);
         if (rc) return rc;
         rc = enbl_collapse_pyramid(ctx, wrap, blp);
         if (rc) return rc;
         return enbl_copy_from_pyramid_if(ctx, blp, 0, ifrac, balpha, bapitch, black, itype, bpitch, dither, seed);
     }
-    // Blend each level right before it is collapsed; the large levels blend,
-    // add the expanded coarser level and (level 0) convert in one pass.
-    const double wv = (double)(255 << mfrac);
+    // Mask Gaussian pyramid without alpha (src/enblend.h:384-388): level 0
+    // and the large levels up to s here, the rest with the image tails.
+    rc = build_level0(ctx, mgp, ENBL_U8, mfrac, mask, mpitch);
+    for (int l = 1; !rc && l <= s; ++l) {
+        const Level& S_ = mgp->lv[l - 1];
+        const Level& D = mgp->lv[l];
+        rc = do_reduce(ctx, mt, 1, 0, level_ptr(mgp, l - 1), S_.w, S_.h, S_.pitch, S_.plane, nullptr, 0,
+                       level_ptr(mgp, l), D.w, D.h, D.pitch, D.plane, nullptr, 0);
+    }
+    if (rc) return rc;
+    // Laplacian pyramids of white and black (src/enblend.h:419-446): the
+    // fused levels, each pyramid keeping its own alpha of level s for the
+    // shared small-level launches.
+    {
+        uint8_t* wab[2];
+        uint8_t* bab[2];
+        long apl;
+        void* bp;
+        rc = alpha_pingpong(ctx, wlp, &wab[0], &wab[1], &apl);
+        if (!rc) rc = scratch(ctx, 14, 2 * (size_t)apl * wlp->lv[1].h, &bp);
+        if (rc) return rc;
+        bab[0] = (uint8_t*)bp;
+        bab[1] = (uint8_t*)bp + (size_t)apl * wlp->lv[1].h;
+        rc = fused_laplacian_levels(ctx, itype, ifrac, white, wpitch, walpha, wapitch, wlp, s, wab, apl);
+        if (!rc) rc = fused_laplacian_levels(ctx, itype, ifrac, black, bpitch, balpha, bapitch, blp, s, bab, apl);
+        if (!rc) rc = enblend_tail(ctx, mgp, wlp, blp, s, wab[(s - 1) & 1], bab[(s - 1) & 1], apl, wv);
+        if (rc) return rc;
+    }
+    // Levels s-1..0: blend, add the expanded coarser level and (level 0)
+    // convert in one pass.
     const int esP = ptype_size(pt), esM = ptype_size(mt);
-    for (int l = levels - 1; l >= 0; --l) {
+    for (int l = s - 1; l >= 0; --l) {
         const Level& L = blp->lv[l];
-        const bool fused = l + 1 < levels && streaming_level(blp->lv[l + 1].w, blp->lv[l + 1].h);
-        if (!fused) {
-            {
-                Timed t(ctx, "blend", (double)L.w * L.h * (esM + 3.0 * C * esP));
-                rc = launch_blend(mt, pt, C, L.w, L.h, level_ptr(mgp, l), mgp->lv[l].pitch, level_ptr(wlp, l), wlp->lv[l].pitch,
-                                  wlp->lv[l].plane, level_ptr(blp, l), L.pitch, L.plane, wv, ctx->stream);
-                if (rc) return fail(rc, "blend: unsupported types");
-                rc = launch_status("blend");
-                if (rc) return rc;
-            }
-            if (l + 1 < levels) {
-                const Level& Cl = blp->lv[l + 1];
-                rc = do_expand(ctx, pt, C, 1, wrap, level_ptr(blp, l + 1), Cl.w, Cl.h, Cl.pitch, Cl.plane,
-                               level_ptr(blp, l), L.w, L.h, L.pitch, L.plane);
-                if (rc) return rc;
-            }
-            continue;
-        }
         const Level& Cl = blp->lv[l + 1];
         FusedCollapseArgs a;
         a.coarse = level_ptr(blp, l + 1); a.cpitch = Cl.pitch; a.cplane = Cl.plane; a.cw = Cl.w; a.ch = Cl.h;

@@ -20,6 +20,7 @@
 #include "enbl_internal.h"
 #include "pyr_common.h"
 #include "reduce_stream.h"
+#include "small_levels.h"
 
 namespace enbl {
 namespace {
@@ -226,213 +227,22 @@ __global__ __launch_bounds__(256) void expand_kernel(ExpandArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Small levels: one thread per output, direct loads.
+// Small levels: one thread per output, direct loads (small_levels.h).
 //
 // A coarse level has too few rows for the streaming kernels to hide the
 // latency of their row-serial loop; here every thread fetches its whole
-// window at once.  Same boundary forms and operation order as above.
+// window at once.
 // ---------------------------------------------------------------------------
 template <class P, int C, bool ALPHA, bool WRAP>
 __global__ __launch_bounds__(256) void reduce_small_kernel(ReduceArgs a)
 {
-    typedef Pix<P> Px;
-    typedef typename Px::A A;
-    typedef typename Px::S S;
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (j >= a.dw || i >= a.dh) return;
-    const int sw = a.sw, sh = a.sh;
-    const int c = 2 * j;
-    const P* src = static_cast<const P*>(a.src);
-    const int rcase = WRAP ? 0 : (c == sw - 1 ? 1 : (c == sw - 2 ? 2 : 0));
-    int col[5];
-    bool in[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int x = c - 2 + k;
-        in[k] = WRAP || (x >= 0 && x < sw);
-        col[k] = WRAP ? wrap_index(x, sw) : (x < 0 ? 0 : (x >= sw ? sw - 1 : x));
-    }
-    // Issue every load of the 5x5 window first (clamped indices are always
-    // inside the level), then evaluate the literal boundary forms.
-    const int r = 2 * i;
-    int rowi[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int y = r - 2 + k;
-        rowi[k] = y < 0 ? 0 : (y >= sh ? sh - 1 : y);
-    }
-    uint8_t al[5][5];
-    S raw[5][5][C];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-#pragma unroll
-        for (int t = 0; t < 5; ++t) {
-            al[k][t] = ALPHA ? a.salpha[(long)rowi[k] * a.sapitch + col[t]] : (uint8_t)1;
-#pragma unroll
-            for (int ch = 0; ch < C; ++ch) raw[k][t][ch] = Px::load(src[ch * a.splane + (long)rowi[k] * a.spitch + col[t]]);
-        }
-    }
-    S hrow[5][C];
-    int mrow[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int y = r - 2 + k;
-        mrow[k] = 0;
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) hrow[k][ch] = A::zero();
-        if (y < 0 || y >= sh) continue;
-        int m[5];
-#pragma unroll
-        for (int t = 0; t < 5; ++t) m[t] = ALPHA ? ((in[t] && al[k][t]) ? 1 : 0) : 1;
-        if (ALPHA) {
-            int hm = m[0] + 4 * m[1] + 6 * m[2];
-            if (rcase != 1) hm += 4 * m[3];
-            if (rcase == 0) hm += m[4];
-            mrow[k] = hm;
-        }
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            S v[5];
-#pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                if (ALPHA) v[t] = m[t] ? raw[k][t][ch] : A::zero();
-                else v[t] = (WRAP || in[t] || t < 2) ? raw[k][t][ch] : A::zero();
-            }
-            const S a0 = A::add(v[0], A::mul(v[1], 4));
-            S hv;
-            if (rcase == 1) {
-                hv = ALPHA ? A::add(a0, A::mul(v[2], 6)) : A::add(a0, A::mul(v[2], 11));
-            } else if (rcase == 2) {
-                const S d4 = A::mul(v[3], 4);
-                hv = ALPHA ? A::add(A::add(a0, A::mul(v[2], 6)), d4)
-                           : A::add(A::add(A::add(a0, A::mul(v[2], 6)), d4), A::div(d4, 4));
-            } else {
-                hv = A::add(A::add(A::add(a0, A::mul(v[2], 6)), A::mul(v[3], 4)), v[4]);
-            }
-            hrow[k][ch] = hv;
-        }
-    }
-    int ap = 0;
-    if (ALPHA) {
-        const int q0 = r == 0 ? 0 : mrow[0] + 4 * mrow[1];
-        if (r == sh - 1) ap = q0 + 6 * mrow[2];
-        else if (r == sh - 2) ap = q0 + 6 * mrow[2] + 4 * mrow[3];
-        else ap = q0 + 6 * mrow[2] + 4 * mrow[3] + mrow[4];
-    }
-    P* dst = static_cast<P*>(a.dst);
-#pragma unroll
-    for (int ch = 0; ch < C; ++ch) {
-        S v;
-        if (ALPHA) {
-            const S a0 = r == 0 ? A::zero() : A::add(hrow[0][ch], A::mul(hrow[1][ch], 4));
-            if (r == sh - 1) v = A::add(a0, A::mul(hrow[2][ch], 6));
-            else if (r == sh - 2) v = A::add(A::add(a0, A::mul(hrow[2][ch], 6)), A::mul(hrow[3][ch], 4));
-            else v = A::add(A::add(A::add(a0, A::mul(hrow[2][ch], 6)), A::mul(hrow[3][ch], 4)), hrow[4][ch]);
-            dst[ch * a.dplane + (long)i * a.dpitch + j] = ap ? Px::wrap(A::div(v, ap)) : (P)0;
-        } else {
-            const S a0 = r == 0 ? A::mul(hrow[2][ch], 5) : A::add(hrow[0][ch], A::mul(hrow[1][ch], 4));
-            if (r == sh - 1) {
-                v = A::add(a0, A::mul(hrow[2][ch], 11));
-            } else if (r == sh - 2) {
-                const S d4 = A::mul(hrow[3][ch], 4);
-                v = A::add(A::add(A::add(a0, A::mul(hrow[2][ch], 6)), d4), A::div(d4, 4));
-            } else {
-                v = A::add(A::add(A::add(a0, A::mul(hrow[2][ch], 6)), A::mul(hrow[3][ch], 4)), hrow[4][ch]);
-            }
-            dst[ch * a.dplane + (long)i * a.dpitch + j] = Px::wrap(A::div(v, 256));
-        }
-    }
-    if (ALPHA) a.dalpha[(long)i * a.dapitch + j] = ap ? 255 : 0;
+    reduce_small_body<P, C, ALPHA, WRAP>(a, blockIdx.x, blockIdx.y);
 }
 
 template <class P, int C, bool ADD, bool WRAP>
 __global__ __launch_bounds__(256) void expand_small_kernel(ExpandArgs a)
 {
-    typedef Pix<P> Px;
-    typedef typename Px::A A;
-    typedef typename Px::S S;
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int sw = a.sw, sh = a.sh, dw = a.dw, dh = a.dh;
-    if (j >= sw || i >= sh) return;
-    const bool dst_even = (dw & 1) == 0;
-    const P* src = static_cast<const P*>(a.src);
-    S hA[3][C], hB[3][C];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int y = i - 1 + k;
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) { hA[k][ch] = A::zero(); hB[k][ch] = A::zero(); }
-        if (y < 0 || y >= sh) continue;
-#pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            const P* row = src + ch * a.splane + (long)y * a.spitch;
-            const S x = Px::load(row[j]);
-            if (sw == 1) {
-                S r1 = A::zero();
-                if (WRAP) r1 = dst_even ? x : A::mul(x, 4);
-                if (WRAP && y == 0) {
-                    hA[k][ch] = dst_even ? A::add(A::add(r1, A::mul(x, 6)), x) : A::add(A::add(r1, A::mul(x, 6)), A::mul(x, 4));
-                    hB[k][ch] = A::mul(A::add(x, x), 4);
-                } else {
-                    hA[k][ch] = A::add(r1, A::mul(x, 6));
-                    hB[k][ch] = A::mul(x, 4);
-                }
-            } else if (j == 0) {
-                S r1 = A::zero();
-                if (WRAP) r1 = dst_even ? Px::load(row[sw - 1]) : A::mul(Px::load(row[sw - 1]), 4);
-                const S xr = Px::load(row[1]);
-                hA[k][ch] = A::add(A::add(r1, A::mul(x, 6)), xr);
-                hB[k][ch] = A::mul(A::add(x, xr), 4);
-            } else if (j == sw - 1) {
-                const S xl = Px::load(row[j - 1]);
-                if (WRAP) {
-                    const S xf = Px::load(row[0]);
-                    hA[k][ch] = dst_even ? A::add(A::add(xl, A::mul(x, 6)), xf) : A::add(A::add(xl, A::mul(x, 6)), A::mul(xf, 4));
-                    hB[k][ch] = A::mul(A::add(x, xf), 4);
-                } else {
-                    hA[k][ch] = A::add(xl, A::mul(x, 6));
-                    hB[k][ch] = A::mul(x, 4);
-                }
-            } else {
-                const S xl = Px::load(row[j - 1]), xr = Px::load(row[j + 1]);
-                hA[k][ch] = A::add(A::add(xl, A::mul(x, 6)), xr);
-                hB[k][ch] = A::mul(A::add(x, xr), 4);
-            }
-        }
-    }
-    const bool more = i + 1 < sh;
-    const int X = 2 * j, Y = 2 * i;
-    const bool xb = X + 1 < dw, yb = Y + 1 < dh;
-    const int hdA = expand_hsum(X, sw, WRAP, dst_even), hdB = expand_hsum(X + 1, sw, WRAP, dst_even);
-    const int v0 = expand_vsum(Y, sh), v1 = expand_vsum(Y + 1, sh);
-    P* dst = static_cast<P*>(a.dst);
-#pragma unroll
-    for (int ch = 0; ch < C; ++ch) {
-        S o00, o10, o01, o11;
-        if (more) {
-            o00 = A::add(A::add(hA[0][ch], A::mul(hA[1][ch], 6)), hA[2][ch]);
-            o10 = A::add(A::add(hB[0][ch], A::mul(hB[1][ch], 6)), hB[2][ch]);
-            o01 = A::add(hA[1][ch], hA[2][ch]);
-            o11 = A::add(hB[1][ch], hB[2][ch]);
-        } else {
-            o00 = A::add(hA[0][ch], A::mul(hA[1][ch], 6));
-            o10 = A::add(hB[0][ch], A::mul(hB[1][ch], 6));
-            o01 = hA[1][ch];
-            o11 = hB[1][ch];
-        }
-        P* row0 = dst + ch * a.dplane + (long)Y * a.dpitch + X;
-        const S e[4] = {A::div(o00, hdA * v0), A::div(o10, hdB * v0), A::div(o01, hdA * v1), A::div(o11, hdB * v1)};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int dx = q & 1, dy = q >> 1;
-            if ((dx && !xb) || (dy && !yb)) continue;
-            P* t = row0 + dy * a.dpitch + dx;
-            const S d = Px::load(*t);
-            *t = ADD ? Px::sat(A::add(d, e[q])) : Px::wrap(A::sub(d, e[q]));
-        }
-    }
+    expand_small_body<P, C, ADD, WRAP>(a, blockIdx.x, blockIdx.y);
 }
 
 // Levels of at most this many (reduce: destination, expand: source) pixels

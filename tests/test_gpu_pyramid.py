@@ -252,6 +252,33 @@ def _enblend_inputs(rng, itype, h, w):
 
 
 @pytest.mark.parametrize("itype", [U8, U16, F32])
+def test_enblend_pair_small_level_launches(enbl, gpu_ctx, oracle, itype):
+    """Automatic kernel choice on a size whose level 1 takes the fused pass
+    (over 2^20 pixels) and whose smaller levels run in the shared multi-job
+    launches (white, black and mask reduces with expand-subtracts, blend,
+    blend + collapse); odd sizes, compared with the oracle."""
+    rng = np.random.default_rng(640 + itype)
+    h, w = 1501, 2899
+    L = oracle.auto_levels(w, h)
+    print(f"enblend_pair auto itype={itype} {w}x{h} L={L}", flush=True)
+    white, wa, black, ba, mask = _enblend_inputs(rng, itype, h, w)
+    want, want_a = oracle.enblend_pair(white, wa, black, ba, mask, L, False, itype, 1, 77)
+    dwh, dwa, dbk, dba, dm = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (white, wa, black, ba, mask))
+    torch.cuda.synchronize()
+    enbl.check(enbl.lib().enbl_enblend_pair(gpu_ctx.h, itype, 3, w, h, L, 0,
+                                            _vp(dwh), 3 * w, _vp(dwa), w, _vp(dbk), 3 * w, _vp(dba), w,
+                                            _vp(dm), w, 1, 77))
+    gpu_ctx.sync()
+    got, got_a = dbk.cpu().numpy(), dba.cpu().numpy()
+    np.testing.assert_array_equal(got_a, want_a)
+    if itype == F32:
+        sel = want_a != 0
+        np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
+    else:
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
 @pytest.mark.parametrize("wrap", [False, True])
 def test_enblend_pair_matches_oracle(enbl, gpu_ctx, oracle, itype, wrap, policy):
     rng = np.random.default_rng(600 + itype + wrap)
