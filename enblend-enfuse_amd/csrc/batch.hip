@@ -230,9 +230,10 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const BReduceJob* __r
         const ReduceArgs a = J.a;
         const int wx = (local % J.groups_x) * 4 + wv;
         const int i0 = (local / J.groups_x) * a.rows;
-        if (J.kind == kReduceImage3) reduce_strip<P, 3, true, false, ImageRows<I, P>>(a, wx, i0);
-        else if (J.kind == kReducePlane3) reduce_strip<P, 3, true, false, PlaneRows<P>>(a, wx, i0);
-        else reduce_strip<P, 1, true, false, PlaneRows<P>>(a, wx, i0);
+        if (J.kind == kReduceImage3W) reduce_strip_ring<P, 3, true, IntImageSrc<I, P, 3, true>, true>(a, wx, i0);
+        else if (J.kind == kReduceImage3) reduce_strip_ring<P, 3, true, IntImageSrc<I, P, 3, false>, false>(a, wx, i0);
+        else if (J.kind == kReducePlane3) reduce_strip_ring<P, 3, true, PlaneSrc<P, 3>, true>(a, wx, i0);
+        else reduce_strip_ring<P, 1, true, PlaneSrc<P, 1>, true>(a, wx, i0);
     }
 }
 

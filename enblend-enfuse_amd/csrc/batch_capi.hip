@@ -343,8 +343,11 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             for (int m = 0; m < S.K; ++m) {
                 uint8_t* da = keep_alpha ? (uint8_t*)at(Y.ial_at(S.K, m) + DA.offset) : nullptr;
                 if (l == 0) {
-                    add_job(kReduceImage3, S.imgs[m], S.ipitches[m], 0, sw, sh, S.alphas[m], S.apitches[m],
-                            at(Y.img_at(m) + D.offset), D, da, DA.pitch);
+                    const size_t isz_b = itype == ENBL_U8 ? 1 : 2;
+                    const bool wide = isz_b == 2 && ((uintptr_t)S.imgs[m] & 3) == 0 && (S.ipitches[m] & 1) == 0 &&
+                                      ((uintptr_t)S.alphas[m] & 1) == 0 && (S.apitches[m] & 1) == 0;
+                    add_job(wide ? kReduceImage3W : kReduceImage3, S.imgs[m], S.ipitches[m], 0, sw, sh, S.alphas[m],
+                            S.apitches[m], at(Y.img_at(m) + D.offset), D, da, DA.pitch);
                 } else {
                     const Level& SL = Y.img[(size_t)l - 1];
                     const Level& SA = Y.ial[(size_t)l - 1];

@@ -843,10 +843,29 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
         if (rc) return rc;
         return enbl_copy_from_pyramid_if(ctx, blp, 0, ifrac, balpha, bapitch, black, itype, bpitch, dither, seed);
     }
-    // Mask Gaussian pyramid without alpha (src/enblend.h:384-388): level 0
-    // and the large levels up to s here, the rest with the image tails.
-    rc = build_level0(ctx, mgp, ENBL_U8, mfrac, mask, mpitch);
-    for (int l = 1; !rc && l <= s; ++l) {
+    // Mask Gaussian pyramid without alpha (src/enblend.h:384-388): the large
+    // levels up to s here, the rest with the image tails.  Level 1 is reduced
+    // straight from the mask bytes, converted on the fly (the fixed-point
+    // level 0 is not stored: the level-0 collapse reads the bytes).
+    {
+        const Level& D = mgp->lv[1];
+        if (w < 2 || h < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
+        ReduceArgs a;
+        memset(&a, 0, sizeof a);
+        a.src = mask; a.spitch = mpitch; a.splane = 0; a.sw = w; a.sh = h;
+        a.dst = level_ptr(mgp, 1); a.dpitch = D.pitch; a.dplane = D.plane; a.dw = D.w; a.dh = D.h;
+        a.channels = 1; a.frac = mfrac;
+        char name[64];
+        snprintf(name, sizeof name, "reduce_mask0_%s", pname(mt));
+        const double esm = ptype_size(mt);
+        // credited bytes: the conversion of level 0 and the reduce it feeds
+        Timed t(ctx, name, (double)w * h * (1.0 + esm) + esm * ((double)w * h + (double)D.w * D.h));
+        rc = launch_reduce_image(ENBL_U8, mt, a, ctx->stream);
+        if (rc) return fail(rc, "mask reduce: unsupported type");
+        rc = launch_status(name);
+        if (rc) return rc;
+    }
+    for (int l = 2; !rc && l <= s; ++l) {
         const Level& S_ = mgp->lv[l - 1];
         const Level& D = mgp->lv[l];
         rc = do_reduce(ctx, mt, 1, 0, level_ptr(mgp, l - 1), S_.w, S_.h, S_.pitch, S_.plane, nullptr, 0,

@@ -44,7 +44,9 @@ struct BWeightJob {
 // Reduce with alpha of one pyramid level (image or mask pyramid), as the
 // streaming strips of reduce_stream.h: a tile is 4 adjacent 62-column wave
 // strips x a.rows output rows.
-enum { kReducePlane3 = 0, kReduceImage3 = 1, kReducePlane1 = 2 };
+// kReduceImage3W: the interleaved image and its alpha rows aligned for
+// whole-word pair loads (4-byte image rows, 2-byte alpha rows).
+enum { kReducePlane3 = 0, kReduceImage3 = 1, kReducePlane1 = 2, kReduceImage3W = 3 };
 struct BReduceJob {
     ReduceArgs a;                   // no wraparound; a.rows = tile rows; a.dalpha may be null
     int groups_x;                   // 4-strip groups across the destination level

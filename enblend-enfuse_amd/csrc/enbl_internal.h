@@ -42,6 +42,9 @@ struct ExpandArgs {
 };
 
 int launch_reduce(int ptype, ReduceArgs a, hipStream_t s);
+// Streaming reduce whose source is an interleaved integer image of itype
+// (level 0 converted on the fly, a.frac); no wraparound.
+int launch_reduce_image(int itype, int ptype, ReduceArgs a, hipStream_t s);
 int launch_expand(int ptype, ExpandArgs a, hipStream_t s);
 
 // Pointwise kernels (pointwise.hip).

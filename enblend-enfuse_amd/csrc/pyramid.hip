@@ -28,11 +28,23 @@ namespace {
 // ---------------------------------------------------------------------------
 // reduce (streaming strips: reduce_stream.h)
 // ---------------------------------------------------------------------------
-template <class P, int C, bool ALPHA, bool WRAP>
+// AW: alpha rows 2-byte aligned (no wraparound only).
+template <class P, int C, bool ALPHA, bool WRAP, bool AW>
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs a)
 {
     const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
-    reduce_strip<P, C, ALPHA, WRAP, PlaneRows<P>>(a, wx, blockIdx.y * a.rows);
+    if constexpr (WRAP) reduce_strip<P, C, ALPHA, true, PlaneRows<P>>(a, wx, blockIdx.y * a.rows);
+    else reduce_strip_ring<P, C, ALPHA, PlaneSrc<P, C>, AW>(a, wx, blockIdx.y * a.rows);
+}
+
+// Level 0 read from the interleaved integer image, (P)v << frac on the fly
+// (ConvertScalarToPyramidFunctor, src/fixmath.h:116-163).  WIDE: image rows
+// aligned for whole-word pair loads; AW: alpha rows 2-byte aligned.
+template <class I, class P, int C, bool ALPHA, bool WIDE, bool AW>
+__global__ __launch_bounds__(256) void reduce_image_kernel(ReduceArgs a)
+{
+    const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    reduce_strip_ring<P, C, ALPHA, IntImageSrc<I, P, C, WIDE>, AW>(a, wx, blockIdx.y * a.rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,8 +285,26 @@ void launch_reduce_t(ReduceArgs a, hipStream_t s)
     if (a.rows <= 0) a.rows = strip_rows(waves_x, a.dh);
     dim3 block(256);
     dim3 grid((waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows);
-    if (a.wrap) hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, false>), grid, block, 0, s, a);
+    const bool aw = !ALPHA || (((uintptr_t)a.salpha & 1) == 0 && (a.sapitch & 1) == 0);
+    if (a.wrap) hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, true, false>), grid, block, 0, s, a);
+    else if (aw) hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((reduce_kernel<P, C, ALPHA, false, false>), grid, block, 0, s, a);
+}
+
+// Streaming reduce of an interleaved integer image (no wraparound).
+template <class I, class P, int C, bool ALPHA>
+void launch_reduce_image_t(ReduceArgs a, hipStream_t s)
+{
+    const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
+    if (a.rows <= 0) a.rows = strip_rows(waves_x, a.dh);
+    dim3 grid((waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows);
+    const bool aw = !ALPHA || (((uintptr_t)a.salpha & 1) == 0 && (a.sapitch & 1) == 0);
+    // pairs are whole words: 4-byte rows for 16-bit samples (and one 8-bit
+    // channel: 2-byte rows); 8-bit RGB always loads per sample
+    const uintptr_t am = sizeof(I) == 2 ? 3 : 1;
+    const bool iw = (sizeof(I) == 2 || C == 1) && ((uintptr_t)a.src & am) == 0 && (a.spitch & 1) == 0;
+    if (iw && aw) hipLaunchKernelGGL((reduce_image_kernel<I, P, C, ALPHA, true, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((reduce_image_kernel<I, P, C, ALPHA, false, false>), grid, dim3(256), 0, s, a);
 }
 
 template <class P, int C, bool ADD>
@@ -320,6 +350,23 @@ int launch_reduce(int ptype, ReduceArgs a, hipStream_t s)
     case ENBL_I32: return reduce_by_c<int32_t>(a, s);
     case ENBL_F64: return reduce_by_c<double>(a, s);
     }
+    return ENBL_E_UNSUPPORTED;
+}
+
+int launch_reduce_image(int itype, int ptype, ReduceArgs a, hipStream_t s)
+{
+    if (a.wrap || (a.channels != 1 && a.channels != 3)) return ENBL_E_UNSUPPORTED;
+    const bool alpha = a.salpha != nullptr;
+#define ENBL_RI(I, P)                                                                                        \
+    do {                                                                                                     \
+        if (a.channels == 1) alpha ? launch_reduce_image_t<I, P, 1, true>(a, s) : launch_reduce_image_t<I, P, 1, false>(a, s); \
+        else alpha ? launch_reduce_image_t<I, P, 3, true>(a, s) : launch_reduce_image_t<I, P, 3, false>(a, s); \
+        return 0;                                                                                            \
+    } while (0)
+    if (itype == ENBL_U8 && ptype == ENBL_I16) ENBL_RI(uint8_t, int16_t);
+    if (itype == ENBL_U8 && ptype == ENBL_I32) ENBL_RI(uint8_t, int32_t);
+    if (itype == ENBL_U16 && ptype == ENBL_I32) ENBL_RI(uint16_t, int32_t);
+#undef ENBL_RI
     return ENBL_E_UNSUPPORTED;
 }
 

@@ -21,6 +21,7 @@
 #include "enbl_device.h"
 #include "enbl_internal.h"
 #include "pyr_common.h"
+#include "row_fetch.h"
 
 namespace enbl {
 
@@ -281,6 +282,196 @@ __device__ __forceinline__ void reduce_strip(const ReduceArgs& a, int wx, int i0
         ra = na;
         rb = nb;
     }
+}
+
+// The same strip without wraparound over raw-row rings: each step loads the
+// two source rows of the next output row through per-row buffer resources
+// (zero-size for rows outside the level or past the strip, out-of-range
+// offsets for lanes outside it) into one half of a 4-slot ring and forms
+// the row sums of the rows loaded by the previous step from the other half;
+// the loop is unrolled by the ring's period of 2 steps, no in-flight
+// register is copied, and every step issues the same memory instructions,
+// so its wait covers the previous step's loads only.  SRCF: a row source of
+// row_fetch.h; AW: alpha rows 2-byte aligned (one 16-bit load per pair).
+template <class P, int C, bool ALPHA, class SRCF, bool AW>
+__device__ __forceinline__ void reduce_strip_ring(const ReduceArgs& a, int wx, int i0)
+{
+    typedef Pix<P> Px;
+    typedef typename Px::A A;
+    typedef typename Px::S S;
+    constexpr bool INT = std::is_same<A, AccI>::value;
+    constexpr uint32_t ES = sizeof(P);
+    struct RawRow {
+        typename SRCF::Raw px;
+        AlphaPair<AW> al;
+    };
+
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wx * kOwned - 1 >= a.dw) return;  // whole wave beyond the level
+    if (i0 >= a.dh) return;
+    const int i1 = min(a.dh, i0 + a.rows);
+    const int sw = a.sw, sh = a.sh;
+    const int j = wx * kOwned + lane - 1;
+    const int c = 2 * j;
+    const bool valid = j >= 0 && j < a.dw;
+    const bool owned = lane >= 1 && lane <= kOwned && valid;
+    const bool both = c + 1 < sw;
+    // boundary case of the row tail (src/pyramid.h:264-299, :410-455,
+    // :688-707): 1 = centre is the last sample, 2 = the second to last
+    const int rcase = c == sw - 1 ? 1 : (c == sw - 2 ? 2 : 0);
+    LaneCols L;
+    L.valid = valid;
+    L.both = both;
+    L.tailw = !wave_all(!valid || both);
+    L.c = c;
+
+    auto fetch = [&](int r, bool want, RawRow& R) {
+        const bool ok = want && r >= 0 && r < sh;
+        const int rr = ok ? r : 0;
+        SRCF::fetch(a, sw, rr, ok, L, R.px);
+        if constexpr (ALPHA) R.al.fetch(a.salpha + (long)rr * a.sapitch, ok, sw, L);
+    };
+    // Horizontal 5-tap sums of a fetched row; the row-tail cases are
+    // selected per lane, in the reference's operation order.
+    auto sums = [&](const RawRow& R, S (&h)[C], int& hm) {
+        S x0[C], x1[C];
+        SRCF::unpack(a, R.px, x0, x1);
+        int m0 = 1, m1 = 1;
+        if constexpr (ALPHA) {
+            m0 = valid && R.al.first() ? 1 : 0;
+            m1 = valid && both && R.al.second() ? 1 : 0;
+            const int ml0 = lane_up(m0), ml1 = lane_up(m1), mr0 = lane_down(m0);
+            const int qa = c >= 2 ? ml0 : 0, qb = c >= 1 ? ml1 : 0;
+            hm = qa + 4 * qb + 6 * m0 + (rcase != 1 ? 4 * m1 : 0) + (rcase == 0 ? mr0 : 0);
+        }
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const S xa = (ALPHA ? m0 : valid) ? x0[ch] : A::zero();
+            const S xb = (ALPHA ? m1 : (valid && both)) ? x1[ch] : A::zero();
+            const S l0 = lane_up(xa), l1 = lane_up(xb), r0 = lane_down(xa);
+            const S va = c >= 2 ? l0 : (ALPHA ? A::zero() : xa);
+            const S vb = c >= 1 ? l1 : (ALPHA ? A::zero() : xa);
+            const S a0 = A::add(va, A::mul(vb, 4));
+            const S a6 = A::add(a0, A::mul(xa, 6));
+            const S t0 = A::add(A::add(a6, A::mul(xb, 4)), r0);
+            S t1, t2;
+            if constexpr (ALPHA) {
+                t1 = a6;
+                t2 = A::add(a6, A::mul(xb, 4));
+            } else {
+                t1 = A::add(a0, A::mul(xa, 11));
+                const S d4 = A::mul(xb, 4);
+                t2 = A::add(A::add(a6, d4), A::div(d4, 4));
+            }
+            h[ch] = rcase == 1 ? t1 : (rcase == 2 ? t2 : t0);
+        }
+    };
+
+    S hm2[C], hm1[C], hc[C], hp1[C], hp2[C];
+    int qm2 = 0, qm1 = 0, qc = 0, qp1 = 0, qp2 = 0;
+#pragma unroll
+    for (int ch = 0; ch < C; ++ch) { hm2[ch] = hm1[ch] = hc[ch] = hp1[ch] = hp2[ch] = A::zero(); }
+    RawRow ring[4];
+    {
+        RawRow t0, t1, t2;
+        fetch(2 * i0 - 2, i0 > 0, t0);
+        fetch(2 * i0 - 1, i0 > 0, t1);
+        fetch(2 * i0, true, t2);
+        fetch(2 * i0 + 1, true, ring[0]);
+        fetch(2 * i0 + 2, true, ring[1]);
+        if (i0 > 0) {
+            sums(t0, hm2, qm2);
+            sums(t1, hm1, qm1);
+        }
+        sums(t2, hc, qc);
+    }
+    P* dst = static_cast<P*>(a.dst);
+    const uint32_t offd = owned ? (uint32_t)j * ES : kBufSkip;
+    const uint32_t offa = owned ? (uint32_t)j : kBufSkip;
+
+    auto step = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        constexpr int F0 = 2 * PH, F1 = 2 * PH + 1, N0 = 2 * (1 - PH), N1 = 2 * (1 - PH) + 1;
+        const int r = 2 * i;
+        const bool next = i + 1 < i1;
+        fetch(r + 3, next, ring[N0]);
+        fetch(r + 4, next, ring[N1]);
+        if (r + 1 < sh) sums(ring[F0], hp1, qp1);
+        if (r + 2 < sh) sums(ring[F1], hp2, qp2);
+
+        int ap = 0;
+        if (ALPHA) {
+            const int q0 = r == 0 ? 0 : qm2 + 4 * qm1;
+            if (r == sh - 1) ap = q0 + 6 * qc;
+            else if (r == sh - 2) ap = q0 + 6 * qc + 4 * qp1;
+            else ap = q0 + 6 * qc + 4 * qp1 + qp2;
+        }
+        S v[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            if (ALPHA) {
+                // src/pyramid.h:350-384, :459-504 (column recursion)
+                const S a0 = r == 0 ? A::zero() : A::add(hm2[ch], A::mul(hm1[ch], 4));
+                if (r == sh - 1) v[ch] = A::add(a0, A::mul(hc[ch], 6));
+                else if (r == sh - 2) v[ch] = A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4));
+                else v[ch] = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4)), hp2[ch]);
+            } else {
+                // src/pyramid.h:669-672, :741-760, :809-846 (clamped columns)
+                const S a0 = r == 0 ? A::mul(hc[ch], 5) : A::add(hm2[ch], A::mul(hm1[ch], 4));
+                if (r == sh - 1) {
+                    v[ch] = A::add(a0, A::mul(hc[ch], 11));
+                } else if (r == sh - 2) {
+                    const S d4 = A::mul(hp1[ch], 4);
+                    v[ch] = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), d4), A::div(d4, 4));
+                } else {
+                    v[ch] = A::add(A::add(A::add(a0, A::mul(hc[ch], 6)), A::mul(hp1[ch], 4)), hp2[ch]);
+                }
+            }
+        }
+        P out[C];
+        if constexpr (ALPHA && INT) {
+            // power-of-two weight sums (full windows) divide by a shift, chosen per wave
+            const int d = ap ? ap : 1;
+            if (wave_all((d & (d - 1)) == 0)) {
+                const int sd = __builtin_ctz((unsigned)d);
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) out[ch] = ap ? Px::wrap(div_pow2(v[ch], sd)) : (P)0;
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) out[ch] = ap ? Px::wrap(v[ch] / d) : (P)0;
+            }
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) out[ch] = ALPHA ? (ap ? Px::wrap(A::div(v[ch], ap)) : (P)0) : Px::wrap(A::div(v[ch], 256));
+        }
+        const P* drow = dst + (long)i * a.dpitch;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(drow + ch * a.dplane, (uint32_t)a.dw * ES);
+            if constexpr (ES == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)out[ch], rs, offd, 0, 0);
+            else if constexpr (ES == 4) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)out[ch], rs, offd, 0, 0);
+            else BPair<P>::store1(rs, offd, out[ch]);
+        }
+        if constexpr (ALPHA) {
+            __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(ap ? 255 : 0),
+                                                 buf_rsrc(a.dalpha + (long)i * a.dapitch, a.dalpha ? (uint32_t)a.dw : 0u),
+                                                 offa, 0, 0);
+        }
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            hm2[ch] = hc[ch];
+            hm1[ch] = hp1[ch];
+            hc[ch] = hp2[ch];
+        }
+        qm2 = qc; qm1 = qp1; qc = qp2;
+    };
+
+    int i = i0;
+    for (; i + 1 < i1; i += 2) {
+        step(Phase<0>(), i);
+        step(Phase<1>(), i + 1);
+    }
+    if (i < i1) step(Phase<0>(), i);
 }
 
 
