@@ -327,14 +327,28 @@ struct TailBuilder {
     TailLaunch t;
     double bytes = 0.0;
     TailBuilder() { memset(&t, 0, sizeof t); }
-    TailJob& add(int kind, int w, int h) {
+    TailJob& add(int kind, int w, int h) { return add_tiles(kind, (w + 63) / 64, (h + 3) / 4); }
+    TailJob& add_tiles(int kind, int tx, int ty) {
         TailJob& J = t.job[t.njobs++];
         J.kind = kind;
-        J.tiles_x = (w + 63) / 64;
+        J.tiles_x = tx;
         J.block0 = t.blocks;
-        t.blocks += J.tiles_x * ((h + 3) / 4);
+        t.blocks += tx * ty;
         return J;
     }
+    // A reduce job: streaming strips for outputs of at least kTailStripPixels
+    // (their 5x5 windows would load every source sample 6 times), one thread
+    // per output below.
+    void add_reduce(bool image, ReduceArgs a) {
+        if ((long)a.dw * a.dh >= kTailStripPixels) {
+            const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
+            a.rows = 4;
+            add_tiles(image ? kTailReduceImageStrip : kTailReduceMaskStrip, (waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows).u.r = a;
+        } else {
+            add(image ? kTailReduceImage : kTailReduceMask, a.dw, a.dh).u.r = a;
+        }
+    }
+    static constexpr long kTailStripPixels = 1L << 16;
 };
 
 ReduceArgs tail_reduce(const enbl_pyramid* p, int l, const uint8_t* sa, long sap, uint8_t* da, long dap)
@@ -418,9 +432,9 @@ int enblend_tail(enbl_context* ctx, const enbl_pyramid* mgp, enbl_pyramid* wlp, 
         const Level& D = blp->lv[l + 1];
         if (S_.w < 2 || S_.h < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
         TailBuilder b;
-        b.add(kTailReduceImage, D.w, D.h).u.r = tail_reduce(wlp, l, walpha(l), apitch(l), walpha(l + 1), apitch(l + 1));
-        b.add(kTailReduceImage, D.w, D.h).u.r = tail_reduce(blp, l, balpha(l), apitch(l), balpha(l + 1), apitch(l + 1));
-        b.add(kTailReduceMask, D.w, D.h).u.r = tail_reduce(mgp, l, nullptr, 0, nullptr, 0);
+        b.add_reduce(true, tail_reduce(wlp, l, walpha(l), apitch(l), walpha(l + 1), apitch(l + 1)));
+        b.add_reduce(true, tail_reduce(blp, l, balpha(l), apitch(l), balpha(l + 1), apitch(l + 1)));
+        b.add_reduce(false, tail_reduce(mgp, l, nullptr, 0, nullptr, 0));
         b.bytes += 2.0 * (esP + 1.0) * ((double)S_.w * S_.h + (double)D.w * D.h) + esM * ((double)S_.w * S_.h + (double)D.w * D.h);
         if (l > s) {
             expand_sub(b, wlp, l - 1);

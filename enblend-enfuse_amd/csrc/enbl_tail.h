@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "enbl_internal.h"
+#include "reduce_stream.h"
 #include "small_levels.h"
 
 namespace enbl {
@@ -18,11 +19,13 @@ enum TailKind : int {
     kTailExpandSub = 2,     // Laplacian level: G_l - expand(G_l+1) (ExpandArgs, P, C)
     kTailBlend = 3,         // blend of one level (BlendArgs)
     kTailCollapse = 4,      // blend of level l + expand of collapsed level l+1 (CollapseSmallArgs)
+    kTailReduceImageStrip = 5,   // as kTailReduceImage in streaming strips: 4 strips x r.rows rows per block
+    kTailReduceMaskStrip = 6,    // as kTailReduceMask in streaming strips
 };
 
 struct TailJob {
     int kind;
-    int tiles_x;   // 64-column tiles across the job's sample grid
+    int tiles_x;   // 64-column tiles (strips: groups of 4 strips) across the job's sample grid
     int block0;    // first block of the job in its launch
     int pad_;
     union {
