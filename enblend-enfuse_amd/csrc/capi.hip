@@ -478,7 +478,7 @@ int enblend_tail(enbl_context* ctx, const enbl_pyramid* mgp, enbl_pyramid* wlp, 
 
 int scratch(enbl_context* ctx, int slot, size_t bytes, void** out)
 {
-    if (slot < 0 || slot >= 16) return fail(ENBL_E_ARG, "scratch slot %d", slot);
+    if (slot < 0 || slot >= kScratchSlots) return fail(ENBL_E_ARG, "scratch slot %d", slot);
     if (ctx->scratch_bytes[slot] < bytes) {
         if (ctx->scratch_buf[slot]) {
             int rc = hip_check(hipStreamSynchronize(ctx->stream), "scratch");
@@ -538,7 +538,7 @@ int enbl_context_create(int device, enbl_context** out)
     c->profile = false;
     c->scratch_alpha = nullptr;
     c->scratch_alpha_bytes = 0;
-    for (int i = 0; i < 16; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
+    for (int i = 0; i < kScratchSlots; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
     c->batch = nullptr;
     for (int i = 0; i < 3; ++i) { c->batch_rows[i] = 0; c->batch_bpc[i] = 0; }
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -557,7 +557,7 @@ int enbl_context_destroy(enbl_context* ctx)
     batch_release(ctx);
     for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
     if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
-    for (int i = 0; i < 16; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
+    for (int i = 0; i < kScratchSlots; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return ENBL_OK;

@@ -77,6 +77,8 @@ std::vector<Level> plan_levels(int ptype, int channels, int w, int h, int levels
 struct enbl_context;
 struct enbl_pyramid;
 
+constexpr int kScratchSlots = 24;
+
 namespace enbl {
 struct BatchState;   // batched enfuse: arena, job tables, captured graph (batch_capi.hip)
 namespace host {
@@ -133,8 +135,8 @@ struct enbl_context {
     std::vector<Cached> cache;   // pipeline scratch pyramids
     void* scratch_alpha;         // per-level alpha scratch for Gaussian builds
     size_t scratch_alpha_bytes;
-    void* scratch_buf[16];       // general device scratch (weights, fused passes)
-    size_t scratch_bytes[16];
+    void* scratch_buf[kScratchSlots];     // general device scratch (weights, fused passes, NFT)
+    size_t scratch_bytes[kScratchSlots];
     enbl::BatchState* batch;     // batched enfuse state, created on first use
     int batch_rows[3];           // launch shape of the batched passes (0: default)
     int batch_bpc[3];

@@ -1,0 +1,266 @@
+// Nearest-feature-transform seam mask (src/nearest.h:424-552, open
+// boundaries) with the distance transform of the reference's OpenMP build:
+// the Felzenszwalb-Huttenlocher 1-D lower-envelope transform in float over
+// every column, then over every row of the intermediate image
+// (src/openmp_vigra.h:244-470).
+//
+// A 1-D transform builds its lower envelope sequentially with a stack, and
+// its float roundings (q^2 beyond 2^24, the envelope intersections, the
+// final sum and the square root) decide which samples are nearest, so each
+// column (first pass) and each row (second pass) is one thread running the
+// reference's operations in the reference's order.  Both distance images
+// (to the white-only and to the black-only pixels) run in the same launches.
+// The stacks live in device scratch, entry k of thread t at k * T + t, so
+// lanes at similar depths share cache lines; the top entry stays in
+// registers.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "enbl_device.h"
+#include "enbl_internal.h"
+
+namespace enbl {
+namespace {
+
+struct NftArgs {
+    const uint8_t* white; long wpitch;
+    const uint8_t* black; long bpitch;
+    int w, h, norm;
+    float* inter; long iplane;     // 2 planes of w x h (pitch w): after the column pass
+    float* dist; long dplane; long dpitch;   // 2 planes of distances (row pitch dpitch)
+    int* sv; float* sz; float* sf;  // stacks: sample index, envelope bound, sample value
+    unsigned* tally;               // 2 counters of exclusive pixels
+    uint8_t* mask; long mpitch;
+};
+
+// EuclideanTransform1D (src/openmp_vigra.h:304-357) of n samples: f(q) reads
+// sample q, out(q, d) writes result q.  Stack entry k of this thread at
+// index k * T (pointers already offset to the thread).
+template <class F, class O>
+__device__ __forceinline__ void euclid_1d(int n, F f, O out, int* v, float* z, float* fv, long T)
+{
+    const float inf = INFINITY;
+    int k = 0;
+    int vk = 0;                 // v[k], fv[k], z[k] of the top entry
+    float fk = f(0), zk = -inf;
+    v[0] = 0;
+    fv[0] = fk;
+    z[0] = zk;
+    for (int q = 1; q < n; ++q) {
+        const float fq = f(q);
+        const float sum_q = fq + (float)q * (float)q;
+        float s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
+        while (s <= zk) {
+            --k;
+            vk = v[k * T];
+            fk = fv[k * T];
+            zk = z[k * T];
+            s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
+        }
+        ++k;
+        vk = q;
+        fk = fq;
+        zk = s;
+        v[k * T] = q;
+        fv[k * T] = fq;
+        z[k * T] = s;
+    }
+    z[(long)(k + 1) * T] = inf;
+    k = 0;
+    vk = v[0];
+    fk = fv[0];
+    float zn = z[T];            // z[k + 1]
+    for (int q = 0; q < n; ++q) {
+        while (zn < (float)q) {
+            ++k;
+            vk = v[k * T];
+            fk = fv[k * T];
+            zn = z[(long)(k + 1) * T];
+        }
+        out(q, (float)((q - vk) * (q - vk)) + fk);
+    }
+}
+
+// ManhattanTransform1D (src/openmp_vigra.h:283-301); `in(q)` reads the
+// result written so far for the backward sweep.
+template <class F, class O, class I>
+__device__ __forceinline__ void manhattan_1d(int n, F f, O out, I in)
+{
+    float d = f(0);
+    out(0, d);
+    for (int q = 1; q < n; ++q) {
+        d = fminf(f(q), d + 1.0f);
+        out(q, d);
+    }
+    for (int q = n - 2; q >= 0; --q) {
+        d = fminf(in(q), d + 1.0f);
+        out(q, d);
+    }
+}
+
+// Columns: t = blockIdx.y (0: white-only features, 1: black-only).
+// f = FLT_MAX on background, 0 on a feature (src/openmp_vigra.h:384-392).
+__global__ __launch_bounds__(64) void nft_columns_kernel(NftArgs a)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int t = blockIdx.y;
+    const bool live = x < a.w;
+    const int xc = live ? x : a.w - 1;
+    const uint8_t* p = t == 0 ? a.white : a.black;
+    const uint8_t* q = t == 0 ? a.black : a.white;   // null: features are the non-zero samples of p
+    const long pp = t == 0 ? a.wpitch : a.bpitch, qp = t == 0 ? a.bpitch : a.wpitch;
+    unsigned count = 0;
+    // saturating difference p - q (src/nearest.h:359-372)
+    auto f = [&](int y) {
+        const uint8_t u = *gmem(p + (long)y * pp + xc), v = q ? *gmem(q + (long)y * qp + xc) : (uint8_t)0;
+        const bool feat = v < u;
+        count += feat ? 1u : 0u;
+        return feat ? 0.0f : FLT_MAX;
+    };
+    float* col = a.inter + t * a.iplane + xc;
+    auto out = [&](int y, float d) { if (live) *gmem(col + (long)y * a.w) = d; };
+    if (a.norm == 1) {
+        manhattan_1d(a.h, f, out, [&](int y) { return *gmem(col + (long)y * a.w); });
+    } else {
+        const long T = (long)gridDim.y * gridDim.x * 64;
+        const long tid = (long)t * gridDim.x * 64 + x;
+        euclid_1d(a.h, f, out, a.sv + tid, a.sz + tid, a.sf + tid, T);
+    }
+    // each sample of the column was read once by f (forward sweep)
+    count = live ? count : 0u;
+    for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o);
+    if (threadIdx.x == 0) atomicAdd(&a.tally[t], count);
+}
+
+// Rows of the intermediate image; the Euclidean result is its square root.
+__global__ __launch_bounds__(64) void nft_rows_kernel(NftArgs a)
+{
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (y >= a.h) return;
+    const float* row = a.inter + t * a.iplane + (long)y * a.w;
+    float* drow = a.dist + t * a.dplane + (long)y * a.dpitch;
+    auto f = [&](int x) { return *gmem(row + x); };
+    if (a.norm == 1) {
+        manhattan_1d(a.w, f, [&](int x, float d) { *gmem(drow + x) = d; }, [&](int x) { return *gmem(drow + x); });
+    } else {
+        const long T = (long)gridDim.y * gridDim.x * 64;
+        const long tid = (long)t * gridDim.x * 64 + y;
+        euclid_1d(a.w, f, [&](int x, float d) { *gmem(drow + x) = sqrtf(d); }, a.sv + tid, a.sz + tid, a.sf + tid, T);
+    }
+}
+
+// mask = dist12 < dist21 ? 255 : 0 (src/nearest.h:540-545)
+__global__ __launch_bounds__(256) void nft_mask_kernel(NftArgs a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= a.w) return;
+    const long i = (long)y * a.dpitch + x;
+    *gmem(a.mask + (long)y * a.mpitch + x) = *gmem(a.dist + i) < *gmem(a.dist + a.dplane + i) ? 255 : 0;
+}
+
+__global__ void nft_zero_kernel(unsigned* p) { if (threadIdx.x < 2) p[threadIdx.x] = 0u; }
+
+}  // namespace
+}  // namespace enbl
+
+using namespace enbl;
+using namespace enbl::host;
+
+namespace {
+
+// The column and row passes of nt (1 or 2) distance transforms.
+int nft_passes(enbl_context* ctx, NftArgs& a, int nt)
+{
+    const int w = a.w, h = a.h;
+    const int gx_c = (w + 63) / 64, gx_r = (h + 63) / 64;
+    const size_t plane = (size_t)w * h;
+    // stacks: n + 1 entries per thread of a pass (columns: n = h, rows: n = w)
+    const size_t entries = std::max((size_t)gx_c * 64 * nt * ((size_t)h + 1), (size_t)gx_r * 64 * nt * ((size_t)w + 1));
+    void *fbuf, *sbuf = nullptr, *tbuf;
+    int rc = scratch(ctx, 16, (size_t)nt * plane * sizeof(float), &fbuf);
+    if (!rc && a.norm == 2) rc = scratch(ctx, 17, entries * 12, &sbuf);
+    if (!rc) rc = scratch(ctx, 18, 64, &tbuf);
+    if (rc) return rc;
+    a.inter = (float*)fbuf; a.iplane = (long)plane;
+    a.sv = nullptr; a.sz = nullptr; a.sf = nullptr;
+    if (a.norm == 2) {
+        a.sv = (int*)sbuf;
+        a.sz = (float*)((char*)sbuf + entries * 4);
+        a.sf = (float*)((char*)sbuf + entries * 8);
+    }
+    a.tally = (unsigned*)tbuf;
+    {
+        Timed t(ctx, "nft_columns", nt * (double)plane * (2.0 + 4.0));
+        hipLaunchKernelGGL(nft_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, a.tally);
+        hipLaunchKernelGGL(nft_columns_kernel, dim3(gx_c, nt), dim3(64), 0, ctx->stream, a);
+        rc = launch_status("nft_columns");
+        if (rc) return rc;
+    }
+    {
+        Timed t(ctx, "nft_rows", nt * (double)plane * 8.0);
+        hipLaunchKernelGGL(nft_rows_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
+        rc = launch_status("nft_rows");
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int enbl_distance_transform(enbl_context* ctx, int w, int h, const uint8_t* src, long spitch, int norm, float* dst,
+                            long dpitch)
+{
+    if (!ctx || !src || !dst) return fail(ENBL_E_ARG, "enbl_distance_transform: null argument");
+    if (w < 1 || h < 1 || spitch < w || dpitch < w) return fail(ENBL_E_SIZE, "enbl_distance_transform: bad size or pitch");
+    if (norm != 1 && norm != 2) return fail(ENBL_E_UNSUPPORTED, "enbl_distance_transform: norm %d (1 Manhattan, 2 Euclidean)", norm);
+    NftArgs a;
+    memset(&a, 0, sizeof a);
+    a.white = src; a.wpitch = spitch; a.black = nullptr; a.bpitch = 0;
+    a.w = w; a.h = h; a.norm = norm;
+    a.dist = dst; a.dplane = 0; a.dpitch = dpitch;
+    return nft_passes(ctx, a, 1);
+}
+
+int enbl_nft_mask(enbl_context* ctx, int w, int h, const uint8_t* white, long wpitch, const uint8_t* black, long bpitch,
+                  int norm, uint8_t* mask, long mpitch)
+{
+    if (!ctx || !white || !black || !mask) return fail(ENBL_E_ARG, "enbl_nft_mask: null argument");
+    if (w < 1 || h < 1 || wpitch < w || bpitch < w || mpitch < w) return fail(ENBL_E_SIZE, "enbl_nft_mask: bad size or pitch");
+    if (norm != 1 && norm != 2) return fail(ENBL_E_UNSUPPORTED, "enbl_nft_mask: norm %d (1 Manhattan, 2 Euclidean)", norm);
+    const size_t plane = (size_t)w * h;
+    void* dbuf;
+    int rc = scratch(ctx, 19, 2 * plane * sizeof(float), &dbuf);
+    if (rc) return rc;
+    NftArgs a;
+    memset(&a, 0, sizeof a);
+    a.white = white; a.wpitch = wpitch; a.black = black; a.bpitch = bpitch;
+    a.w = w; a.h = h; a.norm = norm;
+    a.dist = (float*)dbuf; a.dplane = (long)plane; a.dpitch = w;
+    a.mask = mask; a.mpitch = mpitch;
+    rc = nft_passes(ctx, a, 2);
+    if (rc) return rc;
+    {
+        Timed t(ctx, "nft_mask", (double)plane * 9.0);
+        hipLaunchKernelGGL(nft_mask_kernel, dim3((w + 255) / 256, h), dim3(256), 0, ctx->stream, a);
+        rc = launch_status("nft_mask");
+        if (rc) return rc;
+    }
+    unsigned tally[2];
+    rc = hip_check(hipMemcpyAsync(tally, a.tally, sizeof tally, hipMemcpyDeviceToHost, ctx->stream), "nft tally");
+    if (!rc) rc = hip_check(hipStreamSynchronize(ctx->stream), "nft tally");
+    if (rc) return rc;
+    // too few exclusive pixels in both images: the overlap is (nearly)
+    // complete and the seam line is meaningless (src/nearest.h:449-457, :523-538)
+    const unsigned long threshold = 2UL * 2UL * ((unsigned long)h + (unsigned long)w);
+    if (std::max(tally[0], tally[1]) < threshold)
+        return fail(ENBL_E_PRECONDITION, "excessive image overlap detected; too high risk of defective seam line");
+    return ENBL_OK;
+}
+
+}  // extern "C"

@@ -294,6 +294,29 @@ int enbl_expand_adjoint(enbl_context* ctx, const double* g, long gpitch, int fw,
 int enbl_reduce_adjoint(enbl_context* ctx, const double* g, long gpitch, int dw, int dh,
                         double* gf, long fpitch, int sw, int sh);
 
+/* ---- seam mask ------------------------------------------------------------ */
+
+/* Replaces vigra::omp::distanceTransform (src/openmp_vigra.h:442-470, the
+ * OpenMP build's Felzenszwalb-Huttenlocher transform, src/openmp_vigra.h:
+ * 244-438): dst = distance of every pixel to the nearest non-zero pixel of
+ * src, in float with the reference's operation order; norm 1 = Manhattan,
+ * 2 = Euclidean (square root of the squared distance).  Pixels of an image
+ * without any non-zero pixel get the transform of FLT_MAX.  Device buffers,
+ * pitches in elements. */
+int enbl_distance_transform(enbl_context* ctx, int w, int h, const uint8_t* src, long spitch, int norm,
+                            float* dst, long dpitch);
+
+/* Replaces nearestFeatureTransform with open boundaries (src/nearest.h:
+ * 424-552), the primary seam generator of enblend's fine mask without
+ * optimisation (src/mask.h:1101-1104, :1130-1135, :1176-1179): the
+ * distance transforms of white-minus-black and black-minus-white alpha
+ * (saturating), mask = 255 where the pixel is nearer to the white-only than
+ * to the black-only pixels, else 0.  The mask is written in any case;
+ * ENBL_E_PRECONDITION when neither image has 2 * 2 * (w + h) exclusive
+ * pixels (the reference's "excessive image overlap" exit). */
+int enbl_nft_mask(enbl_context* ctx, int w, int h, const uint8_t* white_alpha, long wapitch,
+                  const uint8_t* black_alpha, long bapitch, int norm, uint8_t* mask, long mpitch);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510

@@ -136,6 +136,16 @@ void orc_diff_expand_T(const double* g, int fw, int fh, double* gc, int cw, int 
 void orc_diff_reduce_T(const double* g, int dw, int dh, double* gf, int sw, int sh);
 
 /* Number of OpenMP threads the pointwise stages use. */
+/* Nearest-feature-transform seam mask (src/nearest.h:424-552) with the
+ * OpenMP build's float Felzenszwalb-Huttenlocher distance transform
+ * (src/openmp_vigra.h:244-470); norm 1 = Manhattan, 2 = Euclidean.
+ * Returns -2 (mask still written) on excessive overlap. */
+int orc_nft_mask(const uint8_t* white, long wpitch, const uint8_t* black, long bpitch, int w, int h, int norm,
+                 uint8_t* mask, long mpitch);
+/* The distance transform alone: distance of every pixel to the nearest
+ * non-zero pixel of src (float, sqrt for the Euclidean norm). */
+int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -34,6 +34,8 @@ _SIGS = {
     "orc_diff_backward": [_i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp],
     "orc_diff_expand_T": [_vp, _i, _i, _vp, _i, _i],
     "orc_diff_reduce_T": [_vp, _i, _i, _vp, _i, _i],
+    "orc_nft_mask": [_vp, _l, _vp, _l, _i, _i, _i, _vp, _l],
+    "orc_distance_transform": [_vp, _l, _i, _i, _i, _vp, _l],
 }
 
 EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE = range(5)
@@ -257,6 +259,31 @@ def _reduce_T(self, g, sw, sh):
     return gf
 
 
+def _distance_transform(self, src, norm=2):
+    """Float distance of every pixel to the nearest non-zero pixel."""
+    src = np.ascontiguousarray(src, np.uint8)
+    h, w = src.shape
+    out = np.zeros((h, w), np.float32)
+    rc = self.lib.orc_distance_transform(_p(src), w, w, h, norm, _p(out), w)
+    if rc:
+        raise ValueError(f"orc_distance_transform rc={rc}")
+    return out
+
+
+def _nft_mask(self, white, black, norm=2):
+    """(mask, rc): rc = -2 on excessive overlap (mask written anyway)."""
+    white = np.ascontiguousarray(white, np.uint8)
+    black = np.ascontiguousarray(black, np.uint8)
+    h, w = white.shape
+    mask = np.zeros((h, w), np.uint8)
+    rc = self.lib.orc_nft_mask(_p(white), w, _p(black), w, w, h, norm, _p(mask), w)
+    if rc not in (0, -2):
+        raise ValueError(f"orc_nft_mask rc={rc}")
+    return mask, rc
+
+
+Oracle.distance_transform = _distance_transform
+Oracle.nft_mask = _nft_mask
 Oracle.diff_forward = _diff_forward
 Oracle.diff_backward = _diff_backward
 Oracle.expand_T = _expand_T

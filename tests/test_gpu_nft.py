@@ -1,0 +1,105 @@
+"""GPU nearest-feature-transform seam mask and float distance transform
+(nft.hip) against the oracle, bit for bit, including columns and rows
+longer than 4096 where the float roundings of the reference decide the
+result."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _gpu_dt(enbl, ctx, src, norm):
+    h, w = src.shape
+    pitch = w + 3
+    d_src = torch.zeros((h, pitch), dtype=torch.uint8, device="cuda")
+    d_src[:, :w] = torch.from_numpy(np.ascontiguousarray(src)).cuda()
+    d_out = torch.full((h, w + 1), -1.0, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    enbl.check(enbl.lib().enbl_distance_transform(ctx.h, w, h, _vp(d_src), pitch, norm, _vp(d_out), w + 1))
+    ctx.sync()
+    out = d_out.cpu().numpy()
+    assert (out[:, w] == -1.0).all(), "wrote past the row"
+    return out[:, :w]
+
+
+def _binary(rng, h, w, density):
+    return ((rng.random((h, w)) < density) * rng.integers(1, 256, size=(h, w))).astype(np.uint8)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 300), (300, 1), (64, 65), (257, 513), (600, 4500), (4200, 300)])
+@pytest.mark.parametrize("norm", [1, 2])
+@pytest.mark.parametrize("density", [0.0005, 0.05, 0.6])
+def test_distance_transform_matches_oracle(enbl, gpu_ctx, oracle, shape, norm, density):
+    h, w = shape
+    print(f"distance transform {w}x{h} norm={norm} density={density}", flush=True)
+    rng = np.random.default_rng(h * 7919 + w * 31 + norm + int(density * 1e4))
+    src = _binary(rng, h, w, density)
+    np.testing.assert_array_equal(_gpu_dt(enbl, gpu_ctx, src, norm), oracle.distance_transform(src, norm))
+
+
+@pytest.mark.parametrize("norm", [1, 2])
+def test_distance_transform_without_features(enbl, gpu_ctx, oracle, norm):
+    src = np.zeros((37, 53), np.uint8)
+    np.testing.assert_array_equal(_gpu_dt(enbl, gpu_ctx, src, norm), oracle.distance_transform(src, norm))
+
+
+def _polygon_alphas(rng, h, w):
+    """Two overlapping convex regions (ellipses with a random tilt)."""
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    out = []
+    for cx in (0.32, 0.68):
+        th = rng.uniform(-0.3, 0.3)
+        u = (xx - cx * w) * np.cos(th) + (yy - 0.5 * h) * np.sin(th)
+        v = -(xx - cx * w) * np.sin(th) + (yy - 0.5 * h) * np.cos(th)
+        out.append(((u / (0.4 * w)) ** 2 + (v / (0.47 * h)) ** 2 < 1).astype(np.uint8) * 255)
+    return out
+
+
+def _gpu_nft(enbl, ctx, white, black, norm=2):
+    h, w = white.shape
+    ap, mp = w + 5, w + 2
+    dw = torch.zeros((h, ap), dtype=torch.uint8, device="cuda")
+    db = torch.zeros((h, ap), dtype=torch.uint8, device="cuda")
+    dw[:, :w] = torch.from_numpy(white).cuda()
+    db[:, :w] = torch.from_numpy(black).cuda()
+    dm = torch.full((h, mp), 7, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    rc = enbl.lib().enbl_nft_mask(ctx.h, w, h, _vp(dw), ap, _vp(db), ap, norm, _vp(dm), mp)
+    ctx.sync()
+    m = dm.cpu().numpy()
+    assert (m[:, w:] == 7).all(), "wrote past the row"
+    return m[:, :w], rc
+
+
+@pytest.mark.parametrize("shape", [(96, 160), (1536, 2048), (700, 5000)])
+@pytest.mark.parametrize("norm", [1, 2])
+def test_nft_mask_matches_oracle(enbl, gpu_ctx, oracle, shape, norm):
+    h, w = shape
+    print(f"nft mask {w}x{h} norm={norm}", flush=True)
+    rng = np.random.default_rng(h + w + norm)
+    white, black = _polygon_alphas(rng, h, w)
+    want, want_rc = oracle.nft_mask(white, black, norm)
+    got, rc = _gpu_nft(enbl, gpu_ctx, white, black, norm)
+    assert want_rc == 0 and rc == 0
+    np.testing.assert_array_equal(got, want)
+
+
+def test_nft_mask_excessive_overlap(enbl, gpu_ctx, oracle):
+    """Identical alphas: the reference's excessive-overlap exit becomes
+    ENBL_E_PRECONDITION; the mask is still written and equals the oracle's."""
+    h, w = 40, 60
+    white = np.full((h, w), 255, np.uint8)
+    black = white.copy()
+    black[:5, :5] = 0
+    want, want_rc = oracle.nft_mask(white, black)
+    got, rc = _gpu_nft(enbl, gpu_ctx, white, black)
+    assert want_rc == -2 and rc == enbl.ENBL_E_PRECONDITION
+    np.testing.assert_array_equal(got, want)

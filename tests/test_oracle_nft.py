@@ -1,0 +1,175 @@
+"""Pins of the oracle's nearest-feature-transform seam mask and its float
+Felzenszwalb-Huttenlocher distance transform (oracle/oracle_nft.cpp).
+
+Where every value the transform forms is an integer below 2^24 (rows and
+columns shorter than 4096) its float arithmetic is exact, so it must equal
+the brute-force distance to the nearest feature pixel; the Manhattan
+transform is exact at every size.  Longer rows round q^2 and the envelope
+intersections in float: there the oracle is checked against the exact
+transform within the rounding bound (partially pinned), and the GPU is
+compared with the oracle bit for bit."""
+import numpy as np
+import pytest
+
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+
+
+def _brute_d2(src):
+    """Exact squared Euclidean distance to the nearest non-zero pixel (int64;
+    -1 where there is none)."""
+    h, w = src.shape
+    ys, xs = np.nonzero(src)
+    if len(ys) == 0:
+        return np.full((h, w), -1, np.int64)
+    yy, xx = np.mgrid[0:h, 0:w]
+    best = np.full((h, w), np.iinfo(np.int64).max, np.int64)
+    for y, x in zip(ys, xs):
+        best = np.minimum(best, (yy - y) ** 2 + (xx - x) ** 2)
+    return best
+
+
+def _brute_l1(src):
+    h, w = src.shape
+    ys, xs = np.nonzero(src)
+    yy, xx = np.mgrid[0:h, 0:w]
+    best = np.full((h, w), np.iinfo(np.int64).max, np.int64)
+    for y, x in zip(ys, xs):
+        best = np.minimum(best, np.abs(yy - y) + np.abs(xx - x))
+    return best
+
+
+def _random_binary(rng, h, w, density):
+    return (rng.random((h, w)) < density).astype(np.uint8) * rng.integers(1, 256)
+
+
+SHAPES = [(1, 1), (1, 9), (9, 1), (2, 2), (7, 5), (16, 33), (40, 27), (64, 64)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("density", [0.02, 0.2, 0.7])
+def test_euclidean_transform_equals_brute_force(oracle, shape, density):
+    rng = np.random.default_rng(hash((shape, density)) & 0xffff)
+    h, w = shape
+    src = _random_binary(rng, h, w, density)
+    got = oracle.distance_transform(src, 2)
+    d2 = _brute_d2(src)
+    if (d2 < 0).all():
+        np.testing.assert_array_equal(got, np.full((h, w), np.sqrt(FLT_MAX), np.float32))
+    else:
+        np.testing.assert_array_equal(got, np.sqrt(d2.astype(np.float32)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("density", [0.02, 0.3])
+def test_manhattan_transform_equals_brute_force(oracle, shape, density):
+    rng = np.random.default_rng(7 + (hash((shape, density)) & 0xffff))
+    h, w = shape
+    src = _random_binary(rng, h, w, density)
+    got = oracle.distance_transform(src, 1)
+    if not src.any():
+        np.testing.assert_array_equal(got, np.full((h, w), FLT_MAX, np.float32))
+    else:
+        np.testing.assert_array_equal(got, _brute_l1(src).astype(np.float32))
+
+
+def test_transform_known_answers(oracle):
+    src = np.zeros((5, 7), np.uint8)
+    src[2, 3] = 1
+    want = np.sqrt(((np.mgrid[0:5, 0:7][0] - 2) ** 2 + (np.mgrid[0:5, 0:7][1] - 3) ** 2).astype(np.float32))
+    np.testing.assert_array_equal(oracle.distance_transform(src, 2), want)
+    # features along one column only: every column without features takes
+    # its distance from the row pass
+    src = np.zeros((6, 40), np.uint8)
+    src[:, 0] = 1
+    np.testing.assert_array_equal(oracle.distance_transform(src, 2), np.tile(np.arange(40, dtype=np.float32), (6, 1)))
+
+
+def test_long_rows_round_within_float_bounds(oracle):
+    """Rows longer than 4096: q^2 and the envelope intersections round in
+    float, so the result may differ from the exact distance, but only by the
+    float rounding of the squared distance and of the sum."""
+    rng = np.random.default_rng(11)
+    h, w = 3, 6000
+    src = np.zeros((h, w), np.uint8)
+    src[:, rng.integers(0, w, size=5)] = 1
+    src[1, :] = 0
+    got = oracle.distance_transform(src, 2).astype(np.float64)
+    ys, xs = np.nonzero(src)
+    xx = np.arange(w)
+    exact = np.sqrt(np.array([[min((y - yy) ** 2 + (x - xc) ** 2 for yy, xc in zip(ys, xs)) for x in xx] for y in range(h)],
+                             np.float64))
+    rel = np.abs(got - exact) / np.maximum(exact, 1.0)
+    assert rel.max() < 2.0 ** -20
+
+
+def _blob_alphas(rng, h, w):
+    yy, xx = np.mgrid[0:h, 0:w]
+    cx, cy = 0.3 * w + rng.normal(0, 1), 0.5 * h
+    white = (((xx - cx) / (0.45 * w)) ** 2 + ((yy - cy) / (0.48 * h)) ** 2 < 1).astype(np.uint8) * 255
+    black = (((xx - 0.7 * w) / (0.45 * w)) ** 2 + ((yy - cy) / (0.48 * h)) ** 2 < 1).astype(np.uint8) * 255
+    return white, black
+
+
+@pytest.mark.parametrize("shape", [(24, 40), (37, 61), (64, 48)])
+def test_nft_mask_equals_brute_force(oracle, shape):
+    rng = np.random.default_rng(shape[0] * 100 + shape[1])
+    h, w = shape
+    white, black = _blob_alphas(rng, h, w)
+    mask, rc = oracle.nft_mask(white, black)
+    assert rc == 0
+    d12 = _brute_d2((white > black).astype(np.uint8))
+    d21 = _brute_d2((black > white).astype(np.uint8))
+    np.testing.assert_array_equal(mask, np.where(d12 < d21, 255, 0).astype(np.uint8))
+
+
+def test_nft_mask_ties_go_to_black(oracle):
+    """Equidistant pixels compare dist12 < dist21 false: 0 (the black image),
+    src/nearest.h:540-545; a planted <= gives a different mask."""
+    h, w = 40, 31
+    white = np.zeros((h, w), np.uint8)
+    black = np.zeros((h, w), np.uint8)
+    white[:, :12] = 255
+    black[:, 19:] = 255
+    white[:, 12:19] = black[:, 12:19] = 200   # common, equal alphas
+    mask, rc = oracle.nft_mask(white, black)
+    assert rc == 0
+    want = np.zeros((h, w), np.uint8)
+    want[:, :15] = 255                        # x = 15 is 4 from both sides
+    np.testing.assert_array_equal(mask, want)
+    d12 = oracle.distance_transform((white > black).astype(np.uint8))
+    d21 = oracle.distance_transform((black > white).astype(np.uint8))
+    np.testing.assert_array_equal(mask, np.where(d12 < d21, 255, 0).astype(np.uint8))
+    assert not np.array_equal(mask, np.where(d12 <= d21, 255, 0).astype(np.uint8))
+
+
+def test_nft_mask_difference_is_saturating(oracle):
+    """Partial alphas: the features of the white image are where its alpha
+    exceeds the black alpha (saturating difference), overlap included; a
+    planted "white set and black unset" rule differs."""
+    rng = np.random.default_rng(5)
+    h, w = 30, 50
+    white, black = _blob_alphas(rng, h, w)
+    white = np.where(white > 0, 180, 0).astype(np.uint8)   # the overlap belongs to white
+    black = np.where(black > 0, 120, 0).astype(np.uint8)
+    mask, rc = oracle.nft_mask(white, black)
+    assert rc == 0
+    d12 = _brute_d2((white > black).astype(np.uint8))
+    d21 = _brute_d2((black > white).astype(np.uint8))
+    np.testing.assert_array_equal(mask, np.where(d12 < d21, 255, 0).astype(np.uint8))
+    planted = np.where(_brute_d2(((white != 0) & (black == 0)).astype(np.uint8)) < d21, 255, 0)
+    assert not np.array_equal(mask, planted)
+
+
+def test_nft_excessive_overlap(oracle):
+    """Fewer than 2 * 2 * (w + h) exclusive pixels in both images is the
+    reference's excessive-overlap exit (src/nearest.h:449-457, :523-538)."""
+    h, w = 20, 30
+    threshold = 4 * (w + h)
+    white = np.full((h, w), 255, np.uint8)
+    black = white.copy()
+    assert oracle.nft_mask(white, black)[1] == -2
+    flat = black.reshape(-1)
+    flat[: threshold - 1] = 0                # white-only pixels: threshold - 1
+    assert oracle.nft_mask(white, black)[1] == -2
+    flat[: threshold] = 0
+    assert oracle.nft_mask(white, black)[1] == 0
