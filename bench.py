@@ -3,7 +3,9 @@
 
 Workload (one step): one enblend blend step of two 4096x3072 RGBA uint16
 images sharing one synthetic scene, 25% horizontal overlap, on their
-7168x3072 union canvas with 10 pyramid levels: mask Gaussian pyramid,
+7168x3072 union canvas with 10 pyramid levels, the seam mask being the
+nearest-feature transform of the two alphas (made once, timed apart as
+`nft`): mask Gaussian pyramid,
 two Laplacian pyramids under the images' alphas, blend, collapse and the
 dithered copy back into the 16-bit image (src/enblend.h:378-510).  Inputs are
 resident in HBM before timing starts.
@@ -351,7 +353,7 @@ def main():
                           "copies": args.steps}), flush=True)
         return
     print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
-    white, wa, black, ba, mask = scenes.enblend_pair(seed=1)
+    white, wa, black, ba, _ = scenes.enblend_pair(seed=1)
     h, w, C = white.shape
     levels = enbl.auto_levels(w, h)
     mpix = 2 * 4096 * 3072 / 1e6
@@ -361,7 +363,7 @@ def main():
     d_wa = torch.from_numpy(wa).to(dev)
     d_black0 = torch.from_numpy(black).to(dev)
     d_ba0 = torch.from_numpy(ba).to(dev)
-    d_mask = torch.from_numpy(mask).to(dev)
+    d_mask = torch.empty((h, w), dtype=torch.uint8, device=dev)
     d_black = d_black0.clone()
     d_ba = d_ba0.clone()
     torch.cuda.synchronize()
@@ -372,6 +374,24 @@ def main():
     def vp(t):
         import ctypes
         return ctypes.c_void_p(t.data_ptr())
+
+    # The seam mask: nearest-feature transform of the two alphas (Euclidean,
+    # open boundaries), the primary seam generator of the fine, unoptimised
+    # mask.
+    print("bench: nearest-feature-transform seam mask", file=sys.stderr, flush=True)
+
+    def nft():
+        enbl.check(L.enbl_nft_mask(ctx.h, w, h, vp(d_wa), w, vp(d_ba0), w, 2, vp(d_mask), w))
+
+    nft()
+    ctx.sync()
+    nft_runs = 5
+    t0 = time.perf_counter()
+    for _ in range(nft_runs):
+        nft()
+    ctx.sync()
+    nft_ms = (time.perf_counter() - t0) * 1e3 / nft_runs
+    mask = d_mask.cpu().numpy()
 
     def step(bk, bka):
         enbl.check(L.enbl_enblend_pair(ctx.h, enbl.U16, C, w, h, levels, 0,
@@ -428,7 +448,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "enblend step, 2 x 4096x3072 RGBA uint16, 25% overlap, 7168x3072 ROI, "
+        "config": {"workload": "enblend step, 2 x 4096x3072 RGBA uint16, 25% overlap, 7168x3072 ROI, NFT seam mask, "
                                f"{levels} levels, Int32 fixed-point pyramids",
                    "input_mpix_per_step": mpix, "levels": levels},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -438,6 +458,9 @@ def main():
                      "dram_GB_per_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "all_kernels_GB_per_s": round(all_gbs, 1), "kernel_ms_per_step": round(total_ms / nprof, 4)},
         "kernels": kernels,
+        "nft": {"workload": f"nearest-feature-transform seam mask of the two alphas, {w}x{h}, Euclidean, "
+                            "float Felzenszwalb-Huttenlocher transforms", "ms": round(nft_ms, 3),
+                "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs},
     }
 
     if not args.no_cpu:
@@ -447,6 +470,15 @@ def main():
                                   "sample": f"full workload, median of {args.cpu_repeats} runs "
                                             f"({cpu_s:.2f} s each)"}
         result["parity"] = bool(np.array_equal(cpu_out, gpu_out) and np.array_equal(cpu_alpha, gpu_alpha))
+        print("bench: CPU nearest-feature transform (oracle)", file=sys.stderr, flush=True)
+        from oracle_binding import load_oracle
+        o = load_oracle()
+        t0 = time.perf_counter()
+        cpu_mask, cpu_rc = o.nft_mask(wa, ba)
+        nft_cpu_s = time.perf_counter() - t0
+        result["nft"]["parity_vs_oracle"] = bool(cpu_rc == 0 and np.array_equal(cpu_mask, mask))
+        result["nft"]["cpu_baseline"] = {"ms": round(nft_cpu_s * 1e3, 1), "cores": o.lib.orc_num_threads(),
+                                         "kind": "port", "sample": "full mask, one run"}
     ctx.close()
     del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
     if not args.no_batch:
