@@ -38,7 +38,7 @@ def _load(name, path):
     return mod
 
 
-TRAFFIC_FILE = "profiles/laplacian_level_i32_c3_enblend_pair_traffic.md"
+TRAFFIC_FILE = "profiles/enblend_pair_u16_4096x3072x2_traffic.md"
 
 
 def measured_traffic(kernel):
@@ -86,7 +86,8 @@ def batch_stacks(enbl, scenes, torch, dev):
 
 def _kernels(rep, runs=1):
     return {k: {"ms": round(v["ms"] / runs, 3), "launches": v["launches"] // runs,
-                "GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}
+                "GB_per_s": round(v["own_bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
+                "credited_GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}
 
 
 def batch_compare(args, enbl, scenes, torch, steps, warmup):
@@ -270,12 +271,53 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
             "params_initial": prm0.tolist(), "params_final": [float(v) for v in prm.detach().cpu()]}
 
 
-def enfuse_workload(args, enbl, scenes, np, torch):
-    """enfuse timings of the 3 x 1024x768 u8 stack (plain path)."""
+def _rel_err_float(got, want):
+    """Largest |got - want| / |want| over the pixels, with the non-finite
+    pattern required to match exactly and want == 0 requiring got == 0."""
+    import numpy as np
+    g = got.astype(np.float64)
+    c = want.astype(np.float64)
+    fin_c = np.isfinite(c)
+    if not (np.array_equal(np.isnan(g), np.isnan(c)) and np.array_equal(np.isinf(g), np.isinf(c))
+            and np.array_equal(g[np.isinf(c)], c[np.isinf(c)])):
+        return float("inf")
+    g, c = g[fin_c], c[fin_c]
+    zero = c == 0
+    if np.any(g[zero] != 0):
+        return float("inf")
+    nz = ~zero
+    return float(np.max(np.abs(g[nz] - c[nz]) / np.abs(c[nz]))) if nz.any() else 0.0
+
+
+def _time_enfuse(enbl, ctx, itype, imgs, alphas, prm, L, out, oa, steps, warmup):
+    def run():
+        enbl.enfuse(ctx, itype, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
+
+    for _ in range(warmup):
+        run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    run()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    return ms, rep
+
+
+def enfuse_c0(args, enbl, scenes, np, torch, cpu=True):
+    """Plain enfuse of the 3-exposure 1024x768 RGB uint8 stack (EV -2, 0, +2;
+    exposure 1.0, saturation 0.2, contrast 0; 8 levels; Int16 fixed-point
+    pyramids): GPU throughput, bit-exact parity with the oracle on the whole
+    workload, and the oracle timed on the host cores."""
     ctx = enbl.Context(0)
     dev = torch.device("cuda:0")
     prm = enbl.enfuse_params(w_exposure=1.0, w_saturation=0.2)
-    imgs = [torch.from_numpy(i).to(dev) for i in scenes.exposure_stack(1024, 768, 3, 0, bits=8)]
+    host = scenes.exposure_stack(1024, 768, 3, 0, bits=8)
+    imgs = [torch.from_numpy(i).to(dev) for i in host]
     h, w = imgs[0].shape[0], imgs[0].shape[1]
     alphas = [torch.full((h, w), 255, dtype=torch.uint8, device=dev) for _ in imgs]
     out = torch.zeros_like(imgs[0])
@@ -283,25 +325,99 @@ def enfuse_workload(args, enbl, scenes, np, torch):
     L = enbl.auto_levels(w, h)
     mpix = len(imgs) * w * h / 1e6
     torch.cuda.synchronize()
-
-    def run():
-        enbl.enfuse(ctx, enbl.U8, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
-
-    for _ in range(args.warmup):
-        run()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run()
-    ctx.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    ctx.profile(True)
-    run()
-    rep = ctx.profile_report()
-    ctx.profile(False)
+    print("bench: enfuse 3 x 1024x768 uint8", file=sys.stderr, flush=True)
+    ms, rep = _time_enfuse(enbl, ctx, enbl.U8, imgs, alphas, prm, L, out, oa, max(args.steps, 20), args.warmup)
+    got, got_a = out.cpu().numpy(), oa.cpu().numpy()
     ctx.close()
-    return {"workload": args.workload, "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3),
-            "mpix_per_s": round(mpix / (ms * 1e-3), 1), "kernels": _kernels(rep)}
+    res = {"workload": "enfuse, 3 x 1024x768 RGB uint8, EV -2/0/+2, exposure 1.0 + saturation 0.2, "
+                       f"{L} levels, Int16 fixed-point pyramids",
+           "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3), "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+           "kernels": _kernels(rep)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_binding import U8, enfuse_params, load_oracle
+        o = load_oracle()
+        oprm = enfuse_params(w_exposure=1.0, w_saturation=0.2)
+        al = [np.full((h, w), 255, np.uint8) for _ in host]
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            want, want_a = o.enfuse(host, al, U8, oprm, L, dither=1, seed=1)
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        res["parity"] = bool(np.array_equal(got, want) and np.array_equal(got_a, want_a))
+        res["cpu_baseline"] = {"value": round(mpix / times[2], 3), "unit": "Mpix/s", "cores": o.lib.orc_num_threads(),
+                               "kind": "port", "sample": f"full workload, median of 5 runs ({times[2]:.3f} s each)"}
+    return res
+
+
+C2_EVS = [k - 3.5 for k in range(8)]
+
+
+def c2_params(mod):
+    return mod.enfuse_params(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3,
+                             contrast_window=5, entropy_window=7)
+
+
+def enfuse_c2(args, enbl, scenes, np, torch, cpu=True, sample=(1500, 1000)):
+    """Plain enfuse of 8 float32 RGB exposures at 6000x4000 (radiance
+    1e-3..1e2, EV -3.5..+3.5 one EV apart, no clipping); weights exposure 1.0,
+    saturation 0.2, contrast 0.5 (5x5 local standard deviation), entropy 0.3
+    (7x7 window, per-channel exact-value histograms as the reference computes
+    them); 10 levels of double pyramids.  Parity (2e-6 relative) and the
+    oracle's CPU time are taken on a bounded sample: the same 8 exposures
+    cropped to `sample` (w, h) from the scene's centre, fused through both
+    paths."""
+    dev = torch.device("cuda:0")
+    W, H = 6000, 4000
+    print("bench: generating the 8 x 6000x4000 float scene", file=sys.stderr, flush=True)
+    imgs, target = scenes.float_scene(torch, W, H, C2_EVS, seed=2)
+    del target
+    alphas = [torch.full((H, W), 255, dtype=torch.uint8, device=dev) for _ in imgs]
+    out = torch.zeros_like(imgs[0])
+    oa = torch.zeros((H, W), dtype=torch.uint8, device=dev)
+    prm = c2_params(enbl)
+    L = enbl.auto_levels(W, H)
+    mpix = len(imgs) * W * H / 1e6
+    torch.cuda.synchronize()
+    ctx = enbl.Context(0)
+    print("bench: enfuse 8 x 6000x4000 float32", file=sys.stderr, flush=True)
+    ms, rep = _time_enfuse(enbl, ctx, enbl.F32, imgs, alphas, prm, L, out, oa, args.c2_steps, 1)
+    res = {"workload": "enfuse, 8 x 6000x4000 RGB float32, EV -3.5..+3.5, exposure 1.0 + saturation 0.2 + "
+                       f"contrast 0.5 (5x5) + entropy 0.3 (7x7), {L} levels, double pyramids",
+           "input_mpix": round(mpix, 1), "ms_per_step": round(ms, 3), "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+           "steps": args.c2_steps, "kernels": _kernels(rep)}
+    sw, sh = sample
+    x0, y0 = (W - sw) // 2, (H - sh) // 2
+    crop = [i[y0:y0 + sh, x0:x0 + sw].contiguous() for i in imgs]
+    calph = [a[:sh, :sw].contiguous() for a in alphas]
+    cout = torch.zeros_like(crop[0])
+    coa = torch.zeros((sh, sw), dtype=torch.uint8, device=dev)
+    Ls = enbl.auto_levels(sw, sh)
+    enbl.enfuse(ctx, enbl.F32, crop, calph, prm, Ls, cout, coa, dither=enbl.DITHER_DET, seed=1)
+    ctx.sync()
+    got = cout.cpu().numpy()
+    host = [c.cpu().numpy() for c in crop]
+    ctx.close()
+    del imgs, alphas, out, oa, crop, calph
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_binding import F32, load_oracle
+        import oracle_binding
+        o = load_oracle()
+        al = [np.full((sh, sw), 255, np.uint8) for _ in host]
+        print(f"bench: oracle enfuse on the 8 x {sw}x{sh} sample", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        want, _ = o.enfuse(host, al, F32, c2_params(oracle_binding), Ls, dither=1, seed=1)
+        cpu_s = time.perf_counter() - t0
+        err = _rel_err_float(got, want)
+        res["parity_rel_err_max"] = err
+        res["parity"] = bool(err <= 2e-6)
+        res["parity_sample"] = f"8 x {sw}x{sh} centre crop, {Ls} levels; non-finite pattern must match"
+        res["cpu_baseline"] = {"value": round(8 * sw * sh / 1e6 / cpu_s, 3), "unit": "Mpix/s",
+                               "cores": o.lib.orc_num_threads(), "kind": "port",
+                               "sample": f"8 x {sw}x{sh} centre crop of the same exposures, one run ({cpu_s:.1f} s)"}
+    return res
 
 
 def main():
@@ -312,8 +428,10 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="enblend",
-                    choices=["enblend", "enfuse-c0", "batch-c3", "adjoint-c4", "copy-calibration"],
-                    help="enblend: the headline line (default); enfuse-c0 / batch-c3: enfuse timings")
+                    choices=["enblend", "enfuse-c0", "enfuse-c2", "batch-c3", "adjoint-c4", "copy-calibration"],
+                    help="enblend: the headline line (default); the others print one part of it alone")
+    ap.add_argument("--c2-steps", type=int, default=3, help="timed steps of the 8 x 6000x4000 float enfuse")
+    ap.add_argument("--no-enfuse", action="store_true", help="leave the enfuse workloads out of the line")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-vs-plain comparison out of the line")
     ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint check and tuning run out of the line")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -331,7 +449,10 @@ def main():
     enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
     scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
     if args.workload == "enfuse-c0":
-        print(json.dumps(enfuse_workload(args, enbl, scenes, np, torch)), flush=True)
+        print(json.dumps(enfuse_c0(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.workload == "enfuse-c2":
+        print(json.dumps(enfuse_c2(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "batch-c3":
         print(json.dumps(batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)), flush=True)
@@ -428,11 +549,15 @@ def main():
     total_ms = sum(v["ms"] for v in rep.values())
     dom_name, dom = max(rep.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dom["ms"] / dom["launches"]
-    avg_bytes = dom["bytes"] / dom["launches"]
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    all_gbs = sum(v["bytes"] for v in rep.values()) / (total_ms * 1e-3) / 1e9
+    avg_own = dom["own_bytes"] / dom["launches"]
+    avg_credit = dom["bytes"] / dom["launches"]
+    achieved = avg_own / (avg_ms * 1e-3) / 1e9
+    credited = avg_credit / (avg_ms * 1e-3) / 1e9
+    all_gbs = sum(v["own_bytes"] for v in rep.values()) / (total_ms * 1e-3) / 1e9
     kernels = {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
-                   "GB_per_s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in sorted(rep.items())}
+                   "GB_per_s": round(v["own_bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
+                   "credited_GB_per_s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
+               for k, v in sorted(rep.items())}
 
     traffic = measured_traffic(dom_name)
     result = {
@@ -453,7 +578,13 @@ def main():
                    "input_mpix_per_step": mpix, "levels": levels},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "achieved_basis": "credited algorithmic bytes per launch (the unfused ops the kernel replaces)",
+                     "achieved_basis": "the fused kernel's own algorithmic bytes per launch (each input read once, "
+                                       "each output written once) over its mean HIP-event launch duration",
+                     "algorithmic_bytes_per_launch": round(avg_own), "launch_ms": round(avg_ms, 5),
+                     "credited_bytes_per_launch": round(avg_credit),
+                     "credited_GB_per_s": round(credited, 1), "credited_frac": round(credited / HBM_PEAK_GBS, 4),
+                     "credited_basis": "bytes of the reference's unfused operations the kernel performs (reduce, "
+                                       "expand-subtract, level-0 conversion), each at reference storage types",
                      "traffic_source": TRAFFIC_FILE if traffic else None,
                      "dram_GB_per_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "all_kernels_GB_per_s": round(all_gbs, 1), "kernel_ms_per_step": round(total_ms / nprof, 4)},
@@ -483,6 +614,9 @@ def main():
     del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
     if not args.no_batch:
         result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
+    if not args.no_enfuse:
+        result["enfuse_c0"] = enfuse_c0(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
+        result["enfuse_c2"] = enfuse_c2(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_c4(args, enbl, scenes, torch)
     print(json.dumps(result), flush=True)

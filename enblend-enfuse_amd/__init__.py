@@ -254,8 +254,8 @@ class Context:
         check(lib().enbl_profile_report(self.h, buf, len(buf)))
         out = {}
         for line in buf.value.decode().splitlines():
-            name, ms, n, nbytes = line.split()
-            out[name] = {"ms": float(ms), "launches": int(n), "bytes": float(nbytes)}
+            name, ms, n, nbytes, own = line.split()
+            out[name] = {"ms": float(ms), "launches": int(n), "bytes": float(nbytes), "own_bytes": float(own)}
         return out
 
 

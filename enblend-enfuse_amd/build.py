@@ -1,6 +1,7 @@
 """Build the gfx950 shared library of the enblend/enfuse pyramid path.
 
-Compiles every HIP source under csrc/ with hipcc into
+Compiles every HIP source under csrc/ with hipcc (one object per source,
+in parallel, under _build/) and links
 enblend-enfuse_amd/libenblend_amd.so (in-tree, so it travels with the source
 tree).  Flags:
   * --offload-arch=gfx950 only (MI355X);
@@ -35,9 +36,28 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def build(verbose=False):
-    """Compile the library; raises CalledProcessError on failure."""
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", LIB + ".tmp"] + sources()
+def build(verbose=False, jobs=None):
+    """Compile every source to an object in parallel, then link the library;
+    raises CalledProcessError on failure."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        cmd = [HIPCC] + compile_flags + inc + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(compile_one, sources()))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

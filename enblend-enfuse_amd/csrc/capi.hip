@@ -49,11 +49,12 @@ int launch_status(const char* what)
     return hip_check(hipGetLastError(), what);
 }
 
-Timed::Timed(enbl_context* c, const char* name, double bytes) : ctx(c), idx((size_t)-1) {
+Timed::Timed(enbl_context* c, const char* name, double bytes, double own) : ctx(c), idx((size_t)-1) {
         if (!ctx->profile) return;
         enbl_context::Timed t;
         t.name = name;
         t.bytes = bytes;
+        t.own = own < 0.0 ? bytes : own;
         if (hipEventCreate(&t.start) != hipSuccess) return;
         if (hipEventCreate(&t.stop) != hipSuccess) { (void)hipEventDestroy(t.start); return; }
         (void)hipEventRecord(t.start, ctx->stream);
@@ -268,11 +269,16 @@ int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* i
         a.rows = 0;
         const double esC = (double)es * p->channels;
         char name[64];
-        snprintf(name, sizeof name, "laplacian_level_%s_c%d", pname(p->ptype), p->channels);
+        snprintf(name, sizeof name, "laplacian_level%s_%s_c%d", l == 0 ? "0" : "", pname(p->ptype), p->channels);
         // credited bytes: the reduce and the expand-subtract it replaces
-        Timed t(ctx, name, (esC + 1.0) * ((double)F.w * F.h + (double)Cl.w * Cl.h) +
-                               esC * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h) +
-                               (l == 0 ? (double)F.w * F.h * p->channels * (ptype_size(itype) + (double)es) : 0.0));
+        // (and the level-0 conversion); own bytes: the source level and its
+        // alpha read once, the Laplacian level and the coarse level with its
+        // alpha written once
+        const double nF = (double)F.w * F.h, nC = (double)Cl.w * Cl.h;
+        const double srcC = l == 0 ? (double)p->channels * ptype_size(itype) : esC;
+        Timed t(ctx, name, (esC + 1.0) * (nF + nC) + esC * (nC + 2.0 * nF) +
+                               (l == 0 ? nF * p->channels * (ptype_size(itype) + (double)es) : 0.0),
+                nF * (srcC + 1.0) + nF * esC + nC * (esC + 1.0));
         int rc = launch_laplacian_level(p->ptype, l == 0 ? itype : 0, a, p->channels, ctx->stream);
         if (rc) return fail(rc, "laplacian level: unsupported type");
         rc = launch_status(name);
@@ -630,7 +636,7 @@ int enbl_profile_report(enbl_context* ctx, char* buf, size_t n)
     if (!ctx || !buf || n == 0) return fail(ENBL_E_ARG, "null argument");
     int rc = hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (rc) return rc;
-    struct Acc { double ms = 0.0, bytes = 0.0; int n = 0; };
+    struct Acc { double ms = 0.0, bytes = 0.0, own = 0.0; int n = 0; };
     std::map<std::string, Acc> acc;
     for (auto& t : ctx->timed) {
         float ms = 0.f;
@@ -638,6 +644,7 @@ int enbl_profile_report(enbl_context* ctx, char* buf, size_t n)
             auto& e = acc[t.name];
             e.ms += ms;
             e.bytes += t.bytes;
+            e.own += t.own;
             e.n += 1;
         }
         (void)hipEventDestroy(t.start);
@@ -647,7 +654,8 @@ int enbl_profile_report(enbl_context* ctx, char* buf, size_t n)
     std::string out;
     for (auto& kv : acc) {
         char line[200];
-        snprintf(line, sizeof line, "%s %.6f %d %.0f\n", kv.first.c_str(), kv.second.ms, kv.second.n, kv.second.bytes);
+        snprintf(line, sizeof line, "%s %.6f %d %.0f %.0f\n", kv.first.c_str(), kv.second.ms, kv.second.n,
+                 kv.second.bytes, kv.second.own);
         out += line;
     }
     snprintf(buf, n, "%s", out.c_str());
@@ -872,8 +880,10 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
         char name[64];
         snprintf(name, sizeof name, "reduce_mask0_%s", pname(mt));
         const double esm = ptype_size(mt);
-        // credited bytes: the conversion of level 0 and the reduce it feeds
-        Timed t(ctx, name, (double)w * h * (1.0 + esm) + esm * ((double)w * h + (double)D.w * D.h));
+        // credited bytes: the conversion of level 0 and the reduce it feeds;
+        // own bytes: the mask bytes read once, level 1 written once
+        Timed t(ctx, name, (double)w * h * (1.0 + esm) + esm * ((double)w * h + (double)D.w * D.h),
+                (double)w * h + esm * (double)D.w * D.h);
         rc = launch_reduce_image(ENBL_U8, mt, a, ctx->stream);
         if (rc) return fail(rc, "mask reduce: unsupported type");
         rc = launch_status(name);
@@ -923,10 +933,16 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
         else { a.mask = level_ptr(mgp, l); a.mpitch = mgp->lv[l].pitch; a.mfrac = 0; }
         char name[64];
         snprintf(name, sizeof name, "collapse_level%s_%s_c%d", l == 0 ? "0" : "", pname(pt), C);
-        // credited bytes: blend + expand-add (+ masked conversion and alpha union at level 0)
-        double bytes = (double)L.w * L.h * (esM + 3.0 * C * esP) + C * esP * ((double)Cl.w * Cl.h + 2.0 * L.w * L.h);
-        if (l == 0) bytes += (double)L.w * L.h * (1.0 + C * (esP + ptype_size(itype))) + 3.0 * w * h;
-        Timed t(ctx, name, bytes);
+        // credited bytes: blend + expand-add (+ masked conversion and alpha
+        // union at level 0); own bytes: the coarse level, both Laplacian
+        // levels and the mask read once, the result written once (level 0:
+        // the two alphas read, the image and the union alpha written)
+        const double nL = (double)L.w * L.h, nC = (double)Cl.w * Cl.h;
+        double bytes = nL * (esM + 3.0 * C * esP) + C * esP * (nC + 2.0 * nL);
+        if (l == 0) bytes += nL * (1.0 + C * (esP + ptype_size(itype))) + 3.0 * w * h;
+        const double own = C * esP * (nC + 2.0 * nL) +
+                           (l == 0 ? nL * (1.0 + 2.0 + C * ptype_size(itype) + 1.0) : nL * (esM + C * esP));
+        Timed t(ctx, name, bytes, own);
         rc = launch_collapse_level(pt, mt, l == 0 ? itype : 0, l == 0, a, C, ctx->stream);
         if (rc) return fail(rc, "collapse level: unsupported types");
         rc = launch_status(name);

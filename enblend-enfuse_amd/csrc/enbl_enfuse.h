@@ -17,8 +17,11 @@ struct WeightArgs {
 };
 
 int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s);
-int launch_stddev(int itype, const void* gray, long gp, const uint8_t* alpha, long ap, int w, int h, int n,
-                  void* out, long op, hipStream_t s);
+// localStdDevIf of K gray planes (plane m at gray + m * gstride elements,
+// result plane m at out + m * ostride), each under its own alpha.
+int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const uint8_t* const* alphas,
+                  const long* apitches, int K, int w, int h, int n, void* out, long op, size_t ostride,
+                  hipStream_t s);
 int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h, int n,
                    const double* tab, void* out, long op, hipStream_t s);
 int launch_weights(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h,

@@ -95,11 +95,12 @@ int hip_check(hipError_t e, const char* what);
 int launch_status(const char* what);
 
 // Brackets one launch with HIP events when the context profiles; `bytes` is
-// the launch's algorithmic byte count.
+// the launch's credited algorithmic byte count (the unfused operations it
+// performs), `own` the kernel's own minimum traffic (negative: same as bytes).
 struct Timed {
     enbl_context* ctx;
     size_t idx;
-    Timed(enbl_context* c, const char* name, double bytes);
+    Timed(enbl_context* c, const char* name, double bytes, double own = -1.0);
     ~Timed();
 };
 
@@ -129,7 +130,7 @@ struct enbl_context {
     hipStream_t own_stream;
     hipStream_t stream;
     bool profile;
-    struct Timed { std::string name; double bytes; hipEvent_t start, stop; };
+    struct Timed { std::string name; double bytes, own; hipEvent_t start, stop; };
     std::vector<Timed> timed;
     struct Cached { int ptype, channels, w, h, levels; enbl_pyramid* p; };
     std::vector<Cached> cache;   // pipeline scratch pyramids

@@ -14,6 +14,7 @@
 //   host-tabulated p*log(p) terms in ascending key order, which is the order
 //   of the reference's std::map histograms.
 #include <float.h>
+#include <string.h>
 
 #include "enbl_device.h"
 #include "enbl_enfuse.h"
@@ -160,6 +161,146 @@ __global__ __launch_bounds__(64) void stddev_float_rows_kernel(const float* __re
     }
 }
 
+// localStdDevIf for float gray images over LDS column chunks, several
+// images per launch (blockIdx.y).  A block of four waves owns 64 consecutive
+// window rows and walks the image in chunks of 64 columns:
+//  * load: the chunk's gray and alpha samples, with the window's rows above
+//    and below, go to LDS (the next chunk is already in flight in registers
+//    while the current one is processed);
+//  * column sums (all waves, lane = column): each window row's column sum of
+//    values and of squares, top to bottom, and its sample count;
+//  * running sums (one wave, lane = row): the float running sums advance
+//    column by column exactly in the reference's order, sum += s - (the
+//    column leaving the window), the leaving column being zero while the
+//    window fills, which leaves the sums unchanged bit for bit; the leaving
+//    columns sit in a register ring of N entries; the sums after each column
+//    overwrite that column's sums in LDS;
+//  * results (all waves, lane = column): the standard deviation of every
+//    window whose centre lies under alpha, stored as coalesced row segments.
+// This is the only serial part of the path: one row is a chain of one float
+// addition per column, so the other phases are spread over all four waves.
+constexpr int kSdCW = 64;
+constexpr int kSdMaxJobs = 8;
+
+struct StddevJobs {
+    const float* gray[kSdMaxJobs];
+    const uint8_t* alpha[kSdMaxJobs];
+    long apitch[kSdMaxJobs];
+    float* out[kSdMaxJobs];
+    long gpitch, opitch;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, int w, int h)
+{
+    constexpr int B = N / 2, TH = 64 + 2 * B, RPW = (TH + 3) / 4;
+    __shared__ float sg[TH][kSdCW + 1];
+    __shared__ uint8_t sa[TH][kSdCW + 4];
+    __shared__ float css[64][kSdCW + 1], csq[64][kSdCW + 1];
+    __shared__ uint8_t csk[64][kSdCW + 4];
+    __shared__ uint8_t prev_a[2][64][4];
+    const int job = blockIdx.y;
+    const float* __restrict__ gray = J.gray[job];
+    const uint8_t* __restrict__ alpha = J.alpha[job];
+    float* __restrict__ out = J.out[job];
+    const long gp = J.gpitch, ap = J.apitch[job], op = J.opitch;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int row0 = blockIdx.x * 64;
+    float pg[RPW];
+    uint8_t pa[RPW];
+    auto fetch = [&](int xb) {
+        const int gx = xb + lane;
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            const int r = wv + 4 * u, gy = row0 + r;
+            const bool in = r < TH && gy < h && gx < w;
+            pg[u] = in ? gray[(long)gy * gp + gx] : 0.f;
+            pa[u] = in ? alpha[(long)gy * ap + gx] : (uint8_t)0;
+        }
+    };
+    float cs[N], cq[N];
+    int cn[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) { cs[i] = 0.f; cq[i] = 0.f; cn[i] = 0; }
+    float sum = 0.f, sq = 0.f;
+    int cnt = 0;
+    prev_a[0][lane][wv] = 0;
+    fetch(0);
+    int par = 0;
+    for (int xb = 0; xb < w; xb += kSdCW, par ^= 1) {
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            const int r = wv + 4 * u;
+            if (r < TH) {
+                sg[r][lane] = pg[u];
+                sa[r][lane] = pa[u];
+            }
+        }
+        __syncthreads();
+        if (xb + kSdCW < w) fetch(xb + kSdCW);
+        const int nx = min(kSdCW, w - xb);
+        // column sums of window rows wv*16 .. wv*16+15, column `lane`
+        for (int rr = 0; rr < 16; ++rr) {
+            const int r = wv * 16 + rr;
+            float s = 0.f, q = 0.f;
+            int k = 0;
+#pragma unroll
+            for (int dy = 0; dy < N; ++dy) {
+                if (sa[r + dy][lane]) {
+                    const float v = sg[r + dy][lane];
+                    s += v;
+                    q += v * v;
+                    ++k;
+                }
+            }
+            css[r][lane] = s;
+            csq[r][lane] = q;
+            csk[r][lane] = (uint8_t)k;
+        }
+        __syncthreads();
+        if (wv == 0) {
+            for (int i = 0; i < nx; ++i) {
+                const float s = css[lane][i], q = csq[lane][i];
+                const int k = csk[lane][i];
+                sum += s - cs[0];
+                sq += q - cq[0];
+                cnt += k - cn[0];
+#pragma unroll
+                for (int j = 0; j + 1 < N; ++j) { cs[j] = cs[j + 1]; cq[j] = cq[j + 1]; cn[j] = cn[j + 1]; }
+                cs[N - 1] = s;
+                cq[N - 1] = q;
+                cn[N - 1] = k;
+                css[lane][i] = sum;
+                csq[lane][i] = sq;
+                csk[lane][i] = (uint8_t)cnt;
+            }
+        }
+        __syncthreads();
+        // Column i = xb + lane completes the window centred on xb + lane - B;
+        // for lane < B that centre lies in the previous chunk, whose last B
+        // columns of centre alpha were kept in prev_a.
+        for (int rr = 0; rr < 16; ++rr) {
+            const int r = wv * 16 + rr;
+            const int y = row0 + r + B, x = xb + lane - B;
+            if (lane < nx && y < h - B && x >= B && x < w - B) {
+                const uint8_t ca = lane >= B ? sa[r + B][lane - B] : prev_a[par][r][lane];
+                if (ca) {
+                    const int c = csk[r][lane];
+                    const float sm = css[r][lane], sqv = csq[r][lane];
+                    float res = 0.f;
+                    if (c > 1) res = (float)sqrt((double)((sqv - sm * sm / (float)c) / (float)(c - 1)));
+                    out[(long)y * op + x] = res;
+                }
+            }
+        }
+        for (int rr = 0; rr < 16; ++rr) {
+            const int r = wv * 16 + rr;
+            if (lane < B) prev_a[par ^ 1][r][lane] = sa[r + B][kSdCW - B + lane];
+        }
+        __syncthreads();
+    }
+}
+
 // localEntropyIf + EntropyFunctor's channel minimum (src/enfuse.h:222-566,
 // :832-861): minimum over channels of fromRealPromote(H_c * max(T)).
 template <class T, int NMAX2>
@@ -206,6 +347,191 @@ __global__ __launch_bounds__(kTile) void entropy_kernel(const T* __restrict__ im
                 }
                 if (bins > 1) e = -acc / plog[bins];
             }
+            const T ec = from_real_img<T>(e * Img<T>::maxv());
+            emin = c == 0 ? ec : (ec < emin ? ec : emin);
+        }
+        res = emin;
+    }
+    out[(long)y * op + x] = res;
+}
+
+// ---- local entropy over LDS tiles with a sorting network -----------------
+//
+// A block of 256 threads owns a 64 x 4 tile of output pixels.  The tile's
+// samples with an n/2 halo (channel-planar) and their alpha go to LDS once;
+// each thread then reads its n x n window from LDS.  A window whose samples
+// all lie under alpha is sorted by a comparator network held in registers
+// (Batcher's odd-even merge sort for the next power of two, pruned to the
+// window size: a comparator with an index past the window only ever meets
+// the +infinity padding and is a no-op), and the bins of equal keys are
+// walked in ascending key order, accumulating the full-window p*log(p) table
+// in the order the reference's std::map iterates them.  Windows that are cut
+// by alpha take the insertion sort of the general kernel on the LDS samples.
+// Equal keys are those that compare neither less nor greater (the map's
+// equivalence), so -0 and +0 share a bin.  NaN samples break the map's strict
+// weak ordering in the reference, so their bins are not defined there.
+
+struct CmpNet {
+    int n;
+    unsigned char a[1024], b[1024];
+};
+
+constexpr CmpNet make_net(int N)
+{
+    CmpNet c{};
+    int P = 1;
+    while (P < N) P <<= 1;
+    int m = 0;
+    for (int p = 1; p < P; p += p)
+        for (int k = p; k > 0; k /= 2)
+            for (int j = k % p; j + k < P; j += k + k)
+                for (int i = 0; i < k && i < P - j - k; ++i)
+                    if ((i + j) / (p + p) == (i + j + k) / (p + p) && i + j + k < N) {
+                        c.a[m] = (unsigned char)(i + j);
+                        c.b[m] = (unsigned char)(i + j + k);
+                        ++m;
+                    }
+    c.n = m;
+    return c;
+}
+
+template <int NN> struct SortNet { static constexpr CmpNet net = make_net(NN); };
+
+template <class T> __device__ __forceinline__ void cas(T& x, T& y) { const T lo = min(x, y), hi = max(x, y); x = lo; y = hi; }
+template <> __device__ __forceinline__ void cas<float>(float& x, float& y)
+{
+    const float lo = __builtin_fminf(x, y), hi = __builtin_fmaxf(x, y);
+    x = lo;
+    y = hi;
+}
+
+// Integer samples are sorted as int (u8/u16 values), float samples as float.
+template <class T> struct SortKey { typedef int type; };
+template <> struct SortKey<float> { typedef float type; };
+
+constexpr int kEntTW = 64, kEntTH = 4;
+
+template <class T, int N>
+__global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* __restrict__ img, long ip,
+                                                                       const uint8_t* __restrict__ alpha, long ap,
+                                                                       int w, int h, const double* __restrict__ tab,
+                                                                       T* __restrict__ out, long op)
+{
+    constexpr int B = N / 2, NN = N * N;
+    constexpr int LW = kEntTW + 2 * B, LH = kEntTH + 2 * B;
+    typedef typename SortKey<T>::type K;
+    __shared__ K sv[3][LH][LW];
+    __shared__ uint8_t sa[LH][LW + 4];
+    __shared__ double spent[NN + 1], splog[NN + 1];
+    const int x0 = blockIdx.x * kEntTW, y0 = blockIdx.y * kEntTH;
+    const int tid = threadIdx.x;
+    // Halo tile: samples outside the image count as alpha 0.  All loads of
+    // a thread are issued before the first LDS store.
+    constexpr int NT = kEntTW * kEntTH;
+    constexpr int NV = (LH * 3 * LW + NT - 1) / NT, NA = (LH * LW + NT - 1) / NT;
+    K lv[NV];
+    uint8_t la[NA];
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        const int i = tid + u * NT;
+        const int ly = i / (3 * LW), e = i - ly * (3 * LW);
+        const int lx = e / 3, c = e - 3 * lx;
+        const int gy = y0 - B + ly, gx = x0 - B + lx;
+        const bool in = i < LH * 3 * LW && gy >= 0 && gy < h && gx >= 0 && gx < w;
+        lv[u] = in ? (K)img[(long)gy * ip + 3L * gx + c] : (K)0;
+    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+        const int i = tid + u * NT;
+        const int ly = i / LW, lx = i - ly * LW;
+        const int gy = y0 - B + ly, gx = x0 - B + lx;
+        const bool in = i < LH * LW && gy >= 0 && gy < h && gx >= 0 && gx < w;
+        la[u] = in ? alpha[(long)gy * ap + gx] : (uint8_t)0;
+    }
+    int unset = 0;
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        const int i = tid + u * NT;
+        const int ly = i / (3 * LW), e = i - ly * (3 * LW);
+        const int lx = e / 3, c = e - 3 * lx;
+        if (i < LH * 3 * LW) sv[c][ly][lx] = lv[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+        const int i = tid + u * NT;
+        const int ly = i / LW, lx = i - ly * LW;
+        if (i < LH * LW) {
+            sa[ly][lx] = la[u];
+            unset |= la[u] == 0;
+        }
+    }
+    for (int i = tid; i <= NN; i += kEntTW * kEntTH) {
+        spent[i] = tab[i];
+        splog[i] = tab[NN + 1 + i];
+    }
+    const bool tile_full = __syncthreads_or(unset) == 0;
+    const int lx = tid % kEntTW, ly = tid / kEntTW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    T res = T(0);
+    if (x >= B && x < w - B && y >= B && y < h - B && sa[ly + B][lx + B]) {
+        bool full = tile_full;
+        if (!full) {
+            full = true;
+#pragma unroll
+            for (int dy = 0; dy < N; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < N; ++dx) full = full && sa[ly + dy][lx + dx] != 0;
+        }
+        T emin = T(0);
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            int bins = 0;
+            if (full) {
+                K v[NN];
+#pragma unroll
+                for (int dy = 0; dy < N; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][ly + dy][lx + dx];
+                constexpr CmpNet net = SortNet<NN>::net;
+#pragma unroll
+                for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
+                int run = 1;
+                bins = 1;
+#pragma unroll
+                for (int k = 1; k < NN; ++k) {
+                    if (v[k - 1] < v[k]) {
+                        acc += spent[run];
+                        ++bins;
+                        run = 1;
+                    } else {
+                        ++run;
+                    }
+                }
+                acc += spent[run];
+            } else {
+                // Cut window: insertion sort of the samples under alpha; the
+                // general p*log(p) table for `cnt` samples.
+                const double* pgen = tab + 2 * (NN + 1);
+                K v[NN];
+                int cnt = 0;
+                for (int dy = 0; dy < N; ++dy)
+                    for (int dx = 0; dx < N; ++dx)
+                        if (sa[ly + dy][lx + dx]) {
+                            const K sm = sv[c][ly + dy][lx + dx];
+                            int k = cnt++;
+                            while (k > 0 && sm < v[k - 1]) { v[k] = v[k - 1]; --k; }
+                            v[k] = sm;
+                        }
+                int run = 1;
+                for (int k = 1; k <= cnt; ++k) {
+                    if (k < cnt && !(v[k - 1] < v[k])) { ++run; continue; }
+                    acc += pgen[cnt * (NN + 1) + run];
+                    ++bins;
+                    run = 1;
+                }
+            }
+            const double e = bins > 1 ? -acc / splog[bins] : 0.0;
             const T ec = from_real_img<T>(e * Img<T>::maxv());
             emin = c == 0 ? ec : (ec < emin ? ec : emin);
         }
@@ -319,19 +645,43 @@ int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, l
     return 0;
 }
 
-int launch_stddev(int itype, const void* gray, long gp, const uint8_t* alpha, long ap, int w, int h, int n,
-                  void* out, long op, hipStream_t s)
+int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const uint8_t* const* alphas,
+                  const long* apitches, int K, int w, int h, int n, void* out, long op, size_t ostride,
+                  hipStream_t s)
 {
     if (itype == ENBL_F32) {
         if (n > 31) return ENBL_E_UNSUPPORTED;
         const int rows = h - 2 * (n / 2);
         if (rows <= 0) return 0;
-        hipLaunchKernelGGL((stddev_float_rows_kernel<31>), dim3((rows + 63) / 64), dim3(64), 0, s,
-                           (const float*)gray, gp, alpha, ap, w, h, n, (float*)out, op);
+        if (n == 3 || n == 5 || n == 7) {
+            for (int m0 = 0; m0 < K; m0 += kSdMaxJobs) {
+                const int nj = min(kSdMaxJobs, K - m0);
+                StddevJobs J;
+                memset(&J, 0, sizeof J);
+                J.gpitch = gp;
+                J.opitch = op;
+                for (int j = 0; j < nj; ++j) {
+                    J.gray[j] = (const float*)gray + (m0 + j) * gstride;
+                    J.alpha[j] = alphas[m0 + j];
+                    J.apitch[j] = apitches[m0 + j];
+                    J.out[j] = (float*)out + (m0 + j) * ostride;
+                }
+                const dim3 grid((rows + 63) / 64, nj), block(256);
+                if (n == 3) hipLaunchKernelGGL((stddev_float_tile_kernel<3>), grid, block, 0, s, J, w, h);
+                else if (n == 5) hipLaunchKernelGGL((stddev_float_tile_kernel<5>), grid, block, 0, s, J, w, h);
+                else hipLaunchKernelGGL((stddev_float_tile_kernel<7>), grid, block, 0, s, J, w, h);
+            }
+            return 0;
+        }
+        for (int m = 0; m < K; ++m)
+            hipLaunchKernelGGL((stddev_float_rows_kernel<31>), dim3((rows + 63) / 64), dim3(64), 0, s,
+                               (const float*)gray + m * gstride, gp, alphas[m], apitches[m], w, h, n,
+                               (float*)out + m * ostride, op);
         return 0;
     }
-    hipLaunchKernelGGL(stddev_int_kernel, grid_for(w, h), dim3(kTile), 0, s, (const int32_t*)gray, gp, alpha, ap, w, h, n,
-                       (int32_t*)out, op);
+    for (int m = 0; m < K; ++m)
+        hipLaunchKernelGGL(stddev_int_kernel, grid_for(w, h), dim3(kTile), 0, s, (const int32_t*)gray + m * gstride, gp,
+                           alphas[m], apitches[m], w, h, n, (int32_t*)out + m * ostride, op);
     return 0;
 }
 
@@ -339,6 +689,22 @@ int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, lo
                    const double* tab, void* out, long op, hipStream_t s)
 {
     if (n * n > 121) return ENBL_E_UNSUPPORTED;
+    if (n == 3 || n == 5 || n == 7) {
+        const dim3 grid((w + kEntTW - 1) / kEntTW, (h + kEntTH - 1) / kEntTH), block(kEntTW * kEntTH);
+#define ENBL_ENT_TILE(T, N) hipLaunchKernelGGL((entropy_tile_kernel<T, N>), grid, block, 0, s, (const T*)img, ip, alpha, ap, w, h, tab, (T*)out, op)
+#define ENBL_ENT_N(T)                   \
+    do {                                \
+        if (n == 3) ENBL_ENT_TILE(T, 3); \
+        else if (n == 5) ENBL_ENT_TILE(T, 5); \
+        else ENBL_ENT_TILE(T, 7);       \
+    } while (0)
+        if (itype == ENBL_U8) ENBL_ENT_N(uint8_t);
+        else if (itype == ENBL_U16) ENBL_ENT_N(uint16_t);
+        else ENBL_ENT_N(float);
+#undef ENBL_ENT_N
+#undef ENBL_ENT_TILE
+        return 0;
+    }
     if (n * n <= 49) {
         if (itype == ENBL_U8) hipLaunchKernelGGL((entropy_kernel<uint8_t, 49>), grid_for(w, h), dim3(kTile), 0, s, (const uint8_t*)img, ip, alpha, ap, w, h, n, tab, (uint8_t*)out, op);
         else if (itype == ENBL_U16) hipLaunchKernelGGL((entropy_kernel<uint16_t, 49>), grid_for(w, h), dim3(kTile), 0, s, (const uint16_t*)img, ip, alpha, ap, w, h, n, tab, (uint16_t*)out, op);

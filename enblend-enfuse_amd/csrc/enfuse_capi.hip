@@ -103,16 +103,18 @@ struct WeightScratch {
     double* tab = nullptr;
     void* ent = nullptr;
     long pitch = 0;   // elements, shared by gray/grad/ent
+    size_t plane = 0; // elements between the images' gray/grad planes
 };
 
-// Host tables and device scratch for one image size (ExposureFunctor table,
-// gray/gradient images, entropy tables).
-int prepare_weights(enbl_context* ctx, int itype, int w, int h, const enbl_enfuse_params* p, const WeightArgs& a,
-                    WeightScratch* ws)
+// Host tables and device scratch for K images of one size (ExposureFunctor
+// table, K gray and K gradient planes, entropy tables).
+int prepare_weights(enbl_context* ctx, int itype, int w, int h, int K, const enbl_enfuse_params* p,
+                    const WeightArgs& a, WeightScratch* ws)
 {
     const long pitch = (long)round_up((size_t)w, 64);
     ws->pitch = pitch;
     const size_t npx = (size_t)pitch * h;
+    ws->plane = npx;
     int rc;
     if (p->w_exposure > 0.0 && itype != ENBL_F32) {
         std::vector<float> lut;
@@ -128,8 +130,8 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, const enbl_enfus
         ws->lut = (float*)d;
     }
     if (p->w_contrast > 0.0) {
-        rc = scratch(ctx, 1, npx * 4, &ws->gray);
-        if (!rc) rc = scratch(ctx, 2, npx * 4, &ws->grad);
+        rc = scratch(ctx, 1, npx * 4 * (size_t)K, &ws->gray);
+        if (!rc) rc = scratch(ctx, 2, npx * 4 * (size_t)K, &ws->grad);
         if (rc) return rc;
     }
     if (p->w_entropy > 0.0) {
@@ -161,31 +163,41 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, const enbl_enfus
     return ENBL_OK;
 }
 
-// Weights of one image into mask; norm += mask; union alpha updated.
-int image_weights(enbl_context* ctx, int itype, int w, int h, const void* img, long ip, const uint8_t* alpha, long ap,
-                  const enbl_enfuse_params* p, const WeightArgs& a, const WeightScratch& ws,
+// Gray images and local standard deviations (src/enfuse.h:1099-1120) of
+// all K images: gradient plane m of ws.grad belongs to image m.
+int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                 const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, const WeightScratch& ws)
+{
+    const hipStream_t s = ctx->stream;
+    const size_t es = 4;   // int32 gray of integer images, float gray of float images
+    int rc;
+    for (int m = 0; m < K; ++m) {
+        Timed t(ctx, "gray", (double)w * h * (3.0 * ptype_size(itype) + 4.0));
+        rc = launch_gray(itype, imgs[m], ips[m], w, h, (char*)ws.gray + m * ws.plane * es, ws.pitch, s);
+        if (!rc) rc = launch_status("gray");
+        if (rc) return rc;
+    }
+    if (itype == ENBL_F32) {
+        rc = launch_fill(ws.grad, ws.plane * es * (size_t)K, s);
+        if (!rc) rc = launch_status("fill");
+        if (rc) return rc;
+    }
+    Timed t(ctx, "stddev", (double)K * w * h * 9.0);
+    rc = launch_stddev(itype, ws.gray, ws.pitch, ws.plane, alphas, aps, K, w, h, p->contrast_window, ws.grad,
+                       ws.pitch, ws.plane, s);
+    if (rc) return fail(rc, "localStdDevIf: unsupported window");
+    return launch_status("stddev");
+}
+
+// Weights of image m into mask (its gradient plane made by contrast_all);
+// norm += mask; union alpha updated.
+int image_weights(enbl_context* ctx, int itype, int w, int h, int m, const void* img, long ip, const uint8_t* alpha,
+                  long ap, const enbl_enfuse_params* p, const WeightArgs& a, const WeightScratch& ws,
                   float* mask, long mp, float* norm, long np_, uint8_t* ua, long up)
 {
     const hipStream_t s = ctx->stream;
     int rc;
-    if (p->w_contrast > 0.0) {
-        {
-            Timed t(ctx, "gray", (double)w * h * (3.0 * ptype_size(itype) + 4.0));
-            rc = launch_gray(itype, img, ip, w, h, ws.gray, ws.pitch, s);
-            if (!rc) rc = launch_status("gray");
-            if (rc) return rc;
-        }
-        if (itype == ENBL_F32) {
-            rc = launch_fill(ws.grad, (size_t)ws.pitch * h * 4, s);
-            if (!rc) rc = launch_status("fill");
-            if (rc) return rc;
-        }
-        Timed t(ctx, "stddev", (double)w * h * 9.0);
-        rc = launch_stddev(itype, ws.gray, ws.pitch, alpha, ap, w, h, p->contrast_window, ws.grad, ws.pitch, s);
-        if (rc) return fail(rc, "localStdDevIf: unsupported window");
-        rc = launch_status("stddev");
-        if (rc) return rc;
-    }
+    const void* grad = ws.grad ? (const void*)((const char*)ws.grad + (size_t)m * ws.plane * 4) : nullptr;
     if (p->w_entropy > 0.0) {
         Timed t(ctx, "entropy", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + ptype_size(itype)));
         rc = launch_entropy(itype, img, ip, alpha, ap, w, h, p->entropy_window, ws.tab, ws.ent, ws.pitch, s);
@@ -194,7 +206,7 @@ int image_weights(enbl_context* ctx, int itype, int w, int h, const void* img, l
         if (rc) return rc;
     }
     Timed t(ctx, "weights", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + 4.0 + 8.0 + 1.0));
-    rc = launch_weights(itype, img, ip, alpha, ap, w, h, a, ws.lut, ws.grad, ws.pitch, ws.ent, ws.pitch,
+    rc = launch_weights(itype, img, ip, alpha, ap, w, h, a, ws.lut, grad, ws.pitch, ws.ent, ws.pitch,
                         mask, mp, norm, np_, ua, up, s);
     if (rc) return fail(rc, "enfuse weights: unsupported image type");
     return launch_status("weights");
@@ -212,7 +224,8 @@ int enbl_enfuse_mask(enbl_context* ctx, int itype, int w, int h, const void* img
     if (rc) return rc;
     const WeightArgs a = weight_args(p);
     WeightScratch ws;
-    rc = prepare_weights(ctx, itype, w, h, p, a, &ws);
+    rc = prepare_weights(ctx, itype, w, h, 1, p, a, &ws);
+    if (!rc && p->w_contrast > 0.0) rc = contrast_all(ctx, itype, w, h, 1, &img, &ip, &alpha, &ap, p, ws);
     if (rc) return rc;
     // norm and union alpha are by-products here; park them in scratch.
     void *norm, *ua;
@@ -222,7 +235,7 @@ int enbl_enfuse_mask(enbl_context* ctx, int itype, int w, int h, const void* img
     if (rc) return rc;
     rc = launch_fill(norm, (size_t)pitch * h * 4, ctx->stream);
     if (rc) return rc;
-    return image_weights(ctx, itype, w, h, img, ip, alpha, ap, p, a, ws, mask, mp, (float*)norm, pitch,
+    return image_weights(ctx, itype, w, h, 0, img, ip, alpha, ap, p, a, ws, mask, mp, (float*)norm, pitch,
                          (uint8_t*)ua, pitch);
 }
 
@@ -239,7 +252,7 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     if (levels < 1 || levels > 29) return fail(ENBL_E_PRECONDITION, "filterHalfWidth: levels outside of permissible range");
     const WeightArgs a = weight_args(p);
     WeightScratch ws;
-    rc = prepare_weights(ctx, itype, w, h, p, a, &ws);
+    rc = prepare_weights(ctx, itype, w, h, K, p, a, &ws);
     if (rc) return rc;
     const hipStream_t s = ctx->stream;
     const long pitch = ws.pitch;
@@ -253,9 +266,15 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     if (!rc) rc = launch_fill(ua, plane, s);
     if (rc) return rc;
 
-    // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388).
+    // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388); the
+    // contrast of all images first, so that their local standard deviations
+    // share launches.
+    if (p->w_contrast > 0.0) {
+        rc = contrast_all(ctx, itype, w, h, K, imgs, ips, alphas, aps, p, ws);
+        if (rc) return rc;
+    }
     for (int m = 0; m < K; ++m) {
-        rc = image_weights(ctx, itype, w, h, imgs[m], ips[m], alphas[m], aps[m], p, a, ws,
+        rc = image_weights(ctx, itype, w, h, m, imgs[m], ips[m], alphas[m], aps[m], p, a, ws,
                            (float*)masks + plane * m, pitch, (float*)norm, pitch, (uint8_t*)ua, pitch);
         if (rc) return rc;
     }

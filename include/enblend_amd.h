@@ -74,8 +74,12 @@ int enbl_set_kernel_policy(int policy);
 const char* enbl_last_error(void);
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the
  * context's stream; the report synchronises and writes one line per kernel
- * name: "name total_ms launches algorithmic_bytes" (bytes = operands at the
- * reference's storage types, each read once and each result written once). */
+ * name: "name total_ms launches credited_bytes own_bytes".  credited_bytes
+ * are the algorithmic bytes of the reference's unfused operations the kernel
+ * performs (operands at the reference's storage types, each read once and
+ * each result written once); own_bytes are the fused kernel's own minimum
+ * traffic (each of its inputs read once, each of its outputs written once),
+ * equal to credited_bytes for a kernel that replaces one operation. */
 int enbl_profile_enable(enbl_context* ctx, int on);
 int enbl_profile_report(enbl_context* ctx, char* buf, size_t buflen);
 

@@ -95,3 +95,48 @@ def test_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, case, policy):
             np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
         else:
             np.testing.assert_array_equal(got, want)
+
+
+def _tied_stack(rng, itype, h, w):
+    """One image with many repeated sample values (so entropy windows hold
+    bins of several samples); float images also mix -0.0 and +0.0."""
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.5 + 0.45 * np.sin(xx / 9.0) * np.cos(yy / 6.0)
+    img = base[..., None] * rng.uniform(0.7, 1.3, size=3)[None, None, :]
+    if itype == F32:
+        v = np.round(img * 16.0) / 16.0 - 0.25
+        v[rng.random(v.shape) < 0.05] = -0.0
+        v[rng.random(v.shape) < 0.05] = 0.0
+        return v.astype(np.float32)
+    m = 255 if itype == U8 else 65535
+    v = np.round(img * 24.0) / 24.0 * m
+    return np.clip(v, 0, m).astype(np.uint8 if itype == U8 else np.uint16)
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("windows", [(3, 3), (5, 7), (7, 5)])
+@pytest.mark.parametrize("holes", [False, True], ids=["full-alpha", "holes"])
+def test_contrast_entropy_tiles_bit_exact(enbl, gpu_ctx, oracle, itype, windows, holes):
+    """Contrast and entropy weights over several LDS tiles and column chunks
+    (150 x 203, window rows past one wave), with full windows (the sorting
+    network path) and windows cut by alpha (the insertion-sort path).  With
+    the exposure weight off every term is computed identically, so the float
+    weights are compared bit for bit."""
+    rng = np.random.default_rng(900 + itype + 10 * windows[0] + windows[1] + (1000 if holes else 0))
+    h, w = 150, 203
+    img = _tied_stack(rng, itype, h, w)
+    a = np.full((h, w), 255, np.uint8)
+    if holes:
+        a[rng.random((h, w)) < 0.15] = 0
+        a[40:60, 100:180] = 0
+    kw = dict(w_exposure=0.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3,
+              contrast_window=windows[0], entropy_window=windows[1])
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(img).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"contrast/entropy tiles itype={itype} windows={windows} holes={holes}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
