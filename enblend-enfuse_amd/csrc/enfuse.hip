@@ -496,19 +496,22 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
                 constexpr CmpNet net = SortNet<NN>::net;
 #pragma unroll
                 for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
-                int run = 1;
+                // Bin boundaries where the sorted keys step up; the bin that
+                // ends at a boundary k started at the previous boundary st.
+                // Non-boundaries add spent[0] = 0.0, which leaves the sum
+                // unchanged bit for bit, so the table reads do not wait on one
+                // another.
+                int st = 0;
                 bins = 1;
 #pragma unroll
                 for (int k = 1; k < NN; ++k) {
-                    if (v[k - 1] < v[k]) {
-                        acc += spent[run];
-                        ++bins;
-                        run = 1;
-                    } else {
-                        ++run;
-                    }
+                    const bool b = v[k - 1] < v[k];
+                    const int len = k - st;
+                    st = b ? k : st;
+                    bins += b ? 1 : 0;
+                    acc += spent[b ? len : 0];
                 }
-                acc += spent[run];
+                acc += spent[NN - st];
             } else {
                 // Cut window: insertion sort of the samples under alpha; the
                 // general p*log(p) table for `cnt` samples.
