@@ -512,6 +512,10 @@ def main():
         nft()
     ctx.sync()
     nft_ms = (time.perf_counter() - t0) * 1e3 / nft_runs
+    ctx.profile(True)
+    nft()
+    nft_rep = ctx.profile_report()
+    ctx.profile(False)
     mask = d_mask.cpu().numpy()
 
     def step(bk, bka):
@@ -591,7 +595,8 @@ def main():
         "kernels": kernels,
         "nft": {"workload": f"nearest-feature-transform seam mask of the two alphas, {w}x{h}, Euclidean, "
                             "float Felzenszwalb-Huttenlocher transforms", "ms": round(nft_ms, 3),
-                "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs},
+                "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs,
+                "kernels": _kernels(nft_rep)},
     }
 
     if not args.no_cpu:

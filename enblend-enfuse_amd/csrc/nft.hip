@@ -14,6 +14,7 @@
 // lanes at similar depths share cache lines; the top entry stays in
 // registers.
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <string.h>
 
@@ -135,6 +136,101 @@ __global__ __launch_bounds__(64) void nft_columns_kernel(NftArgs a)
     if (threadIdx.x == 0) atomicAdd(&a.tally[t], count);
 }
 
+// Euclidean column pass for columns of at most 4096 samples.  The column
+// input is binary (0 on features, FLT_MAX elsewhere), and below 2^24 every
+// float the 1-D transform forms from feature samples is exact: q^2, the
+// intersections (q^2 - v^2) / (2 (q - v)) = (q + v) / 2, and the result
+// (q - v)^2; background samples only ever give FLT_MAX + x = FLT_MAX and
+// intersections of magnitude ~FLT_MAX, which never displace a feature.  So
+// the transform equals the squared distance to the nearest feature of the
+// column, or FLT_MAX in a column without features, which is found without a
+// stack: the nearest feature at or above and the one at or below.
+constexpr int kScanMax = 4096, kSeg = 64;
+
+// Columns are cut into segments of kSeg rows, one lane per (column,
+// segment).  The first kernel finds each segment's first and last feature
+// row (and counts the features for the overlap tally); the second carries
+// the nearest feature of the segments above and below into the segment and
+// runs both sweeps over its rows.
+struct SegArgs {
+    int* first;       // [transform][segment][column]: first feature row, or INT_MAX
+    int* last;        // last feature row, or -1
+    int nseg;
+};
+
+__device__ __forceinline__ bool column_feature(const NftArgs& a, int t, int x, int y)
+{
+    const uint8_t* p = t == 0 ? a.white : a.black;
+    const uint8_t* q = t == 0 ? a.black : a.white;   // null: features are the non-zero samples of p
+    const long pp = t == 0 ? a.wpitch : a.bpitch, qp = t == 0 ? a.bpitch : a.wpitch;
+    const uint8_t u = *gmem(p + (long)y * pp + x), v = q ? *gmem(q + (long)y * qp + x) : (uint8_t)0;
+    return v < u;
+}
+
+// Feature bits of rows y0 .. y0 + kSeg - 1 (rows past h read as background).
+__device__ __forceinline__ uint64_t segment_bits(const NftArgs& a, int t, int x, int y0)
+{
+    bool fe[kSeg];
+#pragma unroll
+    for (int i = 0; i < kSeg; ++i) fe[i] = y0 + i < a.h && column_feature(a, t, x, y0 + i);
+    uint64_t b = 0;
+#pragma unroll
+    for (int i = 0; i < kSeg; ++i) b |= fe[i] ? (1ull << i) : 0ull;
+    return b;
+}
+
+__global__ __launch_bounds__(64) void nft_colseg_summary_kernel(NftArgs a, SegArgs g)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int sg = blockIdx.y, t = blockIdx.z;
+    const bool live = x < a.w;
+    const int xc = live ? x : a.w - 1;
+    const uint64_t b = segment_bits(a, t, xc, sg * kSeg);
+    if (live) {
+        const long o = ((long)t * g.nseg + sg) * a.w + x;
+        g.first[o] = b ? sg * kSeg + __builtin_ctzll(b) : INT_MAX;
+        g.last[o] = b ? sg * kSeg + 63 - __builtin_clzll(b) : -1;
+    }
+    unsigned count = live ? (unsigned)__builtin_popcountll(b) : 0u;
+    for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o);
+    if (threadIdx.x == 0) atomicAdd(&a.tally[t], count);
+}
+
+__global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs g)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int sg = blockIdx.y, t = blockIdx.z;
+    if (x >= a.w) return;
+    const int kNone = 1 << 20;
+    const long base = (long)t * g.nseg * a.w + x;
+    int last = -kNone;
+    for (int s2 = sg - 1; s2 >= 0; --s2) {
+        const int v = g.last[base + (long)s2 * a.w];
+        if (v >= 0) { last = v; break; }
+    }
+    int next = a.h + kNone;
+    for (int s2 = sg + 1; s2 < g.nseg; ++s2) {
+        const int v = g.first[base + (long)s2 * a.w];
+        if (v != INT_MAX) { next = v; break; }
+    }
+    const int y0 = sg * kSeg;
+    const uint64_t b = segment_bits(a, t, x, y0);
+    int dn[kSeg];
+#pragma unroll
+    for (int i = 0; i < kSeg; ++i) {
+        last = (b >> i) & 1 ? y0 + i : last;
+        dn[i] = y0 + i - last;
+    }
+    float* col = a.inter + t * a.iplane + x;
+#pragma unroll
+    for (int i = kSeg - 1; i >= 0; --i) {
+        const int y = y0 + i;
+        next = (b >> i) & 1 ? y : next;
+        const int m = min(dn[i], next - y);
+        if (y < a.h) *gmem(col + (long)y * a.w) = m >= kNone ? FLT_MAX : (float)(m * m);
+    }
+}
+
 // Rows of the intermediate image; the Euclidean result is its square root.
 __global__ __launch_bounds__(64) void nft_rows_kernel(NftArgs a)
 {
@@ -150,6 +246,151 @@ __global__ __launch_bounds__(64) void nft_rows_kernel(NftArgs a)
         const long T = (long)gridDim.y * gridDim.x * 64;
         const long tid = (long)t * gridDim.x * 64 + y;
         euclid_1d(a.w, f, [&](int x, float d) { *gmem(drow + x) = sqrtf(d); }, a.sv + tid, a.sz + tid, a.sf + tid, T);
+    }
+}
+
+// Euclidean row pass: one lane per row, the reference's 1-D transform with
+// its stack in LDS.  The stack's newest kRing entries of every lane live in
+// an LDS ring (entry e in slot e mod kRing); every push is also written to
+// the lane's own contiguous stack in device memory, and a pop below the ring
+// reloads the kRefill entries under it in one batch.  The
+// row samples come through an LDS tile of 64 rows x 64 columns loaded with
+// coalesced reads (the next tile already in flight in registers), and the
+// results leave through the same tile as coalesced row stores.
+//
+// The output sweep runs from the last sample to the first with the stack
+// index falling: for each q it stops at the largest k with z[k] < q, which,
+// the intersections z being strictly increasing, is the k the reference's
+// forward sweep (smallest k with z[k + 1] >= q) stops at, so every result
+// (q - v_k)^2 + f(v_k) is the same float.  The sweep starts at the newest
+// entries, which are still in the ring.
+constexpr int kRing = 128, kRefill = 32, kRT = 64;
+
+__global__ __launch_bounds__(64) void nft_rows_ring_kernel(NftArgs a)
+{
+    __shared__ int rv[kRing][64];
+    __shared__ float rf[kRing][64], rz[kRing][64];
+    __shared__ float tile[kRT][kRT + 1];
+    const int lane = threadIdx.x;
+    const int t = blockIdx.y;
+    const int row0 = blockIdx.x * kRT;
+    const int nrows = min(kRT, a.h - row0);
+    const int n = a.w;
+    const long tid = (long)t * gridDim.x * 64 + (long)blockIdx.x * 64 + lane;
+    int* __restrict__ gv = a.sv + tid * (long)(n + 1);
+    float* __restrict__ gf = a.sf + tid * (long)(n + 1);
+    float* __restrict__ gz = a.sz + tid * (long)(n + 1);
+    const float* __restrict__ src = a.inter + t * a.iplane + (long)row0 * n;
+    float* __restrict__ dst = a.dist + t * a.dplane + (long)row0 * a.dpitch;
+    constexpr int M = kRing - 1;
+    const float inf = INFINITY;
+    float pf[kRT];
+    auto fetch = [&](int x0) {
+#pragma unroll
+        for (int r = 0; r < kRT; ++r)
+            pf[r] = r < nrows && x0 + lane < n ? *gmem(src + (long)r * n + x0 + lane) : 0.f;
+    };
+    int k = 0, lo = 0;
+    int vk = 0;
+    float fk = 0.f, zk = -inf;
+    auto refill = [&]() {
+        const int e0 = max(0, k - kRefill + 1);
+        int tv[kRefill];
+        float tf[kRefill], tz[kRefill];
+#pragma unroll
+        for (int j = 0; j < kRefill; ++j) {
+            const int e = k - j;
+            if (e >= e0) { tv[j] = gv[e]; tf[j] = gf[e]; tz[j] = gz[e]; }
+        }
+#pragma unroll
+        for (int j = 0; j < kRefill; ++j) {
+            const int e = k - j;
+            if (e >= e0) { rv[e & M][lane] = tv[j]; rf[e & M][lane] = tf[j]; rz[e & M][lane] = tz[j]; }
+        }
+        lo = e0;
+    };
+    // Lower envelope, samples left to right.
+    fetch(0);
+    for (int x0 = 0; x0 < n; x0 += kRT) {
+#pragma unroll
+        for (int r = 0; r < kRT; ++r) tile[r][lane] = pf[r];
+        __syncthreads();
+        if (x0 + kRT < n) fetch(x0 + kRT);
+        const int cn = min(kRT, n - x0);
+        float fnext = tile[lane][0];
+        for (int i = 0; i < cn; ++i) {
+            const int q = x0 + i;
+            const float fq = fnext;
+            fnext = tile[lane][i + 1 < cn ? i + 1 : i];
+            if (q == 0) {
+                vk = 0; fk = fq; zk = -inf;
+                rv[0][lane] = 0; rf[0][lane] = fq; rz[0][lane] = -inf;
+                gv[0] = 0; gf[0] = fq; gz[0] = -inf;
+                continue;
+            }
+            const float sum_q = fq + (float)q * (float)q;
+            float s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
+            while (s <= zk) {
+                --k;
+                if (k < lo) refill();
+                vk = rv[k & M][lane];
+                fk = rf[k & M][lane];
+                zk = rz[k & M][lane];
+                s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
+            }
+            ++k;
+            const int slot = k & M;
+            lo = k - lo >= kRing ? k - kRing + 1 : lo;   // entry k - kRing leaves the ring
+            rv[slot][lane] = q;
+            rf[slot][lane] = fq;
+            rz[slot][lane] = s;
+            gv[k] = q;
+            gf[k] = fq;
+            gz[k] = s;
+            vk = q; fk = fq; zk = s;
+        }
+        __syncthreads();
+    }
+    // Output sweep, right to left, in tiles of kRT columns.  Before each
+    // tile every lane whose ring holds fewer than kRT entries reloads the kRT
+    // entries under it, all lanes in one batch, so that lanes do not stall
+    // the wave one after another on their own reloads.
+    const int last0 = ((n - 1) / kRT) * kRT;
+    for (int x0 = last0; x0 >= 0; x0 -= kRT) {
+        const int cn = min(kRT, n - x0);
+        {
+            const bool need = k - lo < kRT && lo > 0;
+            const int e1 = lo - 1, e0 = max(0, lo - kRT);
+            int tv[kRT];
+            float tf[kRT], tz[kRT];
+#pragma unroll
+            for (int j = 0; j < kRT; ++j) {
+                const int e = e1 - j;
+                if (need && e >= e0) { tv[j] = gv[e]; tf[j] = gf[e]; tz[j] = gz[e]; }
+            }
+#pragma unroll
+            for (int j = 0; j < kRT; ++j) {
+                const int e = e1 - j;
+                if (need && e >= e0) { rv[e & M][lane] = tv[j]; rf[e & M][lane] = tf[j]; rz[e & M][lane] = tz[j]; }
+            }
+            lo = need ? e0 : lo;
+        }
+        for (int i = cn - 1; i >= 0; --i) {
+            const int q = x0 + i;
+            while (zk >= (float)q) {
+                --k;
+                if (k < lo) refill();
+                vk = rv[k & M][lane];
+                fk = rf[k & M][lane];
+                zk = rz[k & M][lane];
+            }
+            tile[lane][i] = sqrtf((float)((q - vk) * (q - vk)) + fk);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < kRT; ++r)
+            if (r < nrows && lane < cn) *gmem(dst + (long)r * a.dpitch + x0 + lane) = tile[r][lane];
+        __syncthreads();
     }
 }
 
@@ -197,13 +438,28 @@ int nft_passes(enbl_context* ctx, NftArgs& a, int nt)
     {
         Timed t(ctx, "nft_columns", nt * (double)plane * (2.0 + 4.0));
         hipLaunchKernelGGL(nft_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, a.tally);
-        hipLaunchKernelGGL(nft_columns_kernel, dim3(gx_c, nt), dim3(64), 0, ctx->stream, a);
+        if (a.norm == 2 && h <= kScanMax) {
+            SegArgs g;
+            g.nseg = (h + kSeg - 1) / kSeg;
+            void* sb;
+            rc = scratch(ctx, 20, 2 * (size_t)nt * g.nseg * w * sizeof(int), &sb);
+            if (rc) return rc;
+            g.first = (int*)sb;
+            g.last = (int*)sb + (size_t)nt * g.nseg * w;
+            hipLaunchKernelGGL(nft_colseg_summary_kernel, dim3(gx_c, g.nseg, nt), dim3(64), 0, ctx->stream, a, g);
+            hipLaunchKernelGGL(nft_colseg_apply_kernel, dim3(gx_c, g.nseg, nt), dim3(64), 0, ctx->stream, a, g);
+        } else {
+            hipLaunchKernelGGL(nft_columns_kernel, dim3(gx_c, nt), dim3(64), 0, ctx->stream, a);
+        }
         rc = launch_status("nft_columns");
         if (rc) return rc;
     }
     {
         Timed t(ctx, "nft_rows", nt * (double)plane * 8.0);
-        hipLaunchKernelGGL(nft_rows_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
+        if (a.norm == 2)
+            hipLaunchKernelGGL(nft_rows_ring_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(nft_rows_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
         rc = launch_status("nft_rows");
     }
     return rc;
