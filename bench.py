@@ -359,15 +359,15 @@ def c2_params(mod):
                              contrast_window=5, entropy_window=7)
 
 
-def enfuse_c2(args, enbl, scenes, np, torch, cpu=True, sample=(1500, 1000)):
+def enfuse_c2(args, enbl, scenes, np, torch, cpu=True, sample=(6000, 2000)):
     """Plain enfuse of 8 float32 RGB exposures at 6000x4000 (radiance
     1e-3..1e2, EV -3.5..+3.5 one EV apart, no clipping); weights exposure 1.0,
     saturation 0.2, contrast 0.5 (5x5 local standard deviation), entropy 0.3
     (7x7 window, per-channel exact-value histograms as the reference computes
     them); 10 levels of double pyramids.  Parity (2e-6 relative) and the
     oracle's CPU time are taken on a bounded sample: the same 8 exposures
-    cropped to `sample` (w, h) from the scene's centre, fused through both
-    paths."""
+    cropped to `sample` (w, h) from the scene's centre (full width, the
+    middle half of the rows), fused through both paths."""
     dev = torch.device("cuda:0")
     W, H = 6000, 4000
     print("bench: generating the 8 x 6000x4000 float scene", file=sys.stderr, flush=True)
