@@ -727,12 +727,17 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
     if (i < i1) step(Phase<0>(), i);
 }
 
+// Strip shape of the fused passes: rows per strip chosen so that about
+// g_strip_waves waves cover the level, within [4, g_strip_max_rows]
+// (enbl_set_strip_shape).
+int g_strip_waves = 8192, g_strip_max_rows = 32;
+
 inline int strip_rows_f(int waves_x, int out_rows)
 {
-    const long target = 8192;
+    const long target = g_strip_waves;
     long r = ((long)waves_x * out_rows + target - 1) / target;
     if (r < 4) r = 4;
-    if (r > 32) r = 32;
+    if (r > g_strip_max_rows) r = g_strip_max_rows;
     return (int)r;
 }
 
@@ -838,3 +843,11 @@ int launch_collapse_level(int ptype, int mtype, int out_itype, bool mask8, const
 }
 
 }  // namespace enbl
+
+extern "C" int enbl_set_strip_shape(int target_waves, int max_rows)
+{
+    if (target_waves < 64 || max_rows < 4 || max_rows > 4096) return enbl::host::fail(ENBL_E_ARG, "strip shape %d %d", target_waves, max_rows);
+    enbl::g_strip_waves = target_waves;
+    enbl::g_strip_max_rows = max_rows;
+    return ENBL_OK;
+}

@@ -70,6 +70,12 @@ int enbl_synchronize(enbl_context* ctx);
 #define ENBL_KERNELS_DIRECT 1
 #define ENBL_KERNELS_STREAMING 2
 int enbl_set_kernel_policy(int policy);
+/* Strip shape of the fused Laplacian and collapse passes for every context of
+ * the process: rows per strip such that about target_waves waves cover a
+ * level, at most max_rows (defaults 8192, 32).  A tuning knob with no
+ * counterpart in the reference; results do not depend on it.  Not
+ * thread-safe. */
+int enbl_set_strip_shape(int target_waves, int max_rows);
 /* Message of the last failing call on this thread. */
 const char* enbl_last_error(void);
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the
