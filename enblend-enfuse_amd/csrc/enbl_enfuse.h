@@ -28,7 +28,9 @@ int launch_weights(int itype, const void* img, long ip, const uint8_t* alpha, lo
                    const WeightArgs& a, const float* lut, const void* grad, long gp, const void* ent, long ep,
                    float* mask, long mp, float* norm, long np_, uint8_t* ualpha, long up, hipStream_t s);
 int launch_normalize(int mtype, const float* mask, long mp, const float* norm, long np_, int w, int h,
-                     float fallback, double scale, void* out, long op, hipStream_t s);
+                     float fallback, double scale, int hard, void* out, long op, hipStream_t s);
+// Hard masks over K weight planes (plane m at masks + m * plane).
+int launch_hard_mask(float* masks, long mp, size_t plane, int w, int h, int K, hipStream_t s);
 int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
                  int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
 int launch_fill(void* p, size_t bytes, hipStream_t s);

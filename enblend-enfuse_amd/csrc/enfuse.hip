@@ -603,18 +603,41 @@ __global__ __launch_bounds__(kTile) void weights_kernel(const T* __restrict__ im
 }
 
 // Normalisation (src/enfuse.h:1529-1534) fused with the fixed-point
-// conversion of mask level 0 (src/fixmath.h:188-194).
+// conversion of mask level 0 (src/fixmath.h:188-194); hard masks are
+// converted as they are (the reference skips the normalisation).
 template <class M>
 __global__ __launch_bounds__(kTile) void normalize_kernel(const float* __restrict__ mask, long mp,
                                                           const float* __restrict__ norm, long np_, int w,
-                                                          float fallback, double scale, M* __restrict__ out, long op)
+                                                          float fallback, double scale, int hard,
+                                                          M* __restrict__ out, long op)
 {
     const int x = blockIdx.x * kTile + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= w) return;
+    const float mv = mask[(long)y * mp + x];
     const float nv = norm[(long)y * np_ + x];
-    const float v = nv > 0.0 ? (255.0f * mask[(long)y * mp + x]) / nv : fallback;
+    const float v = hard ? mv : nv > 0.0 ? (255.0f * mv) / nv : fallback;
     out[(long)y * op + x] = from_real<M>((double)v * scale);
+}
+
+// Hard masks (src/enfuse.h:1402-1443) over the K weight planes: per pixel
+// the first image whose weight is the largest above 0 gets 255, the others
+// 0; where no weight is above 0, every image gets float(255) / K.
+__global__ __launch_bounds__(kTile) void hard_mask_kernel(float* __restrict__ masks, long mp, size_t plane,
+                                                          int w, int K)
+{
+    const int x = blockIdx.x * kTile + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    float* p = masks + (long)y * mp + x;
+    float mx = 0.0f;
+    int maxi = 0;
+    for (int m = 0; m < K; ++m) {
+        const float wv = p[m * plane];
+        if (wv > mx) { mx = wv; maxi = m; }
+    }
+    const float fb = (float)255 / K;
+    for (int m = 0; m < K; ++m) p[m * plane] = mx == 0.0f ? fb : (m == maxi ? 255.0f : 0.0f);
 }
 
 // ImageMaskMultiplyFunctor (src/enfuse.h:601-620) fused with the
@@ -738,11 +761,17 @@ int launch_weights(int itype, const void* img, long ip, const uint8_t* alpha, lo
 }
 
 int launch_normalize(int mtype, const float* mask, long mp, const float* norm, long np_, int w, int h,
-                     float fallback, double scale, void* out, long op, hipStream_t s)
+                     float fallback, double scale, int hard, void* out, long op, hipStream_t s)
 {
-    if (mtype == ENBL_I16) hipLaunchKernelGGL((normalize_kernel<int16_t>), grid_for(w, h), dim3(kTile), 0, s, mask, mp, norm, np_, w, fallback, scale, (int16_t*)out, op);
-    else if (mtype == ENBL_I32) hipLaunchKernelGGL((normalize_kernel<int32_t>), grid_for(w, h), dim3(kTile), 0, s, mask, mp, norm, np_, w, fallback, scale, (int32_t*)out, op);
+    if (mtype == ENBL_I16) hipLaunchKernelGGL((normalize_kernel<int16_t>), grid_for(w, h), dim3(kTile), 0, s, mask, mp, norm, np_, w, fallback, scale, hard, (int16_t*)out, op);
+    else if (mtype == ENBL_I32) hipLaunchKernelGGL((normalize_kernel<int32_t>), grid_for(w, h), dim3(kTile), 0, s, mask, mp, norm, np_, w, fallback, scale, hard, (int32_t*)out, op);
     else return ENBL_E_UNSUPPORTED;
+    return 0;
+}
+
+int launch_hard_mask(float* masks, long mp, size_t plane, int w, int h, int K, hipStream_t s)
+{
+    hipLaunchKernelGGL(hard_mask_kernel, grid_for(w, h), dim3(kTile), 0, s, masks, mp, plane, w, K);
     return 0;
 }
 

@@ -1,6 +1,6 @@
 // C ABI of enfuse's weighting and fusion (declared in include/enblend_amd.h).
 //
-// enbl_enfuse follows enfuseMain (src/enfuse.h:1217-1609) stage by stage but
+// enbl_enfuse follows enfuseMain (src/enfuse.h:1217-1609, soft or hard masks) stage by stage but
 // keeps every intermediate on the device: the K weight images and the norm
 // image are built first (the reference's first pass over the inputs), then
 // each input's Laplacian pyramid and normalised mask Gaussian pyramid are
@@ -46,6 +46,7 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
 {
     if (!p) return fail(ENBL_E_ARG, "enfuse: null parameters");
     if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "enfuse: image type %d", itype);
+    if (p->hard_mask != 0 && p->hard_mask != 1) return fail(ENBL_E_ARG, "enfuse: hard_mask %d", p->hard_mask);
     if (p->exposure_function < 0 || p->exposure_function > ENBL_EXPOSURE_BISQUARE)
         return fail(ENBL_E_ARG, "enfuse: exposure function %d", p->exposure_function);
     if (p->w_exposure > 0.0 && (p->exposure_optimum < 0.0 || p->exposure_optimum > 1.0))
@@ -278,6 +279,12 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
                            (float*)masks + plane * m, pitch, (float*)norm, pitch, (uint8_t*)ua, pitch);
         if (rc) return rc;
     }
+    if (p->hard_mask) {
+        Timed t(ctx, "hard_mask", (double)w * h * 8.0 * K);
+        rc = launch_hard_mask((float*)masks, pitch, plane, w, h, K, s);
+        if (!rc) rc = launch_status("hard_mask");
+        if (rc) return rc;
+    }
 
     // Pass 2: per input, Laplacian pyramid, normalised mask Gaussian pyramid
     // over the union alpha, multiply and accumulate (src/enfuse.h:1497-1590).
@@ -300,7 +307,7 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
             const Level& L0 = mgp->lv[0];
             Timed t(ctx, "normalize", (double)w * h * (8.0 + ptype_size(mt)));
             rc = launch_normalize(mt, (const float*)masks + plane * m, pitch, (const float*)norm, pitch, w, h,
-                                  fallback, scale, level_ptr(mgp, 0), L0.pitch, s);
+                                  fallback, scale, p->hard_mask, level_ptr(mgp, 0), L0.pitch, s);
             if (!rc) rc = launch_status("normalize");
             if (rc) return rc;
         }

@@ -191,7 +191,8 @@ typedef struct enbl_enfuse_params {
     int exposure_function;        /* ENBL_EXPOSURE_* */
     int contrast_window;          /* odd, >= 3 */
     int entropy_window;           /* odd, >= 3, window area <= 121 */
-    int reserved;                 /* 0 */
+    int hard_mask;                /* 0: soft masks; 1: --hard-mask, the first image of largest
+                                     weight takes a pixel whole (src/enfuse.h:1402-1443) */
     /* Host table of ExposureWeight::weight(code / max) for every gray code
      * of an integer image (256 or 65536 doubles), e.g. from a user plugin
      * (src/exposure_weight_base.h:29-88).  NULL: the built-in function. */

@@ -101,7 +101,7 @@ typedef struct orc_enfuse_params {
     int exposure_function;          /* ORC_EXP_* built-in */
     int contrast_window;            /* --contrast-window-size, default 5 */
     int entropy_window;             /* --entropy-window-size, default 3 */
-    int pad_;
+    int hard_mask;                  /* 0: soft masks; 1: --hard-mask (src/enfuse.h:1402-1443) */
     const double* exposure_table;   /* optional: weight(code / max) for every
                                        gray code of an integer image (user
                                        ExposureWeight); NULL = built-in */

@@ -282,7 +282,7 @@ template <> int ptype_of<int16_t>() { return ORC_I16; }
 template <> int ptype_of<int32_t>() { return ORC_I32; }
 template <> int ptype_of<double>() { return ORC_F64; }
 
-// enfuseMain, src/enfuse.h:1217-1609 (soft masks).
+// enfuseMain, src/enfuse.h:1217-1609 (soft or hard masks).
 template <class T, class P, class M>
 int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* alphas,
              const orc_enfuse_params* prm, int levels, bool wrap, int img_frac, int mask_frac,
@@ -299,6 +299,22 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
         for (long i = 0; i < N; ++i) {
             if (alphas[m][i]) ua[i] = alphas[m][i];
             norm[i] = masks[m][i] + norm[i];
+        }
+    }
+    if (prm->hard_mask) {
+        // Hard masks (src/enfuse.h:1402-1443): per pixel the first image of
+        // largest weight above 0 gets 255, the others 0; where no weight is
+        // above 0 every image gets float(255) / K.
+#pragma omp parallel for schedule(guided)
+        for (long i = 0; i < N; ++i) {
+            float mx = 0.0f;
+            int maxi = 0;
+            for (int m = 0; m < K; ++m) {
+                const float wv = masks[m][i];
+                if (wv > mx) { mx = wv; maxi = m; }
+            }
+            for (int m = 0; m < K; ++m)
+                masks[m][i] = mx == 0.0f ? (float)255 / K : (m == maxi ? 255.0f : 0.0f);
         }
     }
     long nimg = 0, nmask = 0;
@@ -321,7 +337,7 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
         const std::vector<float>& mk = masks[m];
 #pragma omp parallel for schedule(guided)
         for (long i = 0; i < N; ++i) {
-            const float nm = norm[i] > 0.0 ? (255.0f * mk[i]) / norm[i] : fallback;
+            const float nm = prm->hard_mask ? mk[i] : norm[i] > 0.0 ? (255.0f * mk[i]) / norm[i] : fallback;
             mgp[i] = fromRealPromote<M>((double)nm * scale);
         }
         rc = gaussian_levels<M>(mgp.data(), dm, 1, wrap, ua.data());

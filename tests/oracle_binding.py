@@ -47,15 +47,15 @@ class EnfuseParams(ctypes.Structure):
                 ("w_contrast", ctypes.c_double), ("w_entropy", ctypes.c_double),
                 ("exposure_optimum", ctypes.c_double), ("exposure_width", ctypes.c_double),
                 ("exposure_function", ctypes.c_int), ("contrast_window", ctypes.c_int),
-                ("entropy_window", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("entropy_window", ctypes.c_int), ("hard_mask", ctypes.c_int),
                 ("exposure_table", ctypes.c_void_p)]
 
 
 def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
                   exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXP_GAUSSIAN,
-                  contrast_window=5, entropy_window=3, exposure_table=None):
+                  contrast_window=5, entropy_window=3, exposure_table=None, hard_mask=False):
     p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
-                     exposure_function, contrast_window, entropy_window, 0, None)
+                     exposure_function, contrast_window, entropy_window, int(hard_mask), None)
     if exposure_table is not None:
         tab = np.ascontiguousarray(exposure_table, np.float64)
         p.exposure_table = tab.ctypes.data

@@ -127,10 +127,14 @@ def test_batch_graph_replay_reads_current_inputs(enbl, gpu_ctx):
             np.testing.assert_array_equal(d[2].cpu().numpy(), po, err_msg=f"replay {rep} stack {s}")
 
 
-def test_batch_fallback_params_match_plain(enbl, gpu_ctx):
-    """Contrast/entropy weights take the per-stack path inside the batch call."""
+@pytest.mark.parametrize("kw", [dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3,
+                                     contrast_window=5, entropy_window=3),
+                                dict(w_exposure=1.0, w_saturation=0.2, hard_mask=True)],
+                         ids=["contrast-entropy", "hard-mask"])
+def test_batch_fallback_params_match_plain(enbl, gpu_ctx, kw):
+    """Contrast/entropy weights and hard masks take the per-stack path inside
+    the batch call."""
     rng = np.random.default_rng(78)
-    kw = dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3, contrast_window=5, entropy_window=3)
     geoms = [(83, 61, 3, None, "holes"), (40, 33, 2, None, "full")]
     host, dev = _make_batch(rng, U8, geoms, enbl)
     plain = _plain(enbl, gpu_ctx, U8, dev, kw)

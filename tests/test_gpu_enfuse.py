@@ -140,3 +140,32 @@ def test_contrast_entropy_tiles_bit_exact(enbl, gpu_ctx, oracle, itype, windows,
                                            ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
     gpu_ctx.sync()
     np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("K", [3, 7])
+def test_hard_mask_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, K):
+    """--hard-mask fusion (src/enfuse.h:1402-1443) against the oracle, with
+    alpha holes, and with all weights zero (the float 255 / K share)."""
+    rng = np.random.default_rng(950 + itype + K)
+    h, w = 71, 93
+    imgs = _stack(rng, itype, K, h, w)
+    alphas = [np.where(rng.random((h, w)) < 0.9, 255, 0).astype(np.uint8) for _ in range(K)]
+    L = oracle.auto_levels(w, h)
+    for kw in (dict(w_exposure=1.0, w_saturation=0.2, hard_mask=True), dict(w_exposure=0.0, hard_mask=True)):
+        print(f"hard mask itype={itype} K={K} {kw}", flush=True)
+        want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), L, dither=1, seed=5)
+        dimgs = [torch.from_numpy(i).cuda() for i in imgs]
+        das = [torch.from_numpy(a).cuda() for a in alphas]
+        out = torch.zeros_like(dimgs[0])
+        oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        enbl.enfuse(gpu_ctx, itype, dimgs, das, enbl.enfuse_params(**kw), L, out, oa, dither=1, seed=5)
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
+        got = out.cpu().numpy()
+        if itype == F32:
+            sel = want_a != 0
+            np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
+        else:
+            np.testing.assert_array_equal(got, want)

@@ -206,3 +206,70 @@ def test_fusion_known_answer_constant_images(oracle):
         q, f = v >> 7, v & 127
         want = q + 1 if f >= 64 else q
         assert np.all(out[..., c] == want), (c, out[0, 0], want)
+
+
+def _noise_img(rng, itype, h, w, lo, hi):
+    m = MAXV[itype]
+    v = rng.uniform(lo, hi, size=(h, w, 3)) * m
+    return np.clip(np.rint(v), 0, m).astype(np.uint8 if itype == U8 else np.uint16)
+
+
+def test_hard_mask_dominant_image_passes_through(oracle):
+    """Hard masks (src/enfuse.h:1402-1443): where one image has the largest
+    weight everywhere it takes every pixel whole (mask 255, coefficient
+    exactly 1; the others 0), so the fused output is that image; the soft
+    masks of the same stack blend."""
+    rng = np.random.default_rng(21)
+    h, w = 33, 47
+    for itype in (U8, U16):
+        dark = _noise_img(rng, itype, h, w, 0.0, 0.05)
+        mid = _noise_img(rng, itype, h, w, 0.45, 0.55)
+        bright = _noise_img(rng, itype, h, w, 0.95, 1.0)
+        full = [np.full((h, w), 255, np.uint8)] * 3
+        L = oracle.auto_levels(w, h)
+        out, _ = oracle.enfuse([dark, mid, bright], full, itype, enfuse_params(w_exposure=1.0, hard_mask=True), L,
+                               dither=1, seed=3)
+        np.testing.assert_array_equal(out, mid)
+        soft, _ = oracle.enfuse([dark, mid, bright], full, itype, enfuse_params(w_exposure=1.0), L, dither=1, seed=3)
+        assert not np.array_equal(soft, mid)
+
+
+def test_hard_mask_ties_go_to_the_first_image(oracle):
+    """Equal weights: the first image of largest weight wins (strict >).
+    B is A with its channels rotated, so gray and saturation, hence every
+    weight, are equal, and each order of the stack returns its first image."""
+    rng = np.random.default_rng(22)
+    h, w = 29, 31
+    A = _noise_img(rng, U8, h, w, 0.1, 0.9)
+    B = np.ascontiguousarray(np.roll(A, 1, axis=2))
+    full = [np.full((h, w), 255, np.uint8)] * 2
+    prm = enfuse_params(w_exposure=1.0, w_saturation=0.2, hard_mask=True)
+    L = oracle.auto_levels(w, h)
+    np.testing.assert_array_equal(oracle.enfuse([A, B], full, U8, prm, L)[0], A)
+    np.testing.assert_array_equal(oracle.enfuse([B, A], full, U8, prm, L)[0], B)
+
+
+def test_hard_mask_zero_weights_take_float_share(oracle):
+    """No weight above 0: every image gets float(255) / K, a float division
+    (src/enfuse.h:1430-1432), where the soft fallback is the integer 255 / K
+    (:1534).  Seven constant images of value 100 then fuse to 100 with hard
+    masks (36.428574 -> 4663/32640 per image) and to 99 with soft masks
+    (36 -> 4608/32640), computed here in closed form (no dither)."""
+    h, w, K, v = 20, 24, 7, 100
+    imgs = [np.full((h, w, 3), v, np.uint8) for _ in range(K)]
+    full = [np.full((h, w), 255, np.uint8)] * K
+    L = oracle.auto_levels(w, h)
+
+    def closed_form(mask_value):
+        mfix = round_half_away(float(mask_value) * 128.0)
+        c = mfix / float(255 << 7)
+        s = K * round_half_away(v * 128 * c)
+        q, f = s >> 7, s & 127
+        return q + 1 if f >= 64 else q
+
+    want_hard = closed_form(np.float32(255) / np.float32(K))
+    want_soft = closed_form(255 // K)
+    assert (want_hard, want_soft) == (100, 99)
+    hard, _ = oracle.enfuse(imgs, full, U8, enfuse_params(w_exposure=0.0, hard_mask=True), L, dither=0)
+    soft, _ = oracle.enfuse(imgs, full, U8, enfuse_params(w_exposure=0.0), L, dither=0)
+    assert np.all(hard == want_hard) and np.all(soft == want_soft)
