@@ -467,11 +467,17 @@ def main():
         a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
+        b.copy_(a)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
         for _ in range(args.steps):
             b.copy_(a)
+        t1.record()
         torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / args.steps
         print(json.dumps({"workload": "copy-calibration", "bytes_read_per_copy": n, "bytes_written_per_copy": n,
-                          "copies": args.steps}), flush=True)
+                          "copies": args.steps + 1, "ms_per_copy": round(ms, 4),
+                          "GB_per_s": round(2 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
         return
     print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
     white, wa, black, ba, _ = scenes.enblend_pair(seed=1)

@@ -327,13 +327,13 @@ __device__ __forceinline__ double mask_coeff(int32_t v, double maxpyr, double in
 
 // SpecializedConvertPyramidToScalarFunctor (src/fixmath.h:236-290).
 template <class I>
-__device__ __forceinline__ I to_image(int32_t v, int frac, int dither, uint64_t seed, int x, int y, int c)
+__device__ __forceinline__ I to_image(int32_t v, int frac, int dither, uint32_t h, int c)
 {
     const int32_t q = v >> frac, f = v & ((1 << frac) - 1);
     const int32_t quarter = 1 << (frac - 2), three_quarter = 3 << (frac - 2);
     int32_t r;
     if (f >= quarter && f < three_quarter) {
-        if (dither == ENBL_DITHER_DET) r = dither_rand(seed, x, y, c, frac) <= f ? q + 1 : q;
+        if (dither == ENBL_DITHER_DET) r = dither_draw(h, c, frac) <= f ? q + 1 : q;
         else r = f >= (1 << (frac - 1)) ? q + 1 : q;
     } else if (f >= quarter) {
         r = q + 1;
@@ -372,10 +372,12 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
     I* op = static_cast<I*>(J.out) + (long)Y * J.opitch + 3L * X;
     auto* o = gmem(op);
     I r0[3], r1[3];
+    const uint32_t h0 = dither == ENBL_DITHER_DET ? dither_hash(J.seed, X, Y) : 0u;
+    const uint32_t h1 = dither == ENBL_DITHER_DET ? dither_hash(J.seed, X + 1, Y) : 0u;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        r0[ch] = to_image<I>(v0[ch], frac, dither, J.seed, X, Y, ch);
-        r1[ch] = to_image<I>(v1[ch], frac, dither, J.seed, X + 1, Y, ch);
+        r0[ch] = to_image<I>(v0[ch], frac, dither, h0, ch);
+        r1[ch] = to_image<I>(v1[ch], frac, dither, h1, ch);
     }
     if constexpr (sizeof(I) == 2) {
         if (xb && u0 && u1 && ((uintptr_t)op & 3) == 0) {
@@ -437,8 +439,9 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                     *gmem(J.oalpha + (long)y * J.oapitch + j) = ua;
                     if (ua) {
                         auto* o = gmem(static_cast<I*>(J.out) + (long)y * J.opitch + 3L * j);
+                        const uint32_t hh = dither == ENBL_DITHER_DET ? dither_hash(J.seed, j, y) : 0u;
 #pragma unroll
-                        for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(acc[ch], frac, dither, J.seed, j, y, ch);
+                        for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(acc[ch], frac, dither, hh, ch);
                     }
                 } else {
 #pragma unroll

@@ -236,11 +236,14 @@ int fused_level_count(const enbl_pyramid* p, int wrap, const uint8_t* alpha)
     return s;
 }
 
-// Fused passes over levels 0..s-1 (s >= 1): Laplacian levels 0..s-1 and
+// Fused passes over levels l0..s-1 (s >= 1): Laplacian levels l0..s-1 and
 // Gaussian level s land in the pyramid; the alpha of level l+1 goes to
 // abuf[l & 1] (pitch apl), so level s's alpha ends in abuf[(s - 1) & 1].
+// l0 = 1 starts from Gaussian level 1 in g1 (level-1 pitch and plane) with
+// its alpha in abuf[0]; level 0 of the pyramid is then not written.
 int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* img, long ipitch,
-                           const uint8_t* alpha, long apitch, enbl_pyramid* p, int s, uint8_t* const abuf[2], long apl)
+                           const uint8_t* alpha, long apitch, enbl_pyramid* p, int s, uint8_t* const abuf[2], long apl,
+                           int l0 = 0, const void* g1 = nullptr)
 {
     const size_t es = (size_t)ptype_size(p->ptype);
     void* gbuf[2] = {nullptr, nullptr};
@@ -252,12 +255,12 @@ int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* i
         gbuf[0] = g;
         gbuf[1] = (char*)g + gbytes;
     }
-    for (int l = 0; l < s; ++l) {
+    for (int l = l0; l < s; ++l) {
         const Level& F = p->lv[l];
         const Level& Cl = p->lv[l + 1];
         FusedLapArgs a;
         if (l == 0) { a.src = img; a.spitch = ipitch; a.splane = 0; a.salpha = alpha; a.sapitch = apitch; }
-        else { a.src = gbuf[(l - 1) & 1]; a.spitch = F.pitch; a.splane = F.plane; a.salpha = abuf[(l - 1) & 1]; a.sapitch = apl; }
+        else { a.src = l == 1 && g1 ? g1 : gbuf[(l - 1) & 1]; a.spitch = F.pitch; a.splane = F.plane; a.salpha = abuf[(l - 1) & 1]; a.sapitch = apl; }
         a.frac = frac;
         a.fw = F.w; a.fh = F.h;
         a.lap = level_ptr(p, l); a.lpitch = F.pitch; a.lplane = F.plane;
@@ -899,6 +902,9 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     // Laplacian pyramids of white and black (src/enblend.h:419-446): the
     // fused levels, each pyramid keeping its own alpha of level s for the
     // shared small-level launches.
+    const int esP = ptype_size(pt), esM = ptype_size(mt);
+    const bool otf = s >= 2 && itype != ENBL_F32 && C == 3;
+    void* g1[2] = {nullptr, nullptr};
     {
         uint8_t* wab[2];
         uint8_t* bab[2];
@@ -909,18 +915,65 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
         if (rc) return rc;
         bab[0] = (uint8_t*)bp;
         bab[1] = (uint8_t*)bp + (size_t)apl * wlp->lv[1].h;
-        rc = fused_laplacian_levels(ctx, itype, ifrac, white, wpitch, walpha, wapitch, wlp, s, wab, apl);
-        if (!rc) rc = fused_laplacian_levels(ctx, itype, ifrac, black, bpitch, balpha, bapitch, blp, s, bab, apl);
+        if (otf) {
+            // Level 0 of either Laplacian pyramid is never stored: only the
+            // Gaussian levels 1 (and their alpha) are reduced from the
+            // images here, kept apart for the level-0 collapse, which forms
+            // both Laplacian levels 0 on the fly.
+            const Level& D = wlp->lv[1];
+            const size_t gbytes = round_up((size_t)D.plane * C * esP, 256);
+            void* g;
+            rc = scratch(ctx, 21, 2 * gbytes, &g);
+            if (rc) return rc;
+            g1[0] = g;
+            g1[1] = (char*)g + gbytes;
+            const void* src[2] = {white, black};
+            const long sp[2] = {wpitch, bpitch};
+            const uint8_t* sa[2] = {walpha, balpha};
+            const long sap[2] = {wapitch, bapitch};
+            uint8_t* da[2] = {wab[0], bab[0]};
+            for (int k = 0; k < 2 && !rc; ++k) {
+                ReduceArgs r;
+                memset(&r, 0, sizeof r);
+                r.src = src[k]; r.spitch = sp[k]; r.splane = 0; r.sw = w; r.sh = h;
+                r.salpha = sa[k]; r.sapitch = sap[k];
+                r.dst = g1[k]; r.dpitch = D.pitch; r.dplane = D.plane; r.dw = D.w; r.dh = D.h;
+                r.dalpha = da[k]; r.dapitch = apl;
+                r.channels = C; r.frac = ifrac;
+                char name[64];
+                snprintf(name, sizeof name, "reduce_image0_%s_c%d", pname(pt), C);
+                const double n0 = (double)w * h, n1 = (double)D.w * D.h, esC = (double)esP * C;
+                // credited bytes: the level-0 conversion and the reduce it
+                // feeds; own bytes: image and alpha read, level 1 and its alpha
+                // written
+                Timed t(ctx, name, n0 * C * (ptype_size(itype) + (double)esP) + (esC + 1.0) * (n0 + n1),
+                        n0 * (C * (double)ptype_size(itype) + 1.0) + n1 * (esC + 1.0));
+                rc = launch_reduce_image(itype, pt, r, ctx->stream);
+                if (rc) return fail(rc, "level-0 reduce: unsupported type");
+                rc = launch_status(name);
+            }
+            if (!rc) rc = fused_laplacian_levels(ctx, itype, ifrac, white, wpitch, walpha, wapitch, wlp, s, wab, apl, 1, g1[0]);
+            if (!rc) rc = fused_laplacian_levels(ctx, itype, ifrac, black, bpitch, balpha, bapitch, blp, s, bab, apl, 1, g1[1]);
+        } else {
+            rc = fused_laplacian_levels(ctx, itype, ifrac, white, wpitch, walpha, wapitch, wlp, s, wab, apl);
+            if (!rc) rc = fused_laplacian_levels(ctx, itype, ifrac, black, bpitch, balpha, bapitch, blp, s, bab, apl);
+        }
         if (!rc) rc = enblend_tail(ctx, mgp, wlp, blp, s, wab[(s - 1) & 1], bab[(s - 1) & 1], apl, wv);
         if (rc) return rc;
     }
     // Levels s-1..0: blend, add the expanded coarser level and (level 0)
     // convert in one pass.
-    const int esP = ptype_size(pt), esM = ptype_size(mt);
     for (int l = s - 1; l >= 0; --l) {
         const Level& L = blp->lv[l];
         const Level& Cl = blp->lv[l + 1];
         FusedCollapseArgs a;
+        memset(&a, 0, sizeof a);
+        if (l == 0 && otf) {
+            a.otf = 1;
+            a.wimg = white; a.wipitch = wpitch;
+            a.wg1 = g1[0]; a.wg1pitch = Cl.pitch; a.wg1plane = Cl.plane;
+            a.bg1 = g1[1]; a.bg1pitch = Cl.pitch; a.bg1plane = Cl.plane;
+        }
         a.coarse = level_ptr(blp, l + 1); a.cpitch = Cl.pitch; a.cplane = Cl.plane; a.cw = Cl.w; a.ch = Cl.h;
         a.black = level_ptr(blp, l); a.fpitch = L.pitch; a.fplane = L.plane;
         a.white = level_ptr(wlp, l); a.wpitch = wlp->lv[l].pitch; a.wplane = wlp->lv[l].plane;
@@ -940,8 +993,15 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
         const double nL = (double)L.w * L.h, nC = (double)Cl.w * Cl.h;
         double bytes = nL * (esM + 3.0 * C * esP) + C * esP * (nC + 2.0 * nL);
         if (l == 0) bytes += nL * (1.0 + C * (esP + ptype_size(itype))) + 3.0 * w * h;
-        const double own = C * esP * (nC + 2.0 * nL) +
-                           (l == 0 ? nL * (1.0 + 2.0 + C * ptype_size(itype) + 1.0) : nL * (esM + C * esP));
+        double own = C * esP * (nC + 2.0 * nL) +
+                     (l == 0 ? nL * (1.0 + 2.0 + C * ptype_size(itype) + 1.0) : nL * (esM + C * esP));
+        if (a.otf) {
+            // also the two expand-subtracts of level 0 (with the images'
+            // conversion); own: the images and both Gaussian levels 1 are read
+            // in place of the two Laplacian levels 0
+            bytes += 2.0 * (C * esP * (nC + 2.0 * nL) + nL * C * (ptype_size(itype) + (double)esP));
+            own = C * esP * 3.0 * nC + nL * (1.0 + 2.0 + 2.0 * C * ptype_size(itype)) + nL * (C * ptype_size(itype) + 1.0);
+        }
         Timed t(ctx, name, bytes, own);
         rc = launch_collapse_level(pt, mt, l == 0 ? itype : 0, l == 0, a, C, ctx->stream);
         if (rc) return fail(rc, "collapse level: unsupported types");

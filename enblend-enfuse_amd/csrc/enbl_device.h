@@ -150,18 +150,26 @@ __device__ __forceinline__ double quot255(int32_t v) {
     return __fma_rn(e, r, q0);
 }
 
-// Counter-based dither draw keyed by (seed, x, y, channel), mapped onto the
-// reference's integer grid (r & half_m1) + quarter (src/fixmath.h:246-264).
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
+// Counter-based dither draw: one 32-bit hash per pixel keyed by (seed, x, y)
+// (two rounds of the murmur3 32-bit finaliser); channel c of the pixel takes
+// bits [10c, 10c + frac - 2] as its offset on the reference's integer grid
+// (r & half_m1) + quarter (src/fixmath.h:246-264).  frac <= 11, c <= 2.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
+}
+__device__ __forceinline__ uint32_t dither_hash(uint64_t seed, int x, int y) {
+    const uint32_t s = (uint32_t)seed ^ (uint32_t)(seed >> 32);
+    return fmix32(s ^ fmix32(((uint32_t)y * 0x9E3779B1u) ^ (uint32_t)x));
+}
+__device__ __forceinline__ int dither_draw(uint32_t h, int c, int frac) {
+    return (int)((h >> (10 * c)) & ((1u << (frac - 1)) - 1u)) + (1 << (frac - 2));
 }
 __device__ __forceinline__ int dither_rand(uint64_t seed, int x, int y, int c, int frac) {
-    const uint64_t key = seed ^ ((uint64_t)(uint32_t)y << 32) ^ ((uint64_t)(uint32_t)x << 2) ^ (uint64_t)(uint32_t)c;
-    const uint64_t r = splitmix64(key);
-    return (int)(r & (uint64_t)((1 << (frac - 1)) - 1)) + (1 << (frac - 2));
+    return dither_draw(dither_hash(seed, x, y), c, frac);
 }
 
 }  // namespace enbl

@@ -30,6 +30,14 @@ struct FusedCollapseArgs {
     uint8_t* balpha; long bapitch;
     int dither; uint64_t seed;
     int rows;
+    // Level 0 with both Laplacian levels formed on the fly (otf != 0, integer
+    // pyramids): white = wimg << ifrac - expand(wg1), black likewise from the
+    // black image `img` (read before it is overwritten) and bg1; `white` and
+    // `black` are then unused.
+    int otf;
+    const void* wimg; long wipitch;
+    const void* wg1; long wg1pitch; long wg1plane;
+    const void* bg1; long bg1pitch; long bg1plane;
 };
 
 // src_kind 0: planes of ptype; ENBL_U8/ENBL_U16/ENBL_F32: interleaved image.

@@ -378,19 +378,20 @@ template <class P> __device__ __forceinline__ P blend_px(double wc, P w, P b) {
 
 // SpecializedConvertPyramidToScalarFunctor with the deterministic dither
 // draw (src/fixmath.h:2157-2294), without per-lane branches.
+// h: the pixel's dither hash (dither_hash), shared by its channels.
 template <class P, class I> struct OutConv {
-    __device__ static __forceinline__ I conv(P v, int frac, bool det, uint64_t seed, int x, int y, int c) {
+    __device__ static __forceinline__ I conv(P v, int frac, bool det, uint32_t h, int c) {
         constexpr int32_t hi = sizeof(I) == 1 ? 255 : 65535;
         const int32_t q = (int32_t)v >> frac, f = (int32_t)v & ((1 << frac) - 1);
         const int32_t q1 = 1 << (frac - 2), q3 = 3 << (frac - 2);
-        const bool up_rand = det ? dither_rand(seed, x, y, c, frac) <= f : f >= (1 << (frac - 1));
+        const bool up_rand = det ? dither_draw(h, c, frac) <= f : f >= (1 << (frac - 1));
         const bool band = f >= q1 && f < q3;
         const int32_t r = (band ? up_rand : f >= q1) ? q + 1 : q;
         return (I)min(hi, max(0, r));
     }
 };
 template <> struct OutConv<double, float> {
-    __device__ static __forceinline__ float conv(double v, int, bool, uint64_t, int, int, int) {
+    __device__ static __forceinline__ float conv(double v, int, bool, uint32_t, int) {
         return (float)(v >= 1.0 ? expm1(v - 1.0) : 1.0 - 1.0 / v);
     }
 };
@@ -481,7 +482,14 @@ template <class P> __device__ __forceinline__ P bload_elem(__amdgpu_buffer_rsrc_
 // I = void: the collapsed level goes back into `black` (levels >= 1);
 // otherwise it is converted into the interleaved output image of type I.
 // WIDE: image rows 4-byte aligned and alpha / mask rows 2-byte aligned.
-template <class P, class M, int C, class I, bool MASK8, bool WIDE>
+//
+// OTF (level 0, integer pyramids): the white and black Laplacian values are
+// formed on the fly from the images and their Gaussian levels 1, as the
+// Laplacian pass would have stored them (v << frac minus the expansion,
+// wrapping), so level 0 of neither Laplacian pyramid is written or read.
+struct ImgArgs { const void* src; long spitch; int frac; };
+
+template <class P, class M, int C, class I, bool MASK8, bool WIDE, bool OTF>
 __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a)
 {
     typedef Pix<P> Px;
@@ -490,10 +498,14 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
     typedef BPair<P> B;
     constexpr bool OUT = !std::is_same<I, void>::value;
     typedef typename std::conditional<OUT, I, uint8_t>::type IT;
+    typedef IntImageSrc<IT, P, C, WIDE> ISrc;
+    static_assert(!OTF || (OUT && std::is_integral<P>::value), "on-the-fly Laplacians: level 0, integer pyramids");
     constexpr uint32_t ES = sizeof(P);
     // raw words of one fine row for this lane's pair
+    struct LapPairs { typename B::Pair b[C], w[C]; };
+    struct ImgPairs { typename ISrc::Raw b, w; };
     struct FineRaw {
-        typename B::Pair b[C], w[C];
+        typename std::conditional<OTF, ImgPairs, LapPairs>::type v;
         typename std::conditional<MASK8, AlphaPair<WIDE>, typename BPair<M>::Pair>::type m;
         AlphaPair<WIDE> wa, ba;   // level 0 only
     };
@@ -534,10 +546,15 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
         const uint32_t off = owned ? (uint32_t)X * ES : kBufSkip;
         const P* brow = static_cast<const P*>(a.black) + y * a.fpitch;
         const P* wrow = static_cast<const P*>(a.white) + y * a.wpitch;
+        if constexpr (OTF) {
+            ISrc::fetch(ImgArgs{a.wimg, a.wipitch, a.ifrac}, fw, (int)y, ok, L, F.v.w);
+            ISrc::fetch(ImgArgs{a.img, a.ipitch, a.ifrac}, fw, (int)y, ok, L, F.v.b);
+        } else {
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) {
-            F.b[ch] = B::fetch(buf_rsrc(brow + ch * a.fplane, nb), off);
-            F.w[ch] = B::fetch(buf_rsrc(wrow + ch * a.wplane, nb), off);
+            for (int ch = 0; ch < C; ++ch) {
+                F.v.b[ch] = B::fetch(buf_rsrc(brow + ch * a.fplane, nb), off);
+                F.v.w[ch] = B::fetch(buf_rsrc(wrow + ch * a.wplane, nb), off);
+            }
         }
         if constexpr (MASK8) {
             F.m.fetch(static_cast<const uint8_t*>(a.mask) + y * a.mpitch, ok, fw, L);
@@ -551,13 +568,14 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
             F.ba.fetch(a.balpha + y * a.bapitch, ok, fw, L);
         }
     };
-    auto fetch_coarse = [&](int i, bool want, CoarseRaw& R) {
+    auto fetch_level = [&](const void* base, long pitch, long plane, int i, bool want, CoarseRaw& R) {
         const bool ok = want && i >= 0 && i < chh;
-        const P* row = static_cast<const P*>(a.coarse) + (long)(ok ? i : 0) * a.cpitch;
+        const P* row = static_cast<const P*>(base) + (long)(ok ? i : 0) * pitch;
         const uint32_t nb = ok ? (uint32_t)cw * ES : 0u;
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) R.v[ch] = bload_elem<P>(buf_rsrc(row + ch * a.cplane, nb), offc);
+        for (int ch = 0; ch < C; ++ch) R.v[ch] = bload_elem<P>(buf_rsrc(row + ch * plane, nb), offc);
     };
+    auto fetch_coarse = [&](int i, bool want, CoarseRaw& R) { fetch_level(a.coarse, a.cpitch, a.cplane, i, want, R); };
     // polyphase sums of a coarse row held one value per lane (no wraparound)
     auto coarse_sums = [&](const CoarseRaw& R, S (&hA)[C], S (&hB)[C]) {
 #pragma unroll
@@ -577,8 +595,10 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
     };
     const bool det = a.dither == ENBL_DITHER_DET;
 
-    // blend + add the expansion e (per channel) + store of fine row Y
-    auto finish = [&](int Y, const FineRaw& F, const S (&e0)[C], const S (&e1)[C]) {
+    // blend + add the expansion e (per channel) + store of fine row Y; with
+    // OTF, ew / eb are the expansions of the white / black Gaussian levels 1
+    auto finish = [&](int Y, const FineRaw& F, const S (&e0)[C], const S (&e1)[C], const S (&ew0)[C],
+                      const S (&ew1)[C], const S (&eb0)[C], const S (&eb1)[C]) {
         const bool ok = Y < fh;
         M m0, m1;
         if constexpr (MASK8) {
@@ -592,13 +612,27 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
         }
         const double wc0 = weight(m0), wc1 = weight(m1);
         P o0[C], o1[C];
+        S bi0[C], bi1[C], wi0[C], wi1[C];
+        if constexpr (OTF) {
+            ISrc::unpack(ImgArgs{a.img, a.ipitch, a.ifrac}, F.v.b, bi0, bi1);
+            ISrc::unpack(ImgArgs{a.wimg, a.wipitch, a.ifrac}, F.v.w, wi0, wi1);
+        }
 #pragma unroll
         for (int ch = 0; ch < C; ++ch) {
-            S b0, b1, w0, w1;
-            Vec2<P>::split(F.b[ch], b0, b1);
-            Vec2<P>::split(F.w[ch], w0, w1);
-            o0[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc0, (P)w0, (P)b0)), e0[ch]));
-            o1[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc1, (P)w1, (P)b1)), e1[ch]));
+            P b0, b1, w0, w1;
+            if constexpr (OTF) {
+                b0 = Px::wrap(A::sub(bi0[ch], eb0[ch]));
+                b1 = Px::wrap(A::sub(bi1[ch], eb1[ch]));
+                w0 = Px::wrap(A::sub(wi0[ch], ew0[ch]));
+                w1 = Px::wrap(A::sub(wi1[ch], ew1[ch]));
+            } else {
+                S t0, t1, u0, u1;
+                Vec2<P>::split(F.v.b[ch], t0, t1);
+                Vec2<P>::split(F.v.w[ch], u0, u1);
+                b0 = (P)t0; b1 = (P)t1; w0 = (P)u0; w1 = (P)u1;
+            }
+            o0[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc0, w0, b0)), e0[ch]));
+            o1[ch] = Px::sat(A::add(Px::load(blend_px<P>(wc1, w1, b1)), e1[ch]));
         }
         if constexpr (!OUT) {
             const P* row = static_cast<const P*>(a.black) + (long)(ok ? Y : 0) * a.fpitch;
@@ -615,10 +649,12 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
             const bool u0 = owned && (wa0 || ba0);
             const bool u1 = owned && xb && (wa1 || ba1);
             IT v0[C], v1[C];
+            const bool hashed = det && std::is_integral<P>::value;
+            const uint32_t h0 = hashed ? dither_hash(a.seed, X, Y) : 0u, h1 = hashed ? dither_hash(a.seed, X + 1, Y) : 0u;
 #pragma unroll
             for (int ch = 0; ch < C; ++ch) {
-                v0[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, det, a.seed, X, Y, ch);
-                v1[ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, det, a.seed, X + 1, Y, ch);
+                v0[ch] = OutConv<P, IT>::conv(o0[ch], a.ifrac, det, h0, ch);
+                v1[ch] = OutConv<P, IT>::conv(o1[ch], a.ifrac, det, h1, ch);
             }
             const long y = ok ? Y : 0;
             const IT* irow = static_cast<const IT*>(a.img) + y * a.ipitch;
@@ -683,40 +719,69 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(FusedCollapseArgs a
         }
     };
 
-    S pA[C], pB[C], cA[C], cB[C];
+    // polyphase sums of three consecutive rows (p, c) and the raw row ring of
+    // one coarse level: the collapsed level l+1, and with OTF the white and
+    // black Gaussian levels 1
+    struct CoarseState {
+        S pA[C], pB[C], cA[C], cB[C];
+        CoarseRaw r[2];
+    };
+    constexpr int NS = OTF ? 3 : 1;
+    CoarseState cs[NS];
+    const void* cbase[3] = {a.coarse, a.wg1, a.bg1};
+    const long cpitch[3] = {a.cpitch, a.wg1pitch, a.bg1pitch};
+    const long cplane[3] = {a.cplane, a.wg1plane, a.bg1plane};
     FineRaw fr[4];
-    CoarseRaw cr[2];
-    {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
         CoarseRaw tp, tc;
-        fetch_coarse(i0 - 1, i0 > 0, tp);
-        fetch_coarse(i0, true, tc);
-        fetch_coarse(i0 + 1, true, cr[1]);
-        fetch_fine(2 * i0, true, fr[0]);
-        fetch_fine(2 * i0 + 1, true, fr[1]);
-        coarse_sums(tp, pA, pB);
+        fetch_level(cbase[k], cpitch[k], cplane[k], i0 - 1, i0 > 0, tp);
+        fetch_level(cbase[k], cpitch[k], cplane[k], i0, true, tc);
+        fetch_level(cbase[k], cpitch[k], cplane[k], i0 + 1, true, cs[k].r[1]);
+        coarse_sums(tp, cs[k].pA, cs[k].pB);
         if (i0 == 0) {
 #pragma unroll
-            for (int ch = 0; ch < C; ++ch) pA[ch] = pB[ch] = A::zero();
+            for (int ch = 0; ch < C; ++ch) cs[k].pA[ch] = cs[k].pB[ch] = A::zero();
         }
-        coarse_sums(tc, cA, cB);
+        coarse_sums(tc, cs[k].cA, cs[k].cB);
+    }
+    // With OTF the fine rows are fetched in their own step (no prefetch):
+    // the registers of a second pair of rows would cost a wave per SIMD.
+    constexpr bool PREFETCH = !OTF;
+    if constexpr (PREFETCH) {
+        fetch_fine(2 * i0, true, fr[0]);
+        fetch_fine(2 * i0 + 1, true, fr[1]);
     }
 
     auto step = [&](auto ph, int i) {
         constexpr int PH = decltype(ph)::value;
         constexpr int F0 = 2 * PH, F1 = 2 * PH + 1, N0 = 2 * (1 - PH), N1 = 2 * (1 - PH) + 1;
         const bool next = i + 1 < i1;
-        fetch_coarse(i + 2, next, cr[PH]);
-        fetch_fine(2 * i + 2, next, fr[N0]);
-        fetch_fine(2 * i + 3, next, fr[N1]);
-        const bool more = i + 1 < chh;
-        S nA[C], nB[C];
-        coarse_sums(cr[1 - PH], nA, nB);
-        S e00[C], e10[C], e01[C], e11[C];
-        expand2(i, more, pA, pB, cA, cB, nA, nB, e00, e10, e01, e11);
-        finish(2 * i, fr[F0], e00, e10);
-        finish(2 * i + 1, fr[F1], e01, e11);
 #pragma unroll
-        for (int ch = 0; ch < C; ++ch) { pA[ch] = cA[ch]; pB[ch] = cB[ch]; cA[ch] = nA[ch]; cB[ch] = nB[ch]; }
+        for (int k = 0; k < NS; ++k) fetch_level(cbase[k], cpitch[k], cplane[k], i + 2, next, cs[k].r[PH]);
+        if constexpr (PREFETCH) {
+            fetch_fine(2 * i + 2, next, fr[N0]);
+            fetch_fine(2 * i + 3, next, fr[N1]);
+        } else {
+            fetch_fine(2 * i, true, fr[F0]);
+            fetch_fine(2 * i + 1, true, fr[F1]);
+        }
+        const bool more = i + 1 < chh;
+        S e[NS][4][C];   // per level: rows 2i (pair A, B) and 2i+1 (pair A, B)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            S nA[C], nB[C];
+            coarse_sums(cs[k].r[1 - PH], nA, nB);
+            expand2(i, more, cs[k].pA, cs[k].pB, cs[k].cA, cs[k].cB, nA, nB, e[k][0], e[k][1], e[k][2], e[k][3]);
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) {
+                cs[k].pA[ch] = cs[k].cA[ch]; cs[k].pB[ch] = cs[k].cB[ch];
+                cs[k].cA[ch] = nA[ch]; cs[k].cB[ch] = nB[ch];
+            }
+        }
+        constexpr int KW = OTF ? 1 : 0, KB = OTF ? 2 : 0;
+        finish(2 * i, fr[F0], e[0][0], e[0][1], e[KW][0], e[KW][1], e[KB][0], e[KB][1]);
+        finish(2 * i + 1, fr[F1], e[0][2], e[0][3], e[KW][2], e[KW][3], e[KB][2], e[KB][3]);
     };
 
     int i = i0;
@@ -767,8 +832,16 @@ void launch_col_t(FusedCollapseArgs a, int C, hipStream_t s)
     const int waves_x = (a.cw + kOwned - 1) / kOwned + 1;
     if (a.rows <= 0) a.rows = strip_rows_f(waves_x, a.ch);
     dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
-    if (C == 1) hipLaunchKernelGGL((collapse_level_kernel<P, M, 1, I, MASK8, WIDE>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((collapse_level_kernel<P, M, 3, I, MASK8, WIDE>), grid, dim3(256), 0, s, a);
+    constexpr bool CAN_OTF = !std::is_same<I, void>::value && std::is_integral<P>::value;
+    if (CAN_OTF && a.otf) {
+        if constexpr (CAN_OTF) {
+            if (C == 1) hipLaunchKernelGGL((collapse_level_kernel<P, M, 1, I, MASK8, WIDE, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((collapse_level_kernel<P, M, 3, I, MASK8, WIDE, true>), grid, dim3(256), 0, s, a);
+        }
+        return;
+    }
+    if (C == 1) hipLaunchKernelGGL((collapse_level_kernel<P, M, 1, I, MASK8, WIDE, false>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((collapse_level_kernel<P, M, 3, I, MASK8, WIDE, false>), grid, dim3(256), 0, s, a);
 }
 
 // Output rows aligned for whole-pair stores and 8-bit planes (mask, alphas)
@@ -781,6 +854,9 @@ template <class I> bool collapse_wide(const FusedCollapseArgs& a, bool mask8)
         if (!even(a.walpha, a.wapitch) || !even(a.balpha, a.bapitch)) return false;
         if (sizeof(I) == 2 && (((uintptr_t)a.img & 3) != 0 || (a.ipitch & 1) != 0)) return false;
         if (sizeof(I) == 1 && !even(a.img, a.ipitch)) return false;
+        // on-the-fly Laplacians read the white image with the same word loads
+        if (a.otf && sizeof(I) == 2 && (((uintptr_t)a.wimg & 3) != 0 || (a.wipitch & 1) != 0)) return false;
+        if (a.otf && sizeof(I) == 1 && !even(a.wimg, a.wipitch)) return false;
     }
     return true;
 }

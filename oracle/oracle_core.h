@@ -375,12 +375,25 @@ inline int check_expand_sizes(int sw, int sh, int dw, int dh)
 // ---------------------------------------------------------------------------
 // Converters.  src/fixmath.h:116-206 and :209-358.
 // ---------------------------------------------------------------------------
-inline uint64_t splitmix64(uint64_t x)
+// Deterministic dither draw shared with the GPU (the reference draws GSL
+// MT19937 seeded from clock(), src/mersenne.cc:79-94): one 32-bit hash per
+// pixel keyed by (seed, x, y), two rounds of the murmur3 32-bit finaliser;
+// channel c takes bits [10c, 10c + frac - 2] as its offset on the
+// reference's grid (r & half_m1) + quarter (src/fixmath.h:246-264).
+inline uint32_t fmix32(uint32_t h)
 {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
+}
+
+inline int dither_draw(uint64_t seed, int x, int y, int c, int frac_bits)
+{
+    const uint32_t s = (uint32_t)seed ^ (uint32_t)(seed >> 32);
+    const uint32_t h = fmix32(s ^ fmix32(((uint32_t)y * 0x9E3779B1u) ^ (uint32_t)x));
+    return (int)((h >> (10 * c)) & ((1u << (frac_bits - 1)) - 1u)) + (1 << (frac_bits - 2));
 }
 
 
@@ -436,7 +449,7 @@ void from_pyramid_int(int frac, int C, const P* pl0, int w, int h, long pp, long
                 int32_t r;
                 if (f >= quarter && f < three_quarter) {
                     if (dither == ORC_DITHER_DET) {
-                        r = orc_dither_rand(seed, x, y, c, frac) <= f ? q + 1 : q;
+                        r = dither_draw(seed, x, y, c, frac) <= f ? q + 1 : q;
                     } else {
                         r = f >= (1 << (frac - 1)) ? q + 1 : q;   // round half up
                     }

@@ -35,11 +35,7 @@ using namespace orc;
 
 extern "C" int orc_dither_rand(uint64_t seed, int x, int y, int c, int frac_bits)
 {
-    const uint64_t key = seed ^ ((uint64_t)(uint32_t)y << 32) ^ ((uint64_t)(uint32_t)x << 2) ^ (uint64_t)(uint32_t)c;
-    const uint64_t r = orc::splitmix64(key);
-    const int half_m1 = (1 << (frac_bits - 1)) - 1;
-    const int quarter = 1 << (frac_bits - 2);
-    return (int)(r & (uint64_t)half_m1) + quarter;
+    return orc::dither_draw(seed, x, y, c, frac_bits);
 }
 
 
