@@ -398,6 +398,10 @@ constexpr CmpNet make_net(int N)
 template <int NN> struct SortNet { static constexpr CmpNet net = make_net(NN); };
 
 template <class T> __device__ __forceinline__ void cas(T& x, T& y) { const T lo = min(x, y), hi = max(x, y); x = lo; y = hi; }
+template <class T> __device__ __forceinline__ void kmin(T& m, T v) { m = min(m, v); }
+template <class T> __device__ __forceinline__ void kmax(T& m, T v) { m = max(m, v); }
+template <> __device__ __forceinline__ void kmin<float>(float& m, float v) { m = __builtin_fminf(m, v); }
+template <> __device__ __forceinline__ void kmax<float>(float& m, float v) { m = __builtin_fmaxf(m, v); }
 template <> __device__ __forceinline__ void cas<float>(float& x, float& y)
 {
     const float lo = __builtin_fminf(x, y), hi = __builtin_fmaxf(x, y);
@@ -469,6 +473,7 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
         spent[i] = tab[i];
         splog[i] = tab[NN + 1 + i];
     }
+    const double sdist = tab[2 * (NN + 1) + (NN + 1) * (NN + 1)];
     const bool tile_full = __syncthreads_or(unset) == 0;
     const int lx = tid % kEntTW, ly = tid / kEntTW;
     const int x = x0 + lx, y = y0 + ly;
@@ -493,25 +498,42 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
                 for (int dy = 0; dy < N; ++dy)
 #pragma unroll
                     for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][ly + dy][lx + dx];
-                constexpr CmpNet net = SortNet<NN>::net;
+                // A window of one value is one bin (entropy 0): no sort.
+                K mn = v[0], mx = v[0];
 #pragma unroll
-                for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
-                // Bin boundaries where the sorted keys step up; the bin that
-                // ends at a boundary k started at the previous boundary st.
-                // Non-boundaries add spent[0] = 0.0, which leaves the sum
-                // unchanged bit for bit, so the table reads do not wait on one
-                // another.
-                int st = 0;
+                for (int k = 1; k < NN; ++k) { kmin(mn, v[k]); kmax(mx, v[k]); }
                 bins = 1;
+                if (mn < mx) {
+                    constexpr CmpNet net = SortNet<NN>::net;
 #pragma unroll
-                for (int k = 1; k < NN; ++k) {
-                    const bool b = v[k - 1] < v[k];
-                    const int len = k - st;
-                    st = b ? k : st;
-                    bins += b ? 1 : 0;
-                    acc += spent[b ? len : 0];
+                    for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
+                    bool distinct = true;
+#pragma unroll
+                    for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
+                    if (distinct) {
+                        // NN bins of one sample: the table's one-sample term
+                        // added NN times in order, precomputed
+                        bins = NN;
+                        acc = sdist;
+                    } else {
+                        // Bin boundaries where the sorted keys step up; the
+                        // bin that ends at a boundary k started at the
+                        // previous boundary st.  Non-boundaries add
+                        // spent[0] = 0.0, which leaves the sum unchanged bit
+                        // for bit, so the table reads do not wait on one
+                        // another.
+                        int st = 0;
+#pragma unroll
+                        for (int k = 1; k < NN; ++k) {
+                            const bool b = v[k - 1] < v[k];
+                            const int len = k - st;
+                            st = b ? k : st;
+                            bins += b ? 1 : 0;
+                            acc += spent[b ? len : 0];
+                        }
+                        acc += spent[NN - st];
+                    }
                 }
-                acc += spent[NN - st];
             } else {
                 // Cut window: insertion sort of the samples under alpha; the
                 // general p*log(p) table for `cnt` samples.

@@ -137,7 +137,9 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, int K, const enb
     }
     if (p->w_entropy > 0.0) {
         const int full = p->entropy_window * p->entropy_window;
-        std::vector<double> tab((size_t)(2 * (full + 1) + (full + 1) * (full + 1)), 0.0);
+        // pent, plog, the general table, then the full-window sum of `full`
+        // one-sample terms in order (a window of distinct keys)
+        std::vector<double> tab((size_t)(2 * (full + 1) + (full + 1) * (full + 1) + 1), 0.0);
         double* pent = tab.data();
         double* plog = pent + (full + 1);
         double* pgen = plog + (full + 1);
@@ -151,6 +153,9 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, int K, const enb
                 const double pr = k / (double)t;
                 pgen[t * (full + 1) + k] = pr * log(pr);
             }
+        double sd = 0.0;
+        for (int i = 0; i < full; ++i) sd += pent[1];
+        tab[(size_t)(2 * (full + 1) + (full + 1) * (full + 1))] = sd;
         void* d;
         rc = scratch(ctx, 3, tab.size() * sizeof(double), &d);
         if (rc) return rc;

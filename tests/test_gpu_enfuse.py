@@ -169,3 +169,30 @@ def test_hard_mask_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype, K):
             np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
         else:
             np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("ewin", [3, 5, 7])
+def test_entropy_window_kinds_bit_exact(enbl, gpu_ctx, oracle, itype, ewin):
+    """Full windows of one value (one bin), of distinct values (as many bins
+    as samples) and with ties, side by side under full alpha; the entropy
+    weight alone, compared bit for bit."""
+    rng = np.random.default_rng(1100 + itype + ewin)
+    h, w = 80, 210
+    m = {U8: 255, U16: 65535, F32: 1.0}[itype]
+    img = np.empty((h, w, 3), np.float64)
+    img[:, :70] = 0.37 * m                                                       # flat
+    img[:, 70:140] = rng.permutation(np.linspace(0.05, 0.95, h * 70 * 3)).reshape(h, 70, 3) * m  # distinct
+    img[:, 140:] = np.round(rng.random((h, 70, 3)) * 6) / 6 * m                  # ties
+    img = img.astype(np.float32) if itype == F32 else np.rint(img).astype(np.uint8 if itype == U8 else np.uint16)
+    a = np.full((h, w), 255, np.uint8)
+    kw = dict(w_exposure=0.0, w_entropy=1.0, entropy_window=ewin)
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(img).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"entropy window kinds itype={itype} n={ewin}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
