@@ -255,11 +255,32 @@ int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* i
         gbuf[0] = g;
         gbuf[1] = (char*)g + gbytes;
     }
+    // Float images: the log transform of level 0 is made once per sample by a
+    // pointwise pass into a double scratch plane; inside the fused pass every
+    // fine row would be transformed twice (its horizontal sums and its
+    // Laplacian), and the transform costs far more than the extra bytes.
+    const void* g0 = nullptr;
+    if (l0 == 0 && itype == ENBL_F32) {
+        const Level& F = p->lv[0];
+        void* g;
+        int rc = scratch(ctx, 23, round_up((size_t)F.plane * p->channels * es, 256), &g);
+        if (rc) return rc;
+        char name[64];
+        snprintf(name, sizeof name, "to_pyramid_%s_c%d", pname(p->ptype), p->channels);
+        Timed t(ctx, name, (double)F.w * F.h * p->channels * (ptype_size(itype) + (double)es));
+        rc = launch_to_pyramid(itype, p->ptype, frac, p->channels, img, ipitch, F.w, F.h, g, F.pitch, F.plane,
+                               ctx->stream);
+        if (rc) return fail(rc, "copyToPyramidImage: unsupported image/pyramid type pair");
+        rc = launch_status(name);
+        if (rc) return rc;
+        g0 = g;
+    }
     for (int l = l0; l < s; ++l) {
         const Level& F = p->lv[l];
         const Level& Cl = p->lv[l + 1];
         FusedLapArgs a;
-        if (l == 0) { a.src = img; a.spitch = ipitch; a.splane = 0; a.salpha = alpha; a.sapitch = apitch; }
+        if (l == 0 && g0) { a.src = g0; a.spitch = F.pitch; a.splane = F.plane; a.salpha = alpha; a.sapitch = apitch; }
+        else if (l == 0) { a.src = img; a.spitch = ipitch; a.splane = 0; a.salpha = alpha; a.sapitch = apitch; }
         else { a.src = l == 1 && g1 ? g1 : gbuf[(l - 1) & 1]; a.spitch = F.pitch; a.splane = F.plane; a.salpha = abuf[(l - 1) & 1]; a.sapitch = apl; }
         a.frac = frac;
         a.fw = F.w; a.fh = F.h;
@@ -278,11 +299,12 @@ int fused_laplacian_levels(enbl_context* ctx, int itype, int frac, const void* i
         // alpha read once, the Laplacian level and the coarse level with its
         // alpha written once
         const double nF = (double)F.w * F.h, nC = (double)Cl.w * Cl.h;
-        const double srcC = l == 0 ? (double)p->channels * ptype_size(itype) : esC;
+        const bool from_image = l == 0 && !g0;
+        const double srcC = from_image ? (double)p->channels * ptype_size(itype) : esC;
         Timed t(ctx, name, (esC + 1.0) * (nF + nC) + esC * (nC + 2.0 * nF) +
-                               (l == 0 ? nF * p->channels * (ptype_size(itype) + (double)es) : 0.0),
+                               (from_image ? nF * p->channels * (ptype_size(itype) + (double)es) : 0.0),
                 nF * (srcC + 1.0) + nF * esC + nC * (esC + 1.0));
-        int rc = launch_laplacian_level(p->ptype, l == 0 ? itype : 0, a, p->channels, ctx->stream);
+        int rc = launch_laplacian_level(p->ptype, from_image ? itype : 0, a, p->channels, ctx->stream);
         if (rc) return fail(rc, "laplacian level: unsupported type");
         rc = launch_status(name);
         if (rc) return rc;
