@@ -21,7 +21,7 @@ ENBL_E_OOM = -5
 ENBL_E_HIP = -6
 
 I16, I32, F64 = 0, 1, 2
-U8, U16, F32 = 10, 11, 12
+U8, U16, F32, U32 = 10, 11, 12, 13
 DITHER_NONE, DITHER_DET = 0, 1
 KERNELS_AUTO, KERNELS_DIRECT, KERNELS_STREAMING = 0, 1, 2
 
@@ -74,7 +74,42 @@ _SIGS = {
     "enbl_enblend_pair": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                    _vp, _c_long, _vp, _c_long, _vp, _c_long, _vp, _c_long,
                                    _vp, _c_long, _c_int, ctypes.c_uint64]),
+    "enbl_assemble": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp,
+                               _vp, _c_long, _vp, _c_long, _vp]),
+    "enbl_inspect_overlap": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
+    "enbl_mask_bounds": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _vp]),
+    "enbl_enblend_step": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                   _vp, _c_long, _vp, _c_long, _vp, _vp, _c_long, _vp, _c_long, _vp,
+                                   _c_int, ctypes.c_uint64, _vp]),
+    "enbl_enblend_layers": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int,
+                                     _c_int, _c_int, _c_int, ctypes.c_uint64, _vp, _c_long, _vp, _c_long, _vp,
+                                     _vp]),
+    "enbl_checkpoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int, _vp,
+                                 _c_long, _vp]),
 }
+
+
+class Rect(ctypes.Structure):
+    """enbl_rect: vigra::Rect2D [x0, x1) x [y0, y1)."""
+    _fields_ = [("x0", _c_int), ("y0", _c_int), ("x1", _c_int), ("y1", _c_int)]
+
+    def tuple(self):
+        return (self.x0, self.y0, self.x1, self.y1)
+
+
+class Layer(ctypes.Structure):
+    """enbl_layer: one decoded input image on the union canvas."""
+    _fields_ = [("data", _vp), ("pitch", _c_long), ("w", _c_int), ("h", _c_int), ("x", _c_int), ("y", _c_int),
+                ("ftype", _c_int), ("bands", _c_int)]
+
+
+def layer_table(layers):
+    """ctypes array of Layer for (device tensor (h, w, bands), x, y, ftype)."""
+    tab = (Layer * max(1, len(layers)))()
+    for i, (t, x, y, ft) in enumerate(layers):
+        h, w, bands = t.shape
+        tab[i] = Layer(t.data_ptr(), w * bands, w, h, x, y, ft, bands)
+    return tab
 
 _lib = None
 

@@ -43,6 +43,7 @@ extern "C" {
 #define ENBL_U8 10
 #define ENBL_U16 11
 #define ENBL_F32 12
+#define ENBL_U32 13   /* only as the component type of enbl_checkpoint's output */
 
 /* dither modes of the pyramid-to-image conversion */
 #define ENBL_DITHER_NONE 0 /* round half up inside the dither band */
@@ -343,6 +344,94 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int channels, int w, int h, 
                       void* black, long bpitch, uint8_t* black_alpha, long bapitch,
                       const uint8_t* mask, long mpitch,
                       int dither_mode, uint64_t seed);
+
+/* ---- driver around the blend: import, assembly, overlap, ROI, export ---- */
+
+/* vigra::Rect2D: [x0, x1) x [y0, y1), empty when x1 <= x0 or y1 <= y0. */
+typedef struct enbl_rect { int x0, y0, x1, y1; } enbl_rect;
+
+/* One decoded input image as vigra::importImageAlpha reads it: interleaved
+ * bands of the file's pixel type (ENBL_U8, ENBL_U16 or ENBL_F32), `channels`
+ * image bands followed by the alpha band when bands == channels + 1.  Its
+ * upper-left corner sits at (x, y) of the union canvas. */
+typedef struct enbl_layer {
+    const void* data;   /* device pointer */
+    long pitch;         /* elements per row */
+    int w, h, x, y;
+    int ftype;
+    int bands;
+} enbl_layer;
+
+/* Replaces assemble() with import() (src/assemble.h:231-444): the first
+ * layer with taken[i] == 0 goes onto a zeroed canvas of union_w x union_h
+ * (all of its rectangle); unless one_at_a_time, every later untaken layer
+ * whose thresholded alpha meets none of the canvas alpha is copied where its
+ * alpha is set, in list order.  The alpha band becomes 255 where
+ * T(max/2) <= a <= T(max) (T = the image component type, max = the file
+ * type's range), else 0; no alpha band means 255.  Samples of a narrower
+ * integer file are mapped to the image's range as vigra::linearRangeMapping
+ * does (8-bit into 16-bit or float, 16-bit into float).  taken[] is updated;
+ * *bb receives the bounding box of the canvas alpha.  Returns the number of
+ * layers placed (0: every layer was taken, the canvas is untouched) or a
+ * negative status.  Synchronises the context's stream. */
+int enbl_assemble(enbl_context* ctx, int itype, int channels, int union_w, int union_h, int nlayers,
+                  const enbl_layer* layers, int one_at_a_time, int* taken,
+                  void* image, long ipitch, uint8_t* alpha, long apitch, enbl_rect* bb);
+
+/* Replaces inspectOverlap (src/bounds.h:41-80) over w x h alpha windows:
+ * 0 NoOverlap, 1 PartialOverlap, 2 CompleteOverlap in *overlap (enum Overlap,
+ * src/common.h:193).  Synchronises. */
+int enbl_inspect_overlap(enbl_context* ctx, int w, int h, const uint8_t* alpha1, long p1,
+                         const uint8_t* alpha2, long p2, int* overlap);
+
+/* The transition-line box of maskBounds (src/mask.h:520-560), relative to the
+ * mask; *found = 0 when the mask holds one value only.  Synchronises. */
+int enbl_mask_bounds(enbl_context* ctx, int w, int h, const uint8_t* mask, long mpitch, enbl_rect* bb,
+                     int* found);
+
+/* One iteration of enblendMain's blending loop (src/enblend.h:127-545) for
+ * the fine nearest-feature-transform seam mask without optimisation
+ * (--fine-mask --no-optimize): union and intersection boxes, inspectOverlap
+ * (a redundant white image leaves black as it is: *outcome 1; images that do
+ * not overlap, with exact_levels == 0, are combined by copying: *outcome 2),
+ * the NFT mask over the union box (norm 1 or 2), maskBounds, roiBounds with
+ * minimum_levels and exact_levels (--levels), then the blend step of
+ * enbl_enblend_pair on the ROI (*outcome 0).  black/black_alpha and *black_bb
+ * are updated as the reference updates blackPair and blackBB.  Canvases are
+ * union_w x union_h; *black_bb and *white_bb are the bounding boxes of the
+ * two alphas, as enbl_assemble returns them.  A seam mask with wraparound (wraparound set and the
+ * union box as wide as the canvas) is ENBL_E_UNSUPPORTED.  Synchronises. */
+int enbl_enblend_step(enbl_context* ctx, int itype, int channels, int union_w, int union_h, int wraparound,
+                      int exact_levels, int minimum_levels, int norm,
+                      void* black, long bpitch, uint8_t* black_alpha, long bapitch, enbl_rect* black_bb,
+                      const void* white, long wpitch, const uint8_t* white_alpha, long wapitch,
+                      const enbl_rect* white_bb, int dither_mode, uint64_t seed, int* outcome);
+
+/* enblendMain's whole loop (src/enblend.h:56-594) over n decoded layers on a
+ * union_w x union_h canvas, with the fine NFT seam mask and no optimiser:
+ * the first enbl_assemble becomes black, every later one white, each white
+ * is folded into black by enbl_enblend_step with dither seed seed + m for
+ * the m-th step.  pre_assemble is --pre-assemble (assemble non-overlapping
+ * layers; off: one layer at a time).  The white canvas lives in the
+ * context's scratch.  counts (optional) receives the number of steps that
+ * blended, that found white redundant and that copied without blending. */
+int enbl_enblend_layers(enbl_context* ctx, int itype, int channels, int union_w, int union_h, int nlayers,
+                        const enbl_layer* layers, int pre_assemble, int wraparound, int exact_levels,
+                        int minimum_levels, int norm, int dither_mode, uint64_t seed,
+                        void* black, long bpitch, uint8_t* black_alpha, long bapitch, enbl_rect* black_bb,
+                        int* counts);
+
+/* Replaces checkpoint()'s conversion before the export call (src/assemble.h:
+ * 130-224): for an output file type otype (ENBL_U8, ENBL_U16, ENBL_F32), the
+ * image as the reference hands it to vigra::exportImageAlpha -- unchanged when
+ * the ranges agree, narrowed by linearRangeMapping in the image's own type
+ * when the image range covers the output range, or a float image rescaled
+ * into UInt32 -- interleaved with the alpha band (alpha != 0 -> the
+ * component type's alpha maximum).  out holds channels + 1 bands of the type
+ * written to *out_ctype (the image type, or ENBL_U32).  An 8-bit image written
+ * as 16-bit is ENBL_E_UNSUPPORTED (the reference's conversion overflows). */
+int enbl_checkpoint(enbl_context* ctx, int itype, int channels, int w, int h, const void* image, long ipitch,
+                    const uint8_t* alpha, long apitch, int otype, void* out, long opitch, int* out_ctype);
 
 #ifdef __cplusplus
 }

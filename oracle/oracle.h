@@ -30,7 +30,7 @@ extern "C" {
 #endif
 
 enum { ORC_I16 = 0, ORC_I32 = 1, ORC_F64 = 2 };
-enum { ORC_U8 = 10, ORC_U16 = 11, ORC_F32 = 12 };
+enum { ORC_U8 = 10, ORC_U16 = 11, ORC_F32 = 12, ORC_U32 = 13 };
 enum { ORC_DITHER_NONE = 0, ORC_DITHER_DET = 1 };
 
 /* reduce with alpha (src/pyramid.h:166-573) when alpha != NULL, otherwise
@@ -145,6 +145,37 @@ int orc_nft_mask(const uint8_t* white, long wpitch, const uint8_t* black, long b
 /* The distance transform alone: distance of every pixel to the nearest
  * non-zero pixel of src (float, sqrt for the Euclidean norm). */
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);
+
+/* enblend's driver around the blend (oracle_driver.cpp). */
+typedef struct orc_layer {
+    const void* data;   /* decoded file samples, interleaved bands */
+    long pitch;         /* elements per row */
+    int w, h, x, y;     /* size and position on the union canvas */
+    int ftype;          /* ORC_U8 / ORC_U16 / ORC_F32: the file's pixel type */
+    int bands;          /* channels, or channels + 1 with the alpha band last */
+} orc_layer;
+/* assemble + import (src/assemble.h:231-444): returns the number of layers
+ * placed on the zeroed canvas (0 when every layer is taken); bb = x0,y0,x1,y1
+ * of the canvas alpha. */
+int orc_assemble(int itype, int channels, int union_w, int union_h, int n, const orc_layer* layers,
+                 int one_at_a_time, int* taken, void* image, uint8_t* alpha, int* bb);
+/* inspectOverlap (src/bounds.h:41-80): 0 none, 1 partial, 2 complete. */
+int orc_inspect_overlap(const uint8_t* a1, long p1, const uint8_t* a2, long p2, int w, int h);
+/* maskBounds' transition box relative to the mask (src/mask.h:520-560);
+ * returns 0 when the mask has no transition. */
+int orc_mask_bounds(const uint8_t* mask, int w, int h, int* bb);
+/* One iteration of enblendMain's loop (src/enblend.h:127-545) with the fine
+ * NFT mask: canvases packed (pitch = union width); outcome 0 blended,
+ * 1 white redundant, 2 copied without blending; info (optional) = mBB, roiBB,
+ * levels. */
+int orc_enblend_step(int itype, int channels, int union_w, int union_h, int wrap, int exact_levels,
+                     int min_levels, int norm, void* black, uint8_t* balpha, int* black_bb,
+                     const void* white, const uint8_t* walpha, const int* white_bb, int dither,
+                     uint64_t seed, int* outcome, int* info);
+/* checkpoint (src/assemble.h:130-224): interleaved channels + alpha band of
+ * the type written to *out_ctype. */
+int orc_checkpoint(int itype, int channels, int w, int h, const void* img, const uint8_t* alpha, int otype,
+                   void* out, int* out_ctype);
 
 int orc_num_threads(void);
 

@@ -11,9 +11,9 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 I16, I32, F64 = 0, 1, 2
-U8, U16, F32 = 10, 11, 12
+U8, U16, F32, U32 = 10, 11, 12, 13
 NP_P = {I16: np.int16, I32: np.int32, F64: np.float64}
-NP_I = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+NP_I = {U8: np.uint8, U16: np.uint16, F32: np.float32, U32: np.uint32}
 
 _vp, _i, _l = ctypes.c_void_p, ctypes.c_int, ctypes.c_long
 _SIGS = {
@@ -36,7 +36,29 @@ _SIGS = {
     "orc_diff_reduce_T": [_vp, _i, _i, _vp, _i, _i],
     "orc_nft_mask": [_vp, _l, _vp, _l, _i, _i, _i, _vp, _l],
     "orc_distance_transform": [_vp, _l, _i, _i, _i, _vp, _l],
+    "orc_assemble": [_i, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp],
+    "orc_inspect_overlap": [_vp, _l, _vp, _l, _i, _i],
+    "orc_mask_bounds": [_vp, _i, _i, _vp],
+    "orc_enblend_step": [_i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, ctypes.c_uint64,
+                         _vp, _vp],
+    "orc_checkpoint": [_i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
 }
+
+
+class Layer(ctypes.Structure):
+    """orc_layer / enbl_layer (same layout): one decoded input image."""
+    _fields_ = [("data", ctypes.c_void_p), ("pitch", ctypes.c_long), ("w", ctypes.c_int), ("h", ctypes.c_int),
+                ("x", ctypes.c_int), ("y", ctypes.c_int), ("ftype", ctypes.c_int), ("bands", ctypes.c_int)]
+
+
+def layer_table(layers, ptr=lambda a: a.ctypes.data):
+    """ctypes array of Layer for (samples (h, w, bands), x, y, ftype) tuples;
+    `ptr` gives the address of the samples (host arrays by default)."""
+    tab = (Layer * max(1, len(layers)))()
+    for i, (a, x, y, ft) in enumerate(layers):
+        h, w, bands = a.shape
+        tab[i] = Layer(ptr(a), w * bands, w, h, x, y, ft, bands)
+    return tab
 
 EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE = range(5)
 
@@ -282,6 +304,73 @@ def _nft_mask(self, white, black, norm=2):
     return mask, rc
 
 
+def _assemble(self, layers, itype, C, uw, uh, taken=None, one_at_a_time=False):
+    """(placed, image (uh, uw, C), alpha (uh, uw), bb, taken) of one
+    assemble() call over (samples, x, y, ftype) layers."""
+    layers = [(np.ascontiguousarray(a), x, y, ft) for a, x, y, ft in layers]
+    tab = layer_table(layers)
+    tk = np.zeros(len(layers), np.int32) if taken is None else np.array(taken, np.int32)
+    img = np.empty((uh, uw, C), NP_I[itype])
+    al = np.empty((uh, uw), np.uint8)
+    bb = np.zeros(4, np.int32)
+    n = self.lib.orc_assemble(itype, C, uw, uh, len(layers), tab, int(one_at_a_time), _p(tk), _p(img), _p(al),
+                              _p(bb))
+    if n < 0:
+        raise ValueError(f"orc_assemble rc={n}")
+    return n, img, al, tuple(int(v) for v in bb), tk
+
+
+def _inspect_overlap(self, a1, a2):
+    a1 = np.ascontiguousarray(a1, np.uint8)
+    a2 = np.ascontiguousarray(a2, np.uint8)
+    h, w = a1.shape
+    return self.lib.orc_inspect_overlap(_p(a1), w, _p(a2), w, w, h)
+
+
+def _mask_bounds(self, mask):
+    mask = np.ascontiguousarray(mask, np.uint8)
+    h, w = mask.shape
+    bb = np.zeros(4, np.int32)
+    ok = self.lib.orc_mask_bounds(_p(mask), w, h, _p(bb))
+    return bool(ok), tuple(int(v) for v in bb)
+
+
+def _enblend_step(self, black, balpha, black_bb, white, walpha, white_bb, itype, wrap=False, exact_levels=0,
+                  min_levels=1, norm=2, dither=1, seed=0):
+    """One blending-loop iteration on union canvases: returns (rc, outcome,
+    black, black_alpha, black_bb, info)."""
+    uh, uw, C = black.shape
+    b = np.array(black, NP_I[itype], copy=True)
+    ba = np.array(balpha, np.uint8, copy=True)
+    white = np.ascontiguousarray(white, NP_I[itype])
+    walpha = np.ascontiguousarray(walpha, np.uint8)
+    bbb = np.array(black_bb, np.int32)
+    wbb = np.array(white_bb, np.int32)
+    outcome = np.zeros(1, np.int32)
+    info = np.zeros(9, np.int32)
+    rc = self.lib.orc_enblend_step(itype, C, uw, uh, int(wrap), exact_levels, min_levels, norm, _p(b), _p(ba),
+                                   _p(bbb), _p(white), _p(walpha), _p(wbb), dither, seed, _p(outcome), _p(info))
+    return rc, int(outcome[0]), b, ba, tuple(int(v) for v in bbb), tuple(int(v) for v in info)
+
+
+def _checkpoint(self, img, alpha, itype, otype):
+    """(out (h, w, C + 1), component type) handed to the exporter."""
+    h, w, C = img.shape
+    img = np.ascontiguousarray(img, NP_I[itype])
+    alpha = np.ascontiguousarray(alpha, np.uint8)
+    out = np.zeros((h, w, C + 1), np.uint32 if itype == F32 and otype != F32 else NP_I[itype])
+    ct = np.zeros(1, np.int32)
+    rc = self.lib.orc_checkpoint(itype, C, w, h, _p(img), _p(alpha), otype, _p(out), _p(ct))
+    if rc:
+        raise ValueError(f"orc_checkpoint rc={rc}")
+    return out, int(ct[0])
+
+
+Oracle.assemble = _assemble
+Oracle.inspect_overlap = _inspect_overlap
+Oracle.mask_bounds = _mask_bounds
+Oracle.enblend_step = _enblend_step
+Oracle.checkpoint = _checkpoint
 Oracle.distance_transform = _distance_transform
 Oracle.nft_mask = _nft_mask
 Oracle.diff_forward = _diff_forward
