@@ -19,6 +19,7 @@ algorithm, OpenMP pointwise stages, single-threaded SKIPSM passes) on the
 host cores for the same workload.
 """
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
@@ -351,6 +352,89 @@ def enfuse_c0(args, enbl, scenes, np, torch, cpu=True):
     return res
 
 
+def enblend_layers_c1(args, enbl, scenes, np, torch, cpu=True):
+    """The whole enblend run of the headline scene from decoded input layers:
+    2 interleaved RGBA uint16 images of 4096x3072 at x = 0 and x = 3072 of a
+    7168x3072 canvas (alpha bands 0 / 65535), one at a time (no
+    pre-assembly), the fine NFT seam mask without optimiser, then the
+    checkpoint conversion for a 16-bit output.  One step = enbl_enblend_layers
+    (import with alpha thresholds, the two assemblies, overlap inspection,
+    seam mask, mask bounds, ROI and levels, the blend step) + enbl_checkpoint;
+    layers already resident in HBM.  Parity: the oracle's assemble + step +
+    checkpoint on the same layers, bit for bit."""
+    white, wa, black, ba, _ = scenes.enblend_pair(seed=1)
+    H, W = wa.shape
+    wimg = 4096
+    off = W - wimg
+
+    def rgba(img, a, x0):
+        out = np.empty((H, wimg, 4), np.uint16)
+        out[..., :3] = img[:, x0:x0 + wimg]
+        out[..., 3] = a[:, x0:x0 + wimg].astype(np.uint16) * 257
+        return out
+
+    host_layers = [(rgba(black, ba, 0), 0, 0, enbl.U16), (rgba(white, wa, off), off, 0, enbl.U16)]
+    dev = torch.device("cuda:0")
+    dl = [(torch.from_numpy(a).to(dev), x, y, ft) for a, x, y, ft in host_layers]
+    tab = enbl.layer_table(dl)
+    ctx = enbl.Context(0)
+    L = enbl.lib()
+    d_b = torch.empty((H, W * 3), dtype=torch.uint16, device=dev)
+    d_ba = torch.empty((H, W), dtype=torch.uint8, device=dev)
+    d_out = torch.empty((H, W * 4), dtype=torch.uint16, device=dev)
+    bb = enbl.Rect()
+    counts = (ctypes.c_int * 3)()
+    ct = ctypes.c_int(-1)
+    vp = ctypes.c_void_p
+
+    def run():
+        enbl.check(L.enbl_enblend_layers(ctx.h, enbl.U16, 3, W, H, 2, tab, 0, 0, 0, 1, 2, enbl.DITHER_DET, 1,
+                                         vp(d_b.data_ptr()), W * 3, vp(d_ba.data_ptr()), W, ctypes.byref(bb),
+                                         counts))
+        enbl.check(L.enbl_checkpoint(ctx.h, enbl.U16, 3, W, H, vp(d_b.data_ptr()), W * 3, vp(d_ba.data_ptr()), W,
+                                     enbl.U16, vp(d_out.data_ptr()), W * 4, ctypes.byref(ct)))
+
+    print("bench: enblend from decoded layers (assembly, seam mask, blend, checkpoint)", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    run()
+    ctx.sync()
+    got = d_out.cpu().numpy()
+    for _ in range(args.warmup):
+        run()
+    ctx.sync()
+    steps = max(5, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    run()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    ctx.close()
+    mpix = 2 * wimg * H / 1e6
+    res = {"workload": "enblend of 2 decoded RGBA uint16 layers 4096x3072 on a 7168x3072 canvas: import + "
+                       "assembly, overlap, NFT seam mask, mask bounds, ROI, 10-level blend, 16-bit checkpoint",
+           "input_mpix": round(mpix, 3), "ms_per_step": round(ms, 3), "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+           "steps": steps, "blend_steps": list(counts), "union_box": bb.tuple(), "kernels": _kernels(rep)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_binding import U16, load_oracle
+        o = load_oracle()
+        t0 = time.perf_counter()
+        taken = [0, 0]
+        _, bimg, bal, bbb, taken = o.assemble(host_layers, U16, 3, W, H, taken, True)
+        _, wimg_, wal, wbb, taken = o.assemble(host_layers, U16, 3, W, H, taken, True)
+        rc, _, bimg, bal, bbb, _ = o.enblend_step(bimg, bal, bbb, wimg_, wal, wbb, U16, seed=1)
+        want, _ = o.checkpoint(bimg, bal, U16, U16)
+        cpu_s = time.perf_counter() - t0
+        res["parity"] = bool(rc == 0 and np.array_equal(got.reshape(H, W, 4), want))
+        res["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": o.lib.orc_num_threads(),
+                               "kind": "port", "sample": f"full workload, one run ({cpu_s:.2f} s)"}
+    return res
+
+
 C2_EVS = [k - 3.5 for k in range(8)]
 
 
@@ -428,10 +512,13 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="enblend",
-                    choices=["enblend", "enfuse-c0", "enfuse-c2", "batch-c3", "adjoint-c4", "copy-calibration"],
+                    choices=["enblend", "enblend-layers", "enfuse-c0", "enfuse-c2", "batch-c3", "adjoint-c4",
+                             "copy-calibration"],
                     help="enblend: the headline line (default); the others print one part of it alone")
     ap.add_argument("--c2-steps", type=int, default=3, help="timed steps of the 8 x 6000x4000 float enfuse")
     ap.add_argument("--no-enfuse", action="store_true", help="leave the enfuse workloads out of the line")
+    ap.add_argument("--no-layers", action="store_true",
+                    help="leave the enblend run from decoded layers (assembly through checkpoint) out of the line")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-vs-plain comparison out of the line")
     ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint check and tuning run out of the line")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -448,6 +535,9 @@ def main():
 
     enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
     scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
+    if args.workload == "enblend-layers":
+        print(json.dumps(enblend_layers_c1(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
+        return
     if args.workload == "enfuse-c0":
         print(json.dumps(enfuse_c0(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
         return
@@ -623,6 +713,8 @@ def main():
                                          "kind": "port", "sample": "full mask, one run"}
     ctx.close()
     del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
+    if not args.no_layers:
+        result["enblend_layers"] = enblend_layers_c1(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
     if not args.no_batch:
         result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
     if not args.no_enfuse:

@@ -80,15 +80,41 @@ __device__ void block_box(Box b, int* r)
 }
 
 // FindBoundingRectangle over the non-zero pixels of a w x h byte plane.
+// With 16-byte aligned rows a thread reads 16 columns per row in one access
+// (kBoxRows rows per block); otherwise one column.
+template <bool WIDE>
 __global__ __launch_bounds__(kDT) void alpha_box_kernel(const uint8_t* __restrict__ a, long ap, int w, int h, int* r)
 {
-    const int x = blockIdx.x * kDT + threadIdx.x;
+    const int span = WIDE ? 16 : 1;
+    const int x = (blockIdx.x * kDT + threadIdx.x) * span;
     const int y0 = blockIdx.y * kBoxRows;
     Box b = {INT_MAX, INT_MAX, INT_MIN, INT_MIN};
     if (x < w) {
         const int ye = min(h, y0 + kBoxRows);
-        for (int y = y0; y < ye; ++y)
-            if (*gmem(a + (long)y * ap + x)) box_add(b, x, y, x + 1, y + 1);
+        for (int y = y0; y < ye; ++y) {
+            const uint8_t* row = a + (long)y * ap + x;
+            if (WIDE && x + 16 <= w) {
+                const v4i32 v = *gmem(reinterpret_cast<const v4i32*>(row));
+                const uint32_t wd[4] = {(uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w};
+                int lo = -1, hi = -1;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // bytes of word k that are non-zero, as bits 0..3
+                    const uint32_t t = wd[k];
+                    const uint32_t nz = ((t & 0xffu) != 0) | (((t >> 8) & 0xffu) != 0) << 1 |
+                                        (((t >> 16) & 0xffu) != 0) << 2 | ((t >> 24) != 0) << 3;
+                    if (nz) {
+                        if (lo < 0) lo = 4 * k + __builtin_ctz(nz);
+                        hi = 4 * k + 31 - __builtin_clz(nz);
+                    }
+                }
+                if (lo >= 0) box_add(b, x + lo, y, x + hi + 1, y + 1);
+            } else {
+                const int xe = min(w, x + span);
+                for (int xx = x; xx < xe; ++xx)
+                    if (*gmem(row + (xx - x))) box_add(b, xx, y, xx + 1, y + 1);
+            }
+        }
     }
     block_box(b, r);
 }
@@ -403,7 +429,10 @@ int plane_box(enbl_context* ctx, bool transitions, const uint8_t* p, long pitch,
     if (w > 0 && h > 0) {
         Timed t(ctx, transitions ? "mask_bounds" : "alpha_box", (double)w * h);
         if (transitions) hipLaunchKernelGGL(mask_box_kernel, grid_box(w, h), dim3(kDT), 0, ctx->stream, p, pitch, w, h, r);
-        else hipLaunchKernelGGL(alpha_box_kernel, grid_box(w, h), dim3(kDT), 0, ctx->stream, p, pitch, w, h, r);
+        else if (((uintptr_t)p & 15) == 0 && (pitch & 15) == 0)
+            hipLaunchKernelGGL(alpha_box_kernel<true>, grid_box((w + 15) / 16, h), dim3(kDT), 0, ctx->stream, p, pitch, w,
+                               h, r);
+        else hipLaunchKernelGGL(alpha_box_kernel<false>, grid_box(w, h), dim3(kDT), 0, ctx->stream, p, pitch, w, h, r);
         rc = launch_status(transitions ? "mask_bounds" : "alpha_box");
         if (rc) return rc;
     }

@@ -49,7 +49,9 @@ def _gpu_assemble(enbl, ctx, layers, itype, C, uw, uh, taken, one_at_a_time):
     dl = [(_dev(a), x, y, ft) for a, x, y, ft in layers]
     tab = enbl.layer_table(dl)
     tk = (ctypes.c_int * len(layers))(*taken)
-    ipitch, apitch = uw * C + 5, uw + 3
+    # one-at-a-time runs use 16-byte aligned alpha rows (the wide bounding-box
+    # pass), the others odd pitches
+    ipitch, apitch = uw * C + 5, ((uw + 15) // 16 + 1) * 16 if one_at_a_time else uw + 3
     img = torch.full((uh, ipitch), 77, dtype=getattr(torch, np.dtype(NP[itype]).name), device="cuda")
     al = torch.full((uh, apitch), 9, dtype=torch.uint8, device="cuda")
     bb = enbl.Rect()
