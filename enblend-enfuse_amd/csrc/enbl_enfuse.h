@@ -35,6 +35,29 @@ int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, lon
                  int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
 int launch_fill(void* p, size_t bytes, hipStream_t s);
 
+// Contrast by Laplacian-of-Gaussian edges (src/enfuse.h:1020-1093): the
+// separable passes of VIGRA's laplacianOfGaussian / gaussianSharpening with
+// mirrored borders.  Planes of the gray type G (int32 for integer images,
+// float for float images) and of its real promote R (double, float).
+struct EdgePlanes {
+    const void* src; long spitch;          // G: the gray (or sharpened gray) plane
+    void* tA; void* tB; long tpitch;       // R scratch planes
+    int w, h;
+};
+// Row pass: tA = src (*) ka along x (and tB = src (*) kb when kb != NULL).
+int launch_edge_rows(int itype, const EdgePlanes& e, const double* ka, int ra, const double* kb, int rb,
+                     hipStream_t s);
+// Column pass of gaussianSharpening: out = cast_G((1 + f) gray - f (tA (*) k along y)).
+int launch_edge_sharpen(int itype, const EdgePlanes& e, const double* k, int r, double f, const void* gray,
+                        long gpitch, void* out, long opitch, hipStream_t s);
+// Column passes of laplacianOfGaussian on tA = src (*) d2 and tB = src (*)
+// smooth (rows), the magnitude and the clamping (min_curvature <= 0) or
+// fill-in functor (lc: the local standard deviation), written where alpha
+// is set, 0 elsewhere.
+int launch_edge_log(int itype, const EdgePlanes& e, const double* ks, int rs, const double* kd, int rd,
+                    const uint8_t* alpha, long apitch, const void* lc, long lpitch, double min_curvature,
+                    double scalar_max, void* grad, long gpitch, hipStream_t s);
+
 namespace host {
 // Parameter checks of enfuse (the reference's preconditions and option checks).
 int check_params(int itype, int w, int h, const enbl_enfuse_params* p);

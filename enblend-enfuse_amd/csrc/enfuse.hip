@@ -682,6 +682,86 @@ __global__ __launch_bounds__(kTile) void merge_kernel(const P* __restrict__ lp, 
     }
 }
 
+// ---- contrast by Laplacian-of-Gaussian edges ------------------------------
+// VIGRA's convolveLine with BORDER_TREATMENT_REFLECT: the window's samples
+// from the leftmost (topmost) one, source index mirrored at both ends
+// without repeating the edge sample, summed in double from 0 in that order;
+// the symmetric kernels make tap t weigh k[t].  One thread per pixel; the
+// taps of neighbouring threads overlap, so the re-reads come from cache.
+__device__ __forceinline__ int mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
+
+template <class G> __device__ __forceinline__ G cast_real(double v);
+template <> __device__ __forceinline__ int32_t cast_real<int32_t>(double v) { return from_real<int32_t>(v); }
+template <> __device__ __forceinline__ float cast_real<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ double cast_real<double>(double v) { return v; }
+
+template <class G, class R, bool TWO>
+__global__ __launch_bounds__(256) void edge_rows_kernel(EdgePlanes e, const double* __restrict__ ka, int ra,
+                                                        const double* __restrict__ kb, int rb)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= e.w) return;
+    const G* row = reinterpret_cast<const G*>(e.src) + (long)y * e.spitch;
+    double sa = 0.0;
+    for (int t = 0; t <= 2 * ra; ++t) sa += ka[t] * (double)*gmem(row + mirror(x - ra + t, e.w));
+    *gmem(reinterpret_cast<R*>(e.tA) + (long)y * e.tpitch + x) = cast_real<R>(sa);
+    if (TWO) {
+        double sb = 0.0;
+        for (int t = 0; t <= 2 * rb; ++t) sb += kb[t] * (double)*gmem(row + mirror(x - rb + t, e.w));
+        *gmem(reinterpret_cast<R*>(e.tB) + (long)y * e.tpitch + x) = cast_real<R>(sb);
+    }
+}
+
+template <class G, class R>
+__global__ __launch_bounds__(256) void edge_sharpen_kernel(EdgePlanes e, const double* __restrict__ k, int r, double f,
+                                                           const G* __restrict__ gray, long gp, G* out, long op)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= e.w) return;
+    const R* col = reinterpret_cast<const R*>(e.tA) + x;
+    double sm = 0.0;
+    for (int t = 0; t <= 2 * r; ++t) sm += k[t] * (double)*gmem(col + (long)mirror(y - r + t, e.h) * e.tpitch);
+    const R t2 = cast_real<R>(sm);
+    const double v = (1.0 + f) * (double)*gmem(gray + (long)y * gp + x) - f * (double)t2;
+    *gmem(out + (long)y * op + x) = cast_real<G>(v);
+}
+
+template <class G> __device__ __forceinline__ G g_max();
+template <> __device__ __forceinline__ int32_t g_max<int32_t>() { return INT32_MAX; }
+template <> __device__ __forceinline__ float g_max<float>() { return FLT_MAX; }
+
+template <class G, class R, bool FILL>
+__global__ __launch_bounds__(256) void edge_log_kernel(EdgePlanes e, const double* __restrict__ ks, int rs,
+                                                       const double* __restrict__ kd, int rd,
+                                                       const uint8_t* __restrict__ alpha, long ap,
+                                                       const G* __restrict__ lc, long lp, double mc, double smax,
+                                                       G* grad, long gp)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= e.w) return;
+    G out = (G)0;
+    if (*gmem(alpha + (long)y * ap + x)) {
+        const R* ca = reinterpret_cast<const R*>(e.tA) + x;   // rows: second derivative
+        const R* cb = reinterpret_cast<const R*>(e.tB) + x;   // rows: smoothing
+        double sx = 0.0, sy = 0.0;
+        for (int t = 0; t <= 2 * rs; ++t) sx += ks[t] * (double)*gmem(ca + (long)mirror(y - rs + t, e.h) * e.tpitch);
+        for (int t = 0; t <= 2 * rd; ++t) sy += kd[t] * (double)*gmem(cb + (long)mirror(y - rd + t, e.h) * e.tpitch);
+        const R lap = (R)(cast_real<R>(sx) + cast_real<R>(sy));
+        const G mag = cast_real<G>((double)fabs(lap));   // MagnitudeAccessor
+        if (FILL) {
+            // FillInFunctor(mc, 1, mc / max(ScalarType))
+            const G thr = (G)mc;
+            out = mag >= thr ? cast_real<G>(1.0 * (double)mag)
+                             : cast_real<G>(mc / smax * (double)*gmem(lc + (long)y * lp + x));
+        } else {
+            // ClampingFunctor(-mc -> 0, max -> max)
+            const G lo = (G)(-mc), up = g_max<G>();
+            out = mag <= lo ? (G)0 : (mag >= up ? up : mag);
+        }
+    }
+    *gmem(grad + (long)y * gp + x) = out;
+}
+
 }  // namespace
 
 int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s)
@@ -813,6 +893,56 @@ int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, lon
     if (ptype == ENBL_F64 && mtype == ENBL_I32) ENBL_MERGE(double, int32_t);
 #undef ENBL_MERGE
     return ENBL_E_UNSUPPORTED;
+}
+
+int launch_edge_rows(int itype, const EdgePlanes& e, const double* ka, int ra, const double* kb, int rb, hipStream_t s)
+{
+    const dim3 g((unsigned)((e.w + 255) / 256), (unsigned)e.h);
+    if (itype == ENBL_F32) {
+        if (kb) hipLaunchKernelGGL((edge_rows_kernel<float, float, true>), g, dim3(256), 0, s, e, ka, ra, kb, rb);
+        else hipLaunchKernelGGL((edge_rows_kernel<float, float, false>), g, dim3(256), 0, s, e, ka, ra, kb, rb);
+    } else {
+        if (kb) hipLaunchKernelGGL((edge_rows_kernel<int32_t, double, true>), g, dim3(256), 0, s, e, ka, ra, kb, rb);
+        else hipLaunchKernelGGL((edge_rows_kernel<int32_t, double, false>), g, dim3(256), 0, s, e, ka, ra, kb, rb);
+    }
+    return 0;
+}
+
+int launch_edge_sharpen(int itype, const EdgePlanes& e, const double* k, int r, double f, const void* gray, long gp,
+                        void* out, long op, hipStream_t s)
+{
+    const dim3 g((unsigned)((e.w + 255) / 256), (unsigned)e.h);
+    if (itype == ENBL_F32)
+        hipLaunchKernelGGL((edge_sharpen_kernel<float, float>), g, dim3(256), 0, s, e, k, r, f, (const float*)gray, gp,
+                           (float*)out, op);
+    else
+        hipLaunchKernelGGL((edge_sharpen_kernel<int32_t, double>), g, dim3(256), 0, s, e, k, r, f, (const int32_t*)gray,
+                           gp, (int32_t*)out, op);
+    return 0;
+}
+
+int launch_edge_log(int itype, const EdgePlanes& e, const double* ks, int rs, const double* kd, int rd,
+                    const uint8_t* alpha, long ap, const void* lc, long lp, double mc, double smax, void* grad, long gp,
+                    hipStream_t s)
+{
+    const dim3 g((unsigned)((e.w + 255) / 256), (unsigned)e.h);
+    const bool fill = mc > 0.0;
+    if (itype == ENBL_F32) {
+        if (fill)
+            hipLaunchKernelGGL((edge_log_kernel<float, float, true>), g, dim3(256), 0, s, e, ks, rs, kd, rd, alpha, ap,
+                               (const float*)lc, lp, mc, smax, (float*)grad, gp);
+        else
+            hipLaunchKernelGGL((edge_log_kernel<float, float, false>), g, dim3(256), 0, s, e, ks, rs, kd, rd, alpha, ap,
+                               (const float*)lc, lp, mc, smax, (float*)grad, gp);
+    } else {
+        if (fill)
+            hipLaunchKernelGGL((edge_log_kernel<int32_t, double, true>), g, dim3(256), 0, s, e, ks, rs, kd, rd, alpha,
+                               ap, (const int32_t*)lc, lp, mc, smax, (int32_t*)grad, gp);
+        else
+            hipLaunchKernelGGL((edge_log_kernel<int32_t, double, false>), g, dim3(256), 0, s, e, ks, rs, kd, rd, alpha,
+                               ap, (const int32_t*)lc, lp, mc, smax, (int32_t*)grad, gp);
+    }
+    return 0;
 }
 
 }  // namespace enbl

@@ -10,6 +10,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "enbl_enfuse.h"
@@ -42,6 +43,46 @@ double exposure_host(int fn, double opt, double width, double y)
     }
 }
 
+// VIGRA's Kernel1D<double>::initGaussian (order 0) and
+// initGaussianDerivative (order 2) with the default window: radius
+// (int)(3 sigma + 0.5), resp. (int)(4 sigma + 0.5), at least 1; samples of
+// the Gaussian (or its second derivative, (1 - (x/sigma)^2) g) from -radius
+// up, the derivative's mean removed, then scaled so that sum k = 1, resp.
+// sum k x^2 / 2 = 1.
+int vigra_radius(double sigma, int order)
+{
+    const int r = order == 0 ? (int)(3.0 * sigma + 0.5) : (int)((3.0 + 0.5 * order) * sigma + 0.5);
+    return r == 0 ? 1 : r;
+}
+
+std::vector<double> vigra_kernel(double sigma, int order)
+{
+    const int radius = vigra_radius(sigma, order);
+    const double sigma2 = -0.5 / sigma / sigma;
+    const double norm = order == 0 ? 1.0 / (sqrt(2.0 * M_PI) * sigma)
+                                   : -1.0 / (sqrt(2.0 * M_PI) * (sigma * sigma) * sigma);
+    std::vector<double> k;
+    double dc = 0.0;
+    for (double x = -(double)radius; x <= (double)radius; ++x) {
+        const double g = norm * exp(x * x * sigma2);
+        const double v = order == 0 ? g : (1.0 - (x / sigma) * (x / sigma)) * g;
+        k.push_back(v);
+        dc += v;
+    }
+    double sum = 0.0;
+    if (order == 0) {
+        for (double v : k) sum += v;
+    } else {
+        dc = dc / (2.0 * radius + 1.0);
+        for (double& v : k) v -= dc;
+        double x = -(double)radius;
+        for (size_t i = 0; i < k.size(); ++i, ++x) sum = sum + k[i] * ((-x) * (-x)) / 2.0;
+    }
+    sum = 1.0 / sum;
+    for (double& v : k) v = v * sum;
+    return k;
+}
+
 int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
 {
     if (!p) return fail(ENBL_E_ARG, "enfuse: null parameters");
@@ -53,10 +94,19 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
         return fail(ENBL_E_ARG, "y_optimum");       // ExposureWeight::check_invariant
     if (p->w_exposure > 0.0 && !(p->exposure_width > 0.0))
         return fail(ENBL_E_ARG, "width_parameter");
-    if (p->w_contrast > 0.0) {
+    if (p->w_contrast > 0.0 && !(p->edge_scale >= 0.0)) return fail(ENBL_E_ARG, "enfuse: edge scale");
+    if (p->w_contrast > 0.0 && (p->edge_scale <= 0.0 || p->min_curvature > 0.0)) {
         const int n = p->contrast_window;
         if (n < 3 || !(n & 1)) return fail(ENBL_E_ARG, "enfuse: contrast window must be odd and >= 3");
         if (w < n || h < n) return fail(ENBL_E_PRECONDITION, "localStdDevIf(): window larger than image");
+    }
+    if (p->w_contrast > 0.0 && p->edge_scale > 0.0) {
+        if (!(p->lce_scale >= 0.0) || !(p->lce_factor >= 0.0))
+            return fail(ENBL_E_PRECONDITION, "gaussianSharpening(): scale and sharpening factor must be >= 0");
+        const int rs = vigra_radius(p->edge_scale, 0), rd = vigra_radius(p->edge_scale, 2);
+        const int rl = p->lce_scale > 0.0 ? vigra_radius(p->lce_scale, 0) : 0;
+        if (std::max(std::max(rs, rd), rl) >= std::min(w, h))
+            return fail(ENBL_E_PRECONDITION, "convolveLine(): kernel longer than line");
     }
     if (p->w_entropy > 0.0) {
         const int n = p->entropy_window;
@@ -105,6 +155,12 @@ struct WeightScratch {
     void* ent = nullptr;
     long pitch = 0;   // elements, shared by gray/grad/ent
     size_t plane = 0; // elements between the images' gray/grad planes
+    // contrast by edges: kernels (smoothing, second derivative, LCE smoothing
+    // at ks, kd, kl), two real scratch planes, the sharpened gray plane and
+    // the K local standard deviations of the fill-in functor
+    const double *ks = nullptr, *kd = nullptr, *kl = nullptr;
+    int rs = 0, rd = 0, rl = 0;
+    void *ta = nullptr, *tb = nullptr, *lce = nullptr, *lc = nullptr;
 };
 
 // Host tables and device scratch for K images of one size (ExposureFunctor
@@ -134,6 +190,37 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, int K, const enb
         rc = scratch(ctx, 1, npx * 4 * (size_t)K, &ws->gray);
         if (!rc) rc = scratch(ctx, 2, npx * 4 * (size_t)K, &ws->grad);
         if (rc) return rc;
+    }
+    if (p->w_contrast > 0.0 && p->edge_scale > 0.0) {
+        const std::vector<double> ks = vigra_kernel(p->edge_scale, 0), kd = vigra_kernel(p->edge_scale, 2);
+        std::vector<double> kl;
+        if (p->lce_scale > 0.0) kl = vigra_kernel(p->lce_scale, 0);
+        std::vector<double> all(ks);
+        all.insert(all.end(), kd.begin(), kd.end());
+        all.insert(all.end(), kl.begin(), kl.end());
+        void* d;
+        rc = scratch(ctx, 27, all.size() * sizeof(double), &d);
+        if (rc) return rc;
+        rc = hip_check(hipMemcpyAsync(d, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+                       "edge kernels");
+        if (!rc) rc = hip_check(hipStreamSynchronize(ctx->stream), "edge kernels");
+        if (rc) return rc;
+        ws->ks = (const double*)d;
+        ws->kd = ws->ks + ks.size();
+        ws->kl = p->lce_scale > 0.0 ? ws->kd + kd.size() : nullptr;
+        ws->rs = (int)(ks.size() / 2);
+        ws->rd = (int)(kd.size() / 2);
+        ws->rl = (int)(kl.size() / 2);
+        const size_t es_r = itype == ENBL_F32 ? 4 : 8;
+        void* t;
+        rc = scratch(ctx, 28, 2 * npx * es_r, &t);
+        if (rc) return rc;
+        ws->ta = t;
+        ws->tb = (char*)t + npx * es_r;
+        rc = scratch(ctx, 29, npx * 4 * (size_t)(1 + K), &t);
+        if (rc) return rc;
+        ws->lce = t;
+        ws->lc = (char*)t + npx * 4;
     }
     if (p->w_entropy > 0.0) {
         const int full = p->entropy_window * p->entropy_window;
@@ -169,6 +256,65 @@ int prepare_weights(enbl_context* ctx, int itype, int w, int h, int K, const enb
     return ENBL_OK;
 }
 
+// Contrast by Laplacian-of-Gaussian edges (src/enfuse.h:1020-1093) of all K
+// gray planes: the local standard deviations first when the fill-in functor
+// needs them, then per image the optional sharpening (gaussianSharpening)
+// and the LoG with its clamping or fill-in functor into gradient plane m.
+int edge_contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const uint8_t* const* alphas, const long* aps,
+                      const enbl_enfuse_params* p, const WeightScratch& ws)
+{
+    const hipStream_t s = ctx->stream;
+    const size_t es = 4;
+    const double es_r = itype == ENBL_F32 ? 4.0 : 8.0;
+    const double mc = p->min_curvature;
+    int rc;
+    if (mc > 0.0) {
+        if (itype == ENBL_F32) {
+            rc = launch_fill(ws.lc, ws.plane * es * (size_t)K, s);
+            if (!rc) rc = launch_status("fill");
+            if (rc) return rc;
+        }
+        Timed t(ctx, "stddev", (double)K * w * h * 9.0);
+        rc = launch_stddev(itype, ws.gray, ws.pitch, ws.plane, alphas, aps, K, w, h, p->contrast_window, ws.lc,
+                           ws.pitch, ws.plane, s);
+        if (rc) return fail(rc, "localStdDevIf: unsupported window");
+        rc = launch_status("stddev");
+        if (rc) return rc;
+    }
+    const double smax = itype == ENBL_U8 ? 255.0 : itype == ENBL_U16 ? 65535.0 : (double)FLT_MAX;
+    for (int m = 0; m < K; ++m) {
+        const void* gray = (const char*)ws.gray + (size_t)m * ws.plane * es;
+        EdgePlanes e;
+        e.src = gray; e.spitch = ws.pitch; e.tA = ws.ta; e.tB = ws.tb; e.tpitch = ws.pitch; e.w = w; e.h = h;
+        if (ws.kl) {
+            {
+                Timed t(ctx, "edge_rows", (double)w * h * (4.0 + es_r));
+                rc = launch_edge_rows(itype, e, ws.kl, ws.rl, nullptr, 0, s);
+                if (!rc) rc = launch_status("edge_rows");
+                if (rc) return rc;
+            }
+            Timed t(ctx, "edge_sharpen", (double)w * h * (es_r + 8.0));
+            rc = launch_edge_sharpen(itype, e, ws.kl, ws.rl, p->lce_factor, gray, ws.pitch, ws.lce, ws.pitch, s);
+            if (!rc) rc = launch_status("edge_sharpen");
+            if (rc) return rc;
+            e.src = ws.lce;
+        }
+        {
+            Timed t(ctx, "edge_rows", (double)w * h * (4.0 + 2.0 * es_r));
+            rc = launch_edge_rows(itype, e, ws.kd, ws.rd, ws.ks, ws.rs, s);
+            if (!rc) rc = launch_status("edge_rows");
+            if (rc) return rc;
+        }
+        Timed t(ctx, "edge_log", (double)w * h * (2.0 * es_r + 1.0 + 4.0 + (mc > 0.0 ? 4.0 : 0.0)));
+        rc = launch_edge_log(itype, e, ws.ks, ws.rs, ws.kd, ws.rd, alphas[m], aps[m],
+                             (const char*)ws.lc + (size_t)m * ws.plane * es, ws.pitch, mc, smax,
+                             (char*)ws.grad + (size_t)m * ws.plane * es, ws.pitch, s);
+        if (!rc) rc = launch_status("edge_log");
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // Gray images and local standard deviations (src/enfuse.h:1099-1120) of
 // all K images: gradient plane m of ws.grad belongs to image m.
 int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
@@ -183,6 +329,7 @@ int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* 
         if (!rc) rc = launch_status("gray");
         if (rc) return rc;
     }
+    if (p->edge_scale > 0.0) return edge_contrast_all(ctx, itype, w, h, K, alphas, aps, p, ws);
     if (itype == ENBL_F32) {
         rc = launch_fill(ws.grad, ws.plane * es * (size_t)K, s);
         if (!rc) rc = launch_status("fill");

@@ -198,6 +198,14 @@ typedef struct enbl_enfuse_params {
      * of an integer image (256 or 65536 doubles), e.g. from a user plugin
      * (src/exposure_weight_base.h:29-88).  NULL: the built-in function. */
     const double* exposure_table;
+    /* Contrast by Laplacian-of-Gaussian edges (--contrast-edge-scale,
+     * src/enfuse.h:1020-1093) when edge_scale > 0: the edge scale, the
+     * local-contrast-enhancement scale and factor (0: none; lce_scale in
+     * pixels, lce_factor as a fraction), and the minimum curvature as the
+     * reference instantiates it for the image's scalar type
+     * (static_cast<double>(MinCurvature.instantiate<ScalarType>())).  Zero
+     * edge_scale: the local standard deviation. */
+    double edge_scale, lce_scale, lce_factor, min_curvature;
 } enbl_enfuse_params;
 
 /* Replaces enfuseMask<ImageType, AlphaType, MaskType>(src, mask, result) --

@@ -105,6 +105,7 @@ typedef struct orc_enfuse_params {
     const double* exposure_table;   /* optional: weight(code / max) for every
                                        gray code of an integer image (user
                                        ExposureWeight); NULL = built-in */
+    double edge_scale, lce_scale, lce_factor, min_curvature;   /* LoG contrast (src/enfuse.h:1020-1093) */
 } orc_enfuse_params;
 
 /* enfuseMask (src/enfuse.h:962-1212) for one RGB image with alpha: the float
@@ -135,7 +136,10 @@ int orc_diff_backward(int w, int h, int K, const double* imgs, const double* prm
 void orc_diff_expand_T(const double* g, int fw, int fh, double* gc, int cw, int ch);
 void orc_diff_reduce_T(const double* g, int dw, int dh, double* gf, int sw, int sh);
 
-/* Number of OpenMP threads the pointwise stages use. */
+/* VIGRA's Gaussian (order 0) or second-derivative (order 2) kernel used by
+ * the LoG contrast mode: 2r + 1 taps into k (capacity n); returns r. */
+int orc_vigra_kernel(double sigma, int order, double* k, int n);
+
 /* Nearest-feature-transform seam mask (src/nearest.h:424-552) with the
  * OpenMP build's float Felzenszwalb-Huttenlocher distance transform
  * (src/openmp_vigra.h:244-470); norm 1 = Manhattan, 2 = Euclidean.
@@ -177,6 +181,7 @@ int orc_enblend_step(int itype, int channels, int union_w, int union_h, int wrap
 int orc_checkpoint(int itype, int channels, int w, int h, const void* img, const uint8_t* alpha, int otype,
                    void* out, int* out_ctype);
 
+/* Number of OpenMP threads the pointwise stages use. */
 int orc_num_threads(void);
 
 #ifdef __cplusplus

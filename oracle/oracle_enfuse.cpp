@@ -204,6 +204,128 @@ template <> float contrast_weight<uint8_t>(double wc, int32_t g) { return (float
 template <> float contrast_weight<uint16_t>(double wc, int32_t g) { return (float)(wc * (double)g / 65535.0); }
 template <> float contrast_weight<float>(double wc, float g) { return (float)(wc * (double)g); }
 
+// ---- Laplacian-of-Gaussian contrast, src/enfuse.h:1020-1093 --------------
+// VIGRA's pieces (not vendored; restated from its documented algorithms):
+// Kernel1D<double>::initGaussian / initGaussianDerivative (radius
+// (int)(3 sigma + 0.5), resp. (int)((3 + order / 2) sigma + 0.5), at least 1;
+// samples of the Gaussian functor, derivative kernels with their mean
+// removed, normalised so that sum k = 1, resp. sum k (-x)^2 / 2! = 1),
+// convolveLine with BORDER_TREATMENT_REFLECT (sum over the window from the
+// leftmost source sample, source index mirrored at both ends without
+// repeating the edge, "kernel longer than line" when radius >= length),
+// separableConvolveX/Y, laplacianOfGaussian (d2x * smooth_y + smooth_x *
+// d2y), gaussianSmoothing, gaussianSharpening ((1 + f) src - f smooth).
+std::vector<double> vigra_gaussian_kernel(double sigma, int order)
+{
+    int radius = order == 0 ? (int)(3.0 * sigma + 0.5) : (int)((3.0 + 0.5 * order) * sigma + 0.5);
+    if (radius == 0) radius = 1;
+    const double sigma2 = -0.5 / sigma / sigma;
+    const double norm = order == 0 ? 1.0 / (sqrt(2.0 * M_PI) * sigma)
+                                   : -1.0 / (sqrt(2.0 * M_PI) * (sigma * sigma) * sigma);
+    std::vector<double> k;
+    double dc = 0.0;
+    for (double x = -(double)radius; x <= (double)radius; ++x) {
+        const double g = norm * exp(x * x * sigma2);
+        const double v = order == 0 ? g : (1.0 - (x / sigma) * (x / sigma)) * g;
+        k.push_back(v);
+        dc += v;
+    }
+    double sum = 0.0;
+    if (order == 0) {
+        for (double v : k) sum += v;
+    } else {
+        dc = dc / (2.0 * radius + 1.0);
+        for (double& v : k) v -= dc;
+        double x = -(double)radius;
+        for (size_t i = 0; i < k.size(); ++i, ++x) sum = sum + k[i] * ((-x) * (-x)) / 2.0;
+    }
+    sum = 1.0 / sum;
+    for (double& v : k) v = v * sum;
+    return k;
+}
+
+inline int reflect(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
+
+// One separable pass: out(x, y) = cast(sum over t of k[t] * in(x - r + t, y))
+// along rows (dir 0) or columns (dir 1).
+template <class I, class O>
+void convolve_pass(const I* in, int w, int h, const std::vector<double>& k, int dir, O* out)
+{
+    const int r = (int)(k.size() / 2);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            double sum = 0.0;
+            for (int t = 0; t <= 2 * r; ++t) {
+                const I v = dir == 0 ? in[(long)y * w + reflect(x - r + t, w)] : in[(long)reflect(y - r + t, h) * w + x];
+                sum += k[(size_t)t] * (double)v;
+            }
+            out[(long)y * w + x] = fromRealPromote<O>(sum);
+        }
+}
+
+template <class T> struct LoGTypes;   // gray type G and the convolution images' type R (RealPromote<G>)
+template <> struct LoGTypes<uint8_t> { typedef int32_t G; typedef double R; };
+template <> struct LoGTypes<uint16_t> { typedef int32_t G; typedef double R; };
+template <> struct LoGTypes<float> { typedef float G; typedef float R; };
+
+template <class G> G max_of();
+template <> int32_t max_of<int32_t>() { return INT32_MAX; }
+template <> float max_of<float>() { return FLT_MAX; }
+
+// grad = the contrast image of the edge mode, written where alpha is set
+// (zero elsewhere); lc = the local standard deviation when min_curvature > 0.
+template <class T>
+int log_contrast(const typename LoGTypes<T>::G* gray, const uint8_t* alpha, int w, int h,
+                 const orc_enfuse_params* p, const typename LoGTypes<T>::G* lc, typename LoGTypes<T>::G* grad)
+{
+    typedef typename LoGTypes<T>::G G;
+    typedef typename LoGTypes<T>::R R;
+    const long N = (long)w * h;
+    const std::vector<double> smooth = vigra_gaussian_kernel(p->edge_scale, 0);
+    const std::vector<double> deriv = vigra_gaussian_kernel(p->edge_scale, 2);
+    const int rs = (int)(smooth.size() / 2), rd = (int)(deriv.size() / 2);
+    if (rs >= w || rs >= h || rd >= w || rd >= h) return -1;   // convolveLine(): kernel longer than line
+    std::vector<G> lce;
+    const G* src = gray;
+    if (p->lce_scale > 0.0) {
+        const std::vector<double> ls = vigra_gaussian_kernel(p->lce_scale, 0);
+        const int rl = (int)(ls.size() / 2);
+        if (rl >= w || rl >= h) return -1;
+        std::vector<R> t1((size_t)N), t2((size_t)N);
+        convolve_pass<G, R>(gray, w, h, ls, 0, t1.data());
+        convolve_pass<R, R>(t1.data(), w, h, ls, 1, t2.data());
+        lce.resize((size_t)N);
+        const double f = p->lce_factor;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < N; ++i) lce[i] = fromRealPromote<G>((1.0 + f) * (double)gray[i] - f * (double)t2[i]);
+        src = lce.data();
+    }
+    std::vector<R> tmp((size_t)N), tx((size_t)N), ty((size_t)N);
+    convolve_pass<G, R>(src, w, h, deriv, 0, tmp.data());
+    convolve_pass<R, R>(tmp.data(), w, h, smooth, 1, tx.data());
+    convolve_pass<G, R>(src, w, h, smooth, 0, tmp.data());
+    convolve_pass<R, R>(tmp.data(), w, h, deriv, 1, ty.data());
+    const double mc = p->min_curvature;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        const R sum = (R)(tx[i] + ty[i]);
+        const G lap = fromRealPromote<G>((double)std::fabs(sum));   // MagnitudeAccessor
+        if (!alpha[i]) { grad[i] = G(0); continue; }
+        if (mc <= 0.0) {
+            // ClampingFunctor(-mc -> 0, max -> max)
+            const G lo = (G)(-mc), up = max_of<G>();
+            grad[i] = lap <= lo ? G(0) : (lap >= up ? up : lap);
+        } else {
+            // FillInFunctor(mc, 1, mc / max(ScalarType))
+            const G thr = (G)mc;
+            const double s2 = mc / Max<T>::v();
+            grad[i] = lap >= thr ? fromRealPromote<G>(1.0 * (double)lap) : fromRealPromote<G>(s2 * (double)lc[i]);
+        }
+    }
+    return 0;
+}
+
 template <class T> bool is_integral() { return true; }
 template <> bool is_integral<float>() { return false; }
 
@@ -233,11 +355,23 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
         typedef typename Contrast<T>::G G;
         typedef typename Contrast<T>::S S;
         const int n = p->contrast_window;
-        if (n < 2 || w < n || h < n) return -1;
+        const bool edges = p->edge_scale > 0.0;
+        const bool stddev = !edges || p->min_curvature > 0.0;
+        if (stddev && (n < 2 || w < n || h < n)) return -1;
         std::vector<G> gray((size_t)N), grad((size_t)N, G(0));
 #pragma omp parallel for schedule(guided)
         for (long i = 0; i < N; ++i) gray[i] = Gray<T, G>::get(img + 3 * i);
-        local_stddev<G, S>(gray.data(), alpha, w, h, n, grad.data());
+        if (!edges) {
+            local_stddev<G, S>(gray.data(), alpha, w, h, n, grad.data());
+        } else {
+            std::vector<G> lc;
+            if (stddev) {
+                lc.assign((size_t)N, G(0));
+                local_stddev<G, S>(gray.data(), alpha, w, h, n, lc.data());
+            }
+            if (p->lce_factor < 0.0 || p->lce_scale < 0.0) return -1;
+            if (log_contrast<T>(gray.data(), alpha, w, h, p, lc.data(), grad.data())) return -1;
+        }
 #pragma omp parallel for schedule(guided)
         for (long i = 0; i < N; ++i)
             if (alpha[i]) mask[i] = contrast_weight<T>(p->w_contrast, grad[i]) + mask[i];
@@ -402,3 +536,13 @@ int orc_enfuse(int itype, int w, int h, int K, const void* const* imgs, const ui
 }
 
 }  // extern "C"
+
+// VIGRA's Gaussian (order 0) or second-derivative (order 2) kernel of scale
+// sigma, 2r + 1 taps into k (capacity n); returns r, or -1 when n is short.
+extern "C" int orc_vigra_kernel(double sigma, int order, double* k, int n)
+{
+    const std::vector<double> v = vigra_gaussian_kernel(sigma, order);
+    if ((int)v.size() > n) return -1;
+    for (size_t i = 0; i < v.size(); ++i) k[i] = v[i];
+    return (int)(v.size() / 2);
+}

@@ -42,6 +42,7 @@ _SIGS = {
     "orc_enblend_step": [_i, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, ctypes.c_uint64,
                          _vp, _vp],
     "orc_checkpoint": [_i, _i, _i, _i, _vp, _vp, _i, _vp, _vp],
+    "orc_vigra_kernel": [ctypes.c_double, _i, _vp, _i],
 }
 
 
@@ -70,14 +71,18 @@ class EnfuseParams(ctypes.Structure):
                 ("exposure_optimum", ctypes.c_double), ("exposure_width", ctypes.c_double),
                 ("exposure_function", ctypes.c_int), ("contrast_window", ctypes.c_int),
                 ("entropy_window", ctypes.c_int), ("hard_mask", ctypes.c_int),
-                ("exposure_table", ctypes.c_void_p)]
+                ("exposure_table", ctypes.c_void_p), ("edge_scale", ctypes.c_double),
+                ("lce_scale", ctypes.c_double), ("lce_factor", ctypes.c_double),
+                ("min_curvature", ctypes.c_double)]
 
 
 def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
                   exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXP_GAUSSIAN,
-                  contrast_window=5, entropy_window=3, exposure_table=None, hard_mask=False):
+                  contrast_window=5, entropy_window=3, exposure_table=None, hard_mask=False,
+                  edge_scale=0.0, lce_scale=0.0, lce_factor=0.0, min_curvature=0.0):
     p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
-                     exposure_function, contrast_window, entropy_window, int(hard_mask), None)
+                     exposure_function, contrast_window, entropy_window, int(hard_mask), None,
+                     edge_scale, lce_scale, lce_factor, min_curvature)
     if exposure_table is not None:
         tab = np.ascontiguousarray(exposure_table, np.float64)
         p.exposure_table = tab.ctypes.data
@@ -366,6 +371,15 @@ def _checkpoint(self, img, alpha, itype, otype):
     return out, int(ct[0])
 
 
+def _vigra_kernel(self, sigma, order):
+    k = np.zeros(4096, np.float64)
+    r = self.lib.orc_vigra_kernel(sigma, order, _p(k), len(k))
+    if r < 0:
+        raise ValueError("kernel too long")
+    return k[:2 * r + 1].copy()
+
+
+Oracle.vigra_kernel = _vigra_kernel
 Oracle.assemble = _assemble
 Oracle.inspect_overlap = _inspect_overlap
 Oracle.mask_bounds = _mask_bounds
