@@ -125,17 +125,27 @@ class EnfuseParams(ctypes.Structure):
                 ("entropy_window", ctypes.c_int), ("hard_mask", ctypes.c_int),
                 ("exposure_table", ctypes.c_void_p), ("edge_scale", ctypes.c_double),
                 ("lce_scale", ctypes.c_double), ("lce_factor", ctypes.c_double),
-                ("min_curvature", ctypes.c_double)]
+                ("min_curvature", ctypes.c_double), ("exposure_cutoff", ctypes.c_int),
+                ("entropy_cutoff", ctypes.c_int), ("exposure_lower_cutoff", ctypes.c_double),
+                ("exposure_upper_cutoff", ctypes.c_double), ("entropy_lower_cutoff", ctypes.c_double),
+                ("entropy_upper_cutoff", ctypes.c_double)]
 
 
 def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
                   exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXPOSURE_GAUSSIAN,
                   contrast_window=5, entropy_window=3, exposure_table=None, hard_mask=False,
-                  edge_scale=0.0, lce_scale=0.0, lce_factor=0.0, min_curvature=0.0):
+                  edge_scale=0.0, lce_scale=0.0, lce_factor=0.0, min_curvature=0.0,
+                  exposure_cutoff=None, entropy_cutoff=None):
     """Parameters with the reference's defaults (src/enfuse.cc:118-146)."""
     p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
                      exposure_function, contrast_window, entropy_window, int(hard_mask), None,
                      edge_scale, lce_scale, lce_factor, min_curvature)
+    if exposure_cutoff is not None:   # (lower, upper), instantiated for the scalar type
+        p.exposure_cutoff = 1
+        p.exposure_lower_cutoff, p.exposure_upper_cutoff = exposure_cutoff
+    if entropy_cutoff is not None:
+        p.entropy_cutoff = 1
+        p.entropy_lower_cutoff, p.entropy_upper_cutoff = entropy_cutoff
     if exposure_table is not None:
         import numpy as np
         tab = np.ascontiguousarray(exposure_table, np.float64)

@@ -14,6 +14,8 @@ struct WeightArgs {
     double w_exposure, w_saturation, w_contrast, w_entropy;
     double exposure_optimum, exposure_width;   // width already rescaled to the function's FWHM
     int exposure_function;
+    int exposure_cutoff;                       // CutoffExposureFunctor: min channel >= lo, max channel <= hi
+    double exposure_lo, exposure_hi;
 };
 
 int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s);
@@ -34,6 +36,10 @@ int launch_hard_mask(float* masks, long mp, size_t plane, int w, int h, int K, h
 int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
                  int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
 int launch_fill(void* p, size_t bytes, hipStream_t s);
+// The image clamped for --entropy-cutoff (src/enfuse.h:1179-1185), packed
+// (pitch 3 w): channel <= lo -> 0 (the blue channel tests red), >= up -> max.
+int launch_entropy_clamp(int itype, const void* img, long ip, int w, int h, double lo, double up, void* out,
+                         hipStream_t s);
 
 // Contrast by Laplacian-of-Gaussian edges (src/enfuse.h:1020-1093): the
 // separable passes of VIGRA's laplacianOfGaussian / gaussianSharpening with

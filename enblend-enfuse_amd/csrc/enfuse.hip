@@ -594,7 +594,13 @@ __global__ __launch_bounds__(kTile) void weights_kernel(const T* __restrict__ im
     float m = 0.f;
     if (al) {
         const T* p = img + (long)y * ip + 3L * x;
-        if (a.w_exposure > 0.0) {
+        // CutoffExposureFunctor with the default projectors: a pixel whose
+        // minimum channel is below the lower cutoff or whose maximum channel
+        // is above the upper one weighs 0
+        const bool cut_out = a.exposure_cutoff &&
+                             !((double)min(p[0], min(p[1], p[2])) >= a.exposure_lo &&
+                               (double)max(p[0], max(p[1], p[2])) <= a.exposure_hi);
+        if (a.w_exposure > 0.0 && !cut_out) {
             if (Img<T>::integral) {
                 m = exp_lut[(int)Gray<T, T>::get(p)];
             } else {
@@ -760,6 +766,23 @@ __global__ __launch_bounds__(256) void edge_log_kernel(EdgePlanes e, const doubl
         }
     }
     *gmem(grad + (long)y * gp + x) = out;
+}
+
+// --entropy-cutoff: ClampingFunctor<RGB>(lo -> 0, up -> max) applied to the
+// image before the local entropy; its blue channel tests the red channel
+// against the lower bound, as the reference's does.
+template <class T>
+__global__ __launch_bounds__(kTile) void entropy_clamp_kernel(const T* __restrict__ img, long ip, int w, T lo, T up,
+                                                              T mx, T* __restrict__ out)
+{
+    const int x = blockIdx.x * kTile + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const T* p = img + (long)y * ip + 3L * x;
+    T* q = out + ((long)y * w + x) * 3;
+    const T r = p[0], g = p[1], b = p[2];
+    q[0] = r <= lo ? (T)0 : (r >= up ? mx : r);
+    q[1] = g <= lo ? (T)0 : (g >= up ? mx : g);
+    q[2] = r <= lo ? (T)0 : (b >= up ? mx : b);
 }
 
 }  // namespace
@@ -942,6 +965,22 @@ int launch_edge_log(int itype, const EdgePlanes& e, const double* ks, int rs, co
             hipLaunchKernelGGL((edge_log_kernel<int32_t, double, false>), g, dim3(256), 0, s, e, ks, rs, kd, rd, alpha,
                                ap, (const int32_t*)lc, lp, mc, smax, (int32_t*)grad, gp);
     }
+    return 0;
+}
+
+int launch_entropy_clamp(int itype, const void* img, long ip, int w, int h, double lo, double up, void* out,
+                         hipStream_t s)
+{
+    const dim3 g((unsigned)((w + kTile - 1) / kTile), (unsigned)h);
+    if (itype == ENBL_U8)
+        hipLaunchKernelGGL(entropy_clamp_kernel<uint8_t>, g, dim3(kTile), 0, s, (const uint8_t*)img, ip, w,
+                           (uint8_t)lo, (uint8_t)up, (uint8_t)255, (uint8_t*)out);
+    else if (itype == ENBL_U16)
+        hipLaunchKernelGGL(entropy_clamp_kernel<uint16_t>, g, dim3(kTile), 0, s, (const uint16_t*)img, ip, w,
+                           (uint16_t)lo, (uint16_t)up, (uint16_t)65535, (uint16_t*)out);
+    else
+        hipLaunchKernelGGL(entropy_clamp_kernel<float>, g, dim3(kTile), 0, s, (const float*)img, ip, w, (float)lo,
+                           (float)up, FLT_MAX, (float*)out);
     return 0;
 }
 

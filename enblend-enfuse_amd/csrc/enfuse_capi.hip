@@ -88,6 +88,18 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
     if (!p) return fail(ENBL_E_ARG, "enfuse: null parameters");
     if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "enfuse: image type %d", itype);
     if (p->hard_mask != 0 && p->hard_mask != 1) return fail(ENBL_E_ARG, "enfuse: hard_mask %d", p->hard_mask);
+    if (p->exposure_cutoff && p->w_exposure > 0.0) {
+        if (p->exposure_lower_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative lower exposure cutoff");
+        if (p->exposure_upper_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative upper exposure cutoff");
+        if (p->exposure_lower_cutoff > p->exposure_upper_cutoff)
+            return fail(ENBL_E_PRECONDITION, "lower exposure cutoff exceeds upper cutoff");
+    }
+    if (p->entropy_cutoff && p->w_entropy > 0.0) {
+        if (p->entropy_lower_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative lower entropy cutoff");
+        if (p->entropy_upper_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative upper entropy cutoff");
+        if (p->entropy_lower_cutoff > p->entropy_upper_cutoff)
+            return fail(ENBL_E_PRECONDITION, "lower entropy cutoff exceeds upper cutoff");
+    }
     if (p->exposure_function < 0 || p->exposure_function > ENBL_EXPOSURE_BISQUARE)
         return fail(ENBL_E_ARG, "enfuse: exposure function %d", p->exposure_function);
     if (p->w_exposure > 0.0 && (p->exposure_optimum < 0.0 || p->exposure_optimum > 1.0))
@@ -119,6 +131,7 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
 WeightArgs weight_args(const enbl_enfuse_params* p)
 {
     WeightArgs a;
+    memset(&a, 0, sizeof a);
     a.w_exposure = p->w_exposure;
     a.w_saturation = p->w_saturation;
     a.w_contrast = p->w_contrast;
@@ -127,6 +140,9 @@ WeightArgs weight_args(const enbl_enfuse_params* p)
     a.exposure_function = p->exposure_function;
     a.exposure_width = p->exposure_function == 0 ? p->exposure_width
                                                  : p->exposure_width * kFwhmGaussian / kFwhm[p->exposure_function];
+    a.exposure_cutoff = p->exposure_cutoff ? 1 : 0;
+    a.exposure_lo = p->exposure_cutoff ? p->exposure_lower_cutoff : 0.0;
+    a.exposure_hi = p->exposure_cutoff ? p->exposure_upper_cutoff : 0.0;
     return a;
 }
 
@@ -352,8 +368,21 @@ int image_weights(enbl_context* ctx, int itype, int w, int h, int m, const void*
     int rc;
     const void* grad = ws.grad ? (const void*)((const char*)ws.grad + (size_t)m * ws.plane * 4) : nullptr;
     if (p->w_entropy > 0.0) {
+        const void* src = img;
+        long sp = ip;
+        if (p->entropy_cutoff) {
+            void* tr;
+            rc = scratch(ctx, 30, (size_t)w * h * 3 * ptype_size(itype), &tr);
+            if (rc) return rc;
+            Timed t(ctx, "entropy_clamp", (double)w * h * 6.0 * ptype_size(itype));
+            rc = launch_entropy_clamp(itype, img, ip, w, h, p->entropy_lower_cutoff, p->entropy_upper_cutoff, tr, s);
+            if (!rc) rc = launch_status("entropy_clamp");
+            if (rc) return rc;
+            src = tr;
+            sp = 3L * w;
+        }
         Timed t(ctx, "entropy", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + ptype_size(itype)));
-        rc = launch_entropy(itype, img, ip, alpha, ap, w, h, p->entropy_window, ws.tab, ws.ent, ws.pitch, s);
+        rc = launch_entropy(itype, src, sp, alpha, ap, w, h, p->entropy_window, ws.tab, ws.ent, ws.pitch, s);
         if (rc) return fail(rc, "localEntropyIf: unsupported window");
         rc = launch_status("entropy");
         if (rc) return rc;

@@ -206,6 +206,15 @@ typedef struct enbl_enfuse_params {
      * (static_cast<double>(MinCurvature.instantiate<ScalarType>())).  Zero
      * edge_scale: the local standard deviation. */
     double edge_scale, lce_scale, lce_factor, min_curvature;
+    /* --exposure-cutoff (CutoffExposureFunctor, src/enfuse.h:654-720, with
+     * the default projectors: minimum channel >= lower, maximum channel <=
+     * upper, else weight 0) and --entropy-cutoff (the image clamped before
+     * the local entropy, src/enfuse.h:1145-1188): 1 when the reference finds
+     * the cutoff effective, with the cutoffs instantiated for the image's
+     * scalar type. */
+    int exposure_cutoff, entropy_cutoff;
+    double exposure_lower_cutoff, exposure_upper_cutoff;
+    double entropy_lower_cutoff, entropy_upper_cutoff;
 } enbl_enfuse_params;
 
 /* Replaces enfuseMask<ImageType, AlphaType, MaskType>(src, mask, result) --

@@ -106,6 +106,9 @@ typedef struct orc_enfuse_params {
                                        gray code of an integer image (user
                                        ExposureWeight); NULL = built-in */
     double edge_scale, lce_scale, lce_factor, min_curvature;   /* LoG contrast (src/enfuse.h:1020-1093) */
+    int exposure_cutoff, entropy_cutoff;          /* --exposure-cutoff / --entropy-cutoff in effect */
+    double exposure_lower_cutoff, exposure_upper_cutoff;   /* instantiated for the scalar type */
+    double entropy_lower_cutoff, entropy_upper_cutoff;
 } orc_enfuse_params;
 
 /* enfuseMask (src/enfuse.h:962-1212) for one RGB image with alpha: the float

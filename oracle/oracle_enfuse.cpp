@@ -338,12 +338,28 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
 
     if (p->w_exposure > 0.0) {
         const Exposure ex(p->exposure_function, p->exposure_optimum, p->exposure_width);
+        // CutoffExposureFunctor (src/enfuse.h:654-720) with its default
+        // projectors: the minimum channel ("anti-value") at or above the
+        // lower cutoff and the maximum channel ("value") at or below the upper
+        const bool cut = p->exposure_cutoff != 0;
+        if (cut && (p->exposure_lower_cutoff < 0.0 || p->exposure_upper_cutoff < 0.0 ||
+                    p->exposure_lower_cutoff > p->exposure_upper_cutoff))
+            return -1;
 #pragma omp parallel for schedule(guided)
         for (int y = 0; y < h; ++y)
             for (int x = 0; x < w; ++x) {
                 const long i = (long)y * w + x;
                 if (!alpha[i]) continue;
-                const T g = Gray<T, T>::get(img + 3 * i);
+                const T* px = img + 3 * i;
+                if (cut) {
+                    const T lo = std::min(px[0], std::min(px[1], px[2]));
+                    const T hi = std::max(px[0], std::max(px[1], px[2]));
+                    if (!((double)lo >= p->exposure_lower_cutoff && (double)hi <= p->exposure_upper_cutoff)) {
+                        mask[i] = 0.0f;
+                        continue;
+                    }
+                }
+                const T g = Gray<T, T>::get(px);
                 double wt;
                 if (p->exposure_table && is_integral<T>()) wt = p->exposure_table[GrayIndex<T>::of(g)];
                 else wt = ex.weight(exposure_y<T>(g));
@@ -400,7 +416,25 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
         const int n = p->entropy_window;
         if (w < n || h < n) return -1;
         std::vector<T> ent((size_t)N * 3, T(0));
-        local_entropy<T>(img, alpha, w, h, n, ent.data());
+        if (p->entropy_cutoff) {
+            // the image clamped before the entropy (src/enfuse.h:1145-1188):
+            // ClampingFunctor<RGB>(lower -> 0, upper -> max), whose blue
+            // channel tests the red channel against the lower bound
+            const double lc = p->entropy_lower_cutoff, uc = p->entropy_upper_cutoff;
+            if (lc < 0.0 || uc < 0.0 || lc > uc) return -1;
+            const T lo = (T)lc, up = (T)uc, mx = (T)Max<T>::v();
+            std::vector<T> trunc((size_t)N * 3);
+#pragma omp parallel for schedule(guided)
+            for (long i = 0; i < N; ++i) {
+                const T r = img[3 * i], g = img[3 * i + 1], b = img[3 * i + 2];
+                trunc[3 * i] = r <= lo ? T(0) : (r >= up ? mx : r);
+                trunc[3 * i + 1] = g <= lo ? T(0) : (g >= up ? mx : g);
+                trunc[3 * i + 2] = r <= lo ? T(0) : (b >= up ? mx : b);
+            }
+            local_entropy<T>(trunc.data(), alpha, w, h, n, ent.data());
+        } else {
+            local_entropy<T>(img, alpha, w, h, n, ent.data());
+        }
 #pragma omp parallel for schedule(guided)
         for (long i = 0; i < N; ++i) {
             if (!alpha[i]) continue;
