@@ -33,9 +33,22 @@ int launch_normalize(int mtype, const float* mask, long mp, const float* norm, l
                      float fallback, double scale, int hard, void* out, long op, hipStream_t s);
 // Hard masks over K weight planes (plane m at masks + m * plane).
 int launch_hard_mask(float* masks, long mp, size_t plane, int w, int h, int K, hipStream_t s);
-int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
-                 int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s);
 int launch_fill(void* p, size_t bytes, hipStream_t s);
+
+// Merges of n <= kMergeMax images' Laplacian levels (3 planes each, one
+// geometry) weighted by their mask levels into the result level, in image
+// order; `first`: the result level is assigned by the first image.
+constexpr int kMergeMax = 8;
+struct MergeMany {
+    const void* lp[kMergeMax];
+    const void* g[kMergeMax];
+    long lpp, lpl, gpp;
+    void* res;
+    long rp, rl;
+    double maxpyr;
+    int w, h, n, first;
+};
+int launch_merge_many(int ptype, int mtype, const MergeMany& a, hipStream_t s);
 // The image clamped for --entropy-cutoff (src/enfuse.h:1179-1185), packed
 // (pitch 3 w): channel <= lo -> 0 (the blue channel tests red), >= up -> max.
 int launch_entropy_clamp(int itype, const void* img, long ip, int w, int h, double lo, double up, void* out,

@@ -669,23 +669,36 @@ __global__ __launch_bounds__(kTile) void hard_mask_kernel(float* __restrict__ ma
 }
 
 // ImageMaskMultiplyFunctor (src/enfuse.h:601-620) fused with the
-// accumulation into the result pyramid (src/enfuse.h:1576-1588).
-template <class P, class M, bool FIRST>
-__global__ __launch_bounds__(kTile) void merge_kernel(const P* __restrict__ lp, long lpp, long lpl,
-                                                      const M* __restrict__ g, long gpp, int w,
-                                                      double maxpyr, P* __restrict__ res, long rp, long rl)
+// accumulation into the result pyramid (src/enfuse.h:1576-1588), for up to
+// kMergeMax images at one level in one pass: per image the coefficient mask /
+// max, the product rounded to the pyramid type, added with its wrap, in image
+// order (the first of the call assigns when `first`), so the result keeps
+// every rounding of the image-by-image accumulation while it is read and
+// written once.
+template <class P, class M>
+__global__ __launch_bounds__(kTile) void merge_many_kernel(MergeMany a)
 {
+    typedef typename Pix<P>::S S;
     const int x = blockIdx.x * kTile + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= w) return;
-    const double coeff = (double)g[(long)y * gpp + x] / maxpyr;
+    if (x >= a.w) return;
+    P r[3];
+    P* out = reinterpret_cast<P*>(a.res) + (long)y * a.rp + x;
+    if (!a.first) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const P v = from_real<P>((double)lp[c * lpl + (long)y * lpp + x] * coeff);
-        P* r = res + c * rl + (long)y * rp + x;
-        if (FIRST) *r = v;
-        else *r = Pix<P>::wrap(Pix<P>::A::add((typename Pix<P>::S)v, (typename Pix<P>::S)*r));
+        for (int c = 0; c < 3; ++c) r[c] = *gmem(out + c * a.rl);
     }
+    for (int k = 0; k < a.n; ++k) {
+        const double coeff = (double)*gmem(reinterpret_cast<const M*>(a.g[k]) + (long)y * a.gpp + x) / a.maxpyr;
+        const P* l = reinterpret_cast<const P*>(a.lp[k]) + (long)y * a.lpp + x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const P v = from_real<P>((double)*gmem(l + c * a.lpl) * coeff);
+            r[c] = (a.first && k == 0) ? v : Pix<P>::wrap(Pix<P>::A::add((S)v, (S)r[c]));
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) *gmem(out + c * a.rl) = r[c];
 }
 
 // ---- contrast by Laplacian-of-Gaussian edges ------------------------------
@@ -900,24 +913,6 @@ int launch_hard_mask(float* masks, long mp, size_t plane, int w, int h, int K, h
     return 0;
 }
 
-int launch_merge(int ptype, int mtype, bool first, const void* lp, long lpp, long lpl, const void* g, long gpp,
-                 int w, int h, double maxpyr, void* res, long rp, long rl, hipStream_t s)
-{
-#define ENBL_MERGE(P, M)                                                                                           \
-    do {                                                                                                           \
-        if (first) hipLaunchKernelGGL((merge_kernel<P, M, true>), grid_for(w, h), dim3(kTile), 0, s, (const P*)lp, lpp, lpl, \
-                                      (const M*)g, gpp, w, maxpyr, (P*)res, rp, rl);                               \
-        else hipLaunchKernelGGL((merge_kernel<P, M, false>), grid_for(w, h), dim3(kTile), 0, s, (const P*)lp, lpp, lpl,     \
-                                (const M*)g, gpp, w, maxpyr, (P*)res, rp, rl);                                     \
-        return 0;                                                                                                  \
-    } while (0)
-    if (ptype == ENBL_I16 && mtype == ENBL_I16) ENBL_MERGE(int16_t, int16_t);
-    if (ptype == ENBL_I32 && mtype == ENBL_I32) ENBL_MERGE(int32_t, int32_t);
-    if (ptype == ENBL_F64 && mtype == ENBL_I32) ENBL_MERGE(double, int32_t);
-#undef ENBL_MERGE
-    return ENBL_E_UNSUPPORTED;
-}
-
 int launch_edge_rows(int itype, const EdgePlanes& e, const double* ka, int ra, const double* kb, int rb, hipStream_t s)
 {
     const dim3 g((unsigned)((e.w + 255) / 256), (unsigned)e.h);
@@ -982,6 +977,16 @@ int launch_entropy_clamp(int itype, const void* img, long ip, int w, int h, doub
         hipLaunchKernelGGL(entropy_clamp_kernel<float>, g, dim3(kTile), 0, s, (const float*)img, ip, w, (float)lo,
                            (float)up, FLT_MAX, (float*)out);
     return 0;
+}
+
+int launch_merge_many(int ptype, int mtype, const MergeMany& a, hipStream_t s)
+{
+    if (a.n < 1 || a.n > kMergeMax) return ENBL_E_ARG;
+    const dim3 g((unsigned)((a.w + kTile - 1) / kTile), (unsigned)a.h);
+    if (ptype == ENBL_I16 && mtype == ENBL_I16) { hipLaunchKernelGGL((merge_many_kernel<int16_t, int16_t>), g, dim3(kTile), 0, s, a); return 0; }
+    if (ptype == ENBL_I32 && mtype == ENBL_I32) { hipLaunchKernelGGL((merge_many_kernel<int32_t, int32_t>), g, dim3(kTile), 0, s, a); return 0; }
+    if (ptype == ENBL_F64 && mtype == ENBL_I32) { hipLaunchKernelGGL((merge_many_kernel<double, int32_t>), g, dim3(kTile), 0, s, a); return 0; }
+    return ENBL_E_UNSUPPORTED;
 }
 
 }  // namespace enbl

@@ -473,15 +473,26 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     const int mt = mask_ptype_for(itype);
     const int ifrac = itype == ENBL_F32 ? 0 : 7;
     const int mfrac = itype == ENBL_U8 ? 7 : 15;
-    enbl_pyramid *lp = nullptr, *res = nullptr, *mgp = nullptr;
-    rc = cached_pyramid(ctx, pt, 3, w, h, levels, 0, &lp);
-    if (!rc) rc = cached_pyramid(ctx, pt, 3, w, h, levels, 1, &res);
-    if (!rc) rc = cached_pyramid(ctx, mt, 1, w, h, levels, 0, &mgp);
+    // Every image's Laplacian pyramid and normalised mask pyramid are kept
+    // (context cache: the result pyramid is image slot 0's neighbour), then
+    // each level merges up to kMergeMax images per pass in image order, so
+    // the result level is read and written once per group instead of once
+    // per image.
+    enbl_pyramid* res = nullptr;
+    std::vector<enbl_pyramid*> lps((size_t)K, nullptr), mgps((size_t)K, nullptr);
+    rc = cached_pyramid(ctx, pt, 3, w, h, levels, 1, &res);
     if (rc) return rc;
+    for (int m = 0; m < K; ++m) {
+        rc = cached_pyramid(ctx, pt, 3, w, h, levels, m == 0 ? 0 : m + 1, &lps[(size_t)m]);
+        if (!rc) rc = cached_pyramid(ctx, mt, 1, w, h, levels, m, &mgps[(size_t)m]);
+        if (rc) return rc;
+    }
     const float fallback = (float)(255 / K);
     const double scale = (double)(1u << mfrac);
     const double maxpyr = (double)(255 << mfrac);
     for (int m = 0; m < K; ++m) {
+        enbl_pyramid* lp = lps[(size_t)m];
+        enbl_pyramid* mgp = mgps[(size_t)m];
         rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, imgs[m], ips[m], alphas[m], aps[m], lp);
         if (rc) return rc;
         {
@@ -494,13 +505,30 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         }
         rc = gaussian_levels(ctx, mgp, wrap, (const uint8_t*)ua, pitch);
         if (rc) return rc;
-        for (int l = 0; l < levels; ++l) {
-            const Level& L = lp->lv[l];
-            const Level& G = mgp->lv[l];
-            const Level& R = res->lv[l];
-            Timed t(ctx, "merge", (double)L.w * L.h * ((m ? 3.0 : 2.0) * 3 * ptype_size(pt) + ptype_size(mt)));
-            rc = launch_merge(pt, mt, m == 0, level_ptr(lp, l), L.pitch, L.plane, level_ptr(mgp, l), G.pitch,
-                              L.w, L.h, maxpyr, level_ptr(res, l), R.pitch, R.plane, s);
+    }
+    for (int l = 0; l < levels; ++l) {
+        const Level& L = lps[0]->lv[l];
+        const Level& G = mgps[0]->lv[l];
+        const Level& R = res->lv[l];
+        for (int m0 = 0; m0 < K; m0 += kMergeMax) {
+            MergeMany a;
+            memset(&a, 0, sizeof a);
+            a.n = std::min(kMergeMax, K - m0);
+            for (int k = 0; k < a.n; ++k) {
+                a.lp[k] = level_ptr(lps[(size_t)(m0 + k)], l);
+                a.g[k] = level_ptr(mgps[(size_t)(m0 + k)], l);
+            }
+            a.lpp = L.pitch; a.lpl = L.plane; a.gpp = G.pitch;
+            a.res = level_ptr(res, l); a.rp = R.pitch; a.rl = R.plane;
+            a.maxpyr = maxpyr; a.w = L.w; a.h = L.h; a.first = m0 == 0;
+            // credited: the image-by-image merges it replaces; own: each
+            // image's level and mask read once, the result written once (and
+            // read once for later groups)
+            const double esp = 3.0 * ptype_size(pt), px = (double)L.w * L.h;
+            const double credited = px * (a.n * (esp + ptype_size(mt)) + esp * (2.0 * a.n - (m0 == 0 ? 1.0 : 0.0)));
+            const double own = px * (a.n * (esp + ptype_size(mt)) + esp * (m0 == 0 ? 1.0 : 2.0));
+            Timed t(ctx, "merge", credited, own);
+            rc = launch_merge_many(pt, mt, a, s);
             if (!rc) rc = launch_status("merge");
             if (rc) return rc;
         }
