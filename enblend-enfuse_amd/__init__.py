@@ -39,6 +39,8 @@ _SIGS = {
     "enbl_distance_transform": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _c_int, _vp, _c_long]),
     "enbl_nft_mask": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int,
                                _vp, _c_long]),
+    "enbl_nft_mask_periodic": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int, _c_int,
+                                        _vp, _c_long]),
     "enbl_set_kernel_policy": (_c_int, [_c_int]),
     "enbl_set_strip_shape": (_c_int, [_c_int, _c_int]),
     "enbl_reduce": (_c_int, [_vp, _c_int, _c_int, _c_int,

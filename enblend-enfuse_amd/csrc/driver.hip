@@ -614,7 +614,7 @@ extern "C" int enbl_enblend_step(enbl_context* ctx, int itype, int C, int uw, in
         return 0;
     }
     const int w = uBB.x1 - uBB.x0, h = uBB.y1 - uBB.y0;
-    if (wrap && w == uw) return fail(ENBL_E_UNSUPPORTED, "enbl_enblend_step: seam mask with wraparound");
+    const bool wrap_mask = wrap && w == uw;   // wraparoundForMask
     if (empty(uBB)) return fail(ENBL_E_PRECONDITION, "enbl_enblend_step: neither image has an alpha pixel");
 
     // Seam mask over uBB: nearest-feature transform of the two alphas.
@@ -622,7 +622,8 @@ extern "C" int enbl_enblend_step(enbl_context* ctx, int itype, int C, int uw, in
     rc = scratch(ctx, 25, (size_t)w * h, &mp);
     if (rc) return rc;
     uint8_t* mask = (uint8_t*)mp;
-    rc = enbl_nft_mask(ctx, w, h, walpha + uwa, wapitch, balpha + ua, bapitch, norm, mask, w);
+    rc = wrap_mask ? enbl_nft_mask_periodic(ctx, w, h, walpha + uwa, wapitch, balpha + ua, bapitch, norm, 1, mask, w)
+                   : enbl_nft_mask(ctx, w, h, walpha + uwa, wapitch, balpha + ua, bapitch, norm, mask, w);
     if (rc) return rc;
 
     // maskBounds: mBB relative to the canvas.
@@ -642,12 +643,15 @@ extern "C" int enbl_enblend_step(enbl_context* ctx, int itype, int C, int uw, in
         mBB = enbl_rect{mBB.x0 + uBB.x0, mBB.y0 + uBB.y0, mBB.x1 + uBB.x0, mBB.y1 + uBB.y0};
     }
 
-    // roiBounds: mBB grown by filterHalfWidth(29) = 2^30 - 4 and clipped to
-    // uBB, which is all of uBB for any canvas below 2^29 pixels a side; the
+    // roiBounds: mBB grown by filterHalfWidth(29) = 2^30 - 4 (made [0, uBB.x1)
+    // wide when the mask wraps and the box leaves that) and clipped to uBB,
+    // which is all of uBB for any canvas below 2^29 pixels a side; the
     // reference's copy of white pixels outside the ROI and its alpha union
     // outside the ROI are then empty.
     const int border = (1 << 30) - 4;
-    const enbl_rect roi = rect_intersect(enbl_rect{mBB.x0 - border, mBB.y0 - border, mBB.x1 + border, mBB.y1 + border}, uBB);
+    enbl_rect grown{mBB.x0 - border, mBB.y0 - border, mBB.x1 + border, mBB.y1 + border};
+    if (wrap_mask && (grown.x0 < 0 || grown.x1 > uBB.x1)) { grown.x0 = 0; grown.x1 = uBB.x1; }
+    const enbl_rect roi = rect_intersect(grown, uBB);
     if (roi.x0 != uBB.x0 || roi.y0 != uBB.y0 || roi.x1 != uBB.x1 || roi.y1 != uBB.y1)
         return fail(ENBL_E_UNSUPPORTED, "enbl_enblend_step: ROI smaller than the union box");
     const int levels = enbl_auto_levels(w, h, minimum_levels, exact_levels);

@@ -77,7 +77,7 @@ std::vector<Level> plan_levels(int ptype, int channels, int w, int h, int levels
 struct enbl_context;
 struct enbl_pyramid;
 
-constexpr int kScratchSlots = 31;
+constexpr int kScratchSlots = 32;
 
 namespace enbl {
 struct BatchState;   // batched enfuse: arena, job tables, captured graph (batch_capi.hip)

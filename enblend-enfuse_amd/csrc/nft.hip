@@ -404,6 +404,28 @@ __global__ __launch_bounds__(256) void nft_mask_kernel(NftArgs a)
     *gmem(a.mask + (long)y * a.mpitch + x) = *gmem(a.dist + i) < *gmem(a.dist + a.dplane + i) ? 255 : 0;
 }
 
+// quadruple_image (src/nearest.h:138-185): the w x h alpha tiled tx x ty
+// times into a W x H image (pitch W).
+__global__ __launch_bounds__(256) void nft_tile_kernel(const uint8_t* __restrict__ src, long sp, int w, int h,
+                                                       uint8_t* __restrict__ dst, int W)
+{
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= W) return;
+    const int x = X >= w ? X - w : X, y = Y >= h ? Y - h : Y;
+    *gmem(dst + (long)Y * W + X) = *gmem(src + (long)y * sp + x);
+}
+
+// quater_image (src/nearest.h:200-285) fused with the mask: a doubled axis
+// of length n takes position p + n for p < n / 2, p otherwise.
+__global__ __launch_bounds__(256) void nft_mask_periodic_kernel(NftArgs a, int w, int h, int hx, int hy)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const int X = hx && x < w / 2 ? x + w : x, Y = hy && y < h / 2 ? y + h : y;
+    const long i = (long)Y * a.dpitch + X;
+    *gmem(a.mask + (long)y * a.mpitch + x) = *gmem(a.dist + i) < *gmem(a.dist + a.dplane + i) ? 255 : 0;
+}
+
 __global__ void nft_zero_kernel(unsigned* p) { if (threadIdx.x < 2) p[threadIdx.x] = 0u; }
 
 }  // namespace
@@ -515,6 +537,59 @@ int enbl_nft_mask(enbl_context* ctx, int w, int h, const uint8_t* white, long wp
     // complete and the seam line is meaningless (src/nearest.h:449-457, :523-538)
     const unsigned long threshold = 2UL * 2UL * ((unsigned long)h + (unsigned long)w);
     if (std::max(tally[0], tally[1]) < threshold)
+        return fail(ENBL_E_PRECONDITION, "excessive image overlap detected; too high risk of defective seam line");
+    return ENBL_OK;
+}
+
+int enbl_nft_mask_periodic(enbl_context* ctx, int w, int h, const uint8_t* white, long wpitch, const uint8_t* black,
+                           long bpitch, int norm, int boundary, uint8_t* mask, long mpitch)
+{
+    if (!ctx || !white || !black || !mask) return fail(ENBL_E_ARG, "enbl_nft_mask_periodic: null argument");
+    if (w < 1 || h < 1 || wpitch < w || bpitch < w || mpitch < w)
+        return fail(ENBL_E_SIZE, "enbl_nft_mask_periodic: bad size or pitch");
+    if (norm != 1 && norm != 2) return fail(ENBL_E_UNSUPPORTED, "enbl_nft_mask_periodic: norm %d", norm);
+    if (boundary < 1 || boundary > 3) return fail(ENBL_E_ARG, "enbl_nft_mask_periodic: boundary %d", boundary);
+    const int hx = boundary == 1 || boundary == 3, hy = boundary == 2 || boundary == 3;
+    const int W = hx ? 2 * w : w, H = hy ? 2 * h : h;
+    if ((long)W * H >= (1L << 31)) return fail(ENBL_E_SIZE, "enbl_nft_mask_periodic: image too large");
+    const size_t plane = (size_t)W * H;
+    void *dbuf, *tiles;
+    int rc = scratch(ctx, 19, 2 * plane * sizeof(float), &dbuf);
+    if (!rc) rc = scratch(ctx, 31, 2 * plane, &tiles);
+    if (rc) return rc;
+    uint8_t* w2 = (uint8_t*)tiles;
+    uint8_t* b2 = w2 + plane;
+    {
+        Timed t(ctx, "nft_tile", 2.0 * (double)plane * 2.0);
+        const dim3 g((unsigned)((W + 255) / 256), (unsigned)H);
+        hipLaunchKernelGGL(nft_tile_kernel, g, dim3(256), 0, ctx->stream, white, wpitch, w, h, w2, W);
+        hipLaunchKernelGGL(nft_tile_kernel, g, dim3(256), 0, ctx->stream, black, bpitch, w, h, b2, W);
+        rc = launch_status("nft_tile");
+        if (rc) return rc;
+    }
+    NftArgs a;
+    memset(&a, 0, sizeof a);
+    a.white = w2; a.wpitch = W; a.black = b2; a.bpitch = W;
+    a.w = W; a.h = H; a.norm = norm;
+    a.dist = (float*)dbuf; a.dplane = (long)plane; a.dpitch = W;
+    a.mask = mask; a.mpitch = mpitch;
+    rc = nft_passes(ctx, a, 2);
+    if (rc) return rc;
+    {
+        Timed t(ctx, "nft_mask", (double)w * h * 9.0);
+        hipLaunchKernelGGL(nft_mask_periodic_kernel, dim3((w + 255) / 256, h), dim3(256), 0, ctx->stream, a, w, h, hx, hy);
+        rc = launch_status("nft_mask");
+        if (rc) return rc;
+    }
+    unsigned tally[2];
+    rc = hip_check(hipMemcpyAsync(tally, a.tally, sizeof tally, hipMemcpyDeviceToHost, ctx->stream), "nft tally");
+    if (!rc) rc = hip_check(hipStreamSynchronize(ctx->stream), "nft tally");
+    if (rc) return rc;
+    // the tiles hold every exclusive pixel (1 + hx) * (1 + hy) times; the
+    // reference tallies the original difference images
+    const unsigned copies = (unsigned)((1 + hx) * (1 + hy));
+    const unsigned long threshold = 2UL * 2UL * ((unsigned long)h + (unsigned long)w);
+    if (std::max(tally[0] / copies, tally[1] / copies) < threshold)
         return fail(ENBL_E_PRECONDITION, "excessive image overlap detected; too high risk of defective seam line");
     return ENBL_OK;
 }

@@ -346,6 +346,17 @@ int enbl_distance_transform(enbl_context* ctx, int w, int h, const uint8_t* src,
 int enbl_nft_mask(enbl_context* ctx, int w, int h, const uint8_t* white_alpha, long wapitch,
                   const uint8_t* black_alpha, long bapitch, int norm, uint8_t* mask, long mpitch);
 
+/* nearestFeatureTransform with strip boundaries (src/nearest.h:474-520):
+ * boundary 1 = HorizontalStrip (enblend --wrap=horizontal), 2 =
+ * VerticalStrip, 3 = DoubleStrip.  Each difference image is tiled twice
+ * along the wrapped axes (quadruple_image), transformed with open
+ * boundaries and cut back (quater_image: position p + n for p < n / 2 on a
+ * doubled axis of length n); the overlap tally is that of the original
+ * images.  Same results and errors as enbl_nft_mask otherwise. */
+int enbl_nft_mask_periodic(enbl_context* ctx, int w, int h, const uint8_t* white_alpha, long wapitch,
+                           const uint8_t* black_alpha, long bapitch, int norm, int boundary, uint8_t* mask,
+                           long mpitch);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510
@@ -416,8 +427,9 @@ int enbl_mask_bounds(enbl_context* ctx, int w, int h, const uint8_t* mask, long 
  * enbl_enblend_pair on the ROI (*outcome 0).  black/black_alpha and *black_bb
  * are updated as the reference updates blackPair and blackBB.  Canvases are
  * union_w x union_h; *black_bb and *white_bb are the bounding boxes of the
- * two alphas, as enbl_assemble returns them.  A seam mask with wraparound (wraparound set and the
- * union box as wide as the canvas) is ENBL_E_UNSUPPORTED.  Synchronises. */
+ * two alphas, as enbl_assemble returns them.  With wraparound and a union box
+ * as wide as the canvas the seam mask wraps horizontally
+ * (enbl_nft_mask_periodic, boundary 1).  Synchronises. */
 int enbl_enblend_step(enbl_context* ctx, int itype, int channels, int union_w, int union_h, int wraparound,
                       int exact_levels, int minimum_levels, int norm,
                       void* black, long bpitch, uint8_t* black_alpha, long bapitch, enbl_rect* black_bb,

@@ -149,6 +149,10 @@ int orc_vigra_kernel(double sigma, int order, double* k, int n);
  * Returns -2 (mask still written) on excessive overlap. */
 int orc_nft_mask(const uint8_t* white, long wpitch, const uint8_t* black, long bpitch, int w, int h, int norm,
                  uint8_t* mask, long mpitch);
+/* The same with strip boundaries (boundary 1 horizontal, 2 vertical, 3
+ * double; src/nearest.h:474-520, quadruple_image / quater_image). */
+int orc_nft_mask_periodic(const uint8_t* white, long wpitch, const uint8_t* black, long bpitch, int w, int h,
+                          int norm, int boundary, uint8_t* mask, long mpitch);
 /* The distance transform alone: distance of every pixel to the nearest
  * non-zero pixel of src (float, sqrt for the Euclidean norm). */
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);

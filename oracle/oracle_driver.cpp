@@ -228,13 +228,15 @@ bool mask_bounds(const uint8_t* m, int w, int h, Rect* out)
     return !r.empty();
 }
 
-// roiBounds without the wraparound branch: mBB grown by
-// filterHalfWidth(MAX_PYRAMID_LEVELS = 29) = 2^30 - 4, clipped to uBB, then
-// the level count of the ROI's short side.
-int roi_bounds(const Rect& mBB, const Rect& uBB, int min_levels, int exact_levels, Rect* roi)
+// roiBounds: mBB grown by filterHalfWidth(MAX_PYRAMID_LEVELS = 29) =
+// 2^30 - 4, made as wide as uBB's right edge when the seam mask wraps and
+// the ROI leaves it, clipped to uBB, then the level count of the ROI's short
+// side.
+int roi_bounds(const Rect& mBB, const Rect& uBB, bool wrap_mask, int min_levels, int exact_levels, Rect* roi)
 {
     const int border = (1 << 30) - 4;
     Rect r = {mBB.x0 - border, mBB.y0 - border, mBB.x1 + border, mBB.y1 + border};
+    if (wrap_mask && (r.x0 < 0 || r.x1 > uBB.x1)) { r.x0 = 0; r.x1 = uBB.x1; }
     r = rect_intersect(r, uBB);
     *roi = r;
     unsigned s = (unsigned)std::min(r.w(), r.h());
@@ -274,11 +276,12 @@ int step_t(int itype, int C, int uw, int uh, int wrap, int exact_levels, int min
         *outcome = 2;
         return 0;
     }
-    if (wrap && uBB.w() == uw) return -4;   // wraparound seam mask: not restated
+    const bool wrap_mask = wrap && uBB.w() == uw;
     if (uBB.empty()) return -1;
     const int w = uBB.w(), h = uBB.h();
     std::vector<uint8_t> mask((size_t)w * h);
-    int rc = orc_nft_mask(walpha + uoff, uw, balpha + uoff, uw, w, h, norm, mask.data(), w);
+    int rc = wrap_mask ? orc_nft_mask_periodic(walpha + uoff, uw, balpha + uoff, uw, w, h, norm, 1, mask.data(), w)
+                       : orc_nft_mask(walpha + uoff, uw, balpha + uoff, uw, w, h, norm, mask.data(), w);
     if (rc) return -1;
     Rect mBB;
     if (!mask_bounds(mask.data(), w, h, &mBB)) {
@@ -288,7 +291,7 @@ int step_t(int itype, int C, int uw, int uh, int wrap, int exact_levels, int min
         mBB = {mBB.x0 + uBB.x0, mBB.y0 + uBB.y0, mBB.x1 + uBB.x0, mBB.y1 + uBB.y0};
     }
     Rect roi;
-    const int levels = roi_bounds(mBB, uBB, min_levels, exact_levels, &roi);
+    const int levels = roi_bounds(mBB, uBB, wrap_mask, min_levels, exact_levels, &roi);
     const int wrap_blend = wrap && roi.w() == uw;
     if (info) {
         info[0] = mBB.x0; info[1] = mBB.y0; info[2] = mBB.x1; info[3] = mBB.y1;

@@ -123,4 +123,45 @@ int orc_nft_mask(const uint8_t* white, long wpitch, const uint8_t* black, long b
     return std::max(n12, n21) < threshold ? -2 : 0;
 }
 
+// nearestFeatureTransform with strip boundaries (src/nearest.h:474-520):
+// each difference image copied twice side by side (horizontal strip, 1), one
+// above the other (vertical strip, 2) or four times (double strip, 3)
+// (quadruple_image, src/nearest.h:138-185), transformed with open
+// boundaries, and cut back to w x h (quater_image, src/nearest.h:200-285):
+// a doubled axis of length n takes position p + n for p < n / 2 and p
+// otherwise.  The overlap tally is that of the original difference images.
+int orc_nft_mask_periodic(const uint8_t* white, long wpitch, const uint8_t* black, long bpitch, int w, int h,
+                          int norm, int boundary, uint8_t* mask, long mpitch)
+{
+    if (w < 1 || h < 1 || (norm != 1 && norm != 2) || boundary < 1 || boundary > 3) return -1;
+    const bool hx = boundary == 1 || boundary == 3, hy = boundary == 2 || boundary == 3;
+    const int W = hx ? 2 * w : w, H = hy ? 2 * h : h;
+    std::vector<uint8_t> d12((size_t)W * H), d21((size_t)W * H);
+    long n12 = 0, n21 = 0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const uint8_t a = white[(long)y * wpitch + x], b = black[(long)y * bpitch + x];
+            const uint8_t v12 = b < a ? (uint8_t)(a - b) : 0, v21 = a < b ? (uint8_t)(b - a) : 0;
+            n12 += v12 != 0;
+            n21 += v21 != 0;
+            for (int cy = 0; cy < (hy ? 2 : 1); ++cy)
+                for (int cx = 0; cx < (hx ? 2 : 1); ++cx) {
+                    const size_t i = (size_t)(y + cy * h) * W + (x + cx * w);
+                    d12[i] = v12;
+                    d21[i] = v21;
+                }
+        }
+    std::vector<float> t12((size_t)W * H), t21((size_t)W * H);
+    fh_transform(d12.data(), W, W, H, norm, t12.data(), W);
+    fh_transform(d21.data(), W, W, H, norm, t21.data(), W);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const int X = hx && x < w / 2 ? x + w : x, Y = hy && y < h / 2 ? y + h : y;
+            const size_t i = (size_t)Y * W + X;
+            mask[(long)y * mpitch + x] = t12[i] < t21[i] ? 255 : 0;
+        }
+    const long threshold = 2L * 2L * ((long)h + (long)w);
+    return std::max(n12, n21) < threshold ? -2 : 0;
+}
+
 }  // extern "C"

@@ -36,6 +36,7 @@ _SIGS = {
     "orc_diff_reduce_T": [_vp, _i, _i, _vp, _i, _i],
     "orc_nft_mask": [_vp, _l, _vp, _l, _i, _i, _i, _vp, _l],
     "orc_distance_transform": [_vp, _l, _i, _i, _i, _vp, _l],
+    "orc_nft_mask_periodic": [_vp, _l, _vp, _l, _i, _i, _i, _i, _vp, _l],
     "orc_assemble": [_i, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "orc_inspect_overlap": [_vp, _l, _vp, _l, _i, _i],
     "orc_mask_bounds": [_vp, _i, _i, _vp],
@@ -395,6 +396,19 @@ Oracle.inspect_overlap = _inspect_overlap
 Oracle.mask_bounds = _mask_bounds
 Oracle.enblend_step = _enblend_step
 Oracle.checkpoint = _checkpoint
+def _nft_mask_periodic(self, white, black, boundary, norm=2):
+    """(mask, rc) with strip boundaries: 1 horizontal, 2 vertical, 3 double."""
+    white = np.ascontiguousarray(white, np.uint8)
+    black = np.ascontiguousarray(black, np.uint8)
+    h, w = white.shape
+    mask = np.zeros((h, w), np.uint8)
+    rc = self.lib.orc_nft_mask_periodic(_p(white), w, _p(black), w, w, h, norm, boundary, _p(mask), w)
+    if rc not in (0, -2):
+        raise ValueError(f"orc_nft_mask_periodic rc={rc}")
+    return mask, rc
+
+
+Oracle.nft_mask_periodic = _nft_mask_periodic
 Oracle.distance_transform = _distance_transform
 Oracle.nft_mask = _nft_mask
 Oracle.diff_forward = _diff_forward

@@ -216,19 +216,33 @@ def test_enblend_step_redundant_disjoint_levels(enbl, gpu_ctx, oracle):
         np.testing.assert_array_equal(b, eb)
 
 
-def test_enblend_step_wraparound_mask_unsupported(enbl, gpu_ctx):
-    """A union box as wide as the canvas with wraparound needs the seam mask
-    with wraparound boundaries, which is not provided."""
-    black = torch.zeros((8, 24), dtype=torch.uint8, device="cuda")
-    ba = torch.zeros((8, 8), dtype=torch.uint8, device="cuda")
-    wa = torch.zeros((8, 8), dtype=torch.uint8, device="cuda")
-    ba[:, :6] = 255
-    wa[:, 2:] = 255
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+def test_enblend_step_with_wraparound_matches_oracle(enbl, gpu_ctx, oracle, itype):
+    """A union box as wide as the canvas with wraparound: the strip seam mask,
+    the ROI across the whole width and the blend with wraparound."""
+    rng = np.random.default_rng(60 + itype)
+    uw, uh = 640, 211
+    black = _samples(rng, uh, uw, 3, itype)
+    white = _samples(rng, uh, uw, 3, itype)
+    ba = np.zeros((uh, uw), np.uint8)
+    wa = np.zeros((uh, uw), np.uint8)
+    ba[3:uh - 2, : int(0.62 * uw)] = 255
+    wa[1:uh - 4, int(0.40 * uw):] = 255
+    print(f"wraparound step itype={itype} canvas {uw}x{uh}", flush=True)
+    erc, eout, eb, eba, ebb, _ = oracle.enblend_step(black, ba, _bbox(ba), white, wa, _bbox(wa), itype, wrap=True,
+                                                     seed=5)
+    assert erc == 0 and eout == 0
+    d_b, d_ba, d_w, d_wa = _dev(black), _dev(ba), _dev(white), _dev(wa)
+    bb, wb = enbl.Rect(*_bbox(ba)), enbl.Rect(*_bbox(wa))
     outcome = ctypes.c_int(-1)
-    rc = enbl.lib().enbl_enblend_step(gpu_ctx.h, U8, 3, 8, 8, 1, 0, 1, 2, _vp(black), 24, _vp(ba), 8,
-                                      ctypes.byref(enbl.Rect(0, 0, 6, 8)), _vp(black), 24, _vp(wa), 8,
-                                      ctypes.byref(enbl.Rect(2, 0, 8, 8)), 1, 0, ctypes.byref(outcome))
-    assert rc == enbl.ENBL_E_UNSUPPORTED
+    torch.cuda.synchronize()
+    enbl.check(enbl.lib().enbl_enblend_step(gpu_ctx.h, itype, 3, uw, uh, 1, 0, 1, 2, _vp(d_b), uw * 3, _vp(d_ba), uw,
+                                            ctypes.byref(bb), _vp(d_w), uw * 3, _vp(d_wa), uw, ctypes.byref(wb), 1, 5,
+                                            ctypes.byref(outcome)))
+    gpu_ctx.sync()
+    assert outcome.value == 0 and bb.tuple() == ebb
+    np.testing.assert_array_equal(d_ba.cpu().numpy(), eba)
+    np.testing.assert_array_equal(d_b.cpu().numpy(), eb)
 
 
 @pytest.mark.parametrize("itype", [U8, U16, F32])

@@ -103,3 +103,29 @@ def test_nft_mask_excessive_overlap(enbl, gpu_ctx, oracle):
     got, rc = _gpu_nft(enbl, gpu_ctx, white, black)
     assert want_rc == -2 and rc == enbl.ENBL_E_PRECONDITION
     np.testing.assert_array_equal(got, want)
+
+
+def _edge_alphas(rng, h, w):
+    yy, xx = np.mgrid[0:h, 0:w]
+    white = ((xx < 0.35 * w + rng.integers(0, 3)) & (yy > 0.2 * h)).astype(np.uint8) * 255
+    black = (((xx > 0.6 * w) & (yy > 0.1 * h)) | (yy < 0.08 * h + 1)).astype(np.uint8) * 255
+    return white, black
+
+
+@pytest.mark.parametrize("shape", [(37, 61), (300, 517), (1200, 2100), (2100, 700)])
+@pytest.mark.parametrize("boundary", [1, 2, 3])
+def test_periodic_nft_mask_matches_oracle(enbl, gpu_ctx, oracle, shape, boundary):
+    h, w = shape
+    print(f"periodic nft {w}x{h} boundary={boundary}", flush=True)
+    rng = np.random.default_rng(h + 7 * w + boundary)
+    white, black = _edge_alphas(rng, h, w)
+    want, wrc = oracle.nft_mask_periodic(white, black, boundary)
+    dw, db = torch.from_numpy(white).cuda(), torch.from_numpy(black).cuda()
+    dm = torch.full((h, w + 2), 7, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    rc = enbl.lib().enbl_nft_mask_periodic(gpu_ctx.h, w, h, _vp(dw), w, _vp(db), w, 2, boundary, _vp(dm), w + 2)
+    gpu_ctx.sync()
+    assert (rc == enbl.ENBL_E_PRECONDITION) == (wrc == -2)
+    got = dm.cpu().numpy()
+    assert (got[:, w:] == 7).all()
+    np.testing.assert_array_equal(got[:, :w], want)

@@ -285,6 +285,32 @@ def test_step_blends_on_the_union_box(orc, itype):
     np.testing.assert_array_equal(bal, expa)
 
 
+def test_step_with_wraparound_uses_the_strip_mask(orc):
+    """A union box as wide as the canvas with wraparound: the horizontal-strip
+    NFT mask and the pair step with wraparound on the union box."""
+    rng = np.random.default_rng(21)
+    uw, uh = 96, 40
+    black = _samples(rng, uh, uw, 3, U16)
+    white = _samples(rng, uh, uw, 3, U16)
+    ba = np.zeros((uh, uw), np.uint8)
+    wa = np.zeros((uh, uw), np.uint8)
+    ba[1:uh - 1, : int(0.62 * uw)] = 255
+    wa[2:uh - 2, int(0.40 * uw):] = 255
+    bbb, wbb = _bbox(ba), _bbox(wa)
+    rc, outcome, b, bal, nbb, info = orc.enblend_step(black, ba, bbb, white, wa, wbb, U16, wrap=True, seed=2)
+    assert rc == 0 and outcome == 0 and nbb[0] == 0 and nbb[2] == uw
+    x0, y0, x1, y1 = nbb
+    mask, mrc = orc.nft_mask_periodic(wa[y0:y1, x0:x1], ba[y0:y1, x0:x1], 1)
+    assert mrc == 0
+    assert (mask != orc.nft_mask(wa[y0:y1, x0:x1], ba[y0:y1, x0:x1])[0]).any()
+    levels = orc.auto_levels(x1 - x0, y1 - y0)
+    eb, eba = orc.enblend_pair(white[y0:y1], wa[y0:y1], black[y0:y1], ba[y0:y1], mask, levels, True, U16, 1, 2)
+    exp, expa = black.copy(), ba.copy()
+    exp[y0:y1], expa[y0:y1] = eb, eba
+    np.testing.assert_array_equal(b, exp)
+    np.testing.assert_array_equal(bal, expa)
+
+
 def test_step_redundant_and_disjoint(orc):
     rng = np.random.default_rng(8)
     black, ba, white, wa = _pair_scene(rng, U8)

@@ -122,6 +122,37 @@ def test_nft_mask_equals_brute_force(oracle, shape):
     np.testing.assert_array_equal(mask, np.where(d12 < d21, 255, 0).astype(np.uint8))
 
 
+def _edge_alphas(rng, h, w):
+    """White around the left edge, black around the right edge and a band
+    across the top: the strips make them neighbours across the borders."""
+    yy, xx = np.mgrid[0:h, 0:w]
+    white = ((xx < 0.35 * w + rng.integers(0, 3)) & (yy > 0.2 * h)).astype(np.uint8) * 255
+    black = (((xx > 0.6 * w) & (yy > 0.1 * h)) | (yy < 0.08 * h + 1)).astype(np.uint8) * 255
+    return white, black
+
+
+@pytest.mark.parametrize("shape", [(24, 40), (37, 61), (30, 31)])
+@pytest.mark.parametrize("boundary", [1, 2, 3])
+def test_periodic_nft_mask_equals_brute_force(oracle, shape, boundary):
+    """Strip boundaries: the brute-force transform of the difference images
+    tiled twice along each wrapped axis, cut back by taking position p + n
+    for p < n / 2 on a doubled axis of length n."""
+    rng = np.random.default_rng(boundary * 1000 + shape[0] * 10 + shape[1])
+    h, w = shape
+    white, black = _edge_alphas(rng, h, w)
+    mask, rc = oracle.nft_mask_periodic(white, black, boundary)
+    ty, tx = (2 if boundary in (2, 3) else 1), (2 if boundary in (1, 3) else 1)
+    d12 = _brute_d2(np.tile((white > black).astype(np.uint8), (ty, tx)))
+    d21 = _brute_d2(np.tile((black > white).astype(np.uint8), (ty, tx)))
+    ys = np.arange(h) + (h if ty == 2 else 0) * (np.arange(h) < h // 2)
+    xs = np.arange(w) + (w if tx == 2 else 0) * (np.arange(w) < w // 2)
+    want = np.where(d12[np.ix_(ys, xs)] < d21[np.ix_(ys, xs)], 255, 0).astype(np.uint8)
+    np.testing.assert_array_equal(mask, want)
+    # the wrap matters here: the open-boundary mask differs
+    if boundary == 1:
+        assert (oracle.nft_mask(white, black)[0] != mask).any()
+
+
 def test_nft_mask_ties_go_to_black(oracle):
     """Equidistant pixels compare dist12 < dist21 false: 0 (the black image),
     src/nearest.h:540-545; a planted <= gives a different mask."""
