@@ -116,6 +116,8 @@ def layer_table(layers):
 _lib = None
 
 EXPOSURE_GAUSSIAN, EXPOSURE_LORENTZIAN, EXPOSURE_HALFSINE, EXPOSURE_FULLSINE, EXPOSURE_BISQUARE = range(5)
+(GRAY_AVERAGE, GRAY_LSTAR, GRAY_PLSTAR, GRAY_LIGHTNESS, GRAY_VALUE, GRAY_ANTI_VALUE, GRAY_LUMINANCE,
+ GRAY_MIXER) = range(8)
 
 
 class EnfuseParams(ctypes.Structure):
@@ -130,18 +132,21 @@ class EnfuseParams(ctypes.Structure):
                 ("min_curvature", ctypes.c_double), ("exposure_cutoff", ctypes.c_int),
                 ("entropy_cutoff", ctypes.c_int), ("exposure_lower_cutoff", ctypes.c_double),
                 ("exposure_upper_cutoff", ctypes.c_double), ("entropy_lower_cutoff", ctypes.c_double),
-                ("entropy_upper_cutoff", ctypes.c_double)]
+                ("entropy_upper_cutoff", ctypes.c_double), ("gray_projector", ctypes.c_int),
+                ("mixer_red", ctypes.c_double), ("mixer_green", ctypes.c_double), ("mixer_blue", ctypes.c_double)]
 
 
 def enfuse_params(w_exposure=1.0, w_saturation=0.0, w_contrast=0.0, w_entropy=0.0,
                   exposure_optimum=0.5, exposure_width=0.2, exposure_function=EXPOSURE_GAUSSIAN,
                   contrast_window=5, entropy_window=3, exposure_table=None, hard_mask=False,
                   edge_scale=0.0, lce_scale=0.0, lce_factor=0.0, min_curvature=0.0,
-                  exposure_cutoff=None, entropy_cutoff=None):
+                  exposure_cutoff=None, entropy_cutoff=None, gray_projector=0, mixer=(0.0, 0.0, 0.0)):
     """Parameters with the reference's defaults (src/enfuse.cc:118-146)."""
     p = EnfuseParams(w_exposure, w_saturation, w_contrast, w_entropy, exposure_optimum, exposure_width,
                      exposure_function, contrast_window, entropy_window, int(hard_mask), None,
                      edge_scale, lce_scale, lce_factor, min_curvature)
+    p.gray_projector = gray_projector
+    p.mixer_red, p.mixer_green, p.mixer_blue = mixer
     if exposure_cutoff is not None:   # (lower, upper), instantiated for the scalar type
         p.exposure_cutoff = 1
         p.exposure_lower_cutoff, p.exposure_upper_cutoff = exposure_cutoff

@@ -132,7 +132,7 @@ struct Table {
 int batchable(int itype, const enbl_enfuse_params* p, int wrap)
 {
     return (itype == ENBL_U8 || itype == ENBL_U16) && p->w_contrast <= 0.0 && p->w_entropy <= 0.0 && !p->hard_mask &&
-           !p->exposure_cutoff && !wrap;
+           !p->exposure_cutoff && p->gray_projector == ENBL_GRAY_AVERAGE && !wrap;
 }
 
 int check_stacks(int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p)

@@ -10,15 +10,24 @@
 
 namespace enbl {
 
+// A grayscale projector: ENBL_GRAY_* and the normalised mixer weights.
+struct GrayProj {
+    int kind;
+    int pad_;
+    double wr, wg, wb;
+};
+
 struct WeightArgs {
     double w_exposure, w_saturation, w_contrast, w_entropy;
     double exposure_optimum, exposure_width;   // width already rescaled to the function's FWHM
     int exposure_function;
     int exposure_cutoff;                       // CutoffExposureFunctor: min channel >= lo, max channel <= hi
     double exposure_lo, exposure_hi;
+    GrayProj proj;                             // --gray-projector of the exposure gray
 };
 
-int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s);
+int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, const GrayProj& pj,
+                hipStream_t s);
 // localStdDevIf of K gray planes (plane m at gray + m * gstride elements,
 // result plane m at out + m * ostride), each under its own alpha.
 int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const uint8_t* const* alphas,

@@ -44,15 +44,59 @@ template <> __device__ __forceinline__ uint16_t from_real_img<uint16_t>(double v
 template <> __device__ __forceinline__ float from_real_img<float>(double v) { return (float)v; }
 
 // "average" grayscale projector (src/mga.h:188-193).
-template <class T, class R> struct Gray;
-template <class R> struct Gray<uint8_t, R> {
-    __device__ static R get(const uint8_t* p) { return (R)from_real_img<uint8_t>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
-};
-template <class R> struct Gray<uint16_t, R> {
-    __device__ static R get(const uint16_t* p) { return (R)from_real_img<uint16_t>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
-};
-template <class R> struct Gray<float, R> {
-    __device__ static R get(const float* p) { return (R)((double)(p[0] + p[1] + p[2]) / 3.0); }
+// Grayscale projectors (MultiGrayscaleAccessor, src/mga.h:60-250).  Integer
+// channels promote to double, float channels to float; every result lies in
+// the channel range, so rounding it to the channel type and widening it to R
+// equals rounding it to R.  l-star / pl-star: max * L* / 100 of VIGRA's
+// RGB2LabFunctor / RGBPrime2LabFunctor (Y of the sRGB matrix; gamma 1/0.45
+// for the primed one; L* = Y < 216/24389 ? 24389/27 Y : 116 Y^(1/3) - 16).
+template <class T> struct RealOf { typedef double R; };
+template <> struct RealOf<float> { typedef float R; };
+template <class T> __device__ __forceinline__ double chan_max();
+template <> __device__ __forceinline__ double chan_max<uint8_t>() { return 255.0; }
+template <> __device__ __forceinline__ double chan_max<uint16_t>() { return 65535.0; }
+template <> __device__ __forceinline__ double chan_max<float>() { return (double)FLT_MAX; }
+
+__device__ __forceinline__ double lstar(double r, double g, double b)
+{
+    const double Y = 0.212671 * r + 0.715160 * g + 0.072169 * b;
+    return Y < 216.0 / 24389.0 ? 24389.0 / 27.0 * Y : 116.0 * pow(Y, 1.0 / 3.0) - 16.0;
+}
+__device__ __forceinline__ double gamma_correct(double v, double gm) { return v < 0.0 ? -pow(-v, gm) : pow(v, gm); }
+
+template <class T>
+__device__ __forceinline__ T project(const T* p, const GrayProj& pj)
+{
+    typedef typename RealOf<T>::R RP;
+    const T r = p[0], g = p[1], b = p[2];
+    switch (pj.kind) {
+    case ENBL_GRAY_LSTAR:
+    case ENBL_GRAY_PLSTAR: {
+        const double mx = chan_max<T>();
+        double rr = (double)r / mx, gg = (double)g / mx, bb = (double)b / mx;
+        if (pj.kind == ENBL_GRAY_PLSTAR) {
+            rr = gamma_correct(rr, 1.0 / 0.45);
+            gg = gamma_correct(gg, 1.0 / 0.45);
+            bb = gamma_correct(bb, 1.0 / 0.45);
+        }
+        return from_real_img<T>(mx * (lstar(rr, gg, bb) / 100.0));
+    }
+    case ENBL_GRAY_LIGHTNESS: {
+        const T mn = min(r, min(g, b)), mx = max(r, max(g, b));
+        return from_real_img<T>((double)(mn + mx) / 2.0);
+    }
+    case ENBL_GRAY_VALUE: return max(r, max(g, b));
+    case ENBL_GRAY_ANTI_VALUE: return min(r, min(g, b));
+    case ENBL_GRAY_LUMINANCE: return from_real_img<T>(0.3 * (double)r + 0.59 * (double)g + 0.11 * (double)b);
+    case ENBL_GRAY_MIXER:
+        return from_real_img<T>(pj.wr * (double)(RP)r + pj.wg * (double)(RP)g + pj.wb * (double)(RP)b);
+    default:
+        return from_real_img<T>((double)((RP)r + (RP)g + (RP)b) / 3.0);
+    }
+}
+
+template <class T, class R> struct Gray {
+    __device__ static R get(const T* p, const GrayProj& pj) { return (R)project<T>(p, pj); }
 };
 
 __device__ double exposure_builtin(int fn, double opt, double width, double y)
@@ -72,12 +116,13 @@ __device__ double exposure_builtin(int fn, double opt, double width, double y)
 // Gray image for the contrast weight: int for integer images, float for
 // float images (LongScalarType = Promote<ScalarType>).
 template <class T, class G>
-__global__ __launch_bounds__(kTile) void gray_kernel(const T* __restrict__ img, long ip, int w, G* __restrict__ gray, long gp)
+__global__ __launch_bounds__(kTile) void gray_kernel(const T* __restrict__ img, long ip, int w, G* __restrict__ gray, long gp,
+                                                     GrayProj pj)
 {
     const int x = blockIdx.x * kTile + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= w) return;
-    gray[(long)y * gp + x] = Gray<T, G>::get(img + (long)y * ip + 3L * x);
+    gray[(long)y * gp + x] = Gray<T, G>::get(img + (long)y * ip + 3L * x, pj);
 }
 
 // localStdDevIf for integer gray images (src/enfuse.h:75-219): exact window
@@ -602,9 +647,9 @@ __global__ __launch_bounds__(kTile) void weights_kernel(const T* __restrict__ im
                                (double)max(p[0], max(p[1], p[2])) <= a.exposure_hi);
         if (a.w_exposure > 0.0 && !cut_out) {
             if (Img<T>::integral) {
-                m = exp_lut[(int)Gray<T, T>::get(p)];
+                m = exp_lut[(int)Gray<T, T>::get(p, a.proj)];
             } else {
-                const float g = Gray<T, float>::get(p);
+                const float g = Gray<T, float>::get(p, a.proj);
                 const double yv = (double)(g / FLT_MAX);
                 m = (float)(a.w_exposure * exposure_builtin(a.exposure_function, a.exposure_optimum, a.exposure_width, yv));
             }
@@ -800,11 +845,11 @@ __global__ __launch_bounds__(kTile) void entropy_clamp_kernel(const T* __restric
 
 }  // namespace
 
-int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, hipStream_t s)
+int launch_gray(int itype, const void* img, long ip, int w, int h, void* gray, long gp, const GrayProj& pj, hipStream_t s)
 {
-    if (itype == ENBL_U8) hipLaunchKernelGGL((gray_kernel<uint8_t, int32_t>), grid_for(w, h), dim3(kTile), 0, s, (const uint8_t*)img, ip, w, (int32_t*)gray, gp);
-    else if (itype == ENBL_U16) hipLaunchKernelGGL((gray_kernel<uint16_t, int32_t>), grid_for(w, h), dim3(kTile), 0, s, (const uint16_t*)img, ip, w, (int32_t*)gray, gp);
-    else if (itype == ENBL_F32) hipLaunchKernelGGL((gray_kernel<float, float>), grid_for(w, h), dim3(kTile), 0, s, (const float*)img, ip, w, (float*)gray, gp);
+    if (itype == ENBL_U8) hipLaunchKernelGGL((gray_kernel<uint8_t, int32_t>), grid_for(w, h), dim3(kTile), 0, s, (const uint8_t*)img, ip, w, (int32_t*)gray, gp, pj);
+    else if (itype == ENBL_U16) hipLaunchKernelGGL((gray_kernel<uint16_t, int32_t>), grid_for(w, h), dim3(kTile), 0, s, (const uint16_t*)img, ip, w, (int32_t*)gray, gp, pj);
+    else if (itype == ENBL_F32) hipLaunchKernelGGL((gray_kernel<float, float>), grid_for(w, h), dim3(kTile), 0, s, (const float*)img, ip, w, (float*)gray, gp, pj);
     else return ENBL_E_UNSUPPORTED;
     return 0;
 }

@@ -83,11 +83,31 @@ std::vector<double> vigra_kernel(double sigma, int order)
     return k;
 }
 
+GrayProj gray_proj(const enbl_enfuse_params* p)
+{
+    GrayProj g;
+    memset(&g, 0, sizeof g);
+    g.kind = p->gray_projector;
+    if (g.kind == ENBL_GRAY_MIXER) {
+        const double sum = p->mixer_red + p->mixer_green + p->mixer_blue;
+        g.wr = p->mixer_red / sum;
+        g.wg = p->mixer_green / sum;
+        g.wb = p->mixer_blue / sum;
+    }
+    return g;
+}
+
 int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
 {
     if (!p) return fail(ENBL_E_ARG, "enfuse: null parameters");
     if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "enfuse: image type %d", itype);
     if (p->hard_mask != 0 && p->hard_mask != 1) return fail(ENBL_E_ARG, "enfuse: hard_mask %d", p->hard_mask);
+    if (p->gray_projector < ENBL_GRAY_AVERAGE || p->gray_projector > ENBL_GRAY_MIXER)
+        return fail(ENBL_E_ARG, "unknown grayscale projector %d", p->gray_projector);
+    if (p->gray_projector == ENBL_GRAY_MIXER &&
+        (p->mixer_red < 0.0 || p->mixer_green < 0.0 || p->mixer_blue < 0.0 ||
+         p->mixer_red + p->mixer_green + p->mixer_blue == 0.0))
+        return fail(ENBL_E_ARG, "channel-mixer: negative weight or zero sum of weights");
     if (p->exposure_cutoff && p->w_exposure > 0.0) {
         if (p->exposure_lower_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative lower exposure cutoff");
         if (p->exposure_upper_cutoff < 0.0) return fail(ENBL_E_PRECONDITION, "negative upper exposure cutoff");
@@ -143,6 +163,7 @@ WeightArgs weight_args(const enbl_enfuse_params* p)
     a.exposure_cutoff = p->exposure_cutoff ? 1 : 0;
     a.exposure_lo = p->exposure_cutoff ? p->exposure_lower_cutoff : 0.0;
     a.exposure_hi = p->exposure_cutoff ? p->exposure_upper_cutoff : 0.0;
+    a.proj = gray_proj(p);
     return a;
 }
 
@@ -341,7 +362,7 @@ int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* 
     int rc;
     for (int m = 0; m < K; ++m) {
         Timed t(ctx, "gray", (double)w * h * (3.0 * ptype_size(itype) + 4.0));
-        rc = launch_gray(itype, imgs[m], ips[m], w, h, (char*)ws.gray + m * ws.plane * es, ws.pitch, s);
+        rc = launch_gray(itype, imgs[m], ips[m], w, h, (char*)ws.gray + m * ws.plane * es, ws.pitch, gray_proj(p), s);
         if (!rc) rc = launch_status("gray");
         if (rc) return rc;
     }

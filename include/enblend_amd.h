@@ -45,6 +45,16 @@ extern "C" {
 #define ENBL_F32 12
 #define ENBL_U32 13   /* only as the component type of enbl_checkpoint's output */
 
+/* grayscale projectors of enfuse's exposure and contrast weights */
+#define ENBL_GRAY_AVERAGE 0
+#define ENBL_GRAY_LSTAR 1
+#define ENBL_GRAY_PLSTAR 2
+#define ENBL_GRAY_LIGHTNESS 3
+#define ENBL_GRAY_VALUE 4
+#define ENBL_GRAY_ANTI_VALUE 5
+#define ENBL_GRAY_LUMINANCE 6
+#define ENBL_GRAY_MIXER 7
+
 /* dither modes of the pyramid-to-image conversion */
 #define ENBL_DITHER_NONE 0 /* round half up inside the dither band */
 #define ENBL_DITHER_DET 1  /* counter-based draw keyed by (seed, x, y, channel) */
@@ -215,6 +225,12 @@ typedef struct enbl_enfuse_params {
     int exposure_cutoff, entropy_cutoff;
     double exposure_lower_cutoff, exposure_upper_cutoff;
     double entropy_lower_cutoff, entropy_upper_cutoff;
+    /* --gray-projector (MultiGrayscaleAccessor, src/mga.h:60-250) of the
+     * exposure and contrast weights: ENBL_GRAY_*; the channel mixer's
+     * weights as given (divided by their sum; negative or zero sum is
+     * ENBL_E_ARG). */
+    int gray_projector;
+    double mixer_red, mixer_green, mixer_blue;
 } enbl_enfuse_params;
 
 /* Replaces enfuseMask<ImageType, AlphaType, MaskType>(src, mask, result) --

@@ -93,6 +93,8 @@ int orc_pyramid(int laplacian, int itype, int channels, int w, int h, int levels
 int orc_collapse(int ptype, int channels, int w, int h, int levels, int wrap, void* pyr);
 
 /* enfuse weighting parameters (src/enfuse.cc:118-146 defaults). */
+enum { ORC_GRAY_AVERAGE = 0, ORC_GRAY_LSTAR = 1, ORC_GRAY_PLSTAR = 2, ORC_GRAY_LIGHTNESS = 3, ORC_GRAY_VALUE = 4,
+       ORC_GRAY_ANTI_VALUE = 5, ORC_GRAY_LUMINANCE = 6, ORC_GRAY_MIXER = 7 };
 enum { ORC_EXP_GAUSSIAN = 0, ORC_EXP_LORENTZIAN = 1, ORC_EXP_HALFSINE = 2, ORC_EXP_FULLSINE = 3,
        ORC_EXP_BISQUARE = 4 };
 typedef struct orc_enfuse_params {
@@ -109,6 +111,8 @@ typedef struct orc_enfuse_params {
     int exposure_cutoff, entropy_cutoff;          /* --exposure-cutoff / --entropy-cutoff in effect */
     double exposure_lower_cutoff, exposure_upper_cutoff;   /* instantiated for the scalar type */
     double entropy_lower_cutoff, entropy_upper_cutoff;
+    int gray_projector;                           /* ORC_GRAY_* (--gray-projector) */
+    double mixer_red, mixer_green, mixer_blue;    /* channel-mixer weights */
 } orc_enfuse_params;
 
 /* enfuseMask (src/enfuse.h:962-1212) for one RGB image with alpha: the float

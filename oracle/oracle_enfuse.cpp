@@ -16,18 +16,74 @@ using namespace orc;
 
 namespace {
 
-// ---- grayscale "average" projector, src/mga.h:188-193 ---------------------
+// ---- grayscale projectors, src/mga.h:60-250 ---------------------------------
 // Integer channels promote to double; float channels promote to float
-// (RealPromote<float> = float), so their sum is a float sum.
-template <class T, class R> struct Gray;
-template <class R> struct Gray<uint8_t, R> {
-    static R get(const uint8_t* p) { return fromRealPromote<R>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
-};
-template <class R> struct Gray<uint16_t, R> {
-    static R get(const uint16_t* p) { return fromRealPromote<R>(((double)p[0] + (double)p[1] + (double)p[2]) / 3.0); }
-};
-template <class R> struct Gray<float, R> {
-    static R get(const float* p) { return (R)((double)(p[0] + p[1] + p[2]) / 3.0); }
+// (RealPromote<float> = float), so their sum is a float sum.  Every projector
+// result lies inside the channel range, so rounding it to the scalar type and
+// then to a wider result type R equals rounding it to R directly.
+//   average    (r + g + b) / 3
+//   l-star     max * L* / 100 of linear RGB (VIGRA RGB2LabFunctor: Y of the
+//              sRGB/D65 matrix, L* = Y < 216/24389 ? 24389/27 Y : 116 Y^(1/3) - 16)
+//   pl-star    the same after the gamma 1/0.45 (VIGRA RGBPrime2LabFunctor)
+//   lightness  (min + max) / 2      value  max      anti-value  min
+//   luminance  RGBValue::luminance(): 0.3 r + 0.59 g + 0.11 b, cast to the channel type
+//   channel-mixer  wr r + wg g + wb b with the weights divided by their sum
+// VIGRA's colour functors are not vendored; their constants are restated.
+template <class T> struct RealOf { typedef double R; };
+template <> struct RealOf<float> { typedef float R; };
+
+template <class T> double scalar_max();
+template <> double scalar_max<uint8_t>() { return 255.0; }
+template <> double scalar_max<uint16_t>() { return 65535.0; }
+template <> double scalar_max<float>() { return (double)FLT_MAX; }
+
+template <class T> T to_channel(double v) { return fromRealPromote<T>(v); }
+template <> float to_channel<float>(double v) { return (float)v; }
+
+double lstar_y(double r, double g, double b)
+{
+    const double Y = 0.212671 * r + 0.715160 * g + 0.072169 * b;
+    return Y < 216.0 / 24389.0 ? 24389.0 / 27.0 * Y : 116.0 * pow(Y, 1.0 / 3.0) - 16.0;
+}
+
+double gamma_correct(double v, double gamma) { return v < 0.0 ? -pow(-v, gamma) : pow(v, gamma); }
+
+template <class T>
+T project(const T* p, const orc_enfuse_params* prm)
+{
+    typedef typename RealOf<T>::R RP;
+    const T r = p[0], g = p[1], b = p[2];
+    switch (prm->gray_projector) {
+    case ORC_GRAY_LSTAR:
+    case ORC_GRAY_PLSTAR: {
+        const double mx = scalar_max<T>();
+        double rr = (double)r / mx, gg = (double)g / mx, bb = (double)b / mx;
+        if (prm->gray_projector == ORC_GRAY_PLSTAR) {
+            rr = gamma_correct(rr, 1.0 / 0.45);
+            gg = gamma_correct(gg, 1.0 / 0.45);
+            bb = gamma_correct(bb, 1.0 / 0.45);
+        }
+        return to_channel<T>(mx * (lstar_y(rr, gg, bb) / 100.0));
+    }
+    case ORC_GRAY_LIGHTNESS: {
+        const T mn = std::min(r, std::min(g, b)), mx = std::max(r, std::max(g, b));
+        return to_channel<T>((double)(mn + mx) / 2.0);
+    }
+    case ORC_GRAY_VALUE: return std::max(r, std::max(g, b));
+    case ORC_GRAY_ANTI_VALUE: return std::min(r, std::min(g, b));
+    case ORC_GRAY_LUMINANCE: return to_channel<T>(0.3 * (double)r + 0.59 * (double)g + 0.11 * (double)b);
+    case ORC_GRAY_MIXER: {
+        const double sum = prm->mixer_red + prm->mixer_green + prm->mixer_blue;
+        const double wr = prm->mixer_red / sum, wg = prm->mixer_green / sum, wb = prm->mixer_blue / sum;
+        return to_channel<T>(wr * (double)(RP)r + wg * (double)(RP)g + wb * (double)(RP)b);
+    }
+    default:
+        return to_channel<T>((double)((RP)r + (RP)g + (RP)b) / 3.0);
+    }
+}
+
+template <class T, class R> struct Gray {
+    static R get(const T* p, const orc_enfuse_params* prm) { return (R)project<T>(p, prm); }
 };
 
 template <class T> struct Max;
@@ -335,6 +391,11 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
 {
     const long N = (long)w * h;
     for (long i = 0; i < N; ++i) mask[i] = 0.0f;
+    if (p->gray_projector < ORC_GRAY_AVERAGE || p->gray_projector > ORC_GRAY_MIXER) return -1;
+    if (p->gray_projector == ORC_GRAY_MIXER &&
+        (p->mixer_red < 0.0 || p->mixer_green < 0.0 || p->mixer_blue < 0.0 ||
+         p->mixer_red + p->mixer_green + p->mixer_blue == 0.0))
+        return -1;   // check_weights (src/mga.h:150-180)
 
     if (p->w_exposure > 0.0) {
         const Exposure ex(p->exposure_function, p->exposure_optimum, p->exposure_width);
@@ -359,7 +420,7 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
                         continue;
                     }
                 }
-                const T g = Gray<T, T>::get(px);
+                const T g = Gray<T, T>::get(px, p);
                 double wt;
                 if (p->exposure_table && is_integral<T>()) wt = p->exposure_table[GrayIndex<T>::of(g)];
                 else wt = ex.weight(exposure_y<T>(g));
@@ -376,7 +437,7 @@ int enfuse_mask_t(const T* img, const uint8_t* alpha, int w, int h, const orc_en
         if (stddev && (n < 2 || w < n || h < n)) return -1;
         std::vector<G> gray((size_t)N), grad((size_t)N, G(0));
 #pragma omp parallel for schedule(guided)
-        for (long i = 0; i < N; ++i) gray[i] = Gray<T, G>::get(img + 3 * i);
+        for (long i = 0; i < N; ++i) gray[i] = Gray<T, G>::get(img + 3 * i, p);
         if (!edges) {
             local_stddev<G, S>(gray.data(), alpha, w, h, n, grad.data());
         } else {
