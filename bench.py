@@ -639,6 +639,33 @@ def main():
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / args.steps
 
+    # The same step with the images in pinned host memory: uploads of white,
+    # black, their alphas and the seam mask, the step, downloads of black and
+    # its alpha (the rate a caller handing over host buffers sees; not the
+    # metric).
+    print("bench: PCIe-inclusive steps", file=sys.stderr, flush=True)
+    h_in = [torch.from_numpy(a).pin_memory() for a in (white, wa, black, ba, mask)]
+    h_out = [torch.empty(t.shape, dtype=t.dtype).pin_memory() for t in (d_black, d_ba)]
+    d_in = [d_white, d_wa, d_black, d_ba, d_mask]
+
+    def pcie_step():
+        for d, hh in zip(d_in, h_in):
+            d.copy_(hh, non_blocking=True)
+        torch.cuda.synchronize()
+        step(d_black, d_ba)
+        ctx.sync()
+        for hh, d in zip(h_out, (d_black, d_ba)):
+            hh.copy_(d, non_blocking=True)
+        torch.cuda.synchronize()
+
+    pcie_step()
+    pcie_runs = 5
+    t0 = time.perf_counter()
+    for _ in range(pcie_runs):
+        pcie_step()
+    pcie_ms = (time.perf_counter() - t0) * 1e3 / pcie_runs
+    pcie_bytes = sum(t.numel() * t.element_size() for t in h_in + h_out)
+
     # Profiled pass: HIP events around every launch on the library's stream.
     ctx.profile(True)
     nprof = 3
@@ -689,6 +716,9 @@ def main():
                      "dram_GB_per_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "all_kernels_GB_per_s": round(all_gbs, 1), "kernel_ms_per_step": round(total_ms / nprof, 4)},
         "kernels": kernels,
+        "pcie_inclusive": {"ms_per_step": round(pcie_ms, 3), "mpix_per_s": round(mpix / (pcie_ms * 1e-3), 1),
+                           "host_bytes_per_step": pcie_bytes, "runs": pcie_runs,
+                           "note": "pinned host buffers in and out around the same step; not the metric"},
         "nft": {"workload": f"nearest-feature-transform seam mask of the two alphas, {w}x{h}, Euclidean, "
                             "float Felzenszwalb-Huttenlocher transforms", "ms": round(nft_ms, 3),
                 "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs,
