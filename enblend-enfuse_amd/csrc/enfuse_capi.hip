@@ -133,6 +133,10 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
         if (w < n || h < n) return fail(ENBL_E_PRECONDITION, "localStdDevIf(): window larger than image");
     }
     if (p->w_contrast > 0.0 && p->edge_scale > 0.0) {
+        // kernels of more taps than any line could hold are rejected before
+        // their radius is formed
+        if (p->edge_scale > 1e6 || p->lce_scale > 1e6)
+            return fail(ENBL_E_PRECONDITION, "convolveLine(): kernel longer than line");
         if (!(p->lce_scale >= 0.0) || !(p->lce_factor >= 0.0))
             return fail(ENBL_E_PRECONDITION, "gaussianSharpening(): scale and sharpening factor must be >= 0");
         const int rs = vigra_radius(p->edge_scale, 0), rd = vigra_radius(p->edge_scale, 2);
