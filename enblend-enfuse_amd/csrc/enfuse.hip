@@ -604,6 +604,9 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
             const double e = bins > 1 ? -acc / splog[bins] : 0.0;
             const T ec = from_real_img<T>(e * Img<T>::maxv());
             emin = c == 0 ? ec : (ec < emin ? ec : emin);
+            // every channel's value is +0 or more: once the minimum is 0 the
+            // remaining channels cannot change it
+            if (emin == T(0)) break;
         }
         res = emin;
     }
