@@ -206,7 +206,7 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
     iscale = float(np.abs(gi_h).max())
     checked, unresolved = [], []
 
-    def judge(g, fd1, fd2, floor, what):
+    def compare_fd(g, fd1, fd2, floor, what):
         den = max(abs(fd2), floor)
         rec = {"what": what, "adjoint": float(g), "fd": float(fd2), "rel_err": abs(g - fd2) / den}
         (checked if abs(fd1 - fd2) <= 1e-5 * den else unresolved).append(rec)
@@ -218,7 +218,7 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
             p1[q] += hs
             p2[q] -= hs
             fds.append(loss_diff(imgs, p1, imgs, p2) / (2 * hs))
-        judge(gp_h[q], fds[0], fds[1], 1e-6 * pscale, f"param {q}")
+        compare_fd(gp_h[q], fds[0], fds[1], 1e-6 * pscale, f"param {q}")
     i1, i2 = imgs.clone(), imgs.clone()
     for _ in range(fd_points):
         k, y, x, c = (int(rng.integers(0, K)), int(rng.integers(0, h)), int(rng.integers(0, w)), int(rng.integers(0, 3)))
@@ -234,7 +234,7 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
         # Richardson extrapolation of the central differences (cancels the h^2 term)
         r1 = (4.0 * fds[1] - fds[0]) / 3.0
         r2 = (4.0 * fds[2] - fds[1]) / 3.0
-        judge(gi_h[k, y, x, c], r1, r2, 1e-6 * iscale, f"image {k} ({y},{x}) ch {c} value {old:.3e}")
+        compare_fd(gi_h[k, y, x, c], r1, r2, 1e-6 * iscale, f"image {k} ({y},{x}) ch {c} value {old:.3e}")
     print(f"bench: {adam_steps} Adam steps on the 5 parameters", file=sys.stderr, flush=True)
     prm = prm0.clone().to(dev).requires_grad_(True)
     opt = torch.optim.Adam([prm], lr=0.01)
