@@ -32,7 +32,8 @@ struct NftArgs {
     int w, h, norm;
     float* inter; long iplane;     // 2 planes of w x h (pitch w): after the column pass
     float* dist; long dplane; long dpitch;   // 2 planes of distances (row pitch dpitch)
-    int* sv; float* sz; float* sf;  // stacks: sample index, envelope bound, sample value
+    int* sv; float* sz; float* sf;  // column stacks: sample index, envelope bound, sample value
+    uint64_t* env; int* top;        // row envelopes, (w + 1) entries per row (env_entry); top index per row
     unsigned* tally;               // 2 counters of exclusive pixels
     uint8_t* mask; long mpitch;
 };
@@ -231,166 +232,222 @@ __global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs
     }
 }
 
-// Rows of the intermediate image; the Euclidean result is its square root.
-__global__ __launch_bounds__(64) void nft_rows_kernel(NftArgs a)
+// Manhattan rows of the intermediate image.
+__global__ __launch_bounds__(64) void nft_rows_manhattan_kernel(NftArgs a)
 {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int t = blockIdx.y;
     if (y >= a.h) return;
     const float* row = a.inter + t * a.iplane + (long)y * a.w;
     float* drow = a.dist + t * a.dplane + (long)y * a.dpitch;
-    auto f = [&](int x) { return *gmem(row + x); };
-    if (a.norm == 1) {
-        manhattan_1d(a.w, f, [&](int x, float d) { *gmem(drow + x) = d; }, [&](int x) { return *gmem(drow + x); });
-    } else {
-        const long T = (long)gridDim.y * gridDim.x * 64;
-        const long tid = (long)t * gridDim.x * 64 + y;
-        euclid_1d(a.w, f, [&](int x, float d) { *gmem(drow + x) = sqrtf(d); }, a.sv + tid, a.sz + tid, a.sf + tid, T);
-    }
+    manhattan_1d(a.w, [&](int x) { return *gmem(row + x); }, [&](int x, float d) { *gmem(drow + x) = d; },
+                 [&](int x) { return *gmem(drow + x); });
 }
 
-// Euclidean row pass: one lane per row, the reference's 1-D transform with
-// its stack in LDS.  The stack's newest kRing entries of every lane live in
-// an LDS ring (entry e in slot e mod kRing); every push is also written to
-// the lane's own contiguous stack in device memory, and a pop below the ring
-// reloads the kRefill entries under it in one batch.  The
-// row samples come through an LDS tile of 64 rows x 64 columns loaded with
-// coalesced reads (the next tile already in flight in registers), and the
-// results leave through the same tile as coalesced row stores.
+// Euclidean row pass in two kernels.
 //
-// The output sweep runs from the last sample to the first with the stack
-// index falling: for each q it stops at the largest k with z[k] < q, which,
-// the intersections z being strictly increasing, is the k the reference's
-// forward sweep (smallest k with z[k + 1] >= q) stops at, so every result
-// (q - v_k)^2 + f(v_k) is the same float.  The sweep starts at the newest
-// entries, which are still in the ring.
-constexpr int kRing = 128, kRefill = 32, kRT = 64;
+// The lower envelope of a row is built by one lane running the reference's
+// 1-D transform (src/openmp_vigra.h:304-357) in the reference's order; a wave
+// holds only kEnvRows rows, so that its lanes rarely wait on each other's pops
+// and the 2 h rows of the two transforms spread over many waves.  The stack's
+// newest kEnvRing entries of every lane live in an LDS ring (entry e in slot
+// e mod kEnvRing); every push is also written to the row's envelope in device
+// memory as (sample index, envelope bound), and a pop below the ring reloads
+// the kRefill entries under it in one batch, the sample values from the row
+// itself.  The row samples come through an LDS tile of kEnvRows rows x kRT
+// columns loaded with coalesced reads, the next tile already in flight; all
+// 64 lanes prepare the tile, each one column: the sum f(q) + q^2 and the
+// intersection with the parabola of q - 1, which is the stack top whenever q
+// arrives, so a sample that pops nothing costs the serial lane no division.
+// The envelope's final top index goes to a table.
+//
+// The output does not depend on the order in which the reference's forward
+// output sweep visits the samples: the envelope bounds are finite beyond entry
+// 0 (-inf) and strictly increasing (an entry is pushed only when its bound
+// exceeds the one under it; every sample is finite, FLT_MAX at most), so the
+// sweep's stop for sample q, the smallest k with z[k + 1] >= q, is the largest
+// k with z[k] < q.  The second kernel finds it by bisection for the first of a
+// run of kOutCols samples and walks up from there; every result
+// sqrt((q - v_k)^2 + f(v_k)) is the same float.
+constexpr int kEnvRows = 2, kEnvRing = 256, kRefill = 32, kRT = 64, kOutCols = 8;
 
-__global__ __launch_bounds__(64) void nft_rows_ring_kernel(NftArgs a)
+// Envelope entry: sample index in the low word, the bound's float bits in
+// the high word.
+__device__ __forceinline__ uint64_t env_entry(int v, float z)
 {
-    __shared__ int rv[kRing][64];
-    __shared__ float rf[kRing][64], rz[kRing][64];
-    __shared__ float tile[kRT][kRT + 1];
+    return ((uint64_t)(uint32_t)__float_as_int(z) << 32) | (uint32_t)v;
+}
+__device__ __forceinline__ int env_v(uint64_t e) { return (int)(uint32_t)e; }
+__device__ __forceinline__ float env_z(uint64_t e) { return __int_as_float((int)(uint32_t)(e >> 32)); }
+
+__global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
+{
+    constexpr int R = kEnvRows, M = kEnvRing - 1;
+    __shared__ uint4 ring[kEnvRing][R];   // (v, sum bits, z bits, 0)
+    __shared__ float tsum[R][kRT + 2], ts0[R][kRT + 2];
+    __shared__ int span[R][2];
     const int lane = threadIdx.x;
     const int t = blockIdx.y;
-    const int row0 = blockIdx.x * kRT;
-    const int nrows = min(kRT, a.h - row0);
+    const int row0 = blockIdx.x * R;
+    const int nrows = min(R, a.h - row0);
     const int n = a.w;
-    const long tid = (long)t * gridDim.x * 64 + (long)blockIdx.x * 64 + lane;
-    int* __restrict__ gv = a.sv + tid * (long)(n + 1);
-    float* __restrict__ gf = a.sf + tid * (long)(n + 1);
-    float* __restrict__ gz = a.sz + tid * (long)(n + 1);
+    const bool own = lane < nrows;
+    const int ro = own ? lane : 0;
     const float* __restrict__ src = a.inter + t * a.iplane + (long)row0 * n;
-    float* __restrict__ dst = a.dist + t * a.dplane + (long)row0 * a.dpitch;
-    constexpr int M = kRing - 1;
+    const float* __restrict__ mine = src + (long)ro * n;
+    uint64_t* __restrict__ env = a.env + ((long)t * a.h + row0 + ro) * (long)(n + 1);
     const float inf = INFINITY;
-    float pf[kRT];
+    float pf[R], carry[R];
     auto fetch = [&](int x0) {
 #pragma unroll
-        for (int r = 0; r < kRT; ++r)
+        for (int r = 0; r < R; ++r)
             pf[r] = r < nrows && x0 + lane < n ? *gmem(src + (long)r * n + x0 + lane) : 0.f;
     };
+    // Top entry: index k, sample vk, sum_v = f(v) + v^2 (the reference's
+    // fk + (float)(vk * vk), the same rounding of the same integer square),
+    // bound zk.  Ring slots from lo up to k are valid.
     int k = 0, lo = 0;
     int vk = 0;
-    float fk = 0.f, zk = -inf;
-    auto refill = [&]() {
+    float sk = 0.f, zk = -inf;
+    int mk = -1;   // lowest stack index reached in this tile
+    auto refill = [&]() {   // entries k - kRefill + 1 .. k into the ring
         const int e0 = max(0, k - kRefill + 1);
         int tv[kRefill];
-        float tf[kRefill], tz[kRefill];
+        float tz[kRefill];
 #pragma unroll
         for (int j = 0; j < kRefill; ++j) {
             const int e = k - j;
-            if (e >= e0) { tv[j] = gv[e]; tf[j] = gf[e]; tz[j] = gz[e]; }
+            if (e >= e0) { const uint64_t p = *gmem(env + e); tv[j] = env_v(p); tz[j] = env_z(p); }
         }
 #pragma unroll
         for (int j = 0; j < kRefill; ++j) {
             const int e = k - j;
-            if (e >= e0) { rv[e & M][lane] = tv[j]; rf[e & M][lane] = tf[j]; rz[e & M][lane] = tz[j]; }
+            if (e >= e0) {
+                const float sv = *gmem(mine + tv[j]) + (float)tv[j] * (float)tv[j];
+                ring[e & M][lane] = make_uint4((unsigned)tv[j], __float_as_uint(sv), __float_as_uint(tz[j]), 0u);
+            }
         }
         lo = e0;
     };
-    // Lower envelope, samples left to right.
+#pragma unroll
+    for (int r = 0; r < R; ++r) carry[r] = 0.f;
     fetch(0);
     for (int x0 = 0; x0 < n; x0 += kRT) {
+        // Every lane prepares one column of the tile for all rows: the sum
+        // f(q) + q^2, and the intersection with the parabola of q - 1, which
+        // is the stack top whenever sample q arrives.
+        const int c = x0 + lane;
 #pragma unroll
-        for (int r = 0; r < kRT; ++r) tile[r][lane] = pf[r];
+        for (int r = 0; r < R; ++r) tsum[r][lane + 1] = pf[r] + (float)c * (float)c;
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tsum[r][0] = carry[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ts0[r][lane] = (tsum[r][lane + 1] - tsum[r][lane]) / 2.0f;   // (float)(2 * (q - (q - 1)))
+            carry[r] = tsum[r][kRT];
+        }
         __syncthreads();
         if (x0 + kRT < n) fetch(x0 + kRT);
-        const int cn = min(kRT, n - x0);
-        float fnext = tile[lane][0];
-        for (int i = 0; i < cn; ++i) {
-            const int q = x0 + i;
-            const float fq = fnext;
-            fnext = tile[lane][i + 1 < cn ? i + 1 : i];
-            if (q == 0) {
-                vk = 0; fk = fq; zk = -inf;
-                rv[0][lane] = 0; rf[0][lane] = fq; rz[0][lane] = -inf;
-                gv[0] = 0; gf[0] = fq; gz[0] = -inf;
-                continue;
+        if (own) {
+            const int cn = min(kRT, n - x0);
+            int i = 0;
+            if (x0 == 0) {   // sample 0 starts the envelope
+                vk = 0; sk = tsum[lane][1]; zk = -inf;
+                ring[0][lane] = make_uint4(0u, __float_as_uint(sk), __float_as_uint(-inf), 0u);
+                i = 1;
             }
-            const float sum_q = fq + (float)q * (float)q;
-            float s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
-            while (s <= zk) {
-                --k;
-                if (k < lo) refill();
-                vk = rv[k & M][lane];
-                fk = rf[k & M][lane];
-                zk = rz[k & M][lane];
-                s = (sum_q - (fk + (float)(vk * vk))) / (float)(2 * (q - vk));
+            float sum_n = tsum[lane][i + 1], s0_n = ts0[lane][i];
+            for (; i < cn; ++i) {
+                const int q = x0 + i;
+                const float sum_q = sum_n;
+                float s = s0_n;
+                sum_n = tsum[lane][i + 2];   // the padding column past the tile is never used
+                s0_n = ts0[lane][i + 1];
+                while (s <= zk) {   // never below entry 0, whose bound is -inf
+                    --k;
+                    if (k < lo) refill();
+                    const uint4 e = ring[k & M][lane];
+                    vk = (int)e.x;
+                    sk = __uint_as_float(e.y);
+                    zk = __uint_as_float(e.z);
+                    s = (sum_q - sk) / (float)(2 * (q - vk));
+                }
+                mk = min(mk, k);
+                ++k;
+                lo = k - lo >= kEnvRing ? k - kEnvRing + 1 : lo;   // entry k - kEnvRing leaves the ring
+                ring[k & M][lane] = make_uint4((unsigned)q, __float_as_uint(sum_q), __float_as_uint(s), 0u);
+                vk = q; sk = sum_q; zk = s;
             }
-            ++k;
-            const int slot = k & M;
-            lo = k - lo >= kRing ? k - kRing + 1 : lo;   // entry k - kRing leaves the ring
-            rv[slot][lane] = q;
-            rf[slot][lane] = fq;
-            rz[slot][lane] = s;
-            gv[k] = q;
-            gf[k] = fq;
-            gz[k] = s;
-            vk = q; fk = fq; zk = s;
+            span[lane][0] = mk;
+            span[lane][1] = k;
         }
+        __syncthreads();
+        // The entries this tile pushed, (mk, k] (at most kRT, all still in the
+        // ring), go to the row's envelope in device memory, one coalesced
+        // store per row; a refill only ever reads entries of earlier tiles.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int e = span[r][0] + 1 + lane;
+            if (r < nrows && e <= span[r][1]) {
+                const uint4 v = ring[e & M][r];
+                *gmem(a.env + ((long)t * a.h + row0 + r) * (long)(n + 1) + e) = env_entry((int)v.x, __uint_as_float(v.z));
+            }
+        }
+        mk = k;
         __syncthreads();
     }
-    // Output sweep, right to left, in tiles of kRT columns.  Before each
-    // tile every lane whose ring holds fewer than kRT entries reloads the kRT
-    // entries under it, all lanes in one batch, so that lanes do not stall
-    // the wave one after another on their own reloads.
-    const int last0 = ((n - 1) / kRT) * kRT;
-    for (int x0 = last0; x0 >= 0; x0 -= kRT) {
-        const int cn = min(kRT, n - x0);
-        {
-            const bool need = k - lo < kRT && lo > 0;
-            const int e1 = lo - 1, e0 = max(0, lo - kRT);
-            int tv[kRT];
-            float tf[kRT], tz[kRT];
-#pragma unroll
-            for (int j = 0; j < kRT; ++j) {
-                const int e = e1 - j;
-                if (need && e >= e0) { tv[j] = gv[e]; tf[j] = gf[e]; tz[j] = gz[e]; }
-            }
-#pragma unroll
-            for (int j = 0; j < kRT; ++j) {
-                const int e = e1 - j;
-                if (need && e >= e0) { rv[e & M][lane] = tv[j]; rf[e & M][lane] = tf[j]; rz[e & M][lane] = tz[j]; }
-            }
-            lo = need ? e0 : lo;
+    if (own) *gmem(a.top + (long)t * a.h + row0 + lane) = k;
+}
+
+// Results of kOutCols consecutive samples per lane of one row (blockIdx.x),
+// 256 x kOutCols samples per block, leaving through LDS (a lane's run padded
+// by one word against bank conflicts) as coalesced stores.
+__global__ __launch_bounds__(256) void nft_rows_out_kernel(NftArgs a)
+{
+    __shared__ float res[256 * (kOutCols + 1)];
+    const int y = blockIdx.x, t = blockIdx.z;
+    const int n = a.w;
+    const int c0 = blockIdx.y * 256 * kOutCols;
+    const int q0 = c0 + threadIdx.x * kOutCols;
+    const long row = (long)t * a.h + y;
+    const uint64_t* __restrict__ env = a.env + row * (long)(n + 1);
+    const float* __restrict__ f = a.inter + t * a.iplane + (long)y * n;
+    if (q0 < n) {
+        const int K = *gmem(a.top + row);
+        const float fq0 = (float)q0;
+        int lo = 0, hi = K;   // largest k in [0, K] with z[k] < q0 (z[0] = -inf)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (env_z(*gmem(env + mid)) < fq0) lo = mid;
+            else hi = mid - 1;
         }
-        for (int i = cn - 1; i >= 0; --i) {
-            const int q = x0 + i;
-            while (zk >= (float)q) {
-                --k;
-                if (k < lo) refill();
-                vk = rv[k & M][lane];
-                fk = rf[k & M][lane];
-                zk = rz[k & M][lane];
+        int k = lo;
+        int vk = env_v(*gmem(env + k));
+        float fk = *gmem(f + vk);
+        float zn = k < K ? env_z(*gmem(env + k + 1)) : INFINITY;
+#pragma unroll
+        for (int j = 0; j < kOutCols; ++j) {
+            const int q = q0 + j;
+            if (q < n) {
+                while (zn < (float)q) {
+                    ++k;
+                    vk = env_v(*gmem(env + k));
+                    fk = *gmem(f + vk);
+                    zn = k < K ? env_z(*gmem(env + k + 1)) : INFINITY;
+                }
+                res[threadIdx.x * (kOutCols + 1) + j] = sqrtf((float)((q - vk) * (q - vk)) + fk);
             }
-            tile[lane][i] = sqrtf((float)((q - vk) * (q - vk)) + fk);
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int r = 0; r < kRT; ++r)
-            if (r < nrows && lane < cn) *gmem(dst + (long)r * a.dpitch + x0 + lane) = tile[r][lane];
-        __syncthreads();
+    }
+    __syncthreads();
+    float* __restrict__ drow = a.dist + t * a.dplane + (long)y * a.dpitch;
+#pragma unroll
+    for (int j = 0; j < kOutCols; ++j) {
+        const int c = j * 256 + threadIdx.x;
+        if (c0 + c < n) *gmem(drow + c0 + c) = res[c / kOutCols * (kOutCols + 1) + c % kOutCols];
     }
 }
 
@@ -442,19 +499,27 @@ int nft_passes(enbl_context* ctx, NftArgs& a, int nt)
     const int w = a.w, h = a.h;
     const int gx_c = (w + 63) / 64, gx_r = (h + 63) / 64;
     const size_t plane = (size_t)w * h;
-    // stacks: n + 1 entries per thread of a pass (columns: n = h, rows: n = w)
-    const size_t entries = std::max((size_t)gx_c * 64 * nt * ((size_t)h + 1), (size_t)gx_r * 64 * nt * ((size_t)w + 1));
+    // Euclidean stacks, one buffer for both passes (the column stacks are dead
+    // when the row pass starts): n + 1 entries of 12 bytes per column thread
+    // (only for columns longer than kScanMax), n + 1 entries of 8 bytes per
+    // row plus the row's top index
+    const size_t entries = (size_t)gx_c * 64 * nt * ((size_t)h + 1);
+    const size_t col_bytes = h > kScanMax ? entries * 12 : 0;
+    const size_t env_bytes = (size_t)nt * h * ((size_t)w + 1) * sizeof(uint64_t);
+    const size_t row_bytes = env_bytes + (size_t)nt * h * sizeof(int);
     void *fbuf, *sbuf = nullptr, *tbuf;
     int rc = scratch(ctx, 16, (size_t)nt * plane * sizeof(float), &fbuf);
-    if (!rc && a.norm == 2) rc = scratch(ctx, 17, entries * 12, &sbuf);
+    if (!rc && a.norm == 2) rc = scratch(ctx, 17, std::max(col_bytes, row_bytes), &sbuf);
     if (!rc) rc = scratch(ctx, 18, 64, &tbuf);
     if (rc) return rc;
     a.inter = (float*)fbuf; a.iplane = (long)plane;
-    a.sv = nullptr; a.sz = nullptr; a.sf = nullptr;
+    a.sv = nullptr; a.sz = nullptr; a.sf = nullptr; a.env = nullptr; a.top = nullptr;
     if (a.norm == 2) {
         a.sv = (int*)sbuf;
         a.sz = (float*)((char*)sbuf + entries * 4);
         a.sf = (float*)((char*)sbuf + entries * 8);
+        a.env = (uint64_t*)sbuf;
+        a.top = (int*)((char*)sbuf + env_bytes);
     }
     a.tally = (unsigned*)tbuf;
     {
@@ -476,13 +541,24 @@ int nft_passes(enbl_context* ctx, NftArgs& a, int nt)
         rc = launch_status("nft_columns");
         if (rc) return rc;
     }
-    {
+    if (a.norm != 2) {
         Timed t(ctx, "nft_rows", nt * (double)plane * 8.0);
-        if (a.norm == 2)
-            hipLaunchKernelGGL(nft_rows_ring_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
-        else
-            hipLaunchKernelGGL(nft_rows_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
-        rc = launch_status("nft_rows");
+        hipLaunchKernelGGL(nft_rows_manhattan_kernel, dim3(gx_r, nt), dim3(64), 0, ctx->stream, a);
+        return launch_status("nft_rows");
+    }
+    {
+        // reads the row samples, writes at most w + 1 envelope entries per row
+        Timed t(ctx, "nft_rows_env", nt * (double)plane * 12.0);
+        hipLaunchKernelGGL(nft_rows_env_kernel, dim3((h + kEnvRows - 1) / kEnvRows, nt), dim3(64), 0, ctx->stream, a);
+        rc = launch_status("nft_rows_env");
+        if (rc) return rc;
+    }
+    {
+        // reads the envelope and the samples it names, writes the distances
+        Timed t(ctx, "nft_rows_out", nt * (double)plane * 16.0);
+        const int gy = (w + 256 * kOutCols - 1) / (256 * kOutCols);
+        hipLaunchKernelGGL(nft_rows_out_kernel, dim3(h, gy, nt), dim3(256), 0, ctx->stream, a);
+        rc = launch_status("nft_rows_out");
     }
     return rc;
 }
