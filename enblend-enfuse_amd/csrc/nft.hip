@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "enbl_device.h"
 #include "enbl_internal.h"
@@ -148,87 +149,132 @@ __global__ __launch_bounds__(64) void nft_columns_kernel(NftArgs a)
 // stack: the nearest feature at or above and the one at or below.
 constexpr int kScanMax = 4096, kSeg = 64;
 
-// Columns are cut into segments of kSeg rows, one lane per (column,
+// Columns are cut into segments of kSeg rows, one lane per (CW columns,
 // segment).  The first kernel finds each segment's first and last feature
-// row (and counts the features for the overlap tally); the second carries
-// the nearest feature of the segments above and below into the segment and
-// runs both sweeps over its rows.
+// row (and counts the features for the overlap tally); the second turns
+// them, per column, into the last feature above each segment and the first
+// below it; the third carries those into the segment and runs both sweeps
+// over its rows.  With CW = 4 a lane reads its four columns as one 32-bit
+// word per row (rows 4-byte aligned); the columns past the last multiple of
+// four take a CW = 1 launch of their own.
 struct SegArgs {
-    int* first;       // [transform][segment][column]: first feature row, or INT_MAX
-    int* last;        // last feature row, or -1
+    int* first;       // [transform][segment][column]: first feature row, or INT_MAX; then the first below
+    int* last;        // last feature row, or -1; then the last above
     int nseg;
+    int x0, x1;       // columns of this launch
 };
 
-__device__ __forceinline__ bool column_feature(const NftArgs& a, int t, int x, int y)
+// Feature bits of rows y0 .. y0 + kSeg - 1 of columns x .. x + CW - 1 (rows
+// past h read as background).
+template <int CW>
+__device__ __forceinline__ void segment_bits(const NftArgs& a, int t, int x, int y0, uint64_t (&b)[CW])
 {
     const uint8_t* p = t == 0 ? a.white : a.black;
     const uint8_t* q = t == 0 ? a.black : a.white;   // null: features are the non-zero samples of p
     const long pp = t == 0 ? a.wpitch : a.bpitch, qp = t == 0 ? a.bpitch : a.wpitch;
-    const uint8_t u = *gmem(p + (long)y * pp + x), v = q ? *gmem(q + (long)y * qp + x) : (uint8_t)0;
-    return v < u;
+    using W = typename std::conditional<CW == 4, uint32_t, uint8_t>::type;
+    W u[kSeg], v[kSeg];
+#pragma unroll
+    for (int i = 0; i < kSeg; ++i) {
+        const bool in = y0 + i < a.h;
+        u[i] = in ? *gmem((const W*)(p + (long)(y0 + i) * pp + x)) : (W)0;
+        v[i] = in && q ? *gmem((const W*)(q + (long)(y0 + i) * qp + x)) : (W)0;
+    }
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        uint64_t m = 0;
+#pragma unroll
+        for (int i = 0; i < kSeg; ++i) {
+            const unsigned uc = ((unsigned)u[i] >> (8 * c)) & 255u, vc = ((unsigned)v[i] >> (8 * c)) & 255u;
+            m |= vc < uc ? (1ull << i) : 0ull;
+        }
+        b[c] = m;
+    }
 }
 
-// Feature bits of rows y0 .. y0 + kSeg - 1 (rows past h read as background).
-__device__ __forceinline__ uint64_t segment_bits(const NftArgs& a, int t, int x, int y0)
-{
-    bool fe[kSeg];
-#pragma unroll
-    for (int i = 0; i < kSeg; ++i) fe[i] = y0 + i < a.h && column_feature(a, t, x, y0 + i);
-    uint64_t b = 0;
-#pragma unroll
-    for (int i = 0; i < kSeg; ++i) b |= fe[i] ? (1ull << i) : 0ull;
-    return b;
-}
-
+template <int CW>
 __global__ __launch_bounds__(64) void nft_colseg_summary_kernel(NftArgs a, SegArgs g)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int x = g.x0 + (blockIdx.x * 64 + threadIdx.x) * CW;
     const int sg = blockIdx.y, t = blockIdx.z;
-    const bool live = x < a.w;
-    const int xc = live ? x : a.w - 1;
-    const uint64_t b = segment_bits(a, t, xc, sg * kSeg);
+    const bool live = x < g.x1;
+    const int xc = live ? x : g.x1 - CW;
+    uint64_t b[CW];
+    segment_bits<CW>(a, t, xc, sg * kSeg, b);
+    unsigned count = 0;
     if (live) {
         const long o = ((long)t * g.nseg + sg) * a.w + x;
-        g.first[o] = b ? sg * kSeg + __builtin_ctzll(b) : INT_MAX;
-        g.last[o] = b ? sg * kSeg + 63 - __builtin_clzll(b) : -1;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            g.first[o + c] = b[c] ? sg * kSeg + __builtin_ctzll(b[c]) : INT_MAX;
+            g.last[o + c] = b[c] ? sg * kSeg + 63 - __builtin_clzll(b[c]) : -1;
+            count += (unsigned)__builtin_popcountll(b[c]);
+        }
     }
-    unsigned count = live ? (unsigned)__builtin_popcountll(b) : 0u;
     for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o);
     if (threadIdx.x == 0) atomicAdd(&a.tally[t], count);
 }
 
-__global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs g)
+// Per column: last[s] becomes the last feature row of the segments above s
+// (-kNone if none) and first[s] the first feature row of the segments below
+// s (h + kNone if none).
+constexpr int kNone = 1 << 20, kSegMax = kScanMax / kSeg;
+
+__global__ __launch_bounds__(64) void nft_colseg_carry_kernel(NftArgs a, SegArgs g)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
-    const int sg = blockIdx.y, t = blockIdx.z;
+    const int t = blockIdx.y;
     if (x >= a.w) return;
-    const int kNone = 1 << 20;
     const long base = (long)t * g.nseg * a.w + x;
-    int last = -kNone;
-    for (int s2 = sg - 1; s2 >= 0; --s2) {
-        const int v = g.last[base + (long)s2 * a.w];
-        if (v >= 0) { last = v; break; }
-    }
-    int next = a.h + kNone;
-    for (int s2 = sg + 1; s2 < g.nseg; ++s2) {
-        const int v = g.first[base + (long)s2 * a.w];
-        if (v != INT_MAX) { next = v; break; }
-    }
+    int f[kSegMax], l[kSegMax];
+#pragma unroll
+    for (int s = 0; s < kSegMax; ++s)
+        if (s < g.nseg) { f[s] = g.first[base + (long)s * a.w]; l[s] = g.last[base + (long)s * a.w]; }
+    int above = -kNone;
+#pragma unroll
+    for (int s = 0; s < kSegMax; ++s)
+        if (s < g.nseg) {
+            const int ls = l[s];
+            g.last[base + (long)s * a.w] = above;
+            above = ls >= 0 ? ls : above;
+        }
+    int below = a.h + kNone;
+#pragma unroll
+    for (int s = kSegMax - 1; s >= 0; --s)
+        if (s < g.nseg) {
+            const int fs = f[s];
+            g.first[base + (long)s * a.w] = below;
+            below = fs != INT_MAX ? fs : below;
+        }
+}
+
+template <int CW>
+__global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs g)
+{
+    const int x = g.x0 + (blockIdx.x * 64 + threadIdx.x) * CW;
+    const int sg = blockIdx.y, t = blockIdx.z;
+    if (x >= g.x1) return;
     const int y0 = sg * kSeg;
-    const uint64_t b = segment_bits(a, t, x, y0);
-    int dn[kSeg];
+    uint64_t b[CW];
+    segment_bits<CW>(a, t, x, y0, b);
+    const long o = ((long)t * g.nseg + sg) * a.w + x;
 #pragma unroll
-    for (int i = 0; i < kSeg; ++i) {
-        last = (b >> i) & 1 ? y0 + i : last;
-        dn[i] = y0 + i - last;
-    }
-    float* col = a.inter + t * a.iplane + x;
+    for (int c = 0; c < CW; ++c) {
+        int last = g.last[o + c], next = g.first[o + c];
+        int dn[kSeg];
 #pragma unroll
-    for (int i = kSeg - 1; i >= 0; --i) {
-        const int y = y0 + i;
-        next = (b >> i) & 1 ? y : next;
-        const int m = min(dn[i], next - y);
-        if (y < a.h) *gmem(col + (long)y * a.w) = m >= kNone ? FLT_MAX : (float)(m * m);
+        for (int i = 0; i < kSeg; ++i) {
+            last = (b[c] >> i) & 1 ? y0 + i : last;
+            dn[i] = y0 + i - last;
+        }
+        float* col = a.inter + t * a.iplane + x + c;
+#pragma unroll
+        for (int i = kSeg - 1; i >= 0; --i) {
+            const int y = y0 + i;
+            next = (b[c] >> i) & 1 ? y : next;
+            const int m = min(dn[i], next - y);
+            if (y < a.h) *gmem(col + (long)y * a.w) = m >= kNone ? FLT_MAX : (float)(m * m);
+        }
     }
 }
 
@@ -533,8 +579,18 @@ int nft_passes(enbl_context* ctx, NftArgs& a, int nt)
             if (rc) return rc;
             g.first = (int*)sb;
             g.last = (int*)sb + (size_t)nt * g.nseg * w;
-            hipLaunchKernelGGL(nft_colseg_summary_kernel, dim3(gx_c, g.nseg, nt), dim3(64), 0, ctx->stream, a, g);
-            hipLaunchKernelGGL(nft_colseg_apply_kernel, dim3(gx_c, g.nseg, nt), dim3(64), 0, ctx->stream, a, g);
+            // word reads need every row of both alphas 4-byte aligned
+            const auto al4 = [](const void* p, long pitch) { return p == nullptr || (((uintptr_t)p & 3) == 0 && (pitch & 3) == 0); };
+            const int w4 = al4(a.white, a.wpitch) && al4(a.black, a.bpitch) ? w & ~3 : 0;
+            SegArgs g4 = g, g1 = g;
+            g4.x0 = 0; g4.x1 = w4;
+            g1.x0 = w4; g1.x1 = w;
+            const unsigned gx4 = (unsigned)((w4 / 4 + 63) / 64), gx1 = (unsigned)((w - w4 + 63) / 64);
+            if (gx4) hipLaunchKernelGGL(nft_colseg_summary_kernel<4>, dim3(gx4, g.nseg, nt), dim3(64), 0, ctx->stream, a, g4);
+            if (gx1) hipLaunchKernelGGL(nft_colseg_summary_kernel<1>, dim3(gx1, g.nseg, nt), dim3(64), 0, ctx->stream, a, g1);
+            hipLaunchKernelGGL(nft_colseg_carry_kernel, dim3(gx_c, nt), dim3(64), 0, ctx->stream, a, g);
+            if (gx4) hipLaunchKernelGGL(nft_colseg_apply_kernel<4>, dim3(gx4, g.nseg, nt), dim3(64), 0, ctx->stream, a, g4);
+            if (gx1) hipLaunchKernelGGL(nft_colseg_apply_kernel<1>, dim3(gx1, g.nseg, nt), dim3(64), 0, ctx->stream, a, g1);
         } else {
             hipLaunchKernelGGL(nft_columns_kernel, dim3(gx_c, nt), dim3(64), 0, ctx->stream, a);
         }
