@@ -248,9 +248,14 @@ __global__ __launch_bounds__(64) void nft_colseg_carry_kernel(NftArgs a, SegArgs
         }
 }
 
+// Row by row, the distance of each of the lane's columns to the last
+// feature at or above (the segment's bits up to the row, else the carried
+// one) and to the first at or below; with CW = 4 and rows of a multiple of
+// four floats the four results leave as one 16-byte store.
 template <int CW>
 __global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs g)
 {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const int x = g.x0 + (blockIdx.x * 64 + threadIdx.x) * CW;
     const int sg = blockIdx.y, t = blockIdx.z;
     if (x >= g.x1) return;
@@ -258,23 +263,33 @@ __global__ __launch_bounds__(64) void nft_colseg_apply_kernel(NftArgs a, SegArgs
     uint64_t b[CW];
     segment_bits<CW>(a, t, x, y0, b);
     const long o = ((long)t * g.nseg + sg) * a.w + x;
+    int above[CW], below[CW];
 #pragma unroll
-    for (int c = 0; c < CW; ++c) {
-        int last = g.last[o + c], next = g.first[o + c];
-        int dn[kSeg];
+    for (int c = 0; c < CW; ++c) { above[c] = g.last[o + c]; below[c] = g.first[o + c]; }
+    float* col = a.inter + t * a.iplane + x;
+    const bool wide = CW == 4 && (a.w & 3) == 0;
+    const int rows = min(kSeg, a.h - y0);
+    for (int i = 0; i < rows; ++i) {
+        const int y = y0 + i;
+        float d[CW];
 #pragma unroll
-        for (int i = 0; i < kSeg; ++i) {
-            last = (b[c] >> i) & 1 ? y0 + i : last;
-            dn[i] = y0 + i - last;
+        for (int c = 0; c < CW; ++c) {
+            const uint64_t up = b[c] & ((2ull << i) - 1ull);   // rows y0 .. y (all 64 at i = 63)
+            const uint64_t dn = b[c] >> i;                      // rows y .. y0 + 63
+            const int du = up ? i - (63 - __builtin_clzll(up)) : y - above[c];
+            const int dd = dn ? __builtin_ctzll(dn) : below[c] - y;
+            const int m = min(du, dd);
+            d[c] = m >= kNone ? FLT_MAX : (float)(m * m);
         }
-        float* col = a.inter + t * a.iplane + x + c;
-#pragma unroll
-        for (int i = kSeg - 1; i >= 0; --i) {
-            const int y = y0 + i;
-            next = (b[c] >> i) & 1 ? y : next;
-            const int m = min(dn[i], next - y);
-            if (y < a.h) *gmem(col + (long)y * a.w) = m >= kNone ? FLT_MAX : (float)(m * m);
+        float* r = col + (long)y * a.w;
+        if constexpr (CW == 4) {
+            if (wide) {
+                *gmem((f4*)r) = f4{d[0], d[1], d[2], d[3]};
+                continue;
+            }
         }
+#pragma unroll
+        for (int c = 0; c < CW; ++c) *gmem(r + c) = d[c];
     }
 }
 
