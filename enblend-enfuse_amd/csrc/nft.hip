@@ -341,6 +341,23 @@ __device__ __forceinline__ uint64_t env_entry(int v, float z)
 __device__ __forceinline__ int env_v(uint64_t e) { return (int)(uint32_t)e; }
 __device__ __forceinline__ float env_z(uint64_t e) { return __int_as_float((int)(uint32_t)(e >> 32)); }
 
+// (sum_q - s_v) / d for an even d = 2 (q - v) in [2, 2^24), correctly
+// rounded like the IEEE division it replaces, in fewer dependent steps:
+// y = RN(1/d) by one FMA Newton step from v_rcp_f32 (equal to 1.0f / d for
+// every integer d below 2^24 on the device), q0 = RN(n y) within one ulp of
+// n / d, r = n - d q0 exact by FMA, and RN(q0 + r y) = RN(n / d) (Markstein's
+// correction; equal to n / d on the device for every even d up to 2^18 and
+// 7,107 numerators of every magnitude up to FLT_MAX, both signs, except the
+// sign of a zero quotient from n = -0, which sum_q >= 1 rules out here).
+__device__ __forceinline__ float env_div(float n, int d)
+{
+    const float fd = (float)d;
+    const float y0 = __builtin_amdgcn_rcpf(fd);
+    const float y = __builtin_fmaf(__builtin_fmaf(-fd, y0, 1.0f), y0, y0);
+    const float q0 = n * y;
+    return __builtin_fmaf(__builtin_fmaf(-q0, fd, n), y, q0);
+}
+
 __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
 {
     constexpr int R = kEnvRows, M = kEnvRing - 1;
@@ -407,7 +424,7 @@ __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            ts0[r][lane] = (tsum[r][lane + 1] - tsum[r][lane]) / 2.0f;   // (float)(2 * (q - (q - 1)))
+            ts0[r][lane] = (tsum[r][lane + 1] - tsum[r][lane]) * 0.5f;   // RN(x / 2) either way
             carry[r] = tsum[r][kRT];
         }
         __syncthreads();
@@ -434,7 +451,7 @@ __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
                     vk = (int)e.x;
                     sk = __uint_as_float(e.y);
                     zk = __uint_as_float(e.z);
-                    s = (sum_q - sk) / (float)(2 * (q - vk));
+                    s = env_div(sum_q - sk, 2 * (q - vk));
                 }
                 mk = min(mk, k);
                 ++k;
