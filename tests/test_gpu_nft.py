@@ -129,3 +129,21 @@ def test_periodic_nft_mask_matches_oracle(enbl, gpu_ctx, oracle, shape, boundary
     got = dm.cpu().numpy()
     assert (got[:, w:] == 7).all()
     np.testing.assert_array_equal(got[:, :w], want)
+
+
+@pytest.mark.parametrize("w", [2000, 4700])
+def test_distance_transform_deep_row_envelopes(enbl, gpu_ctx, oracle, w):
+    """Rows whose column distances grow like sqrt(x) keep nearly every sample
+    on the lower envelope (stacks thousands deep, far beyond the LDS ring),
+    then features in the row itself pop them all: the ring's reloads from
+    device memory decide every result."""
+    h = 300
+    src = np.zeros((h, w), np.uint8)
+    xs = np.arange(w)
+    d = np.minimum(np.rint(np.sqrt(xs * (h - 1) ** 2 / w)).astype(int), h - 1)
+    cut = w - w // 20
+    src[d[:cut], xs[:cut]] = 200
+    src[0, cut::7] = 255
+    src[h // 2, w // 3] = 9
+    print(f"deep envelopes {w}x{h}", flush=True)
+    np.testing.assert_array_equal(_gpu_dt(enbl, gpu_ctx, src, 2), oracle.distance_transform(src, 2))
