@@ -37,6 +37,7 @@ _SIGS = {
     "enbl_profile_enable": (_c_int, [_vp, _c_int]),
     "enbl_profile_report": (_c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
     "enbl_distance_transform": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _c_int, _vp, _c_long]),
+    "enbl_nft_quotients": (_c_int, [_vp, _vp, _vp, _vp, _c_long]),
     "enbl_nft_mask": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int,
                                _vp, _c_long]),
     "enbl_nft_mask_periodic": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int, _c_int,

@@ -194,11 +194,12 @@ int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const 
     return launch_status(name);
 }
 
-int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels, int slot, enbl_pyramid** out)
+int cached_pyramid(enbl_context* ctx, int role, int ptype, int C, int w, int h, int levels, int slot,
+                   enbl_pyramid** out)
 {
     int seen = 0;
     for (auto& c : ctx->cache) {
-        if (c.ptype == ptype && c.channels == C && c.w == w && c.h == h && c.levels == levels) {
+        if (c.role == role && c.ptype == ptype && c.channels == C && c.w == w && c.h == h && c.levels == levels) {
             if (seen == slot) { *out = c.p; return ENBL_OK; }
             ++seen;
         }
@@ -206,7 +207,7 @@ int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels
     enbl_pyramid* p = nullptr;
     int rc = enbl_pyramid_create(ctx, ptype, C, w, h, levels, &p);
     if (rc) return rc;
-    ctx->cache.push_back({ptype, C, w, h, levels, p});
+    ctx->cache.push_back({role, ptype, C, w, h, levels, p});
     *out = p;
     return ENBL_OK;
 }
@@ -860,9 +861,9 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     const int ifrac = itype == ENBL_F32 ? 0 : 7;
     const int mfrac = itype == ENBL_U8 ? 7 : 15;
     enbl_pyramid *mgp = nullptr, *wlp = nullptr, *blp = nullptr;
-    int rc = cached_pyramid(ctx, mt, 1, w, h, levels, 0, &mgp);
-    if (!rc) rc = cached_pyramid(ctx, pt, C, w, h, levels, 0, &wlp);
-    if (!rc) rc = cached_pyramid(ctx, pt, C, w, h, levels, 1, &blp);
+    int rc = cached_pyramid(ctx, kRoleMask, mt, 1, w, h, levels, 0, &mgp);
+    if (!rc) rc = cached_pyramid(ctx, kRoleImage, pt, C, w, h, levels, 0, &wlp);
+    if (!rc) rc = cached_pyramid(ctx, kRoleImage, pt, C, w, h, levels, 1, &blp);
     if (rc) return rc;
 
     const double wv = (double)(255 << mfrac);

@@ -63,11 +63,11 @@ int forward(enbl_context* ctx, int w, int h, int K, const double* imgs, const en
     rc = hip_check(hipMemsetAsync(F.ones, 255, pl, s), "alpha plane");
     if (rc) return rc;
     for (int k = 0; k < K; ++k) {
-        rc = cached_pyramid(ctx, ENBL_F64, 3, w, h, L, k, &F.G[k]);
-        if (!rc) rc = cached_pyramid(ctx, ENBL_F64, 1, w, h, L, k, &F.M[k]);
+        rc = cached_pyramid(ctx, kRoleImage, ENBL_F64, 3, w, h, L, k, &F.G[k]);
+        if (!rc) rc = cached_pyramid(ctx, kRoleMask, ENBL_F64, 1, w, h, L, k, &F.M[k]);
         if (rc) return rc;
     }
-    rc = cached_pyramid(ctx, ENBL_F64, 3, w, h, L, K, &F.C);
+    rc = cached_pyramid(ctx, kRoleResult, ENBL_F64, 3, w, h, L, 0, &F.C);
     if (rc) return rc;
     const size_t ipx = (size_t)w * h * 3;
     DiffTerms t{};
@@ -158,7 +158,7 @@ int enbl_diff_enfuse_backward(enbl_context* ctx, int w, int h, int K, const doub
     const size_t ipx = (size_t)w * h * 3;
     // Output transform and collapse adjoint: gC_l, finest first.
     enbl_pyramid* gC = nullptr;
-    rc = cached_pyramid(ctx, ENBL_F64, 3, w, h, L, K + 1, &gC);
+    rc = cached_pyramid(ctx, kRoleAdjoint, ENBL_F64, 3, w, h, L, 0, &gC);
     if (rc) return rc;
     {
         Timed tm(ctx, "diff_gout", (double)w * h * 72.0);
@@ -179,7 +179,7 @@ int enbl_diff_enfuse_backward(enbl_context* ctx, int w, int h, int K, const doub
     // Per image: merge, Laplacian, Gaussian and mask pyramid adjoints.
     enbl_pyramid* gM[kDiffMaxK];
     for (int k = 0; k < K; ++k) {
-        rc = cached_pyramid(ctx, ENBL_F64, 1, w, h, L, K + k, &gM[k]);
+        rc = cached_pyramid(ctx, kRoleAdjoint, ENBL_F64, 1, w, h, L, k, &gM[k]);
         if (rc) return rc;
     }
     void* gGbuf = nullptr;

@@ -112,7 +112,13 @@ void* level_ptr(const enbl_pyramid* p, int l);
 int gaussian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch);
 int laplacian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap);
 int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const void* img, long ipitch);
-int cached_pyramid(enbl_context* ctx, int ptype, int C, int w, int h, int levels, int slot, enbl_pyramid** out);
+// Pipeline scratch pyramids, kept in the context and keyed by role, type,
+// channels, geometry and slot, so pyramids of different roles never alias
+// even where their type and channel count coincide (grayscale images whose
+// pyramid type equals the mask pyramid type).
+enum PyramidRole { kRoleImage = 0, kRoleMask = 1, kRoleResult = 2, kRoleAdjoint = 3 };
+int cached_pyramid(enbl_context* ctx, int role, int ptype, int C, int w, int h, int levels, int slot,
+                   enbl_pyramid** out);
 // Device scratch buffer `slot` of the context, grown to at least `bytes`.
 int scratch(enbl_context* ctx, int slot, size_t bytes, void** out);
 // Whether the pass producing a coarse level of cw x ch takes the streaming
@@ -132,7 +138,7 @@ struct enbl_context {
     bool profile;
     struct Timed { std::string name; double bytes, own; hipEvent_t start, stop; };
     std::vector<Timed> timed;
-    struct Cached { int ptype, channels, w, h, levels; enbl_pyramid* p; };
+    struct Cached { int role, ptype, channels, w, h, levels; enbl_pyramid* p; };
     std::vector<Cached> cache;   // pipeline scratch pyramids
     void* scratch_alpha;         // per-level alpha scratch for Gaussian builds
     size_t scratch_alpha_bytes;

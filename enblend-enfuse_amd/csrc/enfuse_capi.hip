@@ -6,6 +6,7 @@
 // each input's Laplacian pyramid and normalised mask Gaussian pyramid are
 // merged level by level into the result pyramid, which is collapsed and
 // converted once.
+#include <limits>
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -126,6 +127,25 @@ int check_params(int itype, int w, int h, const enbl_enfuse_params* p)
         return fail(ENBL_E_ARG, "y_optimum");       // ExposureWeight::check_invariant
     if (p->w_exposure > 0.0 && !(p->exposure_width > 0.0))
         return fail(ENBL_E_ARG, "width_parameter");
+    if (p->w_exposure > 0.0 && p->exposure_table) {
+        // A user weight function reaches the device only as its table over
+        // the gray codes of an integer image; a float image would need
+        // weight(gray / FLT_MAX) per pixel (src/enfuse.h:637-640).
+        if (itype == ENBL_F32)
+            return fail(ENBL_E_UNSUPPORTED, "exposure weight table given for a float image");
+        // check_weight_function (src/exposure_weight.cc:237-264) over the
+        // table's points; enfuse exits on any result but OK
+        // (src/enfuse.cc:1920-1945).
+        const int n = itype == ENBL_U8 ? 256 : 65536;
+        int zeros = 0;
+        for (int g = 0; g < n; ++g) {
+            const double wt = p->exposure_table[g];
+            if (wt < 0.0) return fail(ENBL_E_PRECONDITION, "unusable exposure weight function: at least one weight is negative");
+            if (wt > 1.0) return fail(ENBL_E_PRECONDITION, "unusable exposure weight function: at least one weight is larger than one");
+            if (wt >= 0.0 && wt <= std::numeric_limits<double>::denorm_min()) ++zeros;
+        }
+        if (zeros == n) return fail(ENBL_E_PRECONDITION, "unusable exposure weight function: too many zeros");
+    }
     if (p->w_contrast > 0.0 && !(p->edge_scale >= 0.0)) return fail(ENBL_E_ARG, "enfuse: edge scale");
     if (p->w_contrast > 0.0 && (p->edge_scale <= 0.0 || p->min_curvature > 0.0)) {
         const int n = p->contrast_window;
@@ -499,17 +519,17 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     const int ifrac = itype == ENBL_F32 ? 0 : 7;
     const int mfrac = itype == ENBL_U8 ? 7 : 15;
     // Every image's Laplacian pyramid and normalised mask pyramid are kept
-    // (context cache: the result pyramid is image slot 0's neighbour), then
+    // (context cache), then
     // each level merges up to kMergeMax images per pass in image order, so
     // the result level is read and written once per group instead of once
     // per image.
     enbl_pyramid* res = nullptr;
     std::vector<enbl_pyramid*> lps((size_t)K, nullptr), mgps((size_t)K, nullptr);
-    rc = cached_pyramid(ctx, pt, 3, w, h, levels, 1, &res);
+    rc = cached_pyramid(ctx, kRoleResult, pt, 3, w, h, levels, 0, &res);
     if (rc) return rc;
     for (int m = 0; m < K; ++m) {
-        rc = cached_pyramid(ctx, pt, 3, w, h, levels, m == 0 ? 0 : m + 1, &lps[(size_t)m]);
-        if (!rc) rc = cached_pyramid(ctx, mt, 1, w, h, levels, m, &mgps[(size_t)m]);
+        rc = cached_pyramid(ctx, kRoleImage, pt, 3, w, h, levels, m, &lps[(size_t)m]);
+        if (!rc) rc = cached_pyramid(ctx, kRoleMask, mt, 1, w, h, levels, m, &mgps[(size_t)m]);
         if (rc) return rc;
     }
     const float fallback = (float)(255 / K);

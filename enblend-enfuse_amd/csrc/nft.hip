@@ -346,9 +346,12 @@ __device__ __forceinline__ float env_z(uint64_t e) { return __int_as_float((int)
 // y = RN(1/d) by one FMA Newton step from v_rcp_f32 (equal to 1.0f / d for
 // every integer d below 2^24 on the device), q0 = RN(n y) within one ulp of
 // n / d, r = n - d q0 exact by FMA, and RN(q0 + r y) = RN(n / d) (Markstein's
-// correction; equal to n / d on the device for every even d up to 2^18 and
-// 7,107 numerators of every magnitude up to FLT_MAX, both signs, except the
-// sign of a zero quotient from n = -0, which sum_q >= 1 rules out here).
+// correction, which holds whenever y = RN(1/d) and nothing under- or
+// overflows; the sign of a zero quotient from n = -0 may differ, which
+// sum_q >= 1 rules out here).  tests/test_gpu_nft.py checks both steps on
+// the device through enbl_nft_quotients: y for every integer d below 2^24
+// (every float significand), and n / d for every even d up to 2^16 and
+// powers of two to 2^30 over numerators of every magnitude and sign.
 __device__ __forceinline__ float env_div(float n, int d)
 {
     const float fd = (float)d;
@@ -563,6 +566,15 @@ __global__ __launch_bounds__(256) void nft_mask_periodic_kernel(NftArgs a, int w
 
 __global__ void nft_zero_kernel(unsigned* p) { if (threadIdx.x < 2) p[threadIdx.x] = 0u; }
 
+// The envelope quotient of the row pass over arrays, so that its equality
+// with the IEEE division can be checked on the device it runs on.
+__global__ __launch_bounds__(256) void nft_quotient_kernel(const float* __restrict__ num, const int* __restrict__ den,
+                                                           float* __restrict__ out, long count)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) *gmem(out + i) = env_div(*gmem(num + i), *gmem(den + i));
+}
+
 }  // namespace
 }  // namespace enbl
 
@@ -667,6 +679,17 @@ int enbl_distance_transform(enbl_context* ctx, int w, int h, const uint8_t* src,
     a.w = w; a.h = h; a.norm = norm;
     a.dist = dst; a.dplane = 0; a.dpitch = dpitch;
     return nft_passes(ctx, a, 1);
+}
+
+int enbl_nft_quotients(enbl_context* ctx, const float* num, const int* den, float* out, long count)
+{
+    if (!ctx || (count > 0 && (!num || !den || !out))) return fail(ENBL_E_ARG, "enbl_nft_quotients: null argument");
+    if (count < 0) return fail(ENBL_E_SIZE, "enbl_nft_quotients: negative count");
+    if (count == 0) return ENBL_OK;
+    const long blocks = (count + 255) / 256;
+    if (blocks > 0x7fffffffL) return fail(ENBL_E_SIZE, "enbl_nft_quotients: count too large");
+    hipLaunchKernelGGL(nft_quotient_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, num, den, out, count);
+    return launch_status("nft_quotients");
 }
 
 int enbl_nft_mask(enbl_context* ctx, int w, int h, const uint8_t* white, long wpitch, const uint8_t* black, long bpitch,

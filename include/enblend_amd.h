@@ -206,7 +206,13 @@ typedef struct enbl_enfuse_params {
                                      weight takes a pixel whole (src/enfuse.h:1402-1443) */
     /* Host table of ExposureWeight::weight(code / max) for every gray code
      * of an integer image (256 or 65536 doubles), e.g. from a user plugin
-     * (src/exposure_weight_base.h:29-88).  NULL: the built-in function. */
+     * (src/exposure_weight_base.h:29-88).  NULL: the built-in function.
+     * With w_exposure > 0 the table is checked as check_weight_function
+     * checks the plugin (src/exposure_weight.cc:237-264): a negative entry,
+     * an entry above 1 or all entries zero give ENBL_E_PRECONDITION (enfuse
+     * exits there, src/enfuse.cc:1920-1945).  A table for an ENBL_F32
+     * image gives ENBL_E_UNSUPPORTED (the reference evaluates the plugin per
+     * float pixel, src/enfuse.h:637-640). */
     const double* exposure_table;
     /* Contrast by Laplacian-of-Gaussian edges (--contrast-edge-scale,
      * src/enfuse.h:1020-1093) when edge_scale > 0: the edge scale, the
@@ -350,6 +356,15 @@ int enbl_reduce_adjoint(enbl_context* ctx, const double* g, long gpitch, int dw,
  * pitches in elements. */
 int enbl_distance_transform(enbl_context* ctx, int w, int h, const uint8_t* src, long spitch, int norm,
                             float* dst, long dpitch);
+
+/* The quotient of the envelope transform's intersection,
+ * (sum_q - sum_v) / (2 * (q - v)) (src/openmp_vigra.h:322, :328: a float
+ * divided by an int converted to float), as the row pass of
+ * enbl_distance_transform computes it on the device: out[i] = num[i] /
+ * (float)den[i] for i < count, which must equal the IEEE quotient bit for
+ * bit.  Exposed so that equality can be checked on the device; device
+ * buffers, stream-ordered. */
+int enbl_nft_quotients(enbl_context* ctx, const float* num, const int* den, float* out, long count);
 
 /* Replaces nearestFeatureTransform with open boundaries (src/nearest.h:
  * 424-552), the primary seam generator of enblend's fine mask without

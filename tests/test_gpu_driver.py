@@ -192,6 +192,24 @@ def test_enblend_step_matches_oracle(enbl, gpu_ctx, oracle, itype, geom):
     np.testing.assert_array_equal(b, eb)
 
 
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+def test_enblend_step_grayscale_matches_oracle(enbl, gpu_ctx, oracle, itype):
+    """One-channel layers through the loop step (NFT mask, ROI, blend): the
+    mask pyramid must not alias an image pyramid of the same type."""
+    uw, uh, bx, wx = 1200, 700, 800, 403
+    print(f"enblend step C=1 itype={itype} canvas {uw}x{uh}", flush=True)
+    rng = np.random.default_rng(90 + itype)
+    black, ba, white, wa = _pair_scene(rng, itype, uw, uh, bx, wx, C=1)
+    bbb, wbb = _bbox(ba), _bbox(wa)
+    erc, eout, eb, eba, ebb, _ = oracle.enblend_step(black, ba, bbb, white, wa, wbb, itype, seed=5)
+    assert erc == 0 and eout == 0
+    rc, out, b, bal, nbb = _gpu_step(enbl, gpu_ctx, black, ba, bbb, white, wa, wbb, itype)
+    enbl.check(rc)
+    assert out == eout and nbb == ebb
+    np.testing.assert_array_equal(bal, eba)
+    np.testing.assert_array_equal(b, eb)
+
+
 def test_enblend_step_redundant_disjoint_levels(enbl, gpu_ctx, oracle):
     rng = np.random.default_rng(17)
     black, ba, white, wa = _pair_scene(rng, U16, 300, 120, 180, 120)

@@ -196,3 +196,91 @@ def test_entropy_window_kinds_bit_exact(enbl, gpu_ctx, oracle, itype, ewin):
                                            ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
     gpu_ctx.sync()
     np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+def _table_call(enbl, ctx, itype, table, entry):
+    """Calls one enfuse entry point on a tiny stack with an exposure table;
+    returns the status code."""
+    h, w, K = 8, 8, 2
+    dt = {U8: torch.uint8, U16: torch.int16, F32: torch.float32}[itype]
+    imgs = [torch.zeros((h, w, 3), dtype=dt, device="cuda") for _ in range(K)]
+    alphas = [torch.full((h, w), 255, dtype=torch.uint8, device="cuda") for _ in range(K)]
+    out = torch.zeros((h, w, 3), dtype=dt, device="cuda")
+    oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+    m = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    prm = enbl.enfuse_params(w_exposure=1.0, w_saturation=0.2, exposure_table=table)
+    torch.cuda.synchronize()
+    try:
+        if entry == "mask":
+            enbl.check(enbl.lib().enbl_enfuse_mask(ctx.h, itype, w, h, _vp(imgs[0]), 3 * w, _vp(alphas[0]), w,
+                                                   ctypes.byref(prm), _vp(m), w))
+        elif entry == "enfuse":
+            enbl.enfuse(ctx, itype, imgs, alphas, prm, 2, out, oa)
+        else:
+            enbl.enfuse_batch(ctx, itype, [(imgs, alphas, out, oa, 2, 0)], prm)
+        ctx.sync()
+        return enbl.ENBL_OK
+    except enbl.EnblError as e:
+        return e.code
+
+
+@pytest.mark.parametrize("entry", ["mask", "enfuse", "batch"])
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_exposure_table_checked_like_check_weight_function(enbl, gpu_ctx, itype, entry):
+    """A user weight table is checked as check_weight_function checks the
+    plugin (src/exposure_weight.cc:237-264): a negative weight, a weight above
+    one or all weights zero are rejected; zeros elsewhere, denormals counted
+    as zero, and NaN (neither negative, above one nor zero) pass."""
+    n = 256 if itype == U8 else 65536
+    ok = np.linspace(0.0, 1.0, n)
+    print(f"exposure table checks itype={itype} entry={entry}", flush=True)
+    assert _table_call(enbl, gpu_ctx, itype, ok, entry) == enbl.ENBL_OK
+    bad = ok.copy()
+    bad[n // 3] = -1e-300
+    assert _table_call(enbl, gpu_ctx, itype, bad, entry) == enbl.ENBL_E_PRECONDITION
+    bad = ok.copy()
+    bad[-1] = np.nextafter(1.0, 2.0)
+    assert _table_call(enbl, gpu_ctx, itype, bad, entry) == enbl.ENBL_E_PRECONDITION
+    zero = np.zeros(n)
+    zero[5] = 4.9e-324          # denorm_min counts as zero
+    assert _table_call(enbl, gpu_ctx, itype, zero, entry) == enbl.ENBL_E_PRECONDITION
+    zero[5] = 1e-300            # one weight above denorm_min is enough
+    assert _table_call(enbl, gpu_ctx, itype, zero, entry) == enbl.ENBL_OK
+    nan = ok.copy()
+    nan[7] = np.nan
+    assert _table_call(enbl, gpu_ctx, itype, nan, entry) == enbl.ENBL_OK
+
+
+@pytest.mark.parametrize("entry", ["mask", "enfuse", "batch"])
+def test_exposure_table_for_float_image_unsupported(enbl, gpu_ctx, entry):
+    """The reference evaluates the weight function per float pixel
+    (src/enfuse.h:637-640); a table cannot stand for it, so a table given for
+    a float image is an error, not silently replaced by the built-in."""
+    print(f"float image with exposure table entry={entry}", flush=True)
+    assert _table_call(enbl, gpu_ctx, F32, np.linspace(0.0, 1.0, 65536), entry) == enbl.ENBL_E_UNSUPPORTED
+    assert _table_call(enbl, gpu_ctx, F32, None, entry) == enbl.ENBL_OK
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_exposure_table_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype):
+    """A plugin's table (a linear ramp, as doc/examples/enfuse/linear.cc's
+    weight) through the whole fusion, against the oracle's table path."""
+    rng = np.random.default_rng(880 + itype)
+    h, w, K = 90, 117, 3
+    n = 256 if itype == U8 else 65536
+    tab = np.arange(n, dtype=np.float64) / (n - 1)
+    imgs = _stack(rng, itype, K, h, w)
+    alphas = [np.full((h, w), 255, np.uint8) for _ in range(K)]
+    L = oracle.auto_levels(w, h)
+    kw = dict(w_exposure=1.0, w_saturation=0.2, exposure_table=tab)
+    print(f"enfuse with exposure table itype={itype}", flush=True)
+    want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), L, dither=1, seed=4)
+    dimgs = [torch.from_numpy(i).cuda() for i in imgs]
+    das = [torch.from_numpy(a).cuda() for a in alphas]
+    out = torch.zeros_like(dimgs[0])
+    oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    enbl.enfuse(gpu_ctx, itype, dimgs, das, enbl.enfuse_params(**kw), L, out, oa, dither=1, seed=4)
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
+    np.testing.assert_array_equal(out.cpu().numpy(), want)

@@ -147,3 +147,56 @@ def test_distance_transform_deep_row_envelopes(enbl, gpu_ctx, oracle, w):
     src[h // 2, w // 3] = 9
     print(f"deep envelopes {w}x{h}", flush=True)
     np.testing.assert_array_equal(_gpu_dt(enbl, gpu_ctx, src, 2), oracle.distance_transform(src, 2))
+
+
+def _quotients(enbl, ctx, num, den):
+    dn = torch.from_numpy(np.ascontiguousarray(num, np.float32)).cuda()
+    dd = torch.from_numpy(np.ascontiguousarray(den, np.int32)).cuda()
+    out = torch.empty_like(dn)
+    torch.cuda.synchronize()
+    enbl.check(enbl.lib().enbl_nft_quotients(ctx.h, _vp(dn), _vp(dd), _vp(out), dn.numel()))
+    ctx.sync()
+    return out.cpu().numpy()
+
+
+def test_envelope_reciprocals_are_correctly_rounded(enbl, gpu_ctx):
+    """The row pass's quotient is one FMA correction of n * y (Markstein),
+    which is the correctly rounded n / d when y is the correctly rounded
+    1 / d.  With n = 1 the kernel returns exactly that y-based quotient;
+    checked for every integer d below 2^24, i.e. every float significand
+    (1 / (2^k d) only shifts the exponent, so larger d are covered)."""
+    print("envelope reciprocals d < 2^24", flush=True)
+    den = np.arange(1, 1 << 24, dtype=np.int32)
+    num = np.ones(den.shape, np.float32)
+    got = _quotients(enbl, gpu_ctx, num, den)
+    want = np.float32(1.0) / den.astype(np.float32)
+    bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
+    assert bad.size == 0, f"{bad.size} reciprocals differ, first d={den[bad[:5]]}"
+
+
+def test_envelope_quotients_equal_ieee_division(enbl, gpu_ctx):
+    """(sum_q - sum_v) / (2 (q - v)) on the device equals the IEEE float
+    quotient bit for bit, for every even divisor up to 2^16 (rows up to
+    32768 samples apart) and 2^17 + 2 and powers of two to 2^30, over
+    numerators of every magnitude and both signs (integers, half-integers
+    and random significands up to FLT_MAX)."""
+    rng = np.random.default_rng(2024)
+    nums = [np.float32(v) for v in (1.0, -1.0, 3.0, 0.5, -0.5, 2.5, 16777215.0, 16777217.0,
+                                    np.finfo(np.float32).max, -np.finfo(np.float32).max, 2.0 ** -23)]
+    # the numerators are differences of sums of at least 1 (multiples of
+    # 2^-23 or zero), so quotients stay clear of the subnormal range
+    mags = rng.uniform(-6, 38, size=300)
+    sig = rng.uniform(1.0, 2.0, size=300) * rng.choice([-1.0, 1.0], size=300)
+    nums += list((sig * 10.0 ** mags).astype(np.float32))
+    nums += list(rng.integers(-(1 << 26), 1 << 26, size=100).astype(np.float32))
+    nums += list((rng.integers(-(1 << 24), 1 << 24, size=100) + 0.5).astype(np.float32))
+    num_set = np.array([v for v in nums if np.isfinite(v)], np.float32)
+    dens = np.concatenate([np.arange(2, (1 << 16) + 1, 2), [(1 << 17) + 2], 2 ** np.arange(17, 31)]).astype(np.int32)
+    print(f"envelope quotients: {num_set.size} numerators x {dens.size} divisors", flush=True)
+    num = np.repeat(num_set, dens.size)
+    den = np.tile(dens, num_set.size)
+    got = _quotients(enbl, gpu_ctx, num, den)
+    with np.errstate(over="ignore", under="ignore"):
+        want = num / den.astype(np.float32)
+    bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
+    assert bad.size == 0, f"{bad.size} quotients differ, first {list(zip(num[bad[:5]], den[bad[:5]]))}"

@@ -242,8 +242,8 @@ def test_blend_level_matches_oracle(enbl, gpu_ctx, oracle, itype):
         np.testing.assert_array_equal(blp.read(l), want, err_msg=f"level {l}")
 
 
-def _enblend_inputs(rng, itype, h, w):
-    white, black = _img(rng, itype, h, w), _img(rng, itype, h, w)
+def _enblend_inputs(rng, itype, h, w, C=3):
+    white, black = _img(rng, itype, h, w, C), _img(rng, itype, h, w, C)
     yy, xx = np.mgrid[0:h, 0:w]
     wa = np.where((xx - 0.3 * w) ** 2 / (0.45 * w) ** 2 + (yy - 0.5 * h) ** 2 / (0.48 * h) ** 2 < 1, 255, 0).astype(np.uint8)
     ba = np.where((xx - 0.7 * w) ** 2 / (0.45 * w) ** 2 + (yy - 0.5 * h) ** 2 / (0.48 * h) ** 2 < 1, 255, 0).astype(np.uint8)
@@ -292,6 +292,37 @@ def test_enblend_pair_matches_oracle(enbl, gpu_ctx, oracle, itype, wrap, policy)
         enbl.check(enbl.lib().enbl_enblend_pair(gpu_ctx.h, itype, 3, w, h, L, int(wrap),
                                                 _vp(dwh), 3 * w, _vp(dwa), w, _vp(dbk), 3 * w, _vp(dba), w,
                                                 _vp(dm), w, 1, 77))
+        gpu_ctx.sync()
+        got, got_a = dbk.cpu().numpy(), dba.cpu().numpy()
+        np.testing.assert_array_equal(got_a, want_a)
+        if itype == F32:
+            sel = want_a != 0
+            np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0)
+        else:
+            np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("shape", [(96, 160), (301, 457), (1501, 2899)])
+def test_enblend_pair_grayscale_matches_oracle(enbl, gpu_ctx, oracle, itype, shape):
+    """One-channel images: the mask Gaussian pyramid and the white Laplacian
+    pyramid share type and channel count here (Int16 for 8-bit, Int32 for
+    16-bit images), so they must still be distinct pyramids.  Sizes cover the
+    plain level-by-level path, the shared small-level launches and the fused
+    streaming passes; two calls in a row on the same context."""
+    h, w = shape
+    rng = np.random.default_rng(700 + itype + h)
+    L = oracle.auto_levels(w, h)
+    for rep in range(2):
+        print(f"enblend_pair C=1 itype={itype} {w}x{h} L={L} call {rep}", flush=True)
+        white, wa, black, ba, mask = _enblend_inputs(rng, itype, h, w, C=1)
+        want, want_a = oracle.enblend_pair(white, wa, black, ba, mask, L, False, itype, 1, 31 + rep)
+        dwh, dwa, dbk, dba, dm = (torch.from_numpy(np.ascontiguousarray(v)).cuda()
+                                  for v in (white, wa, black, ba, mask))
+        torch.cuda.synchronize()
+        enbl.check(enbl.lib().enbl_enblend_pair(gpu_ctx.h, itype, 1, w, h, L, 0,
+                                                _vp(dwh), w, _vp(dwa), w, _vp(dbk), w, _vp(dba), w,
+                                                _vp(dm), w, 1, 31 + rep))
         gpu_ctx.sync()
         got, got_a = dbk.cpu().numpy(), dba.cpu().numpy()
         np.testing.assert_array_equal(got_a, want_a)
