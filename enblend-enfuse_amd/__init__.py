@@ -58,6 +58,7 @@ _SIGS = {
     "enbl_pyramid_copy_level": (_c_int, [_vp, _c_int, _vp, _c_int]),
     "enbl_auto_levels": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
     "enbl_gaussian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
+    "enbl_gaussian_pyramid_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_laplacian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_collapse_pyramid": (_c_int, [_vp, _c_int, _vp]),
     "enbl_blend": (_c_int, [_vp, _vp, _vp, _vp, ctypes.c_double]),

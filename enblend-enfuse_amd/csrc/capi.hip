@@ -155,6 +155,13 @@ int gaussian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t*
         int rc = alpha_scratch(ctx, (size_t)apl * p->lv[1].h, &abuf[0], &abuf[1]);
         if (rc) return rc;
     }
+    return gaussian_levels_with(ctx, p, wrap, alpha, apitch, abuf[0], abuf[1], apl);
+}
+
+int gaussian_levels_with(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch,
+                         uint8_t* a0, uint8_t* a1, long apl)
+{
+    uint8_t* abuf[2] = {a0, a1};
     const uint8_t* sa = alpha;
     long sap = apitch;
     for (int l = 1; l < p->levels; ++l) {

@@ -110,6 +110,11 @@ int mask_ptype_for(int itype);
 int image_ptype_for(int itype);
 void* level_ptr(const enbl_pyramid* p, int l);
 int gaussian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch);
+// Levels 1.. of a Gaussian pyramid whose level 0 is built, with the per-level
+// alpha ping-ponging between a0 and a1 (pitch apl, each at least apl rows of
+// level 1; unused without alpha).
+int gaussian_levels_with(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch,
+                         uint8_t* a0, uint8_t* a1, long apl);
 int laplacian_levels(enbl_context* ctx, enbl_pyramid* p, int wrap);
 int build_level0(enbl_context* ctx, enbl_pyramid* p, int itype, int frac, const void* img, long ipitch);
 // Pipeline scratch pyramids, kept in the context and keyed by role, type,

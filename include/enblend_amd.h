@@ -161,6 +161,19 @@ int enbl_gaussian_pyramid(enbl_context* ctx, int wraparound, int itype, int frac
                           const void* img, long ipitch, const uint8_t* alpha, long apitch,
                           enbl_pyramid* out);
 
+/* No counterpart in the reference: enbl_gaussian_pyramid for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors; the per-level alpha lives in a 16 MiB device array that the
+ * library holds on every GPU it runs on until the process ends, so the call
+ * allocates nothing.  ENBL_E_SIZE when the level-1 alpha ping-pong,
+ * 2 * round_up(level-1 width, 256) * level-1 height bytes, exceeds it
+ * (e.g. above 8192 x 4096 images).  All calls on one GPU share that array:
+ * calls that overlap on one GPU race.  No profile events are recorded. */
+int enbl_gaussian_pyramid_noalloc(enbl_context* ctx, int wraparound, int itype, int frac_bits,
+                                  const void* img, long ipitch, const uint8_t* alpha, long apitch,
+                                  enbl_pyramid* out);
+
 /* Replaces laplacianPyramid<...>(exportName, numLevels, wraparound, src,
  * alpha) -- src/pyramid.h:1646-1747. */
 int enbl_laplacian_pyramid(enbl_context* ctx, int wraparound, int itype, int frac_bits,
