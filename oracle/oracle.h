@@ -117,6 +117,10 @@ typedef struct orc_enfuse_params {
 
 /* enfuseMask (src/enfuse.h:962-1212) for one RGB image with alpha: the float
  * weight image, written where alpha != 0 (zero elsewhere). */
+/* localStdDevIf on a float gray image (src/enfuse.h:75-219), writing only
+ * interior pixels under the mask; planted = 1 (double sums) and 2 (every
+ * window summed anew) are deliberately wrong variants for tests. */
+int orc_local_stddev_f32(const float* gray, const uint8_t* mask, int w, int h, int n, int planted, float* out);
 int orc_enfuse_mask(int itype, const void* img, const uint8_t* alpha, int w, int h,
                     const orc_enfuse_params* prm, float* mask);
 
@@ -159,6 +163,10 @@ int orc_nft_mask_periodic(const uint8_t* white, long wpitch, const uint8_t* blac
                           int norm, int boundary, uint8_t* mask, long mpitch);
 /* The distance transform alone: distance of every pixel to the nearest
  * non-zero pixel of src (float, sqrt for the Euclidean norm). */
+/* One Euclidean 1-D lower-envelope transform (src/openmp_vigra.h:304-357)
+ * of n values; planted = 1 uses a deliberately reordered intersection
+ * formula (for tests that must detect it). */
+int orc_euclid_1d(const float* f, int n, int planted, float* d);
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);
 
 /* enblend's driver around the blend (oracle_driver.cpp). */

@@ -128,7 +128,10 @@ template <class T> struct GrayIndex { static int of(T g) { return (int)g; } };
 
 // ---- local standard deviation, src/enfuse.h:75-219 -----------------------
 // S = RealPromote of the gray type (double for int, float for float).
-template <class G, class S>
+// FRESH = true is a deliberately wrong variant that sums every window anew
+// instead of sliding the running sums (reachable only through
+// orc_local_stddev_f32, so that tests can show they detect it).
+template <class G, class S, bool FRESH = false>
 void local_stddev(const G* gray, const uint8_t* mask, int w, int h, int n, G* out)
 {
     const int b = n / 2;
@@ -167,6 +170,11 @@ void local_stddev(const G* gray, const uint8_t* mask, int w, int h, int n, G* ou
             size_t k;
             column(xn, s, q, k);
             cs[xn] = s; cq[xn] = q; cn[xn] = k;
+            if (FRESH) {
+                sum = S(0); sumSqr = S(0); cnt = 0;
+                for (int x = xn - n + 1; x <= xn; ++x) { sum += cs[x]; sumSqr += cq[x]; cnt += cn[x]; }
+                continue;
+            }
             sum += s - cs[xn - n];
             sumSqr += q - cq[xn - n];
             cnt += k - cn[xn - n];
@@ -599,6 +607,19 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
 }  // namespace
 
 extern "C" {
+
+// The float local standard deviation of the contrast weight on its own
+// (localStdDevIf over a float gray image, RealPromote<float> = float sums).
+// planted 1 and 2 are deliberately wrong variants for tests that must
+// detect them: 1 accumulates in double, 2 sums every window anew.
+int orc_local_stddev_f32(const float* gray, const uint8_t* mask, int w, int h, int n, int planted, float* out)
+{
+    if (n < 3 || !(n & 1) || w < n || h < n || planted < 0 || planted > 2) return -1;
+    if (planted == 1) local_stddev<float, double>(gray, mask, w, h, n, out);
+    else if (planted == 2) local_stddev<float, float, true>(gray, mask, w, h, n, out);
+    else local_stddev<float, float>(gray, mask, w, h, n, out);
+    return 0;
+}
 
 int orc_enfuse_mask(int itype, const void* img, const uint8_t* alpha, int w, int h,
                     const orc_enfuse_params* prm, float* mask)

@@ -33,8 +33,20 @@ void manhattan_1d(float* d, const float* f, int n)
     for (int q = n - 2; q >= 0; --q) d[q] = std::min<float>(d[q], d[q + 1] + 1.0f);
 }
 
+// Intersection of the parabolas of q and v (src/openmp_vigra.h:322, :328):
+// (sum_q - (f[v] + v^2)) / (2 (q - v)).  PLANTED = 1 is a deliberately wrong
+// operation order, ((sum_q - f[v]) - v^2) / (2 (q - v)), reachable only
+// through orc_euclid_1d so that tests can show they detect it.
+template <int PLANTED>
+inline float intersection(float sum_q, float fv, int v, int q)
+{
+    if (PLANTED == 1) return ((sum_q - fv) - (float)(v * v)) / (float)(2 * (q - v));
+    return (sum_q - (fv + (float)(v * v))) / (float)(2 * (q - v));
+}
+
 // EuclideanTransform1D (src/openmp_vigra.h:304-357): v, z scratch of n and
 // n + 1 entries.
+template <int PLANTED = 0>
 void euclid_1d(float* d, const float* f, int n, int* v, float* z)
 {
     const float inf = INFINITY;
@@ -44,10 +56,10 @@ void euclid_1d(float* d, const float* f, int n, int* v, float* z)
     z[1] = inf;
     for (int q = 1; q < n; ++q) {
         const float sum_q = f[q] + (float)q * (float)q;
-        float s = (sum_q - (f[v[k]] + (float)(v[k] * v[k]))) / (float)(2 * (q - v[k]));
+        float s = intersection<PLANTED>(sum_q, f[v[k]], v[k], q);
         while (s <= z[k]) {
             --k;
-            s = (sum_q - (f[v[k]] + (float)(v[k] * v[k]))) / (float)(2 * (q - v[k]));
+            s = intersection<PLANTED>(sum_q, f[v[k]], v[k], q);
         }
         ++k;
         v[k] = q;
@@ -90,6 +102,16 @@ void fh_transform(const uint8_t* src, long spitch, int w, int h, int norm, float
 }  // namespace
 
 extern "C" {
+
+int orc_euclid_1d(const float* f, int n, int planted, float* d)
+{
+    if (n < 1 || planted < 0 || planted > 1) return -1;
+    std::vector<int> v((size_t)n);
+    std::vector<float> z((size_t)n + 1);
+    if (planted) euclid_1d<1>(d, f, n, v.data(), z.data());
+    else euclid_1d<0>(d, f, n, v.data(), z.data());
+    return 0;
+}
 
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch)
 {

@@ -36,6 +36,8 @@ _SIGS = {
     "orc_diff_reduce_T": [_vp, _i, _i, _vp, _i, _i],
     "orc_nft_mask": [_vp, _l, _vp, _l, _i, _i, _i, _vp, _l],
     "orc_distance_transform": [_vp, _l, _i, _i, _i, _vp, _l],
+    "orc_euclid_1d": [_vp, _i, _i, _vp],
+    "orc_local_stddev_f32": [_vp, _vp, _i, _i, _i, _i, _vp],
     "orc_nft_mask_periodic": [_vp, _l, _vp, _l, _i, _i, _i, _i, _vp, _l],
     "orc_assemble": [_i, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "orc_inspect_overlap": [_vp, _l, _vp, _l, _i, _i],
@@ -313,6 +315,28 @@ def _distance_transform(self, src, norm=2):
     return out
 
 
+def _euclid_1d(self, f, planted=0):
+    """One Euclidean lower-envelope transform of the float values f."""
+    f = np.ascontiguousarray(f, np.float32)
+    out = np.zeros(f.shape, np.float32)
+    rc = self.lib.orc_euclid_1d(_p(f), f.size, planted, _p(out))
+    if rc:
+        raise ValueError(f"orc_euclid_1d rc={rc}")
+    return out
+
+
+def _local_stddev_f32(self, gray, mask, n, planted=0):
+    """Float localStdDevIf; pixels it does not write stay 0."""
+    gray = np.ascontiguousarray(gray, np.float32)
+    mask = np.ascontiguousarray(mask, np.uint8)
+    h, w = gray.shape
+    out = np.zeros((h, w), np.float32)
+    rc = self.lib.orc_local_stddev_f32(_p(gray), _p(mask), w, h, n, planted, _p(out))
+    if rc:
+        raise ValueError(f"orc_local_stddev_f32 rc={rc}")
+    return out
+
+
 def _nft_mask(self, white, black, norm=2):
     """(mask, rc): rc = -2 on excessive overlap (mask written anyway)."""
     white = np.ascontiguousarray(white, np.uint8)
@@ -400,6 +424,8 @@ Oracle.assemble = _assemble
 Oracle.inspect_overlap = _inspect_overlap
 Oracle.mask_bounds = _mask_bounds
 Oracle.enblend_step = _enblend_step
+Oracle.euclid_1d = _euclid_1d
+Oracle.local_stddev_f32 = _local_stddev_f32
 Oracle.checkpoint = _checkpoint
 def _nft_mask_periodic(self, white, black, boundary, norm=2):
     """(mask, rc) with strip boundaries: 1 horizontal, 2 vertical, 3 double."""

@@ -284,3 +284,35 @@ def test_exposure_table_enfuse_matches_oracle(enbl, gpu_ctx, oracle, itype):
     gpu_ctx.sync()
     np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
     np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def test_float_contrast_known_answers(enbl, gpu_ctx, oracle):
+    """The float local standard deviation's known answers through the GPU
+    contrast weight (equal channels, so gray = the sample): a constant 5x5
+    window of 4097.0f gives NaN from the float sums, and sliding past a
+    column of 8192.0f gives NaN at the next window while the first window
+    gives RN(sqrt(11182079)); see tests/test_oracle_enfuse.py for the
+    derivations.  Plus a larger image against the oracle's float
+    std-dev."""
+    rng = np.random.default_rng(12)
+    g3 = np.ones((5, 6), np.float32)
+    g3[:, 0] = 8192.0
+    for g in (np.full((5, 5), 4097.0, np.float32), g3,
+              (rng.integers(0, 1 << 12, size=(61, 130)) / 64.0).astype(np.float32)):
+        h, w = g.shape
+        print(f"float contrast known answer {w}x{h}", flush=True)
+        img = np.ascontiguousarray(np.repeat(g[..., None], 3, axis=2))
+        a = np.full((h, w), 255, np.uint8)
+        kw = dict(w_exposure=0.0, w_contrast=1.0, contrast_window=5)
+        dimg, da = torch.from_numpy(img).cuda(), torch.from_numpy(a).cuda()
+        dm = torch.full((h, w), -1.0, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, F32, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                               ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+        gpu_ctx.sync()
+        got = dm.cpu().numpy()
+        np.testing.assert_array_equal(got, oracle.local_stddev_f32(g, a, 5))
+        if (h, w) == (5, 5):
+            assert np.isnan(got[2, 2])
+        elif (h, w) == (5, 6):
+            assert got[2, 2] == np.sqrt(np.float32(11182079.0), dtype=np.float32) and np.isnan(got[2, 3])

@@ -165,14 +165,30 @@ def test_pyramids_match_oracle(enbl, gpu_ctx, oracle, itype, wrap, policy):
             torch.cuda.synchronize()
             enbl.check(fn(gpu_ctx.h, int(wrap), itype, frac, _vp(dimg), w * 3, _vp(da), w, p.h))
             gpu_ctx.sync()
-            if itype == F32:
-                # identical level 0 from the oracle isolates the pyramid ops
-                lv0 = oracle.to_pyramid(img, F32, F64, 0)
-                got0 = p.read(0) if not lap else None
-                if got0 is not None:
-                    np.testing.assert_allclose(got0, lv0, rtol=1e-15)
-                continue
             want = oracle.pyramid(img, a, L, wrap, lap, itype)
+            if itype == F32:
+                # level 0 is 1 + log1p(v) (or 1 / (1 - v)) from the device's
+                # log1p, which may differ from glibc's in the last bit; every
+                # later level follows the double reduce / expand in the same
+                # operation order (bit-exact from identical inputs:
+                # test_double_pyramid_ops_bit_exact), so the levels agree to
+                # a few ulps of the level-0 magnitude.
+                lv0 = oracle.to_pyramid(img, F32, F64, 0)
+                scale = float(np.abs(lv0).max())
+                for l in range(L):
+                    got = p.read(l)
+                    if not lap:
+                        np.testing.assert_allclose(got, want[l], rtol=2e-15 * (l + 1), atol=0,
+                                                   err_msg=f"gaussian level {l}")
+                    else:
+                        np.testing.assert_allclose(got, want[l], rtol=0, atol=4e-15 * scale * (l + 1),
+                                                   err_msg=f"laplacian level {l}")
+                if lap:
+                    enbl.check(enbl.lib().enbl_collapse_pyramid(gpu_ctx.h, int(wrap), p.h))
+                    gpu_ctx.sync()
+                    np.testing.assert_allclose(p.read(0), lv0, rtol=0, atol=4e-15 * scale * L)
+                p.close()
+                continue
             for l in range(L):
                 np.testing.assert_array_equal(p.read(l), want[l], err_msg=f"level {l} lap={lap}")
             if lap:

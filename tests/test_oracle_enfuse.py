@@ -273,3 +273,126 @@ def test_hard_mask_zero_weights_take_float_share(oracle):
     hard, _ = oracle.enfuse(imgs, full, U8, enfuse_params(w_exposure=0.0, hard_mask=True), L, dither=0)
     soft, _ = oracle.enfuse(imgs, full, U8, enfuse_params(w_exposure=0.0), L, dither=0)
     assert np.all(hard == want_hard) and np.all(soft == want_soft)
+
+
+# ---- float local standard deviation (src/enfuse.h:75-219 with
+# RealPromote<float> = float): pinned by closed forms where every partial
+# sum is exact, and by known answers that only the reference's float running
+# sums in its sliding update order produce.
+
+def _ulp32(x):
+    x = np.abs(np.asarray(x, np.float32))
+    return (np.nextafter(x, np.float32(np.inf)) - x).astype(np.float64)
+
+
+def _closed_form_sd(g, m, n):
+    """Exact sample standard deviation of the masked window samples, the
+    window sums, count and the border/mask rule (interior pixels under the
+    mask only; fewer than two samples give 0)."""
+    h, w = g.shape
+    b = n // 2
+    sd = np.zeros((h, w))
+    sums = np.zeros((h, w))
+    sqs = np.zeros((h, w))
+    cnt = np.zeros((h, w), np.int64)
+    written = np.zeros((h, w), bool)
+    for y in range(b, h - b):
+        for x in range(b, w - b):
+            if not m[y, x]:
+                continue
+            win = g[y - b:y + b + 1, x - b:x + b + 1].astype(np.float64)
+            sel = win[m[y - b:y + b + 1, x - b:x + b + 1] != 0]
+            written[y, x] = True
+            cnt[y, x] = sel.size
+            sums[y, x], sqs[y, x] = sel.sum(), (sel * sel).sum()
+            if sel.size > 1:
+                sd[y, x] = math.sqrt(max(0.0, (sqs[y, x] - sums[y, x] ** 2 / sel.size) / (sel.size - 1)))
+    return sd, sums, sqs, cnt, written
+
+
+@pytest.mark.parametrize("n", [3, 5, 7])
+@pytest.mark.parametrize("planted", [0, 1])
+def test_float_stddev_exact_sums_match_closed_form(oracle, n, planted):
+    """Samples k/8 (k <= 16): every column sum, running sum and sum of
+    squares is exact in float, so only the final formula rounds; the result
+    must equal the exact sample standard deviation within those roundings
+    (RN(sum^2 / n), the subtraction, the division by n - 1 and the square
+    root), pixels with fewer than two samples give exactly 0, borders and
+    pixels outside the mask stay 0.  A double accumulator passes this test
+    too (the sums are exact); the known answers below tell them apart."""
+    rng = np.random.default_rng(40 + n)
+    h, w = 23, 31
+    g = (rng.integers(0, 17, size=(h, w)) / 8.0).astype(np.float32)
+    m = (rng.random((h, w)) < 0.8).astype(np.uint8)
+    m[5, 5:5 + n] = 0                               # sparse windows: counts 0..n
+    m[6:6 + n, 9] = 0
+    m[10:10 + n, 12:12 + n] = 0
+    m[10 + n // 2, 12 + n // 2] = 1                 # a window with one sample
+    got = oracle.local_stddev_f32(g, m, n, planted).astype(np.float64)
+    sd, sums, sqs, cnt, written = _closed_form_sd(g, m, n)
+    assert np.all(got[~written] == 0)
+    assert np.all(got[written & (cnt <= 1)] == 0)
+    assert np.any(written & (cnt == 1))
+    sel = written & (cnt > 1)
+    c = cnt[sel].astype(np.float64)
+    # rounding of sum^2 / n and of the subtraction, carried through / (n-1)
+    tol_var = (_ulp32(sums[sel] ** 2 / c) + _ulp32(sqs[sel])) / (c - 1) + _ulp32(sd[sel] ** 2)
+    err_var = np.abs(got[sel] ** 2 - sd[sel] ** 2)
+    assert np.all(err_var <= tol_var + 2 * got[sel] * _ulp32(got[sel])), (err_var - tol_var).max()
+
+
+def test_float_stddev_known_answer_float_sums():
+    """A 5x5 window of 4097.0f (n = 5), by hand in float: 4097^2 =
+    16785409 rounds to 16785408; the column sums of squares grow exactly to
+    83927040, the window's to 419635200 (sum 102425 exact).  sum^2 =
+    10490880625 rounds to 10490881024, / 25 to 419635232, and
+    419635200 - 419635232 = -32, so the variance is negative and the result
+    NaN.  Double sums give exactly 0 (the window is constant)."""
+    from oracle_binding import load_oracle
+    orc = load_oracle()
+    g = np.full((5, 5), 4097.0, np.float32)
+    m = np.ones((5, 5), np.uint8)
+    got = orc.local_stddev_f32(g, m, 5)
+    assert np.isnan(got[2, 2])
+    assert np.all(np.delete(got.ravel(), 12) == 0)
+    # the planted double accumulator misses it
+    assert orc.local_stddev_f32(g, m, 5, planted=1)[2, 2] == 0.0
+
+
+def test_float_stddev_known_answer_sliding_update():
+    """5 x 6 image, n = 5, column 0 = 8192.0f, columns 1-5 = 1.0f; outputs
+    at (2, 2) and (2, 3).  First window: sum = 5 * 8192 + 20 = 40980 (exact),
+    sum of squares 5 * 2^26 + 20 rounds to 335544320 (spacing 32), sum^2 =
+    1679360400 rounds to 1679360384, / 25 to 67174416, the difference
+    268369904 / 24 to 11182079: result RN(sqrt(11182079)).  Slide by one
+    column in the reference's order, sum += s_new - s_old: the squares'
+    update 5 - 335544320 rounds to -335544320, so the running sum of
+    squares becomes 0 while the sum becomes exactly 25: (0 - 625 / 25) / 24
+    < 0 and the result is NaN.  Summing the window anew (or in double)
+    gives 25 - 25 = 0, i.e. 0."""
+    from oracle_binding import load_oracle
+    orc = load_oracle()
+    g = np.ones((5, 6), np.float32)
+    g[:, 0] = 8192.0
+    m = np.ones((5, 6), np.uint8)
+    got = orc.local_stddev_f32(g, m, 5)
+    assert got[2, 2] == np.sqrt(np.float32(11182079.0), dtype=np.float32)
+    assert np.isnan(got[2, 3])
+    for planted in (1, 2):
+        assert orc.local_stddev_f32(g, m, 5, planted=planted)[2, 3] == 0.0
+
+
+def test_float_contrast_weight_is_the_float_stddev(oracle):
+    """The contrast weight of a float image with equal channels is
+    float(w_con * sd) of its own gray values (average projector: (3v)/3 =
+    v exactly for values whose triple is exact in float), through
+    enfuse_mask, on the known answers and a random image."""
+    rng = np.random.default_rng(3)
+    for g in (np.full((5, 5), 4097.0, np.float32), np.where(np.arange(6) == 0, 8192.0, 1.0).astype(np.float32)[None, :]
+              .repeat(5, 0), (rng.integers(0, 1 << 12, size=(17, 19)) / 64.0).astype(np.float32)):
+        h, w = g.shape
+        m = np.full((h, w), 255, np.uint8)
+        img = np.repeat(g[..., None], 3, axis=2)
+        got = oracle.enfuse_mask(img, m, F32, enfuse_params(w_exposure=0.0, w_contrast=1.0, contrast_window=5))
+        want = oracle.local_stddev_f32(g, m, 5)
+        np.testing.assert_array_equal(got, want)
