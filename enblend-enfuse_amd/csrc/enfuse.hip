@@ -400,21 +400,32 @@ __global__ __launch_bounds__(kTile) void entropy_kernel(const T* __restrict__ im
     out[(long)y * op + x] = res;
 }
 
-// ---- local entropy over LDS tiles with a sorting network -----------------
+// ---- local entropy over LDS tiles ------------------------------------------
 //
-// A block of 256 threads owns a 64 x 4 tile of output pixels.  The tile's
-// samples with an n/2 halo (channel-planar) and their alpha go to LDS once;
-// each thread then reads its n x n window from LDS.  A window whose samples
-// all lie under alpha is sorted by a comparator network held in registers
-// (Batcher's odd-even merge sort for the next power of two, pruned to the
-// window size: a comparator with an index past the window only ever meets
-// the +infinity padding and is a no-op), and the bins of equal keys are
-// walked in ascending key order, accumulating the full-window p*log(p) table
-// in the order the reference's std::map iterates them.  Windows that are cut
-// by alpha take the insertion sort of the general kernel on the LDS samples.
-// Equal keys are those that compare neither less nor greater (the map's
-// equivalence), so -0 and +0 share a bin.  NaN samples break the map's strict
-// weak ordering in the reference, so their bins are not defined there.
+// A block of 256 threads owns a 64 x 8 tile of output pixels (two rows per
+// thread).  The tile's samples with an n/2 halo (channel-planar) and their
+// alpha go to LDS once, then the minimum and maximum of every n-sample window
+// column.  Per pixel and channel, from those column extremes alone:
+//   * a window of one value (minimum == maximum) is one bin: entropy 0, so
+//     the pixel's channel minimum is 0 and nothing else is computed;
+//   * a window whose n columns are each constant has one bin per distinct
+//     column value, of n samples per column: its n column keys are sorted
+//     and walked like the full window's keys would be;
+//   * any other window goes to a per-block queue.  After a barrier every
+//     thread takes queue entries in turn, sorts the entry's n*n keys by a
+//     comparator network held in registers (Batcher's odd-even merge sort for
+//     the next power of two, pruned to the window size: a comparator with an
+//     index past the window only ever meets the +infinity padding and is a
+//     no-op) and walks the bins of equal keys in ascending key order,
+//     accumulating the full-window p*log(p) table in the order the
+//     reference's std::map iterates them.  So the sorting work of a block is
+//     spread over all its lanes instead of being paid by a whole wave for
+//     any one lane that needs it.
+// Windows that are cut by alpha take an insertion sort of the samples under
+// alpha and the general table, in place.  Equal keys are those that compare
+// neither less nor greater (the map's equivalence), so -0 and +0 share a bin.
+// NaN samples break the map's strict weak ordering in the reference, so their
+// bins are not defined there.
 
 struct CmpNet {
     int n;
@@ -458,73 +469,188 @@ template <> __device__ __forceinline__ void cas<float>(float& x, float& y)
 template <class T> struct SortKey { typedef int type; };
 template <> struct SortKey<float> { typedef float type; };
 
-constexpr int kEntTW = 64, kEntTH = 4;
+constexpr int kEntTW = 64, kEntTH = 8, kEntThreads = 256;
+
+// Bins of the ascending keys k[0..M-1] (at least two distinct), each key
+// standing for `mult` samples: adds the full-window table term of every bin
+// in ascending key order.  Non-boundaries add spent[0] = 0.0, which leaves the
+// sum unchanged bit for bit, so the table reads do not wait on one another.
+template <class K, int M>
+__device__ __forceinline__ double walk_bins(const K (&k)[M], int mult, const double* spent, int& bins)
+{
+    double acc = 0.0;
+    int st = 0;
+    bins = 1;
+#pragma unroll
+    for (int i = 1; i < M; ++i) {
+        const bool b = k[i - 1] < k[i];
+        const int len = i - st;
+        st = b ? i : st;
+        bins += b ? 1 : 0;
+        acc += spent[b ? len * mult : 0];
+    }
+    return acc + spent[(M - st) * mult];
+}
+
+// Appends the lanes with `pred` set to a block queue with one LDS atomic per
+// wave; returns each such lane's slot.  Called by every lane of the wave.
+__device__ __forceinline__ int wave_append(int* counter, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return -1;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + rank : -1;
+}
 
 template <class T, int N>
-__global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* __restrict__ img, long ip,
-                                                                       const uint8_t* __restrict__ alpha, long ap,
-                                                                       int w, int h, const double* __restrict__ tab,
-                                                                       T* __restrict__ out, long op)
+__global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __restrict__ img, long ip,
+                                                                   const uint8_t* __restrict__ alpha, long ap,
+                                                                   int w, int h, const double* __restrict__ tab,
+                                                                   T* __restrict__ out, long op)
 {
     constexpr int B = N / 2, NN = N * N;
     constexpr int LW = kEntTW + 2 * B, LH = kEntTH + 2 * B;
+    constexpr int NT = kEntThreads, NW = NT / 64, RPT = kEntTW * kEntTH / NT, RSTEP = NT / kEntTW;
+    constexpr int WROWS = (LH + NW - 1) / NW, VCH = (3 * LW + 63) / 64, ACH = (LW + 63) / 64;
+    constexpr int QMAX = 3 * kEntTW * kEntTH;
     typedef typename SortKey<T>::type K;
     __shared__ K sv[3][LH][LW];
+    __shared__ K cmn[3][kEntTH][LW];
+    // the column maxima are dead once the window classes are made; the
+    // channel entropies take their place
+    __shared__ union {
+        K cmx[3][kEntTH][LW];
+        T ent[3][kEntTH][kEntTW];
+    } u;
+    __shared__ uint8_t cls[3][kEntTH][kEntTW];
     __shared__ uint8_t sa[LH][LW + 4];
     __shared__ double spent[NN + 1], splog[NN + 1];
+    __shared__ unsigned short queue[QMAX], cqueue[QMAX];
+    __shared__ int qn, cqn;
+    auto& cmx = u.cmx;
+    auto& ent = u.ent;
     const int x0 = blockIdx.x * kEntTW, y0 = blockIdx.y * kEntTH;
-    const int tid = threadIdx.x;
-    // Halo tile: samples outside the image count as alpha 0.  All loads of
-    // a thread are issued before the first LDS store.
-    constexpr int NT = kEntTW * kEntTH;
-    constexpr int NV = (LH * 3 * LW + NT - 1) / NT, NA = (LH * LW + NT - 1) / NT;
-    K lv[NV];
-    uint8_t la[NA];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // Halo tile, one wave per tile row (rows wv, wv + NW, ...), lanes along
+    // the row's interleaved samples; samples outside the image count as
+    // alpha 0.  All loads of a thread are issued before the first LDS store.
+    K lv[WROWS][VCH];
+    uint8_t la[WROWS][ACH];
+    int voff[VCH];
+    bool vin[VCH];
 #pragma unroll
-    for (int u = 0; u < NV; ++u) {
-        const int i = tid + u * NT;
-        const int ly = i / (3 * LW), e = i - ly * (3 * LW);
+    for (int j = 0; j < VCH; ++j) {
+        const int e = lane + 64 * j;
         const int lx = e / 3, c = e - 3 * lx;
-        const int gy = y0 - B + ly, gx = x0 - B + lx;
-        const bool in = i < LH * 3 * LW && gy >= 0 && gy < h && gx >= 0 && gx < w;
-        lv[u] = in ? (K)img[(long)gy * ip + 3L * gx + c] : (K)0;
+        const int gx = x0 - B + lx;
+        vin[j] = e < 3 * LW && gx >= 0 && gx < w;
+        voff[j] = c * (LH * LW) + lx;
     }
 #pragma unroll
-    for (int u = 0; u < NA; ++u) {
-        const int i = tid + u * NT;
-        const int ly = i / LW, lx = i - ly * LW;
-        const int gy = y0 - B + ly, gx = x0 - B + lx;
-        const bool in = i < LH * LW && gy >= 0 && gy < h && gx >= 0 && gx < w;
-        la[u] = in ? alpha[(long)gy * ap + gx] : (uint8_t)0;
+    for (int k = 0; k < WROWS; ++k) {
+        const int ly = wv + NW * k, gy = y0 - B + ly;
+        const bool rin = ly < LH && gy >= 0 && gy < h;
+        const T* row = img + (long)gy * ip + 3L * (x0 - B);
+        const uint8_t* arow = alpha + (long)gy * ap + (x0 - B);
+#pragma unroll
+        for (int j = 0; j < VCH; ++j) lv[k][j] = rin && vin[j] ? (K)row[lane + 64 * j] : (K)0;
+#pragma unroll
+        for (int j = 0; j < ACH; ++j) {
+            const int lx = lane + 64 * j, gx = x0 - B + lx;
+            la[k][j] = rin && lx < LW && gx >= 0 && gx < w ? arow[lx] : (uint8_t)0;
+        }
     }
     int unset = 0;
 #pragma unroll
-    for (int u = 0; u < NV; ++u) {
-        const int i = tid + u * NT;
-        const int ly = i / (3 * LW), e = i - ly * (3 * LW);
-        const int lx = e / 3, c = e - 3 * lx;
-        if (i < LH * 3 * LW) sv[c][ly][lx] = lv[u];
-    }
+    for (int k = 0; k < WROWS; ++k) {
+        const int ly = wv + NW * k;
+        if (ly >= LH) continue;
 #pragma unroll
-    for (int u = 0; u < NA; ++u) {
-        const int i = tid + u * NT;
-        const int ly = i / LW, lx = i - ly * LW;
-        if (i < LH * LW) {
-            sa[ly][lx] = la[u];
-            unset |= la[u] == 0;
+        for (int j = 0; j < VCH; ++j)
+            if (lane + 64 * j < 3 * LW) (&sv[0][0][0])[voff[j] + ly * LW] = lv[k][j];
+#pragma unroll
+        for (int j = 0; j < ACH; ++j) {
+            const int lx = lane + 64 * j;
+            if (lx < LW) {
+                sa[ly][lx] = la[k][j];
+                unset |= la[k][j] == 0;
+            }
         }
     }
-    for (int i = tid; i <= NN; i += kEntTW * kEntTH) {
+    for (int i = tid; i <= NN; i += NT) {
         spent[i] = tab[i];
         splog[i] = tab[NN + 1 + i];
     }
+    if (tid == 0) { qn = 0; cqn = 0; }
     const double sdist = tab[2 * (NN + 1) + (NN + 1) * (NN + 1)];
     const bool tile_full = __syncthreads_or(unset) == 0;
-    const int lx = tid % kEntTW, ly = tid / kEntTW;
-    const int x = x0 + lx, y = y0 + ly;
-    if (x >= w || y >= h) return;
-    T res = T(0);
-    if (x >= B && x < w - B && y >= B && y < h - B && sa[ly + B][lx + B]) {
+    // Extremes of every window column (N rows from output row r): one thread
+    // per tile column and channel reads the column once.
+    for (int i = tid; i < 3 * LW; i += NT) {
+        const int c = i / LW, cx = i - c * LW;
+        K col[LH];
+#pragma unroll
+        for (int r = 0; r < LH; ++r) col[r] = sv[c][r][cx];
+#pragma unroll
+        for (int r = 0; r < kEntTH; ++r) {
+            K mn = col[r], mx = col[r];
+#pragma unroll
+            for (int dy = 1; dy < N; ++dy) {
+                kmin(mn, col[r + dy]);
+                kmax(mx, col[r + dy]);
+            }
+            cmn[c][r][cx] = mn;
+            cmx[c][r][cx] = mx;
+        }
+    }
+    __syncthreads();
+    // Window classes from the column extremes, sliding along each row in
+    // runs of kEntSeg outputs: bit 0 = one value (flat), bit 1 = every
+    // column constant.
+    constexpr int kEntSeg = 8, NSEG = kEntTW / kEntSeg;
+    for (int i = tid; i < 3 * kEntTH * NSEG; i += NT) {
+        const int c = i / (kEntTH * NSEG), e = i - c * (kEntTH * NSEG);
+        const int r = e / NSEG, x0s = (e - r * NSEG) * kEntSeg;
+        K rmn[kEntSeg + N - 1], rmx[kEntSeg + N - 1];
+#pragma unroll
+        for (int j = 0; j < kEntSeg + N - 1; ++j) {
+            rmn[j] = cmn[c][r][x0s + j];
+            rmx[j] = cmx[c][r][x0s + j];
+        }
+#pragma unroll
+        for (int j = 0; j < kEntSeg; ++j) {
+            K mn = rmn[j], mx = rmx[j];
+            bool cols = !(rmn[j] < rmx[j]);
+#pragma unroll
+            for (int dx = 1; dx < N; ++dx) {
+                kmin(mn, rmn[j + dx]);
+                kmax(mx, rmx[j + dx]);
+                cols = cols && !(rmn[j + dx] < rmx[j + dx]);
+            }
+            cls[c][r][x0s + j] = (uint8_t)((mn < mx ? 0 : 1) | (cols ? 2 : 0));
+        }
+    }
+    __syncthreads();
+    // Per pixel: cut windows in place; a flat channel makes the minimum 0;
+    // every other channel goes to the queue of constant-column windows or
+    // to the sorting queue.
+    const int lx = tid % kEntTW;
+    T res[RPT];
+    int pend[RPT];                 // bit c: channel c's value comes from a queue
+    bool store[RPT];
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+        const int ly = tid / kEntTW + p * RSTEP;
+        const int x = x0 + lx, y = y0 + ly;
+        res[p] = T(0);
+        pend[p] = 0;
+        store[p] = x < w && y < h;
+        if (!(store[p] && x >= B && x < w - B && y >= B && y < h - B && sa[ly + B][lx + B])) continue;
         bool full = tile_full;
         if (!full) {
             full = true;
@@ -533,56 +659,12 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
 #pragma unroll
                 for (int dx = 0; dx < N; ++dx) full = full && sa[ly + dy][lx + dx] != 0;
         }
-        T emin = T(0);
-        for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-            int bins = 0;
-            if (full) {
-                K v[NN];
-#pragma unroll
-                for (int dy = 0; dy < N; ++dy)
-#pragma unroll
-                    for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][ly + dy][lx + dx];
-                // A window of one value is one bin (entropy 0): no sort.
-                K mn = v[0], mx = v[0];
-#pragma unroll
-                for (int k = 1; k < NN; ++k) { kmin(mn, v[k]); kmax(mx, v[k]); }
-                bins = 1;
-                if (mn < mx) {
-                    constexpr CmpNet net = SortNet<NN>::net;
-#pragma unroll
-                    for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
-                    bool distinct = true;
-#pragma unroll
-                    for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
-                    if (distinct) {
-                        // NN bins of one sample: the table's one-sample term
-                        // added NN times in order, precomputed
-                        bins = NN;
-                        acc = sdist;
-                    } else {
-                        // Bin boundaries where the sorted keys step up; the
-                        // bin that ends at a boundary k started at the
-                        // previous boundary st.  Non-boundaries add
-                        // spent[0] = 0.0, which leaves the sum unchanged bit
-                        // for bit, so the table reads do not wait on one
-                        // another.
-                        int st = 0;
-#pragma unroll
-                        for (int k = 1; k < NN; ++k) {
-                            const bool b = v[k - 1] < v[k];
-                            const int len = k - st;
-                            st = b ? k : st;
-                            bins += b ? 1 : 0;
-                            acc += spent[b ? len : 0];
-                        }
-                        acc += spent[NN - st];
-                    }
-                }
-            } else {
-                // Cut window: insertion sort of the samples under alpha; the
-                // general p*log(p) table for `cnt` samples.
-                const double* pgen = tab + 2 * (NN + 1);
+        if (!full) {
+            // Cut window: insertion sort of the samples under alpha; the
+            // general p*log(p) table for `cnt` samples.
+            const double* pgen = tab + 2 * (NN + 1);
+            T emin = T(0);
+            for (int c = 0; c < 3; ++c) {
                 K v[NN];
                 int cnt = 0;
                 for (int dy = 0; dy < N; ++dy)
@@ -593,24 +675,105 @@ __global__ __launch_bounds__(kEntTW * kEntTH) void entropy_tile_kernel(const T* 
                             while (k > 0 && sm < v[k - 1]) { v[k] = v[k - 1]; --k; }
                             v[k] = sm;
                         }
-                int run = 1;
+                double acc = 0.0;
+                int bins = 0, run = 1;
                 for (int k = 1; k <= cnt; ++k) {
                     if (k < cnt && !(v[k - 1] < v[k])) { ++run; continue; }
                     acc += pgen[cnt * (NN + 1) + run];
                     ++bins;
                     run = 1;
                 }
+                const double e = bins > 1 ? -acc / splog[bins] : 0.0;
+                const T ec = from_real_img<T>(e * Img<T>::maxv());
+                emin = c == 0 ? ec : (ec < emin ? ec : emin);
+                // every channel's value is +0 or more: once the minimum is 0
+                // the remaining channels cannot change it
+                if (emin == T(0)) break;
             }
-            const double e = bins > 1 ? -acc / splog[bins] : 0.0;
-            const T ec = from_real_img<T>(e * Img<T>::maxv());
-            emin = c == 0 ? ec : (ec < emin ? ec : emin);
-            // every channel's value is +0 or more: once the minimum is 0 the
-            // remaining channels cannot change it
-            if (emin == T(0)) break;
+            res[p] = emin;
+            continue;
         }
-        res = emin;
+        const int k0 = cls[0][ly][lx], k1 = cls[1][ly][lx], k2 = cls[2][ly][lx];
+        if ((k0 | k1 | k2) & 1) continue;     // a one-bin channel: the minimum is 0
+        pend[p] = 7 | (((k0 >> 1) & 1) << 3) | (((k1 >> 1) & 1) << 4) | (((k2 >> 1) & 1) << 5);
     }
-    out[(long)y * op + x] = res;
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+        const int pix = (tid / kEntTW + p * RSTEP) * kEntTW + lx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool queued = (pend[p] >> c) & 1;
+            const bool colc = (pend[p] >> (3 + c)) & 1;
+            const int cs = wave_append(&cqn, queued && colc);
+            const int ss = wave_append(&qn, queued && !colc);
+            if (cs >= 0) cqueue[cs] = (unsigned short)((pix << 2) | c);
+            if (ss >= 0) queue[ss] = (unsigned short)((pix << 2) | c);
+        }
+    }
+    __syncthreads();
+    // Constant-column windows: one bin per distinct column value, N samples
+    // per column; the N column keys are sorted and walked.
+    const int ncq = cqn;
+    for (int i = tid; i < ncq; i += NT) {
+        const int item = cqueue[i];
+        const int c = item & 3, pix = item >> 2;
+        const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
+        K k[N];
+#pragma unroll
+        for (int dx = 0; dx < N; ++dx) k[dx] = cmn[c][qy][qx + dx];
+        constexpr CmpNet net = SortNet<N>::net;
+#pragma unroll
+        for (int q = 0; q < net.n; ++q) cas(k[net.a[q]], k[net.b[q]]);
+        int bins;
+        const double acc = walk_bins(k, N, spent, bins);
+        ent[c][qy][qx] = from_real_img<T>(-acc / splog[bins] * Img<T>::maxv());
+    }
+    // Other windows: all NN keys sorted.
+    const int nq = qn;
+    for (int i = tid; i < nq; i += NT) {
+        const int item = queue[i];
+        const int c = item & 3, pix = item >> 2;
+        const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
+        K v[NN];
+#pragma unroll
+        for (int dy = 0; dy < N; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][qy + dy][qx + dx];
+        constexpr CmpNet net = SortNet<NN>::net;
+#pragma unroll
+        for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
+        bool distinct = true;
+#pragma unroll
+        for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
+        double acc;
+        int bins;
+        if (distinct) {
+            // NN bins of one sample: the table's one-sample term added NN
+            // times in order, precomputed
+            bins = NN;
+            acc = sdist;
+        } else {
+            acc = walk_bins(v, 1, spent, bins);
+        }
+        const double e = bins > 1 ? -acc / splog[bins] : 0.0;
+        ent[c][qy][qx] = from_real_img<T>(e * Img<T>::maxv());
+    }
+    __syncthreads();
+    // Channel minimum (entropies are +0 or more and never NaN, so the order
+    // of the comparisons does not matter).
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+        if (!store[p]) continue;
+        const int ly = tid / kEntTW + p * RSTEP;
+        T r = res[p];
+        if (pend[p] & 7) {
+            r = ent[0][ly][lx];
+            const T e1 = ent[1][ly][lx], e2 = ent[2][ly][lx];
+            r = e1 < r ? e1 : r;
+            r = e2 < r ? e2 : r;
+        }
+        out[(long)(y0 + ly) * op + x0 + lx] = r;
+    }
 }
 
 template <class T> __device__ __forceinline__ float contrast_term(double wc, double g);
@@ -902,7 +1065,7 @@ int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, lo
 {
     if (n * n > 121) return ENBL_E_UNSUPPORTED;
     if (n == 3 || n == 5 || n == 7) {
-        const dim3 grid((w + kEntTW - 1) / kEntTW, (h + kEntTH - 1) / kEntTH), block(kEntTW * kEntTH);
+        const dim3 grid((w + kEntTW - 1) / kEntTW, (h + kEntTH - 1) / kEntTH), block(kEntThreads);
 #define ENBL_ENT_TILE(T, N) hipLaunchKernelGGL((entropy_tile_kernel<T, N>), grid, block, 0, s, (const T*)img, ip, alpha, ap, w, h, tab, (T*)out, op)
 #define ENBL_ENT_N(T)                   \
     do {                                \

@@ -198,6 +198,46 @@ def test_entropy_window_kinds_bit_exact(enbl, gpu_ctx, oracle, itype, ewin):
     np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("ewin", [3, 5, 7])
+def test_entropy_constant_column_windows_bit_exact(enbl, gpu_ctx, oracle, itype, ewin):
+    """Windows whose columns are each constant (values depending on x only):
+    one bin per distinct column value of n samples each, including columns
+    of equal value (bins of 2n, 3n samples), beside windows where one column
+    or one row breaks the pattern, a single-channel flat region (the pixel's
+    minimum is 0) and -0 / +0 columns for float images; the entropy weight
+    alone, bit for bit against the oracle."""
+    rng = np.random.default_rng(1300 + itype + ewin)
+    h, w = 96, 260
+    m = {U8: 255, U16: 65535, F32: 1.0}[itype]
+    xs = np.arange(w)
+    colv = np.empty((w, 3))
+    colv[:, 0] = (np.floor(xs / 2) % 9) / 9.0           # pairs of equal columns
+    colv[:, 1] = rng.permutation(np.linspace(0.1, 0.9, w))  # distinct columns
+    colv[:, 2] = (np.floor(xs / 3) % 5) / 5.0           # triples of equal columns
+    img = np.repeat(colv[None, :, :], h, axis=0) * m
+    img[40:44, 100:160] = rng.random((4, 60, 3)) * m     # rows that break the columns
+    img[70, 200:] = 0.5 * m
+    img[:, 220:230, 1] = 0.25 * m                          # flat in one channel only
+    if itype == F32:
+        img = img.astype(np.float32)
+        img[:, 150:152, 0] = -0.0                          # -0 and +0 share a bin
+        img[:, 152:154, 0] = 0.0
+    else:
+        img = np.rint(img).astype(np.uint8 if itype == U8 else np.uint16)
+    a = np.full((h, w), 255, np.uint8)
+    kw = dict(w_exposure=0.0, w_entropy=1.0, entropy_window=ewin)
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(np.ascontiguousarray(img)).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"entropy constant-column windows itype={itype} n={ewin}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
 def _table_call(enbl, ctx, itype, table, entry):
     """Calls one enfuse entry point on a tiny stack with an exposure table;
     returns the status code."""
