@@ -1,22 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the enblend/enfuse pyramid path on one MI355X.
 
-Workload (one step): one enblend blend step of two 4096x3072 RGBA uint16
-images sharing one synthetic scene, 25% horizontal overlap, on their
-7168x3072 union canvas with 10 pyramid levels, the seam mask being the
-nearest-feature transform of the two alphas (made once, timed apart as
-`nft`): mask Gaussian pyramid,
-two Laplacian pyramids under the images' alphas, blend, collapse and the
-dithered copy back into the 16-bit image (src/enblend.h:378-510).  Inputs are
-resident in HBM before timing starts.
+Headline workload (one step): enfuse of 8 float32 RGB exposures at
+6000x4000 (one synthetic scene, EV -3.5..+3.5), weights exposure 1.0 +
+saturation 0.2 + contrast 0.5 (5x5) + entropy 0.3 (7x7), 10 levels of double
+pyramids (src/enfuse.h:1217-1609).  Inputs are resident in HBM before timing
+starts.
 
-Prints one JSON line.  `value` is input Mpix (2 x 4096 x 3072 per step)
-per second of wall time over the timed steps.  `roofline` is the dominant
-kernel's algorithmic bytes per launch over its HIP-event launch duration,
-measured on the library's stream in a profiled pass of the same workload.
+Prints one JSON line.  `value` is input Mpix (8 x 6000 x 4000 per step) per
+second of wall time over the timed steps; `host_resident` is the same step
+with pinned host buffers uploaded and downloaded around it.  `roofline` is
+the dominant kernel's algorithmic bytes per launch over its HIP-event launch
+duration, measured on the library's stream in a profiled pass of the same
+workload; `pyramid_kernels` is the same fraction over all pyramid kernels.
 `cpu_baseline` times the oracle (the C++ restatement of the reference's
 algorithm, OpenMP pointwise stages, single-threaded SKIPSM passes) on the
-host cores for the same workload.
+host cores for the full workload, whose output is also the parity check.
+Further parts: the enblend step of two 4096x3072 RGBA uint16 images with
+its seam mask (`enblend_c1`), the whole enblend run from decoded layers,
+the 64-stack batch against the plain path, the small 8-bit enfuse and the
+adjoint check.
 """
 import argparse
 import ctypes
@@ -39,14 +42,11 @@ def _load(name, path):
     return mod
 
 
-TRAFFIC_FILE = "profiles/enblend_pair_u16_4096x3072x2_traffic.md"
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
+def measured_traffic(kernel, traffic_file):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (calibrated on a plain copy), or None."""
     try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+        with open(os.path.join(ROOT, traffic_file)) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
@@ -87,8 +87,7 @@ def batch_stacks(enbl, scenes, torch, dev):
 
 def _kernels(rep, runs=1):
     return {k: {"ms": round(v["ms"] / runs, 3), "launches": v["launches"] // runs,
-                "GB_per_s": round(v["own_bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
-                "credited_GB_per_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}
+                "GB_per_s": round(v["own_bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in sorted(rep.items())}
 
 
 def batch_compare(args, enbl, scenes, torch, steps, warmup):
@@ -443,15 +442,59 @@ def c2_params(mod):
                              contrast_window=5, entropy_window=7)
 
 
-def enfuse_c2(args, enbl, scenes, np, torch, cpu=True, sample=(6000, 2000)):
-    """Plain enfuse of 8 float32 RGB exposures at 6000x4000 (radiance
-    1e-3..1e2, EV -3.5..+3.5 one EV apart, no clipping); weights exposure 1.0,
-    saturation 0.2, contrast 0.5 (5x5 local standard deviation), entropy 0.3
-    (7x7 window, per-channel exact-value histograms as the reference computes
-    them); 10 levels of double pyramids.  Parity (2e-6 relative) and the
-    oracle's CPU time are taken on a bounded sample: the same 8 exposures
-    cropped to `sample` (w, h) from the scene's centre (full width, the
-    middle half of the rows), fused through both paths."""
+C2_WORKLOAD = ("enfuse, 8 x 6000x4000 RGB float32, EV -3.5..+3.5 one EV apart, exposure 1.0 + saturation 0.2 + "
+               "contrast 0.5 (5x5 local std-dev) + entropy 0.3 (7x7), 10 levels, double pyramids")
+C2_TRAFFIC = "profiles/enfuse_f32_6000x4000x8_traffic.md"
+C1_TRAFFIC = "profiles/enblend_pair_u16_4096x3072x2_traffic.md"
+# Kernels of the pyramid build, merge and collapse (the reduce / expand
+# passes and the passes that fuse them) as the library's profile names them.
+PYRAMID_KERNEL_PREFIXES = ("reduce", "expand", "laplacian", "collapse", "merge", "tail", "to_pyramid", "from_pyramid")
+
+
+def _roofline(rep, runs, traffic_file):
+    """Roofline object for the dominant kernel (largest total time) of a
+    profiled pass: its own algorithmic bytes per launch (each input read once,
+    each output written once) over its mean HIP-event launch duration."""
+    total_ms = sum(v["ms"] for v in rep.values())
+    dom_name, dom = max(rep.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = dom["ms"] / dom["launches"]
+    avg_own = dom["own_bytes"] / dom["launches"]
+    achieved = avg_own / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom_name, traffic_file)
+    return {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "achieved_basis": "the kernel's own algorithmic bytes per launch (each input read once, each output "
+                              "written once) over its mean HIP-event launch duration on the library's stream",
+            "algorithmic_bytes_per_launch": round(avg_own), "launch_ms": round(avg_ms, 5),
+            "launches_per_step": dom["launches"] // runs,
+            "traffic_source": traffic_file if traffic else None,
+            "dram_GB_per_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "kernel_ms_per_step": round(total_ms / runs, 4)}
+
+
+def _pyramid_kernels(rep, runs):
+    """HBM fraction of the pyramid kernels together: their own bytes over
+    their summed HIP-event time."""
+    sel = {k: v for k, v in rep.items() if k.startswith(PYRAMID_KERNEL_PREFIXES)}
+    ms = sum(v["ms"] for v in sel.values())
+    own = sum(v["own_bytes"] for v in sel.values())
+    gbs = own / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"kernels": sorted(sel), "ms_per_step": round(ms / runs, 4), "own_bytes_per_step": round(own / runs),
+            "GB_per_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def enfuse_c2(args, enbl, scenes, np, torch, cpu=True):
+    """The headline: plain enfuse of 8 float32 RGB exposures at 6000x4000
+    (radiance 1e-3..1e2, EV -3.5..+3.5 one EV apart, no clipping); weights
+    exposure 1.0, saturation 0.2, contrast 0.5 (5x5 local standard
+    deviation), entropy 0.3 (7x7 window, per-channel exact-value histograms
+    as the reference computes them); 10 levels of double pyramids; full alpha.
+    One step = enbl_enfuse on the 8 device-resident exposures (weights,
+    normalisation, the 8 Laplacian and mask pyramids, the merge of every
+    level, collapse and the conversion into the float output).  Also: the same
+    step with host-resident inputs and outputs (pinned buffers, PCIe
+    included), and the oracle on the full workload for the CPU baseline and
+    the parity check (2e-6 relative, non-finite pattern equal)."""
     dev = torch.device("cuda:0")
     W, H = 6000, 4000
     print("bench: generating the 8 x 6000x4000 float scene", file=sys.stderr, flush=True)
@@ -465,42 +508,242 @@ def enfuse_c2(args, enbl, scenes, np, torch, cpu=True, sample=(6000, 2000)):
     mpix = len(imgs) * W * H / 1e6
     torch.cuda.synchronize()
     ctx = enbl.Context(0)
-    print("bench: enfuse 8 x 6000x4000 float32", file=sys.stderr, flush=True)
-    ms, rep = _time_enfuse(enbl, ctx, enbl.F32, imgs, alphas, prm, L, out, oa, args.c2_steps, 1)
-    res = {"workload": "enfuse, 8 x 6000x4000 RGB float32, EV -3.5..+3.5, exposure 1.0 + saturation 0.2 + "
-                       f"contrast 0.5 (5x5) + entropy 0.3 (7x7), {L} levels, double pyramids",
-           "input_mpix": round(mpix, 1), "ms_per_step": round(ms, 3), "mpix_per_s": round(mpix / (ms * 1e-3), 1),
-           "steps": args.c2_steps, "kernels": _kernels(rep)}
-    sw, sh = sample
-    x0, y0 = (W - sw) // 2, (H - sh) // 2
-    crop = [i[y0:y0 + sh, x0:x0 + sw].contiguous() for i in imgs]
-    calph = [a[:sh, :sw].contiguous() for a in alphas]
-    cout = torch.zeros_like(crop[0])
-    coa = torch.zeros((sh, sw), dtype=torch.uint8, device=dev)
-    Ls = enbl.auto_levels(sw, sh)
-    enbl.enfuse(ctx, enbl.F32, crop, calph, prm, Ls, cout, coa, dither=enbl.DITHER_DET, seed=1)
+
+    def run():
+        enbl.enfuse(ctx, enbl.F32, imgs, alphas, prm, L, out, oa, dither=enbl.DITHER_DET, seed=1)
+
+    print(f"bench: enfuse 8 x 6000x4000 float32: {args.warmup} warmup + {args.steps} timed steps", file=sys.stderr,
+          flush=True)
+    run()
     ctx.sync()
-    got = cout.cpu().numpy()
-    host = [c.cpu().numpy() for c in crop]
+    got = out.cpu().numpy()
+    got_a = oa.cpu().numpy()
+    for _ in range(args.warmup):
+        run()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    nprof = 2
+    ctx.profile(True)
+    for _ in range(nprof):
+        run()
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    res = {
+        "metric": METRIC,
+        "value": round(mpix / (ms * 1e-3), 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": C2_WORKLOAD, "input_mpix_per_step": mpix, "levels": L,
+                   "inputs": "resident in HBM when the timed region starts"},
+        "roofline": _roofline(rep, nprof, C2_TRAFFIC),
+        "pyramid_kernels": _pyramid_kernels(rep, nprof),
+        "kernels": {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
+                        "GB_per_s": round(v["own_bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
+                    for k, v in sorted(rep.items())},
+    }
+    if not args.no_host:
+        # Host-resident inputs and outputs: the 8 exposures and alphas uploaded
+        # from pinned host memory, the step, the output image and alpha
+        # downloaded (what a caller handing over host buffers sees).
+        print("bench: host-resident (PCIe-inclusive) steps", file=sys.stderr, flush=True)
+        h_in = [t.cpu().pin_memory() for t in imgs + alphas]
+        h_out = [torch.empty(t.shape, dtype=t.dtype).pin_memory() for t in (out, oa)]
+        d_in = imgs + alphas
+
+        def host_step():
+            for d, hh in zip(d_in, h_in):
+                d.copy_(hh, non_blocking=True)
+            torch.cuda.synchronize()
+            run()
+            ctx.sync()
+            for hh, d in zip(h_out, (out, oa)):
+                hh.copy_(d, non_blocking=True)
+            torch.cuda.synchronize()
+
+        host_step()
+        hruns = 3
+        t0 = time.perf_counter()
+        for _ in range(hruns):
+            host_step()
+        hms = (time.perf_counter() - t0) * 1e3 / hruns
+        hbytes = sum(t.numel() * t.element_size() for t in h_in + h_out)
+        res["host_resident"] = {"ms_per_step": round(hms, 3), "mpix_per_s": round(mpix / (hms * 1e-3), 1),
+                                "host_bytes_per_step": hbytes, "runs": hruns,
+                                "note": "pinned host buffers in and out around the same step (PCIe included)"}
+        del h_in, h_out
+    host = [t.cpu().numpy() for t in imgs]
     ctx.close()
-    del imgs, alphas, out, oa, crop, calph
+    del imgs, alphas, out, oa
+    torch.cuda.empty_cache()
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_binding import F32, load_oracle
         import oracle_binding
         o = load_oracle()
-        al = [np.full((sh, sw), 255, np.uint8) for _ in host]
-        print(f"bench: oracle enfuse on the 8 x {sw}x{sh} sample", file=sys.stderr, flush=True)
+        al = [np.full((H, W), 255, np.uint8) for _ in host]
+        print("bench: oracle enfuse on the full 8 x 6000x4000 workload", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        want, _ = o.enfuse(host, al, F32, c2_params(oracle_binding), Ls, dither=1, seed=1)
+        want, want_a = o.enfuse(host, al, F32, c2_params(oracle_binding), L, dither=1, seed=1)
         cpu_s = time.perf_counter() - t0
         err = _rel_err_float(got, want)
         res["parity_rel_err_max"] = err
-        res["parity"] = bool(err <= 2e-6)
-        res["parity_sample"] = f"8 x {sw}x{sh} centre crop, {Ls} levels; non-finite pattern must match"
-        res["cpu_baseline"] = {"value": round(8 * sw * sh / 1e6 / cpu_s, 3), "unit": "Mpix/s",
-                               "cores": o.lib.orc_num_threads(), "kind": "port",
-                               "sample": f"8 x {sw}x{sh} centre crop of the same exposures, one run ({cpu_s:.1f} s)"}
+        res["parity"] = bool(err <= 2e-6 and np.array_equal(got_a, want_a))
+        res["parity_sample"] = "the full workload; non-finite pattern must match"
+        res["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": o.lib.orc_num_threads(),
+                               "kind": "port", "sample": f"the full 8 x 6000x4000 workload, one run ({cpu_s:.1f} s)"}
+    return res
+
+
+def enblend_c1(args, enbl, scenes, np, torch, cpu=True):
+    """One enblend blend step of two 4096x3072 RGBA uint16 images sharing one
+    synthetic scene, 25% horizontal overlap, on their 7168x3072 union canvas
+    with 10 pyramid levels (src/enblend.h:378-510): mask Gaussian pyramid,
+    two Laplacian pyramids under the images' alphas, blend, collapse and the
+    dithered copy back into the 16-bit image.  The seam mask is the
+    nearest-feature transform of the two alphas (`nft`, timed apart and
+    added in `step_with_nft_ms`).  Every timed step blends its own staged copy
+    of the black image and alpha (the step updates them in place), so each
+    one sees the stated inputs."""
+    print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
+    white, wa, black, ba, _ = scenes.enblend_pair(seed=1)
+    h, w, C = white.shape
+    levels = enbl.auto_levels(w, h)
+    mpix = 2 * 4096 * 3072 / 1e6
+    dev = torch.device("cuda:0")
+    d_white = torch.from_numpy(white).to(dev)
+    d_wa = torch.from_numpy(wa).to(dev)
+    d_black0 = torch.from_numpy(black).to(dev)
+    d_ba0 = torch.from_numpy(ba).to(dev)
+    d_mask = torch.empty((h, w), dtype=torch.uint8, device=dev)
+    steps = max(args.steps, 5)
+    nprof = 3
+    # staged pristine copies: parity, warmup, timed and profiled steps
+    ncopies = 1 + args.warmup + steps + nprof
+    stage = [(d_black0.clone(), d_ba0.clone()) for _ in range(ncopies)]
+    torch.cuda.synchronize()
+    ctx = enbl.Context(0)
+    L = enbl.lib()
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    print("bench: nearest-feature-transform seam mask", file=sys.stderr, flush=True)
+
+    def nft():
+        enbl.check(L.enbl_nft_mask(ctx.h, w, h, vp(d_wa), w, vp(d_ba0), w, 2, vp(d_mask), w))
+
+    nft()
+    ctx.sync()
+    nft_runs = 5
+    t0 = time.perf_counter()
+    for _ in range(nft_runs):
+        nft()
+    ctx.sync()
+    nft_ms = (time.perf_counter() - t0) * 1e3 / nft_runs
+    ctx.profile(True)
+    nft()
+    nft_rep = ctx.profile_report()
+    ctx.profile(False)
+    mask = d_mask.cpu().numpy()
+
+    def step(bk, bka):
+        enbl.check(L.enbl_enblend_pair(ctx.h, enbl.U16, C, w, h, levels, 0,
+                                       vp(d_white), w * C, vp(d_wa), w, vp(bk), w * C, vp(bka), w,
+                                       vp(d_mask), w, enbl.DITHER_DET, 1))
+
+    print("bench: enblend step for parity", file=sys.stderr, flush=True)
+    it = iter(stage)
+    b, bal = next(it)
+    step(b, bal)
+    ctx.sync()
+    gpu_out, gpu_alpha = b.cpu().numpy(), bal.cpu().numpy()
+    for _ in range(args.warmup):
+        step(*next(it))
+    ctx.sync()
+    torch.cuda.synchronize()
+    print(f"bench: timing {steps} enblend steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(*next(it))
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    for _ in range(nprof):
+        step(*next(it))
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    res = {"workload": "enblend step, 2 x 4096x3072 RGBA uint16, 25% overlap, 7168x3072 ROI, NFT seam mask, "
+                       f"{levels} levels, Int32 fixed-point pyramids",
+           "input_mpix_per_step": mpix, "steps": steps, "ms_per_step": round(ms, 4),
+           "mpix_per_s": round(mpix / (ms * 1e-3), 1),
+           "step_with_nft_ms": round(ms + nft_ms, 4),
+           "step_with_nft_mpix_per_s": round(mpix / ((ms + nft_ms) * 1e-3), 1),
+           "roofline": _roofline(rep, nprof, C1_TRAFFIC),
+           "pyramid_kernels": _pyramid_kernels(rep, nprof),
+           "kernels": {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
+                           "GB_per_s": round(v["own_bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
+                       for k, v in sorted(rep.items())},
+           "nft": {"workload": f"nearest-feature-transform seam mask of the two alphas, {w}x{h}, Euclidean, "
+                               "float Felzenszwalb-Huttenlocher transforms", "ms": round(nft_ms, 3),
+                   "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs,
+                   "kernels": _kernels(nft_rep)}}
+    if not args.no_host:
+        print("bench: PCIe-inclusive enblend steps", file=sys.stderr, flush=True)
+        d_black, d_ba = stage[0]
+        h_in = [torch.from_numpy(a).pin_memory() for a in (white, wa, black, ba, mask)]
+        h_out = [torch.empty(t.shape, dtype=t.dtype).pin_memory() for t in (d_black, d_ba)]
+        d_in = [d_white, d_wa, d_black, d_ba, d_mask]
+
+        def pcie_step():
+            for d, hh in zip(d_in, h_in):
+                d.copy_(hh, non_blocking=True)
+            torch.cuda.synchronize()
+            step(d_black, d_ba)
+            ctx.sync()
+            for hh, d in zip(h_out, (d_black, d_ba)):
+                hh.copy_(d, non_blocking=True)
+            torch.cuda.synchronize()
+
+        pcie_step()
+        pcie_runs = 5
+        t0 = time.perf_counter()
+        for _ in range(pcie_runs):
+            pcie_step()
+        pcie_ms = (time.perf_counter() - t0) * 1e3 / pcie_runs
+        res["host_resident"] = {"ms_per_step": round(pcie_ms, 3), "mpix_per_s": round(mpix / (pcie_ms * 1e-3), 1),
+                                "host_bytes_per_step": sum(t.numel() * t.element_size() for t in h_in + h_out),
+                                "runs": pcie_runs,
+                                "note": "pinned host buffers in and out around the same step (PCIe included)"}
+    ctx.close()
+    del stage, d_white, d_wa, d_black0, d_ba0, d_mask
+    torch.cuda.empty_cache()
+    if cpu:
+        print("bench: CPU baseline of the enblend step (oracle)", file=sys.stderr, flush=True)
+        cpu_s, threads, (cpu_out, cpu_alpha) = cpu_baseline(white, wa, black, ba, mask, levels, args.cpu_repeats)
+        res["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "sample": f"full workload, median of {args.cpu_repeats} runs ({cpu_s:.2f} s each)"}
+        res["parity"] = bool(np.array_equal(cpu_out, gpu_out) and np.array_equal(cpu_alpha, gpu_alpha))
+        from oracle_binding import load_oracle
+        o = load_oracle()
+        t0 = time.perf_counter()
+        cpu_mask, cpu_rc = o.nft_mask(wa, ba)
+        nft_cpu_s = time.perf_counter() - t0
+        res["nft"]["parity_vs_oracle"] = bool(cpu_rc == 0 and np.array_equal(cpu_mask, mask))
+        res["nft"]["cpu_baseline"] = {"ms": round(nft_cpu_s * 1e3, 1), "cores": o.lib.orc_num_threads(),
+                                      "kind": "port", "sample": "full mask, one run"}
     return res
 
 
@@ -510,17 +753,19 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-repeats", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="enblend",
-                    choices=["enblend", "enblend-layers", "enfuse-c0", "enfuse-c2", "batch-c3", "adjoint-c4",
+    ap.add_argument("--no-cpu", action="store_true", help="leave out the oracle (CPU baseline and parity)")
+    ap.add_argument("--no-host", action="store_true", help="leave out the host-resident (PCIe-inclusive) steps")
+    ap.add_argument("--workload", default="enfuse-c2",
+                    choices=["enfuse-c2", "enblend-c1", "enblend-layers", "enfuse-c0", "batch-c3", "adjoint-c4",
                              "copy-calibration"],
-                    help="enblend: the headline line (default); the others print one part of it alone")
-    ap.add_argument("--c2-steps", type=int, default=3, help="timed steps of the 8 x 6000x4000 float enfuse")
-    ap.add_argument("--no-enfuse", action="store_true", help="leave the enfuse workloads out of the line")
+                    help="enfuse-c2: the headline line with every part (default); the others print one part alone")
+    ap.add_argument("--no-c1", action="store_true", help="leave the enblend step out of the line")
+    ap.add_argument("--no-enfuse", action="store_true", help="leave the small enfuse workload out of the line")
     ap.add_argument("--no-layers", action="store_true",
                     help="leave the enblend run from decoded layers (assembly through checkpoint) out of the line")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-vs-plain comparison out of the line")
     ap.add_argument("--no-adjoint", action="store_true", help="leave the adjoint check and tuning run out of the line")
+    ap.add_argument("--headline-only", action="store_true", help="the headline workload alone")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--tuning", default=os.path.join(ROOT, "enblend-enfuse_amd", "batch_tuning.json"),
                     help="launch shapes of the batched passes (JSON written by --autotune)")
@@ -535,14 +780,15 @@ def main():
 
     enbl = _load("enblend_enfuse_amd", os.path.join(ROOT, "enblend-enfuse_amd", "__init__.py"))
     scenes = _load("enblend_scenes", os.path.join(ROOT, "enblend-enfuse_amd", "scenes.py"))
+    cpu = not args.no_cpu
+    if args.workload == "enblend-c1":
+        print(json.dumps(enblend_c1(args, enbl, scenes, np, torch, cpu=cpu)), flush=True)
+        return
     if args.workload == "enblend-layers":
-        print(json.dumps(enblend_layers_c1(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(enblend_layers_c1(args, enbl, scenes, np, torch, cpu=cpu)), flush=True)
         return
     if args.workload == "enfuse-c0":
-        print(json.dumps(enfuse_c0(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
-        return
-    if args.workload == "enfuse-c2":
-        print(json.dumps(enfuse_c2(args, enbl, scenes, np, torch, cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(enfuse_c0(args, enbl, scenes, np, torch, cpu=cpu)), flush=True)
         return
     if args.workload == "batch-c3":
         print(json.dumps(batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)), flush=True)
@@ -569,189 +815,18 @@ def main():
                           "copies": args.steps + 1, "ms_per_copy": round(ms, 4),
                           "GB_per_s": round(2 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
         return
-    print("bench: generating the 2 x 4096x3072 RGBA u16 scene", file=sys.stderr, flush=True)
-    white, wa, black, ba, _ = scenes.enblend_pair(seed=1)
-    h, w, C = white.shape
-    levels = enbl.auto_levels(w, h)
-    mpix = 2 * 4096 * 3072 / 1e6
-
-    dev = torch.device("cuda:0")
-    d_white = torch.from_numpy(white).to(dev)
-    d_wa = torch.from_numpy(wa).to(dev)
-    d_black0 = torch.from_numpy(black).to(dev)
-    d_ba0 = torch.from_numpy(ba).to(dev)
-    d_mask = torch.empty((h, w), dtype=torch.uint8, device=dev)
-    d_black = d_black0.clone()
-    d_ba = d_ba0.clone()
-    torch.cuda.synchronize()
-
-    ctx = enbl.Context(0)
-    L = enbl.lib()
-
-    def vp(t):
-        import ctypes
-        return ctypes.c_void_p(t.data_ptr())
-
-    # The seam mask: nearest-feature transform of the two alphas (Euclidean,
-    # open boundaries), the primary seam generator of the fine, unoptimised
-    # mask.
-    print("bench: nearest-feature-transform seam mask", file=sys.stderr, flush=True)
-
-    def nft():
-        enbl.check(L.enbl_nft_mask(ctx.h, w, h, vp(d_wa), w, vp(d_ba0), w, 2, vp(d_mask), w))
-
-    nft()
-    ctx.sync()
-    nft_runs = 5
-    t0 = time.perf_counter()
-    for _ in range(nft_runs):
-        nft()
-    ctx.sync()
-    nft_ms = (time.perf_counter() - t0) * 1e3 / nft_runs
-    ctx.profile(True)
-    nft()
-    nft_rep = ctx.profile_report()
-    ctx.profile(False)
-    mask = d_mask.cpu().numpy()
-
-    def step(bk, bka):
-        enbl.check(L.enbl_enblend_pair(ctx.h, enbl.U16, C, w, h, levels, 0,
-                                       vp(d_white), w * C, vp(d_wa), w, vp(bk), w * C, vp(bka), w,
-                                       vp(d_mask), w, enbl.DITHER_DET, 1))
-
-    # Parity of this exact workload against the oracle (bit-exact).
-    print("bench: GPU step for parity", file=sys.stderr, flush=True)
-    step(d_black, d_ba)
-    ctx.sync()
-    gpu_out = d_black.cpu().numpy()
-    gpu_alpha = d_ba.cpu().numpy()
-
-    for _ in range(args.warmup):
-        step(d_black, d_ba)
-    ctx.sync()
-    torch.cuda.synchronize()
-    print(f"bench: timing {args.steps} steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(d_black, d_ba)
-    ctx.sync()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / args.steps
-
-    # The same step with the images in pinned host memory: uploads of white,
-    # black, their alphas and the seam mask, the step, downloads of black and
-    # its alpha (the rate a caller handing over host buffers sees; not the
-    # metric).
-    print("bench: PCIe-inclusive steps", file=sys.stderr, flush=True)
-    h_in = [torch.from_numpy(a).pin_memory() for a in (white, wa, black, ba, mask)]
-    h_out = [torch.empty(t.shape, dtype=t.dtype).pin_memory() for t in (d_black, d_ba)]
-    d_in = [d_white, d_wa, d_black, d_ba, d_mask]
-
-    def pcie_step():
-        for d, hh in zip(d_in, h_in):
-            d.copy_(hh, non_blocking=True)
-        torch.cuda.synchronize()
-        step(d_black, d_ba)
-        ctx.sync()
-        for hh, d in zip(h_out, (d_black, d_ba)):
-            hh.copy_(d, non_blocking=True)
-        torch.cuda.synchronize()
-
-    pcie_step()
-    pcie_runs = 5
-    t0 = time.perf_counter()
-    for _ in range(pcie_runs):
-        pcie_step()
-    pcie_ms = (time.perf_counter() - t0) * 1e3 / pcie_runs
-    pcie_bytes = sum(t.numel() * t.element_size() for t in h_in + h_out)
-
-    # Profiled pass: HIP events around every launch on the library's stream.
-    ctx.profile(True)
-    nprof = 3
-    for _ in range(nprof):
-        step(d_black, d_ba)
-    rep = ctx.profile_report()
-    ctx.profile(False)
-    total_ms = sum(v["ms"] for v in rep.values())
-    dom_name, dom = max(rep.items(), key=lambda kv: kv[1]["ms"])
-    avg_ms = dom["ms"] / dom["launches"]
-    avg_own = dom["own_bytes"] / dom["launches"]
-    avg_credit = dom["bytes"] / dom["launches"]
-    achieved = avg_own / (avg_ms * 1e-3) / 1e9
-    credited = avg_credit / (avg_ms * 1e-3) / 1e9
-    all_gbs = sum(v["own_bytes"] for v in rep.values()) / (total_ms * 1e-3) / 1e9
-    kernels = {k: {"ms_per_step": round(v["ms"] / nprof, 4), "launches_per_step": v["launches"] // nprof,
-                   "GB_per_s": round(v["own_bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
-                   "credited_GB_per_s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)}
-               for k, v in sorted(rep.items())}
-
-    traffic = measured_traffic(dom_name)
-    result = {
-        "metric": METRIC,
-        "value": round(mpix / (ms * 1e-3), 2),
-        "unit": "Mpix/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "enblend step, 2 x 4096x3072 RGBA uint16, 25% overlap, 7168x3072 ROI, NFT seam mask, "
-                               f"{levels} levels, Int32 fixed-point pyramids",
-                   "input_mpix_per_step": mpix, "levels": levels},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "achieved_basis": "the fused kernel's own algorithmic bytes per launch (each input read once, "
-                                       "each output written once) over its mean HIP-event launch duration",
-                     "algorithmic_bytes_per_launch": round(avg_own), "launch_ms": round(avg_ms, 5),
-                     "credited_bytes_per_launch": round(avg_credit),
-                     "credited_GB_per_s": round(credited, 1), "credited_frac": round(credited / HBM_PEAK_GBS, 4),
-                     "credited_basis": "bytes of the reference's unfused operations the kernel performs (reduce, "
-                                       "expand-subtract, level-0 conversion), each at reference storage types",
-                     "traffic_source": TRAFFIC_FILE if traffic else None,
-                     "dram_GB_per_s": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
-                     "all_kernels_GB_per_s": round(all_gbs, 1), "kernel_ms_per_step": round(total_ms / nprof, 4)},
-        "kernels": kernels,
-        "pcie_inclusive": {"ms_per_step": round(pcie_ms, 3), "mpix_per_s": round(mpix / (pcie_ms * 1e-3), 1),
-                           "host_bytes_per_step": pcie_bytes, "runs": pcie_runs,
-                           "note": "pinned host buffers in and out around the same step; not the metric"},
-        "nft": {"workload": f"nearest-feature-transform seam mask of the two alphas, {w}x{h}, Euclidean, "
-                            "float Felzenszwalb-Huttenlocher transforms", "ms": round(nft_ms, 3),
-                "mpix_per_s": round(w * h / 1e6 / (nft_ms * 1e-3), 1), "runs": nft_runs,
-                "kernels": _kernels(nft_rep)},
-    }
-
-    if not args.no_cpu:
-        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu_s, threads, (cpu_out, cpu_alpha) = cpu_baseline(white, wa, black, ba, mask, levels, args.cpu_repeats)
-        result["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
-                                  "sample": f"full workload, median of {args.cpu_repeats} runs "
-                                            f"({cpu_s:.2f} s each)"}
-        result["parity"] = bool(np.array_equal(cpu_out, gpu_out) and np.array_equal(cpu_alpha, gpu_alpha))
-        print("bench: CPU nearest-feature transform (oracle)", file=sys.stderr, flush=True)
-        from oracle_binding import load_oracle
-        o = load_oracle()
-        t0 = time.perf_counter()
-        cpu_mask, cpu_rc = o.nft_mask(wa, ba)
-        nft_cpu_s = time.perf_counter() - t0
-        result["nft"]["parity_vs_oracle"] = bool(cpu_rc == 0 and np.array_equal(cpu_mask, mask))
-        result["nft"]["cpu_baseline"] = {"ms": round(nft_cpu_s * 1e3, 1), "cores": o.lib.orc_num_threads(),
-                                         "kind": "port", "sample": "full mask, one run"}
-    ctx.close()
-    del d_white, d_wa, d_black0, d_ba0, d_mask, d_black, d_ba
-    if not args.no_layers:
-        result["enblend_layers"] = enblend_layers_c1(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
-    if not args.no_batch:
-        result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
-    if not args.no_enfuse:
-        result["enfuse_c0"] = enfuse_c0(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
-        result["enfuse_c2"] = enfuse_c2(args, enbl, scenes, np, torch, cpu=not args.no_cpu)
-    if not args.no_adjoint:
-        result["adjoint"] = adjoint_c4(args, enbl, scenes, torch)
+    result = enfuse_c2(args, enbl, scenes, np, torch, cpu=cpu)
+    if not args.headline_only:
+        if not args.no_c1:
+            result["enblend_c1"] = enblend_c1(args, enbl, scenes, np, torch, cpu=cpu)
+        if not args.no_layers:
+            result["enblend_layers"] = enblend_layers_c1(args, enbl, scenes, np, torch, cpu=cpu)
+        if not args.no_batch:
+            result["batch"] = batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)
+        if not args.no_enfuse:
+            result["enfuse_c0"] = enfuse_c0(args, enbl, scenes, np, torch, cpu=cpu)
+        if not args.no_adjoint:
+            result["adjoint"] = adjoint_c4(args, enbl, scenes, torch)
     print(json.dumps(result), flush=True)
 
 
