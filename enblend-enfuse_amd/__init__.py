@@ -57,6 +57,7 @@ _SIGS = {
                                     ctypes.POINTER(_c_int), ctypes.POINTER(_c_long), ctypes.POINTER(_c_long)]),
     "enbl_pyramid_copy_level": (_c_int, [_vp, _c_int, _vp, _c_int]),
     "enbl_auto_levels": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
+    "enbl_copy_to_pyramid": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp]),
     "enbl_gaussian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_gaussian_pyramid_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_laplacian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),

@@ -801,6 +801,12 @@ int enbl_gaussian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, cons
     return gaussian_levels(ctx, out, wrap, alpha, apitch);
 }
 
+int enbl_copy_to_pyramid(enbl_context* ctx, int itype, int frac, const void* img, long ipitch, enbl_pyramid* out)
+{
+    if (!ctx || !img || !out) return fail(ENBL_E_ARG, "enbl_copy_to_pyramid: null argument");
+    return build_level0(ctx, out, itype, frac, img, ipitch);
+}
+
 int enbl_laplacian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
                            const uint8_t* alpha, long apitch, enbl_pyramid* out)
 {

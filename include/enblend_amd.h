@@ -152,6 +152,16 @@ int enbl_pyramid_copy_level(enbl_pyramid* p, int level, void* buf, int to_pyrami
  * exact_levels > 0 caps it, exact_levels < 0 subtracts (--levels). */
 int enbl_auto_levels(int w, int h, int minimum_levels, int exact_levels);
 
+/* Replaces copyToPyramidImage<SrcImageType, PyramidImageType, IntBits,
+ * FracBits>(src, dest) -- src/fixmath.h:2018-2067, identity colour space --
+ * into level 0 of `out`: integer images v << frac_bits (ConvertScalarTo-
+ * PyramidFunctor, src/fixmath.h:116-206), float images into integer
+ * pyramids fromRealPromote(v * 2^frac_bits), float images into double
+ * pyramids 1 + log1p(v) for v >= 0, else 1 / (1 - v).  `img` is an
+ * interleaved device image with the pyramid's channel count. */
+int enbl_copy_to_pyramid(enbl_context* ctx, int itype, int frac_bits, const void* img, long ipitch,
+                         enbl_pyramid* out);
+
 /* Replaces gaussianPyramid<...>(numLevels, wraparound, src, alpha) --
  * src/pyramid.h:1467-1559 (alpha != NULL) and :1562-1636 (alpha == NULL) --
  * with copyToPyramidImage (src/fixmath.h:2018-2067) for level 0.  `img` is

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../enblend_amd.h"
+#include "exposure_weight.hpp"
 
 namespace enblend_amd {
 
@@ -223,6 +224,104 @@ std::vector<PyramidImageType*>* new_levels(const Pyramid& p, int levels)
         v->push_back(new PyramidImageType(w, h));
     }
     return v;
+}
+
+// NumericTraits<T>::fromRealPromote as the kernels apply it: round half away
+// from zero (v -/+ 0.5, truncate) and clamp; NaN gives INT32_MIN for Int32
+// and 0 for Int16, what x86-64 conversions give.
+template <class T> inline T from_real(double v);
+template <> inline int32_t from_real<int32_t>(double v)
+{
+    if (v != v) return INT32_MIN;
+    if (v < 0.0) return v < -2147483648.0 ? INT32_MIN : (int32_t)(v - 0.5);
+    return v > 2147483647.0 ? INT32_MAX : (int32_t)(v + 0.5);
+}
+template <> inline int16_t from_real<int16_t>(double v)
+{
+    if (v != v) return 0;
+    if (v < 0.0) return v < -32768.0 ? (int16_t)-32768 : (int16_t)(int32_t)(v - 0.5);
+    return v > 32767.0 ? (int16_t)32767 : (int16_t)(int32_t)(v + 0.5);
+}
+template <> inline double from_real<double>(double v) { return v; }
+
+// enfuse's weighting options: the globals of src/enfuse.cc:118-146 (same
+// names and defaults) and the exposure weight function made from them.
+// enblend::enfuseMask reads them, as the reference's enfuseMask reads the
+// globals; a caller sets them where it parses its options.
+struct EnfuseOptions {
+    double WExposure = 1.0;
+    double ExposureOptimum = 0.5;
+    double ExposureWidth = 0.2;
+    double WContrast = 0.0;
+    double WSaturation = 0.0;
+    double WEntropy = 0.0;
+    int ContrastWindowSize = 5;
+    int EntropyWindowSize = 3;
+    // The exposure weight: a user function (plugin object) when not null,
+    // else the built-in ENBL_EXPOSURE_* of ExposureWeightBuiltin.  A user
+    // function is tabulated per gray code for integer images; float images
+    // need the built-ins (the reference calls a plugin per float pixel).
+    ExposureWeight* ExposureWeightFunction = nullptr;
+    int ExposureWeightBuiltin = ENBL_EXPOSURE_GAUSSIAN;
+    // --contrast-edge-scale=EDGE:LCE:FACTOR and --contrast-min-curvature
+    // (the latter as instantiated for the image's scalar type).
+    double ContrastEdgeScale = 0.0, ContrastLceScale = 0.0, ContrastLceFactor = 0.0, MinCurvature = 0.0;
+    // --exposure-cutoff / --entropy-cutoff when effective, instantiated for
+    // the image's scalar type.
+    bool ExposureCutoff = false, EntropyCutoff = false;
+    double ExposureLowerCutoff = 0.0, ExposureUpperCutoff = 0.0, EntropyLowerCutoff = 0.0, EntropyUpperCutoff = 0.0;
+    // --gray-projector (ENBL_GRAY_*) and the channel mixer's weights.
+    int GrayscaleProjector = ENBL_GRAY_AVERAGE;
+    double MixerRed = 0.0, MixerGreen = 0.0, MixerBlue = 0.0;
+};
+
+inline EnfuseOptions& enfuse_options()
+{
+    static EnfuseOptions o;
+    return o;
+}
+
+// The C-ABI parameters of the options; `table` receives the user function's
+// table when there is one (checked like check_weight_function first).
+inline enbl_enfuse_params enfuse_params_of(const EnfuseOptions& o, int itype, std::vector<double>& table)
+{
+    enbl_enfuse_params p{};
+    p.w_exposure = o.WExposure;
+    p.w_saturation = o.WSaturation;
+    p.w_contrast = o.WContrast;
+    p.w_entropy = o.WEntropy;
+    p.exposure_optimum = o.ExposureOptimum;
+    p.exposure_width = o.ExposureWidth;
+    p.exposure_function = o.ExposureWeightBuiltin;
+    p.contrast_window = o.ContrastWindowSize;
+    p.entropy_window = o.EntropyWindowSize;
+    p.edge_scale = o.ContrastEdgeScale;
+    p.lce_scale = o.ContrastLceScale;
+    p.lce_factor = o.ContrastLceFactor;
+    p.min_curvature = o.MinCurvature;
+    p.exposure_cutoff = o.ExposureCutoff ? 1 : 0;
+    p.entropy_cutoff = o.EntropyCutoff ? 1 : 0;
+    p.exposure_lower_cutoff = o.ExposureLowerCutoff;
+    p.exposure_upper_cutoff = o.ExposureUpperCutoff;
+    p.entropy_lower_cutoff = o.EntropyLowerCutoff;
+    p.entropy_upper_cutoff = o.EntropyUpperCutoff;
+    p.gray_projector = o.GrayscaleProjector;
+    p.mixer_red = o.MixerRed;
+    p.mixer_green = o.MixerGreen;
+    p.mixer_blue = o.MixerBlue;
+    if (o.WExposure > 0.0 && o.ExposureWeightFunction) {
+        if (itype == ENBL_F32)
+            throw error(ENBL_E_UNSUPPORTED, "enfuseMask: a user exposure weight function needs an integer image");
+        switch (check_weight_function(o.ExposureWeightFunction)) {
+        case WEIGHT_NEGATIVE: throw error(ENBL_E_PRECONDITION, "unusable exposure weight function: at least one weight is negative");
+        case WEIGHT_NON_UNIT: throw error(ENBL_E_PRECONDITION, "unusable exposure weight function: at least one weight is larger than one");
+        case WEIGHT_DEGENERATE: throw error(ENBL_E_PRECONDITION, "unusable exposure weight function: too many zeros");
+        case WEIGHT_OK: break;
+        }
+        table = exposure_weight_table(o.ExposureWeightFunction, itype == ENBL_U8 ? 256 : 65536);
+        p.exposure_table = table.data();
+    }
+    return p;
 }
 
 }  // namespace enblend_amd
@@ -426,6 +525,195 @@ void expand(bool add, bool wraparound,
     scatter(d0, dest_upperleft, dw, dh, da, true);
 }
 
+/** Replaces reduce without alpha (src/pyramid.h:601-848): clamp-to-edge
+ * extension and the fixed divisor 256. */
+template <typename SKIPSMImagePixelType,
+          typename SrcImageIterator, typename SrcAccessor,
+          typename DestImageIterator, typename DestAccessor>
+void reduce(bool wraparound,
+            SrcImageIterator src_upperleft, SrcImageIterator src_lowerright, SrcAccessor sa,
+            DestImageIterator dest_upperleft, DestImageIterator dest_lowerright, DestAccessor da)
+{
+    using namespace enblend_amd;
+    typedef typename SrcAccessor::value_type SV;
+    typedef typename pixel<SV>::component Comp;
+    const int C = pixel<SV>::channels;
+    const int sw = src_lowerright.x - src_upperleft.x, sh = src_lowerright.y - src_upperleft.y;
+    const int dw = dest_lowerright.x - dest_upperleft.x, dh = dest_lowerright.y - dest_upperleft.y;
+    const long sp = (sw + 1) & ~1, dp = (dw + 1) & ~1;
+    std::vector<Comp> src((size_t)C * sp * sh, Comp(0));
+    std::vector<Comp> g = gather<Comp>(src_upperleft, sw, sh, sa, true);
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < sh; ++y)
+            for (int x = 0; x < sw; ++x) src[((size_t)c * sh + y) * sp + x] = g[((size_t)c * sh + y) * sw + x];
+    DeviceBuffer ds(src.size() * sizeof(Comp)), dd((size_t)C * dp * dh * sizeof(Comp));
+    ds.upload(src.data(), src.size() * sizeof(Comp));
+    check(enbl_reduce(context(), pyramid_code<Comp>(), C, wraparound, ds.get(), sw, sh, sp, sp * sh, nullptr, 0,
+                      dd.get(), dw, dh, dp, dp * dh, nullptr, 0));
+    std::vector<Comp> out((size_t)C * dp * dh), packed((size_t)C * dw * dh);
+    dd.download(out.data(), out.size() * sizeof(Comp));
+    for (int c = 0; c < C; ++c)
+        for (int y = 0; y < dh; ++y)
+            for (int x = 0; x < dw; ++x) packed[((size_t)c * dh + y) * dw + x] = out[((size_t)c * dh + y) * dp + x];
+    scatter(packed, dest_upperleft, dw, dh, da, true);
+}
+
+/** Argument-object form of reduce with alpha (src/pyramid.h:577-594). */
+template <typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType,
+          typename SrcTriple, typename AlphaPair, typename DestTriple, typename DestAlphaTriple>
+inline void reduce(bool wraparound, SrcTriple src, AlphaPair mask, DestTriple dest, DestAlphaTriple destMask)
+{
+    reduce<SKIPSMImagePixelType, SKIPSMAlphaPixelType>(wraparound, src.first, src.second, src.third,
+                                                       mask.first, mask.second, dest.first, dest.second, dest.third,
+                                                       destMask.first, destMask.second, destMask.third);
+}
+
+/** Argument-object form of reduce without alpha (src/pyramid.h:852-863). */
+template <typename SKIPSMImagePixelType, typename SrcTriple, typename DestTriple>
+inline void reduce(bool wraparound, SrcTriple src, DestTriple dest)
+{
+    reduce<SKIPSMImagePixelType>(wraparound, src.first, src.second, src.third, dest.first, dest.second, dest.third);
+}
+
+// Combine functors of expand's argument-object form: add = saturating sum,
+// subtract = wrapping difference (src/pyramid.h:1425-1457).  The device
+// kernels apply the same rules; the functor only selects the mode.
+template <typename T1, typename T2, typename T3>
+struct FromPromotePlusFunctorWrapper {};
+
+/** Argument-object form of expand (src/pyramid.h:1439-1457). */
+template <typename SKIPSMImagePixelType, typename SrcTriple, typename DestTriple>
+inline void expand(bool add, bool wraparound, SrcTriple src, DestTriple dest)
+{
+    expand<SKIPSMImagePixelType>(add, wraparound, src.first, src.second, src.third,
+                                 dest.first, dest.second, dest.third, FromPromotePlusFunctorWrapper<int, int, int>());
+}
+
+/** Argument-object form of gaussianPyramid with alpha (src/pyramid.h:1544-1559). */
+template <typename SrcImageType, typename AlphaImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType,
+          typename SrcTriple, typename AlphaPair>
+inline std::vector<PyramidImageType*>* gaussianPyramid(unsigned int numLevels, bool wraparound, SrcTriple src,
+                                                       AlphaPair alpha)
+{
+    return gaussianPyramid<SrcImageType, AlphaImageType, PyramidImageType, PyramidIntegerBits, PyramidFractionBits,
+                           SKIPSMImagePixelType, SKIPSMAlphaPixelType>(numLevels, wraparound, src.first, src.second,
+                                                                       src.third, alpha.first, alpha.second);
+}
+
+/** Argument-object form of gaussianPyramid without alpha (src/pyramid.h:1622-1636). */
+template <typename SrcImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType, typename SrcTriple>
+inline std::vector<PyramidImageType*>* gaussianPyramid(unsigned int numLevels, bool wraparound, SrcTriple src)
+{
+    return gaussianPyramid<SrcImageType, PyramidImageType, PyramidIntegerBits, PyramidFractionBits,
+                           SKIPSMImagePixelType>(numLevels, wraparound, src.first, src.second, src.third);
+}
+
+/** Argument-object form of laplacianPyramid (src/pyramid.h:1732-1747). */
+template <typename SrcImageType, typename AlphaImageType, typename PyramidImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SKIPSMImagePixelType, typename SKIPSMAlphaPixelType,
+          typename SrcTriple, typename AlphaPair>
+inline std::vector<PyramidImageType*>* laplacianPyramid(const char* exportName, unsigned int numLevels,
+                                                        bool wraparound, SrcTriple src, AlphaPair alpha)
+{
+    return laplacianPyramid<SrcImageType, AlphaImageType, PyramidImageType, PyramidIntegerBits, PyramidFractionBits,
+                            SKIPSMImagePixelType, SKIPSMAlphaPixelType>(exportName, numLevels, wraparound,
+                                                                        src.first, src.second, src.third,
+                                                                        alpha.first, alpha.second);
+}
+
+/** Replaces copyToPyramidImage (src/fixmath.h:2018-2067, :2119-2131),
+ * identity colour space: integer images v << PyramidFractionBits, float
+ * masks fromRealPromote(v * 2^PyramidFractionBits) into integer pyramids,
+ * float images 1 + log1p(v) (v >= 0) or 1 / (1 - v) into double pyramids. */
+template <typename SrcImageType, typename PyramidImageType, int PyramidIntegerBits, int PyramidFractionBits>
+void copyToPyramidImage(typename SrcImageType::const_traverser src_upperleft,
+                        typename SrcImageType::const_traverser src_lowerright,
+                        typename SrcImageType::ConstAccessor sa,
+                        typename PyramidImageType::traverser dest_upperleft,
+                        typename PyramidImageType::Accessor da)
+{
+    using namespace enblend_amd;
+    typedef typename SrcImageType::value_type SV;
+    typedef typename pixel<SV>::component SComp;
+    typedef typename pixel<typename PyramidImageType::value_type>::component PComp;
+    const int C = pixel<SV>::channels;
+    const int w = src_lowerright.x - src_upperleft.x, h = src_lowerright.y - src_upperleft.y;
+    std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
+    DeviceBuffer dimg(img.size() * sizeof(SComp));
+    dimg.upload(img.data(), img.size() * sizeof(SComp));
+    Pyramid p(pyramid_code<PComp>(), C, w, h, 1);
+    check(enbl_copy_to_pyramid(context(), image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C, p.get()));
+    scatter(p.read<PComp>(0, C), dest_upperleft, w, h, da, true);
+}
+
+/** Argument-object form of copyToPyramidImage (src/fixmath.h:2135-2145). */
+template <typename SrcImageType, typename PyramidImageType, int PyramidIntegerBits, int PyramidFractionBits,
+          typename SrcTriple, typename DestPair>
+inline void copyToPyramidImage(SrcTriple src, DestPair dest)
+{
+    copyToPyramidImage<SrcImageType, PyramidImageType, PyramidIntegerBits, PyramidFractionBits>(
+        src.first, src.second, src.third, dest.first, dest.second);
+}
+
+/** enfuseMain's multiply of a Laplacian level by its mask level
+ * (src/enfuse.h:601-620): fromRealPromote(pixel * (mask / divisor)), the
+ * coefficient and the product in double, per channel. */
+template <typename MaskPixelType>
+class ImageMaskMultiplyFunctor {
+public:
+    ImageMaskMultiplyFunctor(MaskPixelType d) : divisor(static_cast<double>(d)) {}
+
+    template <typename ImagePixelType>
+    ImagePixelType operator()(const ImagePixelType& iP, const MaskPixelType& maskP) const
+    {
+        typedef enblend_amd::pixel<ImagePixelType> PX;
+        typedef typename PX::component Comp;
+        const double coeff = static_cast<double>(maskP) / divisor;
+        Comp out[3];
+        for (int c = 0; c < PX::channels; ++c)
+            out[c] = enblend_amd::from_real<Comp>(static_cast<double>(PX::get(iP, c)) * coeff);
+        return PX::make(out);
+    }
+
+protected:
+    const double divisor;
+};
+
+/** Replaces enfuseMask<ImageType, AlphaType, MaskType>(src, mask, result)
+ * (src/enfuse.h:962-1212): the float weight of every pixel under alpha
+ * (exposure, contrast, saturation, entropy, added in that order) from the
+ * options in enblend_amd::enfuse_options(); pixels outside alpha keep their
+ * value.  A user exposure function (ExposureWeightFunction) is checked like
+ * check_weight_function and tabulated per gray code. */
+template <typename ImageType, typename AlphaType, typename MaskType,
+          typename SrcTriple, typename AlphaPair, typename ResultPair>
+void enfuseMask(SrcTriple src, AlphaPair mask, ResultPair result)
+{
+    using namespace enblend_amd;
+    typedef typename ImageType::value_type V;
+    typedef typename pixel<V>::component Comp;
+    static_assert(pixel<V>::channels == 3, "enfuseMask: RGB images");
+    const int w = src.second.x - src.first.x, h = src.second.y - src.first.y;
+    const int itype = image_code<Comp>();
+    std::vector<double> table;
+    const enbl_enfuse_params prm = enfuse_params_of(enfuse_options(), itype, table);
+    std::vector<Comp> img = gather<Comp>(src.first, w, h, src.third, false);
+    std::vector<uint8_t> al = gather_mask(mask.first, w, h, mask.second);
+    DeviceBuffer dimg(img.size() * sizeof(Comp)), dal(al.size()), dout((size_t)w * h * sizeof(float));
+    dimg.upload(img.data(), img.size() * sizeof(Comp));
+    dal.upload(al.data(), al.size());
+    check(enbl_enfuse_mask(context(), itype, w, h, dimg.get(), (long)w * 3, (const uint8_t*)dal.get(), w, &prm,
+                           (float*)dout.get(), w));
+    std::vector<float> out((size_t)w * h);
+    dout.download(out.data(), out.size() * sizeof(float));
+    scatter(out, result.first, w, h, result.second, false, al.data());
+}
+
 /** Replaces copyFromPyramidImageIf (src/fixmath.h:2157-2294), identity
  * colour space; integer pyramids dither deterministically (seed 0). */
 template <typename PyramidImageType, typename MaskImageType, typename DestImageType,
@@ -454,6 +742,16 @@ void copyFromPyramidImageIf(typename PyramidImageType::const_traverser src_upper
     std::vector<DComp> out((size_t)w * h * C);
     dimg.download(out.data(), out.size() * sizeof(DComp));
     scatter(out, dest_upperleft, w, h, da, false, m.data());
+}
+
+/** Argument-object form of copyFromPyramidImageIf (src/fixmath.h:2282-2294). */
+template <typename PyramidImageType, typename MaskImageType, typename DestImageType,
+          int PyramidIntegerBits, int PyramidFractionBits,
+          typename SrcTriple, typename MaskPair, typename DestPair>
+inline void copyFromPyramidImageIf(SrcTriple src, MaskPair mask, DestPair dest)
+{
+    copyFromPyramidImageIf<PyramidImageType, MaskImageType, DestImageType, PyramidIntegerBits, PyramidFractionBits>(
+        src.first, src.second, src.third, mask.first, mask.second, dest.first, dest.second);
 }
 
 }  // namespace enblend

@@ -54,4 +54,61 @@ private:
     std::vector<T> d_;
 };
 
+// Argument objects of the call sites (vigra::triple / vigra::pair and the
+// srcImageRange ... destImage factories), found by argument-dependent lookup
+// as VIGRA's are.
+template <class A, class B, class C>
+struct triple {
+    A first;
+    B second;
+    C third;
+};
+
+template <class A, class B>
+struct pair {
+    A first;
+    B second;
+};
+
+template <class T>
+triple<Traverser<T>, Traverser<T>, PixelAccessor<T>> srcImageRange(const Image<T>& im)
+{
+    return {im.upperLeft(), im.lowerRight(), im.accessor()};
+}
+
+template <class T>
+triple<Traverser<T>, Traverser<T>, PixelAccessor<T>> destImageRange(Image<T>& im)
+{
+    return {im.upperLeft(), im.lowerRight(), im.accessor()};
+}
+
+template <class T>
+pair<Traverser<T>, PixelAccessor<T>> srcImage(const Image<T>& im) { return {im.upperLeft(), im.accessor()}; }
+
+template <class T>
+pair<Traverser<T>, PixelAccessor<T>> maskImage(const Image<T>& im) { return {im.upperLeft(), im.accessor()}; }
+
+template <class T>
+pair<Traverser<T>, PixelAccessor<T>> destImage(Image<T>& im) { return {im.upperLeft(), im.accessor()}; }
+
+// The VIGRA algorithms the call sites use around the pyramid templates,
+// restated as plain loops over whole images.
+template <class T, class M>
+void initImageIf(triple<Traverser<T>, Traverser<T>, PixelAccessor<T>> dest, pair<Traverser<M>, PixelAccessor<M>> mask,
+                 T v)
+{
+    for (int y = dest.first.y; y < dest.second.y; ++y)
+        for (int x = dest.first.x; x < dest.second.x; ++x) {
+            Traverser<M> m = mask.first;
+            m.x += x - dest.first.x;
+            m.y += y - dest.first.y;
+            if (mask.second(m)) {
+                Traverser<T> d = dest.first;
+                d.x = x;
+                d.y = y;
+                dest.third.set(v, d);
+            }
+        }
+}
+
 }  // namespace mini
