@@ -23,6 +23,7 @@ ENBL_E_HIP = -6
 I16, I32, F64 = 0, 1, 2
 U8, U16, F32, U32 = 10, 11, 12, 13
 DITHER_NONE, DITHER_DET = 0, 1
+CS_IDENTITY, CS_CIELAB, CS_CIELUV, CS_CIECAM = 0, 1, 2, 3
 KERNELS_AUTO, KERNELS_DIRECT, KERNELS_STREAMING = 0, 1, 2
 
 _c_int, _c_long, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
@@ -58,6 +59,10 @@ _SIGS = {
     "enbl_pyramid_copy_level": (_c_int, [_vp, _c_int, _vp, _c_int]),
     "enbl_auto_levels": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
     "enbl_copy_to_pyramid": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp]),
+    "enbl_set_blend_colorspace": (_c_int, [_vp, _c_int]),
+    "enbl_rgb_to_blend_space": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long]),
+    "enbl_blend_space_to_rgb_if": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp,
+                                            _c_long]),
     "enbl_gaussian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_gaussian_pyramid_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),
     "enbl_laplacian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),

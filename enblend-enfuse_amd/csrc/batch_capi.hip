@@ -582,7 +582,9 @@ int enbl_enfuse_batch(enbl_context* ctx, int itype, int nstacks, const enbl_stac
     if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_enfuse_batch: null argument");
     int rc = check_stacks(itype, nstacks, stacks, p);
     if (rc || nstacks == 0) return rc;
-    if (!batchable(itype, p, wrap)) {
+    // A blend colour space converts every image around the pyramids; those
+    // stacks run through enbl_enfuse one by one.
+    if (!batchable(itype, p, wrap) || ctx->colorspace != ENBL_COLORSPACE_IDENTITY) {
         for (int s = 0; s < nstacks; ++s) {
             const enbl_stack& S = stacks[s];
             rc = enbl_enfuse(ctx, itype, S.w, S.h, S.K, S.imgs, S.ipitches, S.alphas, S.apitches, p, S.levels, wrap,
@@ -627,7 +629,7 @@ int enbl_batch_autotune(enbl_context* ctx, int itype, int nstacks, const enbl_st
     if (!ctx || !p || !best) return fail(ENBL_E_ARG, "enbl_batch_autotune: null argument");
     int rc = check_stacks(itype, nstacks, stacks, p);
     if (rc) return rc;
-    if (nstacks == 0 || !batchable(itype, p, 0))
+    if (nstacks == 0 || !batchable(itype, p, 0) || ctx->colorspace != ENBL_COLORSPACE_IDENTITY)
         return fail(ENBL_E_UNSUPPORTED, "enbl_batch_autotune: this batch does not take the batched schedule");
     if (repeats < 1) repeats = 1;
     BatchState* b;

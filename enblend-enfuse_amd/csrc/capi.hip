@@ -580,6 +580,7 @@ int enbl_context_create(int device, enbl_context** out)
     for (int i = 0; i < kScratchSlots; ++i) { c->scratch_buf[i] = nullptr; c->scratch_bytes[i] = 0; }
     c->batch = nullptr;
     for (int i = 0; i < 3; ++i) { c->batch_rows[i] = 0; c->batch_bpc[i] = 0; }
+    c->colorspace = ENBL_COLORSPACE_IDENTITY;
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) { delete c; return rc; }
     c->stream = c->own_stream;
@@ -862,12 +863,51 @@ int enbl_copy_from_pyramid_if(enbl_context* ctx, const enbl_pyramid* p, int leve
     return launch_status("from_pyramid");
 }
 
+namespace {
+int enblend_pair_rgb(enbl_context* ctx, int itype, int C, int w, int h, int levels, int wrap,
+                     const void* white, long wpitch, const uint8_t* walpha, long wapitch,
+                     void* black, long bpitch, uint8_t* balpha, long bapitch,
+                     const uint8_t* mask, long mpitch, int dither, uint64_t seed);
+}  // namespace
+
+// One blend step; in a CIELAB / CIELUV blend space both RGB images are
+// converted into scaled blend-space images first, the step runs on those,
+// and its result goes back to RGB under the union alpha, as the reference's
+// converters around copyToPyramidImage / copyFromPyramidImageIf do
+// (src/fixmath.h:2018-2294).
 int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int levels, int wrap,
                       const void* white, long wpitch, const uint8_t* walpha, long wapitch,
                       void* black, long bpitch, uint8_t* balpha, long bapitch,
                       const uint8_t* mask, long mpitch, int dither, uint64_t seed)
 {
     if (!ctx || !white || !walpha || !black || !balpha || !mask) return fail(ENBL_E_ARG, "enbl_enblend_pair: null argument");
+    const int cs = C == 3 ? ctx->colorspace : ENBL_COLORSPACE_IDENTITY;
+    if (cs == ENBL_COLORSPACE_IDENTITY)
+        return enblend_pair_rgb(ctx, itype, C, w, h, levels, wrap, white, wpitch, walpha, wapitch, black, bpitch,
+                                balpha, bapitch, mask, mpitch, dither, seed);
+    int rc = check_blend_space(cs, itype, C);
+    if (rc) return rc;
+    if (w < 1 || h < 1) return fail(ENBL_E_SIZE, "enbl_enblend_pair: empty image");
+    const size_t es = itype == ENBL_U8 ? 1 : 2, plane = (size_t)w * h * 3 * es;
+    void* buf;
+    rc = scratch(ctx, 32, 2 * plane, &buf);
+    if (rc) return rc;
+    char* wl = (char*)buf;
+    char* bl = wl + plane;
+    rc = rgb_to_blend_space(ctx, cs, itype, w, h, white, wpitch, wl, 3L * w);
+    if (!rc) rc = rgb_to_blend_space(ctx, cs, itype, w, h, black, bpitch, bl, 3L * w);
+    if (!rc) rc = enblend_pair_rgb(ctx, itype, C, w, h, levels, wrap, wl, 3L * w, walpha, wapitch, bl, 3L * w,
+                                   balpha, bapitch, mask, mpitch, dither, seed);
+    if (!rc) rc = blend_space_to_rgb_if(ctx, cs, itype, w, h, bl, 3L * w, balpha, bapitch, black, bpitch);
+    return rc;
+}
+
+namespace {
+int enblend_pair_rgb(enbl_context* ctx, int itype, int C, int w, int h, int levels, int wrap,
+                     const void* white, long wpitch, const uint8_t* walpha, long wapitch,
+                     void* black, long bpitch, uint8_t* balpha, long bapitch,
+                     const uint8_t* mask, long mpitch, int dither, uint64_t seed)
+{
     if (itype != ENBL_U8 && itype != ENBL_U16 && itype != ENBL_F32) return fail(ENBL_E_UNSUPPORTED, "image type %d", itype);
     const int pt = image_ptype_for(itype);
     const int mt = mask_ptype_for(itype);
@@ -1046,5 +1086,6 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     }
     return ENBL_OK;
 }
+}  // namespace
 
 }  // extern "C"

@@ -77,7 +77,7 @@ std::vector<Level> plan_levels(int ptype, int channels, int w, int h, int levels
 struct enbl_context;
 struct enbl_pyramid;
 
-constexpr int kScratchSlots = 32;
+constexpr int kScratchSlots = 40;
 
 namespace enbl {
 struct BatchState;   // batched enfuse: arena, job tables, captured graph (batch_capi.hip)
@@ -130,6 +130,13 @@ int scratch(enbl_context* ctx, int slot, size_t bytes, void** out);
 // (and fused) kernels under the current kernel policy.
 bool streaming_level(int cw, int ch);
 int alpha_pingpong(enbl_context* ctx, const enbl_pyramid* p, uint8_t** a, uint8_t** b, long* pitch);
+// Blend colour spaces (colorspace.hip): validity for an image type and
+// channel count, and the forward / inverse conversions of RGB images.
+int check_blend_space(int space, int itype, int channels);
+int rgb_to_blend_space(enbl_context* ctx, int space, int itype, int w, int h, const void* img, long ip, void* out,
+                       long op);
+int blend_space_to_rgb_if(enbl_context* ctx, int space, int itype, int w, int h, const void* src, long sp,
+                          const uint8_t* mask, long mp, void* img, long ip);
 int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
                           const uint8_t* alpha, long apitch, enbl_pyramid* out);
 
@@ -152,6 +159,7 @@ struct enbl_context {
     enbl::BatchState* batch;     // batched enfuse state, created on first use
     int batch_rows[3];           // launch shape of the batched passes (0: default)
     int batch_bpc[3];
+    int colorspace;              // ENBL_COLORSPACE_* of the pipelines (enbl_set_blend_colorspace)
 };
 
 struct enbl_pyramid {

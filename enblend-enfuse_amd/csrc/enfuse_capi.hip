@@ -477,6 +477,9 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     int rc = check_params(itype, w, h, p);
     if (rc) return rc;
     if (levels < 1 || levels > 29) return fail(ENBL_E_PRECONDITION, "filterHalfWidth: levels outside of permissible range");
+    const int cs = ctx->colorspace;
+    rc = check_blend_space(cs, itype, 3);
+    if (rc) return rc;
     const WeightArgs a = weight_args(p);
     WeightScratch ws;
     rc = prepare_weights(ctx, itype, w, h, K, p, a, &ws);
@@ -484,6 +487,26 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     const hipStream_t s = ctx->stream;
     const long pitch = ws.pitch;
     const size_t plane = (size_t)pitch * h;
+    // In a CIELAB / CIELUV blend space the weights come from the RGB images
+    // and the pyramids from the images converted into scaled blend-space
+    // images (ConvertVectorTo{Lab,Luv}PyramidFunctor); the result goes back
+    // to RGB under the union alpha.
+    std::vector<const void*> pimgs(imgs, imgs + K);
+    std::vector<long> pips(ips, ips + K);
+    void* csout = nullptr;
+    if (cs != ENBL_COLORSPACE_IDENTITY) {
+        const size_t cplane = (size_t)w * h * 3 * (itype == ENBL_U8 ? 1 : 2);
+        void* cbuf;
+        rc = scratch(ctx, 33, cplane * (size_t)K, &cbuf);
+        if (!rc) rc = scratch(ctx, 34, cplane, &csout);
+        if (rc) return rc;
+        for (int m = 0; m < K; ++m) {
+            pimgs[(size_t)m] = (char*)cbuf + cplane * m;
+            pips[(size_t)m] = 3L * w;
+            rc = rgb_to_blend_space(ctx, cs, itype, w, h, imgs[m], ips[m], (void*)pimgs[(size_t)m], 3L * w);
+            if (rc) return rc;
+        }
+    }
     void *norm, *ua, *masks;
     rc = scratch(ctx, 5, plane * 4, &norm);
     if (!rc) rc = scratch(ctx, 6, plane, &ua);
@@ -538,7 +561,7 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     for (int m = 0; m < K; ++m) {
         enbl_pyramid* lp = lps[(size_t)m];
         enbl_pyramid* mgp = mgps[(size_t)m];
-        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, imgs[m], ips[m], alphas[m], aps[m], lp);
+        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, pimgs[(size_t)m], pips[(size_t)m], alphas[m], aps[m], lp);
         if (rc) return rc;
         {
             const Level& L0 = mgp->lv[0];
@@ -580,7 +603,13 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     }
     rc = enbl_collapse_pyramid(ctx, wrap, res);
     if (rc) return rc;
-    rc = enbl_copy_from_pyramid_if(ctx, res, 0, ifrac, (const uint8_t*)ua, pitch, out, itype, op, dither, seed);
+    if (cs != ENBL_COLORSPACE_IDENTITY) {
+        rc = enbl_copy_from_pyramid_if(ctx, res, 0, ifrac, (const uint8_t*)ua, pitch, csout, itype, 3L * w, dither,
+                                       seed);
+        if (!rc) rc = blend_space_to_rgb_if(ctx, cs, itype, w, h, csout, 3L * w, (const uint8_t*)ua, pitch, out, op);
+    } else {
+        rc = enbl_copy_from_pyramid_if(ctx, res, 0, ifrac, (const uint8_t*)ua, pitch, out, itype, op, dither, seed);
+    }
     if (rc) return rc;
     return hip_check(hipMemcpy2DAsync(oa, (size_t)oap, ua, (size_t)pitch, (size_t)w, (size_t)h, hipMemcpyDeviceToDevice, s),
                      "union alpha");

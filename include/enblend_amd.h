@@ -411,6 +411,45 @@ int enbl_nft_mask_periodic(enbl_context* ctx, int w, int h, const uint8_t* white
                            const uint8_t* black_alpha, long bapitch, int norm, int boundary, uint8_t* mask,
                            long mpitch);
 
+/* ---- blend colour spaces ------------------------------------------------- */
+
+/* --blend-colorspace (BlendColorspace, src/enfuse.cc:110, :1628-1650,
+ * src/enblend.cc:124): the space the pyramids of RGB images are built,
+ * blended and collapsed in.  enfuse and enblend choose CIELUV by themselves
+ * for integer images that carry an ICC profile (src/enfuse.cc:2380-2383). */
+#define ENBL_COLORSPACE_IDENTITY 0
+#define ENBL_COLORSPACE_CIELAB 1
+#define ENBL_COLORSPACE_CIELUV 2
+#define ENBL_COLORSPACE_CIECAM 3   /* CIECAM02: ENBL_E_UNSUPPORTED */
+
+/* Sets the context's blend colour space for enbl_enblend_pair,
+ * enbl_enblend_step, enbl_enblend_layers, enbl_enfuse and
+ * enbl_enfuse_batch (default identity).  With CIELAB or CIELUV those convert
+ * 8- and 16-bit RGB images as the reference's
+ * ConvertVectorTo{Lab,Luv}PyramidFunctor and Convert{Lab,Luv}PyramidTo-
+ * VectorFunctor do (src/fixmath.h:429-1330; sRGB input profile, the
+ * reference's assumption for images without an ICC profile); enfuse's
+ * weights stay computed on the RGB images.  Float RGB images with a
+ * non-identity space give ENBL_E_UNSUPPORTED; one-channel images are not
+ * converted (the reference's scalar path ignores the space). */
+int enbl_set_blend_colorspace(enbl_context* ctx, int space);
+
+/* The forward conversion alone: interleaved 8- or 16-bit RGB `img` into the
+ * scaled blend-space image `out` of the same type (L' = round(s L / 100),
+ * a' = round(s (200 + a) / 400), s = 2^(IntegerBits - 2) = 128 for 8-bit,
+ * 32768 for 16-bit images; PyramidScale, src/fixmath.h:430-548), the values
+ * the pyramid's level 0 holds before the fraction shift. */
+int enbl_rgb_to_blend_space(enbl_context* ctx, int space, int itype, int w, int h, const void* img, long ipitch,
+                            void* out, long opitch);
+
+/* The inverse conversion alone: a scaled blend-space image (as the integer
+ * output conversion of the collapsed pyramid leaves it) into RGB `img`,
+ * fromRealPromote(max * rgb), where mask != 0; other pixels are kept.
+ * Out-of-gamut colours are clamped (the reference polishes them with a
+ * numeric minimiser, src/fixmath.h:835-1045). */
+int enbl_blend_space_to_rgb_if(enbl_context* ctx, int space, int itype, int w, int h, const void* src,
+                               long spitch, const uint8_t* mask, long mpitch, void* img, long ipitch);
+
 /* ---- pipelines ---------------------------------------------------------- */
 
 /* One enblend blend step on the ROI canvas, replacing src/enblend.h:378-510

@@ -28,6 +28,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../enblend_amd.h"
@@ -76,10 +77,40 @@ public:
     void* get() const { return p_; }
     void upload(const void* src, size_t bytes) { check(enbl_copy(context(), p_, src, bytes)); }
     void download(void* dst, size_t bytes) const { check(enbl_copy(context(), dst, p_, bytes)); }
+    void swap(DeviceBuffer& o) { std::swap(p_, o.p_); }
 
 private:
     void* p_ = nullptr;
 };
+
+// The blend colour space (ENBL_COLORSPACE_*), the reference's global
+// BlendColorspace (src/enblend.cc:124, src/enfuse.cc:110): a caller mirrors
+// it where it parses --blend-colorspace.  With CIELAB or CIELUV set, RGB
+// images enter the pyramids converted (copyToPyramidImage, and the
+// Gaussian and Laplacian pyramids, which start with it) and leave them
+// converted back (copyFromPyramidImageIf), as src/fixmath.h:2060-2116 and
+// :2207-2254 dispatch; one-channel images are never converted.  CIECAM02
+// and float RGB images with a space set throw ENBL_E_UNSUPPORTED.
+inline int& blend_colorspace()
+{
+    static int s = ENBL_COLORSPACE_IDENTITY;
+    return s;
+}
+
+inline bool converts(int channels) { return channels == 3 && blend_colorspace() != ENBL_COLORSPACE_IDENTITY; }
+
+// Replaces a staged RGB image (w x h, interleaved, tight rows) by its
+// scaled blend-space image when a blend colour space is set.
+inline void enter_blend_space(DeviceBuffer& img, int itype, int w, int h, int channels, size_t bytes)
+{
+    if (!converts(channels)) return;
+    if (blend_colorspace() == ENBL_COLORSPACE_CIECAM)
+        throw error(ENBL_E_UNSUPPORTED, "blend colour space CIECAM02 is not supported");
+    DeviceBuffer conv(bytes);
+    check(enbl_rgb_to_blend_space(context(), blend_colorspace(), itype, w, h, img.get(), (long)w * 3, conv.get(),
+                                  (long)w * 3));
+    img.swap(conv);
+}
 
 // ---- pixel type mapping ---------------------------------------------------
 template <class T, class = void> struct pixel {
@@ -357,6 +388,7 @@ gaussianPyramid(unsigned int numLevels, bool wraparound,
     std::vector<uint8_t> al = gather_mask(alpha_upperleft, w, h, aa);
     DeviceBuffer dimg(img.size() * sizeof(SComp)), dal(al.size());
     dimg.upload(img.data(), img.size() * sizeof(SComp));
+    enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     dal.upload(al.data(), al.size());
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
     check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
@@ -385,6 +417,7 @@ gaussianPyramid(unsigned int numLevels, bool wraparound,
     std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
     DeviceBuffer dimg(img.size() * sizeof(SComp));
     dimg.upload(img.data(), img.size() * sizeof(SComp));
+    enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
     check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
                                 nullptr, 0, p.get()));
@@ -415,6 +448,7 @@ laplacianPyramid(const char* /*exportName*/, unsigned int numLevels, bool wrapar
     std::vector<uint8_t> al = gather_mask(alpha_upperleft, w, h, aa);
     DeviceBuffer dimg(img.size() * sizeof(SComp)), dal(al.size());
     dimg.upload(img.data(), img.size() * sizeof(SComp));
+    enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     dal.upload(al.data(), al.size());
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
     check(enbl_laplacian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
@@ -626,8 +660,8 @@ inline std::vector<PyramidImageType*>* laplacianPyramid(const char* exportName, 
                                                                         alpha.first, alpha.second);
 }
 
-/** Replaces copyToPyramidImage (src/fixmath.h:2018-2067, :2119-2131),
- * identity colour space: integer images v << PyramidFractionBits, float
+/** Replaces copyToPyramidImage (src/fixmath.h:2018-2131), RGB images
+ * through the blend colour space first when one is set: integer images v << PyramidFractionBits, float
  * masks fromRealPromote(v * 2^PyramidFractionBits) into integer pyramids,
  * float images 1 + log1p(v) (v >= 0) or 1 / (1 - v) into double pyramids. */
 template <typename SrcImageType, typename PyramidImageType, int PyramidIntegerBits, int PyramidFractionBits>
@@ -646,6 +680,7 @@ void copyToPyramidImage(typename SrcImageType::const_traverser src_upperleft,
     std::vector<SComp> img = gather<SComp>(src_upperleft, w, h, sa, false);
     DeviceBuffer dimg(img.size() * sizeof(SComp));
     dimg.upload(img.data(), img.size() * sizeof(SComp));
+    enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     Pyramid p(pyramid_code<PComp>(), C, w, h, 1);
     check(enbl_copy_to_pyramid(context(), image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C, p.get()));
     scatter(p.read<PComp>(0, C), dest_upperleft, w, h, da, true);
@@ -714,8 +749,9 @@ void enfuseMask(SrcTriple src, AlphaPair mask, ResultPair result)
     scatter(out, result.first, w, h, result.second, false, al.data());
 }
 
-/** Replaces copyFromPyramidImageIf (src/fixmath.h:2157-2294), identity
- * colour space; integer pyramids dither deterministically (seed 0). */
+/** Replaces copyFromPyramidImageIf (src/fixmath.h:2157-2294), RGB images
+ * back from the blend colour space when one is set; integer pyramids dither
+ * deterministically (seed 0). */
 template <typename PyramidImageType, typename MaskImageType, typename DestImageType,
           int PyramidIntegerBits, int PyramidFractionBits>
 void copyFromPyramidImageIf(typename PyramidImageType::const_traverser src_upperleft,
@@ -739,6 +775,13 @@ void copyFromPyramidImageIf(typename PyramidImageType::const_traverser src_upper
     dm.upload(m.data(), m.size());
     check(enbl_copy_from_pyramid_if(context(), p.get(), 0, PyramidFractionBits, (const uint8_t*)dm.get(), w, dimg.get(),
                                     image_code<DComp>(), (long)w * C, ENBL_DITHER_DET, 0));
+    if (converts(C)) {
+        // the scaled blend-space values back to RGB under the mask
+        DeviceBuffer rgb((size_t)w * h * C * sizeof(DComp));
+        check(enbl_blend_space_to_rgb_if(context(), blend_colorspace(), image_code<DComp>(), w, h, dimg.get(),
+                                         (long)w * 3, (const uint8_t*)dm.get(), w, rgb.get(), (long)w * 3));
+        dimg.swap(rgb);
+    }
     std::vector<DComp> out((size_t)w * h * C);
     dimg.download(out.data(), out.size() * sizeof(DComp));
     scatter(out, dest_upperleft, w, h, da, false, m.data());

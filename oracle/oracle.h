@@ -167,6 +167,20 @@ int orc_nft_mask_periodic(const uint8_t* white, long wpitch, const uint8_t* blac
  * of n values; planted = 1 uses a deliberately reordered intersection
  * formula (for tests that must detect it). */
 int orc_euclid_1d(const float* f, int n, int planted, float* d);
+/* Blend colour spaces (src/fixmath.h:429-1330; oracle_colorspace.cpp).
+ * orc_set_blend_colorspace sets the space orc_enblend_pair (and hence
+ * orc_enblend_step) and orc_enfuse use for 8- and 16-bit RGB images. */
+#define ORC_COLORSPACE_IDENTITY 0
+#define ORC_COLORSPACE_CIELAB 1
+#define ORC_COLORSPACE_CIELUV 2
+int orc_set_blend_colorspace(int space);
+int orc_blend_colorspace(void);
+int orc_rgb_to_blend_space(int space, int itype, int w, int h, const void* img, void* out);
+int orc_blend_space_to_rgb_if(int space, int itype, int w, int h, const void* src, const uint8_t* mask, void* img);
+int orc_colorspace_forward(int space, const double* rgb, long n, double* out);
+int orc_colorspace_inverse(int space, const double* v, long n, double* rgb);
+void orc_rgb_to_xyz_matrix(double* m);
+
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);
 
 /* enblend's driver around the blend (oracle_driver.cpp). */

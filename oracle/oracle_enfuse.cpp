@@ -561,9 +561,20 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
     const float fallback = (float)(255 / K);
     const double max_mask_pyr = (double)((M)(255 << mask_frac));
     const double scale = (double)(1u << mask_frac);
+    // A CIELAB / CIELUV blend space: the pyramids are built from the images
+    // converted to the scaled blend space, the result goes back to RGB under
+    // the union alpha (src/fixmath.h:2018-2294); the weights above used RGB.
+    const int cs = std::is_same<T, float>::value ? ORC_COLORSPACE_IDENTITY : orc_blend_colorspace();
+    const int cstype = sizeof(T) == 1 ? ORC_U8 : ORC_U16;
+    std::vector<T> scaled(cs != ORC_COLORSPACE_IDENTITY ? (size_t)N * 3 : 0);
     for (int m = 0; m < K; ++m) {
         std::vector<P> lp((size_t)nimg);
-        image_to_level0(img_frac, 3, imgs[m], w, h, lp.data());
+        const T* src = imgs[m];
+        if (cs != ORC_COLORSPACE_IDENTITY) {
+            orc_rgb_to_blend_space(cs, cstype, w, h, imgs[m], scaled.data());
+            src = scaled.data();
+        }
+        image_to_level0(img_frac, 3, src, w, h, lp.data());
         int rc = gaussian_levels<P>(lp.data(), di, 3, wrap, alphas[m]);
         if (rc) return rc;
         laplacian_levels<P>(lp.data(), di, 3, wrap);
@@ -599,7 +610,12 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
         }
     }
     collapse_levels<P>(result.data(), di, 3, wrap);
-    level0_to_image(img_frac, 3, result.data(), w, h, ua.data(), out, dither, seed);
+    if (cs != ORC_COLORSPACE_IDENTITY) {
+        level0_to_image(img_frac, 3, result.data(), w, h, ua.data(), scaled.data(), dither, seed);
+        orc_blend_space_to_rgb_if(cs, cstype, w, h, scaled.data(), ua.data(), out);
+    } else {
+        level0_to_image(img_frac, 3, result.data(), w, h, ua.data(), out, dither, seed);
+    }
     memcpy(out_alpha, ua.data(), (size_t)N);
     return 0;
 }

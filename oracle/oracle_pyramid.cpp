@@ -118,6 +118,22 @@ int orc_enblend_pair(int itype, int C, int w, int h, int levels, int wrap,
                      const uint8_t* mask, int dither, uint64_t seed)
 {
     if (levels < 1 || levels > 29) return -1;
+    const int cs = orc_blend_colorspace();
+    if (cs != ORC_COLORSPACE_IDENTITY && C == 3) {
+        // both images into the scaled blend space, the step there, the
+        // result back to RGB under the union alpha
+        if (itype != ORC_U8 && itype != ORC_U16) return -4;
+        const size_t n = (size_t)w * h * 3 * (itype == ORC_U8 ? 1 : 2);
+        std::vector<uint8_t> wl(n), bl(n);
+        orc_rgb_to_blend_space(cs, itype, w, h, white, wl.data());
+        orc_rgb_to_blend_space(cs, itype, w, h, black, bl.data());
+        orc_set_blend_colorspace(ORC_COLORSPACE_IDENTITY);
+        const int rc = orc_enblend_pair(itype, C, w, h, levels, wrap, wl.data(), walpha, bl.data(), balpha, mask,
+                                        dither, seed);
+        orc_set_blend_colorspace(cs);
+        if (rc) return rc;
+        return orc_blend_space_to_rgb_if(cs, itype, w, h, bl.data(), balpha, black);
+    }
     int rc;
     if (itype == ORC_U8) {
         rc = enblend_pair_t<uint8_t, int16_t, int16_t>(C, w, h, levels, wrap, 7, 7, (const uint8_t*)white, walpha, (uint8_t*)black, balpha, mask, dither, seed);

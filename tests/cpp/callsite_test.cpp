@@ -12,8 +12,11 @@
 //            image, its alpha, and the exposure-weight table used
 //   ops:     w h, then an RGB16 image and alpha -> the single operations'
 //            results (see pyramid_ops)
+// An optional argv[4] sets the blend colour space (ENBL_COLORSPACE_*), as
+// --blend-colorspace sets BlendColorspace.
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -281,13 +284,14 @@ int pyramid_ops(FILE* in, FILE* out)
 int main(int argc, char** argv)
 {
     if (argc < 4) {
-        fprintf(stderr, "usage: callsite_test enblend|enfuse8|enfuse16|ops in.bin out.bin\n");
+        fprintf(stderr, "usage: callsite_test enblend|enfuse8|enfuse16|ops in.bin out.bin [colorspace]\n");
         return 2;
     }
     FILE* in = fopen(argv[2], "rb");
     FILE* out = fopen(argv[3], "wb");
     if (!in || !out) return 2;
     int rc = 2;
+    if (argc > 4) enblend_amd::blend_colorspace() = atoi(argv[4]);
     try {
         if (!strcmp(argv[1], "enblend")) rc = enblend_step(in, out);
         else if (!strcmp(argv[1], "enfuse8")) rc = enfuse<uint8_t, int16_t, int16_t, 9, 7, 9, 7>(in, out);

@@ -37,6 +37,12 @@ _SIGS = {
     "orc_nft_mask": [_vp, _l, _vp, _l, _i, _i, _i, _vp, _l],
     "orc_distance_transform": [_vp, _l, _i, _i, _i, _vp, _l],
     "orc_euclid_1d": [_vp, _i, _i, _vp],
+    "orc_set_blend_colorspace": [_i],
+    "orc_rgb_to_blend_space": [_i, _i, _i, _i, _vp, _vp],
+    "orc_blend_space_to_rgb_if": [_i, _i, _i, _i, _vp, _vp, _vp],
+    "orc_colorspace_forward": [_i, _vp, _l, _vp],
+    "orc_colorspace_inverse": [_i, _vp, _l, _vp],
+    "orc_rgb_to_xyz_matrix": [_vp],
     "orc_local_stddev_f32": [_vp, _vp, _i, _i, _i, _i, _vp],
     "orc_nft_mask_periodic": [_vp, _l, _vp, _l, _i, _i, _i, _i, _vp, _l],
     "orc_assemble": [_i, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp],
@@ -67,6 +73,9 @@ def layer_table(layers, ptr=lambda a: a.ctypes.data):
 EXP_GAUSSIAN, EXP_LORENTZIAN, EXP_HALFSINE, EXP_FULLSINE, EXP_BISQUARE = range(5)
 (GRAY_AVERAGE, GRAY_LSTAR, GRAY_PLSTAR, GRAY_LIGHTNESS, GRAY_VALUE, GRAY_ANTI_VALUE, GRAY_LUMINANCE,
  GRAY_MIXER) = range(8)
+
+
+CS_IDENTITY, CS_CIELAB, CS_CIELUV, CS_CIECAM = 0, 1, 2, 3
 
 
 class EnfuseParams(ctypes.Structure):
@@ -337,6 +346,52 @@ def _local_stddev_f32(self, gray, mask, n, planted=0):
     return out
 
 
+def _set_blend_colorspace(self, space):
+    if self.lib.orc_set_blend_colorspace(space):
+        raise ValueError(f"bad colour space {space}")
+
+
+def _rgb_to_blend_space(self, space, img, itype):
+    img = np.ascontiguousarray(img, NP_I[itype])
+    h, w, _ = img.shape
+    out = np.zeros_like(img)
+    if self.lib.orc_rgb_to_blend_space(space, itype, w, h, _p(img), _p(out)):
+        raise ValueError("orc_rgb_to_blend_space")
+    return out
+
+
+def _blend_space_to_rgb_if(self, space, src, mask, img, itype):
+    src = np.ascontiguousarray(src, NP_I[itype])
+    mask = np.ascontiguousarray(mask, np.uint8)
+    out = np.array(img, NP_I[itype], copy=True)
+    h, w, _ = src.shape
+    if self.lib.orc_blend_space_to_rgb_if(space, itype, w, h, _p(src), _p(mask), _p(out)):
+        raise ValueError("orc_blend_space_to_rgb_if")
+    return out
+
+
+def _colorspace_forward(self, space, rgb):
+    rgb = np.ascontiguousarray(rgb, np.float64).reshape(-1, 3)
+    out = np.zeros_like(rgb)
+    if self.lib.orc_colorspace_forward(space, _p(rgb), len(rgb), _p(out)):
+        raise ValueError("orc_colorspace_forward")
+    return out
+
+
+def _colorspace_inverse(self, space, v):
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+    out = np.zeros_like(v)
+    if self.lib.orc_colorspace_inverse(space, _p(v), len(v), _p(out)):
+        raise ValueError("orc_colorspace_inverse")
+    return out
+
+
+def _rgb_to_xyz_matrix(self):
+    m = np.zeros(9, np.float64)
+    self.lib.orc_rgb_to_xyz_matrix(_p(m))
+    return m.reshape(3, 3)
+
+
 def _nft_mask(self, white, black, norm=2):
     """(mask, rc): rc = -2 on excessive overlap (mask written anyway)."""
     white = np.ascontiguousarray(white, np.uint8)
@@ -425,6 +480,12 @@ Oracle.inspect_overlap = _inspect_overlap
 Oracle.mask_bounds = _mask_bounds
 Oracle.enblend_step = _enblend_step
 Oracle.euclid_1d = _euclid_1d
+Oracle.set_blend_colorspace = _set_blend_colorspace
+Oracle.rgb_to_blend_space = _rgb_to_blend_space
+Oracle.blend_space_to_rgb_if = _blend_space_to_rgb_if
+Oracle.colorspace_forward = _colorspace_forward
+Oracle.colorspace_inverse = _colorspace_inverse
+Oracle.rgb_to_xyz_matrix = _rgb_to_xyz_matrix
 Oracle.local_stddev_f32 = _local_stddev_f32
 Oracle.checkpoint = _checkpoint
 def _nft_mask_periodic(self, white, black, boundary, norm=2):

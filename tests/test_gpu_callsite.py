@@ -29,12 +29,13 @@ def binary():
     return BIN
 
 
-def _run(binary, mode, tmp_path, parts):
+def _run(binary, mode, tmp_path, parts, space=None):
     src, dst = tmp_path / f"{mode}.in", tmp_path / f"{mode}.out"
     with open(src, "wb") as f:
         for p in parts:
             f.write(np.ascontiguousarray(p).tobytes())
-    r = subprocess.run([binary, mode, str(src), str(dst)], capture_output=True, text=True, timeout=120)
+    extra = [] if space is None else [str(space)]
+    r = subprocess.run([binary, mode, str(src), str(dst)] + extra, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     return open(dst, "rb").read()
@@ -45,9 +46,19 @@ def _take(buf, off, dtype, shape):
     return np.frombuffer(buf[off:off + n], dtype).reshape(shape), off + n
 
 
-@pytest.mark.parametrize("shape", [(96, 160), (301, 457)])
-def test_enblend_call_site_matches_oracle(binary, oracle, tmp_path, shape):
+@pytest.fixture
+def oracle_space(oracle):
+    """The oracle's blend colour space, put back to identity afterwards."""
+    yield oracle.set_blend_colorspace
+    oracle.set_blend_colorspace(0)
+
+
+@pytest.mark.parametrize("shape,space", [((96, 160), 0), ((301, 457), 0), ((301, 457), 1), ((96, 160), 2)])
+def test_enblend_call_site_matches_oracle(binary, oracle, oracle_space, tmp_path, shape, space):
+    """Also with --blend-colorspace CIELAB / CIELUV (BlendColorspace set
+    before the loop)."""
     h, w = shape
+    oracle_space(space)
     rng = np.random.default_rng(h + w)
     white = rng.integers(0, 65536, size=(h, w, 3)).astype(np.uint16)
     black = rng.integers(0, 65536, size=(h, w, 3)).astype(np.uint16)
@@ -57,8 +68,8 @@ def test_enblend_call_site_matches_oracle(binary, oracle, tmp_path, shape):
     wa, ba = wa.astype(np.uint8), ba.astype(np.uint8)
     mask = np.where(xx < 0.5 * w + 0.1 * (yy - h / 2), 0, 255).astype(np.uint8)
     L = oracle.auto_levels(w, h)
-    print(f"enblend call site {w}x{h} L={L}", flush=True)
-    buf = _run(binary, "enblend", tmp_path, [np.array([w, h, L], np.int32), white, wa, black, ba, mask])
+    print(f"enblend call site {w}x{h} L={L} colour space {space}", flush=True)
+    buf = _run(binary, "enblend", tmp_path, [np.array([w, h, L], np.int32), white, wa, black, ba, mask], space)
     got, off = _take(buf, 0, np.uint16, (h, w, 3))
     got_a, off = _take(buf, off, np.uint8, (h, w))
     assert off == len(buf)
@@ -80,16 +91,19 @@ def _stack(rng, itype, K, h, w):
     return out
 
 
-@pytest.mark.parametrize("itype,shape,K", [(U8, (64, 96), 3), (U16, (133, 101), 4)])
-def test_enfuse_call_site_with_user_weight_matches_oracle(binary, oracle, tmp_path, itype, shape, K):
+@pytest.mark.parametrize("itype,shape,K,space", [(U8, (64, 96), 3, 0), (U16, (133, 101), 4, 0),
+                                                 (U8, (64, 96), 3, 2), (U16, (133, 101), 4, 1)])
+def test_enfuse_call_site_with_user_weight_matches_oracle(binary, oracle, oracle_space, tmp_path, itype, shape, K,
+                                                          space):
     h, w = shape
+    oracle_space(space)
     rng = np.random.default_rng(31 + itype + K)
     imgs = _stack(rng, itype, K, h, w)
     alphas = [np.where(rng.random((h, w)) < 0.95, 255, 0).astype(np.uint8) for _ in range(K)]
     L = oracle.auto_levels(w, h)
     mode = "enfuse8" if itype == U8 else "enfuse16"
-    print(f"enfuse call site {mode} {w}x{h} K={K} L={L}", flush=True)
-    buf = _run(binary, mode, tmp_path, [np.array([w, h, L, K], np.int32)] + imgs + alphas)
+    print(f"enfuse call site {mode} {w}x{h} K={K} L={L} colour space {space}", flush=True)
+    buf = _run(binary, mode, tmp_path, [np.array([w, h, L, K], np.int32)] + imgs + alphas, space)
     dt = np.uint8 if itype == U8 else np.uint16
     got, off = _take(buf, 0, dt, (h, w, 3))
     got_a, off = _take(buf, off, np.uint8, (h, w))

@@ -1,0 +1,217 @@
+"""GPU parity of the blend colour spaces (--blend-colorspace=CIELAB/CIELUV,
+src/fixmath.h:429-1330) against the oracle, through the C ABI: the
+conversions alone and the enblend pair, enblend step and enfuse pipelines
+with a colour space set on the context.  Integer images: bit-exact."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle_binding import CS_CIELAB, CS_CIELUV, F32, U8, U16, enfuse_params as orc_params
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+NP = {U8: np.uint8, U16: np.uint16, F32: np.float32}
+MAXV = {U8: 255, U16: 65535}
+SPACES = [CS_CIELAB, CS_CIELUV]
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _rgb(rng, itype, h, w):
+    m = MAXV[itype]
+    img = rng.integers(0, m + 1, size=(h, w, 3)).astype(NP[itype])
+    # primaries, secondaries, white, black and grays in the first row
+    spec = np.array([[1, 1, 1], [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [0, 1, 1], [1, 0, 1],
+                     [0.5, 0.5, 0.5], [0.04045, 0.04045, 0.04045], [0.01, 0.01, 0.01]])
+    img[0, :len(spec)] = np.round(spec * m).astype(NP[itype])
+    return img
+
+
+@pytest.fixture
+def space_ctx(enbl, gpu_ctx, oracle):
+    """Sets a colour space on the context and the oracle; puts both back to
+    identity afterwards."""
+    def set_space(space):
+        enbl.check(enbl.lib().enbl_set_blend_colorspace(gpu_ctx.h, space))
+        oracle.set_blend_colorspace(space)
+    yield set_space
+    enbl.check(enbl.lib().enbl_set_blend_colorspace(gpu_ctx.h, enbl.CS_IDENTITY))
+    oracle.set_blend_colorspace(0)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("space", SPACES)
+def test_rgb_to_blend_space_matches_oracle(enbl, gpu_ctx, oracle, itype, space):
+    rng = np.random.default_rng(900 + itype + space)
+    h, w = 301, 457
+    img = _rgb(rng, itype, h, w)
+    want = oracle.rgb_to_blend_space(space, img, itype)
+    pitch = 3 * w + 21                      # a padded row pitch, in elements
+    dimg = torch.zeros((h, pitch), dtype=getattr(torch, str(np.dtype(NP[itype]))), device="cuda")
+    dimg[:, :3 * w] = torch.from_numpy(img.reshape(h, 3 * w)).cuda()
+    dout = torch.zeros((h, w, 3), dtype=dimg.dtype, device="cuda")
+    torch.cuda.synchronize()
+    print(f"rgb_to_blend_space space={space} itype={itype} {w}x{h}", flush=True)
+    enbl.check(enbl.lib().enbl_rgb_to_blend_space(gpu_ctx.h, space, itype, w, h, _vp(dimg), pitch, _vp(dout), 3 * w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dout.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("space", SPACES)
+def test_blend_space_to_rgb_if_matches_oracle(enbl, gpu_ctx, oracle, itype, space):
+    """Scaled values from the forward conversion and arbitrary ones (out of
+    gamut, clamped), under a random mask over a random image."""
+    rng = np.random.default_rng(950 + itype + space)
+    h, w = 211, 389
+    s = 128 if itype == U8 else 32768
+    src = oracle.rgb_to_blend_space(space, _rgb(rng, itype, h, w), itype)
+    src[h // 2:] = rng.integers(0, s + 1, size=src[h // 2:].shape).astype(NP[itype])
+    mask = np.where(rng.random((h, w)) < 0.6, 255, 0).astype(np.uint8)
+    img = _rgb(rng, itype, h, w)
+    want = oracle.blend_space_to_rgb_if(space, src, mask, img, itype)
+    dsrc, dm, dimg = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (src, mask, img))
+    torch.cuda.synchronize()
+    print(f"blend_space_to_rgb_if space={space} itype={itype} {w}x{h}", flush=True)
+    enbl.check(enbl.lib().enbl_blend_space_to_rgb_if(gpu_ctx.h, space, itype, w, h, _vp(dsrc), 3 * w, _vp(dm), w,
+                                                     _vp(dimg), 3 * w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dimg.cpu().numpy(), want)
+
+
+def _pair_inputs(rng, itype, h, w, C=3):
+    m = MAXV.get(itype, 1.0)
+    if itype == F32:
+        white = rng.random((h, w, C)).astype(np.float32)
+        black = rng.random((h, w, C)).astype(np.float32)
+    else:
+        white = rng.integers(0, m + 1, size=(h, w, C)).astype(NP[itype])
+        black = rng.integers(0, m + 1, size=(h, w, C)).astype(NP[itype])
+    yy, xx = np.mgrid[0:h, 0:w]
+    wa = np.where((xx - 0.3 * w) ** 2 / (0.45 * w) ** 2 + (yy - 0.5 * h) ** 2 / (0.48 * h) ** 2 < 1, 255, 0)
+    ba = np.where((xx - 0.7 * w) ** 2 / (0.45 * w) ** 2 + (yy - 0.5 * h) ** 2 / (0.48 * h) ** 2 < 1, 255, 0)
+    mask = np.where(xx < 0.5 * w + 0.1 * (yy - h / 2), 0, 255).astype(np.uint8)
+    return white, wa.astype(np.uint8), black, ba.astype(np.uint8), mask
+
+
+def _gpu_pair(enbl, ctx, itype, C, L, white, wa, black, ba, mask, seed):
+    h, w = wa.shape
+    dwh, dwa, dbk, dba, dm = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (white, wa, black, ba, mask))
+    torch.cuda.synchronize()
+    rc = enbl.lib().enbl_enblend_pair(ctx.h, itype, C, w, h, L, 0, _vp(dwh), C * w, _vp(dwa), w, _vp(dbk), C * w,
+                                      _vp(dba), w, _vp(dm), w, 1, seed)
+    ctx.sync()
+    return rc, dbk.cpu().numpy(), dba.cpu().numpy()
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("space", SPACES)
+@pytest.mark.parametrize("shape", [(96, 160), (301, 457), (1501, 2899)])
+def test_enblend_pair_in_blend_space_matches_oracle(enbl, gpu_ctx, oracle, space_ctx, itype, space, shape):
+    """Sizes cover the level-by-level path, the shared small-level launches
+    and the fused streaming passes."""
+    h, w = shape
+    rng = np.random.default_rng(1000 + itype + space + h)
+    L = oracle.auto_levels(w, h)
+    space_ctx(space)
+    white, wa, black, ba, mask = _pair_inputs(rng, itype, h, w)
+    want, want_a = oracle.enblend_pair(white, wa, black, ba, mask, L, False, itype, 1, 5)
+    print(f"enblend_pair space={space} itype={itype} {w}x{h} L={L}", flush=True)
+    rc, got, got_a = _gpu_pair(enbl, gpu_ctx, itype, 3, L, white, wa, black, ba, mask, 5)
+    enbl.check(rc)
+    np.testing.assert_array_equal(got_a, want_a)
+    np.testing.assert_array_equal(got, want)
+    # the black image keeps its values outside the union of the alphas
+    out = (want_a == 0)
+    np.testing.assert_array_equal(got[out], black[out])
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_grayscale_pair_ignores_blend_space(enbl, gpu_ctx, oracle, space_ctx, itype):
+    """One-channel images blend in their own space whatever the setting
+    (the reference converts only RGB pyramids)."""
+    h, w = 120, 200
+    rng = np.random.default_rng(1100 + itype)
+    L = oracle.auto_levels(w, h)
+    white, wa, black, ba, mask = _pair_inputs(rng, itype, h, w, C=1)
+    want, want_a = oracle.enblend_pair(white, wa, black, ba, mask, L, False, itype, 1, 9)
+    space_ctx(CS_CIELAB)
+    print(f"enblend_pair C=1 space=lab itype={itype}", flush=True)
+    rc, got, got_a = _gpu_pair(enbl, gpu_ctx, itype, 1, L, white, wa, black, ba, mask, 9)
+    enbl.check(rc)
+    np.testing.assert_array_equal(got_a, want_a)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_unsupported_spaces_and_types(enbl, gpu_ctx, oracle, space_ctx):
+    lib = enbl.lib()
+    assert lib.enbl_set_blend_colorspace(gpu_ctx.h, enbl.CS_CIECAM) == enbl.ENBL_E_UNSUPPORTED
+    assert lib.enbl_set_blend_colorspace(gpu_ctx.h, 7) == enbl.ENBL_E_ARG
+    h, w = 40, 64
+    rng = np.random.default_rng(3)
+    white, wa, black, ba, mask = _pair_inputs(rng, F32, h, w)
+    space_ctx(CS_CIELUV)
+    print("enblend_pair float with CIELUV", flush=True)
+    rc, got, _ = _gpu_pair(enbl, gpu_ctx, F32, 3, 3, white, wa, black, ba, mask, 1)
+    assert rc == enbl.ENBL_E_UNSUPPORTED
+    np.testing.assert_array_equal(got, black)           # untouched
+    d = torch.zeros((h, w, 3), dtype=torch.float32, device="cuda")
+    assert lib.enbl_rgb_to_blend_space(gpu_ctx.h, CS_CIELAB, F32, w, h, _vp(d), 3 * w, _vp(d), 3 * w) \
+        == enbl.ENBL_E_UNSUPPORTED
+    d16 = torch.zeros((h, w, 3), dtype=torch.uint16, device="cuda")
+    assert lib.enbl_rgb_to_blend_space(gpu_ctx.h, enbl.CS_IDENTITY, U16, w, h, _vp(d16), 3 * w, _vp(d16), 3 * w) \
+        == enbl.ENBL_E_ARG
+    assert lib.enbl_rgb_to_blend_space(gpu_ctx.h, CS_CIELAB, U16, w, h, _vp(d16), 3 * w - 1, _vp(d16), 3 * w) \
+        == enbl.ENBL_E_SIZE
+    gpu_ctx.sync()
+
+
+def _stack(rng, itype, K, h, w):
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.5 + 0.4 * np.sin(xx / 7.0) * np.cos(yy / 5.0)
+    tint = rng.uniform(0.7, 1.3, size=3)
+    m = MAXV[itype]
+    out = []
+    for k in range(K):
+        rad = base[..., None] * tint[None, None, :] * 2.0 ** (k - (K - 1) / 2)
+        v = np.clip(rad, 0, 1) ** (1 / 2.2) * m + rng.normal(0, 1, rad.shape)
+        out.append(np.clip(np.rint(v), 0, m).astype(NP[itype]))
+    return out
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("space", SPACES)
+def test_enfuse_in_blend_space_matches_oracle(enbl, gpu_ctx, oracle, space_ctx, itype, space):
+    """Weights from the RGB images, pyramids in the blend space; the batched
+    entry point takes the same path stack by stack."""
+    kw = dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.3, w_entropy=0.2, contrast_window=5,
+              entropy_window=3)
+    rng = np.random.default_rng(1200 + itype + space)
+    space_ctx(space)
+    for (h, w, K) in [(64, 96, 3), (333, 517, 4)]:
+        imgs = _stack(rng, itype, K, h, w)
+        alphas = [np.where(rng.random((h, w)) < 0.95, 255, 0).astype(np.uint8) for _ in range(K)]
+        L = oracle.auto_levels(w, h)
+        print(f"enfuse space={space} itype={itype} {w}x{h} K={K} L={L}", flush=True)
+        want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), L, dither=1, seed=4)
+        dimgs = [torch.from_numpy(i).cuda() for i in imgs]
+        das = [torch.from_numpy(a).cuda() for a in alphas]
+        out = torch.zeros_like(dimgs[0])
+        oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        enbl.enfuse(gpu_ctx, itype, dimgs, das, enbl.enfuse_params(**kw), L, out, oa, dither=1, seed=4)
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
+        np.testing.assert_array_equal(out.cpu().numpy(), want)
+        out2 = torch.zeros_like(dimgs[0])
+        oa2 = torch.zeros_like(oa)
+        print(f"enfuse_batch space={space} itype={itype} {w}x{h}", flush=True)
+        enbl.enfuse_batch(gpu_ctx, itype, [(dimgs, das, out2, oa2, L, 4)], enbl.enfuse_params(**kw))
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(out2.cpu().numpy(), want)
+        np.testing.assert_array_equal(oa2.cpu().numpy(), want_a)
