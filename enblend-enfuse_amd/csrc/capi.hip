@@ -83,11 +83,42 @@ bool aligned2(const void* p, int ptype) { return ((uintptr_t)p % (2 * (size_t)pt
 int mask_ptype_for(int itype) { return itype == ENBL_U8 ? ENBL_I16 : ENBL_I32; }
 int image_ptype_for(int itype) { return itype == ENBL_U8 ? ENBL_I16 : itype == ENBL_U16 ? ENBL_I32 : ENBL_F64; }
 
+int sync_streams(enbl_context* ctx)
+{
+    int rc = hip_check(hipStreamSynchronize(ctx->forked_from ? ctx->forked_from : ctx->stream), "stream synchronize");
+    if (!rc) rc = hip_check(hipStreamSynchronize(ctx->aux_stream), "stream synchronize");
+    return rc;
+}
+
+int fork_aux(enbl_context* ctx)
+{
+    if (ctx->forked_from) return fail(ENBL_E_ARG, "auxiliary stream already in use");
+    int rc = hip_check(hipEventRecord(ctx->aux_fork, ctx->stream), "hipEventRecord");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(ctx->aux_stream, ctx->aux_fork, 0), "hipStreamWaitEvent");
+    if (rc) return rc;
+    ctx->forked_from = ctx->stream;
+    ctx->stream = ctx->aux_stream;
+    return ENBL_OK;
+}
+
+int end_aux(enbl_context* ctx)
+{
+    if (!ctx->forked_from) return ENBL_OK;
+    ctx->stream = ctx->forked_from;
+    ctx->forked_from = nullptr;
+    return hip_check(hipEventRecord(ctx->aux_join, ctx->aux_stream), "hipEventRecord");
+}
+
+int join_aux(enbl_context* ctx)
+{
+    return hip_check(hipStreamWaitEvent(ctx->stream, ctx->aux_join, 0), "hipStreamWaitEvent");
+}
+
 int alpha_scratch(enbl_context* ctx, size_t bytes, uint8_t** a, uint8_t** b)
 {
     if (ctx->scratch_alpha_bytes < 2 * bytes) {
         if (ctx->scratch_alpha) {
-            int rc = hip_check(hipStreamSynchronize(ctx->stream), "alpha scratch");
+            int rc = sync_streams(ctx);
             if (rc) return rc;
             (void)hipFree(ctx->scratch_alpha);
         }
@@ -378,15 +409,20 @@ struct TailBuilder {
     // A reduce job: streaming strips for outputs of at least kTailStripPixels
     // (their 5x5 windows would load every source sample 6 times), one thread
     // per output below.
-    void add_reduce(bool image, ReduceArgs a) {
+    void add_reduce(bool image, ReduceArgs a) { add_reduce_kind(image ? kTailReduceImage : kTailReduceMask, a); }
+    // kind: kTailReduceImage, kTailReduceMask or kTailReduceMaskAlpha
+    void add_reduce_kind(int kind, ReduceArgs a) {
         if ((long)a.dw * a.dh >= kTailStripPixels) {
             const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
             a.rows = 4;
-            add_tiles(image ? kTailReduceImageStrip : kTailReduceMaskStrip, (waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows).u.r = a;
+            const int strip = kind == kTailReduceImage ? kTailReduceImageStrip
+                            : (kind == kTailReduceMask ? kTailReduceMaskStrip : kTailReduceMaskAlphaStrip);
+            add_tiles(strip, (waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows).u.r = a;
         } else {
-            add(image ? kTailReduceImage : kTailReduceMask, a.dw, a.dh).u.r = a;
+            add(kind, a.dw, a.dh).u.r = a;
         }
     }
+    bool full() const { return t.njobs == kTailMaxJobs; }
     static constexpr long kTailStripPixels = 1L << 16;
 };
 
@@ -515,12 +551,197 @@ int enblend_tail(enbl_context* ctx, const enbl_pyramid* mgp, enbl_pyramid* wlp, 
     return ENBL_OK;
 }
 
+namespace {
+
+// Multi-job launches filled job by job: a launch goes out when it holds
+// kTailMaxJobs jobs and at flush().  pt / mt / C: the pyramid types and
+// image channels the launch is instantiated for.
+struct TailSink {
+    enbl_context* ctx;
+    int pt, mt, C;
+    TailBuilder b;
+    TailSink(enbl_context* c, int p, int m, int ch) : ctx(c), pt(p), mt(m), C(ch) {}
+    int room() {
+        if (!b.full()) return ENBL_OK;
+        return flush();
+    }
+    int flush() {
+        int rc = b.t.njobs ? run_tail(ctx, pt, mt, C, b) : ENBL_OK;
+        b = TailBuilder();
+        return rc;
+    }
+    int reduce(int kind, const ReduceArgs& a, double bytes) {
+        int rc = room();
+        if (rc) return rc;
+        b.add_reduce_kind(kind, a);
+        b.bytes += bytes;
+        return ENBL_OK;
+    }
+    int expand_sub(const enbl_pyramid* p, int l) {
+        int rc = room();
+        if (rc) return rc;
+        const Level& F = p->lv[l];
+        const Level& Cl = p->lv[l + 1];
+        ExpandArgs e;
+        memset(&e, 0, sizeof e);
+        e.src = level_ptr(p, l + 1); e.spitch = Cl.pitch; e.splane = Cl.plane; e.sw = Cl.w; e.sh = Cl.h;
+        e.dst = level_ptr(p, l); e.dpitch = F.pitch; e.dplane = F.plane; e.dw = F.w; e.dh = F.h;
+        e.channels = p->channels;
+        b.add(kTailExpandSub, Cl.w, Cl.h).u.e = e;
+        b.bytes += (double)ptype_size(p->ptype) * p->channels * ((double)Cl.w * Cl.h + 2.0 * F.w * F.h);
+        return ENBL_OK;
+    }
+};
+
+// The pyramid-type pair a multi-job launch is instantiated for: image and
+// mask pyramids of the same image type.
+int tail_mask_type(int pt) { return pt == ENBL_I16 ? ENBL_I16 : ENBL_I32; }
+int tail_image_type(int mt) { return mt == ENBL_I16 ? ENBL_I16 : ENBL_I32; }
+
+}  // namespace
+
+// Laplacian pyramids of K images under their alphas, no wraparound.  The
+// levels whose coarse level streams are built image by image (the fused
+// Laplacian passes); the levels below, of all K pyramids, go through shared
+// multi-job launches: the launch for level l reduces level l of every
+// pyramid and expand-subtracts its level l-1, split into launches of at most
+// kTailMaxJobs jobs (no job of a level writes what another job of that level
+// reads).  Same results as laplacian_pyramid_any per image.  Scratch: the
+// alpha ping-pong, slots 8 and 23 (fused passes) and 35 (alpha levels).
+int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const void* const* imgs, const long* ips,
+                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps)
+{
+    const enbl_pyramid* p0 = lps[0];
+    const int L = p0->levels;
+    const int pt = p0->ptype, C = p0->channels;
+    const int s = fused_level_count(p0, 0, alphas[0]);
+    const double esP = (double)ptype_size(pt) * C;
+    auto tpitch = [&](int l) { return (long)round_up((size_t)p0->lv[l].w, 64); };
+    // Per image: its level-s alpha as the fused passes leave it (pitch apl,
+    // sized for every level those passes write into that ping-pong half),
+    // then its levels s+1..L-1.
+    uint8_t* pp[2] = {nullptr, nullptr};
+    long apl = 0;
+    size_t own_s = 0;
+    if (s >= 1) {
+        int rc = alpha_pingpong(ctx, p0, &pp[0], &pp[1], &apl);
+        if (rc) return rc;
+        for (int l = s; l >= 1; l -= 2) own_s = std::max(own_s, (size_t)apl * p0->lv[l].h);
+        own_s = round_up(own_s, 256);
+    }
+    std::vector<size_t> toff(L, 0);
+    size_t tbytes = own_s;
+    for (int l = s + 1; l < L; ++l) {
+        toff[l] = tbytes;
+        tbytes += round_up((size_t)tpitch(l) * p0->lv[l].h, 256);
+    }
+    uint8_t* base = nullptr;
+    if (tbytes) {
+        void* v;
+        int rc = scratch(ctx, 35, tbytes * (size_t)K, &v);
+        if (rc) return rc;
+        base = (uint8_t*)v;
+    }
+    auto ialpha = [&](int m, int l) -> uint8_t* {
+        if (l == 0) return const_cast<uint8_t*>(alphas[m]);
+        return base + tbytes * (size_t)m + (l == s ? 0 : toff[l]);
+    };
+    auto ipitch = [&](int m, int l) { return l == 0 ? aps[m] : (l == s ? apl : tpitch(l)); };
+    for (int m = 0; m < K; ++m) {
+        int rc;
+        if (s == 0) {
+            rc = build_level0(ctx, lps[m], itype, frac, imgs[m], ips[m]);
+        } else {
+            uint8_t* abuf[2] = {pp[0], pp[1]};
+            abuf[(s - 1) & 1] = ialpha(m, s);
+            rc = fused_laplacian_levels(ctx, itype, frac, imgs[m], ips[m], alphas[m], aps[m], lps[m], s, abuf, apl);
+        }
+        if (rc) return rc;
+    }
+    // Launch t reduces level s+t of every pyramid and expand-subtracts its
+    // level s+t-1.
+    TailSink sink(ctx, pt, tail_mask_type(pt), C);
+    for (int l = s; l < L; ++l) {
+        for (int m = 0; m < K; ++m) {
+            enbl_pyramid* p = lps[m];
+            int rc = ENBL_OK;
+            if (l + 1 < L) {
+                if (p->lv[l].w < 2 || p->lv[l].h < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
+                rc = sink.reduce(kTailReduceImage,
+                                 tail_reduce(p, l, ialpha(m, l), ipitch(m, l), ialpha(m, l + 1), ipitch(m, l + 1)),
+                                 (esP + 1.0) * ((double)p->lv[l].w * p->lv[l].h + (double)p->lv[l + 1].w * p->lv[l + 1].h));
+            }
+            if (!rc && l > s) rc = sink.expand_sub(p, l - 1);
+            if (rc) return rc;
+        }
+        int rc = sink.flush();
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
+// Gaussian pyramids of K masks (level 0 written) under one shared alpha `ua`,
+// no wraparound: the streaming levels mask by mask (the first writes the
+// alpha's levels), the levels below of all K in shared multi-job launches.
+// Same results as gaussian_levels per mask.  Scratch: slot 36 (the alpha's
+// levels).
+int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap)
+{
+    const enbl_pyramid* g0 = mgps[0];
+    const int L = g0->levels;
+    const int mt = g0->ptype;
+    const int s = fused_level_count(g0, 0, ua);
+    const double esM = ptype_size(mt);
+    auto tpitch = [&](int l) { return (long)round_up((size_t)g0->lv[l].w, 64); };
+    std::vector<size_t> uoff(L, 0);
+    size_t ubytes = 0;
+    for (int l = 1; l < L; ++l) {
+        uoff[l] = ubytes;
+        ubytes += round_up((size_t)tpitch(l) * g0->lv[l].h, 256);
+    }
+    uint8_t* base = nullptr;
+    if (ubytes) {
+        void* v;
+        int rc = scratch(ctx, 36, ubytes, &v);
+        if (rc) return rc;
+        base = (uint8_t*)v;
+    }
+    auto ualpha = [&](int l) -> uint8_t* { return l == 0 ? const_cast<uint8_t*>(ua) : base + uoff[l]; };
+    auto upitch = [&](int l) { return l == 0 ? uap : tpitch(l); };
+    for (int m = 0; m < K; ++m) {
+        enbl_pyramid* g = mgps[m];
+        for (int l = 0; l < s; ++l) {
+            const Level& S_ = g->lv[l];
+            const Level& D = g->lv[l + 1];
+            int rc = do_reduce(ctx, mt, 1, 0, level_ptr(g, l), S_.w, S_.h, S_.pitch, S_.plane, ualpha(l), upitch(l),
+                               level_ptr(g, l + 1), D.w, D.h, D.pitch, D.plane, m == 0 ? ualpha(l + 1) : nullptr,
+                               upitch(l + 1));
+            if (rc) return rc;
+        }
+    }
+    TailSink sink(ctx, tail_image_type(mt), mt, 1);
+    for (int l = s; l + 1 < L; ++l) {
+        for (int m = 0; m < K; ++m) {
+            enbl_pyramid* g = mgps[m];
+            if (g->lv[l].w < 2 || g->lv[l].h < 2) return fail(ENBL_E_PRECONDITION, "src image too small in reduce");
+            int rc = sink.reduce(kTailReduceMaskAlpha,
+                                 tail_reduce(g, l, ualpha(l), upitch(l), m == 0 ? ualpha(l + 1) : nullptr, upitch(l + 1)),
+                                 (esM + 1.0) * (double)g->lv[l].w * g->lv[l].h +
+                                     (esM + (m == 0 ? 1.0 : 0.0)) * (double)g->lv[l + 1].w * g->lv[l + 1].h);
+            if (rc) return rc;
+        }
+        int rc = sink.flush();
+        if (rc) return rc;
+    }
+    return ENBL_OK;
+}
+
 int scratch(enbl_context* ctx, int slot, size_t bytes, void** out)
 {
     if (slot < 0 || slot >= kScratchSlots) return fail(ENBL_E_ARG, "scratch slot %d", slot);
     if (ctx->scratch_bytes[slot] < bytes) {
         if (ctx->scratch_buf[slot]) {
-            int rc = hip_check(hipStreamSynchronize(ctx->stream), "scratch");
+            int rc = sync_streams(ctx);
             if (rc) return rc;
             (void)hipFree(ctx->scratch_buf[slot]);
             ctx->scratch_buf[slot] = nullptr;
@@ -583,6 +804,12 @@ int enbl_context_create(int device, enbl_context** out)
     c->colorspace = ENBL_COLORSPACE_IDENTITY;
     rc = hip_check(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) { delete c; return rc; }
+    rc = hip_check(hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc) { (void)hipStreamDestroy(c->own_stream); delete c; return rc; }
+    rc = hip_check(hipEventCreateWithFlags(&c->aux_fork, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&c->aux_join, hipEventDisableTiming), "hipEventCreate");
+    if (rc) { (void)hipStreamDestroy(c->aux_stream); (void)hipStreamDestroy(c->own_stream); delete c; return rc; }
+    c->forked_from = nullptr;
     c->stream = c->own_stream;
     *out = c;
     return ENBL_OK;
@@ -592,12 +819,15 @@ int enbl_context_destroy(enbl_context* ctx)
 {
     if (!ctx) return ENBL_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)sync_streams(ctx);
     for (auto& c : ctx->cache) enbl_pyramid_destroy(c.p);
     batch_release(ctx);
     for (auto& t : ctx->timed) { (void)hipEventDestroy(t.start); (void)hipEventDestroy(t.stop); }
     if (ctx->scratch_alpha) (void)hipFree(ctx->scratch_alpha);
     for (int i = 0; i < kScratchSlots; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
+    (void)hipEventDestroy(ctx->aux_fork);
+    (void)hipEventDestroy(ctx->aux_join);
+    (void)hipStreamDestroy(ctx->aux_stream);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return ENBL_OK;

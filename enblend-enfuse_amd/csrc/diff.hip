@@ -38,9 +38,9 @@ constexpr int kTile = 256;
 
 inline dim3 grid2(int w, int h) { return dim3((w + kTile - 1) / kTile, h); }
 
-__device__ __forceinline__ double T(double v) { return v >= 0.0 ? 1.0 + log1p(v) : 1.0 / (1.0 - v); }
+__device__ __forceinline__ double T(double v) { return float_to_log_pyramid(v); }
 __device__ __forceinline__ double dT(double v) { return v >= 0.0 ? 1.0 / (1.0 + v) : 1.0 / ((1.0 - v) * (1.0 - v)); }
-__device__ __forceinline__ double Tinv(double c) { return c >= 1.0 ? expm1(c - 1.0) : 1.0 - 1.0 / c; }
+__device__ __forceinline__ double Tinv(double c) { return log_pyramid_to_float(c); }
 __device__ __forceinline__ double dTinv(double c) { return c >= 1.0 ? exp(c - 1.0) : 1.0 / (c * c); }
 
 __device__ __forceinline__ int bin5(int d) { return d == 0 ? 6 : ((d == 1 || d == -1) ? 4 : 1); }

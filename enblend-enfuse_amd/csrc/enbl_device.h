@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "logexp.h"
+
 namespace enbl {
 
 // Every buffer the kernels touch is device (global) memory.  Loads and stores

@@ -139,6 +139,20 @@ int blend_space_to_rgb_if(enbl_context* ctx, int space, int itype, int w, int h,
                           const uint8_t* mask, long mp, void* img, long ip);
 int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
                           const uint8_t* alpha, long apitch, enbl_pyramid* out);
+// enfuse's K Laplacian pyramids (image alphas) and K mask Gaussian pyramids
+// (one shared alpha, level 0 written) without wraparound, the small levels
+// of each set in shared multi-job launches.
+int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const void* const* imgs, const long* ips,
+                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps);
+int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap);
+// Overlapping work: fork_aux makes the auxiliary stream wait for the work
+// issued so far and directs later launches to it; end_aux records its end
+// and directs launches back to the main stream; join_aux makes the main
+// stream wait for the auxiliary work.  sync_streams waits for both.
+int fork_aux(enbl_context* ctx);
+int end_aux(enbl_context* ctx);
+int join_aux(enbl_context* ctx);
+int sync_streams(enbl_context* ctx);
 
 }  // namespace host
 }  // namespace enbl
@@ -147,6 +161,12 @@ struct enbl_context {
     int device;
     hipStream_t own_stream;
     hipStream_t stream;
+    // A second stream for work that overlaps the main sequence
+    // (fork_aux / end_aux / join_aux); `forked_from` is the main stream while
+    // work is being issued on aux_stream, else null.
+    hipStream_t aux_stream;
+    hipEvent_t aux_fork, aux_join;
+    hipStream_t forked_from;
     bool profile;
     struct Timed { std::string name; double bytes, own; hipEvent_t start, stop; };
     std::vector<Timed> timed;

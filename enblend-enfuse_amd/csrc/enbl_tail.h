@@ -1,7 +1,8 @@
-// Multi-job launches for the small levels of the enblend step (tail.hip):
-// reduces, expand-subtracts, blends and blend + collapse steps of several
-// pyramids that do not depend on each other run in one launch, each job on
-// its own range of 256-thread blocks (64 columns x 4 rows of samples).
+// Multi-job launches for the small levels of the enblend step and of
+// enfuse's pyramids (tail.hip): reduces, expand-subtracts, blends and blend
+// + collapse steps of several pyramids that do not depend on each other run
+// in one launch, each job on its own range of 256-thread blocks (64 columns
+// x 4 rows of samples).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +22,8 @@ enum TailKind : int {
     kTailCollapse = 4,      // blend of level l + expand of collapsed level l+1 (CollapseSmallArgs)
     kTailReduceImageStrip = 5,   // as kTailReduceImage in streaming strips: 4 strips x r.rows rows per block
     kTailReduceMaskStrip = 6,    // as kTailReduceMask in streaming strips
+    kTailReduceMaskAlpha = 7,    // mask level with alpha (ReduceArgs, M, 1 channel; dalpha may be null)
+    kTailReduceMaskAlphaStrip = 8,   // as kTailReduceMaskAlpha in streaming strips
 };
 
 struct TailJob {
@@ -36,7 +39,7 @@ struct TailJob {
     } u;
 };
 
-constexpr int kTailMaxJobs = 8;
+constexpr int kTailMaxJobs = 16;
 
 struct TailLaunch {
     int njobs;

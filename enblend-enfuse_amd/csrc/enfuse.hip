@@ -209,17 +209,21 @@ __global__ __launch_bounds__(64) void stddev_float_rows_kernel(const float* __re
 // localStdDevIf for float gray images over LDS column chunks, several
 // images per launch (blockIdx.y).  A block of four waves owns 64 consecutive
 // window rows and walks the image in chunks of 64 columns:
-//  * load: the chunk's gray and alpha samples, with the window's rows above
-//    and below, go to LDS (the next chunk is already in flight in registers
-//    while the current one is processed);
+//  * load: the chunk's samples, with the window's rows above and below, go
+//    to LDS as (value under alpha or +0, 1 or 0) pairs (the next chunk is
+//    already in flight in registers while the current one is processed);
 //  * column sums (all waves, lane = column): each window row's column sum of
-//    values and of squares, top to bottom, and its sample count;
+//    values and of squares, top to bottom, and its sample count, with no
+//    branch: a sample outside alpha adds +0, which leaves every sum unchanged
+//    bit for bit (a sum that starts at +0 never becomes -0);
 //  * running sums (one wave, lane = row): the float running sums advance
 //    column by column exactly in the reference's order, sum += s - (the
 //    column leaving the window), the leaving column being zero while the
-//    window fills, which leaves the sums unchanged bit for bit; the leaving
-//    columns sit in a register ring of N entries; the sums after each column
-//    overwrite that column's sums in LDS;
+//    window fills; the column sums of the chunk sit N slots to the right of
+//    the running sums, so step i reads slots N + i (entering column) and i
+//    (leaving column, or the previous chunk's last N columns, kept in
+//    registers, for i < N) and writes its running sums to slot i, which no
+//    later step reads; the steps issue their loads ahead of the chain;
 //  * results (all waves, lane = column): the standard deviation of every
 //    window whose centre lies under alpha, stored as coalesced row segments.
 // This is the only serial part of the path: one row is a chain of one float
@@ -238,18 +242,18 @@ struct StddevJobs {
 template <int N>
 __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, int w, int h)
 {
-    constexpr int B = N / 2, TH = 64 + 2 * B, RPW = (TH + 3) / 4;
-    __shared__ float sg[TH][kSdCW + 1];
-    __shared__ uint8_t sa[TH][kSdCW + 4];
-    __shared__ float css[64][kSdCW + 1], csq[64][kSdCW + 1];
-    __shared__ uint8_t csk[64][kSdCW + 4];
+    constexpr int B = N / 2, TH = 64 + 2 * B, RPW = (TH + 3) / 4, CS = N + kSdCW;
+    __shared__ float2 sg[TH][kSdCW + 1];             // (value under alpha or +0, alpha flag)
+    __shared__ float css[64][CS + 1], csq[64][CS + 1];
+    __shared__ uint8_t csk[64][CS + 3];
     __shared__ uint8_t prev_a[2][64][4];
     const int job = blockIdx.y;
     const float* __restrict__ gray = J.gray[job];
     const uint8_t* __restrict__ alpha = J.alpha[job];
     float* __restrict__ out = J.out[job];
     const long gp = J.gpitch, ap = J.apitch[job], op = J.opitch;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    // the wave index is uniform: row addresses stay scalar
+    const int tid = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int row0 = blockIdx.x * 64;
     float pg[RPW];
     uint8_t pa[RPW];
@@ -263,12 +267,14 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
             pa[u] = in ? alpha[(long)gy * ap + gx] : (uint8_t)0;
         }
     };
-    float cs[N], cq[N];
-    int cn[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) { cs[i] = 0.f; cq[i] = 0.f; cn[i] = 0; }
+    // chain state (wave 0): running sums and the previous chunk's last N
+    // column sums
     float sum = 0.f, sq = 0.f;
     int cnt = 0;
+    float ps[N], pq[N];
+    int pk[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) { ps[i] = 0.f; pq[i] = 0.f; pk[i] = 0; }
     prev_a[0][lane][wv] = 0;
     fetch(0);
     int par = 0;
@@ -276,48 +282,62 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
 #pragma unroll
         for (int u = 0; u < RPW; ++u) {
             const int r = wv + 4 * u;
-            if (r < TH) {
-                sg[r][lane] = pg[u];
-                sa[r][lane] = pa[u];
-            }
+            if (r < TH) sg[r][lane] = make_float2(pa[u] ? pg[u] : 0.f, pa[u] ? 1.f : 0.f);
         }
         __syncthreads();
         if (xb + kSdCW < w) fetch(xb + kSdCW);
         const int nx = min(kSdCW, w - xb);
         // column sums of window rows wv*16 .. wv*16+15, column `lane`
+#pragma unroll 4
         for (int rr = 0; rr < 16; ++rr) {
             const int r = wv * 16 + rr;
-            float s = 0.f, q = 0.f;
-            int k = 0;
+            float s = 0.f, q = 0.f, k = 0.f;
 #pragma unroll
             for (int dy = 0; dy < N; ++dy) {
-                if (sa[r + dy][lane]) {
-                    const float v = sg[r + dy][lane];
-                    s += v;
-                    q += v * v;
-                    ++k;
-                }
+                const float2 v = sg[r + dy][lane];
+                s += v.x;
+                q += v.x * v.x;
+                k += v.y;
             }
-            css[r][lane] = s;
-            csq[r][lane] = q;
-            csk[r][lane] = (uint8_t)k;
+            css[r][N + lane] = s;
+            csq[r][N + lane] = q;
+            csk[r][N + lane] = (uint8_t)k;
         }
         __syncthreads();
         if (wv == 0) {
-            for (int i = 0; i < nx; ++i) {
-                const float s = css[lane][i], q = csq[lane][i];
-                const int k = csk[lane][i];
-                sum += s - cs[0];
-                sq += q - cq[0];
-                cnt += k - cn[0];
 #pragma unroll
-                for (int j = 0; j + 1 < N; ++j) { cs[j] = cs[j + 1]; cq[j] = cq[j + 1]; cn[j] = cn[j + 1]; }
-                cs[N - 1] = s;
-                cq[N - 1] = q;
-                cn[N - 1] = k;
+            for (int i = 0; i < N; ++i) {
+                if (i < nx) {
+                    const float s = css[lane][N + i], q = csq[lane][N + i];
+                    const int k = csk[lane][N + i];
+                    sum += s - ps[i];
+                    sq += q - pq[i];
+                    cnt += k - pk[i];
+                    css[lane][i] = sum;
+                    csq[lane][i] = sq;
+                    csk[lane][i] = (uint8_t)cnt;
+                }
+            }
+#pragma unroll 8
+            for (int i = N; i < nx; ++i) {
+                const float s = css[lane][N + i], q = csq[lane][N + i];
+                const int k = csk[lane][N + i];
+                const float so = css[lane][i], qo = csq[lane][i];
+                const int ko = csk[lane][i];
+                sum += s - so;
+                sq += q - qo;
+                cnt += k - ko;
                 css[lane][i] = sum;
                 csq[lane][i] = sq;
                 csk[lane][i] = (uint8_t)cnt;
+            }
+            if (nx == kSdCW) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    ps[j] = css[lane][kSdCW + j];
+                    pq[j] = csq[lane][kSdCW + j];
+                    pk[j] = csk[lane][kSdCW + j];
+                }
             }
         }
         __syncthreads();
@@ -328,7 +348,7 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
             const int r = wv * 16 + rr;
             const int y = row0 + r + B, x = xb + lane - B;
             if (lane < nx && y < h - B && x >= B && x < w - B) {
-                const uint8_t ca = lane >= B ? sa[r + B][lane - B] : prev_a[par][r][lane];
+                const bool ca = lane >= B ? sg[r + B][lane - B].y != 0.f : prev_a[par][r][lane] != 0;
                 if (ca) {
                     const int c = csk[r][lane];
                     const float sm = css[r][lane], sqv = csq[r][lane];
@@ -340,7 +360,7 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
         }
         for (int rr = 0; rr < 16; ++rr) {
             const int r = wv * 16 + rr;
-            if (lane < B) prev_a[par ^ 1][r][lane] = sa[r + B][kSdCW - B + lane];
+            if (lane < B) prev_a[par ^ 1][r][lane] = sg[r + B][kSdCW - B + lane].y != 0.f ? 1 : 0;
         }
         __syncthreads();
     }
@@ -535,7 +555,7 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
     auto& cmx = u.cmx;
     auto& ent = u.ent;
     const int x0 = blockIdx.x * kEntTW, y0 = blockIdx.y * kEntTH;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     // Halo tile, one wave per tile row (rows wv, wv + NW, ...), lanes along
     // the row's interleaved samples; samples outside the image count as
     // alpha 0.  All loads of a thread are issued before the first LDS store.

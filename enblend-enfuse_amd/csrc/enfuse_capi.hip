@@ -516,6 +516,43 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     if (!rc) rc = launch_fill(ua, plane, s);
     if (rc) return rc;
 
+    const int pt = image_ptype_for(itype);
+    const int mt = mask_ptype_for(itype);
+    const int ifrac = itype == ENBL_F32 ? 0 : 7;
+    const int mfrac = itype == ENBL_U8 ? 7 : 15;
+    // Every image's Laplacian pyramid and normalised mask pyramid are kept
+    // (context cache), then each level merges up to kMergeMax images per
+    // pass in image order, so the result level is read and written once per
+    // group instead of once per image.
+    enbl_pyramid* res = nullptr;
+    std::vector<enbl_pyramid*> lps((size_t)K, nullptr), mgps((size_t)K, nullptr);
+    rc = cached_pyramid(ctx, kRoleResult, pt, 3, w, h, levels, 0, &res);
+    if (rc) return rc;
+    for (int m = 0; m < K; ++m) {
+        rc = cached_pyramid(ctx, kRoleImage, pt, 3, w, h, levels, m, &lps[(size_t)m]);
+        if (!rc) rc = cached_pyramid(ctx, kRoleMask, mt, 1, w, h, levels, m, &mgps[(size_t)m]);
+        if (rc) return rc;
+    }
+    // The images' Laplacian pyramids need the images alone: without
+    // wraparound they are built on the auxiliary stream while the weights
+    // are computed on the main one (memory-bound passes beside the
+    // arithmetic-bound weight kernels); the main stream waits for them
+    // before the merge, and on every error path.
+    struct AuxJoin {
+        enbl_context* ctx;
+        bool active = false;
+        ~AuxJoin() { if (active) (void)join_aux(ctx); }
+    } aux{ctx};
+    if (!wrap) {
+        rc = fork_aux(ctx);
+        if (rc) return rc;
+        aux.active = true;
+        rc = enfuse_image_pyramids(ctx, itype, ifrac, K, pimgs.data(), pips.data(), alphas, aps, lps.data());
+        const int rc2 = end_aux(ctx);
+        if (!rc) rc = rc2;
+        if (rc) return rc;
+    }
+
     // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388); the
     // contrast of all images first, so that their local standard deviations
     // share launches.
@@ -537,42 +574,34 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
 
     // Pass 2: per input, Laplacian pyramid, normalised mask Gaussian pyramid
     // over the union alpha, multiply and accumulate (src/enfuse.h:1497-1590).
-    const int pt = image_ptype_for(itype);
-    const int mt = mask_ptype_for(itype);
-    const int ifrac = itype == ENBL_F32 ? 0 : 7;
-    const int mfrac = itype == ENBL_U8 ? 7 : 15;
-    // Every image's Laplacian pyramid and normalised mask pyramid are kept
-    // (context cache), then
-    // each level merges up to kMergeMax images per pass in image order, so
-    // the result level is read and written once per group instead of once
-    // per image.
-    enbl_pyramid* res = nullptr;
-    std::vector<enbl_pyramid*> lps((size_t)K, nullptr), mgps((size_t)K, nullptr);
-    rc = cached_pyramid(ctx, kRoleResult, pt, 3, w, h, levels, 0, &res);
-    if (rc) return rc;
-    for (int m = 0; m < K; ++m) {
-        rc = cached_pyramid(ctx, kRoleImage, pt, 3, w, h, levels, m, &lps[(size_t)m]);
-        if (!rc) rc = cached_pyramid(ctx, kRoleMask, mt, 1, w, h, levels, m, &mgps[(size_t)m]);
-        if (rc) return rc;
-    }
     const float fallback = (float)(255 / K);
     const double scale = (double)(1u << mfrac);
     const double maxpyr = (double)(255 << mfrac);
-    for (int m = 0; m < K; ++m) {
-        enbl_pyramid* lp = lps[(size_t)m];
+    auto normalize = [&](int m) {
         enbl_pyramid* mgp = mgps[(size_t)m];
-        rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, pimgs[(size_t)m], pips[(size_t)m], alphas[m], aps[m], lp);
+        const Level& L0 = mgp->lv[0];
+        Timed t(ctx, "normalize", (double)w * h * (8.0 + ptype_size(mt)));
+        int r = launch_normalize(mt, (const float*)masks + plane * m, pitch, (const float*)norm, pitch, w, h,
+                                 fallback, scale, p->hard_mask, level_ptr(mgp, 0), L0.pitch, s);
+        return r ? r : launch_status("normalize");
+    };
+    if (!wrap) {
+        // mask level 0 of every image, the mask pyramids with their small
+        // levels in shared launches, then the image pyramids' end
+        for (int m = 0; m < K && !rc; ++m) rc = normalize(m);
+        if (!rc) rc = enfuse_mask_pyramids(ctx, K, mgps.data(), (const uint8_t*)ua, pitch);
         if (rc) return rc;
-        {
-            const Level& L0 = mgp->lv[0];
-            Timed t(ctx, "normalize", (double)w * h * (8.0 + ptype_size(mt)));
-            rc = launch_normalize(mt, (const float*)masks + plane * m, pitch, (const float*)norm, pitch, w, h,
-                                  fallback, scale, p->hard_mask, level_ptr(mgp, 0), L0.pitch, s);
-            if (!rc) rc = launch_status("normalize");
+        aux.active = false;
+        rc = join_aux(ctx);
+        if (rc) return rc;
+    } else {
+        for (int m = 0; m < K; ++m) {
+            rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, pimgs[(size_t)m], pips[(size_t)m], alphas[m], aps[m],
+                                        lps[(size_t)m]);
+            if (!rc) rc = normalize(m);
+            if (!rc) rc = gaussian_levels(ctx, mgps[(size_t)m], wrap, (const uint8_t*)ua, pitch);
             if (rc) return rc;
         }
-        rc = gaussian_levels(ctx, mgp, wrap, (const uint8_t*)ua, pitch);
-        if (rc) return rc;
     }
     for (int l = 0; l < levels; ++l) {
         const Level& L = lps[0]->lv[l];

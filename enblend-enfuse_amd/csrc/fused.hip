@@ -393,7 +393,7 @@ template <class P, class I> struct OutConv {
 };
 template <> struct OutConv<double, float> {
     __device__ static __forceinline__ float conv(double v, int, bool, uint32_t, int) {
-        return (float)(v >= 1.0 ? expm1(v - 1.0) : 1.0 - 1.0 / v);
+        return (float)log_pyramid_to_float(v);
     }
 };
 

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kTile) void to_pyramid_log(const float* __restrict_
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double v = (double)s[c];
-        out[c * pl + (long)y * pp + x] = v >= 0.0 ? 1.0 + log1p(v) : 1.0 / (1.0 - v);
+        out[c * pl + (long)y * pp + x] = float_to_log_pyramid(v);
     }
 }
 
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kTile) void from_pyramid_log(const double* __restri
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double v = pyr[c * pl + (long)y * pp + x];
-        d[c] = (float)(v >= 1.0 ? expm1(v - 1.0) : 1.0 - 1.0 / v);
+        d[c] = (float)log_pyramid_to_float(v);
     }
 }
 

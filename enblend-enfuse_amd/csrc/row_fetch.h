@@ -185,7 +185,7 @@ template <int C> struct FloatImageSrc {
     struct Raw { float f[2 * C]; };
     __device__ static __forceinline__ double conv(float f) {
         const double v = (double)f;
-        return v >= 0.0 ? 1.0 + log1p(v) : 1.0 / (1.0 - v);
+        return float_to_log_pyramid(v);
     }
     template <class Args>
     __device__ static __forceinline__ void fetch(const Args& a, int fw, int r, bool row_ok, const LaneCols& L, Raw& R) {

@@ -126,7 +126,7 @@ __device__ __forceinline__ void reduce_small_body(const ReduceArgs& a, int bx, i
             *gmem(dst + ch * a.dplane + (long)i * a.dpitch + j) = Px::wrap(A::div(v, 256));
         }
     }
-    if (ALPHA) *gmem(a.dalpha + (long)i * a.dapitch + j) = ap ? 255 : 0;
+    if (ALPHA && a.dalpha) *gmem(a.dalpha + (long)i * a.dapitch + j) = ap ? 255 : 0;
 }
 
 // Horizontal polyphase sums of one source row at column j for the even

@@ -22,15 +22,18 @@ __global__ __launch_bounds__(256) void tail_kernel(TailLaunch t)
     switch (J.kind) {
     case kTailReduceImage: reduce_small_body<P, C, true, false>(J.u.r, bx, by); break;
     case kTailReduceMask: reduce_small_body<M, 1, false, false>(J.u.r, bx, by); break;
+    case kTailReduceMaskAlpha: reduce_small_body<M, 1, true, false>(J.u.r, bx, by); break;
     case kTailExpandSub: expand_small_body<P, C, false, false>(J.u.e, bx, by); break;
     case kTailBlend: blend_small_body<P, M, C>(J.u.b, bx, by); break;
     case kTailCollapse: collapse_small_body<P, M, C>(J.u.c, bx, by); break;
     case kTailReduceImageStrip:
-    case kTailReduceMaskStrip: {
+    case kTailReduceMaskStrip:
+    case kTailReduceMaskAlphaStrip: {
         const int wx = bx * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const int i0 = by * J.u.r.rows;
         if (J.kind == kTailReduceImageStrip) reduce_strip_ring<P, C, true, PlaneSrc<P, C>, true>(J.u.r, wx, i0);
-        else reduce_strip_ring<M, 1, false, PlaneSrc<M, 1>, true>(J.u.r, wx, i0);
+        else if (J.kind == kTailReduceMaskStrip) reduce_strip_ring<M, 1, false, PlaneSrc<M, 1>, true>(J.u.r, wx, i0);
+        else reduce_strip_ring<M, 1, true, PlaneSrc<M, 1>, true>(J.u.r, wx, i0);
         break;
     }
     default: break;

@@ -4,7 +4,8 @@ cmsCreate_sRGBProfile, cmsCreateXYZProfile, cmsCreateLab2Profile at the sRGB
 profile's media white point, INTENT_PERCEPTUAL, cmsFLAGS_NOCACHE |
 cmsFLAGS_HIGHRESPRECALC), evaluated by the system's LittleCMS 2 runtime
 (liblcms2.so.2) through ctypes on a grid of RGB values: RGB -> XYZ, RGB -> Lab,
-and back Lab -> RGB, XYZ -> RGB.  Run in a container that has the library;
+and back Lab -> RGB, XYZ -> RGB; and for 16-bit codes the scaled Lab values
+the pyramid holds and the RGB LittleCMS gives back for them.  Run in a container that has the library;
 the fixture is data (inputs and LittleCMS's outputs)."""
 import ctypes
 import os
@@ -67,8 +68,21 @@ def main():
     lab = run(t_lab, rgb)
     rgb_from_lab = run(t_lab_inv, lab)
     rgb_from_xyz = run(t_xyz_inv, xyz)
+    # 16-bit codes: LittleCMS's Lab of code / 65535, the PyramidScale'd
+    # values of it (s = 32768, src/fixmath.h:486-525), and LittleCMS's RGB of
+    # the Lab those scaled values stand for
+    codes = np.concatenate([np.array([[65535] * 3, [0] * 3, [65535, 0, 0], [0, 65535, 0], [0, 0, 65535],
+                                      [32768] * 3, [2651] * 3, [1] * 3], np.int64),
+                            rng.integers(0, 65536, size=(600, 3))]).astype(np.uint16)
+    lab16 = run(t_lab, codes / 65535.0)
+    s = 32768.0
+    q16 = np.stack([np.round(s * lab16[:, 0] / 100.0), np.round(s * (200.0 + lab16[:, 1]) / 400.0),
+                    np.round(s * (200.0 + lab16[:, 2]) / 400.0)], axis=1).astype(np.uint16)
+    lab_q = np.stack([q16[:, 0] * 100.0 / s, q16[:, 1] * 400.0 / s - 200.0, q16[:, 2] * 400.0 / s - 200.0], axis=1)
+    rgb_from_q16 = run(t_lab_inv, lab_q)
     np.savez_compressed(os.path.join(HERE, "colorspace_lcms.npz"), rgb=rgb, xyz=xyz, lab=lab,
-                        rgb_from_lab=rgb_from_lab, rgb_from_xyz=rgb_from_xyz,
+                        rgb_from_lab=rgb_from_lab, rgb_from_xyz=rgb_from_xyz, codes16=codes, lab16=lab16,
+                        q16=q16, rgb_from_q16=rgb_from_q16,
                         lab_white_xyY=np.array(list(white)), lcms_version=np.array(L.cmsGetEncodedCMMversion()))
     print("wrote", len(rgb), "samples; Lab white xyY", list(white))
 

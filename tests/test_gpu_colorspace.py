@@ -215,3 +215,31 @@ def test_enfuse_in_blend_space_matches_oracle(enbl, gpu_ctx, oracle, space_ctx, 
         gpu_ctx.sync()
         np.testing.assert_array_equal(out2.cpu().numpy(), want)
         np.testing.assert_array_equal(oa2.cpu().numpy(), want_a)
+
+
+def test_16bit_codes_against_lcms(enbl, gpu_ctx):
+    """The device conversions against LittleCMS directly
+    (tests/golden/colorspace_lcms.npz): 16-bit codes into PyramidScale'd Lab,
+    and the scaled values back to RGB; equal but for one unit where
+    LittleCMS's single-precision pipeline crosses a half."""
+    import os
+    fix = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "colorspace_lcms.npz"))
+    codes = np.ascontiguousarray(fix["codes16"])
+    n = len(codes)
+    dimg = torch.from_numpy(codes).cuda()
+    dq = torch.zeros_like(dimg)
+    print(f"rgb_to_blend_space vs LittleCMS, {n} codes", flush=True)
+    enbl.check(enbl.lib().enbl_rgb_to_blend_space(gpu_ctx.h, CS_CIELAB, U16, n, 1, _vp(dimg), 3 * n, _vp(dq), 3 * n))
+    gpu_ctx.sync()
+    d = np.abs(dq.cpu().numpy().astype(np.int64) - fix["q16"].astype(np.int64))
+    assert d.max() <= 1 and np.mean(d == 0) > 0.995
+    dsrc = torch.from_numpy(np.ascontiguousarray(fix["q16"])).cuda()
+    dm = torch.full((n,), 255, dtype=torch.uint8, device="cuda")
+    dout = torch.zeros_like(dsrc)
+    enbl.check(enbl.lib().enbl_blend_space_to_rgb_if(gpu_ctx.h, CS_CIELAB, U16, n, 1, _vp(dsrc), 3 * n, _vp(dm), n,
+                                                     _vp(dout), 3 * n))
+    gpu_ctx.sync()
+    r = 65535.0 * fix["rgb_from_q16"]
+    want = np.clip(np.where(r >= 0, np.floor(r + 0.5), np.ceil(r - 0.5)), 0, 65535).astype(np.int64)
+    d = np.abs(dout.cpu().numpy().astype(np.int64) - want)
+    assert d.max() <= 1 and np.mean(d == 0) > 0.995

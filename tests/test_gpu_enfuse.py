@@ -356,3 +356,35 @@ def test_float_contrast_known_answers(enbl, gpu_ctx, oracle):
             assert np.isnan(got[2, 2])
         elif (h, w) == (5, 6):
             assert got[2, 2] == np.sqrt(np.float32(11182079.0), dtype=np.float32) and np.isnan(got[2, 3])
+
+
+@pytest.mark.parametrize("itype,K", [(U16, 9), (U8, 3)])
+def test_enfuse_streaming_levels_and_shared_tails_match_oracle(enbl, gpu_ctx, oracle, itype, K):
+    """2200x2000 images: level 1 takes the fused streaming passes (built on
+    the auxiliary stream, beside the weights), the levels below of all
+    pyramids go through shared multi-job launches (with 9 images, more jobs
+    per level than one launch holds); two calls in a row on one context."""
+    h, w = 2000, 2200
+    rng = np.random.default_rng(1300 + itype + K)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.5 + 0.4 * np.sin(xx / 37.0) * np.cos(yy / 23.0)
+    m = 255 if itype == U8 else 65535
+    dt = np.uint8 if itype == U8 else np.uint16
+    imgs = [np.clip(np.rint(np.clip(base[..., None] * rng.uniform(0.7, 1.3, 3) * 2.0 ** (k - K // 2), 0, 1)
+                            ** (1 / 2.2) * m + rng.normal(0, m / 2000.0, (h, w, 3))), 0, m).astype(dt)
+            for k in range(K)]
+    alphas = [np.where(rng.random((h, w)) < 0.97, 255, 0).astype(np.uint8) for _ in range(K)]
+    L = oracle.auto_levels(w, h)
+    kw = dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.3, contrast_window=5)
+    want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), L, dither=1, seed=2)
+    dimgs = [torch.from_numpy(i).cuda() for i in imgs]
+    das = [torch.from_numpy(a).cuda() for a in alphas]
+    for rep in range(2):
+        out = torch.zeros_like(dimgs[0])
+        oa = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        print(f"enfuse {w}x{h} itype={itype} K={K} L={L} call {rep}", flush=True)
+        enbl.enfuse(gpu_ctx, itype, dimgs, das, enbl.enfuse_params(**kw), L, out, oa, dither=1, seed=2)
+        gpu_ctx.sync()
+        np.testing.assert_array_equal(oa.cpu().numpy(), want_a)
+        np.testing.assert_array_equal(out.cpu().numpy(), want)

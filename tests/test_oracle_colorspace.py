@@ -149,3 +149,22 @@ def test_scaled_values_against_lcms_lab(orc):
     d = np.abs(scale(mine) - scale(FIX["lab"]))
     assert d.max() <= 1
     assert np.mean(d == 0) > 0.99
+
+
+def _from_real_u16(r):
+    r = 65535.0 * r
+    return np.clip(np.where(r >= 0, np.floor(r + 0.5), np.ceil(r - 0.5)), 0, 65535).astype(np.int64)
+
+
+def test_16bit_codes_against_lcms(orc):
+    """16-bit codes through PyramidScale'd Lab and back, against
+    LittleCMS's Lab of the same codes and its RGB of the scaled values:
+    equal but for one unit where LittleCMS's float rounding crosses a half."""
+    codes = FIX["codes16"].reshape(1, -1, 3)
+    q = orc.rgb_to_blend_space(CS_CIELAB, codes, U16).reshape(-1, 3).astype(np.int64)
+    d = np.abs(q - FIX["q16"].astype(np.int64))
+    assert d.max() <= 1 and np.mean(d == 0) > 0.995
+    mask = np.full(codes.shape[:2], 255, np.uint8)
+    back = orc.blend_space_to_rgb_if(CS_CIELAB, FIX["q16"].reshape(1, -1, 3), mask, np.zeros_like(codes), U16)
+    d = np.abs(back.reshape(-1, 3).astype(np.int64) - _from_real_u16(FIX["rgb_from_q16"]))
+    assert d.max() <= 1 and np.mean(d == 0) > 0.995
