@@ -61,12 +61,13 @@ template <class I, class P> struct ImageLoad {
 };
 
 // "average" grayscale projector (src/mga.h:188-193) with the integer
-// fromRealPromote of the image type.
+// fromRealPromote of the image type: (int)((p0 + p1 + p2) / 3.0 + 0.5).  For
+// an integer sum S the quotient S / 3 is never within 1/6 of a half, so the
+// double form equals (S + 1) / 3 exactly (checked over every sum of 8- and
+// 16-bit samples by the test suite's exhaustive comparison).
 template <class T> __device__ __forceinline__ int gray_code(T p0, T p1, T p2)
 {
-    const double v = ((double)p0 + (double)p1 + (double)p2) / 3.0;
-    if (v < 0.0) return 0;
-    return v > ImgMax<T>::d ? ImgMax<T>::i : (int)(int32_t)(v + 0.5);
+    return ((int)p0 + (int)p1 + (int)p2 + 1) / 3;
 }
 
 // ---------------------------------------------------------------------------
@@ -78,35 +79,49 @@ template <class T> __device__ __forceinline__ int gray_code(T p0, T p1, T p2)
 // The weights of a pixel pair (x, x+1) for every image of the stack, all
 // loads issued before any arithmetic; NK bounds the stack depth this
 // instance keeps in registers (deeper stacks take the looping instance).
+// The raw samples of a pixel pair (x, x+1) of an interleaved RGB image,
+// packed: two 16-bit RGB pixels are the pair's 12 bytes (one dwordx3 load
+// when aligned), two 8-bit ones six bytes in d[0] and the low half of d[1].
+// Keeping the words packed until the weights are formed holds a deep
+// stack's loads in flight with few registers.
 template <class T>
-__device__ __forceinline__ void load_pair(const T* p, bool two, T (&v)[6])
+__device__ __forceinline__ void load_raw(const T* p, bool two, uint32_t (&d)[3])
 {
     if constexpr (sizeof(T) == 2) {
-        if (two && ((uintptr_t)p & 3) == 0) {   // two RGB16 pixels: one dwordx3
-            const auto* d = gmem(reinterpret_cast<const uint32_t*>(p));
-            const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
-            v[0] = (T)(d0 & 0xffffu); v[1] = (T)(d0 >> 16);
-            v[2] = (T)(d1 & 0xffffu); v[3] = (T)(d1 >> 16);
-            v[4] = (T)(d2 & 0xffffu); v[5] = (T)(d2 >> 16);
+        if (two && ((uintptr_t)p & 3) == 0) {
+            const auto* g = gmem(reinterpret_cast<const uint32_t*>(p));
+            d[0] = g[0]; d[1] = g[1]; d[2] = g[2];
             return;
         }
-    }
-    const auto* g = gmem(p);
+        const auto* g = gmem(p);
+        uint32_t v[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? g[k] : (T)0;
-}
-
-__device__ __forceinline__ void load_alpha_pair(const uint8_t* p, bool two, uint8_t& a0, uint8_t& a1)
-{
-    if (two && ((uintptr_t)p & 1) == 0) {
-        const uint16_t u = *gmem(reinterpret_cast<const uint16_t*>(p));
-        a0 = (uint8_t)(u & 0xffu);
-        a1 = (uint8_t)(u >> 8);
+        for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? (uint32_t)g[k] : 0u;
+        d[0] = v[0] | (v[1] << 16); d[1] = v[2] | (v[3] << 16); d[2] = v[4] | (v[5] << 16);
     } else {
         const auto* g = gmem(p);
-        a0 = g[0];
-        a1 = two ? g[1] : (uint8_t)0;
+        uint32_t v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = (k < 3 || two) ? (uint32_t)g[k] : 0u;
+        d[0] = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
+        d[1] = v[4] | (v[5] << 8);
+        d[2] = 0u;
     }
+}
+
+// Sample k (0..5: R, G, B of x, then of x+1) of a packed pair.
+template <class T> __device__ __forceinline__ T raw_at(const uint32_t (&d)[3], int k)
+{
+    if constexpr (sizeof(T) == 2) return (T)((d[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+    else return (T)((d[k >> 2] >> (8 * (k & 3))) & 0xffu);
+}
+
+// The alpha pair (x, x+1): low byte x, next byte x+1 (0 past the row).
+__device__ __forceinline__ uint32_t load_alpha_raw(const uint8_t* p, bool two)
+{
+    if (two && ((uintptr_t)p & 1) == 0) return (uint32_t)*gmem(reinterpret_cast<const uint16_t*>(p));
+    const auto* g = gmem(p);
+    return (uint32_t)g[0] | (two ? (uint32_t)g[1] << 8 : 0u);
 }
 
 // enfuseMask's exposure + saturation sum of one pixel (zero outside alpha).
@@ -152,43 +167,48 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
         for (int y = ybase + wv; y < yend; y += 4) {
             float w0[NK > 0 ? NK : 1], w1[NK > 0 ? NK : 1];
             float n0 = 0.f, n1 = 0.f;
-            uint8_t u0 = 0, u1 = 0;
+            uint32_t un = 0;   // union alpha pair
+            auto pair_weights = [&](const uint32_t (&d)[3], uint32_t al, float& wa, float& wb) {
+                wa = weight_of<T>(raw_at<T>(d, 0), raw_at<T>(d, 1), raw_at<T>(d, 2), (uint8_t)(al & 0xffu), lut, w_exp, w_sat);
+                wb = weight_of<T>(raw_at<T>(d, 3), raw_at<T>(d, 4), raw_at<T>(d, 5), (uint8_t)(al >> 8), lut, w_exp, w_sat);
+            };
             if constexpr (NK > 0) {
-                T px[NK][6];
-                uint8_t a0[NK], a1[NK];
+                uint32_t raw[NK][3], ar[NK];
 #pragma unroll
                 for (int m = 0; m < NK; ++m) {
                     if (m < K) {
-                        load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px[m]);
-                        load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0[m], a1[m]);
+                        load_raw<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, raw[m]);
+                        ar[m] = load_alpha_raw(im[m].alpha + (long)y * im[m].apitch + x, two);
                     }
                 }
 #pragma unroll
                 for (int m = 0; m < NK; ++m) {
                     if (m < K) {
-                        w0[m] = weight_of<T>(px[m][0], px[m][1], px[m][2], a0[m], lut, w_exp, w_sat);
-                        w1[m] = weight_of<T>(px[m][3], px[m][4], px[m][5], a1[m], lut, w_exp, w_sat);
-                        if (a0[m]) u0 = a0[m];
-                        if (a1[m]) u1 = a1[m];
+                        pair_weights(raw[m], ar[m], w0[m], w1[m]);
+                        // the last non-zero alpha of the pixel, as the
+                        // reference's union of the alpha masks
+                        if (ar[m] & 0xffu) un = (un & 0xff00u) | (ar[m] & 0xffu);
+                        if (ar[m] & 0xff00u) un = (un & 0xffu) | (ar[m] & 0xff00u);
                         n0 = w0[m] + n0;
                         n1 = w1[m] + n1;
                     }
                 }
             } else {
                 for (int m = 0; m < K; ++m) {
-                    T px[6];
-                    uint8_t a0, a1;
-                    load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px);
-                    load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0, a1);
-                    n0 = weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat) + n0;
-                    n1 = weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat) + n1;
-                    if (a0) u0 = a0;
-                    if (a1) u1 = a1;
+                    uint32_t d[3];
+                    load_raw<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, d);
+                    const uint32_t al = load_alpha_raw(im[m].alpha + (long)y * im[m].apitch + x, two);
+                    float wa, wb;
+                    pair_weights(d, al, wa, wb);
+                    n0 = wa + n0;
+                    n1 = wb + n1;
+                    if (al & 0xffu) un = (un & 0xff00u) | (al & 0xffu);
+                    if (al & 0xff00u) un = (un & 0xffu) | (al & 0xff00u);
                 }
             }
             uint8_t* ur = J.ualpha + (long)y * J.upitch + x;   // x even, rows 64-byte aligned
-            if (two) *gmem(reinterpret_cast<uint16_t*>(ur)) = (uint16_t)(u0 | (u1 << 8));
-            else *gmem(ur) = u0;
+            if (two) *gmem(reinterpret_cast<uint16_t*>(ur)) = (uint16_t)un;
+            else *gmem(ur) = (uint8_t)(un & 0xffu);
             // Normalisation and fixed-point mask level 0 (src/enfuse.h:1529-1546).
             auto store_mask = [&](int m, float wm0, float wm1) {
                 const float nm0 = n0 > 0.0 ? (255.0f * wm0) / n0 : J.fallback;
@@ -202,12 +222,12 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
                     if (m < K) store_mask(m, w0[m], w1[m]);
             } else {
                 for (int m = 0; m < K; ++m) {
-                    T px[6];
-                    uint8_t a0, a1;
-                    load_pair<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, px);
-                    load_alpha_pair(im[m].alpha + (long)y * im[m].apitch + x, two, a0, a1);
-                    store_mask(m, weight_of<T>(px[0], px[1], px[2], a0, lut, w_exp, w_sat),
-                               weight_of<T>(px[3], px[4], px[5], a1, lut, w_exp, w_sat));
+                    uint32_t d[3];
+                    load_raw<T>(static_cast<const T*>(im[m].img) + (long)y * im[m].ipitch + 3L * x, two, d);
+                    const uint32_t al = load_alpha_raw(im[m].alpha + (long)y * im[m].apitch + x, two);
+                    float wa, wb;
+                    pair_weights(d, al, wa, wb);
+                    store_mask(m, wa, wb);
                 }
             }
         }
@@ -549,7 +569,7 @@ int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int
 #define ENBL_BW_K(T, M)                                  \
     do {                                                 \
         if (max_k <= 4) ENBL_BW(T, M, 4);                \
-        else if (max_k <= 8) ENBL_BW(T, M, 8);           \
+        else if (max_k <= 9) ENBL_BW(T, M, 9);           \
         else if (max_k <= 12) ENBL_BW(T, M, 12);         \
         else if (max_k <= 16) ENBL_BW(T, M, 16);         \
         else ENBL_BW(T, M, 0);                           \
