@@ -19,6 +19,7 @@ ENBL_E_ARG = -3
 ENBL_E_UNSUPPORTED = -4
 ENBL_E_OOM = -5
 ENBL_E_HIP = -6
+ENBL_E_IO = -7
 
 I16, I32, F64 = 0, 1, 2
 U8, U16, F32, U32 = 10, 11, 12, 13
@@ -96,6 +97,9 @@ _SIGS = {
                                      _vp]),
     "enbl_checkpoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_int, _vp,
                                  _c_long, _vp]),
+    "enbl_tiff_info": (_c_int, [ctypes.c_char_p, _c_int, _vp]),
+    "enbl_tiff_read": (_c_int, [ctypes.c_char_p, _c_int, _vp, _c_long]),
+    "enbl_tiff_write": (_c_int, [ctypes.c_char_p, _c_int, _vp, _c_int]),
 }
 
 
@@ -120,6 +124,62 @@ def layer_table(layers):
         h, w, bands = t.shape
         tab[i] = Layer(t.data_ptr(), w * bands, w, h, x, y, ft, bands)
     return tab
+
+class TiffLayerInfo(ctypes.Structure):
+    """enbl_tiff_layer_info."""
+    _fields_ = [("w", _c_int), ("h", _c_int), ("bands", _c_int), ("extra", _c_int), ("ftype", _c_int),
+                ("x", _c_int), ("y", _c_int), ("canvas_w", _c_int), ("canvas_h", _c_int),
+                ("compression", _c_int), ("pages", _c_int)]
+
+
+class TiffPage(ctypes.Structure):
+    """enbl_tiff_page."""
+    _fields_ = [("w", _c_int), ("h", _c_int), ("ftype", _c_int), ("bands", _c_int), ("extra", _c_int),
+                ("data", _vp), ("pitch", _c_long), ("x", _c_int), ("y", _c_int), ("canvas_w", _c_int),
+                ("canvas_h", _c_int)]
+
+
+TIFF_NONE, TIFF_LZW, TIFF_DEFLATE, TIFF_PACKBITS = 1, 5, 8, 32773
+_NP_OF_FTYPE = {U8: "uint8", U16: "uint16", U32: "uint32", F32: "float32"}
+
+
+def _ftype_of(dtype):
+    import numpy as np
+    for k, v in _NP_OF_FTYPE.items():
+        if np.dtype(v) == dtype:
+            return k
+    raise ValueError(f"no TIFF component type for {dtype}")
+
+
+def tiff_info(path, page=0):
+    """Header of one TIFF directory as a dict (enbl_tiff_info)."""
+    info = TiffLayerInfo()
+    check(lib().enbl_tiff_info(os.fsencode(path), page, ctypes.byref(info)))
+    return {k: getattr(info, k) for k, _ in TiffLayerInfo._fields_}
+
+
+def tiff_read(path, page=0):
+    """(array (h, w, bands) on the host, info dict) of one TIFF directory."""
+    import numpy as np
+    info = tiff_info(path, page)
+    arr = np.zeros((info["h"], info["w"], info["bands"]), dtype=_NP_OF_FTYPE[info["ftype"]])
+    check(lib().enbl_tiff_read(os.fsencode(path), page, arr.ctypes.data, info["w"] * info["bands"]))
+    return arr, info
+
+
+def tiff_write(path, pages, compression=TIFF_NONE):
+    """Write layers (array (h, w, bands), extra, x, y[, canvas_w, canvas_h]) as one TIFF."""
+    import numpy as np
+    keep = [np.ascontiguousarray(p[0]) for p in pages]
+    tab = (TiffPage * len(pages))()
+    for i, (p, a) in enumerate(zip(pages, keep)):
+        if a.ndim == 2:
+            a = keep[i] = a[:, :, None]
+        h, w, bands = a.shape
+        cw, ch = (p[4], p[5]) if len(p) > 4 else (0, 0)
+        tab[i] = TiffPage(w, h, _ftype_of(a.dtype), bands, p[1], a.ctypes.data, w * bands, p[2], p[3], cw, ch)
+    check(lib().enbl_tiff_write(os.fsencode(path), len(pages), tab, compression))
+
 
 _lib = None
 

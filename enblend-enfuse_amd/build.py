@@ -1,7 +1,8 @@
 """Build the gfx950 shared library of the enblend/enfuse pyramid path.
 
-Compiles every HIP source under csrc/ with hipcc (one object per source,
-in parallel, under _build/) and links
+Compiles every HIP source under csrc/ with hipcc and the host-only C++
+sources (the TIFF codec) with g++ (one object per source, in parallel, under
+_build/) and links, with zlib,
 enblend-enfuse_amd/libenblend_amd.so (in-tree, so it travels with the source
 tree).  Flags:
   * --offload-arch=gfx950 only (MI355X);
@@ -19,6 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libenblend_amd.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -36,6 +38,11 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+def host_sources():
+    """Host-only C++ (no device code): compiled with g++."""
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".cpp"))
+
+
 def build(verbose=False, jobs=None):
     """Compile every source to an object in parallel, then link the library;
     raises CalledProcessError on failure."""
@@ -47,8 +54,11 @@ def build(verbose=False, jobs=None):
     compile_flags = [f for f in FLAGS if f != "-shared"]
 
     def compile_one(src):
-        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
-        cmd = [HIPCC] + compile_flags + inc + ["-c", src, "-o", obj]
+        obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        if src.endswith(".cpp"):
+            cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall"] + inc + ["-c", src, "-o", obj]
+        else:
+            cmd = [HIPCC] + compile_flags + inc + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
@@ -56,8 +66,8 @@ def build(verbose=False, jobs=None):
 
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as pool:
-        objs = list(pool.map(compile_one, sources()))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+        objs = list(pool.map(compile_one, sources() + host_sources()))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -93,6 +93,10 @@ int sync_streams(enbl_context* ctx)
 int fork_aux(enbl_context* ctx)
 {
     if (ctx->forked_from) return fail(ENBL_E_ARG, "auxiliary stream already in use");
+    // While the context profiles, the work stays on the main stream: the
+    // timed launches then run one at a time and their event durations are
+    // their own, not stretched by kernels running beside them.
+    if (ctx->profile) return ENBL_OK;
     int rc = hip_check(hipEventRecord(ctx->aux_fork, ctx->stream), "hipEventRecord");
     if (!rc) rc = hip_check(hipStreamWaitEvent(ctx->aux_stream, ctx->aux_fork, 0), "hipStreamWaitEvent");
     if (rc) return rc;

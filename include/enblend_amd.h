@@ -35,6 +35,7 @@ extern "C" {
 #define ENBL_E_UNSUPPORTED (-4)  /* type / channel combination not provided */
 #define ENBL_E_OOM (-5)          /* device allocation failed (std::bad_alloc) */
 #define ENBL_E_HIP (-6)          /* HIP runtime error */
+#define ENBL_E_IO (-7)           /* file missing, unreadable or malformed */
 
 /* pixel type codes */
 #define ENBL_I16 0
@@ -554,6 +555,59 @@ int enbl_enblend_layers(enbl_context* ctx, int itype, int channels, int union_w,
  * as 16-bit is ENBL_E_UNSUPPORTED (the reference's conversion overflows). */
 int enbl_checkpoint(enbl_context* ctx, int itype, int channels, int w, int h, const void* image, long ipitch,
                     const uint8_t* alpha, long apitch, int otype, void* out, long opitch, int* out_ctype);
+
+/* ---- TIFF layer files (host side of import() and checkpoint()) -------- */
+
+/* What vigra::ImageImportInfo reports for one directory (layer) of a TIFF
+ * file as enblend's import() reads it (src/assemble.h:231-289): size, bands
+ * (extra samples, i.e. alpha, included and last), the component type, the
+ * canvas position (XPosition / YPosition times the resolution, rounded) and
+ * the canvas size of the PIXAR_IMAGEFULLWIDTH/LENGTH tags (0 when absent). */
+typedef struct enbl_tiff_layer_info {
+    int w, h;
+    int bands;          /* samples per pixel */
+    int extra;          /* extra samples (alpha) among them */
+    int ftype;          /* ENBL_U8, ENBL_U16, ENBL_U32 or ENBL_F32 */
+    int x, y;           /* position on the canvas */
+    int canvas_w, canvas_h;
+    int compression;    /* TIFF Compression tag */
+    int pages;          /* directories in the file */
+} enbl_tiff_layer_info;
+
+/* One layer to write: interleaved host samples, alpha bands last. */
+typedef struct enbl_tiff_page {
+    int w, h;
+    int ftype;          /* ENBL_U8, ENBL_U16, ENBL_U32 or ENBL_F32 */
+    int bands, extra;
+    const void* data;   /* host pointer */
+    long pitch;         /* elements per row */
+    int x, y;           /* canvas position (>= 0) */
+    int canvas_w, canvas_h;  /* 0: no canvas tags */
+} enbl_tiff_page;
+
+/* TIFF compression codes accepted by enbl_tiff_write */
+#define ENBL_TIFF_NONE 1
+#define ENBL_TIFF_LZW 5
+#define ENBL_TIFF_DEFLATE 8
+#define ENBL_TIFF_PACKBITS 32773
+
+/* Header of directory `page` (vigra::ImageImportInfo, src/assemble.h:231-289).
+ * Reads uncompressed, PackBits, LZW and Deflate strips or tiles, chunky or
+ * planar, either byte order, predictors 1-3. */
+int enbl_tiff_info(const char* path, int page, enbl_tiff_layer_info* info);
+
+/* The samples of directory `page`, interleaved as vigra::importImageAlpha
+ * hands them over (colour bands, then alpha), into the host buffer `buf`
+ * (pitch >= w * bands elements).  enbl_assemble takes them from there once
+ * they are on the device. */
+int enbl_tiff_read(const char* path, int page, void* buf, long pitch);
+
+/* Writes `npages` layers as one little-endian TIFF (one directory each):
+ * what vigra::exportImageAlpha does with checkpoint()'s buffer
+ * (src/assemble.h:130-224).  compression is one of ENBL_TIFF_*; LZW and
+ * Deflate use horizontal differencing on integer samples.  Alpha bands are
+ * written as unassociated alpha. */
+int enbl_tiff_write(const char* path, int npages, const enbl_tiff_page* pages, int compression);
 
 #ifdef __cplusplus
 }

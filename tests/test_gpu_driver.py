@@ -330,3 +330,53 @@ def test_checkpoint_rejects_overflowing_conversion(enbl, gpu_ctx):
     ct = ctypes.c_int(-1)
     rc = enbl.lib().enbl_checkpoint(gpu_ctx.h, U8, 3, 2, 2, _vp(d), 6, _vp(d), 2, U16, _vp(o), 8, ctypes.byref(ct))
     assert rc == enbl.ENBL_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("itype,otype", [(U8, U8), (U16, U16), (U16, U8), (F32, U16)])
+def test_tiff_layers_to_tiff_output(enbl, gpu_ctx, oracle, itype, otype, tmp_path):
+    """enblend's whole file path: layers in one multi-directory TIFF (each with
+    its canvas position), decoded on the host, blended on the device, the
+    checkpoint conversion, written as TIFF and read back -- against the
+    oracle's loop and checkpoint over the same decoded layers."""
+    print(f"tiff pipeline itype={itype} otype={otype}", flush=True)
+    rng = np.random.default_rng(700 + itype + otype)
+    uw, uh = 520, 200
+    specs = [(200, 260, 0, 0), (190, 250, 200, 8), (120, 150, 360, 60)]
+    src = str(tmp_path / "layers.tif")
+    enbl.tiff_write(src, [(_layer(rng, h, w, itype), 1, x, y, uw, uh) for h, w, x, y in specs], enbl.TIFF_LZW)
+    layers = []
+    for k in range(enbl.tiff_info(src)["pages"]):
+        a, info = enbl.tiff_read(src, k)
+        assert (info["canvas_w"], info["canvas_h"], info["ftype"]) == (uw, uh, itype)
+        layers.append((a, info["x"], info["y"], info["ftype"]))
+    seed = 3
+    taken = [0] * len(layers)
+    _, black, ba, bbb, taken = oracle.assemble(layers, itype, 3, uw, uh, taken, False)
+    m = 0
+    while not all(taken):
+        _, white, wa, wbb, taken = oracle.assemble(layers, itype, 3, uw, uh, taken, False)
+        rc, _, black, ba, bbb, _ = oracle.enblend_step(black, ba, bbb, white, wa, wbb, itype, seed=seed + m)
+        assert rc == 0
+        m += 1
+    eout, ect = oracle.checkpoint(black, ba, itype, otype)
+
+    tab = enbl.layer_table([(_dev(a), x, y, ft) for a, x, y, ft in layers])
+    d_b = torch.zeros((uh, uw * 3), dtype=getattr(torch, np.dtype(NP[itype]).name), device="cuda")
+    d_ba = torch.zeros((uh, uw), dtype=torch.uint8, device="cuda")
+    bb = enbl.Rect()
+    torch.cuda.synchronize()
+    enbl.check(enbl.lib().enbl_enblend_layers(gpu_ctx.h, itype, 3, uw, uh, len(layers), tab, 1, 0, 0, 1, 2, 1, seed,
+                                              _vp(d_b), uw * 3, _vp(d_ba), uw, ctypes.byref(bb), None))
+    tdt = {U8: torch.uint8, U16: torch.uint16, F32: torch.float32, U32: torch.uint32}[ect]
+    d_o = torch.zeros((uh, uw * 4), dtype=tdt, device="cuda")
+    ct = ctypes.c_int(-1)
+    enbl.check(enbl.lib().enbl_checkpoint(gpu_ctx.h, itype, 3, uw, uh, _vp(d_b), uw * 3, _vp(d_ba), uw, otype,
+                                          _vp(d_o), uw * 4, ctypes.byref(ct)))
+    gpu_ctx.sync()
+    assert ct.value == ect
+    out = d_o.cpu().numpy().reshape(uh, uw, 4)
+    dst = str(tmp_path / "out.tif")
+    enbl.tiff_write(dst, [(out, 1, 0, 0)], enbl.TIFF_DEFLATE)
+    back, info = enbl.tiff_read(dst)
+    assert info["extra"] == 1 and info["ftype"] == ect
+    np.testing.assert_array_equal(back, eout)
