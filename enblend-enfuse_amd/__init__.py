@@ -62,6 +62,8 @@ _SIGS = {
     "enbl_copy_to_pyramid": (_c_int, [_vp, _c_int, _c_int, _vp, _c_long, _vp]),
     "enbl_set_blend_colorspace": (_c_int, [_vp, _c_int]),
     "enbl_rgb_to_blend_space": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long]),
+    "enbl_rgb_to_jch": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _c_long]),
+    "enbl_jch_to_rgb_if": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _c_long, _vp, _c_long, _vp, _c_long]),
     "enbl_blend_space_to_rgb_if": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp,
                                             _c_long]),
     "enbl_gaussian_pyramid": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_long, _vp, _c_long, _vp]),

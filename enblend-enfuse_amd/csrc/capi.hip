@@ -1102,13 +1102,58 @@ int enblend_pair_rgb(enbl_context* ctx, int itype, int C, int w, int h, int leve
                      const void* white, long wpitch, const uint8_t* walpha, long wapitch,
                      void* black, long bpitch, uint8_t* balpha, long bapitch,
                      const uint8_t* mask, long mpitch, int dither, uint64_t seed);
+
+// The step in the CIECAM02 blend space: the JCh level 0 carries fraction
+// bits, so it is written straight into each image's pyramid
+// (copyToPyramidImage with ConvertVectorToJCHPyramidFunctor), the levels
+// follow level by level, and the collapsed level 0 goes back to RGB under
+// the union alpha (copyFromPyramidImageIf with
+// ConvertJCHPyramidToVectorFunctor; no dither: the pyramid is read as
+// double).  src/enblend.h:378-510, src/fixmath.h:2018-2294.
+int enblend_pair_jch(enbl_context* ctx, int itype, int w, int h, int levels, int wrap,
+                     const void* white, long wpitch, const uint8_t* walpha, long wapitch,
+                     void* black, long bpitch, uint8_t* balpha, long bapitch, const uint8_t* mask, long mpitch)
+{
+    const int pt = image_ptype_for(itype), mt = mask_ptype_for(itype);
+    const int mfrac = itype == ENBL_U8 ? 7 : 15;
+    enbl_pyramid *mgp = nullptr, *wlp = nullptr, *blp = nullptr;
+    int rc = cached_pyramid(ctx, kRoleMask, mt, 1, w, h, levels, 0, &mgp);
+    if (!rc) rc = cached_pyramid(ctx, kRoleImage, pt, 3, w, h, levels, 0, &wlp);
+    if (!rc) rc = cached_pyramid(ctx, kRoleImage, pt, 3, w, h, levels, 1, &blp);
+    if (rc) return rc;
+    rc = enbl_gaussian_pyramid(ctx, wrap, ENBL_U8, mfrac, mask, mpitch, nullptr, 0, mgp);
+    if (rc) return rc;
+    const void* img[2] = {white, black};
+    const long ip[2] = {wpitch, bpitch};
+    const uint8_t* al[2] = {walpha, balpha};
+    const long ap[2] = {wapitch, bapitch};
+    enbl_pyramid* lp[2] = {wlp, blp};
+    for (int k = 0; k < 2; ++k) {
+        const Level& L0 = lp[k]->lv[0];
+        rc = rgb_to_jch(ctx, itype, w, h, img[k], ip[k], level_ptr(lp[k], 0), L0.pitch, L0.plane);
+        if (!rc) rc = gaussian_levels(ctx, lp[k], wrap, al[k], ap[k]);
+        if (!rc) rc = laplacian_levels(ctx, lp[k], wrap);
+        if (rc) return rc;
+    }
+    {
+        Timed t(ctx, "alpha_union", 3.0 * w * h);
+        launch_alpha_union(walpha, wapitch, balpha, bapitch, w, h, ctx->stream);
+        rc = launch_status("alpha_union");
+        if (rc) return rc;
+    }
+    rc = enbl_blend(ctx, mgp, wlp, blp, (double)(255 << mfrac));
+    if (!rc) rc = enbl_collapse_pyramid(ctx, wrap, blp);
+    if (rc) return rc;
+    const Level& B0 = blp->lv[0];
+    return jch_to_rgb_if(ctx, itype, w, h, level_ptr(blp, 0), B0.pitch, B0.plane, balpha, bapitch, black, bpitch);
+}
 }  // namespace
 
 // One blend step; in a CIELAB / CIELUV blend space both RGB images are
 // converted into scaled blend-space images first, the step runs on those,
 // and its result goes back to RGB under the union alpha, as the reference's
 // converters around copyToPyramidImage / copyFromPyramidImageIf do
-// (src/fixmath.h:2018-2294).
+// (src/fixmath.h:2018-2294); CIECAM02 takes enblend_pair_jch.
 int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int levels, int wrap,
                       const void* white, long wpitch, const uint8_t* walpha, long wapitch,
                       void* black, long bpitch, uint8_t* balpha, long bapitch,
@@ -1122,6 +1167,9 @@ int enbl_enblend_pair(enbl_context* ctx, int itype, int C, int w, int h, int lev
     int rc = check_blend_space(cs, itype, C);
     if (rc) return rc;
     if (w < 1 || h < 1) return fail(ENBL_E_SIZE, "enbl_enblend_pair: empty image");
+    if (cs == ENBL_COLORSPACE_CIECAM)
+        return enblend_pair_jch(ctx, itype, w, h, levels, wrap, white, wpitch, walpha, wapitch, black, bpitch, balpha,
+                                bapitch, mask, mpitch);
     const size_t es = itype == ENBL_U8 ? 1 : 2, plane = (size_t)w * h * 3 * es;
     void* buf;
     rc = scratch(ctx, 32, 2 * plane, &buf);

@@ -137,6 +137,11 @@ int rgb_to_blend_space(enbl_context* ctx, int space, int itype, int w, int h, co
                        long op);
 int blend_space_to_rgb_if(enbl_context* ctx, int space, int itype, int w, int h, const void* src, long sp,
                           const uint8_t* mask, long mp, void* img, long ip);
+// CIECAM02: RGB into the planar JCh pyramid level 0 (pitch op, channel
+// planes opl apart), and a collapsed JCh level 0 back to RGB under a mask.
+int rgb_to_jch(enbl_context* ctx, int itype, int w, int h, const void* img, long ip, void* out, long op, long opl);
+int jch_to_rgb_if(enbl_context* ctx, int itype, int w, int h, const void* lv, long lp, long lpl, const uint8_t* mask,
+                  long mp, void* img, long ip);
 int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, const void* img, long ipitch,
                           const uint8_t* alpha, long apitch, enbl_pyramid* out);
 // enfuse's K Laplacian pyramids (image alphas) and K mask Gaussian pyramids

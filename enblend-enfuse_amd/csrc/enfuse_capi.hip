@@ -494,7 +494,10 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     std::vector<const void*> pimgs(imgs, imgs + K);
     std::vector<long> pips(ips, ips + K);
     void* csout = nullptr;
-    if (cs != ENBL_COLORSPACE_IDENTITY) {
+    // CIECAM02: the JCh level 0 (with fraction bits) goes straight into each
+    // image's pyramid and the result's level 0 back to RGB.
+    const bool jch = cs == ENBL_COLORSPACE_CIECAM;
+    if (cs == ENBL_COLORSPACE_CIELAB || cs == ENBL_COLORSPACE_CIELUV) {
         const size_t cplane = (size_t)w * h * 3 * (itype == ENBL_U8 ? 1 : 2);
         void* cbuf;
         rc = scratch(ctx, 33, cplane * (size_t)K, &cbuf);
@@ -538,12 +541,25 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     // are computed on the main one (memory-bound passes beside the
     // arithmetic-bound weight kernels); the main stream waits for them
     // before the merge, and on every error path.
+    auto jch_laplacian = [&](int m) {
+        enbl_pyramid* lp = lps[(size_t)m];
+        const Level& L0 = lp->lv[0];
+        int r = rgb_to_jch(ctx, itype, w, h, imgs[m], ips[m], level_ptr(lp, 0), L0.pitch, L0.plane);
+        if (!r) r = gaussian_levels(ctx, lp, wrap, alphas[m], aps[m]);
+        if (!r) r = laplacian_levels(ctx, lp, wrap);
+        return r;
+    };
     struct AuxJoin {
         enbl_context* ctx;
         bool active = false;
         ~AuxJoin() { if (active) (void)join_aux(ctx); }
     } aux{ctx};
-    if (!wrap) {
+    if (!wrap && jch) {
+        // (level by level on the main stream: these reduces share the
+        // context's alpha scratch with the mask pyramids)
+        for (int m = 0; m < K && !rc; ++m) rc = jch_laplacian(m);
+        if (rc) return rc;
+    } else if (!wrap) {
         rc = fork_aux(ctx);
         if (rc) return rc;
         aux.active = true;
@@ -596,8 +612,9 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         if (rc) return rc;
     } else {
         for (int m = 0; m < K; ++m) {
-            rc = enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, pimgs[(size_t)m], pips[(size_t)m], alphas[m], aps[m],
-                                        lps[(size_t)m]);
+            rc = jch ? jch_laplacian(m)
+                     : enbl_laplacian_pyramid(ctx, wrap, itype, ifrac, pimgs[(size_t)m], pips[(size_t)m], alphas[m],
+                                              aps[m], lps[(size_t)m]);
             if (!rc) rc = normalize(m);
             if (!rc) rc = gaussian_levels(ctx, mgps[(size_t)m], wrap, (const uint8_t*)ua, pitch);
             if (rc) return rc;
@@ -632,7 +649,10 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
     }
     rc = enbl_collapse_pyramid(ctx, wrap, res);
     if (rc) return rc;
-    if (cs != ENBL_COLORSPACE_IDENTITY) {
+    if (jch) {
+        const Level& R0 = res->lv[0];
+        rc = jch_to_rgb_if(ctx, itype, w, h, level_ptr(res, 0), R0.pitch, R0.plane, (const uint8_t*)ua, pitch, out, op);
+    } else if (cs != ENBL_COLORSPACE_IDENTITY) {
         rc = enbl_copy_from_pyramid_if(ctx, res, 0, ifrac, (const uint8_t*)ua, pitch, csout, itype, 3L * w, dither,
                                        seed);
         if (!rc) rc = blend_space_to_rgb_if(ctx, cs, itype, w, h, csout, 3L * w, (const uint8_t*)ua, pitch, out, op);

@@ -421,7 +421,7 @@ int enbl_nft_mask_periodic(enbl_context* ctx, int w, int h, const uint8_t* white
 #define ENBL_COLORSPACE_IDENTITY 0
 #define ENBL_COLORSPACE_CIELAB 1
 #define ENBL_COLORSPACE_CIELUV 2
-#define ENBL_COLORSPACE_CIECAM 3   /* CIECAM02: ENBL_E_UNSUPPORTED */
+#define ENBL_COLORSPACE_CIECAM 3   /* CIECAM02 JCh */
 
 /* Sets the context's blend colour space for enbl_enblend_pair,
  * enbl_enblend_step, enbl_enblend_layers, enbl_enfuse and
@@ -429,7 +429,10 @@ int enbl_nft_mask_periodic(enbl_context* ctx, int w, int h, const uint8_t* white
  * 8- and 16-bit RGB images as the reference's
  * ConvertVectorTo{Lab,Luv}PyramidFunctor and Convert{Lab,Luv}PyramidTo-
  * VectorFunctor do (src/fixmath.h:429-1330; sRGB input profile, the
- * reference's assumption for images without an ICC profile); enfuse's
+ * reference's assumption for images without an ICC profile); with CIECAM as
+ * ConvertVectorToJCHPyramidFunctor / ConvertJCHPyramidToVectorFunctor
+ * (src/fixmath.h:1336-1990: CIECAM02 under the viewing conditions of
+ * src/enblend.cc:2235-2244, JCh in the pyramid as fixed point); enfuse's
  * weights stay computed on the RGB images.  Float RGB images with a
  * non-identity space give ENBL_E_UNSUPPORTED; one-channel images are not
  * converted (the reference's scalar path ignores the space). */
@@ -450,6 +453,21 @@ int enbl_rgb_to_blend_space(enbl_context* ctx, int space, int itype, int w, int 
  * numeric minimiser, src/fixmath.h:835-1045). */
 int enbl_blend_space_to_rgb_if(enbl_context* ctx, int space, int itype, int w, int h, const void* src,
                                long spitch, const uint8_t* mask, long mpitch, void* img, long ipitch);
+
+/* CIECAM02 (ConvertVectorToJCHPyramidFunctor, src/fixmath.h:1406-1464):
+ * interleaved 8- or 16-bit RGB `img` into the JCh pyramid level 0 `out`,
+ * three planes `oplane` elements apart of Int16 (8-bit) or Int32 (16-bit)
+ * fixed point with 7 fraction bits: (J s, C sin h s, C cos h s) with
+ * s = 2^(IntegerBits - 8) = 2 for 8-bit, 512 for 16-bit images. */
+int enbl_rgb_to_jch(enbl_context* ctx, int itype, int w, int h, const void* img, long ipitch, void* out, long opitch,
+                    long oplane);
+
+/* ConvertJCHPyramidToVectorFunctor (src/fixmath.h:1854-1977): a JCh level
+ * 0 (planar, as enbl_rgb_to_jch writes it) into RGB `img` where mask != 0;
+ * J <= 0 gives black; out-of-gamut colours are clamped (the reference
+ * searches a close in-gamut colour with GSL minimisers). */
+int enbl_jch_to_rgb_if(enbl_context* ctx, int itype, int w, int h, const void* lv, long lpitch, long lplane,
+                       const uint8_t* mask, long mpitch, void* img, long ipitch);
 
 /* ---- pipelines ---------------------------------------------------------- */
 

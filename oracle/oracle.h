@@ -173,6 +173,7 @@ int orc_euclid_1d(const float* f, int n, int planted, float* d);
 #define ORC_COLORSPACE_IDENTITY 0
 #define ORC_COLORSPACE_CIELAB 1
 #define ORC_COLORSPACE_CIELUV 2
+#define ORC_COLORSPACE_CIECAM 3
 int orc_set_blend_colorspace(int space);
 int orc_blend_colorspace(void);
 int orc_rgb_to_blend_space(int space, int itype, int w, int h, const void* img, void* out);
@@ -180,6 +181,12 @@ int orc_blend_space_to_rgb_if(int space, int itype, int w, int h, const void* sr
 int orc_colorspace_forward(int space, const double* rgb, long n, double* out);
 int orc_colorspace_inverse(int space, const double* v, long n, double* rgb);
 void orc_rgb_to_xyz_matrix(double* m);
+/* CIECAM02 with the reference's viewing conditions, and the JCh pyramid
+ * level 0 (src/fixmath.h:1336-1990). */
+int orc_ciecam_forward(const double* xyz, long n, double* jch);
+int orc_ciecam_reverse(const double* jch, long n, double* xyz);
+int orc_rgb_to_jch_level0(int itype, int w, int h, const void* img, void* out, long plane);
+int orc_jch_level0_to_rgb_if(int itype, int w, int h, const void* lv, long plane, const uint8_t* mask, void* img);
 
 int orc_distance_transform(const uint8_t* src, long spitch, int w, int h, int norm, float* dst, long dpitch);
 

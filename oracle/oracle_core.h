@@ -583,9 +583,18 @@ void collapse_levels(P* buf, const std::vector<Dims>& d, int C, bool wrap)
     }
 }
 
+template <class I> constexpr int itype_code() { return sizeof(I) == 1 ? ORC_U8 : ORC_U16; }
+
+// copyToPyramidImage (src/fixmath.h:2018-2115): an RGB image in the CIECAM02
+// blend space enters as JCh (ConvertVectorToJCHPyramidFunctor), anything
+// else through ConvertScalarToPyramidFunctor.
 template <class I, class P>
 void image_to_level0(int frac, int C, const I* img, int w, int h, P* out)
 {
+    if (C == 3 && orc_blend_colorspace() == ORC_COLORSPACE_CIECAM) {
+        orc_rgb_to_jch_level0(itype_code<I>(), w, h, img, out, (long)w * h);
+        return;
+    }
     to_pyramid_int<I, P>(frac, C, img, w, h, (long)w * C, out, w, (long)w * h);
 }
 
@@ -594,10 +603,15 @@ inline void image_to_level0(int, int C, const float* img, int w, int h, double* 
     to_pyramid_log(C, img, w, h, (long)w * C, out, w, (long)w * h);
 }
 
+// copyFromPyramidImageIf (src/fixmath.h:2157-2294), JCh likewise.
 template <class P, class I>
 void level0_to_image(int frac, int C, const P* lv, int w, int h, const uint8_t* mask, I* img,
                      int dither, uint64_t seed)
 {
+    if (C == 3 && orc_blend_colorspace() == ORC_COLORSPACE_CIECAM) {
+        orc_jch_level0_to_rgb_if(itype_code<I>(), w, h, lv, (long)w * h, mask, img);
+        return;
+    }
     from_pyramid_int<P, I>(frac, C, lv, w, h, w, (long)w * h, mask, w, img, (long)w * C, dither, seed);
 }
 

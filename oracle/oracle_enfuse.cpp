@@ -564,7 +564,10 @@ int enfuse_t(int w, int h, int K, const T* const* imgs, const uint8_t* const* al
     // A CIELAB / CIELUV blend space: the pyramids are built from the images
     // converted to the scaled blend space, the result goes back to RGB under
     // the union alpha (src/fixmath.h:2018-2294); the weights above used RGB.
-    const int cs = std::is_same<T, float>::value ? ORC_COLORSPACE_IDENTITY : orc_blend_colorspace();
+    // (CIECAM02 enters and leaves the pyramids through image_to_level0 /
+    // level0_to_image themselves.)
+    const int cs_all = std::is_same<T, float>::value ? ORC_COLORSPACE_IDENTITY : orc_blend_colorspace();
+    const int cs = cs_all == ORC_COLORSPACE_CIECAM ? ORC_COLORSPACE_IDENTITY : cs_all;
     const int cstype = sizeof(T) == 1 ? ORC_U8 : ORC_U16;
     std::vector<T> scaled(cs != ORC_COLORSPACE_IDENTITY ? (size_t)N * 3 : 0);
     for (int m = 0; m < K; ++m) {

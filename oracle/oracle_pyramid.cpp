@@ -119,7 +119,8 @@ int orc_enblend_pair(int itype, int C, int w, int h, int levels, int wrap,
 {
     if (levels < 1 || levels > 29) return -1;
     const int cs = orc_blend_colorspace();
-    if (cs != ORC_COLORSPACE_IDENTITY && C == 3) {
+    if (C == 3 && cs == ORC_COLORSPACE_CIECAM && itype != ORC_U8 && itype != ORC_U16) return -4;
+    if ((cs == ORC_COLORSPACE_CIELAB || cs == ORC_COLORSPACE_CIELUV) && C == 3) {
         // both images into the scaled blend space, the step there, the
         // result back to RGB under the union alpha
         if (itype != ORC_U8 && itype != ORC_U16) return -4;

@@ -41,6 +41,10 @@ _SIGS = {
     "orc_rgb_to_blend_space": [_i, _i, _i, _i, _vp, _vp],
     "orc_blend_space_to_rgb_if": [_i, _i, _i, _i, _vp, _vp, _vp],
     "orc_colorspace_forward": [_i, _vp, _l, _vp],
+    "orc_ciecam_forward": [_vp, _l, _vp],
+    "orc_ciecam_reverse": [_vp, _l, _vp],
+    "orc_rgb_to_jch_level0": [_i, _i, _i, _vp, _vp, _l],
+    "orc_jch_level0_to_rgb_if": [_i, _i, _i, _vp, _l, _vp, _vp],
     "orc_colorspace_inverse": [_i, _vp, _l, _vp],
     "orc_rgb_to_xyz_matrix": [_vp],
     "orc_local_stddev_f32": [_vp, _vp, _i, _i, _i, _i, _vp],
@@ -386,6 +390,40 @@ def _colorspace_inverse(self, space, v):
     return out
 
 
+def _ciecam_forward(self, xyz):
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    out = np.zeros_like(xyz)
+    self.lib.orc_ciecam_forward(_p(xyz), len(xyz), _p(out))
+    return out
+
+
+def _ciecam_reverse(self, jch):
+    jch = np.ascontiguousarray(jch, np.float64).reshape(-1, 3)
+    out = np.zeros_like(jch)
+    self.lib.orc_ciecam_reverse(_p(jch), len(jch), _p(out))
+    return out
+
+
+def _rgb_to_jch_level0(self, img, itype):
+    """JCh pyramid level 0 of an RGB image: planar (3, h, w), int16 / int32."""
+    img = np.ascontiguousarray(img, NP_I[itype])
+    h, w, _ = img.shape
+    out = np.zeros((3, h, w), np.int16 if itype == U8 else np.int32)
+    if self.lib.orc_rgb_to_jch_level0(itype, w, h, _p(img), _p(out), w * h):
+        raise ValueError("orc_rgb_to_jch_level0")
+    return out
+
+
+def _jch_level0_to_rgb_if(self, lv, mask, img, itype):
+    lv = np.ascontiguousarray(lv, np.int16 if itype == U8 else np.int32)
+    mask = np.ascontiguousarray(mask, np.uint8)
+    out = np.array(img, NP_I[itype], copy=True)
+    _, h, w = lv.shape
+    if self.lib.orc_jch_level0_to_rgb_if(itype, w, h, _p(lv), w * h, _p(mask), _p(out)):
+        raise ValueError("orc_jch_level0_to_rgb_if")
+    return out
+
+
 def _rgb_to_xyz_matrix(self):
     m = np.zeros(9, np.float64)
     self.lib.orc_rgb_to_xyz_matrix(_p(m))
@@ -486,6 +524,10 @@ Oracle.blend_space_to_rgb_if = _blend_space_to_rgb_if
 Oracle.colorspace_forward = _colorspace_forward
 Oracle.colorspace_inverse = _colorspace_inverse
 Oracle.rgb_to_xyz_matrix = _rgb_to_xyz_matrix
+Oracle.ciecam_forward = _ciecam_forward
+Oracle.ciecam_reverse = _ciecam_reverse
+Oracle.rgb_to_jch_level0 = _rgb_to_jch_level0
+Oracle.jch_level0_to_rgb_if = _jch_level0_to_rgb_if
 Oracle.local_stddev_f32 = _local_stddev_f32
 Oracle.checkpoint = _checkpoint
 def _nft_mask_periodic(self, white, black, boundary, norm=2):

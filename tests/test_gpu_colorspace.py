@@ -1,5 +1,5 @@
-"""GPU parity of the blend colour spaces (--blend-colorspace=CIELAB/CIELUV,
-src/fixmath.h:429-1330) against the oracle, through the C ABI: the
+"""GPU parity of the blend colour spaces (--blend-colorspace=CIELAB/CIELUV/
+CIECAM, src/fixmath.h:429-1990) against the oracle, through the C ABI: the
 conversions alone and the enblend pair, enblend step and enfuse pipelines
 with a colour space set on the context.  Integer images: bit-exact."""
 import ctypes
@@ -7,7 +7,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from oracle_binding import CS_CIELAB, CS_CIELUV, F32, U8, U16, enfuse_params as orc_params
+from oracle_binding import CS_CIECAM, CS_CIELAB, CS_CIELUV, F32, U8, U16, enfuse_params as orc_params
 
 pytestmark = pytest.mark.gpu
 
@@ -15,7 +15,8 @@ torch = pytest.importorskip("torch")
 
 NP = {U8: np.uint8, U16: np.uint16, F32: np.float32}
 MAXV = {U8: 255, U16: 65535}
-SPACES = [CS_CIELAB, CS_CIELUV]
+SPACES = [CS_CIELAB, CS_CIELUV, CS_CIECAM]
+PTYPE = {U8: np.int16, U16: np.int32}
 
 
 def _vp(t):
@@ -45,7 +46,7 @@ def space_ctx(enbl, gpu_ctx, oracle):
 
 
 @pytest.mark.parametrize("itype", [U8, U16])
-@pytest.mark.parametrize("space", SPACES)
+@pytest.mark.parametrize("space", [CS_CIELAB, CS_CIELUV])
 def test_rgb_to_blend_space_matches_oracle(enbl, gpu_ctx, oracle, itype, space):
     rng = np.random.default_rng(900 + itype + space)
     h, w = 301, 457
@@ -63,7 +64,7 @@ def test_rgb_to_blend_space_matches_oracle(enbl, gpu_ctx, oracle, itype, space):
 
 
 @pytest.mark.parametrize("itype", [U8, U16])
-@pytest.mark.parametrize("space", SPACES)
+@pytest.mark.parametrize("space", [CS_CIELAB, CS_CIELUV])
 def test_blend_space_to_rgb_if_matches_oracle(enbl, gpu_ctx, oracle, itype, space):
     """Scaled values from the forward conversion and arbitrary ones (out of
     gamut, clamped), under a random mask over a random image."""
@@ -148,9 +149,50 @@ def test_grayscale_pair_ignores_blend_space(enbl, gpu_ctx, oracle, space_ctx, it
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_rgb_to_jch_matches_oracle(enbl, gpu_ctx, oracle, itype):
+    """The CIECAM02 JCh level 0, written into planes at a padded pitch."""
+    rng = np.random.default_rng(1400 + itype)
+    h, w = 257, 391
+    img = _rgb(rng, itype, h, w)
+    want = oracle.rgb_to_jch_level0(img, itype)
+    pitch, plane = w + 13, (w + 13) * (h + 2)
+    dimg = torch.from_numpy(img).cuda()
+    dout = torch.full((3 * plane,), 7, dtype=getattr(torch, str(np.dtype(PTYPE[itype]))), device="cuda")
+    torch.cuda.synchronize()
+    print(f"rgb_to_jch itype={itype} {w}x{h}", flush=True)
+    enbl.check(enbl.lib().enbl_rgb_to_jch(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(dout), pitch, plane))
+    gpu_ctx.sync()
+    got = dout.cpu().numpy()
+    for c in range(3):
+        np.testing.assert_array_equal(got[c * plane:c * plane + h * pitch].reshape(h, pitch)[:, :w], want[c])
+    # white: J = 100 exactly, so the J plane holds 100 s 2^7
+    assert want[0][0, 0] == (25600 if itype == U8 else 6553600)
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_jch_to_rgb_if_matches_oracle(enbl, gpu_ctx, oracle, itype):
+    """Levels from the forward conversion and arbitrary ones (J <= 0,
+    out of gamut: clamped), under a random mask over a random image."""
+    rng = np.random.default_rng(1450 + itype)
+    h, w = 203, 311
+    lv = oracle.rgb_to_jch_level0(_rgb(rng, itype, h, w), itype)
+    s = (2 if itype == U8 else 512) * 128
+    lv[:, h // 2:] = rng.integers(-40 * s, 120 * s, size=lv[:, h // 2:].shape).astype(PTYPE[itype])
+    mask = np.where(rng.random((h, w)) < 0.6, 255, 0).astype(np.uint8)
+    img = _rgb(rng, itype, h, w)
+    want = oracle.jch_level0_to_rgb_if(lv, mask, img, itype)
+    dlv, dm, dimg = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (lv, mask, img))
+    torch.cuda.synchronize()
+    print(f"jch_to_rgb_if itype={itype} {w}x{h}", flush=True)
+    enbl.check(enbl.lib().enbl_jch_to_rgb_if(gpu_ctx.h, itype, w, h, _vp(dlv), w, w * h, _vp(dm), w, _vp(dimg),
+                                             3 * w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dimg.cpu().numpy(), want)
+
+
 def test_unsupported_spaces_and_types(enbl, gpu_ctx, oracle, space_ctx):
     lib = enbl.lib()
-    assert lib.enbl_set_blend_colorspace(gpu_ctx.h, enbl.CS_CIECAM) == enbl.ENBL_E_UNSUPPORTED
     assert lib.enbl_set_blend_colorspace(gpu_ctx.h, 7) == enbl.ENBL_E_ARG
     h, w = 40, 64
     rng = np.random.default_rng(3)
@@ -168,6 +210,13 @@ def test_unsupported_spaces_and_types(enbl, gpu_ctx, oracle, space_ctx):
         == enbl.ENBL_E_ARG
     assert lib.enbl_rgb_to_blend_space(gpu_ctx.h, CS_CIELAB, U16, w, h, _vp(d16), 3 * w - 1, _vp(d16), 3 * w) \
         == enbl.ENBL_E_SIZE
+    # JCh lives in pyramid levels, not in images of the image type
+    assert lib.enbl_rgb_to_blend_space(gpu_ctx.h, CS_CIECAM, U16, w, h, _vp(d16), 3 * w, _vp(d16), 3 * w) \
+        == enbl.ENBL_E_ARG
+    assert lib.enbl_rgb_to_jch(gpu_ctx.h, F32, w, h, _vp(d), 3 * w, _vp(d), w, w * h) == enbl.ENBL_E_UNSUPPORTED
+    space_ctx(CS_CIECAM)
+    rc, got, _ = _gpu_pair(enbl, gpu_ctx, F32, 3, 3, white, wa, black, ba, mask, 1)
+    assert rc == enbl.ENBL_E_UNSUPPORTED
     gpu_ctx.sync()
 
 

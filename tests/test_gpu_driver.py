@@ -360,7 +360,8 @@ def test_tiff_layers_to_tiff_output(enbl, gpu_ctx, oracle, itype, otype, tmp_pat
         m += 1
     eout, ect = oracle.checkpoint(black, ba, itype, otype)
 
-    tab = enbl.layer_table([(_dev(a), x, y, ft) for a, x, y, ft in layers])
+    dl = [(_dev(a), x, y, ft) for a, x, y, ft in layers]   # kept alive while the device reads them
+    tab = enbl.layer_table(dl)
     d_b = torch.zeros((uh, uw * 3), dtype=getattr(torch, np.dtype(NP[itype]).name), device="cuda")
     d_ba = torch.zeros((uh, uw), dtype=torch.uint8, device="cuda")
     bb = enbl.Rect()

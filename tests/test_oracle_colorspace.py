@@ -168,3 +168,68 @@ def test_16bit_codes_against_lcms(orc):
     back = orc.blend_space_to_rgb_if(CS_CIELAB, FIX["q16"].reshape(1, -1, 3), mask, np.zeros_like(codes), U16)
     d = np.abs(back.reshape(-1, 3).astype(np.int64) - _from_real_u16(FIX["rgb_from_q16"]))
     assert d.max() <= 1 and np.mean(d == 0) > 0.995
+
+
+# ---- CIECAM02 (JCh blend space, src/fixmath.h:1336-1990) ---------------------
+
+CAM = np.load(os.path.join(HERE, "golden", "ciecam_lcms.npz"))
+
+
+def test_ciecam_viewing_conditions_are_the_references():
+    """The fixture was written with the reference's viewing conditions
+    (src/enblend.cc:2235-2244): white 100 x D50."""
+    np.testing.assert_allclose(CAM["white"], 100.0 * D50, rtol=1e-4)
+
+
+def test_ciecam_forward_matches_lcms(orc):
+    """LittleCMS's own cmsCIECAM02Forward on the same XYZ: J to 1e-12,
+    C and h to 1e-9 relative (where C is not tiny)."""
+    got = orc.ciecam_forward(CAM["xyz"])
+    ref = CAM["jch"]
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 1e-12
+    m = ref[:, 1] > 1.0
+    assert (np.abs(got[m, 1:] - ref[m, 1:]) / ref[m, 1:]).max() < 1e-9
+    assert np.abs(got[:, 1] - ref[:, 1]).max() < 1e-7
+
+
+def test_ciecam_reverse_matches_lcms(orc):
+    """cmsCIECAM02Reverse over the forward results and over JCh all round the
+    hue circle: equal to 1e-12 relative.  (The inverse HPE matrix entry that
+    LittleCMS evaluates with the opposite sign of the published one, and its
+    pi = 3.141592654, are what brings this to rounding level: with the
+    published sign alone the XYZ differ by about 2e-5.)"""
+    got = orc.ciecam_reverse(CAM["jch_in"])
+    ref = CAM["xyz_back"]
+    assert (np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)).max() < 1e-12
+
+
+def test_ciecam_known_answers(orc):
+    """The adopted white has J = 100 exactly (A = Aw); black has J = C = 0 up
+    to rounding; hue angles lie in [0, 360)."""
+    jw = orc.ciecam_forward(100.0 * D50[None, :])[0]
+    assert jw[0] == 100.0 and jw[1] < 0.01
+    jb = orc.ciecam_forward(np.zeros((1, 3)))[0]
+    assert abs(jb[0]) < 1e-20 and abs(jb[1]) < 1e-6
+    j = orc.ciecam_forward(CAM["xyz"])
+    assert (j[:, 2] >= 0).all() and (j[:, 2] < 360).all()
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+def test_jch_level0_scale_and_round_trip(orc, itype):
+    """PyramidScaleJCh: the J plane of white holds 100 x 2^(IntegerBits-8) x
+    2^7 exactly, black is all zeros; forward then inverse under a full mask
+    gives the codes back within the asymmetry of LittleCMS's forward and
+    reverse models (its reverse is not the exact inverse of its forward)."""
+    m = 255 if itype == U8 else 65535
+    rng = np.random.default_rng(itype)
+    img = rng.integers(0, m + 1, size=(60, 80, 3)).astype(np.uint8 if itype == U8 else np.uint16)
+    img[0, 0] = m
+    img[0, 1] = 0
+    lv = orc.rgb_to_jch_level0(img, itype)
+    s = 2 if itype == U8 else 512
+    assert lv[0, 0, 0] == 100 * s * 128
+    assert (lv[:, 0, 1] == 0).all()
+    back = orc.jch_level0_to_rgb_if(lv, np.full(img.shape[:2], 255, np.uint8), np.zeros_like(img), itype)
+    d = np.abs(back.astype(np.int64) - img.astype(np.int64))
+    assert d.max() <= (1 if itype == U8 else 32), d.max()
+    assert np.median(d) <= (0 if itype == U8 else 8)
