@@ -1049,6 +1049,15 @@ int enbl_laplacian_pyramid(enbl_context* ctx, int wrap, int itype, int frac, con
     return laplacian_pyramid_any(ctx, wrap, itype, frac, img, ipitch, alpha, apitch, out);
 }
 
+int enbl_pyramid_build_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch,
+                              int laplacian)
+{
+    if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_pyramid_build_levels: null argument");
+    int rc = gaussian_levels(ctx, p, wrap, alpha, apitch);
+    if (!rc && laplacian) rc = laplacian_levels(ctx, p, wrap);
+    return rc;
+}
+
 int enbl_collapse_pyramid(enbl_context* ctx, int wrap, enbl_pyramid* p)
 {
     if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_collapse_pyramid: null argument");

@@ -191,6 +191,13 @@ int enbl_laplacian_pyramid(enbl_context* ctx, int wraparound, int itype, int fra
                            const void* img, long ipitch, const uint8_t* alpha, long apitch,
                            enbl_pyramid* out);
 
+/* The levels above 0 of a pyramid whose level 0 is already written (for
+ * instance by enbl_rgb_to_jch): the reduces of gaussianPyramid
+ * (src/pyramid.h:1467-1559; alpha may be NULL) and, with laplacian != 0,
+ * the expand-subtracts of laplacianPyramid (src/pyramid.h:1646-1747). */
+int enbl_pyramid_build_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, const uint8_t* alpha, long apitch,
+                              int laplacian);
+
 /* Replaces collapsePyramid<SKIPSMImagePixelType>(wraparound, p) --
  * src/pyramid.h:1757-1784.  In place; level 0 holds the result. */
 int enbl_collapse_pyramid(enbl_context* ctx, int wraparound, enbl_pyramid* p);

@@ -85,12 +85,12 @@ private:
 
 // The blend colour space (ENBL_COLORSPACE_*), the reference's global
 // BlendColorspace (src/enblend.cc:124, src/enfuse.cc:110): a caller mirrors
-// it where it parses --blend-colorspace.  With CIELAB or CIELUV set, RGB
-// images enter the pyramids converted (copyToPyramidImage, and the
+// it where it parses --blend-colorspace.  With CIELAB, CIELUV or CIECAM set,
+// RGB images enter the pyramids converted (copyToPyramidImage, and the
 // Gaussian and Laplacian pyramids, which start with it) and leave them
 // converted back (copyFromPyramidImageIf), as src/fixmath.h:2060-2116 and
-// :2207-2254 dispatch; one-channel images are never converted.  CIECAM02
-// and float RGB images with a space set throw ENBL_E_UNSUPPORTED.
+// :2207-2254 dispatch; one-channel images are never converted.  Float RGB
+// images with a space set throw ENBL_E_UNSUPPORTED.
 inline int& blend_colorspace()
 {
     static int s = ENBL_COLORSPACE_IDENTITY;
@@ -98,19 +98,24 @@ inline int& blend_colorspace()
 }
 
 inline bool converts(int channels) { return channels == 3 && blend_colorspace() != ENBL_COLORSPACE_IDENTITY; }
+// CIECAM02's JCh enters the pyramid as fixed point with fraction bits: it is
+// written straight into level 0 instead of through a converted image.
+inline bool jch(int channels) { return channels == 3 && blend_colorspace() == ENBL_COLORSPACE_CIECAM; }
 
 // Replaces a staged RGB image (w x h, interleaved, tight rows) by its
-// scaled blend-space image when a blend colour space is set.
+// scaled blend-space image when CIELAB or CIELUV is set.
 inline void enter_blend_space(DeviceBuffer& img, int itype, int w, int h, int channels, size_t bytes)
 {
-    if (!converts(channels)) return;
-    if (blend_colorspace() == ENBL_COLORSPACE_CIECAM)
-        throw error(ENBL_E_UNSUPPORTED, "blend colour space CIECAM02 is not supported");
+    if (!converts(channels) || jch(channels)) return;
     DeviceBuffer conv(bytes);
     check(enbl_rgb_to_blend_space(context(), blend_colorspace(), itype, w, h, img.get(), (long)w * 3, conv.get(),
                                   (long)w * 3));
     img.swap(conv);
 }
+
+class Pyramid;
+inline void build_pyramid(Pyramid& p, int itype, int frac, const DeviceBuffer& img, int C, int w, int h, bool wrap,
+                          const uint8_t* alpha, bool laplacian);
 
 // ---- pixel type mapping ---------------------------------------------------
 template <class T, class = void> struct pixel {
@@ -217,6 +222,26 @@ public:
 private:
     enbl_pyramid* p_ = nullptr;
 };
+
+// The Gaussian (or Laplacian) pyramid of a staged image: from the image (in
+// CIELAB / CIELUV already converted by enter_blend_space), or, in CIECAM02,
+// from its JCh level 0 written straight into the pyramid.
+inline void build_pyramid(Pyramid& p, int itype, int frac, const DeviceBuffer& img, int C, int w, int h, bool wrap,
+                          const uint8_t* alpha, bool laplacian)
+{
+    if (jch(C)) {
+        void* l0;
+        int lw, lh;
+        long lp, lpl;
+        check(enbl_pyramid_level(p.get(), 0, &l0, &lw, &lh, &lp, &lpl));
+        check(enbl_rgb_to_jch(context(), itype, w, h, img.get(), (long)w * C, l0, lp, lpl));
+        check(enbl_pyramid_build_levels(context(), p.get(), wrap, alpha, w, laplacian));
+    } else if (laplacian) {
+        check(enbl_laplacian_pyramid(context(), wrap, itype, frac, img.get(), (long)w * C, alpha, w, p.get()));
+    } else {
+        check(enbl_gaussian_pyramid(context(), wrap, itype, frac, img.get(), (long)w * C, alpha, w, p.get()));
+    }
+}
 
 // Host pyramid levels <-> device pyramid.
 template <class PyramidImageType>
@@ -391,8 +416,8 @@ gaussianPyramid(unsigned int numLevels, bool wraparound,
     enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     dal.upload(al.data(), al.size());
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
-    check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
-                                (const uint8_t*)dal.get(), w, p.get()));
+    build_pyramid(p, image_code<SComp>(), PyramidFractionBits, dimg, C, w, h, wraparound, (const uint8_t*)dal.get(),
+                  false);
     auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
     from_device(p, *out);
     return out;
@@ -419,8 +444,7 @@ gaussianPyramid(unsigned int numLevels, bool wraparound,
     dimg.upload(img.data(), img.size() * sizeof(SComp));
     enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
-    check(enbl_gaussian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
-                                nullptr, 0, p.get()));
+    build_pyramid(p, image_code<SComp>(), PyramidFractionBits, dimg, C, w, h, wraparound, nullptr, false);
     auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
     from_device(p, *out);
     return out;
@@ -451,8 +475,8 @@ laplacianPyramid(const char* /*exportName*/, unsigned int numLevels, bool wrapar
     enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     dal.upload(al.data(), al.size());
     Pyramid p(pyramid_code<PComp>(), C, w, h, (int)numLevels);
-    check(enbl_laplacian_pyramid(context(), wraparound, image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C,
-                                 (const uint8_t*)dal.get(), w, p.get()));
+    build_pyramid(p, image_code<SComp>(), PyramidFractionBits, dimg, C, w, h, wraparound, (const uint8_t*)dal.get(),
+                  true);
     auto* out = new_levels<PyramidImageType>(p, (int)numLevels);
     from_device(p, *out);
     return out;
@@ -682,7 +706,15 @@ void copyToPyramidImage(typename SrcImageType::const_traverser src_upperleft,
     dimg.upload(img.data(), img.size() * sizeof(SComp));
     enter_blend_space(dimg, image_code<SComp>(), w, h, C, img.size() * sizeof(SComp));
     Pyramid p(pyramid_code<PComp>(), C, w, h, 1);
-    check(enbl_copy_to_pyramid(context(), image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C, p.get()));
+    if (jch(C)) {
+        void* l0;
+        int lw, lh;
+        long lp, lpl;
+        check(enbl_pyramid_level(p.get(), 0, &l0, &lw, &lh, &lp, &lpl));
+        check(enbl_rgb_to_jch(context(), image_code<SComp>(), w, h, dimg.get(), (long)w * C, l0, lp, lpl));
+    } else {
+        check(enbl_copy_to_pyramid(context(), image_code<SComp>(), PyramidFractionBits, dimg.get(), (long)w * C, p.get()));
+    }
     scatter(p.read<PComp>(0, C), dest_upperleft, w, h, da, true);
 }
 
@@ -773,9 +805,19 @@ void copyFromPyramidImageIf(typename PyramidImageType::const_traverser src_upper
     std::vector<uint8_t> m = gather_mask(mask_upperleft, w, h, ma);
     DeviceBuffer dm(m.size()), dimg((size_t)w * h * C * sizeof(DComp));
     dm.upload(m.data(), m.size());
-    check(enbl_copy_from_pyramid_if(context(), p.get(), 0, PyramidFractionBits, (const uint8_t*)dm.get(), w, dimg.get(),
-                                    image_code<DComp>(), (long)w * C, ENBL_DITHER_DET, 0));
-    if (converts(C)) {
+    if (jch(C)) {
+        // the pyramid read as double and converted from JCh, under the mask
+        void* l0;
+        int lw, lh;
+        long lp, lpl;
+        check(enbl_pyramid_level(p.get(), 0, &l0, &lw, &lh, &lp, &lpl));
+        check(enbl_jch_to_rgb_if(context(), image_code<DComp>(), w, h, l0, lp, lpl, (const uint8_t*)dm.get(), w,
+                                 dimg.get(), (long)w * C));
+    } else {
+        check(enbl_copy_from_pyramid_if(context(), p.get(), 0, PyramidFractionBits, (const uint8_t*)dm.get(), w,
+                                        dimg.get(), image_code<DComp>(), (long)w * C, ENBL_DITHER_DET, 0));
+    }
+    if (converts(C) && !jch(C)) {
         // the scaled blend-space values back to RGB under the mask
         DeviceBuffer rgb((size_t)w * h * C * sizeof(DComp));
         check(enbl_blend_space_to_rgb_if(context(), blend_colorspace(), image_code<DComp>(), w, h, dimg.get(),

@@ -53,10 +53,11 @@ def oracle_space(oracle):
     oracle.set_blend_colorspace(0)
 
 
-@pytest.mark.parametrize("shape,space", [((96, 160), 0), ((301, 457), 0), ((301, 457), 1), ((96, 160), 2)])
+@pytest.mark.parametrize("shape,space", [((96, 160), 0), ((301, 457), 0), ((301, 457), 1), ((96, 160), 2),
+                                         ((133, 211), 3)])
 def test_enblend_call_site_matches_oracle(binary, oracle, oracle_space, tmp_path, shape, space):
-    """Also with --blend-colorspace CIELAB / CIELUV (BlendColorspace set
-    before the loop)."""
+    """Also with --blend-colorspace CIELAB / CIELUV / CIECAM (BlendColorspace
+    set before the loop)."""
     h, w = shape
     oracle_space(space)
     rng = np.random.default_rng(h + w)
@@ -92,7 +93,8 @@ def _stack(rng, itype, K, h, w):
 
 
 @pytest.mark.parametrize("itype,shape,K,space", [(U8, (64, 96), 3, 0), (U16, (133, 101), 4, 0),
-                                                 (U8, (64, 96), 3, 2), (U16, (133, 101), 4, 1)])
+                                                 (U8, (64, 96), 3, 2), (U16, (133, 101), 4, 1),
+                                                 (U8, (64, 96), 3, 3), (U16, (133, 101), 4, 3)])
 def test_enfuse_call_site_with_user_weight_matches_oracle(binary, oracle, oracle_space, tmp_path, itype, shape, K,
                                                           space):
     h, w = shape
