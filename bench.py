@@ -155,7 +155,7 @@ def batch_compare(args, enbl, scenes, torch, steps, warmup):
             "batched_kernels": _kernels(rep_b), "plain_kernels": _kernels(rep_p)}
 
 
-def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
+def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200, cpu=True):
     """Differentiable enfuse, 5 x 2048x1536 float32 exposures (EV -4..+4),
     loss = sum of squared differences from the Reinhard tone map of the scene
     radiance: hand-written adjoint vs FP64 central differences of the GPU's
@@ -196,44 +196,137 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
     gi_h, gp_h = gi.cpu().numpy(), gp.cpu().numpy()
     print(f"bench: finite differences on {fd_points} samples + 5 parameters", file=sys.stderr, flush=True)
     rng = np.random.default_rng(4)
-    step = 1e-4
-    # Every estimate is taken at two refinements; one that moves by more than
-    # 1e-5 relative between them (FP64 rounding noise of the whole forward, or a
-    # kink of max/min within the step) cannot test a 1e-4 claim and is reported
-    # apart.
+    # Parameters: central differences at h = 1e-4, 5e-5, 2.5e-5, Richardson
+    # extrapolated.  Image samples: Ridders' extrapolation of central
+    # differences (a Neville tableau over steps shrinking by 1.4 from
+    # h0 = 0.1 |v|, keeping the entry of smallest estimated error), which
+    # balances the truncation error of the steep terms of dark pixels (the
+    # local standard deviation of dark windows, their saturation) against the
+    # rounding noise of the 47-Mpixel forward.  The saturation's max/min is
+    # the one kink of the relaxed model: no step reaches another channel's
+    # value (h0 <= d / 2, d the distance to the nearest other channel of the
+    # pixel); a sample level with another channel (d = 0) is a kink, checked
+    # against the matching one-sided derivative and listed apart with both
+    # one-sided values as evidence.
     pscale = max(float(np.abs(gp_h).max()), 1e-300)
     iscale = float(np.abs(gi_h).max())
-    checked, unresolved = [], []
+    floor_i = 1e-6 * iscale
+    smooth, kinks = [], []
 
-    def compare_fd(g, fd1, fd2, floor, what):
-        den = max(abs(fd2), floor)
-        rec = {"what": what, "adjoint": float(g), "fd": float(fd2), "rel_err": abs(g - fd2) / den}
-        (checked if abs(fd1 - fd2) <= 1e-5 * den else unresolved).append(rec)
+    def record(g, est, err, floor, what, extra=None):
+        den = max(abs(est), floor)
+        rec = {"what": what, "adjoint": float(g), "fd": float(est), "rel_err": abs(g - est) / den,
+               "fd_err_estimate": abs(err) / den}
+        if extra:
+            rec.update(extra)
+        smooth.append(rec)
 
     for q in range(5):
         fds = []
-        for hs in (step, step / 2):
+        for hs in (1e-4, 5e-5, 2.5e-5):
             p1, p2 = prm0.clone(), prm0.clone()
             p1[q] += hs
             p2[q] -= hs
             fds.append(loss_diff(imgs, p1, imgs, p2) / (2 * hs))
-        compare_fd(gp_h[q], fds[0], fds[1], 1e-6 * pscale, f"param {q}")
+        r1, r2 = (4.0 * fds[1] - fds[0]) / 3.0, (4.0 * fds[2] - fds[1]) / 3.0
+        record(gp_h[q], r2, r1 - r2, 1e-6 * pscale, f"param {q}")
     i1, i2 = imgs.clone(), imgs.clone()
+
+    def central(k, y, x, c, old, hs):
+        i1[k, y, x, c] = old + hs
+        i2[k, y, x, c] = old - hs
+        v = loss_diff(i1, prm0, i2, prm0) / (2 * hs)
+        i1[k, y, x, c] = old
+        i2[k, y, x, c] = old
+        return v
+
+    def ridders(f, h0, ntab=10, con=1.4):
+        """Ridders' extrapolation of f(h) -> f(0) (f even in h): (estimate, error)."""
+        a = [[0.0] * ntab for _ in range(ntab)]
+        a[0][0] = f(h0)
+        hh, err, ans = h0, float("inf"), a[0][0]
+        for i in range(1, ntab):
+            hh /= con
+            a[0][i] = f(hh)
+            fac = con * con
+            for j in range(1, i + 1):
+                a[j][i] = (a[j - 1][i] * fac - a[j - 1][i - 1]) / (fac - 1.0)
+                fac *= con * con
+                errt = max(abs(a[j][i] - a[j - 1][i]), abs(a[j][i] - a[j - 1][i - 1]))
+                if errt <= err:
+                    err, ans = errt, a[j][i]
+            if abs(a[i][i] - a[i - 1][i - 1]) >= 2.0 * err:
+                break
+        return ans, err
+
+    def one_sided(k, y, x, c, old, hs, sign):
+        # (L(v + s h) - L(v)) / (s h), extrapolated over h, h/2 (first order)
+        vals = []
+        for hh in (hs, hs / 2):
+            i1[k, y, x, c] = old + sign * hh
+            vals.append(loss_diff(i1, prm0, imgs, prm0) / (sign * hh))
+            i1[k, y, x, c] = old
+        return 2.0 * vals[1] - vals[0]
+
     for _ in range(fd_points):
         k, y, x, c = (int(rng.integers(0, K)), int(rng.integers(0, h)), int(rng.integers(0, w)), int(rng.integers(0, 3)))
         old = float(imgs[k, y, x, c])
-        fds = []
-        for rel in (1e-2, 5e-3, 2.5e-3):
-            hs = rel * max(abs(old), 1e-6)
-            i1[k, y, x, c] = old + hs
-            i2[k, y, x, c] = old - hs
-            fds.append(loss_diff(i1, prm0, i2, prm0) / (2 * hs))
-        i1[k, y, x, c] = old
-        i2[k, y, x, c] = old
-        # Richardson extrapolation of the central differences (cancels the h^2 term)
-        r1 = (4.0 * fds[1] - fds[0]) / 3.0
-        r2 = (4.0 * fds[2] - fds[1]) / 3.0
-        compare_fd(gi_h[k, y, x, c], r1, r2, 1e-6 * iscale, f"image {k} ({y},{x}) ch {c} value {old:.3e}")
+        pix = [float(imgs[k, y, x, o]) for o in range(3)]
+        d = min(abs(old - pix[o]) for o in range(3) if o != c)
+        what = f"image {k} ({y},{x}) ch {c} value {old:.3e}"
+        g = float(gi_h[k, y, x, c])
+        if d == 0.0:
+            hs = 1e-3 * max(abs(old), 1e-6)
+            left, right = one_sided(k, y, x, c, old, hs, -1.0), one_sided(k, y, x, c, old, hs, 1.0)
+            side = min(abs(g - v) / max(abs(v), floor_i) for v in (left, right))
+            kinks.append({"what": what, "adjoint": g, "fd_left": left, "fd_right": right, "rel_err_matching_side": side})
+            continue
+        h0 = min(0.1 * max(abs(old), 1e-6), d / 2.0)
+        est, err = ridders(lambda hh: central(k, y, x, c, old, hh), h0)
+        record(g, est, err, floor_i, what, {"h0": h0})
+    # Directional derivatives along random directions over every sample of
+    # every image: V = n I with one N(0,1) factor n per pixel for all three
+    # channels, so a step scales each pixel and never reorders its channels
+    # (no max/min kink is crossed), and the loss changes coherently over 47 M
+    # samples, far above the rounding noise of the forward.  The full-size
+    # adjoint is checked to the extrapolation's precision: <dL/dI, V> against
+    # Ridders' central differences of L(I + e V).
+    gen = torch.Generator(device=dev).manual_seed(44)
+    directional = []
+    gi_dev = gi.to(dev) if hasattr(gi, "to") else torch.from_numpy(gi_h).to(dev)
+    for t in range(3):
+        V = torch.randn(imgs.shape[:3] + (1,), generator=gen, dtype=torch.float64, device=dev) * imgs
+        ad = float(torch.sum(gi_dev * V))
+
+        def dirf(e):
+            return loss_diff(imgs + e * V, prm0, imgs - e * V, prm0) / (2 * e)
+
+        # (the loss is strongly curved along such scalings: central
+        # differences at 1e-4 are off by up to 6e-4, so the steps start at 2e-5)
+        est, err = ridders(dirf, 2e-5)
+        directional.append({"adjoint": ad, "fd": est, "rel_err": abs(ad - est) / abs(est),
+                            "fd_err_estimate": abs(err) / abs(est)})
+    # The oracle's adjoint (oracle/oracle_diff.cpp, FP64 on the host) on the
+    # same full-size inputs and output gradient: the CPU baseline of the
+    # backward pass and its parity check.
+    oracle_check = None
+    if cpu:
+        print("bench: oracle adjoint on the full workload", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_binding import load_oracle
+        o = load_oracle()
+        ih, gh = imgs.cpu().numpy(), gout.cpu().numpy()
+        t0 = time.perf_counter()
+        ogi, ogp = o.diff_backward(ih, prm0.numpy(), L, gh)
+        osec = time.perf_counter() - t0
+        big = float(np.abs(ogi).max())
+        nz = np.abs(ogi) > 1e-6 * big
+        oracle_check = {"gimg_rel_err_max": float(np.max(np.abs(gi_h[nz] - ogi[nz]) / np.abs(ogi[nz]))),
+                        "gimg_abs_err_max_over_max": float(np.abs(gi_h - ogi).max() / big),
+                        "gparam_rel_err_max": float(np.max(np.abs(gp_h - ogp) / np.abs(ogp))),
+                        "cpu_baseline": {"value": round(osec * 1e3, 1), "unit": "ms (backward)",
+                                         "cores": o.lib.orc_num_threads(), "kind": "port",
+                                         "sample": "the full 5 x 2048x1536 backward, one run"}}
     print(f"bench: {adam_steps} Adam steps on the 5 parameters", file=sys.stderr, flush=True)
     prm = prm0.clone().to(dev).requires_grad_(True)
     opt = torch.optim.Adam([prm], lr=0.01)
@@ -255,16 +348,25 @@ def adjoint_c4(args, enbl, scenes, torch, fd_points=256, adam_steps=200):
     mpix = K * w * h / 1e6
     return {"workload": "differentiable enfuse (relaxed float variant), 5 x 2048x1536 RGB, EV -4..+4, FP64 pyramids, "
                         f"{L} levels, L2 loss to the Reinhard tone map",
-            "adjoint_rel_err_max": max(r["rel_err"] for r in checked),
-            "adjoint_rel_err_params": [r["rel_err"] for r in checked + unresolved if r["what"].startswith("param")],
-            "fd_checked": len(checked), "fd_unresolved": len(unresolved),
-            "fd_unresolved_worst": sorted(unresolved, key=lambda r: -r["rel_err"])[:3],
-            "fd": "central differences of the GPU's FP64 forward. Parameters: h = 1e-4, error against the h/2 "
-                  "estimate. Image samples (values down to 6e-5): steps 1e-2, 5e-3, 2.5e-3 x |value| with Richardson "
-                  "extrapolation, error against the finer extrapolate. An estimate that moves by > 1e-5 relative "
-                  "between its two refinements (FP64 rounding noise of the whole forward, or a max/min kink within "
-                  "the step) is reported as unresolved. Denominator floor 1e-6 x the largest gradient component",
-            "fd_points": fd_points + 5,
+            "adjoint_rel_err_max": max(r["rel_err"] for r in smooth),
+            "adjoint_within_1e-4": sum(1 for r in smooth if r["rel_err"] <= 1e-4),
+            "adjoint_within_1e-4_or_3x_fd_error": sum(1 for r in smooth
+                                                       if r["rel_err"] <= max(1e-4, 3.0 * r["fd_err_estimate"])),
+            "adjoint_rel_err_max_fd_resolved": max([r["rel_err"] for r in smooth if r["fd_err_estimate"] <= 1e-5],
+                                                   default=0.0),
+            "fd_resolved": sum(1 for r in smooth if r["fd_err_estimate"] <= 1e-5),
+            "directional": directional, "vs_oracle_adjoint": oracle_check,
+            "adjoint_rel_err_params": [r["rel_err"] for r in smooth if r["what"].startswith("param")],
+            "fd_points": fd_points + 5, "fd_smooth": len(smooth), "fd_kinks": kinks,
+            "fd_err_estimate_max": max(r["fd_err_estimate"] for r in smooth),
+            "fd_worst": sorted(smooth, key=lambda r: -r["rel_err"])[:3],
+            "fd": "central differences of the GPU's FP64 forward. Parameters: h = 1e-4, 5e-5, 2.5e-5, Richardson "
+                  "extrapolated. Image samples (values down to 6e-5): Ridders' extrapolation of central differences "
+                  "from h0 = 0.1 |value| down by 1.4 per step, the tableau entry of smallest estimated error, no "
+                  "step reaching another channel's value (the saturation's max/min kink). adjoint_rel_err_max is "
+                  "over every point that is not a kink; a sample equal to another channel of its pixel is a kink, "
+                  "checked against the matching one-sided derivative and listed with both one-sided values. "
+                  "Denominator floor 1e-6 x the largest gradient component",
             "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
             "forward_mpix_per_s": round(mpix / (fwd_ms * 1e-3), 1),
             "adam_steps": adam_steps, "adam_s": round(adam_s, 2), "loss_first": losses[0], "loss_last": losses[-1],
@@ -794,7 +896,7 @@ def main():
         print(json.dumps(batch_compare(args, enbl, scenes, torch, args.batch_steps, 1)), flush=True)
         return
     if args.workload == "adjoint-c4":
-        print(json.dumps(adjoint_c4(args, enbl, scenes, torch)), flush=True)
+        print(json.dumps(adjoint_c4(args, enbl, scenes, torch, cpu=cpu)), flush=True)
         return
     if args.workload == "copy-calibration":
         # A plain device copy of known size: the reference reading for the
@@ -826,7 +928,7 @@ def main():
         if not args.no_enfuse:
             result["enfuse_c0"] = enfuse_c0(args, enbl, scenes, np, torch, cpu=cpu)
         if not args.no_adjoint:
-            result["adjoint"] = adjoint_c4(args, enbl, scenes, torch)
+            result["adjoint"] = adjoint_c4(args, enbl, scenes, torch, cpu=cpu)
     print(json.dumps(result), flush=True)
 
 
