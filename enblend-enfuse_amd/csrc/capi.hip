@@ -6,6 +6,7 @@
 // stream and errors come back as status codes with a thread-local message.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -93,10 +94,12 @@ int sync_streams(enbl_context* ctx)
 int fork_aux(enbl_context* ctx)
 {
     if (ctx->forked_from) return fail(ENBL_E_ARG, "auxiliary stream already in use");
-    // While the context profiles, the work stays on the main stream: the
-    // timed launches then run one at a time and their event durations are
+    // While the context profiles (or ENBL_SERIAL_STREAMS is set in the
+    // environment, for an external profiler), the work stays on the main
+    // stream: the launches then run one at a time and their durations are
     // their own, not stretched by kernels running beside them.
-    if (ctx->profile) return ENBL_OK;
+    static const bool serial = getenv("ENBL_SERIAL_STREAMS") != nullptr;
+    if (ctx->profile || serial) return ENBL_OK;
     int rc = hip_check(hipEventRecord(ctx->aux_fork, ctx->stream), "hipEventRecord");
     if (!rc) rc = hip_check(hipStreamWaitEvent(ctx->aux_stream, ctx->aux_fork, 0), "hipStreamWaitEvent");
     if (rc) return rc;

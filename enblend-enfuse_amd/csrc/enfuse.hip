@@ -349,13 +349,13 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
             const int y = row0 + r + B, x = xb + lane - B;
             if (lane < nx && y < h - B && x >= B && x < w - B) {
                 const bool ca = lane >= B ? sg[r + B][lane - B].y != 0.f : prev_a[par][r][lane] != 0;
+                float res = 0.f;   // pixels outside the alpha keep localStdDevIf's zero
                 if (ca) {
                     const int c = csk[r][lane];
                     const float sm = css[r][lane], sqv = csq[r][lane];
-                    float res = 0.f;
                     if (c > 1) res = (float)sqrt((double)((sqv - sm * sm / (float)c) / (float)(c - 1)));
-                    out[(long)y * op + x] = res;
                 }
+                out[(long)y * op + x] = res;
             }
         }
         for (int rr = 0; rr < 16; ++rr) {
@@ -363,6 +363,22 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
             if (lane < B) prev_a[par ^ 1][r][lane] = sg[r + B][kSdCW - B + lane].y != 0.f ? 1 : 0;
         }
         __syncthreads();
+    }
+}
+
+// The border of a float standard-deviation plane that localStdDevIf leaves
+// at zero (rows and columns within n/2 of the edge): the tile kernel writes
+// every interior pixel itself.  One block per row of one plane.
+__global__ __launch_bounds__(64) void stddev_border_kernel(StddevJobs J, int w, int h, int b)
+{
+    const int y = blockIdx.x, job = blockIdx.y;
+    float* row = J.out[job] + (long)y * J.opitch;
+    const bool whole = y < b || y >= h - b;
+    if (whole) {
+        for (int x = threadIdx.x; x < w; x += 64) row[x] = 0.f;
+    } else if ((int)threadIdx.x < 2 * b) {
+        const int x = (int)threadIdx.x < b ? (int)threadIdx.x : w - 2 * b + (int)threadIdx.x;
+        if (x >= 0 && x < w) row[x] = 0.f;
     }
 }
 
@@ -1047,7 +1063,6 @@ int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const ui
     if (itype == ENBL_F32) {
         if (n > 31) return ENBL_E_UNSUPPORTED;
         const int rows = h - 2 * (n / 2);
-        if (rows <= 0) return 0;
         if (n == 3 || n == 5 || n == 7) {
             for (int m0 = 0; m0 < K; m0 += kSdMaxJobs) {
                 const int nj = min(kSdMaxJobs, K - m0);
@@ -1061,13 +1076,17 @@ int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const ui
                     J.apitch[j] = apitches[m0 + j];
                     J.out[j] = (float*)out + (m0 + j) * ostride;
                 }
-                const dim3 grid((rows + 63) / 64, nj), block(256);
-                if (n == 3) hipLaunchKernelGGL((stddev_float_tile_kernel<3>), grid, block, 0, s, J, w, h);
-                else if (n == 5) hipLaunchKernelGGL((stddev_float_tile_kernel<5>), grid, block, 0, s, J, w, h);
-                else hipLaunchKernelGGL((stddev_float_tile_kernel<7>), grid, block, 0, s, J, w, h);
+                if (rows > 0) {
+                    const dim3 grid((rows + 63) / 64, nj), block(256);
+                    if (n == 3) hipLaunchKernelGGL((stddev_float_tile_kernel<3>), grid, block, 0, s, J, w, h);
+                    else if (n == 5) hipLaunchKernelGGL((stddev_float_tile_kernel<5>), grid, block, 0, s, J, w, h);
+                    else hipLaunchKernelGGL((stddev_float_tile_kernel<7>), grid, block, 0, s, J, w, h);
+                }
+                hipLaunchKernelGGL(stddev_border_kernel, dim3(h, nj), dim3(64), 0, s, J, w, h, n / 2);
             }
             return 0;
         }
+        if (rows <= 0) return 0;
         for (int m = 0; m < K; ++m)
             hipLaunchKernelGGL((stddev_float_rows_kernel<31>), dim3((rows + 63) / 64), dim3(64), 0, s,
                                (const float*)gray + m * gstride, gp, alphas[m], apitches[m], w, h, n,

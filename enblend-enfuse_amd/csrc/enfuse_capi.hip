@@ -391,7 +391,9 @@ int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* 
         if (rc) return rc;
     }
     if (p->edge_scale > 0.0) return edge_contrast_all(ctx, itype, w, h, K, alphas, aps, p, ws);
-    if (itype == ENBL_F32) {
+    if (itype == ENBL_F32 && p->contrast_window != 3 && p->contrast_window != 5 && p->contrast_window != 7) {
+        // the row kernel of other windows writes the interior under alpha
+        // only (the tile kernel of 3, 5 and 7 writes every pixel)
         rc = launch_fill(ws.grad, ws.plane * es * (size_t)K, s);
         if (!rc) rc = launch_status("fill");
         if (rc) return rc;
