@@ -35,6 +35,12 @@ int launch_stddev(int itype, const void* gray, long gp, size_t gstride, const ui
                   hipStream_t s);
 int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h, int n,
                    const double* tab, void* out, long op, hipStream_t s);
+// The local entropy (windows 3, 5 and 7) and the whole weights pass in one
+// launch: mask, norm and union alpha as launch_weights writes them, no
+// entropy plane.
+int launch_entropy_weights(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h, int n,
+                           const double* tab, const WeightArgs& a, const float* lut, const void* grad, long gp,
+                           float* mask, long mp, float* norm, long np_, uint8_t* ualpha, long up, hipStream_t s);
 int launch_weights(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h,
                    const WeightArgs& a, const float* lut, const void* grad, long gp, const void* ent, long ep,
                    float* mask, long mp, float* norm, long np_, uint8_t* ualpha, long up, hipStream_t s);

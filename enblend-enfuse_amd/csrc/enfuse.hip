@@ -366,6 +366,65 @@ __global__ __launch_bounds__(256) void stddev_float_tile_kernel(StddevJobs J, in
     }
 }
 
+template <class T> __device__ __forceinline__ float contrast_term(double wc, double g);
+template <> __device__ __forceinline__ float contrast_term<uint8_t>(double wc, double g) { return (float)(wc * g / 255.0); }
+template <> __device__ __forceinline__ float contrast_term<uint16_t>(double wc, double g) { return (float)(wc * g / 65535.0); }
+template <> __device__ __forceinline__ float contrast_term<float>(double wc, double g) { return (float)(wc * g); }
+
+template <class T> struct GradT { typedef int32_t type; };
+template <> struct GradT<float> { typedef float type; };
+
+// enfuseMask's weight of one pixel under alpha (src/enfuse.h:1099-1211):
+// exposure, then + contrast, + saturation, + entropy, float additions in the
+// reference's order.
+template <class T>
+__device__ __forceinline__ float pixel_weight(const T (&p)[3], const WeightArgs& a, const float* __restrict__ exp_lut,
+                                              typename GradT<T>::type gradv, T entv)
+{
+    float m = 0.f;
+    // CutoffExposureFunctor with the default projectors: a pixel whose
+    // minimum channel is below the lower cutoff or whose maximum channel
+    // is above the upper one weighs 0
+    const bool cut_out = a.exposure_cutoff &&
+                         !((double)min(p[0], min(p[1], p[2])) >= a.exposure_lo &&
+                           (double)max(p[0], max(p[1], p[2])) <= a.exposure_hi);
+    if (a.w_exposure > 0.0 && !cut_out) {
+        if (Img<T>::integral) {
+            m = exp_lut[(int)Gray<T, T>::get(p, a.proj)];
+        } else {
+            const float g = Gray<T, float>::get(p, a.proj);
+            const double yv = (double)(g / FLT_MAX);
+            m = (float)(a.w_exposure * exposure_builtin(a.exposure_function, a.exposure_optimum, a.exposure_width, yv));
+        }
+    }
+    if (a.w_contrast > 0.0) m = contrast_term<T>(a.w_contrast, (double)gradv) + m;
+    if (a.w_saturation > 0.0) {
+        const T mx = max(p[0], max(p[1], p[2]));
+        const T mn = min(p[0], min(p[1], p[2]));
+        float s = 0.f;
+        if (!(mx == mn)) {
+            const double max_value = Img<T>::integral ? Img<T>::maxv() : 1.0;
+            const double sum = (double)mx + (double)mn;
+            const double diff = (double)mx - (double)mn;
+            s = (float)(a.w_saturation * (sum <= max_value ? diff / sum : diff / (2.0 * max_value - sum)));
+        }
+        m = s + m;
+    }
+    if (a.w_entropy > 0.0) m = (float)(a.w_entropy * (double)(float)entv) + m;
+    return m;
+}
+
+// Outputs of the weights pass that the entropy kernel takes over when it
+// computes the whole weight of the pixels it already holds.
+template <class T> struct FusedWeights {
+    WeightArgs a;
+    const float* exp_lut;
+    const typename GradT<T>::type* grad; long gp;
+    float* mask; long mp;
+    float* norm; long np_;
+    uint8_t* ualpha; long up;
+};
+
 // The border of a float standard-deviation plane that localStdDevIf leaves
 // at zero (rows and columns within n/2 of the edge): the tile kernel writes
 // every interior pixel itself.  One block per row of one plane.
@@ -543,11 +602,14 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred)
     return pred ? base + rank : -1;
 }
 
-template <class T, int N>
+// FUSEW: the kernel also makes the pixel's whole weight (the weights pass
+// of pixel_weight, with the entropy it has just computed and the pixel and
+// alpha it holds in LDS) instead of writing the entropy plane.
+template <class T, int N, bool FUSEW>
 __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __restrict__ img, long ip,
                                                                    const uint8_t* __restrict__ alpha, long ap,
                                                                    int w, int h, const double* __restrict__ tab,
-                                                                   T* __restrict__ out, long op)
+                                                                   T* __restrict__ out, long op, FusedWeights<T> fw)
 {
     constexpr int B = N / 2, NN = N * N;
     constexpr int LW = kEntTW + 2 * B, LH = kEntTH + 2 * B;
@@ -808,17 +870,23 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
             r = e1 < r ? e1 : r;
             r = e2 < r ? e2 : r;
         }
-        out[(long)(y0 + ly) * op + x0 + lx] = r;
+        const long y = y0 + ly, x = x0 + lx;
+        if constexpr (FUSEW) {
+            const uint8_t al = sa[ly + B][lx + B];
+            float m = 0.f;
+            if (al) {
+                const T px[3] = {(T)sv[0][ly + B][lx + B], (T)sv[1][ly + B][lx + B], (T)sv[2][ly + B][lx + B]};
+                m = pixel_weight<T>(px, fw.a, fw.exp_lut, fw.a.w_contrast > 0.0 ? fw.grad[y * fw.gp + x] : 0, r);
+                fw.ualpha[y * fw.up + x] = al;
+            }
+            fw.mask[y * fw.mp + x] = m;
+            float* nrm = fw.norm + y * fw.np_ + x;
+            *nrm = m + *nrm;
+        } else {
+            out[y * op + x] = r;
+        }
     }
 }
-
-template <class T> __device__ __forceinline__ float contrast_term(double wc, double g);
-template <> __device__ __forceinline__ float contrast_term<uint8_t>(double wc, double g) { return (float)(wc * g / 255.0); }
-template <> __device__ __forceinline__ float contrast_term<uint16_t>(double wc, double g) { return (float)(wc * g / 65535.0); }
-template <> __device__ __forceinline__ float contrast_term<float>(double wc, double g) { return (float)(wc * g); }
-
-template <class T> struct GradT { typedef int32_t type; };
-template <> struct GradT<float> { typedef float type; };
 
 // Combines the weights of one image in the reference's order (exposure,
 // contrast, saturation, entropy; float additions), and updates the norm
@@ -841,35 +909,9 @@ __global__ __launch_bounds__(kTile) void weights_kernel(const T* __restrict__ im
     float m = 0.f;
     if (al) {
         const T* p = img + (long)y * ip + 3L * x;
-        // CutoffExposureFunctor with the default projectors: a pixel whose
-        // minimum channel is below the lower cutoff or whose maximum channel
-        // is above the upper one weighs 0
-        const bool cut_out = a.exposure_cutoff &&
-                             !((double)min(p[0], min(p[1], p[2])) >= a.exposure_lo &&
-                               (double)max(p[0], max(p[1], p[2])) <= a.exposure_hi);
-        if (a.w_exposure > 0.0 && !cut_out) {
-            if (Img<T>::integral) {
-                m = exp_lut[(int)Gray<T, T>::get(p, a.proj)];
-            } else {
-                const float g = Gray<T, float>::get(p, a.proj);
-                const double yv = (double)(g / FLT_MAX);
-                m = (float)(a.w_exposure * exposure_builtin(a.exposure_function, a.exposure_optimum, a.exposure_width, yv));
-            }
-        }
-        if (a.w_contrast > 0.0) m = contrast_term<T>(a.w_contrast, (double)grad[(long)y * gp + x]) + m;
-        if (a.w_saturation > 0.0) {
-            const T mx = max(p[0], max(p[1], p[2]));
-            const T mn = min(p[0], min(p[1], p[2]));
-            float s = 0.f;
-            if (!(mx == mn)) {
-                const double max_value = Img<T>::integral ? Img<T>::maxv() : 1.0;
-                const double sum = (double)mx + (double)mn;
-                const double diff = (double)mx - (double)mn;
-                s = (float)(a.w_saturation * (sum <= max_value ? diff / sum : diff / (2.0 * max_value - sum)));
-            }
-            m = s + m;
-        }
-        if (a.w_entropy > 0.0) m = (float)(a.w_entropy * (double)(float)ent[(long)y * ep + x]) + m;
+        const T px[3] = {p[0], p[1], p[2]};
+        m = pixel_weight<T>(px, a, exp_lut, a.w_contrast > 0.0 ? grad[(long)y * gp + x] : 0,
+                            a.w_entropy > 0.0 ? ent[(long)y * ep + x] : T(0));
         ualpha[(long)y * up + x] = al;
     }
     mask[(long)y * mp + x] = m;
@@ -1105,7 +1147,7 @@ int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, lo
     if (n * n > 121) return ENBL_E_UNSUPPORTED;
     if (n == 3 || n == 5 || n == 7) {
         const dim3 grid((w + kEntTW - 1) / kEntTW, (h + kEntTH - 1) / kEntTH), block(kEntThreads);
-#define ENBL_ENT_TILE(T, N) hipLaunchKernelGGL((entropy_tile_kernel<T, N>), grid, block, 0, s, (const T*)img, ip, alpha, ap, w, h, tab, (T*)out, op)
+#define ENBL_ENT_TILE(T, N) hipLaunchKernelGGL((entropy_tile_kernel<T, N, false>), grid, block, 0, s, (const T*)img, ip, alpha, ap, w, h, tab, (T*)out, op, FusedWeights<T>{})
 #define ENBL_ENT_N(T)                   \
     do {                                \
         if (n == 3) ENBL_ENT_TILE(T, 3); \
@@ -1128,6 +1170,30 @@ int launch_entropy(int itype, const void* img, long ip, const uint8_t* alpha, lo
         else if (itype == ENBL_U16) hipLaunchKernelGGL((entropy_kernel<uint16_t, 121>), grid_for(w, h), dim3(kTile), 0, s, (const uint16_t*)img, ip, alpha, ap, w, h, n, tab, (uint16_t*)out, op);
         else hipLaunchKernelGGL((entropy_kernel<float, 121>), grid_for(w, h), dim3(kTile), 0, s, (const float*)img, ip, alpha, ap, w, h, n, tab, (float*)out, op);
     }
+    return 0;
+}
+
+int launch_entropy_weights(int itype, const void* img, long ip, const uint8_t* alpha, long ap, int w, int h, int n,
+                           const double* tab, const WeightArgs& a, const float* lut, const void* grad, long gp,
+                           float* mask, long mp, float* norm, long np_, uint8_t* ualpha, long up, hipStream_t s)
+{
+    if (n != 3 && n != 5 && n != 7) return ENBL_E_UNSUPPORTED;
+    const dim3 grid((w + kEntTW - 1) / kEntTW, (h + kEntTH - 1) / kEntTH), block(kEntThreads);
+#define ENBL_ENTW_TILE(T, N)                                                                                       \
+    hipLaunchKernelGGL((entropy_tile_kernel<T, N, true>), grid, block, 0, s, (const T*)img, ip, alpha, ap, w, h, tab, \
+                       (T*)nullptr, 0L,                                                                           \
+                       FusedWeights<T>{a, lut, (const typename GradT<T>::type*)grad, gp, mask, mp, norm, np_, ualpha, up})
+#define ENBL_ENTW_N(T)                  \
+    do {                                \
+        if (n == 3) ENBL_ENTW_TILE(T, 3); \
+        else if (n == 5) ENBL_ENTW_TILE(T, 5); \
+        else ENBL_ENTW_TILE(T, 7);       \
+    } while (0)
+    if (itype == ENBL_U8) ENBL_ENTW_N(uint8_t);
+    else if (itype == ENBL_U16) ENBL_ENTW_N(uint16_t);
+    else ENBL_ENTW_N(float);
+#undef ENBL_ENTW_N
+#undef ENBL_ENTW_TILE
     return 0;
 }
 

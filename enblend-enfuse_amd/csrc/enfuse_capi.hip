@@ -414,6 +414,15 @@ int image_weights(enbl_context* ctx, int itype, int w, int h, int m, const void*
     const hipStream_t s = ctx->stream;
     int rc;
     const void* grad = ws.grad ? (const void*)((const char*)ws.grad + (size_t)m * ws.plane * 4) : nullptr;
+    const int n = p->entropy_window;
+    if (p->w_entropy > 0.0 && !p->entropy_cutoff && (n == 3 || n == 5 || n == 7)) {
+        // the entropy kernel holds the pixels and makes the whole weight
+        Timed t(ctx, "entropy_weights", (double)w * h * (3.0 * ptype_size(itype) + 1.0 + 4.0 + 4.0 + 8.0 + 1.0));
+        rc = launch_entropy_weights(itype, img, ip, alpha, ap, w, h, n, ws.tab, a, ws.lut, grad, ws.pitch, mask, mp,
+                                    norm, np_, ua, up, s);
+        if (rc) return fail(rc, "localEntropyIf: unsupported window");
+        return launch_status("entropy_weights");
+    }
     if (p->w_entropy > 0.0) {
         const void* src = img;
         long sp = ip;
