@@ -320,7 +320,13 @@ __global__ __launch_bounds__(64) void nft_rows_manhattan_kernel(NftArgs a)
 // 64 lanes prepare the tile, each one column: the sum f(q) + q^2 and the
 // intersection with the parabola of q - 1, which is the stack top whenever q
 // arrives, so a sample that pops nothing costs the serial lane no division.
-// The envelope's final top index goes to a table.
+// The entry under the top is kept in registers, so a sample that pops one
+// entry reads nothing from LDS either.  Two runs of a tile take one step, each
+// only after checking the condition that makes the reference's loop do
+// exactly that: a run of FLT_MAX samples (columns without features) where each
+// sample pops its predecessor and lands on the same entry, and a suffix whose
+// intersections increase, where every sample pushes.  The envelope's final top
+// index goes to a table.
 //
 // The output does not depend on the order in which the reference's forward
 // output sweep visits the samples: the envelope bounds are finite beyond entry
@@ -331,6 +337,8 @@ __global__ __launch_bounds__(64) void nft_rows_manhattan_kernel(NftArgs a)
 // run of kOutCols samples and walks up from there; every result
 // sqrt((q - v_k)^2 + f(v_k)) is the same float.
 constexpr int kEnvRows = 2, kEnvRing = 256, kRefill = 32, kRT = 64, kOutCols = 8;
+// A tile's pushes (up to kRT) stay in the ring until they are written out.
+static_assert(kEnvRing >= kRT && (kEnvRing & (kEnvRing - 1)) == 0, "ring size");
 
 // Envelope entry: sample index in the low word, the bound's float bits in
 // the high word.
@@ -363,10 +371,13 @@ __device__ __forceinline__ float env_div(float n, int d)
 
 __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
 {
-    constexpr int R = kEnvRows, M = kEnvRing - 1;
+    constexpr int R = kEnvRows, M = kEnvRing - 1, TP = kRT + 4;
     __shared__ uint4 ring[kEnvRing][R];   // (v, sum bits, z bits, 0)
-    __shared__ float tsum[R][kRT + 2], ts0[R][kRT + 2];
+    // Per row and tile column j: the sum f(q) + q^2 and the intersection with
+    // the parabola of q - 1.
+    __shared__ float tq[R][TP], t0[R][TP];
     __shared__ int span[R][2];
+    __shared__ int bulk[R][2];   // a row's run of pushes: first tile column, stack index under it
     const int lane = threadIdx.x;
     const int t = blockIdx.y;
     const int row0 = blockIdx.x * R;
@@ -386,26 +397,32 @@ __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
     };
     // Top entry: index k, sample vk, sum_v = f(v) + v^2 (the reference's
     // fk + (float)(vk * vk), the same rounding of the same integer square),
-    // bound zk.  Ring slots from lo up to k are valid.
+    // bound zk.  The entry under it (k - 1) is kept in registers too while hb
+    // is set (bv, bs, bz), so that a sample popping one entry reads nothing
+    // from LDS.  Ring slots from lo up to k are valid.
     int k = 0, lo = 0;
-    int vk = 0;
-    float sk = 0.f, zk = -inf;
+    int vk = 0, bv = 0;
+    float sk = 0.f, zk = -inf, bs = 0.f, bz = -inf;
+    bool hb = false;
     int mk = -1;   // lowest stack index reached in this tile
-    auto refill = [&]() {   // entries k - kRefill + 1 .. k into the ring
+    auto refill = [&]() {   // entries k - kRefill + 1 .. k into the ring, 8 at a time
         const int e0 = max(0, k - kRefill + 1);
-        int tv[kRefill];
-        float tz[kRefill];
+#pragma unroll 1
+        for (int c = k; c >= e0; c -= 8) {
+            int tv[8];
+            float tz[8];
 #pragma unroll
-        for (int j = 0; j < kRefill; ++j) {
-            const int e = k - j;
-            if (e >= e0) { const uint64_t p = *gmem(env + e); tv[j] = env_v(p); tz[j] = env_z(p); }
-        }
+            for (int j = 0; j < 8; ++j) {
+                const int e = c - j;
+                if (e >= e0) { const uint64_t p = *gmem(env + e); tv[j] = env_v(p); tz[j] = env_z(p); }
+            }
 #pragma unroll
-        for (int j = 0; j < kRefill; ++j) {
-            const int e = k - j;
-            if (e >= e0) {
-                const float sv = *gmem(mine + tv[j]) + (float)tv[j] * (float)tv[j];
-                ring[e & M][lane] = make_uint4((unsigned)tv[j], __float_as_uint(sv), __float_as_uint(tz[j]), 0u);
+            for (int j = 0; j < 8; ++j) {
+                const int e = c - j;
+                if (e >= e0) {
+                    const float sv = *gmem(mine + tv[j]) + (float)tv[j] * (float)tv[j];
+                    ring[e & M][lane] = make_uint4((unsigned)tv[j], __float_as_uint(sv), __float_as_uint(tz[j]), 0u);
+                }
             }
         }
         lo = e0;
@@ -414,56 +431,143 @@ __global__ __launch_bounds__(64) void nft_rows_env_kernel(NftArgs a)
     for (int r = 0; r < R; ++r) carry[r] = 0.f;
     fetch(0);
     for (int x0 = 0; x0 < n; x0 += kRT) {
-        // Every lane prepares one column of the tile for all rows: the sum
-        // f(q) + q^2, and the intersection with the parabola of q - 1, which
-        // is the stack top whenever sample q arrives.
+        const int cn = min(kRT, n - x0);
+        // Every lane prepares one column of the tile for all rows.
         const int c = x0 + lane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) tsum[r][lane + 1] = pf[r] + (float)c * (float)c;
-        if (lane == 0) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) tsum[r][0] = carry[r];
-        }
+        for (int r = 0; r < R; ++r) tq[r][lane] = pf[r] + (float)c * (float)c;
         __syncthreads();
+        // Per row, from ballots over the columns: lb, the last column j >= 1
+        // whose intersection does not exceed its left neighbour's (0 if none),
+        // and fs, the first column from which every sample to the tile end is
+        // FLT_MAX (a column without features in the column pass).
+        int my_lb = 0, my_fs = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            ts0[r][lane] = (tsum[r][lane + 1] - tsum[r][lane]) * 0.5f;   // RN(x / 2) either way
-            carry[r] = tsum[r][kRT];
+            const float prev = lane ? tq[r][lane - 1] : carry[r];
+            const float tz = (tq[r][lane] - prev) * 0.5f;   // RN(x / 2) either way
+            t0[r][lane] = tz;
+            carry[r] = tq[r][kRT - 1];
+            const float tzp = __shfl_up(tz, 1);
+            const unsigned long long brk = __ballot(lane >= 1 && lane < cn && !(tz > tzp));
+            const unsigned long long fin = __ballot(lane < cn && tq[r][lane] != FLT_MAX);
+            const int lbv = brk ? 63 - __builtin_clzll(brk) : 0;
+            const int fsv = fin ? 64 - __builtin_clzll(fin) : 0;
+            my_lb = lane == r ? lbv : my_lb;
+            my_fs = lane == r ? fsv : my_fs;
         }
         __syncthreads();
         if (x0 + kRT < n) fetch(x0 + kRT);
         if (own) {
-            const int cn = min(kRT, n - x0);
             int i = 0;
             if (x0 == 0) {   // sample 0 starts the envelope
-                vk = 0; sk = tsum[lane][1]; zk = -inf;
+                vk = 0; sk = tq[lane][0]; zk = -inf; hb = false;
                 ring[0][lane] = make_uint4(0u, __float_as_uint(sk), __float_as_uint(-inf), 0u);
                 i = 1;
             }
-            float sum_n = tsum[lane][i + 1], s0_n = ts0[lane][i];
+            int pst = -1, pk = 0;
+            const int my_ff = min(my_fs, my_lb);
+            float sum_n = tq[lane][i], s0_n = t0[lane][i];
             for (; i < cn; ++i) {
                 const int q = x0 + i;
                 const float sum_q = sum_n;
-                float s = s0_n;
-                sum_n = tsum[lane][i + 2];   // the padding column past the tile is never used
-                s0_n = ts0[lane][i + 1];
-                while (s <= zk) {   // never below entry 0, whose bound is -inf
-                    --k;
-                    if (k < lo) refill();
-                    const uint4 e = ring[k & M][lane];
-                    vk = (int)e.x;
-                    sk = __uint_as_float(e.y);
-                    zk = __uint_as_float(e.z);
-                    s = env_div(sum_q - sk, 2 * (q - vk));
+                const float s = s0_n;   // against q - 1, the top whenever q arrives
+                sum_n = tq[lane][i + 1];   // TP > kRT: the read past the last column is never used
+                s0_n = t0[lane][i + 1];
+                if (i >= my_ff) {
+                    if (i >= my_fs && sk == FLT_MAX && hb && zk >= 0.f) {
+                        // Samples q .. b are FLT_MAX, and so is the top q - 1:
+                        // each one meets the top at (FLT_MAX - FLT_MAX) / 2 = 0
+                        // <= zk, pops it and meets the entry under it, A, at
+                        // RN((FLT_MAX - sum_A) / (2 (p - v_A))) >= 0, which
+                        // decreases with p, so if it still exceeds z_A at b
+                        // every sample pushes onto A: A, b with the bound at b
+                        // remain.
+                        const int b = x0 + cn - 1;
+                        const float sb = env_div(FLT_MAX - bs, 2 * (b - bv));
+                        if (sb > bz) {
+                            mk = min(mk, k - 1);
+                            vk = b; sk = FLT_MAX; zk = sb;
+                            ring[k & M][lane] = make_uint4((unsigned)b, __float_as_uint(FLT_MAX), __float_as_uint(sb), 0u);
+                            break;
+                        }
+                    }
+                    if (i >= my_lb && s > zk) {
+                        // q pops nothing, and every later intersection of the
+                        // tile exceeds its predecessor's: q .. b all push (their
+                        // ring slots are written below by the whole wave).
+                        pst = i;
+                        pk = k;
+                        const int last = cn - 1;
+                        if (last > i) {
+                            bv = x0 + last - 1; bs = tq[lane][last - 1]; bz = t0[lane][last - 1];
+                        } else {
+                            bv = vk; bs = sk; bz = zk;
+                        }
+                        hb = true;
+                        vk = x0 + last; sk = tq[lane][last]; zk = t0[lane][last];
+                        k += cn - i;
+                        lo = max(lo, k - kEnvRing + 1);
+                        break;
+                    }
                 }
-                mk = min(mk, k);
-                ++k;
+                // The common cases without branches: q pops nothing (p0), or
+                // pops the top and lands on the entry under it (p1, known in
+                // registers).  Anything else runs the reference's loop.
+                const float s1 = env_div(sum_q - bs, 2 * (q - bv));
+                bool p0 = s > zk;
+                const bool p1 = !p0 && hb && s1 > bz;
+                float zq = p0 ? s : s1;
+                if (!p0 && !p1) {
+                    --k;
+                    if (hb) {
+                        vk = bv; sk = bs; zk = bz;
+                        hb = false;
+                    } else {
+                        if (k < lo) refill();
+                        const uint4 e = ring[k & M][lane];
+                        vk = (int)e.x;
+                        sk = __uint_as_float(e.y);
+                        zk = __uint_as_float(e.z);
+                    }
+                    float ss = env_div(sum_q - sk, 2 * (q - vk));
+                    while (ss <= zk) {   // never below entry 0, whose bound is -inf
+                        --k;
+                        if (k < lo) refill();
+                        const uint4 e = ring[k & M][lane];
+                        vk = (int)e.x;
+                        sk = __uint_as_float(e.y);
+                        zk = __uint_as_float(e.z);
+                        ss = env_div(sum_q - sk, 2 * (q - vk));
+                    }
+                    mk = min(mk, k);
+                    zq = ss;
+                    p0 = true;   // q goes onto the entry reached
+                }
+                // p0: q goes onto the top, which becomes the entry under it;
+                // p1: q replaces the top, over the same entry under it.
+                mk = p0 ? mk : min(mk, k - 1);
+                bv = p0 ? vk : bv;
+                bs = p0 ? sk : bs;
+                bz = p0 ? zk : bz;
+                hb = true;
+                k += p0 ? 1 : 0;
                 lo = k - lo >= kEnvRing ? k - kEnvRing + 1 : lo;   // entry k - kEnvRing leaves the ring
-                ring[k & M][lane] = make_uint4((unsigned)q, __float_as_uint(sum_q), __float_as_uint(s), 0u);
-                vk = q; sk = sum_q; zk = s;
+                ring[k & M][lane] = make_uint4((unsigned)q, __float_as_uint(sum_q), __float_as_uint(zq), 0u);
+                vk = q; sk = sum_q; zk = zq;
             }
+            bulk[lane][0] = pst;
+            bulk[lane][1] = pk;
             span[lane][0] = mk;
             span[lane][1] = k;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j0 = bulk[r][0];
+            if (r < nrows && j0 >= 0 && lane >= j0 && lane < cn)
+                ring[(bulk[r][1] + 1 + lane - j0) & M][r] = make_uint4((unsigned)(x0 + lane), __float_as_uint(tq[r][lane]),
+                                                                    __float_as_uint(t0[r][lane]), 0u);
         }
         __syncthreads();
         // The entries this tile pushed, (mk, k] (at most kRT, all still in the
