@@ -557,7 +557,195 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
     }
 }
 
+// Fine samples (X, X+1) of row Y of level 0 of an image pyramid, formed from
+// the interleaved image: (P)v << frac for integer images, the log transform
+// of src/fixmath.h:165-186 in double for float images.
+template <class I, class P> struct L0Fine {
+    typedef typename Pix<P>::S S;
+    __device__ static __forceinline__ void load(const void* img, long ip, int Y, int X, bool xb, int frac, S (&x0)[3],
+                                                S (&x1)[3]) {
+        const RowSrc rs{img, ip, 0, frac};
+        ImageRows<I, P>::template load<3>(rs, Y, X, xb, x0, x1);
+    }
+};
+template <> struct L0Fine<float, double> {
+    __device__ static __forceinline__ void load(const void* img, long ip, int Y, int X, bool xb, int, double (&x0)[3],
+                                                double (&x1)[3]) {
+        const auto* g = gmem(static_cast<const float*>(img) + (long)Y * ip + 3L * X);
+        float v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = (k < 3 || xb) ? g[k] : 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            x0[ch] = float_to_log_pyramid((double)v[ch]);
+            x1[ch] = xb ? float_to_log_pyramid((double)v[3 + ch]) : 0.0;
+        }
+    }
+};
+
+// Output of one fine pixel pair (X, X+1) of row Y: the union alpha, and the
+// image under it, SpecializedConvertPyramidToScalarFunctor with the dither
+// for integer images (src/fixmath.h:236-290), the inverse log transform for
+// float images (:320-329).
+template <class P, class I>
+__device__ __forceinline__ void emit_l0(const L0FuseArgs& a, int X, int Y, bool xb, const P (&v0)[3], const P (&v1)[3])
+{
+    const uint8_t* ur = a.ualpha + (long)Y * a.upitch + X;
+    const uint8_t u0 = *gmem(ur), u1 = xb ? *gmem(ur + 1) : (uint8_t)0;
+    auto* oa = gmem(a.oalpha + (long)Y * a.oapitch + X);
+    oa[0] = u0;
+    if (xb) oa[1] = u1;
+    auto* o = gmem(static_cast<I*>(a.out) + (long)Y * a.opitch + 3L * X);
+    if constexpr (std::is_same<I, float>::value) {
+        if (u0) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) o[ch] = (float)log_pyramid_to_float(v0[ch]);
+        }
+        if (u1) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) o[3 + ch] = (float)log_pyramid_to_float(v1[ch]);
+        }
+    } else {
+        const uint32_t h0 = a.dither == ENBL_DITHER_DET ? dither_hash(a.seed, X, Y) : 0u;
+        const uint32_t h1 = a.dither == ENBL_DITHER_DET ? dither_hash(a.seed, X + 1, Y) : 0u;
+        if (u0) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(v0[ch], a.frac, a.dither, h0, ch);
+        }
+        if (u1) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) o[3 + ch] = to_image<I>(v1[ch], a.frac, a.dither, h1, ch);
+        }
+    }
+}
+
+// Wave strips as in batch_fuse_kernel: lane L of wave wx owns coarse column
+// j = 62 wx + L - 1 (L in [1, 62]) and its 2 x 2 fine block; a block is four
+// waves side by side over a.rows coarse rows.
+// One coarse row i of the level-0 merge + collapse (see enfuse_level0_kernel).
+// P2: every divisor of the wave's expand is a power of two (interior rows and
+// columns), so the divisions are exact products with the reciprocals.
+template <class P, class M, class I, bool G0, bool P2>
+__device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale, int i, int j, bool valid, bool owned,
+                                           int X, bool xb, const int (&dv)[4])
+{
+    typedef Pix<P> Px;
+    typedef typename Px::A A;
+    typedef typename Px::S S;
+    const int cw = a.cw, chh = a.ch;
+    const int Y = 2 * i;
+    const bool yb = Y + 1 < a.fh;
+    double inv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) inv[q] = P2 ? __builtin_amdgcn_ldexp(1.0, -__builtin_ctz((unsigned)dv[q])) : 0.0;
+    auto qdiv = [&](S v, int q) -> S {
+        if constexpr (P2 && std::is_same<P, double>::value) return v * inv[q];
+        else return A::div(v, dv[q]);
+    };
+    S acc[4][3];
+    for (int m = 0; m < a.K; ++m) {
+        S c[3][3], f[4][3];
+        int32_t mv[4] = {0, 0, 0, 0};
+        load_coarse<P, 3>(static_cast<const P*>(a.g1[m]), a.gpitch, a.gplane, i, j, chh, valid, c);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) f[q][ch] = A::zero();
+        if (valid) {
+            const M* mrow = static_cast<const M*>(a.mask[m]) + (long)Y * a.mpitch + X;
+            if constexpr (G0) {
+                const RowSrc rs{a.g0[m], a.g0pitch, a.g0plane, 0};
+                PlaneRows<P>::template load<3>(rs, Y, X, xb, f[0], f[1]);
+                if (yb) PlaneRows<P>::template load<3>(rs, Y + 1, X, xb, f[2], f[3]);
+            } else {
+                L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y, X, xb, a.frac, f[0], f[1]);
+                if (yb) L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y + 1, X, xb, a.frac, f[2], f[3]);
+            }
+            Vec2<M>::load(mrow, xb, mv[0], mv[1]);
+            if (yb) Vec2<M>::load(mrow + a.mpitch, xb, mv[2], mv[3]);
+        }
+        S o[4][3];
+        expand_sums<P, 3>(c, i, j, cw, chh, o);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // Laplacian value (src/pyramid.h:1705-1707), mask multiply
+            // (src/enfuse.h:601-620) and accumulation (:1576-1588).
+            const double coeff = mask_coeff(mv[q], a.maxpyr, inv_scale);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const P lapv = Px::wrap(A::sub(f[q][ch], qdiv(o[q][ch], q)));
+                const S v = Px::load(from_real<P>((double)lapv * coeff));
+                acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
+            }
+        }
+    }
+    // Collapse step: + expand(result[1]), saturating (src/pyramid.h:1837-1886).
+    S rcv[3][3], o[4][3];
+    load_coarse<P, 3>(static_cast<const P*>(a.rc), a.rcpitch, a.rcplane, i, j, chh, valid, rcv);
+    expand_sums<P, 3>(rcv, i, j, cw, chh, o);
+    P outv[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) outv[q][ch] = Px::sat(A::add(acc[q][ch], qdiv(o[q][ch], q)));
+    if (!owned) return;
+    emit_l0<P, I>(a, X, Y, xb, outv[0], outv[1]);
+    if (yb) emit_l0<P, I>(a, X, Y + 1, xb, outv[2], outv[3]);
+}
+
+// Wave strips as in batch_fuse_kernel: lane L of wave wx owns coarse column
+// j = 62 wx + L - 1 (L in [1, 62]) and its 2 x 2 fine block; a block is four
+// waves side by side over a.rows coarse rows.  G0: level 0 read from the
+// converted planes a.g0 instead of the images.
+template <class P, class M, class I, bool G0>
+__global__ __launch_bounds__(256) void enfuse_level0_kernel(L0FuseArgs a, double inv_scale)
+{
+    const int lane = threadIdx.x & 63;
+    const int wx = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int cw = a.cw, chh = a.ch;
+    if (wx * kOwned >= cw) return;   // the whole wave is beyond the level
+    const int i0 = blockIdx.y * a.rows;
+    const int i1 = min(chh, i0 + a.rows);
+    const int j = wx * kOwned + lane - 1;
+    const bool valid = j >= 0 && j < cw;
+    const bool owned = valid && lane >= 1 && lane <= kOwned;
+    const int X = 2 * j;
+    const bool xb = X + 1 < a.fw;
+    const int hdA = expand_hsum(X, cw, false, true), hdB = expand_hsum(X + 1, cw, false, true);
+    for (int i = i0; i < i1; ++i) {
+        const int Y = 2 * i;
+        const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
+        const int dv[4] = {hdA * v0, hdB * v0, hdA * v1, hdB * v1};
+        // (only the owned lanes' results are kept)
+        bool p2 = true;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p2 = p2 && dv[q] > 0 && (dv[q] & (dv[q] - 1)) == 0;
+        p2 = p2 || !owned;
+        if (wave_all(p2)) level0_row<P, M, I, G0, true>(a, inv_scale, i, j, valid, owned, X, xb, dv);
+        else level0_row<P, M, I, G0, false>(a, inv_scale, i, j, valid, owned, X, xb, dv);
+    }
+}
+
 }  // namespace
+
+int launch_enfuse_level0(int itype, L0FuseArgs a, hipStream_t s)
+{
+    if (a.K < 1 || a.K > kL0Max || a.cw < 1 || a.ch < 1) return ENBL_E_ARG;
+    if (a.rows <= 0) a.rows = 8;
+    const int waves_x = (a.cw + kOwned - 1) / kOwned;
+    const dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
+    const double inv_scale = 255.0 / a.maxpyr;   // 2^-mfrac, exact
+    if (itype == ENBL_U8)
+        hipLaunchKernelGGL((enfuse_level0_kernel<int16_t, int16_t, uint8_t, false>), grid, dim3(256), 0, s, a, inv_scale);
+    else if (itype == ENBL_U16)
+        hipLaunchKernelGGL((enfuse_level0_kernel<int32_t, int32_t, uint16_t, false>), grid, dim3(256), 0, s, a, inv_scale);
+    else if (itype == ENBL_F32 && a.g0[0])
+        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, true>), grid, dim3(256), 0, s, a, inv_scale);
+    else if (itype == ENBL_F32)
+        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, false>), grid, dim3(256), 0, s, a, inv_scale);
+    else return ENBL_E_UNSUPPORTED;
+    return 0;
+}
 
 int launch_batch_weights(int itype, int max_k, const BWeightJob* jobs, const int* tile0, int njobs, int total,
                          const BImage* images, const float* lut, double w_exposure, double w_saturation, double scale, int rows, int grid,

@@ -150,6 +150,7 @@ int do_reduce(enbl_context* ctx, int ptype, int C, int wrap,
     if (dw != (sw + 1) / 2 || dh != (sh + 1) / 2)
         return fail(ENBL_E_SIZE, "reduce: destination %dx%d does not match source %dx%d", dw, dh, sw, sh);
     ReduceArgs a;
+    memset(&a, 0, sizeof a);
     a.src = src; a.spitch = sp; a.splane = spl; a.sw = sw; a.sh = sh;
     a.salpha = sa; a.sapitch = sap;
     a.dst = dst; a.dpitch = dp; a.dplane = dpl; a.dw = dw; a.dh = dh;
@@ -615,8 +616,10 @@ int tail_image_type(int mt) { return mt == ENBL_I16 ? ENBL_I16 : ENBL_I32; }
 // kTailMaxJobs jobs (no job of a level writes what another job of that level
 // reads).  Same results as laplacian_pyramid_any per image.  Scratch: the
 // alpha ping-pong, slots 8 and 23 (fused passes) and 35 (alpha levels).
+int enfuse_level0_count(const enbl_pyramid* p, const uint8_t* alpha) { return fused_level_count(p, 0, alpha); }
+
 int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const void* const* imgs, const long* ips,
-                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps)
+                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps, void* const* g1s)
 {
     const enbl_pyramid* p0 = lps[0];
     const int L = p0->levels;
@@ -654,10 +657,43 @@ int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const v
         return base + tbytes * (size_t)m + (l == s ? 0 : toff[l]);
     };
     auto ipitch = [&](int m, int l) { return l == 0 ? aps[m] : (l == s ? apl : tpitch(l)); };
+    if (g1s && s < 1) return fail(ENBL_E_ARG, "enfuse_image_pyramids: level 1 is not a streaming level");
     for (int m = 0; m < K; ++m) {
         int rc;
         if (s == 0) {
             rc = build_level0(ctx, lps[m], itype, frac, imgs[m], ips[m]);
+        } else if (g1s) {
+            // Gaussian level 1 (and its alpha) reduced from the image, kept
+            // apart for the level-0 merge, which forms Laplacian level 0 on
+            // the fly; the fused passes go on from level 1.
+            uint8_t* abuf[2] = {pp[0], pp[1]};
+            abuf[(s - 1) & 1] = ialpha(m, s);
+            const Level& F = p0->lv[0];
+            const Level& D = p0->lv[1];
+            ReduceArgs r;
+            memset(&r, 0, sizeof r);
+            r.src = imgs[m]; r.spitch = ips[m]; r.splane = 0; r.sw = F.w; r.sh = F.h;
+            r.salpha = alphas[m]; r.sapitch = aps[m];
+            r.dst = g1s[m]; r.dpitch = D.pitch; r.dplane = D.plane; r.dw = D.w; r.dh = D.h;
+            r.dalpha = abuf[0]; r.dapitch = apl;
+            r.channels = C; r.frac = frac;
+            // a float image's level 0 (its log transform) is also written, into
+            // the pyramid's level 0, for the level-0 merge to read
+            const bool conv = itype == ENBL_F32;
+            if (conv) { r.conv = level_ptr(lps[m], 0); r.cpitch = F.pitch; r.cplane = F.plane; }
+            char name[64];
+            snprintf(name, sizeof name, "reduce_image0_%s_c%d", pname(pt), C);
+            const double n0 = (double)F.w * F.h, n1 = (double)D.w * D.h;
+            Timed t(ctx, name, n0 * (C * (double)ptype_size(itype) + 1.0 + (conv ? esP : 0.0)) + n1 * (esP + 1.0));
+            rc = launch_reduce_image(itype, pt, r, ctx->stream);
+            if (rc) return fail(rc, "level-0 reduce: unsupported type");
+            rc = launch_status(name);
+            if (!rc && s >= 2)
+                rc = fused_laplacian_levels(ctx, itype, frac, imgs[m], ips[m], alphas[m], aps[m], lps[m], s, abuf, apl, 1,
+                                            g1s[m]);
+            else if (!rc)
+                rc = hip_check(hipMemcpyAsync(level_ptr(lps[m], 1), g1s[m], (size_t)D.plane * C * ptype_size(pt),
+                                              hipMemcpyDeviceToDevice, ctx->stream), "level 1");
         } else {
             uint8_t* abuf[2] = {pp[0], pp[1]};
             abuf[(s - 1) & 1] = ialpha(m, s);
@@ -759,6 +795,18 @@ int scratch(enbl_context* ctx, int slot, size_t bytes, void** out)
         ctx->scratch_bytes[slot] = bytes;
     }
     *out = ctx->scratch_buf[slot];
+    return ENBL_OK;
+}
+
+int collapse_levels(enbl_context* ctx, int wrap, enbl_pyramid* p, int lmin)
+{
+    for (int l = p->levels - 2; l >= lmin; --l) {
+        const Level& f = p->lv[l];
+        const Level& c = p->lv[l + 1];
+        int rc = do_expand(ctx, p->ptype, p->channels, 1, wrap, level_ptr(p, l + 1), c.w, c.h, c.pitch, c.plane,
+                           level_ptr(p, l), f.w, f.h, f.pitch, f.plane);
+        if (rc) return rc;
+    }
     return ENBL_OK;
 }
 
@@ -1064,14 +1112,7 @@ int enbl_pyramid_build_levels(enbl_context* ctx, enbl_pyramid* p, int wrap, cons
 int enbl_collapse_pyramid(enbl_context* ctx, int wrap, enbl_pyramid* p)
 {
     if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_collapse_pyramid: null argument");
-    for (int l = p->levels - 2; l >= 0; --l) {
-        const Level& f = p->lv[l];
-        const Level& c = p->lv[l + 1];
-        int rc = do_expand(ctx, p->ptype, p->channels, 1, wrap, level_ptr(p, l + 1), c.w, c.h, c.pitch, c.plane,
-                           level_ptr(p, l), f.w, f.h, f.pitch, f.plane);
-        if (rc) return rc;
-    }
-    return ENBL_OK;
+    return collapse_levels(ctx, wrap, p, 0);
 }
 
 int enbl_blend(enbl_context* ctx, const enbl_pyramid* m, const enbl_pyramid* wlp, enbl_pyramid* blp, double wv)

@@ -87,4 +87,27 @@ int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int
 int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, const int* tile0, int njobs,
                       int total_tiles, const BPlane* planes, double maxpyr, int dither, int rows, int grid, hipStream_t s);
 
+// Level 0 of one stack's merge and collapse with every image's Laplacian
+// level 0 formed on the fly from the image (ConvertScalarToPyramidFunctor or
+// the float log transform) and its Gaussian level 1: the result, plus the
+// expanded collapsed result level 1, goes straight into the output image
+// under the union alpha, and the union alpha into the output alpha.  Passed
+// by value (no tables).
+constexpr int kL0Max = 16;
+struct L0FuseArgs {
+    int K, fw, fh, cw, ch, rows;
+    const void* img[kL0Max]; long ipitch[kL0Max];   // interleaved RGB images, pitch in elements
+    const void* g0[kL0Max]; long g0pitch, g0plane;   // float images: their converted level 0 (or null)
+    const void* g1[kL0Max]; long gpitch, gplane;     // Gaussian level 1 of every image
+    const void* mask[kL0Max]; long mpitch;           // normalised mask level 0 of every image
+    const void* rc; long rcpitch, rcplane;           // collapsed result level 1
+    const uint8_t* ualpha; long upitch;
+    void* out; long opitch;                          // pitch in elements
+    uint8_t* oalpha; long oapitch;
+    double maxpyr;
+    int frac, dither;
+    uint64_t seed;
+};
+int launch_enfuse_level0(int itype, L0FuseArgs a, hipStream_t s);
+
 }  // namespace enbl

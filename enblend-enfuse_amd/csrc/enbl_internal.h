@@ -32,6 +32,9 @@ struct ReduceArgs {
     int force_small;   // ENBL_KERNELS_* policy
     int frac;          // interleaved integer image source: fixed-point shift
     int pad_;          // 0 (no uninitialised bytes: batched job tables are compared bytewise)
+    // Image sources of the level-0 reduce of a float image: the converted
+    // level 0 is also written here (planes, pitch in elements) when non-null.
+    void* conv; long cpitch; long cplane;
 };
 
 struct ExpandArgs {
@@ -147,8 +150,18 @@ int laplacian_pyramid_any(enbl_context* ctx, int wrap, int itype, int frac, cons
 // enfuse's K Laplacian pyramids (image alphas) and K mask Gaussian pyramids
 // (one shared alpha, level 0 written) without wraparound, the small levels
 // of each set in shared multi-job launches.
+// With g1s (one buffer of level-1 geometry per image): level 0 is only
+// reduced, from the image, into g1s[m], and the pyramids hold Laplacian
+// levels 1.. and, for float images, the converted level 0 (Gaussian, not
+// Laplacian; integer images leave level 0 unwritten).  Needs
+// enfuse_level0_count >= 1.
 int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const void* const* imgs, const long* ips,
-                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps);
+                          const uint8_t* const* alphas, const long* aps, enbl_pyramid* const* lps,
+                          void* const* g1s = nullptr);
+// Levels whose coarse level is built by a streaming pass (no wraparound).
+int enfuse_level0_count(const enbl_pyramid* p, const uint8_t* alpha);
+// collapsePyramid (src/pyramid.h:1837-1886) of levels levels-2 .. lmin.
+int collapse_levels(enbl_context* ctx, int wrap, enbl_pyramid* p, int lmin);
 int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap);
 // Overlapping work: fork_aux makes the auxiliary stream wait for the work
 // issued so far and directs later launches to it; end_aux records its end

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "enbl_batch.h"
 #include "enbl_enfuse.h"
 #include "enbl_internal.h"
 
@@ -565,6 +566,22 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         bool active = false;
         ~AuxJoin() { if (active) (void)join_aux(ctx); }
     } aux{ctx};
+    // Level 0 on the fly: when level 1 is a streaming level, each image's
+    // level 0 is only reduced into its Gaussian level 1 (kept apart), and the
+    // merge of level 0, the last collapse step and the conversion into the
+    // output are one pass that forms every Laplacian level 0 from the image
+    // and that level 1: no level 0 of any pyramid is stored.
+    const bool fuse0 = !wrap && cs == ENBL_COLORSPACE_IDENTITY && levels >= 2 && K <= kL0Max &&
+                       enfuse_level0_count(lps[0], alphas[0]) >= 1;
+    std::vector<void*> g1s;
+    if (fuse0) {
+        const Level& D = lps[0]->lv[1];
+        const size_t gbytes = round_up((size_t)D.plane * 3 * ptype_size(pt), 256);
+        void* g;
+        rc = scratch(ctx, 37, gbytes * (size_t)K, &g);
+        if (rc) return rc;
+        for (int m = 0; m < K; ++m) g1s.push_back((char*)g + gbytes * (size_t)m);
+    }
     if (!wrap && jch) {
         // (level by level on the main stream: these reduces share the
         // context's alpha scratch with the mask pyramids)
@@ -574,7 +591,8 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         rc = fork_aux(ctx);
         if (rc) return rc;
         aux.active = true;
-        rc = enfuse_image_pyramids(ctx, itype, ifrac, K, pimgs.data(), pips.data(), alphas, aps, lps.data());
+        rc = enfuse_image_pyramids(ctx, itype, ifrac, K, pimgs.data(), pips.data(), alphas, aps, lps.data(),
+                                   fuse0 ? g1s.data() : nullptr);
         const int rc2 = end_aux(ctx);
         if (!rc) rc = rc2;
         if (rc) return rc;
@@ -631,7 +649,7 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
             if (rc) return rc;
         }
     }
-    for (int l = 0; l < levels; ++l) {
+    for (int l = fuse0 ? 1 : 0; l < levels; ++l) {
         const Level& L = lps[0]->lv[l];
         const Level& G = mgps[0]->lv[l];
         const Level& R = res->lv[l];
@@ -658,8 +676,41 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
             if (rc) return rc;
         }
     }
-    rc = enbl_collapse_pyramid(ctx, wrap, res);
+    rc = collapse_levels(ctx, wrap, res, fuse0 ? 1 : 0);
     if (rc) return rc;
+    if (fuse0) {
+        L0FuseArgs a;
+        memset(&a, 0, sizeof a);
+        const Level& C1 = lps[0]->lv[1];
+        const Level& R1 = res->lv[1];
+        a.K = K; a.fw = w; a.fh = h; a.cw = C1.w; a.ch = C1.h;
+        const bool g0 = itype == ENBL_F32;   // the converted level 0 is in each image pyramid's level 0
+        for (int m = 0; m < K; ++m) {
+            a.img[m] = imgs[m]; a.ipitch[m] = ips[m];
+            a.g0[m] = g0 ? level_ptr(lps[(size_t)m], 0) : nullptr;
+            a.g1[m] = g1s[(size_t)m];
+            a.mask[m] = level_ptr(mgps[(size_t)m], 0);
+        }
+        a.g0pitch = lps[0]->lv[0].pitch; a.g0plane = lps[0]->lv[0].plane;
+        a.gpitch = C1.pitch; a.gplane = C1.plane;
+        a.mpitch = mgps[0]->lv[0].pitch;
+        a.rc = level_ptr(res, 1); a.rcpitch = R1.pitch; a.rcplane = R1.plane;
+        a.ualpha = (const uint8_t*)ua; a.upitch = pitch;
+        a.out = out; a.opitch = op;
+        a.oalpha = oa; a.oapitch = oap;
+        a.maxpyr = maxpyr; a.frac = ifrac; a.dither = dither; a.seed = seed;
+        // own bytes: per image its pixels, mask level 0 and Gaussian level 1
+        // read once; result level 1 read once; union alpha read, output image
+        // and alpha written
+        const double esP = 3.0 * ptype_size(pt), es_i = 3.0 * ptype_size(itype);
+        const double n0 = (double)w * h, n1 = (double)C1.w * C1.h;
+        char name[64];
+        snprintf(name, sizeof name, "merge_level0_%s_c3", pname(pt));
+        Timed t(ctx, name, K * (n0 * ((g0 ? esP : es_i) + ptype_size(mt)) + n1 * esP) + n1 * esP + n0 * (2.0 + es_i));
+        rc = launch_enfuse_level0(itype, a, s);
+        if (rc) return fail(rc, "level-0 merge: unsupported type");
+        return launch_status(name);
+    }
     if (jch) {
         const Level& R0 = res->lv[0];
         rc = jch_to_rgb_if(ctx, itype, w, h, level_ptr(res, 0), R0.pitch, R0.plane, (const uint8_t*)ua, pitch, out, op);

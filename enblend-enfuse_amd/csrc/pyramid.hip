@@ -47,6 +47,17 @@ __global__ __launch_bounds__(256) void reduce_image_kernel(ReduceArgs a)
     reduce_strip_ring<P, C, ALPHA, IntImageSrc<I, P, C, WIDE>, AW>(a, wx, blockIdx.y * a.rows);
 }
 
+// Level 0 read from the interleaved float image, 1 + log1p(v) or
+// 1 / (1 - v) in double on the fly (src/fixmath.h:165-186): the reduce of
+// a float image's level 0 without storing that level.
+// W0: the converted level 0 is written to a.conv as well.
+template <int C, bool ALPHA, bool AW, bool W0>
+__global__ __launch_bounds__(256) void reduce_float_image_kernel(ReduceArgs a)
+{
+    const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    reduce_strip_ring<double, C, ALPHA, FloatImageSrc<C>, AW, W0>(a, wx, blockIdx.y * a.rows);
+}
+
 // ---------------------------------------------------------------------------
 // expand
 // ---------------------------------------------------------------------------
@@ -307,6 +318,22 @@ void launch_reduce_image_t(ReduceArgs a, hipStream_t s)
     else hipLaunchKernelGGL((reduce_image_kernel<I, P, C, ALPHA, false, false>), grid, dim3(256), 0, s, a);
 }
 
+template <int C, bool ALPHA>
+void launch_reduce_float_image_t(ReduceArgs a, hipStream_t s)
+{
+    const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
+    if (a.rows <= 0) a.rows = strip_rows(waves_x, a.dh);
+    dim3 grid((waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows);
+    const bool aw = !ALPHA || (((uintptr_t)a.salpha & 1) == 0 && (a.sapitch & 1) == 0);
+    if (a.conv) {
+        if (aw) hipLaunchKernelGGL((reduce_float_image_kernel<C, ALPHA, true, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((reduce_float_image_kernel<C, ALPHA, false, true>), grid, dim3(256), 0, s, a);
+    } else {
+        if (aw) hipLaunchKernelGGL((reduce_float_image_kernel<C, ALPHA, true, false>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((reduce_float_image_kernel<C, ALPHA, false, false>), grid, dim3(256), 0, s, a);
+    }
+}
+
 template <class P, int C, bool ADD>
 void launch_expand_t(ExpandArgs a, hipStream_t s)
 {
@@ -367,6 +394,11 @@ int launch_reduce_image(int itype, int ptype, ReduceArgs a, hipStream_t s)
     if (itype == ENBL_U8 && ptype == ENBL_I32) ENBL_RI(uint8_t, int32_t);
     if (itype == ENBL_U16 && ptype == ENBL_I32) ENBL_RI(uint16_t, int32_t);
 #undef ENBL_RI
+    if (itype == ENBL_F32 && ptype == ENBL_F64) {
+        if (a.channels == 1) alpha ? launch_reduce_float_image_t<1, true>(a, s) : launch_reduce_float_image_t<1, false>(a, s);
+        else alpha ? launch_reduce_float_image_t<3, true>(a, s) : launch_reduce_float_image_t<3, false>(a, s);
+        return 0;
+    }
     return ENBL_E_UNSUPPORTED;
 }
 

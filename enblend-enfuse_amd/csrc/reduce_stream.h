@@ -293,7 +293,10 @@ __device__ __forceinline__ void reduce_strip(const ReduceArgs& a, int wx, int i0
 // register is copied, and every step issues the same memory instructions,
 // so its wait covers the previous step's loads only.  SRCF: a row source of
 // row_fetch.h; AW: alpha rows 2-byte aligned (one 16-bit load per pair).
-template <class P, int C, bool ALPHA, class SRCF, bool AW>
+// W0: also write the source rows as converted (the source's level 0, planar
+// doubles at a.conv), each fine row by the strip that owns it: rows
+// [2 i0, 2 i1) of the strip's coarse rows [i0, i1), columns by owned lanes.
+template <class P, int C, bool ALPHA, class SRCF, bool AW, bool W0 = false>
 __device__ __forceinline__ void reduce_strip_ring(const ReduceArgs& a, int wx, int i0)
 {
     typedef Pix<P> Px;
@@ -333,9 +336,22 @@ __device__ __forceinline__ void reduce_strip_ring(const ReduceArgs& a, int wx, i
     };
     // Horizontal 5-tap sums of a fetched row; the row-tail cases are
     // selected per lane, in the reference's operation order.
-    auto sums = [&](const RawRow& R, S (&h)[C], int& hm) {
+    auto sums = [&](const RawRow& R, S (&h)[C], int& hm, int wrow) {
         S x0[C], x1[C];
         SRCF::unpack(a, R.px, x0, x1);
+        if constexpr (W0) {
+            if (wrow >= 0 && owned) {
+                const S* crow = static_cast<const S*>(a.conv) + (long)wrow * a.cpitch;
+#pragma unroll
+                for (int ch = 0; ch < C; ++ch) {
+                    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(crow + ch * a.cplane, (uint32_t)sw * (uint32_t)sizeof(S));
+                    if (both) BPair<S>::store2(rs, (uint32_t)c * (uint32_t)sizeof(S), x0[ch], x1[ch]);
+                    else BPair<S>::store1(rs, (uint32_t)c * (uint32_t)sizeof(S), x0[ch]);
+                }
+            }
+        } else {
+            (void)wrow;
+        }
         int m0 = 1, m1 = 1;
         if constexpr (ALPHA) {
             m0 = valid && R.al.first() ? 1 : 0;
@@ -380,10 +396,10 @@ __device__ __forceinline__ void reduce_strip_ring(const ReduceArgs& a, int wx, i
         fetch(2 * i0 + 1, true, ring[0]);
         fetch(2 * i0 + 2, true, ring[1]);
         if (i0 > 0) {
-            sums(t0, hm2, qm2);
-            sums(t1, hm1, qm1);
+            sums(t0, hm2, qm2, -1);
+            sums(t1, hm1, qm1, -1);
         }
-        sums(t2, hc, qc);
+        sums(t2, hc, qc, 2 * i0);
     }
     P* dst = static_cast<P*>(a.dst);
     const uint32_t offd = owned ? (uint32_t)j * ES : kBufSkip;
@@ -396,8 +412,8 @@ __device__ __forceinline__ void reduce_strip_ring(const ReduceArgs& a, int wx, i
         const bool next = i + 1 < i1;
         fetch(r + 3, next, ring[N0]);
         fetch(r + 4, next, ring[N1]);
-        if (r + 1 < sh) sums(ring[F0], hp1, qp1);
-        if (r + 2 < sh) sums(ring[F1], hp2, qp2);
+        if (r + 1 < sh) sums(ring[F0], hp1, qp1, r + 1);
+        if (r + 2 < sh) sums(ring[F1], hp2, qp2, next ? r + 2 : -1);
 
         int ap = 0;
         if (ALPHA) {
