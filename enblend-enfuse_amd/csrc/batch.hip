@@ -635,11 +635,19 @@ __device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale
     const int cw = a.cw, chh = a.ch;
     const int Y = 2 * i;
     const bool yb = Y + 1 < a.fh;
+    int sh[4];
     double inv[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) inv[q] = P2 ? __builtin_amdgcn_ldexp(1.0, -__builtin_ctz((unsigned)dv[q])) : 0.0;
+    for (int q = 0; q < 4; ++q) {
+        sh[q] = P2 ? __builtin_ctz((unsigned)dv[q]) : 0;
+        inv[q] = P2 ? __builtin_amdgcn_ldexp(1.0, -sh[q]) : 0.0;
+    }
+    // x / dv[q]: the product with the reciprocal (double) or the shift with
+    // the round-toward-zero correction (integers) when the wave's divisors
+    // are powers of two
     auto qdiv = [&](S v, int q) -> S {
         if constexpr (P2 && std::is_same<P, double>::value) return v * inv[q];
+        else if constexpr (P2) return (S)((v + ((v >> 31) & ((1 << sh[q]) - 1))) >> sh[q]);
         else return A::div(v, dv[q]);
     };
     S acc[4][3];
@@ -735,11 +743,9 @@ int launch_enfuse_level0(int itype, L0FuseArgs a, hipStream_t s)
     const int waves_x = (a.cw + kOwned - 1) / kOwned;
     const dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
     const double inv_scale = 255.0 / a.maxpyr;   // 2^-mfrac, exact
-    if (itype == ENBL_U8)
-        hipLaunchKernelGGL((enfuse_level0_kernel<int16_t, int16_t, uint8_t, false>), grid, dim3(256), 0, s, a, inv_scale);
-    else if (itype == ENBL_U16)
-        hipLaunchKernelGGL((enfuse_level0_kernel<int32_t, int32_t, uint16_t, false>), grid, dim3(256), 0, s, a, inv_scale);
-    else if (itype == ENBL_F32 && a.g0[0])
+    // (8- and 16-bit stacks keep their stored level 0 in the per-stack path;
+    // the integer instances would form it as (P)v << frac)
+    if (itype == ENBL_F32 && a.g0[0])
         hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, true>), grid, dim3(256), 0, s, a, inv_scale);
     else if (itype == ENBL_F32)
         hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, false>), grid, dim3(256), 0, s, a, inv_scale);
