@@ -650,6 +650,17 @@ __device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale
         else if constexpr (P2) return (S)((v + ((v >> 31) & ((1 << sh[q]) - 1))) >> sh[q]);
         else return A::div(v, dv[q]);
     };
+    // the norm of the 2 x 2 block, shared by the images' mask values
+    float nv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) {
+        const float* nrow = a.norm + (long)Y * a.npitch + X;
+        nv[0] = *gmem(nrow);
+        nv[1] = xb ? *gmem(nrow + 1) : 0.f;
+        if (yb) {
+            nv[2] = *gmem(nrow + a.npitch);
+            nv[3] = xb ? *gmem(nrow + a.npitch + 1) : 0.f;
+        }
+    }
     S acc[4][3];
     for (int m = 0; m < a.K; ++m) {
         S c[3][3], f[4][3];
@@ -660,7 +671,15 @@ __device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) f[q][ch] = A::zero();
         if (valid) {
-            const M* mrow = static_cast<const M*>(a.mask[m]) + (long)Y * a.mpitch + X;
+            const float* wrow = a.wt[m] + (long)Y * a.wpitch + X;
+            float wv[4];
+            wv[0] = *gmem(wrow);
+            wv[1] = xb ? *gmem(wrow + 1) : 0.f;
+            wv[2] = yb ? *gmem(wrow + a.wpitch) : 0.f;
+            wv[3] = xb && yb ? *gmem(wrow + a.wpitch + 1) : 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                mv[q] = (int32_t)from_real<M>(norm_mask_value(wv[q], nv[q], a.fallback, a.hard, a.mscale));
             if constexpr (G0) {
                 const RowSrc rs{a.g0[m], a.g0pitch, a.g0plane, 0};
                 PlaneRows<P>::template load<3>(rs, Y, X, xb, f[0], f[1]);
@@ -669,8 +688,6 @@ __device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale
                 L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y, X, xb, a.frac, f[0], f[1]);
                 if (yb) L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y + 1, X, xb, a.frac, f[2], f[3]);
             }
-            Vec2<M>::load(mrow, xb, mv[0], mv[1]);
-            if (yb) Vec2<M>::load(mrow + a.mpitch, xb, mv[2], mv[3]);
         }
         S o[4][3];
         expand_sums<P, 3>(c, i, j, cw, chh, o);

@@ -728,7 +728,8 @@ int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const v
 // alpha's levels), the levels below of all K in shared multi-job launches.
 // Same results as gaussian_levels per mask.  Scratch: slot 36 (the alpha's
 // levels).
-int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap)
+int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap,
+                         const MaskLevel0* l0)
 {
     const enbl_pyramid* g0 = mgps[0];
     const int L = g0->levels;
@@ -751,14 +752,35 @@ int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, co
     }
     auto ualpha = [&](int l) -> uint8_t* { return l == 0 ? const_cast<uint8_t*>(ua) : base + uoff[l]; };
     auto upitch = [&](int l) { return l == 0 ? uap : tpitch(l); };
+    if (l0 && s < 1) return fail(ENBL_E_ARG, "enfuse_mask_pyramids: level 1 is not a streaming level");
     for (int m = 0; m < K; ++m) {
         enbl_pyramid* g = mgps[m];
         for (int l = 0; l < s; ++l) {
             const Level& S_ = g->lv[l];
             const Level& D = g->lv[l + 1];
-            int rc = do_reduce(ctx, mt, 1, 0, level_ptr(g, l), S_.w, S_.h, S_.pitch, S_.plane, ualpha(l), upitch(l),
+            int rc;
+            if (l == 0 && l0) {
+                // level 0 formed from the weights and the norm on the fly
+                NormReduceArgs a;
+                memset(&a, 0, sizeof a);
+                a.src = l0->w[m]; a.spitch = l0->wpitch; a.splane = 0; a.sw = S_.w; a.sh = S_.h;
+                a.salpha = ua; a.sapitch = uap;
+                a.dst = level_ptr(g, 1); a.dpitch = D.pitch; a.dplane = D.plane; a.dw = D.w; a.dh = D.h;
+                a.dalpha = m == 0 ? ualpha(1) : nullptr; a.dapitch = upitch(1);
+                a.channels = 1;
+                a.norm = l0->norm; a.npitch = l0->npitch;
+                a.scale = l0->scale; a.fallback = l0->fallback; a.hard = l0->hard;
+                char name[64];
+                snprintf(name, sizeof name, "reduce_mask0_norm_%s", pname(mt));
+                Timed t(ctx, name, (double)S_.w * S_.h * 9.0 + (double)D.w * D.h * (esM + (m == 0 ? 1.0 : 0.0)));
+                rc = launch_reduce_norm_mask(mt, a, ctx->stream);
+                if (rc) return fail(rc, "mask level-0 reduce: unsupported type");
+                rc = launch_status(name);
+            } else {
+                rc = do_reduce(ctx, mt, 1, 0, level_ptr(g, l), S_.w, S_.h, S_.pitch, S_.plane, ualpha(l), upitch(l),
                                level_ptr(g, l + 1), D.w, D.h, D.pitch, D.plane, m == 0 ? ualpha(l + 1) : nullptr,
                                upitch(l + 1));
+            }
             if (rc) return rc;
         }
     }

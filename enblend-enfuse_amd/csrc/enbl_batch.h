@@ -99,7 +99,9 @@ struct L0FuseArgs {
     const void* img[kL0Max]; long ipitch[kL0Max];   // interleaved RGB images, pitch in elements
     const void* g0[kL0Max]; long g0pitch, g0plane;   // float images: their converted level 0 (or null)
     const void* g1[kL0Max]; long gpitch, gplane;     // Gaussian level 1 of every image
-    const void* mask[kL0Max]; long mpitch;           // normalised mask level 0 of every image
+    const float* wt[kL0Max]; long wpitch;            // weight plane of every image (mask level 0
+    const float* norm; long npitch;                  //   = fromRealPromote(normalised weight * mscale),
+    double mscale; float fallback; int hard;         //   formed on the fly)
     const void* rc; long rcpitch, rcplane;           // collapsed result level 1
     const uint8_t* ualpha; long upitch;
     void* out; long opitch;                          // pitch in elements

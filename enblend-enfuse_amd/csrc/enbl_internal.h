@@ -37,6 +37,17 @@ struct ReduceArgs {
     void* conv; long cpitch; long cplane;
 };
 
+// A mask pyramid's reduce from level 0 formed on the fly: src is the image's
+// float weight plane, norm the float norm plane (pitch in elements), and each
+// level-0 value fromRealPromote((hard ? w : norm > 0 ? 255 w / norm :
+// fallback) * scale).
+struct NormReduceArgs : ReduceArgs {
+    const float* norm; long npitch;
+    double scale;
+    float fallback;
+    int hard;
+};
+
 struct ExpandArgs {
     const void* src; long spitch; long splane; int sw; int sh;
     void* dst; long dpitch; long dplane; int dw; int dh;
@@ -48,6 +59,7 @@ int launch_reduce(int ptype, ReduceArgs a, hipStream_t s);
 // Streaming reduce whose source is an interleaved integer image of itype
 // (level 0 converted on the fly, a.frac); no wraparound.
 int launch_reduce_image(int itype, int ptype, ReduceArgs a, hipStream_t s);
+int launch_reduce_norm_mask(int mtype, NormReduceArgs a, hipStream_t s);
 int launch_expand(int ptype, ExpandArgs a, hipStream_t s);
 
 // Pointwise kernels (pointwise.hip).
@@ -162,7 +174,18 @@ int enfuse_image_pyramids(enbl_context* ctx, int itype, int frac, int K, const v
 int enfuse_level0_count(const enbl_pyramid* p, const uint8_t* alpha);
 // collapsePyramid (src/pyramid.h:1837-1886) of levels levels-2 .. lmin.
 int collapse_levels(enbl_context* ctx, int wrap, enbl_pyramid* p, int lmin);
-int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap);
+// l0: level 0 of each mask is not stored but formed from the images' weight
+// planes and the norm (the normalisation pass's values) by the first reduce
+// (needs a streaming level 1).
+struct MaskLevel0 {
+    const float* const* w; long wpitch;
+    const float* norm; long npitch;
+    double scale;
+    float fallback;
+    int hard;
+};
+int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap,
+                         const MaskLevel0* l0 = nullptr);
 // Overlapping work: fork_aux makes the auxiliary stream wait for the work
 // issued so far and directs later launches to it; end_aux records its end
 // and directs launches back to the main stream; join_aux makes the main

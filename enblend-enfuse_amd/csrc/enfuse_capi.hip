@@ -633,11 +633,16 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
                                  fallback, scale, p->hard_mask, level_ptr(mgp, 0), L0.pitch, s);
         return r ? r : launch_status("normalize");
     };
+    std::vector<const float*> wts;
+    for (int m = 0; m < K; ++m) wts.push_back((const float*)masks + plane * m);
+    const MaskLevel0 ml0{wts.data(), pitch, (const float*)norm, pitch, scale, fallback, p->hard_mask};
     if (!wrap) {
-        // mask level 0 of every image, the mask pyramids with their small
-        // levels in shared launches, then the image pyramids' end
-        for (int m = 0; m < K && !rc; ++m) rc = normalize(m);
-        if (!rc) rc = enfuse_mask_pyramids(ctx, K, mgps.data(), (const uint8_t*)ua, pitch);
+        // mask level 0 of every image (formed on the fly from the weights and
+        // the norm where level 0 is merged on the fly), the mask pyramids with
+        // their small levels in shared launches, then the image pyramids' end
+        if (!fuse0)
+            for (int m = 0; m < K && !rc; ++m) rc = normalize(m);
+        if (!rc) rc = enfuse_mask_pyramids(ctx, K, mgps.data(), (const uint8_t*)ua, pitch, fuse0 ? &ml0 : nullptr);
         if (rc) return rc;
         aux.active = false;
         rc = join_aux(ctx);
@@ -692,11 +697,12 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
             a.img[m] = imgs[m]; a.ipitch[m] = ips[m];
             a.g0[m] = g0 ? level_ptr(lps[(size_t)m], 0) : nullptr;
             a.g1[m] = g1s[(size_t)m];
-            a.mask[m] = level_ptr(mgps[(size_t)m], 0);
+            a.wt[m] = wts[(size_t)m];
         }
+        a.wpitch = pitch; a.norm = (const float*)norm; a.npitch = pitch;
+        a.mscale = scale; a.fallback = fallback; a.hard = p->hard_mask;
         a.g0pitch = lps[0]->lv[0].pitch; a.g0plane = lps[0]->lv[0].plane;
         a.gpitch = C1.pitch; a.gplane = C1.plane;
-        a.mpitch = mgps[0]->lv[0].pitch;
         a.rc = level_ptr(res, 1); a.rcpitch = R1.pitch; a.rcplane = R1.plane;
         a.ualpha = (const uint8_t*)ua; a.upitch = pitch;
         a.out = out; a.opitch = op;
@@ -709,7 +715,7 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         const double n0 = (double)w * h, n1 = (double)C1.w * C1.h;
         char name[64];
         snprintf(name, sizeof name, "merge_level0_%s_c3", pname(pt));
-        Timed t(ctx, name, K * (n0 * ((g0 ? esP : es_i) + ptype_size(mt)) + n1 * esP) + n1 * esP + n0 * (2.0 + es_i));
+        Timed t(ctx, name, K * (n0 * ((g0 ? esP : es_i) + 4.0) + n1 * esP) + n1 * esP + n0 * (2.0 + 4.0 + es_i));
         rc = launch_enfuse_level0(itype, a, s);
         if (rc) return fail(rc, "level-0 merge: unsupported type");
         return launch_status(name);

@@ -58,6 +58,15 @@ __global__ __launch_bounds__(256) void reduce_float_image_kernel(ReduceArgs a)
     reduce_strip_ring<double, C, ALPHA, FloatImageSrc<C>, AW, W0>(a, wx, blockIdx.y * a.rows);
 }
 
+// Level 0 -> 1 of a mask Gaussian pyramid under the union alpha, level 0
+// formed on the fly from the weight and norm planes (NormMaskSrc).
+template <class M, bool AW>
+__global__ __launch_bounds__(256) void reduce_norm_mask_kernel(NormReduceArgs a)
+{
+    const int wx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    reduce_strip_ring<M, 1, true, NormMaskSrc<M>, AW>(a, wx, blockIdx.y * a.rows);
+}
+
 // ---------------------------------------------------------------------------
 // expand
 // ---------------------------------------------------------------------------
@@ -397,6 +406,26 @@ int launch_reduce_image(int itype, int ptype, ReduceArgs a, hipStream_t s)
     if (itype == ENBL_F32 && ptype == ENBL_F64) {
         if (a.channels == 1) alpha ? launch_reduce_float_image_t<1, true>(a, s) : launch_reduce_float_image_t<1, false>(a, s);
         else alpha ? launch_reduce_float_image_t<3, true>(a, s) : launch_reduce_float_image_t<3, false>(a, s);
+        return 0;
+    }
+    return ENBL_E_UNSUPPORTED;
+}
+
+int launch_reduce_norm_mask(int mtype, NormReduceArgs a, hipStream_t s)
+{
+    if (a.wrap || a.channels != 1 || !a.salpha) return ENBL_E_UNSUPPORTED;
+    const int waves_x = (a.dw + kOwned - 1) / kOwned + 1;
+    if (a.rows <= 0) a.rows = strip_rows(waves_x, a.dh);
+    dim3 grid((waves_x + 3) / 4, (a.dh + a.rows - 1) / a.rows);
+    const bool aw = ((uintptr_t)a.salpha & 1) == 0 && (a.sapitch & 1) == 0;
+    if (mtype == ENBL_I32) {
+        if (aw) hipLaunchKernelGGL((reduce_norm_mask_kernel<int32_t, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((reduce_norm_mask_kernel<int32_t, false>), grid, dim3(256), 0, s, a);
+        return 0;
+    }
+    if (mtype == ENBL_I16) {
+        if (aw) hipLaunchKernelGGL((reduce_norm_mask_kernel<int16_t, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((reduce_norm_mask_kernel<int16_t, false>), grid, dim3(256), 0, s, a);
         return 0;
     }
     return ENBL_E_UNSUPPORTED;

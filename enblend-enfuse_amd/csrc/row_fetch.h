@@ -179,6 +179,36 @@ template <class I, class P, int C, bool WIDE> struct IntImageSrc {
     }
 };
 
+// Mask level 0 formed from the image's weight plane (src, float, pitch
+// spitch) and the norm plane: the normalisation (src/enfuse.h:1529-1534) with
+// the fixed-point conversion (src/fixmath.h:188-194), as normalize_kernel
+// computes it; hard masks are converted as they are.
+__device__ __forceinline__ double norm_mask_value(float mv, float nv, float fallback, int hard, double scale)
+{
+    const float v = hard ? mv : nv > 0.0 ? (255.0f * mv) / nv : fallback;
+    return (double)v * scale;
+}
+template <class M> struct NormMaskSrc {
+    typedef int32_t S;
+    struct Raw { v2i32 w, n; };
+    template <class Args>
+    __device__ static __forceinline__ void fetch(const Args& a, int fw, int r, bool row_ok, const LaneCols& L, Raw& R) {
+        const NormReduceArgs& na = static_cast<const NormReduceArgs&>(a);
+        const float* wrow = static_cast<const float*>(a.src) + (long)r * a.spitch;
+        const float* nrow = na.norm + (long)r * na.npitch;
+        const uint32_t bytes = row_ok ? round_up2((uint32_t)fw) * 4u : 0u;
+        const uint32_t off = L.valid ? (uint32_t)L.c * 4u : kBufSkip;
+        R.w = bload<v2i32>(buf_rsrc(wrow, bytes), off);
+        R.n = bload<v2i32>(buf_rsrc(nrow, bytes), off);
+    }
+    template <class Args>
+    __device__ static __forceinline__ void unpack(const Args& a, const Raw& R, S (&x0)[1], S (&x1)[1]) {
+        const NormReduceArgs& na = static_cast<const NormReduceArgs&>(a);
+        x0[0] = (S)from_real<M>(norm_mask_value(__int_as_float(R.w.x), __int_as_float(R.n.x), na.fallback, na.hard, na.scale));
+        x1[0] = (S)from_real<M>(norm_mask_value(__int_as_float(R.w.y), __int_as_float(R.n.y), na.fallback, na.hard, na.scale));
+    }
+};
+
 // Interleaved float image, 1 + log1p(v) or 1 / (1 - v) (src/fixmath.h:165-186).
 template <int C> struct FloatImageSrc {
     typedef double S;
