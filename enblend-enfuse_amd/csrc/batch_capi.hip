@@ -96,6 +96,11 @@ struct Plan {
     int frac = 0, dither = 0;
     int max_k = 0;   // deepest stack
     size_t images_off = 0, planes_off = 0;   // BImage / BPlane tables
+    bool stack_weights = false;   // weights made stack by stack (enfuse_stack_masks), no weights launch
+    // per stack: its images' mask level 0 and its union alpha in the arena
+    std::vector<std::vector<void*>> mask0;
+    std::vector<long> mpitch, upitch;
+    std::vector<uint8_t*> ualpha;
 };
 
 // Device layout of one stack inside the arena.
@@ -129,10 +134,20 @@ struct Table {
     template <class T> const T* dev(size_t off) const { return reinterpret_cast<const T*>((uintptr_t)dbase + off); }
 };
 
+// The batched weights pass: exposure (average projector, no cutoff) and
+// saturation of integer images, soft masks.
 int batchable(int itype, const enbl_enfuse_params* p, int wrap)
 {
     return (itype == ENBL_U8 || itype == ENBL_U16) && p->w_contrast <= 0.0 && p->w_entropy <= 0.0 && !p->hard_mask &&
            !p->exposure_cutoff && p->gray_projector == ENBL_GRAY_AVERAGE && !wrap;
+}
+
+// The batched pyramid passes: integer images without wraparound, any weights
+// (made stack by stack by the weight kernels of enbl_enfuse when the batched
+// weights pass does not cover them).
+int batchable_pyramids(int itype, int wrap)
+{
+    return (itype == ENBL_U8 || itype == ENBL_U16) && !wrap;
 }
 
 int check_stacks(int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p)
@@ -245,6 +260,10 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
     const int es = ptype_size(pt), ms = ptype_size(mt);
     P.tab.clear();
     P.launches.clear();
+    P.mask0.clear();
+    P.mpitch.clear();
+    P.upitch.clear();
+    P.ualpha.clear();
     Table T(P.tab, dbase);
     P.lut_off = T.put(lut.data(), lut.size());
     auto tiles = [](int w, int h, int rows, int* tx) { *tx = (w + 63) / 64; return *tx * ((h + rows - 1) / rows); };
@@ -299,6 +318,11 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             J.mpitch = Y.msk[0].pitch;
             J.ualpha = (uint8_t*)at(Y.ual_at(S.K) + Y.ual[0].offset);
             J.upitch = Y.ual[0].pitch;
+            P.mask0.emplace_back();
+            for (int m = 0; m < S.K; ++m) P.mask0.back().push_back(images[(size_t)J.img0 + m].mask0);
+            P.mpitch.push_back(J.mpitch);
+            P.ualpha.push_back(J.ualpha);
+            P.upitch.push_back(J.upitch);
             J.fallback = (float)(255 / S.K);
             jobs.push_back(J);
             t0.push_back(total);
@@ -459,6 +483,7 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
     std::vector<hipEvent_t> ev;
     int rc = ENBL_OK;
     for (const Launch& L : P.launches) {
+        if (L.kind == kBatchWeights && P.stack_weights) continue;
         const bool timed = kind_ms && L.kind == time_kind;
         if (timed) {
             hipEvent_t a, b;
@@ -505,8 +530,9 @@ int run_launches(enbl_context* ctx, int itype, const Plan& P, char* dtab, int ti
 
 // Plans the batch, sizes and fills the device arena and table buffer.
 int prepare(enbl_context* ctx, BatchState* b, int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p,
-            int dither, const Tuning& tu, Plan& P, bool* same)
+            int dither, const Tuning& tu, Plan& P, bool* same, bool stack_weights = false)
 {
+    P.stack_weights = stack_weights;
     std::vector<StackLayout> lay;
     size_t arena_need = 0;
     layout_stacks(itype, n, st, lay, &arena_need);
@@ -530,7 +556,7 @@ int prepare(enbl_context* ctx, BatchState* b, int itype, int n, const enbl_stack
     for (const Launch& L : P.launches) append(&L, sizeof L);
     const double sc[4] = {P.w_exp, P.w_sat, P.scale, P.maxpyr};
     append(sc, sizeof sc);
-    const int iv[4] = {P.frac, P.dither, itype, P.max_k};
+    const int iv[5] = {P.frac, P.dither, itype, P.max_k, P.stack_weights ? 1 : 0};
     append(iv, sizeof iv);
     *same = b->exec && b->exec_stream == ctx->stream && key == b->key;
     if (*same) return ENBL_OK;
@@ -542,16 +568,29 @@ int prepare(enbl_context* ctx, BatchState* b, int itype, int n, const enbl_stack
     return ENBL_OK;
 }
 
-int run_batch(enbl_context* ctx, int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p, int dither)
+int run_batch(enbl_context* ctx, int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p, int dither,
+              bool stack_weights = false)
 {
     BatchState* b;
     int rc = state(ctx, &b);
     if (rc) return rc;
     Plan P;
     bool same = false;
-    rc = prepare(ctx, b, itype, n, st, p, dither, context_tuning(ctx), P, &same);
+    rc = prepare(ctx, b, itype, n, st, p, dither, context_tuning(ctx), P, &same, stack_weights);
     if (rc) return rc;
     const hipStream_t s = ctx->stream;
+    if (stack_weights) {
+        // the weights the batched pass does not cover, stack by stack, into
+        // the arena's mask levels 0 and union alphas (ahead of the batched
+        // passes on the same stream)
+        for (int k = 0; k < n && !rc; ++k) {
+            const enbl_stack& S = st[k];
+            rc = enfuse_stack_masks(ctx, itype, S.w, S.h, S.K, S.imgs, S.ipitches, S.alphas, S.apitches, p,
+                                    P.ualpha[(size_t)k], P.upitch[(size_t)k], P.mask0[(size_t)k].data(),
+                                    P.mpitch[(size_t)k]);
+        }
+        if (rc) return rc;
+    }
     // Profiling times every launch with events: direct launches then.
     const bool graph = !ctx->profile && s != nullptr;
     if (same && graph) return hip_check(hipGraphLaunch(b->exec, s), "batch graph launch");
@@ -582,18 +621,22 @@ int enbl_enfuse_batch(enbl_context* ctx, int itype, int nstacks, const enbl_stac
     if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_enfuse_batch: null argument");
     int rc = check_stacks(itype, nstacks, stacks, p);
     if (rc || nstacks == 0) return rc;
-    // A blend colour space converts every image around the pyramids; those
-    // stacks run through enbl_enfuse one by one.
-    if (!batchable(itype, p, wrap) || ctx->colorspace != ENBL_COLORSPACE_IDENTITY) {
-        for (int s = 0; s < nstacks; ++s) {
-            const enbl_stack& S = stacks[s];
-            rc = enbl_enfuse(ctx, itype, S.w, S.h, S.K, S.imgs, S.ipitches, S.alphas, S.apitches, p, S.levels, wrap,
-                             S.out, S.opitch, S.out_alpha, S.oapitch, dither, S.seed);
-            if (rc) return rc;
-        }
-        return ENBL_OK;
+    // Integer stacks take the batched pyramid passes, with the batched
+    // weights pass when it covers the weights and stack-by-stack weights
+    // otherwise.  A blend colour space (which converts every image around the
+    // pyramids), float images and wraparound run through enbl_enfuse one stack
+    // at a time.
+    if (ctx->colorspace == ENBL_COLORSPACE_IDENTITY && batchable(itype, p, wrap))
+        return run_batch(ctx, itype, nstacks, stacks, p, dither);
+    if (ctx->colorspace == ENBL_COLORSPACE_IDENTITY && batchable_pyramids(itype, wrap))
+        return run_batch(ctx, itype, nstacks, stacks, p, dither, true);
+    for (int s = 0; s < nstacks; ++s) {
+        const enbl_stack& S = stacks[s];
+        rc = enbl_enfuse(ctx, itype, S.w, S.h, S.K, S.imgs, S.ipitches, S.alphas, S.apitches, p, S.levels, wrap,
+                         S.out, S.opitch, S.out_alpha, S.oapitch, dither, S.seed);
+        if (rc) return rc;
     }
-    return run_batch(ctx, itype, nstacks, stacks, p, dither);
+    return ENBL_OK;
 }
 
 int enbl_batch_get_tuning(const enbl_context* ctx, enbl_batch_tuning* t)

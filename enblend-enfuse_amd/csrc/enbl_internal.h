@@ -186,6 +186,12 @@ struct MaskLevel0 {
 };
 int enfuse_mask_pyramids(enbl_context* ctx, int K, enbl_pyramid* const* mgps, const uint8_t* ua, long uap,
                          const MaskLevel0* l0 = nullptr);
+// enbl_enfuse's weights of one stack normalised into the fixed-point mask
+// level 0 of every image (mask0[m], pitch mpitch in elements) and its union
+// alpha (ua, pitch uap).
+int enfuse_stack_masks(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                       const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, uint8_t* ua,
+                       long uap, void* const* mask0, long mpitch);
 // Overlapping work: fork_aux makes the auxiliary stream wait for the work
 // issued so far and directs later launches to it; end_aux records its end
 // and directs launches back to the main stream; join_aux makes the main

@@ -453,6 +453,51 @@ int image_weights(enbl_context* ctx, int itype, int w, int h, int m, const void*
 
 }  // namespace
 
+namespace enbl {
+namespace host {
+
+// The first pass of enbl_enfuse for one stack (src/enfuse.h:1277-1388 and
+// the normalisation of :1529-1546): every image's weight, the norm and the
+// union alpha, then the fixed-point mask level 0 of every image.
+int enfuse_stack_masks(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                       const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, uint8_t* ua,
+                       long uap, void* const* mask0, long mpitch)
+{
+    const WeightArgs a = weight_args(p);
+    WeightScratch ws;
+    int rc = prepare_weights(ctx, itype, w, h, K, p, a, &ws);
+    if (rc) return rc;
+    const hipStream_t s = ctx->stream;
+    const long pitch = ws.pitch;
+    const size_t plane = (size_t)pitch * h;
+    void *norm, *masks;
+    rc = scratch(ctx, 5, plane * 4, &norm);
+    if (!rc) rc = scratch(ctx, 7, plane * 4 * (size_t)K, &masks);
+    if (!rc) rc = launch_fill(norm, plane * 4, s);
+    if (!rc) rc = launch_fill(ua, (size_t)uap * h, s);
+    if (!rc && p->w_contrast > 0.0) rc = contrast_all(ctx, itype, w, h, K, imgs, ips, alphas, aps, p, ws);
+    for (int m = 0; m < K && !rc; ++m)
+        rc = image_weights(ctx, itype, w, h, m, imgs[m], ips[m], alphas[m], aps[m], p, a, ws,
+                           (float*)masks + plane * m, pitch, (float*)norm, pitch, ua, uap);
+    if (!rc && p->hard_mask) {
+        Timed t(ctx, "hard_mask", (double)w * h * 8.0 * K);
+        rc = launch_hard_mask((float*)masks, pitch, plane, w, h, K, s);
+        if (!rc) rc = launch_status("hard_mask");
+    }
+    const int mt = mask_ptype_for(itype);
+    const int mfrac = itype == ENBL_U8 ? 7 : 15;
+    for (int m = 0; m < K && !rc; ++m) {
+        Timed t(ctx, "normalize", (double)w * h * (8.0 + ptype_size(mt)));
+        rc = launch_normalize(mt, (const float*)masks + plane * m, pitch, (const float*)norm, pitch, w, h,
+                              (float)(255 / K), (double)(1u << mfrac), p->hard_mask, mask0[m], mpitch, s);
+        if (!rc) rc = launch_status("normalize");
+    }
+    return rc;
+}
+
+}  // namespace host
+}  // namespace enbl
+
 extern "C" {
 
 int enbl_enfuse_mask(enbl_context* ctx, int itype, int w, int h, const void* img, long ip,

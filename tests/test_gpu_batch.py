@@ -127,22 +127,42 @@ def test_batch_graph_replay_reads_current_inputs(enbl, gpu_ctx):
             np.testing.assert_array_equal(d[2].cpu().numpy(), po, err_msg=f"replay {rep} stack {s}")
 
 
-@pytest.mark.parametrize("kw", [dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3,
-                                     contrast_window=5, entropy_window=3),
-                                dict(w_exposure=1.0, w_saturation=0.2, hard_mask=True)],
-                         ids=["contrast-entropy", "hard-mask"])
-def test_batch_fallback_params_match_plain(enbl, gpu_ctx, kw):
-    """Contrast/entropy weights and hard masks take the per-stack path inside
-    the batch call."""
-    rng = np.random.default_rng(78)
-    geoms = [(83, 61, 3, None, "holes"), (40, 33, 2, None, "full")]
-    host, dev = _make_batch(rng, U8, geoms, enbl)
-    plain = _plain(enbl, gpu_ctx, U8, dev, kw)
-    enbl.enfuse_batch(gpu_ctx, U8, dev, enbl.enfuse_params(**kw), dither=1)
-    gpu_ctx.sync()
-    for d, (po, pa) in zip(dev, plain):
-        np.testing.assert_array_equal(d[3].cpu().numpy(), pa)
-        np.testing.assert_array_equal(d[2].cpu().numpy(), po)
+STACK_WEIGHT_PARAMS = [
+    dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.5, w_entropy=0.3, contrast_window=5, entropy_window=3),
+    dict(w_exposure=1.0, w_saturation=0.2, hard_mask=True),
+    dict(w_exposure=0.8, w_contrast=0.4, contrast_window=3, gray_projector=2),   # + an exposure cutoff
+]
+
+
+@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("case", range(len(STACK_WEIGHT_PARAMS)), ids=["contrast-entropy", "hard-mask", "cutoff-projector"])
+def test_batch_stack_weights_match_plain_and_oracle(enbl, gpu_ctx, oracle, itype, case):
+    """Weights the batched weights pass does not cover (contrast, entropy,
+    hard masks, cutoffs, other projectors) are made stack by stack, then every
+    stack goes through the batched pyramid passes: bit-exact with one
+    enbl_enfuse per stack and with the oracle, twice in a row (the second
+    call replays the captured graph of the pyramid passes)."""
+    rng = np.random.default_rng(78 + 10 * itype + case)
+    kw = dict(STACK_WEIGHT_PARAMS[case])
+    if case == 2:
+        mx = 255.0 if itype == U8 else 65535.0
+        kw["exposure_cutoff"] = (float(int(0.05 * mx)), float(int(0.95 * mx)))
+    # (local windows need images at least as large as the window)
+    host, dev = _make_batch(rng, itype, [g for g in GEOMS if min(g[0], g[1]) >= 7], enbl)
+    for rep in range(2):
+        print(f"batch stack weights itype={itype} case={case} call {rep}", flush=True)
+        plain = _plain(enbl, gpu_ctx, itype, dev, kw)
+        enbl.enfuse_batch(gpu_ctx, itype, dev, enbl.enfuse_params(**kw), dither=1)
+        gpu_ctx.sync()
+        for s, ((imgs, alphas, levels, seed, fill), d, (po, pa)) in enumerate(zip(host, dev, plain)):
+            got, got_a = d[2].cpu().numpy(), d[3].cpu().numpy()
+            np.testing.assert_array_equal(got_a, pa, err_msg=f"stack {s} alpha")
+            np.testing.assert_array_equal(got, po, err_msg=f"stack {s}")
+            if rep == 0:
+                want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), levels, dither=1, seed=seed)
+                sel = want_a != 0
+                np.testing.assert_array_equal(got_a, want_a)
+                np.testing.assert_array_equal(got[sel], want[sel], err_msg=f"stack {s} vs oracle")
 
 
 def test_batch_autotune_keeps_results(enbl, gpu_ctx):
