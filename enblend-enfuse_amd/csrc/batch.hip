@@ -59,6 +59,12 @@ template <class I, class P> struct ImageLoad {
         return (S)(P)((P)*gmem(static_cast<const I*>(b.p) + (long)y * b.pitch + 3L * x + ch) << frac);
     }
 };
+// Float images: the log transform of src/fixmath.h:165-186 in double.
+template <> struct ImageLoad<float, double> {
+    __device__ static __forceinline__ double get(const BPlane& b, int y, int x, int ch, int) {
+        return float_to_log_pyramid((double)*gmem(static_cast<const float*>(b.p) + (long)y * b.pitch + 3L * x + ch));
+    }
+};
 
 // "average" grayscale projector (src/mga.h:188-193) with the integer
 // fromRealPromote of the image type: (int)((p0 + p1 + p2) / 3.0 + 0.5).  For
@@ -238,7 +244,12 @@ __global__ __launch_bounds__(256) void batch_weights_kernel(const BWeightJob* __
 // Reduce with alpha, no wraparound: the streaming strips of reduce_stream.h
 // (src/pyramid.h:166-573), one tile = 4 adjacent strips x a.rows rows.
 // ---------------------------------------------------------------------------
-template <class P, class I>
+// Level-0 source of an image pyramid: the integer image (v << frac) or the
+// float image (its log transform).
+template <class I, class P, bool WIDE> struct BatchImageSrc { typedef IntImageSrc<I, P, 3, WIDE> type; };
+template <bool WIDE> struct BatchImageSrc<float, double, WIDE> { typedef FloatImageSrc<3> type; };
+
+template <class P, class M, class I>
 __global__ __launch_bounds__(256) void batch_reduce_kernel(const BReduceJob* __restrict__ jobs, const int* __restrict__ tile0,
                                                            int njobs, int total)
 {
@@ -250,10 +261,10 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const BReduceJob* __r
         const ReduceArgs a = J.a;
         const int wx = (local % J.groups_x) * 4 + wv;
         const int i0 = (local / J.groups_x) * a.rows;
-        if (J.kind == kReduceImage3W) reduce_strip_ring<P, 3, true, IntImageSrc<I, P, 3, true>, true>(a, wx, i0);
-        else if (J.kind == kReduceImage3) reduce_strip_ring<P, 3, true, IntImageSrc<I, P, 3, false>, false>(a, wx, i0);
+        if (J.kind == kReduceImage3W) reduce_strip_ring<P, 3, true, typename BatchImageSrc<I, P, true>::type, true>(a, wx, i0);
+        else if (J.kind == kReduceImage3) reduce_strip_ring<P, 3, true, typename BatchImageSrc<I, P, false>::type, false>(a, wx, i0);
         else if (J.kind == kReducePlane3) reduce_strip_ring<P, 3, true, PlaneSrc<P, 3>, true>(a, wx, i0);
-        else reduce_strip_ring<P, 1, true, PlaneSrc<P, 1>, true>(a, wx, i0);
+        else reduce_strip_ring<M, 1, true, PlaneSrc<M, 1>, true>(a, wx, i0);
     }
 }
 
@@ -363,6 +374,37 @@ __device__ __forceinline__ I to_image(int32_t v, int frac, int dither, uint32_t 
     return (I)min(ImgMax<I>::i, max(0, r));
 }
 
+// The output sample of a level-0 result value: the dithered integer
+// conversion, or the inverse log transform of a float image (src/fixmath.h:
+// 320-329).
+template <class I, class P>
+__device__ __forceinline__ I out_value(P v, int frac, int dither, uint32_t h, int c)
+{
+    if constexpr (std::is_same<I, float>::value) return (float)log_pyramid_to_float((double)v);
+    else return to_image<I>(v, frac, dither, h, c);
+}
+
+// Fine rows (X, X+1) of level 0 of a float image's pyramid: the log
+// transform of the interleaved samples in double.
+struct FloatImageRows {
+    static constexpr bool planar = false;
+    template <int C, class Args>
+    __device__ static __forceinline__ void load(const Args& a, int r, int c, bool both, double (&x0)[C], double (&x1)[C]) {
+        const auto* g = gmem(static_cast<const float*>(a.src) + (long)r * a.spitch + (long)c * C);
+        float v[2 * C];
+#pragma unroll
+        for (int k = 0; k < 2 * C; ++k) v[k] = (k < C || both) ? g[k] : 0.f;
+#pragma unroll
+        for (int ch = 0; ch < C; ++ch) {
+            x0[ch] = float_to_log_pyramid((double)v[ch]);
+            x1[ch] = both ? float_to_log_pyramid((double)v[C + ch]) : 0.0;
+        }
+    }
+};
+
+template <class I, class P> struct BatchRows { typedef ImageRows<I, P> type; };
+template <> struct BatchRows<float, double> { typedef FloatImageRows type; };
+
 // Fine-level row source of a job entry, in the form the row loaders take.
 struct RowSrc {
     const void* src;
@@ -396,8 +438,8 @@ __device__ __forceinline__ void emit_pair(const BFuseJob& J, int X, int Y, bool 
     const uint32_t h1 = dither == ENBL_DITHER_DET ? dither_hash(J.seed, X + 1, Y) : 0u;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        r0[ch] = to_image<I>(v0[ch], frac, dither, h0, ch);
-        r1[ch] = to_image<I>(v1[ch], frac, dither, h1, ch);
+        r0[ch] = out_value<I, P>(v0[ch], frac, dither, h0, ch);
+        r1[ch] = out_value<I, P>(v1[ch], frac, dither, h1, ch);
     }
     if constexpr (sizeof(I) == 2) {
         if (xb && u0 && u1 && ((uintptr_t)op & 3) == 0) {
@@ -426,7 +468,7 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
-    typedef typename std::conditional<LEVEL0, ImageRows<I, P>, PlaneRows<P>>::type FINE;
+    typedef typename std::conditional<LEVEL0, typename BatchRows<I, P>::type, PlaneRows<P>>::type FINE;
     typedef typename std::conditional<LEVEL0, ImageLoad<I, P>, PlaneLoad<P>>::type FINE1;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -461,7 +503,7 @@ __global__ __launch_bounds__(256) void batch_fuse_kernel(const BFuseJob* __restr
                         auto* o = gmem(static_cast<I*>(J.out) + (long)y * J.opitch + 3L * j);
                         const uint32_t hh = dither == ENBL_DITHER_DET ? dither_hash(J.seed, j, y) : 0u;
 #pragma unroll
-                        for (int ch = 0; ch < 3; ++ch) o[ch] = to_image<I>(acc[ch], frac, dither, hh, ch);
+                        for (int ch = 0; ch < 3; ++ch) o[ch] = out_value<I, P>((P)acc[ch], frac, dither, hh, ch);
                     }
                 } else {
 #pragma unroll
@@ -565,21 +607,7 @@ template <class I, class P> struct L0Fine {
     __device__ static __forceinline__ void load(const void* img, long ip, int Y, int X, bool xb, int frac, S (&x0)[3],
                                                 S (&x1)[3]) {
         const RowSrc rs{img, ip, 0, frac};
-        ImageRows<I, P>::template load<3>(rs, Y, X, xb, x0, x1);
-    }
-};
-template <> struct L0Fine<float, double> {
-    __device__ static __forceinline__ void load(const void* img, long ip, int Y, int X, bool xb, int, double (&x0)[3],
-                                                double (&x1)[3]) {
-        const auto* g = gmem(static_cast<const float*>(img) + (long)Y * ip + 3L * X);
-        float v[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) v[k] = (k < 3 || xb) ? g[k] : 0.f;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            x0[ch] = float_to_log_pyramid((double)v[ch]);
-            x1[ch] = xb ? float_to_log_pyramid((double)v[3 + ch]) : 0.0;
-        }
+        BatchRows<I, P>::type::template load<3>(rs, Y, X, xb, x0, x1);
     }
 };
 
@@ -797,9 +825,11 @@ int launch_batch_reduce(int itype, const BReduceJob* jobs, const int* tile0, int
 {
     if (total <= 0) return 0;
     if (itype == ENBL_U8)
-        hipLaunchKernelGGL((batch_reduce_kernel<int16_t, uint8_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
+        hipLaunchKernelGGL((batch_reduce_kernel<int16_t, int16_t, uint8_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
     else if (itype == ENBL_U16)
-        hipLaunchKernelGGL((batch_reduce_kernel<int32_t, uint16_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
+        hipLaunchKernelGGL((batch_reduce_kernel<int32_t, int32_t, uint16_t>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
+    else if (itype == ENBL_F32)
+        hipLaunchKernelGGL((batch_reduce_kernel<double, int32_t, float>), dim3(grid), dim3(256), 0, s, jobs, tile0, njobs, total);
     else return ENBL_E_UNSUPPORTED;
     return 0;
 }
@@ -818,6 +848,7 @@ int launch_batch_fuse(int itype, int frac, int level0, const BFuseJob* jobs, con
     } while (0)
     if (itype == ENBL_U8) ENBL_BFUSE(int16_t, int16_t, uint8_t);
     else if (itype == ENBL_U16) ENBL_BFUSE(int32_t, int32_t, uint16_t);
+    else if (itype == ENBL_F32) ENBL_BFUSE(double, int32_t, float);
     else return ENBL_E_UNSUPPORTED;
 #undef ENBL_BFUSE
     return 0;

@@ -142,12 +142,12 @@ int batchable(int itype, const enbl_enfuse_params* p, int wrap)
            !p->exposure_cutoff && p->gray_projector == ENBL_GRAY_AVERAGE && !wrap;
 }
 
-// The batched pyramid passes: integer images without wraparound, any weights
-// (made stack by stack by the weight kernels of enbl_enfuse when the batched
-// weights pass does not cover them).
+// The batched pyramid passes: 8/16-bit and float images without wraparound,
+// any weights (made stack by stack by the weight kernels of enbl_enfuse when
+// the batched weights pass does not cover them).
 int batchable_pyramids(int itype, int wrap)
 {
-    return (itype == ENBL_U8 || itype == ENBL_U16) && !wrap;
+    return (itype == ENBL_U8 || itype == ENBL_U16 || itype == ENBL_F32) && !wrap;
 }
 
 int check_stacks(int itype, int n, const enbl_stack* st, const enbl_enfuse_params* p)
@@ -368,8 +368,7 @@ void build_plan(int itype, int n, const enbl_stack* st, const std::vector<StackL
             for (int m = 0; m < S.K; ++m) {
                 uint8_t* da = keep_alpha ? (uint8_t*)at(Y.ial_at(S.K, m) + DA.offset) : nullptr;
                 if (l == 0) {
-                    const size_t isz_b = itype == ENBL_U8 ? 1 : 2;
-                    const bool wide = isz_b == 2 && ((uintptr_t)S.imgs[m] & 3) == 0 && (S.ipitches[m] & 1) == 0 &&
+                    const bool wide = itype == ENBL_U16 && ((uintptr_t)S.imgs[m] & 3) == 0 && (S.ipitches[m] & 1) == 0 &&
                                       ((uintptr_t)S.alphas[m] & 1) == 0 && (S.apitches[m] & 1) == 0;
                     add_job(wide ? kReduceImage3W : kReduceImage3, S.imgs[m], S.ipitches[m], 0, sw, sh, S.alphas[m],
                             S.apitches[m], at(Y.img_at(m) + D.offset), D, da, DA.pitch);
@@ -538,7 +537,8 @@ int prepare(enbl_context* ctx, BatchState* b, int itype, int n, const enbl_stack
     layout_stacks(itype, n, st, lay, &arena_need);
     int rc = grow(ctx, b, &b->arena, &b->arena_bytes, std::max<size_t>(arena_need, 256), "batch arena");
     if (rc) return rc;
-    const std::vector<float>& lut = lut_for(b, itype, p);
+    static const std::vector<float> no_lut;
+    const std::vector<float>& lut = stack_weights ? no_lut : lut_for(b, itype, p);
     const int mfrac = itype == ENBL_U8 ? 7 : 15;
     P.w_exp = p->w_exposure;
     P.w_sat = p->w_saturation;
@@ -621,11 +621,11 @@ int enbl_enfuse_batch(enbl_context* ctx, int itype, int nstacks, const enbl_stac
     if (!ctx || !p) return fail(ENBL_E_ARG, "enbl_enfuse_batch: null argument");
     int rc = check_stacks(itype, nstacks, stacks, p);
     if (rc || nstacks == 0) return rc;
-    // Integer stacks take the batched pyramid passes, with the batched
-    // weights pass when it covers the weights and stack-by-stack weights
-    // otherwise.  A blend colour space (which converts every image around the
-    // pyramids), float images and wraparound run through enbl_enfuse one stack
-    // at a time.
+    // 8/16-bit and float stacks take the batched pyramid passes, with the
+    // batched weights pass when it covers the weights (integer images only)
+    // and stack-by-stack weights otherwise.  A blend colour space (which
+    // converts every image around the pyramids) and wraparound run through
+    // enbl_enfuse one stack at a time.
     if (ctx->colorspace == ENBL_COLORSPACE_IDENTITY && batchable(itype, p, wrap))
         return run_batch(ctx, itype, nstacks, stacks, p, dither);
     if (ctx->colorspace == ENBL_COLORSPACE_IDENTITY && batchable_pyramids(itype, wrap))

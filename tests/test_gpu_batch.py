@@ -3,7 +3,7 @@ per-stack path (enbl_enfuse) and the oracle, bit-exact."""
 import numpy as np
 import pytest
 
-from oracle_binding import U8, U16, enfuse_params as orc_params
+from oracle_binding import F32, U8, U16, enfuse_params as orc_params
 
 pytestmark = pytest.mark.gpu
 
@@ -14,10 +14,13 @@ def _stack(rng, itype, K, h, w):
     yy, xx = np.mgrid[0:h, 0:w]
     base = 0.5 + 0.4 * np.sin(xx / 7.0 + rng.uniform(0, 3)) * np.cos(yy / 5.0)
     tint = rng.uniform(0.7, 1.3, size=3)
-    m = 255 if itype == U8 else 65535
     imgs = []
     for k in range(K):
         rad = base[..., None] * tint[None, None, :] * 2.0 ** ((k - (K - 1) / 2) * 1.0)
+        if itype == F32:   # linear radiance, values above 1 included
+            imgs.append((rad * (1.0 + 0.01 * rng.normal(0, 1, rad.shape))).astype(np.float32))
+            continue
+        m = 255 if itype == U8 else 65535
         v = np.clip(rad, 0, 1) ** (1 / 2.2) * m + rng.normal(0, 1, rad.shape)
         imgs.append(np.clip(np.rint(v), 0, m).astype(np.uint8 if itype == U8 else np.uint16))
     return imgs
@@ -56,7 +59,8 @@ def _make_batch(rng, itype, geoms, enbl):
         levels = L if L is not None else enbl.auto_levels(w, h)
         seed = int(rng.integers(0, 1 << 40))
         fill = rng.integers(0, 255)
-        out = torch.full((h, w, 3), int(fill), dtype=torch.uint8 if itype == U8 else torch.uint16, device="cuda")
+        odt = {U8: torch.uint8, U16: torch.uint16, F32: torch.float32}[itype]
+        out = torch.full((h, w, 3), int(fill), dtype=odt, device="cuda")
         oa = torch.full((h, w), 7, dtype=torch.uint8, device="cuda")
         host.append((imgs, alphas, levels, seed, int(fill)))
         dev.append(([torch.from_numpy(i).cuda() for i in imgs], [torch.from_numpy(a).cuda() for a in alphas],
@@ -134,19 +138,22 @@ STACK_WEIGHT_PARAMS = [
 ]
 
 
-@pytest.mark.parametrize("itype", [U8, U16])
+@pytest.mark.parametrize("itype", [U8, U16, F32])
 @pytest.mark.parametrize("case", range(len(STACK_WEIGHT_PARAMS)), ids=["contrast-entropy", "hard-mask", "cutoff-projector"])
 def test_batch_stack_weights_match_plain_and_oracle(enbl, gpu_ctx, oracle, itype, case):
     """Weights the batched weights pass does not cover (contrast, entropy,
     hard masks, cutoffs, other projectors) are made stack by stack, then every
     stack goes through the batched pyramid passes: bit-exact with one
     enbl_enfuse per stack and with the oracle, twice in a row (the second
-    call replays the captured graph of the pyramid passes)."""
+    call replays the captured graph of the pyramid passes).  Float stacks:
+    bit-exact with enbl_enfuse, within 2e-6 relative of the oracle (whose
+    log/exp transform is the C library's) with the alpha exact."""
     rng = np.random.default_rng(78 + 10 * itype + case)
     kw = dict(STACK_WEIGHT_PARAMS[case])
     if case == 2:
-        mx = 255.0 if itype == U8 else 65535.0
-        kw["exposure_cutoff"] = (float(int(0.05 * mx)), float(int(0.95 * mx)))
+        mx = {U8: 255.0, U16: 65535.0, F32: 1.0}[itype]
+        lo, hi = 0.05 * mx, 0.95 * mx
+        kw["exposure_cutoff"] = (lo, hi) if itype == F32 else (float(int(lo)), float(int(hi)))
     # (local windows need images at least as large as the window)
     host, dev = _make_batch(rng, itype, [g for g in GEOMS if min(g[0], g[1]) >= 7], enbl)
     for rep in range(2):
@@ -162,7 +169,10 @@ def test_batch_stack_weights_match_plain_and_oracle(enbl, gpu_ctx, oracle, itype
                 want, want_a = oracle.enfuse(imgs, alphas, itype, orc_params(**kw), levels, dither=1, seed=seed)
                 sel = want_a != 0
                 np.testing.assert_array_equal(got_a, want_a)
-                np.testing.assert_array_equal(got[sel], want[sel], err_msg=f"stack {s} vs oracle")
+                if itype == F32:
+                    np.testing.assert_allclose(got[sel], want[sel], rtol=2e-6, atol=0, err_msg=f"stack {s} vs oracle")
+                else:
+                    np.testing.assert_array_equal(got[sel], want[sel], err_msg=f"stack {s} vs oracle")
 
 
 def test_batch_autotune_keeps_results(enbl, gpu_ctx):
