@@ -516,6 +516,10 @@ __global__ __launch_bounds__(kTile) void entropy_kernel(const T* __restrict__ im
 //     reference's std::map iterates them.  So the sorting work of a block is
 //     spread over all its lanes instead of being paid by a whole wave for
 //     any one lane that needs it.
+// A tile whose halo lies wholly under alpha and inside the image and has a
+// channel with one value throughout (every sample equal to the tile's first
+// one) has a one-bin window in that channel at every pixel, so its entropies
+// are all 0: such a tile skips the column extremes, classes and queues.
 // Windows that are cut by alpha take an insertion sort of the samples under
 // alpha and the general table, in place.  Equal keys are those that compare
 // neither less nor greater (the map's equivalence), so -0 and +0 share a bin.
@@ -630,6 +634,7 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
     __shared__ double spent[NN + 1], splog[NN + 1];
     __shared__ unsigned short queue[QMAX], cqueue[QMAX];
     __shared__ int qn, cqn;
+    __shared__ int wbits[NW];
     auto& cmx = u.cmx;
     auto& ent = u.ent;
     const int x0 = blockIdx.x * kEntTW, y0 = blockIdx.y * kEntTH;
@@ -663,14 +668,26 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
             la[k][j] = rin && lx < LW && gx >= 0 && gx < w ? arow[lx] : (uint8_t)0;
         }
     }
-    int unset = 0;
+    // the tile's first halo sample per channel (NaN-free keys compare equal
+    // to themselves; a NaN reference leaves the channel unflagged)
+    K ref[3];
+    const bool corner = x0 - B >= 0 && y0 - B >= 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ref[c] = corner ? (K)img[(long)(y0 - B) * ip + 3L * (x0 - B) + c] : (K)0;
+    int unset = corner ? 0 : 1;
+    int diff = 0;   // bit c: a sample of channel c differs from ref[c]
 #pragma unroll
     for (int k = 0; k < WROWS; ++k) {
         const int ly = wv + NW * k;
         if (ly >= LH) continue;
 #pragma unroll
         for (int j = 0; j < VCH; ++j)
-            if (lane + 64 * j < 3 * LW) (&sv[0][0][0])[voff[j] + ly * LW] = lv[k][j];
+            if (lane + 64 * j < 3 * LW) {
+                (&sv[0][0][0])[voff[j] + ly * LW] = lv[k][j];
+                const int c = voff[j] / (LH * LW);
+                const K rv = c == 0 ? ref[0] : (c == 1 ? ref[1] : ref[2]);
+                diff |= lv[k][j] == rv ? 0 : (1 << c);
+            }
 #pragma unroll
         for (int j = 0; j < ACH; ++j) {
             const int lx = lane + 64 * j;
@@ -686,177 +703,198 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
     }
     if (tid == 0) { qn = 0; cqn = 0; }
     const double sdist = tab[2 * (NN + 1) + (NN + 1) * (NN + 1)];
-    const bool tile_full = __syncthreads_or(unset) == 0;
-    // Extremes of every window column (N rows from output row r): one thread
-    // per tile column and channel reads the column once.
-    for (int i = tid; i < 3 * LW; i += NT) {
-        const int c = i / LW, cx = i - c * LW;
-        K col[LH];
+    {
+        const int bits = (unset ? 8 : 0) | diff;
+        int wb = 0;
 #pragma unroll
-        for (int r = 0; r < LH; ++r) col[r] = sv[c][r][cx];
-#pragma unroll
-        for (int r = 0; r < kEntTH; ++r) {
-            K mn = col[r], mx = col[r];
-#pragma unroll
-            for (int dy = 1; dy < N; ++dy) {
-                kmin(mn, col[r + dy]);
-                kmax(mx, col[r + dy]);
-            }
-            cmn[c][r][cx] = mn;
-            cmx[c][r][cx] = mx;
-        }
+        for (int q = 0; q < 4; ++q) wb |= __ballot((bits >> q) & 1) ? (1 << q) : 0;
+        if (lane == 0) wbits[wv] = wb;
     }
     __syncthreads();
-    // Window classes from the column extremes, sliding along each row in
-    // runs of kEntSeg outputs: bit 0 = one value (flat), bit 1 = every
-    // column constant.
-    constexpr int kEntSeg = 8, NSEG = kEntTW / kEntSeg;
-    for (int i = tid; i < 3 * kEntTH * NSEG; i += NT) {
-        const int c = i / (kEntTH * NSEG), e = i - c * (kEntTH * NSEG);
-        const int r = e / NSEG, x0s = (e - r * NSEG) * kEntSeg;
-        K rmn[kEntSeg + N - 1], rmx[kEntSeg + N - 1];
+    int tbits = 0;
 #pragma unroll
-        for (int j = 0; j < kEntSeg + N - 1; ++j) {
-            rmn[j] = cmn[c][r][x0s + j];
-            rmx[j] = cmx[c][r][x0s + j];
-        }
-#pragma unroll
-        for (int j = 0; j < kEntSeg; ++j) {
-            K mn = rmn[j], mx = rmx[j];
-            bool cols = !(rmn[j] < rmx[j]);
-#pragma unroll
-            for (int dx = 1; dx < N; ++dx) {
-                kmin(mn, rmn[j + dx]);
-                kmax(mx, rmx[j + dx]);
-                cols = cols && !(rmn[j + dx] < rmx[j + dx]);
-            }
-            cls[c][r][x0s + j] = (uint8_t)((mn < mx ? 0 : 1) | (cols ? 2 : 0));
-        }
-    }
-    __syncthreads();
-    // Per pixel: cut windows in place; a flat channel makes the minimum 0;
-    // every other channel goes to the queue of constant-column windows or
-    // to the sorting queue.
+    for (int q = 0; q < NW; ++q) tbits |= wbits[q];
+    const bool tile_full = (tbits & 8) == 0;
+    // a channel with one value over the whole tile: every entropy is 0
+    const bool flat_tile = tile_full && (tbits & 7) != 7;
     const int lx = tid % kEntTW;
     T res[RPT];
     int pend[RPT];                 // bit c: channel c's value comes from a queue
     bool store[RPT];
 #pragma unroll
     for (int p = 0; p < RPT; ++p) {
-        const int ly = tid / kEntTW + p * RSTEP;
-        const int x = x0 + lx, y = y0 + ly;
         res[p] = T(0);
         pend[p] = 0;
-        store[p] = x < w && y < h;
-        if (!(store[p] && x >= B && x < w - B && y >= B && y < h - B && sa[ly + B][lx + B])) continue;
-        bool full = tile_full;
-        if (!full) {
-            full = true;
-#pragma unroll
-            for (int dy = 0; dy < N; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < N; ++dx) full = full && sa[ly + dy][lx + dx] != 0;
-        }
-        if (!full) {
-            // Cut window: insertion sort of the samples under alpha; the
-            // general p*log(p) table for `cnt` samples.
-            const double* pgen = tab + 2 * (NN + 1);
-            T emin = T(0);
-            for (int c = 0; c < 3; ++c) {
-                K v[NN];
-                int cnt = 0;
-                for (int dy = 0; dy < N; ++dy)
-                    for (int dx = 0; dx < N; ++dx)
-                        if (sa[ly + dy][lx + dx]) {
-                            const K sm = sv[c][ly + dy][lx + dx];
-                            int k = cnt++;
-                            while (k > 0 && sm < v[k - 1]) { v[k] = v[k - 1]; --k; }
-                            v[k] = sm;
-                        }
-                double acc = 0.0;
-                int bins = 0, run = 1;
-                for (int k = 1; k <= cnt; ++k) {
-                    if (k < cnt && !(v[k - 1] < v[k])) { ++run; continue; }
-                    acc += pgen[cnt * (NN + 1) + run];
-                    ++bins;
-                    run = 1;
+        store[p] = x0 + lx < w && y0 + tid / kEntTW + p * RSTEP < h;
+    }
+    if (!flat_tile) {
+        // Extremes of every window column (N rows from output row r): one thread
+        // per tile column and channel reads the column once.
+        for (int i = tid; i < 3 * LW; i += NT) {
+            const int c = i / LW, cx = i - c * LW;
+            K col[LH];
+    #pragma unroll
+            for (int r = 0; r < LH; ++r) col[r] = sv[c][r][cx];
+    #pragma unroll
+            for (int r = 0; r < kEntTH; ++r) {
+                K mn = col[r], mx = col[r];
+    #pragma unroll
+                for (int dy = 1; dy < N; ++dy) {
+                    kmin(mn, col[r + dy]);
+                    kmax(mx, col[r + dy]);
                 }
-                const double e = bins > 1 ? -acc / splog[bins] : 0.0;
-                const T ec = from_real_img<T>(e * Img<T>::maxv());
-                emin = c == 0 ? ec : (ec < emin ? ec : emin);
-                // every channel's value is +0 or more: once the minimum is 0
-                // the remaining channels cannot change it
-                if (emin == T(0)) break;
+                cmn[c][r][cx] = mn;
+                cmx[c][r][cx] = mx;
             }
-            res[p] = emin;
-            continue;
         }
-        const int k0 = cls[0][ly][lx], k1 = cls[1][ly][lx], k2 = cls[2][ly][lx];
-        if ((k0 | k1 | k2) & 1) continue;     // a one-bin channel: the minimum is 0
-        pend[p] = 7 | (((k0 >> 1) & 1) << 3) | (((k1 >> 1) & 1) << 4) | (((k2 >> 1) & 1) << 5);
-    }
-#pragma unroll
-    for (int p = 0; p < RPT; ++p) {
-        const int pix = (tid / kEntTW + p * RSTEP) * kEntTW + lx;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const bool queued = (pend[p] >> c) & 1;
-            const bool colc = (pend[p] >> (3 + c)) & 1;
-            const int cs = wave_append(&cqn, queued && colc);
-            const int ss = wave_append(&qn, queued && !colc);
-            if (cs >= 0) cqueue[cs] = (unsigned short)((pix << 2) | c);
-            if (ss >= 0) queue[ss] = (unsigned short)((pix << 2) | c);
+        __syncthreads();
+        // Window classes from the column extremes, sliding along each row in
+        // runs of kEntSeg outputs: bit 0 = one value (flat), bit 1 = every
+        // column constant.
+        constexpr int kEntSeg = 8, NSEG = kEntTW / kEntSeg;
+        for (int i = tid; i < 3 * kEntTH * NSEG; i += NT) {
+            const int c = i / (kEntTH * NSEG), e = i - c * (kEntTH * NSEG);
+            const int r = e / NSEG, x0s = (e - r * NSEG) * kEntSeg;
+            K rmn[kEntSeg + N - 1], rmx[kEntSeg + N - 1];
+    #pragma unroll
+            for (int j = 0; j < kEntSeg + N - 1; ++j) {
+                rmn[j] = cmn[c][r][x0s + j];
+                rmx[j] = cmx[c][r][x0s + j];
+            }
+    #pragma unroll
+            for (int j = 0; j < kEntSeg; ++j) {
+                K mn = rmn[j], mx = rmx[j];
+                bool cols = !(rmn[j] < rmx[j]);
+    #pragma unroll
+                for (int dx = 1; dx < N; ++dx) {
+                    kmin(mn, rmn[j + dx]);
+                    kmax(mx, rmx[j + dx]);
+                    cols = cols && !(rmn[j + dx] < rmx[j + dx]);
+                }
+                cls[c][r][x0s + j] = (uint8_t)((mn < mx ? 0 : 1) | (cols ? 2 : 0));
+            }
         }
-    }
-    __syncthreads();
-    // Constant-column windows: one bin per distinct column value, N samples
-    // per column; the N column keys are sorted and walked.
-    const int ncq = cqn;
-    for (int i = tid; i < ncq; i += NT) {
-        const int item = cqueue[i];
-        const int c = item & 3, pix = item >> 2;
-        const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
-        K k[N];
-#pragma unroll
-        for (int dx = 0; dx < N; ++dx) k[dx] = cmn[c][qy][qx + dx];
-        constexpr CmpNet net = SortNet<N>::net;
-#pragma unroll
-        for (int q = 0; q < net.n; ++q) cas(k[net.a[q]], k[net.b[q]]);
-        int bins;
-        const double acc = walk_bins(k, N, spent, bins);
-        ent[c][qy][qx] = from_real_img<T>(-acc / splog[bins] * Img<T>::maxv());
-    }
-    // Other windows: all NN keys sorted.
-    const int nq = qn;
-    for (int i = tid; i < nq; i += NT) {
-        const int item = queue[i];
-        const int c = item & 3, pix = item >> 2;
-        const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
-        K v[NN];
-#pragma unroll
-        for (int dy = 0; dy < N; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][qy + dy][qx + dx];
-        constexpr CmpNet net = SortNet<NN>::net;
-#pragma unroll
-        for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
-        bool distinct = true;
-#pragma unroll
-        for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
-        double acc;
-        int bins;
-        if (distinct) {
-            // NN bins of one sample: the table's one-sample term added NN
-            // times in order, precomputed
-            bins = NN;
-            acc = sdist;
-        } else {
-            acc = walk_bins(v, 1, spent, bins);
+        __syncthreads();
+        // Per pixel: cut windows in place; a flat channel makes the minimum 0;
+        // every other channel goes to the queue of constant-column windows or
+        // to the sorting queue.
+    #pragma unroll
+        for (int p = 0; p < RPT; ++p) {
+            const int ly = tid / kEntTW + p * RSTEP;
+            const int x = x0 + lx, y = y0 + ly;
+            res[p] = T(0);
+            pend[p] = 0;
+            store[p] = x < w && y < h;
+            if (!(store[p] && x >= B && x < w - B && y >= B && y < h - B && sa[ly + B][lx + B])) continue;
+            bool full = tile_full;
+            if (!full) {
+                full = true;
+    #pragma unroll
+                for (int dy = 0; dy < N; ++dy)
+    #pragma unroll
+                    for (int dx = 0; dx < N; ++dx) full = full && sa[ly + dy][lx + dx] != 0;
+            }
+            if (!full) {
+                // Cut window: insertion sort of the samples under alpha; the
+                // general p*log(p) table for `cnt` samples.
+                const double* pgen = tab + 2 * (NN + 1);
+                T emin = T(0);
+                for (int c = 0; c < 3; ++c) {
+                    K v[NN];
+                    int cnt = 0;
+                    for (int dy = 0; dy < N; ++dy)
+                        for (int dx = 0; dx < N; ++dx)
+                            if (sa[ly + dy][lx + dx]) {
+                                const K sm = sv[c][ly + dy][lx + dx];
+                                int k = cnt++;
+                                while (k > 0 && sm < v[k - 1]) { v[k] = v[k - 1]; --k; }
+                                v[k] = sm;
+                            }
+                    double acc = 0.0;
+                    int bins = 0, run = 1;
+                    for (int k = 1; k <= cnt; ++k) {
+                        if (k < cnt && !(v[k - 1] < v[k])) { ++run; continue; }
+                        acc += pgen[cnt * (NN + 1) + run];
+                        ++bins;
+                        run = 1;
+                    }
+                    const double e = bins > 1 ? -acc / splog[bins] : 0.0;
+                    const T ec = from_real_img<T>(e * Img<T>::maxv());
+                    emin = c == 0 ? ec : (ec < emin ? ec : emin);
+                    // every channel's value is +0 or more: once the minimum is 0
+                    // the remaining channels cannot change it
+                    if (emin == T(0)) break;
+                }
+                res[p] = emin;
+                continue;
+            }
+            const int k0 = cls[0][ly][lx], k1 = cls[1][ly][lx], k2 = cls[2][ly][lx];
+            if ((k0 | k1 | k2) & 1) continue;     // a one-bin channel: the minimum is 0
+            pend[p] = 7 | (((k0 >> 1) & 1) << 3) | (((k1 >> 1) & 1) << 4) | (((k2 >> 1) & 1) << 5);
         }
-        const double e = bins > 1 ? -acc / splog[bins] : 0.0;
-        ent[c][qy][qx] = from_real_img<T>(e * Img<T>::maxv());
+    #pragma unroll
+        for (int p = 0; p < RPT; ++p) {
+            const int pix = (tid / kEntTW + p * RSTEP) * kEntTW + lx;
+    #pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bool queued = (pend[p] >> c) & 1;
+                const bool colc = (pend[p] >> (3 + c)) & 1;
+                const int cs = wave_append(&cqn, queued && colc);
+                const int ss = wave_append(&qn, queued && !colc);
+                if (cs >= 0) cqueue[cs] = (unsigned short)((pix << 2) | c);
+                if (ss >= 0) queue[ss] = (unsigned short)((pix << 2) | c);
+            }
+        }
+        __syncthreads();
+        // Constant-column windows: one bin per distinct column value, N samples
+        // per column; the N column keys are sorted and walked.
+        const int ncq = cqn;
+        for (int i = tid; i < ncq; i += NT) {
+            const int item = cqueue[i];
+            const int c = item & 3, pix = item >> 2;
+            const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
+            K k[N];
+    #pragma unroll
+            for (int dx = 0; dx < N; ++dx) k[dx] = cmn[c][qy][qx + dx];
+            constexpr CmpNet net = SortNet<N>::net;
+    #pragma unroll
+            for (int q = 0; q < net.n; ++q) cas(k[net.a[q]], k[net.b[q]]);
+            int bins;
+            const double acc = walk_bins(k, N, spent, bins);
+            ent[c][qy][qx] = from_real_img<T>(-acc / splog[bins] * Img<T>::maxv());
+        }
+        // Other windows: all NN keys sorted.
+        const int nq = qn;
+        for (int i = tid; i < nq; i += NT) {
+            const int item = queue[i];
+            const int c = item & 3, pix = item >> 2;
+            const int qy = pix / kEntTW, qx = pix - qy * kEntTW;
+            K v[NN];
+    #pragma unroll
+            for (int dy = 0; dy < N; ++dy)
+    #pragma unroll
+                for (int dx = 0; dx < N; ++dx) v[dy * N + dx] = sv[c][qy + dy][qx + dx];
+            constexpr CmpNet net = SortNet<NN>::net;
+    #pragma unroll
+            for (int q = 0; q < net.n; ++q) cas(v[net.a[q]], v[net.b[q]]);
+            bool distinct = true;
+    #pragma unroll
+            for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
+            double acc;
+            int bins;
+            if (distinct) {
+                // NN bins of one sample: the table's one-sample term added NN
+                // times in order, precomputed
+                bins = NN;
+                acc = sdist;
+            } else {
+                acc = walk_bins(v, 1, spent, bins);
+            }
+            const double e = bins > 1 ? -acc / splog[bins] : 0.0;
+            ent[c][qy][qx] = from_real_img<T>(e * Img<T>::maxv());
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // Channel minimum (entropies are +0 or more and never NaN, so the order
     // of the comparisons does not matter).
 #pragma unroll

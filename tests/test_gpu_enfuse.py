@@ -238,6 +238,46 @@ def test_entropy_constant_column_windows_bit_exact(enbl, gpu_ctx, oracle, itype,
     np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("ewin", [3, 7])
+@pytest.mark.parametrize("fused", [False, True])
+def test_entropy_flat_tiles_bit_exact(enbl, gpu_ctx, oracle, itype, ewin, fused):
+    """Tiles whose whole window halo has one value in a channel (entropy 0 at
+    every pixel without sorting): all channels flat, one channel flat beside
+    random ones, -0 / +0 mixed in a flat float region, a flat region broken
+    by a single sample or by an alpha hole, and flat regions that end inside
+    a tile; the entropy weight alone (fused: with exposure and saturation in
+    the same pass), bit for bit against the oracle."""
+    rng = np.random.default_rng(1500 + itype + ewin + (7 if fused else 0))
+    h, w = 72, 520
+    m = {U8: 255, U16: 65535, F32: 1.0}[itype]
+    img = rng.random((h, w, 3)) * m
+    img[:, 30:200] = 0.4 * m                           # all channels flat over whole tiles
+    img[:, 230:330, 1] = 0.7 * m                       # one channel flat
+    img[20:60, 350:470] = 0.2 * m
+    img[37, 410] = 0.3 * m                             # one sample breaks a flat tile
+    img[5:13, 100] = 0.6 * m                           # a short flat-breaking column
+    a = np.full((h, w), 255, np.uint8)
+    a[50, 150] = 0                                     # an alpha hole inside a flat region
+    if itype == F32:
+        img = img.astype(np.float32)
+        img[:, 60:130, 2] = 0.0
+        img[::2, 60:130, 2] = -0.0                     # -0 and +0 share a bin
+    else:
+        img = np.rint(img).astype(np.uint8 if itype == U8 else np.uint16)
+    kw = dict(w_exposure=0.5 if fused else 0.0, w_saturation=0.3 if fused else 0.0, w_entropy=1.0,
+              entropy_window=ewin)
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(np.ascontiguousarray(img)).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"entropy flat tiles itype={itype} n={ewin} fused={fused}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
 def _table_call(enbl, ctx, itype, table, entry):
     """Calls one enfuse entry point on a tiny stack with an exposure table;
     returns the status code."""
