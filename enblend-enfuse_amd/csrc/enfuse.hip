@@ -865,6 +865,7 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
         }
         // Other windows: all NN keys sorted.
         const int nq = qn;
+        const T ent_dist = from_real_img<T>(-sdist / splog[NN] * Img<T>::maxv());
         for (int i = tid; i < nq; i += NT) {
             const int item = queue[i];
             const int c = item & 3, pix = item >> 2;
@@ -880,18 +881,17 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
             bool distinct = true;
     #pragma unroll
             for (int k = 1; k < NN; ++k) distinct = distinct && v[k - 1] < v[k];
-            double acc;
-            int bins;
-            if (distinct) {
-                // NN bins of one sample: the table's one-sample term added NN
-                // times in order, precomputed
-                bins = NN;
-                acc = sdist;
-            } else {
-                acc = walk_bins(v, 1, spent, bins);
+            // NN bins of one sample: the table's one-sample term added NN
+            // times in order, precomputed with its entropy; the walk over the
+            // bins runs only in waves that hold a window with tied samples
+            T ev = ent_dist;
+            if (__ballot(!distinct) != 0ull) {
+                int bins;
+                const double acc = walk_bins(v, 1, spent, bins);
+                const double e = bins > 1 ? -acc / splog[bins] : 0.0;
+                if (!distinct) ev = from_real_img<T>(e * Img<T>::maxv());
             }
-            const double e = bins > 1 ? -acc / splog[bins] : 0.0;
-            ent[c][qy][qx] = from_real_img<T>(e * Img<T>::maxv());
+            ent[c][qy][qx] = ev;
         }
         __syncthreads();
     }
