@@ -630,6 +630,7 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
         T ent[3][kEntTH][kEntTW];
     } u;
     __shared__ uint8_t cls[3][kEntTH][kEntTW];
+    __shared__ uint8_t cmo[3][kEntTH][LW];   // window column strictly monotone
     __shared__ uint8_t sa[LH][LW + 4];
     __shared__ double spent[NN + 1], splog[NN + 1];
     __shared__ unsigned short queue[QMAX], cqueue[QMAX];
@@ -735,43 +736,67 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
             K col[LH];
     #pragma unroll
             for (int r = 0; r < LH; ++r) col[r] = sv[c][r][cx];
+            bool lt[LH - 1], gt[LH - 1];
+    #pragma unroll
+            for (int r = 0; r + 1 < LH; ++r) {
+                lt[r] = col[r] < col[r + 1];
+                gt[r] = col[r + 1] < col[r];
+            }
     #pragma unroll
             for (int r = 0; r < kEntTH; ++r) {
                 K mn = col[r], mx = col[r];
+                bool inc = true, dec = true;
     #pragma unroll
                 for (int dy = 1; dy < N; ++dy) {
                     kmin(mn, col[r + dy]);
                     kmax(mx, col[r + dy]);
+                    inc = inc && lt[r + dy - 1];
+                    dec = dec && gt[r + dy - 1];
                 }
                 cmn[c][r][cx] = mn;
                 cmx[c][r][cx] = mx;
+                cmo[c][r][cx] = (uint8_t)(inc || dec);
             }
         }
         __syncthreads();
         // Window classes from the column extremes, sliding along each row in
         // runs of kEntSeg outputs: bit 0 = one value (flat), bit 1 = every
-        // column constant.
+        // column constant, bit 2 = every column strictly monotone and the
+        // column ranges strictly ordered along the row (so the N*N samples
+        // are distinct: N*N bins of one sample, no sort needed).
         constexpr int kEntSeg = 8, NSEG = kEntTW / kEntSeg;
         for (int i = tid; i < 3 * kEntTH * NSEG; i += NT) {
             const int c = i / (kEntTH * NSEG), e = i - c * (kEntTH * NSEG);
             const int r = e / NSEG, x0s = (e - r * NSEG) * kEntSeg;
             K rmn[kEntSeg + N - 1], rmx[kEntSeg + N - 1];
+            bool rmo[kEntSeg + N - 1], up[kEntSeg + N - 2], dn[kEntSeg + N - 2];
     #pragma unroll
             for (int j = 0; j < kEntSeg + N - 1; ++j) {
                 rmn[j] = cmn[c][r][x0s + j];
                 rmx[j] = cmx[c][r][x0s + j];
+                rmo[j] = cmo[c][r][x0s + j] != 0;
+            }
+    #pragma unroll
+            for (int j = 0; j + 1 < kEntSeg + N - 1; ++j) {
+                up[j] = rmx[j] < rmn[j + 1];
+                dn[j] = rmx[j + 1] < rmn[j];
             }
     #pragma unroll
             for (int j = 0; j < kEntSeg; ++j) {
                 K mn = rmn[j], mx = rmx[j];
                 bool cols = !(rmn[j] < rmx[j]);
+                bool mono = rmo[j], ou = true, od = true;
     #pragma unroll
                 for (int dx = 1; dx < N; ++dx) {
                     kmin(mn, rmn[j + dx]);
                     kmax(mx, rmx[j + dx]);
                     cols = cols && !(rmn[j + dx] < rmx[j + dx]);
+                    mono = mono && rmo[j + dx];
+                    ou = ou && up[j + dx - 1];
+                    od = od && dn[j + dx - 1];
                 }
-                cls[c][r][x0s + j] = (uint8_t)((mn < mx ? 0 : 1) | (cols ? 2 : 0));
+                const bool ordered = mono && (ou || od);
+                cls[c][r][x0s + j] = (uint8_t)((mn < mx ? 0 : 1) | (cols ? 2 : 0) | (ordered ? 4 : 0));
             }
         }
         __syncthreads();
@@ -830,15 +855,21 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
             }
             const int k0 = cls[0][ly][lx], k1 = cls[1][ly][lx], k2 = cls[2][ly][lx];
             if ((k0 | k1 | k2) & 1) continue;     // a one-bin channel: the minimum is 0
-            pend[p] = 7 | (((k0 >> 1) & 1) << 3) | (((k1 >> 1) & 1) << 4) | (((k2 >> 1) & 1) << 5);
+            pend[p] = 7 | (((k0 >> 1) & 1) << 3) | (((k1 >> 1) & 1) << 4) | (((k2 >> 1) & 1) << 5) |
+                      (((k0 >> 2) & 1) << 6) | (((k1 >> 2) & 1) << 7) | (((k2 >> 2) & 1) << 8);
         }
+        // the entropy of N*N distinct samples (the sorting queue's distinct
+        // windows take it too)
+        const T ent_dist = from_real_img<T>(-sdist / splog[NN] * Img<T>::maxv());
     #pragma unroll
         for (int p = 0; p < RPT; ++p) {
             const int pix = (tid / kEntTW + p * RSTEP) * kEntTW + lx;
     #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const bool queued = (pend[p] >> c) & 1;
+                const bool ordered = (pend[p] >> (6 + c)) & 1;
+                const bool queued = ((pend[p] >> c) & 1) && !ordered;
                 const bool colc = (pend[p] >> (3 + c)) & 1;
+                if (ordered) ent[c][pix / kEntTW][lx] = ent_dist;
                 const int cs = wave_append(&cqn, queued && colc);
                 const int ss = wave_append(&qn, queued && !colc);
                 if (cs >= 0) cqueue[cs] = (unsigned short)((pix << 2) | c);
@@ -865,7 +896,6 @@ __global__ __launch_bounds__(kEntThreads) void entropy_tile_kernel(const T* __re
         }
         // Other windows: all NN keys sorted.
         const int nq = qn;
-        const T ent_dist = from_real_img<T>(-sdist / splog[NN] * Img<T>::maxv());
         for (int i = tid; i < nq; i += NT) {
             const int item = queue[i];
             const int c = item & 3, pix = item >> 2;

@@ -278,6 +278,44 @@ def test_entropy_flat_tiles_bit_exact(enbl, gpu_ctx, oracle, itype, ewin, fused)
     np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("itype", [U8, U16, F32])
+@pytest.mark.parametrize("ewin", [3, 5, 7])
+def test_entropy_ordered_windows_bit_exact(enbl, gpu_ctx, oracle, itype, ewin):
+    """Windows of linear ramps: columns strictly monotone with their ranges
+    strictly ordered along the row (all samples distinct, no sort), ranges
+    that touch (a sample shared by neighbouring columns: tied bins), ramps
+    falling to the left or upwards, ramps along y (rows ordered, sorted),
+    and ramps broken by one sample or by a plateau; the entropy weight alone,
+    bit for bit against the oracle."""
+    h, w = 48, 448
+    n = ewin
+    m = {U8: 255, U16: 65535, F32: 1.0}[itype]
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    step = 1.0 if itype != F32 else 1.0 / 4096
+    img = np.empty((h, w, 3), np.float64)
+    lo = 0.0 if itype == U8 else 100.0 * step
+    img[:, :, 0] = lo + (n * xx + yy) * step * (0.25 if itype == U8 else 1.0)           # ordered columns
+    img[:, :, 1] = lo + ((n - 1) * xx + yy) * step * (0.25 if itype == U8 else 1.0)     # touching ranges
+    img[:, :, 2] = lo + (n * (w - xx) + (h - yy)) * step * (0.25 if itype == U8 else 1.0)   # falling ramps
+    img[:, 200:260, 0] = lo + (xx[:, 200:260] + n * yy[:, 200:260]) * step               # rows ordered
+    img[20, 300, :] = img[20, 301, :]                                                   # one sample breaks
+    img[:, 330:336, 1] = img[:, 330:331, 1]                                             # a plateau
+    if itype == U8:
+        img = np.clip(img, 0, 255)
+    img = img.astype(np.float32) if itype == F32 else np.rint(img).astype(np.uint8 if itype == U8 else np.uint16)
+    a = np.full((h, w), 255, np.uint8)
+    kw = dict(w_exposure=0.0, w_entropy=1.0, entropy_window=ewin)
+    want = oracle.enfuse_mask(img, a, itype, orc_params(**kw))
+    dimg, da = torch.from_numpy(np.ascontiguousarray(img)).cuda(), torch.from_numpy(a).cuda()
+    dm = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    print(f"entropy ordered windows itype={itype} n={ewin}", flush=True)
+    enbl.check(enbl.lib().enbl_enfuse_mask(gpu_ctx.h, itype, w, h, _vp(dimg), 3 * w, _vp(da), w,
+                                           ctypes.byref(enbl.enfuse_params(**kw)), _vp(dm), w))
+    gpu_ctx.sync()
+    np.testing.assert_array_equal(dm.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
 def _table_call(enbl, ctx, itype, table, entry):
     """Calls one enfuse entry point on a tiny stack with an exposure table;
     returns the status code."""
