@@ -647,95 +647,97 @@ __device__ __forceinline__ void emit_l0(const L0FuseArgs& a, int X, int Y, bool 
     }
 }
 
-// Wave strips as in batch_fuse_kernel: lane L of wave wx owns coarse column
-// j = 62 wx + L - 1 (L in [1, 62]) and its 2 x 2 fine block; a block is four
-// waves side by side over a.rows coarse rows.
-// One coarse row i of the level-0 merge + collapse (see enfuse_level0_kernel).
-// P2: every divisor of the wave's expand is a power of two (interior rows and
-// columns), so the divisions are exact products with the reciprocals.
+// One coarse row i of one image of the level-0 merge (see
+// enfuse_level0_kernel): the Laplacian values of the 2 x 2 fine block, times
+// the image's mask values, added to acc in image order.  c: the image's
+// Gaussian level-1 rows i-1, i, i+1.  P2: every divisor of the wave's expand
+// is a power of two (interior rows and columns), so the divisions are exact
+// products with the reciprocals.
 template <class P, class M, class I, bool G0, bool P2>
-__device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale, int i, int j, bool valid, bool owned,
-                                           int X, bool xb, const int (&dv)[4])
+__device__ __forceinline__ void level0_image_row(const L0FuseArgs& a, double inv_scale, int m, int i, int j, bool valid,
+                                                 int X, bool xb, const int (&dv)[4], const float (&nv)[4],
+                                                 const typename Pix<P>::S (&c)[3][3], typename Pix<P>::S (&acc)[4][3])
 {
     typedef Pix<P> Px;
     typedef typename Px::A A;
     typedef typename Px::S S;
-    const int cw = a.cw, chh = a.ch;
     const int Y = 2 * i;
     const bool yb = Y + 1 < a.fh;
-    int sh[4];
-    double inv[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        sh[q] = P2 ? __builtin_ctz((unsigned)dv[q]) : 0;
-        inv[q] = P2 ? __builtin_amdgcn_ldexp(1.0, -sh[q]) : 0.0;
-    }
-    // x / dv[q]: the product with the reciprocal (double) or the shift with
-    // the round-toward-zero correction (integers) when the wave's divisors
-    // are powers of two
     auto qdiv = [&](S v, int q) -> S {
-        if constexpr (P2 && std::is_same<P, double>::value) return v * inv[q];
-        else if constexpr (P2) return (S)((v + ((v >> 31) & ((1 << sh[q]) - 1))) >> sh[q]);
-        else return A::div(v, dv[q]);
-    };
-    // the norm of the 2 x 2 block, shared by the images' mask values
-    float nv[4] = {0.f, 0.f, 0.f, 0.f};
-    if (valid) {
-        const float* nrow = a.norm + (long)Y * a.npitch + X;
-        nv[0] = *gmem(nrow);
-        nv[1] = xb ? *gmem(nrow + 1) : 0.f;
-        if (yb) {
-            nv[2] = *gmem(nrow + a.npitch);
-            nv[3] = xb ? *gmem(nrow + a.npitch + 1) : 0.f;
+        if constexpr (P2 && std::is_same<P, double>::value) {
+            return v * __builtin_amdgcn_ldexp(1.0, -(int)__builtin_ctz((unsigned)dv[q]));
+        } else if constexpr (P2) {
+            const int sh = __builtin_ctz((unsigned)dv[q]);
+            return (S)((v + ((v >> 31) & ((1 << sh) - 1))) >> sh);
+        } else {
+            return A::div(v, dv[q]);
         }
-    }
-    S acc[4][3];
-    for (int m = 0; m < a.K; ++m) {
-        S c[3][3], f[4][3];
-        int32_t mv[4] = {0, 0, 0, 0};
-        load_coarse<P, 3>(static_cast<const P*>(a.g1[m]), a.gpitch, a.gplane, i, j, chh, valid, c);
+    };
+    S f[4][3];
+    int32_t mv[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) f[q][ch] = A::zero();
+    if (valid) {
+        const float* wrow = a.wt[m] + (long)Y * a.wpitch + X;
+        float wv[4];
+        wv[0] = *gmem(wrow);
+        wv[1] = xb ? *gmem(wrow + 1) : 0.f;
+        wv[2] = yb ? *gmem(wrow + a.wpitch) : 0.f;
+        wv[3] = xb && yb ? *gmem(wrow + a.wpitch + 1) : 0.f;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) f[q][ch] = A::zero();
-        if (valid) {
-            const float* wrow = a.wt[m] + (long)Y * a.wpitch + X;
-            float wv[4];
-            wv[0] = *gmem(wrow);
-            wv[1] = xb ? *gmem(wrow + 1) : 0.f;
-            wv[2] = yb ? *gmem(wrow + a.wpitch) : 0.f;
-            wv[3] = xb && yb ? *gmem(wrow + a.wpitch + 1) : 0.f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                mv[q] = (int32_t)from_real<M>(norm_mask_value(wv[q], nv[q], a.fallback, a.hard, a.mscale));
-            if constexpr (G0) {
-                const RowSrc rs{a.g0[m], a.g0pitch, a.g0plane, 0};
-                PlaneRows<P>::template load<3>(rs, Y, X, xb, f[0], f[1]);
-                if (yb) PlaneRows<P>::template load<3>(rs, Y + 1, X, xb, f[2], f[3]);
-            } else {
-                L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y, X, xb, a.frac, f[0], f[1]);
-                if (yb) L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y + 1, X, xb, a.frac, f[2], f[3]);
-            }
-        }
-        S o[4][3];
-        expand_sums<P, 3>(c, i, j, cw, chh, o);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            // Laplacian value (src/pyramid.h:1705-1707), mask multiply
-            // (src/enfuse.h:601-620) and accumulation (:1576-1588).
-            const double coeff = mask_coeff(mv[q], a.maxpyr, inv_scale);
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const P lapv = Px::wrap(A::sub(f[q][ch], qdiv(o[q][ch], q)));
-                const S v = Px::load(from_real<P>((double)lapv * coeff));
-                acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
-            }
+            mv[q] = (int32_t)from_real<M>(norm_mask_value(wv[q], nv[q], a.fallback, a.hard, a.mscale));
+        if constexpr (G0) {
+            const RowSrc rs{a.g0[m], a.g0pitch, a.g0plane, 0};
+            PlaneRows<P>::template load<3>(rs, Y, X, xb, f[0], f[1]);
+            if (yb) PlaneRows<P>::template load<3>(rs, Y + 1, X, xb, f[2], f[3]);
+        } else {
+            L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y, X, xb, a.frac, f[0], f[1]);
+            if (yb) L0Fine<I, P>::load(a.img[m], a.ipitch[m], Y + 1, X, xb, a.frac, f[2], f[3]);
         }
     }
-    // Collapse step: + expand(result[1]), saturating (src/pyramid.h:1837-1886).
-    S rcv[3][3], o[4][3];
-    load_coarse<P, 3>(static_cast<const P*>(a.rc), a.rcpitch, a.rcplane, i, j, chh, valid, rcv);
-    expand_sums<P, 3>(rcv, i, j, cw, chh, o);
+    S o[4][3];
+    expand_sums<P, 3>(c, i, j, a.cw, a.ch, o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        // Laplacian value (src/pyramid.h:1705-1707), mask multiply
+        // (src/enfuse.h:601-620) and accumulation (:1576-1588).
+        const double coeff = mask_coeff(mv[q], a.maxpyr, inv_scale);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const P lapv = Px::wrap(A::sub(f[q][ch], qdiv(o[q][ch], q)));
+            const S v = Px::load(from_real<P>((double)lapv * coeff));
+            acc[q][ch] = m == 0 ? v : Px::load(Px::wrap(A::add(v, acc[q][ch])));
+        }
+    }
+}
+
+// Collapse step of coarse row i: + expand(result[1]), saturating
+// (src/pyramid.h:1837-1886), and the output of the 2 x 2 block.
+template <class P, class I, bool P2>
+__device__ __forceinline__ void level0_collapse_row(const L0FuseArgs& a, int i, int j, bool owned, int X, bool xb,
+                                                    const int (&dv)[4], const typename Pix<P>::S (&rcv)[3][3],
+                                                    const typename Pix<P>::S (&acc)[4][3])
+{
+    typedef Pix<P> Px;
+    typedef typename Px::A A;
+    typedef typename Px::S S;
+    const int Y = 2 * i;
+    const bool yb = Y + 1 < a.fh;
+    auto qdiv = [&](S v, int q) -> S {
+        if constexpr (P2 && std::is_same<P, double>::value) {
+            return v * __builtin_amdgcn_ldexp(1.0, -(int)__builtin_ctz((unsigned)dv[q]));
+        } else if constexpr (P2) {
+            const int sh = __builtin_ctz((unsigned)dv[q]);
+            return (S)((v + ((v >> 31) & ((1 << sh) - 1))) >> sh);
+        } else {
+            return A::div(v, dv[q]);
+        }
+    };
+    S o[4][3];
+    expand_sums<P, 3>(rcv, i, j, a.cw, a.ch, o);
     P outv[4][3];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -746,36 +748,104 @@ __device__ __forceinline__ void level0_row(const L0FuseArgs& a, double inv_scale
     if (yb) emit_l0<P, I>(a, X, Y + 1, xb, outv[2], outv[3]);
 }
 
+// One coarse row y of a planar level into c (zero outside the level and on
+// lanes without a column).
+template <class P>
+__device__ __forceinline__ void load_coarse_row(const P* p, long pitch, long plane, int y, int j, int chh, bool valid,
+                                                typename Pix<P>::S (&c)[3])
+{
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        c[ch] = (valid && y >= 0 && y < chh) ? Pix<P>::load(*gmem(p + ch * plane + (long)y * pitch + j)) : Pix<P>::A::zero();
+}
+
 // Wave strips as in batch_fuse_kernel: lane L of wave wx owns coarse column
 // j = 62 wx + L - 1 (L in [1, 62]) and its 2 x 2 fine block; a block is four
-// waves side by side over a.rows coarse rows.  G0: level 0 read from the
-// converted planes a.g0 instead of the images.
-template <class P, class M, class I, bool G0>
+// waves side by side over R coarse rows.  Each wave walks the images in
+// order, and for each image its R rows with a rolling window of three
+// Gaussian level-1 rows, so a level-1 row is read once per image and strip
+// (plus the two halo rows) while the R rows' sums stay in registers; then the
+// collapse of the R rows, again with a rolling window of result rows.
+// G0: level 0 read from the converted planes a.g0 instead of the images.
+template <class P, class M, class I, bool G0, int R>
 __global__ __launch_bounds__(256) void enfuse_level0_kernel(L0FuseArgs a, double inv_scale)
 {
+    typedef typename Pix<P>::S S;
     const int lane = threadIdx.x & 63;
     const int wx = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int cw = a.cw, chh = a.ch;
     if (wx * kOwned >= cw) return;   // the whole wave is beyond the level
-    const int i0 = blockIdx.y * a.rows;
-    const int i1 = min(chh, i0 + a.rows);
+    const int i0 = blockIdx.y * R;
+    const int nr = min(R, chh - i0);
     const int j = wx * kOwned + lane - 1;
     const bool valid = j >= 0 && j < cw;
     const bool owned = valid && lane >= 1 && lane <= kOwned;
     const int X = 2 * j;
     const bool xb = X + 1 < a.fw;
     const int hdA = expand_hsum(X, cw, false, true), hdB = expand_hsum(X + 1, cw, false, true);
-    for (int i = i0; i < i1; ++i) {
-        const int Y = 2 * i;
-        const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
-        const int dv[4] = {hdA * v0, hdB * v0, hdA * v1, hdB * v1};
-        // (only the owned lanes' results are kept)
-        bool p2 = true;
+    int dv[R][4];
+    bool p2[R];
+    float nv[R][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) p2 = p2 && dv[q] > 0 && (dv[q] & (dv[q] - 1)) == 0;
-        p2 = p2 || !owned;
-        if (wave_all(p2)) level0_row<P, M, I, G0, true>(a, inv_scale, i, j, valid, owned, X, xb, dv);
-        else level0_row<P, M, I, G0, false>(a, inv_scale, i, j, valid, owned, X, xb, dv);
+    for (int r = 0; r < R; ++r) {
+        const int i = i0 + r, Y = 2 * i;
+        const int v0 = expand_vsum(Y, chh), v1 = expand_vsum(Y + 1, chh);
+        dv[r][0] = hdA * v0; dv[r][1] = hdB * v0; dv[r][2] = hdA * v1; dv[r][3] = hdB * v1;
+        // (only the owned lanes' results are kept)
+        bool q2 = true;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) q2 = q2 && dv[r][q] > 0 && (dv[r][q] & (dv[r][q] - 1)) == 0;
+        p2[r] = wave_all(q2 || !owned || r >= nr);
+        // the norm of the 2 x 2 block, shared by the images' mask values
+        const bool yb = Y + 1 < a.fh;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nv[r][q] = 0.f;
+        if (valid && r < nr) {
+            const float* nrow = a.norm + (long)Y * a.npitch + X;
+            nv[r][0] = *gmem(nrow);
+            nv[r][1] = xb ? *gmem(nrow + 1) : 0.f;
+            if (yb) {
+                nv[r][2] = *gmem(nrow + a.npitch);
+                nv[r][3] = xb ? *gmem(nrow + a.npitch + 1) : 0.f;
+            }
+        }
+    }
+    S acc[R][4][3];
+    for (int m = 0; m < a.K; ++m) {
+        const P* g1 = static_cast<const P*>(a.g1[m]);
+        S c[3][3];
+        load_coarse_row<P>(g1, a.gpitch, a.gplane, i0 - 1, j, chh, valid, c[0]);
+        load_coarse_row<P>(g1, a.gpitch, a.gplane, i0, j, chh, valid, c[1]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) break;
+            const int i = i0 + r;
+            load_coarse_row<P>(g1, a.gpitch, a.gplane, i + 1, j, chh, valid, c[2]);
+            if (p2[r]) level0_image_row<P, M, I, G0, true>(a, inv_scale, m, i, j, valid, X, xb, dv[r], nv[r], c, acc[r]);
+            else level0_image_row<P, M, I, G0, false>(a, inv_scale, m, i, j, valid, X, xb, dv[r], nv[r], c, acc[r]);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                c[0][ch] = c[1][ch];
+                c[1][ch] = c[2][ch];
+            }
+        }
+    }
+    const P* rc = static_cast<const P*>(a.rc);
+    S rcv[3][3];
+    load_coarse_row<P>(rc, a.rcpitch, a.rcplane, i0 - 1, j, chh, valid, rcv[0]);
+    load_coarse_row<P>(rc, a.rcpitch, a.rcplane, i0, j, chh, valid, rcv[1]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r >= nr) break;
+        const int i = i0 + r;
+        load_coarse_row<P>(rc, a.rcpitch, a.rcplane, i + 1, j, chh, valid, rcv[2]);
+        if (p2[r]) level0_collapse_row<P, I, true>(a, i, j, owned, X, xb, dv[r], rcv, acc[r]);
+        else level0_collapse_row<P, I, false>(a, i, j, owned, X, xb, dv[r], rcv, acc[r]);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            rcv[0][ch] = rcv[1][ch];
+            rcv[1][ch] = rcv[2][ch];
+        }
     }
 }
 
@@ -784,16 +854,16 @@ __global__ __launch_bounds__(256) void enfuse_level0_kernel(L0FuseArgs a, double
 int launch_enfuse_level0(int itype, L0FuseArgs a, hipStream_t s)
 {
     if (a.K < 1 || a.K > kL0Max || a.cw < 1 || a.ch < 1) return ENBL_E_ARG;
-    if (a.rows <= 0) a.rows = 8;
+    constexpr int R = 3;   // coarse rows per wave (their sums stay in registers)
     const int waves_x = (a.cw + kOwned - 1) / kOwned;
-    const dim3 grid((waves_x + 3) / 4, (a.ch + a.rows - 1) / a.rows);
+    const dim3 grid((waves_x + 3) / 4, (a.ch + R - 1) / R);
     const double inv_scale = 255.0 / a.maxpyr;   // 2^-mfrac, exact
     // (8- and 16-bit stacks keep their stored level 0 in the per-stack path;
     // the integer instances would form it as (P)v << frac)
     if (itype == ENBL_F32 && a.g0[0])
-        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, true>), grid, dim3(256), 0, s, a, inv_scale);
+        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, true, R>), grid, dim3(256), 0, s, a, inv_scale);
     else if (itype == ENBL_F32)
-        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, false>), grid, dim3(256), 0, s, a, inv_scale);
+        hipLaunchKernelGGL((enfuse_level0_kernel<double, int32_t, float, false, R>), grid, dim3(256), 0, s, a, inv_scale);
     else return ENBL_E_UNSUPPORTED;
     return 0;
 }
