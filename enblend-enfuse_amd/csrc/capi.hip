@@ -885,6 +885,7 @@ int enbl_context_create(int device, enbl_context** out)
     if (rc) { (void)hipStreamDestroy(c->own_stream); delete c; return rc; }
     rc = hip_check(hipEventCreateWithFlags(&c->aux_fork, hipEventDisableTiming), "hipEventCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&c->aux_join, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&c->aux_mid, hipEventDisableTiming), "hipEventCreate");
     if (rc) { (void)hipStreamDestroy(c->aux_stream); (void)hipStreamDestroy(c->own_stream); delete c; return rc; }
     c->forked_from = nullptr;
     c->stream = c->own_stream;
@@ -904,6 +905,7 @@ int enbl_context_destroy(enbl_context* ctx)
     for (int i = 0; i < kScratchSlots; ++i) if (ctx->scratch_buf[i]) (void)hipFree(ctx->scratch_buf[i]);
     (void)hipEventDestroy(ctx->aux_fork);
     (void)hipEventDestroy(ctx->aux_join);
+    (void)hipEventDestroy(ctx->aux_mid);
     (void)hipStreamDestroy(ctx->aux_stream);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;

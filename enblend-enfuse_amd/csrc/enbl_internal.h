@@ -212,7 +212,7 @@ struct enbl_context {
     // (fork_aux / end_aux / join_aux); `forked_from` is the main stream while
     // work is being issued on aux_stream, else null.
     hipStream_t aux_stream;
-    hipEvent_t aux_fork, aux_join;
+    hipEvent_t aux_fork, aux_join, aux_mid;
     hipStream_t forked_from;
     bool profile;
     struct Timed { std::string name; double bytes, own; hipEvent_t start, stop; };

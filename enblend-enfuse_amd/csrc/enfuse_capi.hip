@@ -379,8 +379,10 @@ int edge_contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const u
 
 // Gray images and local standard deviations (src/enfuse.h:1099-1120) of
 // all K images: gradient plane m of ws.grad belongs to image m.
-int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
-                 const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, const WeightScratch& ws)
+// Gray planes of all images (and, for float windows the tile kernel does not
+// take, the cleared gradient planes).
+int contrast_gray(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                  const enbl_enfuse_params* p, const WeightScratch& ws)
 {
     const hipStream_t s = ctx->stream;
     const size_t es = 4;   // int32 gray of integer images, float gray of float images
@@ -391,19 +393,38 @@ int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* 
         if (!rc) rc = launch_status("gray");
         if (rc) return rc;
     }
-    if (p->edge_scale > 0.0) return edge_contrast_all(ctx, itype, w, h, K, alphas, aps, p, ws);
-    if (itype == ENBL_F32 && p->contrast_window != 3 && p->contrast_window != 5 && p->contrast_window != 7) {
+    if (p->edge_scale <= 0.0 && itype == ENBL_F32 && p->contrast_window != 3 && p->contrast_window != 5 &&
+        p->contrast_window != 7) {
         // the row kernel of other windows writes the interior under alpha
         // only (the tile kernel of 3, 5 and 7 writes every pixel)
         rc = launch_fill(ws.grad, ws.plane * es * (size_t)K, s);
         if (!rc) rc = launch_status("fill");
         if (rc) return rc;
     }
-    Timed t(ctx, "stddev", (double)K * w * h * 9.0);
-    rc = launch_stddev(itype, ws.gray, ws.pitch, ws.plane, alphas, aps, K, w, h, p->contrast_window, ws.grad,
-                       ws.pitch, ws.plane, s);
+    return ENBL_OK;
+}
+
+// Local standard deviations of images m0 .. m1-1 from their gray planes.
+int contrast_stddev(enbl_context* ctx, int itype, int w, int h, int m0, int m1, const uint8_t* const* alphas,
+                    const long* aps, const enbl_enfuse_params* p, const WeightScratch& ws)
+{
+    if (m1 <= m0) return ENBL_OK;
+    const size_t es = 4;
+    const size_t off = (size_t)m0 * ws.plane * es;
+    Timed t(ctx, "stddev", (double)(m1 - m0) * w * h * 9.0);
+    const int rc = launch_stddev(itype, (const char*)ws.gray + off, ws.pitch, ws.plane, alphas + m0, aps + m0, m1 - m0, w,
+                                 h, p->contrast_window, (char*)ws.grad + off, ws.pitch, ws.plane, ctx->stream);
     if (rc) return fail(rc, "localStdDevIf: unsupported window");
     return launch_status("stddev");
+}
+
+int contrast_all(enbl_context* ctx, int itype, int w, int h, int K, const void* const* imgs, const long* ips,
+                 const uint8_t* const* alphas, const long* aps, const enbl_enfuse_params* p, const WeightScratch& ws)
+{
+    int rc = contrast_gray(ctx, itype, w, h, K, imgs, ips, p, ws);
+    if (rc) return rc;
+    if (p->edge_scale > 0.0) return edge_contrast_all(ctx, itype, w, h, K, alphas, aps, p, ws);
+    return contrast_stddev(ctx, itype, w, h, 0, K, alphas, aps, p, ws);
 }
 
 // Weights of image m into mask (its gradient plane made by contrast_all);
@@ -635,25 +656,48 @@ int enbl_enfuse(enbl_context* ctx, int itype, int w, int h, int K, const void* c
         // context's alpha scratch with the mask pyramids)
         for (int m = 0; m < K && !rc; ++m) rc = jch_laplacian(m);
         if (rc) return rc;
-    } else if (!wrap) {
+    }
+    // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388); the
+    // contrast of all images first, so that their local standard deviations
+    // share launches.  Beside the image pyramids (auxiliary stream), the
+    // standard deviations of the second half of the images run on the
+    // auxiliary stream too, ahead of the pyramids, while the first half's
+    // weights are made; the second half's weights wait for them.
+    const bool sd_split = !wrap && !jch && p->w_contrast > 0.0 && p->edge_scale <= 0.0 && K >= 2;
+    const int half = sd_split ? (K + 1) / 2 : K;
+    bool forked = false;
+    if (sd_split) {
+        rc = contrast_gray(ctx, itype, w, h, K, imgs, ips, p, ws);
+        if (rc) return rc;
+    }
+    if (!wrap && !jch) {
         rc = fork_aux(ctx);
         if (rc) return rc;
         aux.active = true;
-        rc = enfuse_image_pyramids(ctx, itype, ifrac, K, pimgs.data(), pips.data(), alphas, aps, lps.data(),
-                                   fuse0 ? g1s.data() : nullptr);
+        forked = ctx->forked_from != nullptr;
+        if (sd_split) {
+            rc = contrast_stddev(ctx, itype, w, h, half, K, alphas, aps, p, ws);
+            if (!rc && forked) rc = hip_check(hipEventRecord(ctx->aux_mid, ctx->stream), "hipEventRecord");
+        }
+        if (!rc)
+            rc = enfuse_image_pyramids(ctx, itype, ifrac, K, pimgs.data(), pips.data(), alphas, aps, lps.data(),
+                                       fuse0 ? g1s.data() : nullptr);
         const int rc2 = end_aux(ctx);
         if (!rc) rc = rc2;
         if (rc) return rc;
     }
-
-    // Pass 1: weights, norm, union alpha (src/enfuse.h:1277-1388); the
-    // contrast of all images first, so that their local standard deviations
-    // share launches.
-    if (p->w_contrast > 0.0) {
+    if (sd_split) {
+        rc = contrast_stddev(ctx, itype, w, h, 0, half, alphas, aps, p, ws);
+        if (rc) return rc;
+    } else if (p->w_contrast > 0.0) {
         rc = contrast_all(ctx, itype, w, h, K, imgs, ips, alphas, aps, p, ws);
         if (rc) return rc;
     }
     for (int m = 0; m < K; ++m) {
+        if (sd_split && forked && m == half) {
+            rc = hip_check(hipStreamWaitEvent(ctx->stream, ctx->aux_mid, 0), "hipStreamWaitEvent");
+            if (rc) return rc;
+        }
         rc = image_weights(ctx, itype, w, h, m, imgs[m], ips[m], alphas[m], aps[m], p, a, ws,
                            (float*)masks + plane * m, pitch, (float*)norm, pitch, (uint8_t*)ua, pitch);
         if (rc) return rc;
