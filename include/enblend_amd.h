@@ -312,12 +312,14 @@ typedef struct enbl_stack {
 /* Batched form of enbl_enfuse -- no counterpart in the reference, which runs
  * enfuseMain (src/enfuse.h:1217-1609) once per stack: `nstacks` stacks of
  * any sizes, depths and level counts fused with the same parameters, with
- * results identical to one enbl_enfuse call per stack.  Integer images with
- * exposure and saturation weights and no wraparound take the batched
- * schedule (one persistent launch per pyramid level over all stacks, replayed
- * from a HIP graph when the same batch comes again); any other combination
- * runs enbl_enfuse per stack.  The context keeps the intermediate pyramids
- * and the captured graph until it is destroyed. */
+ * results identical to one enbl_enfuse call per stack.  8/16-bit and float
+ * images without wraparound take the batched pyramid schedule (one
+ * persistent launch per pyramid level over all stacks, replayed from a HIP
+ * graph when the same batch comes again); their weights are batched too for
+ * integer images with exposure and saturation weights, and made stack by
+ * stack otherwise.  A blend colour space or wraparound runs enbl_enfuse per
+ * stack.  The context keeps the intermediate pyramids and the captured graph
+ * until it is destroyed. */
 int enbl_enfuse_batch(enbl_context* ctx, int itype, int nstacks, const enbl_stack* stacks,
                       const enbl_enfuse_params* prm, int wraparound, int dither_mode);
 
