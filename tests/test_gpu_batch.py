@@ -175,6 +175,33 @@ def test_batch_stack_weights_match_plain_and_oracle(enbl, gpu_ctx, oracle, itype
                     np.testing.assert_array_equal(got[sel], want[sel], err_msg=f"stack {s} vs oracle")
 
 
+def test_batch_float_special_values_match_plain(enbl, gpu_ctx):
+    """Float stacks with negative samples (the reciprocal branch of the log
+    transform), +inf, NaN and huge values: the batched pyramid passes give
+    exactly what one enbl_enfuse per stack gives (NaN where it gives NaN)."""
+    rng = np.random.default_rng(81)
+    host, dev = _make_batch(rng, F32, [(83, 61, 3, None, "holes"), (130, 67, 4, None, "full"), (37, 23, 2, 2, "band")],
+                            enbl)
+    for (imgs, alphas, out, oa, levels, seed) in dev:
+        for k, t in enumerate(imgs):
+            a = t.cpu().numpy()
+            a[3, 5, 0] = -0.5
+            a[7, 2, 1] = np.inf
+            a[10 + k, 11, 2] = np.nan
+            a[1, 1, :] = 3.0e38
+            a[2, 9, 2] = -0.0
+            t.copy_(torch.from_numpy(a).cuda())
+    torch.cuda.synchronize()
+    kw = dict(w_exposure=1.0, w_saturation=0.2, w_contrast=0.3, contrast_window=3)
+    print("batch float special values", flush=True)
+    plain = _plain(enbl, gpu_ctx, F32, dev, kw)
+    enbl.enfuse_batch(gpu_ctx, F32, dev, enbl.enfuse_params(**kw), dither=1)
+    gpu_ctx.sync()
+    for s_, (d, (po, pa)) in enumerate(zip(dev, plain)):
+        np.testing.assert_array_equal(d[3].cpu().numpy(), pa, err_msg=f"stack {s_} alpha")
+        np.testing.assert_array_equal(d[2].cpu().numpy(), po, err_msg=f"stack {s_}")
+
+
 def test_batch_autotune_keeps_results(enbl, gpu_ctx):
     rng = np.random.default_rng(79)
     kw = PARAMS[0]
