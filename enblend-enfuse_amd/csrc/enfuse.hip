@@ -508,10 +508,8 @@ __global__ __launch_bounds__(kTile) void entropy_kernel(const T* __restrict__ im
 //     and walked like the full window's keys would be;
 //   * any other window goes to a per-block queue.  After a barrier every
 //     thread takes queue entries in turn, sorts the entry's n*n keys by a
-//     comparator network held in registers (Batcher's odd-even merge sort for
-//     the next power of two, pruned to the window size: a comparator with an
-//     index past the window only ever meets the +infinity padding and is a
-//     no-op) and walks the bins of equal keys in ascending key order,
+//     comparator network held in registers (Batcher's merge exchange) and
+//     walks the bins of equal keys in ascending key order,
 //     accumulating the full-window p*log(p) table in the order the
 //     reference's std::map iterates them.  So the sorting work of a block is
 //     spread over all its lanes instead of being paid by a whole wave for
@@ -531,21 +529,31 @@ struct CmpNet {
     unsigned char a[1024], b[1024];
 };
 
+// Batcher's merge exchange for N keys (Knuth, TAOCP vol. 3, 5.2.2,
+// Algorithm M): a sorting network for any N, with no comparator past the
+// last key (383 comparators for 49 keys, against 394 for the odd-even merge
+// sort of 64 pruned to 49).
 constexpr CmpNet make_net(int N)
 {
     CmpNet c{};
-    int P = 1;
-    while (P < N) P <<= 1;
+    int t = 1;
+    while ((1 << t) < N) ++t;
     int m = 0;
-    for (int p = 1; p < P; p += p)
-        for (int k = p; k > 0; k /= 2)
-            for (int j = k % p; j + k < P; j += k + k)
-                for (int i = 0; i < k && i < P - j - k; ++i)
-                    if ((i + j) / (p + p) == (i + j + k) / (p + p) && i + j + k < N) {
-                        c.a[m] = (unsigned char)(i + j);
-                        c.b[m] = (unsigned char)(i + j + k);
-                        ++m;
-                    }
+    for (int p = 1 << (t - 1); p > 0; p >>= 1) {
+        int q = 1 << (t - 1), r = 0, d = p;
+        for (;;) {
+            for (int i = 0; i < N - d; ++i)
+                if ((i & p) == r) {
+                    c.a[m] = (unsigned char)i;
+                    c.b[m] = (unsigned char)(i + d);
+                    ++m;
+                }
+            if (q == p) break;
+            d = q - p;
+            q >>= 1;
+            r = p;
+        }
+    }
     c.n = m;
     return c;
 }
